@@ -1,0 +1,1108 @@
+// Host runtime behind the C ABI (include/dust2_amd.h): argument checks with
+// the reference's messages, device buffers owned by handles, RNG stream
+// expansion, launch of the kernels and conversion between the caller's
+// [group][particle][state] layout and the device's per-group SoA layout.
+#include "../../include/dust2_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace dust2_amd;
+
+namespace {
+
+struct hip_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw hip_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+  }
+}
+
+int fail(const std::exception& e, char* err, size_t errlen) {
+  if (err && errlen) std::snprintf(err, errlen, "%s", e.what());
+  return 1;
+}
+
+// Device allocation owned by a handle.
+class dbuf {
+ public:
+  dbuf() = default;
+  explicit dbuf(size_t bytes) { alloc(bytes); }
+  ~dbuf() { release(); }
+  dbuf(const dbuf&) = delete;
+  dbuf& operator=(const dbuf&) = delete;
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) hip_check(hipMalloc(&ptr_, bytes), "hipMalloc");
+    bytes_ = bytes;
+  }
+  void release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template <typename T = void>
+  T* get() const {
+    return static_cast<T*>(ptr_);
+  }
+  size_t bytes() const { return bytes_; }
+  void swap(dbuf& o) {
+    std::swap(ptr_, o.ptr_);
+    std::swap(bytes_, o.bytes_);
+  }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+void to_device(dbuf& d, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes) hip_check(hipMemcpyAsync(d.get(), src, bytes, hipMemcpyHostToDevice, s), "copy to device");
+}
+
+void to_host(void* dst, const dbuf& d, size_t bytes, hipStream_t s) {
+  if (bytes) {
+    hip_check(hipMemcpyAsync(dst, d.get(), bytes, hipMemcpyDeviceToHost, s), "copy to host");
+  }
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+// ---- argument checks (dust2/r/helpers.hpp) --------------------------------
+void check_dt(double dt) {
+  // dt must be positive and an integer or 1/integer (helpers.hpp:232-262)
+  if (!(dt > 0)) {
+    throw std::runtime_error("'dt' must be greater than 0");
+  }
+  if (dt > 1) {
+    if (std::fmod(dt, 1) != 0) {
+      throw std::runtime_error("'dt' must be an integer or the inverse of an integer");
+    }
+  } else {
+    const double inv = 1 / dt;
+    if (std::fabs(inv - std::round(inv)) > 1e-8) {
+      throw std::runtime_error("'dt' must be an integer or the inverse of an integer");
+    }
+  }
+}
+
+void check_time_sequence(double time_start, const double* time, size_t n) {
+  if (n == 0) {
+    throw std::runtime_error("Expected at least one value in 'time'");
+  }
+  if (!(time[0] > time_start)) {
+    throw std::runtime_error("Expected 'time[1]' (" + std::to_string(time[0]) +
+                             ") to be larger than the previous value (" +
+                             std::to_string(time_start) + ")");
+  }
+  for (size_t i = 1; i < n; ++i) {
+    if (!(time[i] > time[i - 1])) {
+      throw std::runtime_error("Expected 'time' to be strictly increasing");
+    }
+  }
+}
+
+void check_sizes(size_t n_groups, size_t n_particles) {
+  if (n_groups == 0) throw std::runtime_error("'n_groups' must be at least 1");
+  if (n_particles == 0) throw std::runtime_error("'n_particles' must be at least 1");
+  if (n_particles > (1u << 24) || n_groups > (1u << 24)) {
+    throw std::runtime_error("'n_particles' or 'n_groups' is too large");
+  }
+}
+
+std::vector<size_t> check_index(const size_t* index, size_t n_index, size_t n_groups) {
+  std::vector<size_t> ret;
+  if (index == nullptr) {
+    for (size_t i = 0; i < n_groups; ++i) ret.push_back(i);
+    return ret;
+  }
+  for (size_t i = 0; i < n_index; ++i) {
+    if (index[i] >= n_groups) {
+      throw std::runtime_error("Invalid index for 'index_group': values must lie in [0, " +
+                               std::to_string(n_groups - 1) + "]");
+    }
+    ret.push_back(index[i]);
+  }
+  return ret;
+}
+
+// ---- xoshiro256+ on the host (seeding only) --------------------------------
+struct host_rng {
+  uint64_t s[4];
+};
+
+uint64_t host_next(host_rng& r) {
+  const uint64_t out = r.s[0] + r.s[3];
+  const uint64_t t = r.s[1] << 17;
+  r.s[2] ^= r.s[0];
+  r.s[3] ^= r.s[1];
+  r.s[1] ^= r.s[2];
+  r.s[0] ^= r.s[3];
+  r.s[2] ^= t;
+  r.s[3] = (r.s[3] << 45) | (r.s[3] >> 19);
+  return out;
+}
+
+// Published xoshiro256 jump polynomial (2^128 steps).
+constexpr uint64_t kJumpPoly[4] = {0x180ec6d33cfd0abaULL, 0xd5a61266f0c9392cULL,
+                                   0xa9582618e03fc9aaULL, 0x39abdc4529b1661cULL};
+
+void host_jump(host_rng& r) {
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (int w = 0; w < 4; ++w) {
+    for (int b = 0; b < 64; ++b) {
+      if (kJumpPoly[w] & (uint64_t(1) << b)) {
+        for (int i = 0; i < 4; ++i) acc[i] ^= r.s[i];
+      }
+      host_next(r);
+    }
+  }
+  std::memcpy(r.s, acc, sizeof(acc));
+}
+
+// splitmix64-based expansion of an integer seed into the first stream.
+void host_seed_words(uint64_t seed, uint64_t* out) {
+  for (int i = 0; i < 4; ++i) {
+    uint64_t z = seed + 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    seed = z ^ (z >> 31);
+    out[i] = seed;
+  }
+}
+
+constexpr int kJumpMats = 32;  // supports 2^32 streams
+
+// J_i = jump^(2^i) as column-major GF(2) matrices: column j = J_i e_j.
+const std::vector<uint64_t>& host_jump_matrices() {
+  static std::vector<uint64_t> mats;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    mats.assign(static_cast<size_t>(kJumpMats) * 256 * 4, 0);
+    for (int j = 0; j < 256; ++j) {
+      host_rng r{{0, 0, 0, 0}};
+      r.s[j / 64] = uint64_t(1) << (j % 64);
+      host_jump(r);
+      std::memcpy(&mats[j * 4], r.s, 32);
+    }
+    for (int i = 1; i < kJumpMats; ++i) {
+      const uint64_t* prev = &mats[static_cast<size_t>(i - 1) * 1024];
+      uint64_t* cur = &mats[static_cast<size_t>(i) * 1024];
+      for (int j = 0; j < 256; ++j) {
+        const uint64_t* v = prev + j * 4;  // J_{i-1} e_j
+        uint64_t acc[4] = {0, 0, 0, 0};
+        for (int b = 0; b < 256; ++b) {
+          if ((v[b / 64] >> (b % 64)) & 1u) {
+            for (int w = 0; w < 4; ++w) acc[w] ^= prev[b * 4 + w];
+          }
+        }
+        std::memcpy(cur + j * 4, acc, 32);
+      }
+    }
+  });
+  return mats;
+}
+
+struct device_tables {
+  dbuf jump;
+};
+
+device_tables& tables_for_current_device() {
+  static std::mutex mu;
+  static std::vector<std::unique_ptr<device_tables>> per_device;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(mu);
+  if (per_device.size() <= static_cast<size_t>(dev)) per_device.resize(dev + 1);
+  if (!per_device[dev]) {
+    auto t = std::make_unique<device_tables>();
+    const auto& m = host_jump_matrices();
+    t->jump.alloc(m.size() * sizeof(uint64_t));
+    hip_check(hipMemcpy(t->jump.get(), m.data(), m.size() * sizeof(uint64_t),
+                        hipMemcpyHostToDevice),
+              "upload jump tables");
+    per_device[dev] = std::move(t);
+  }
+  return *per_device[dev];
+}
+
+// Fill d_aos (n_streams x 4 words) with the seeded streams.
+void expand_streams(const uint64_t* seed_state, size_t n_seed, uint64_t seed, size_t n_streams,
+                    uint64_t* d_aos, hipStream_t s) {
+  if (n_streams > (uint64_t(1) << kJumpMats)) {
+    throw std::runtime_error("too many random number streams");
+  }
+  size_t n_copy = 0;
+  uint64_t s0[4];
+  if (seed_state && n_seed > 0) {
+    n_copy = std::min(n_seed, n_streams);
+    hip_check(hipMemcpyAsync(d_aos, seed_state, n_copy * 32, hipMemcpyHostToDevice, s),
+              "copy seed");
+    host_rng r;
+    std::memcpy(r.s, seed_state + 4 * (n_seed - 1), 32);
+    host_jump(r);
+    std::memcpy(s0, r.s, 32);
+  } else {
+    host_seed_words(seed, s0);
+  }
+  if (n_copy < n_streams) {
+    dbuf d_s0(32);
+    hip_check(hipMemcpyAsync(d_s0.get(), s0, 32, hipMemcpyHostToDevice, s), "copy s0");
+    hip_check(launch_seed_streams(d_s0.get<uint64_t>(), tables_for_current_device().jump.get<uint64_t>(),
+                                  kJumpMats, n_streams - n_copy, d_aos + 4 * n_copy, s),
+              "seed streams");
+    hip_check(hipStreamSynchronize(s), "seed sync");
+  }
+}
+
+// ---- error reporting (dust2/errors.hpp:34-55) --------------------------------
+std::string error_text(const error_record& r) {
+  char buf[256];
+  switch (r.code) {
+    case ERR_BINOMIAL:
+      std::snprintf(buf, sizeof(buf), "Invalid call to binomial with n = %.0f, p = %g, q = %g",
+                    r.a, r.b, 1 - r.b);
+      return buf;
+    case ERR_ODE_MAX_STEPS: return "too many steps";
+    case ERR_ODE_STEP_SMALL: return "step too small";
+    case ERR_ODE_VANISHED: return "step size vanished";
+    case ERR_ODE_INIT: return "Initial step size was not finite";
+    default: return "unknown error";
+  }
+}
+
+// records: one per particle, in particle order; count = number set.
+std::string error_report(const std::vector<error_record>& records, size_t count,
+                         const char* title = "particles", size_t n_max = 4) {
+  std::stringstream msg;
+  msg << count << " " << title << " reported errors.";
+  const size_t n_report = std::min(n_max, count);
+  for (size_t i = 0, j = 0; i < records.size() && j < n_report; ++i) {
+    if (records[i].code != ERR_NONE) {
+      msg << std::endl << "  - " << i + 1 << ": " << error_text(records[i]);
+      ++j;
+    }
+  }
+  if (n_report < count) {
+    msg << std::endl << "  - (and " << (count - n_report) << " more)";
+  }
+  return msg.str();
+}
+
+int max_lds_per_block() {
+  int dev = 0;
+  int v = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+            "hipDeviceGetAttribute");
+  return v;
+}
+
+// SIR parameter row -> kernel parameter block (pars[8]):
+// N, I0, beta, gamma, exp_noise, p_IR (fp64), p_IR (fp32).
+void sir_kernel_pars(const double* row, double dt, double* out) {
+  for (int i = 0; i < 5; ++i) out[i] = row[i];
+  // FP64: the reference's expression (inst/examples/sir.cpp:64); FP32: the
+  // same probability without the cancellation of 1 - exp(-x) in float
+  out[5] = 1 - std::exp(-row[3] * dt);
+  out[6] = static_cast<double>(static_cast<float>(-std::expm1(-row[3] * dt)));
+  out[7] = 0;
+}
+
+constexpr size_t kSirPars = 5;
+constexpr int kSirState = 5;
+
+void check_sir_pars(const double* row) {
+  const char* names[] = {"N", "I0", "beta", "gamma", "exp_noise"};
+  for (size_t i = 0; i < kSirPars; ++i) {
+    if (!std::isfinite(row[i])) {
+      throw std::runtime_error(std::string("Parameter '") + names[i] + "' must be finite");
+    }
+  }
+}
+
+}  // namespace
+
+// ============================================================================
+// Filter
+// ============================================================================
+struct dust2_filter {
+  bool fp64 = true;
+  size_t G = 0, P = 0, n_time = 0, n_groups_data = 0;
+  double time_start = 0, dt = 1;
+  std::vector<double> pars;  // G x 5 (host rows)
+  std::vector<double> times;
+  hipStream_t stream = nullptr;
+  launch_config cfg{0, 0};
+  dbuf d_times, d_obs, d_lfact, d_state, d_rng, d_ll, d_time, d_slots, d_errors, d_counters,
+      d_ancestry, d_traj;
+  bool traj_valid = false;
+  bool ancestry_valid = false;
+  double last_ms = 0;
+  double last_fallbacks = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  size_t real_size() const { return fp64 ? 8 : 4; }
+
+  ~dust2_filter() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void aos_to_soa(const double* src, void* dst, size_t G, size_t P, size_t S, bool fp64) {
+  for (size_t g = 0; g < G; ++g) {
+    for (size_t p = 0; p < P; ++p) {
+      for (size_t c = 0; c < S; ++c) {
+        const double v = src[(g * P + p) * S + c];
+        const size_t o = (g * S + c) * P + p;
+        if (fp64) {
+          static_cast<double*>(dst)[o] = v;
+        } else {
+          static_cast<float*>(dst)[o] = static_cast<float>(v);
+        }
+      }
+    }
+  }
+}
+
+void soa_to_aos(const void* src, double* dst, size_t G, size_t P, size_t S, bool fp64) {
+  for (size_t g = 0; g < G; ++g) {
+    for (size_t p = 0; p < P; ++p) {
+      for (size_t c = 0; c < S; ++c) {
+        const size_t o = (g * S + c) * P + p;
+        dst[(g * P + p) * S + c] = fp64 ? static_cast<const double*>(src)[o]
+                                        : static_cast<const float*>(src)[o];
+      }
+    }
+  }
+}
+
+void filter_upload_data(dust2_filter* h, const double* data) {
+  const size_t n = h->n_time * h->n_groups_data;
+  std::vector<double> lfact(n);
+  for (size_t i = 0; i < n; ++i) {
+    lfact[i] = std::isnan(data[i]) ? 0.0 : std::lgamma(data[i] + 1);
+  }
+  h->d_obs.alloc(n * sizeof(double));
+  h->d_lfact.alloc(n * sizeof(double));
+  to_device(h->d_obs, data, n * sizeof(double), h->stream);
+  to_device(h->d_lfact, lfact.data(), n * sizeof(double), h->stream);
+}
+
+void check_filter(dust2_filter* h) {
+  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (filter)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int dust2_amd_init(char* err, size_t errlen) {
+  try {
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (n < 1) throw std::runtime_error("no GPU device available");
+    tables_for_current_device();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+const char* dust2_amd_version(void) { return "0.1.0"; }
+
+int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                           double time_start, const double* time, size_t n_time, double dt,
+                           const double* data, size_t n_groups_data,
+                           const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                           int precision, dust2_filter** out, char* err, size_t errlen) {
+  std::unique_ptr<dust2_filter> h;
+  try {
+    check_sizes(n_groups, n_particles);
+    check_dt(dt);
+    check_time_sequence(time_start, time, n_time);
+    if (n_groups_data != 1 && n_groups_data != n_groups) {
+      throw std::runtime_error("Expected 'data' to have 1 or " + std::to_string(n_groups) +
+                               " groups, but it had " + std::to_string(n_groups_data));
+    }
+    if (precision != DUST2_FP64 && precision != DUST2_FP32) {
+      throw std::runtime_error("'precision' must be DUST2_FP64 or DUST2_FP32");
+    }
+    for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * kSirPars);
+    h = std::make_unique<dust2_filter>();
+    h->fp64 = precision == DUST2_FP64;
+    h->G = n_groups;
+    h->P = n_particles;
+    h->n_time = n_time;
+    h->n_groups_data = n_groups_data;
+    h->time_start = time_start;
+    h->dt = dt;
+    h->pars.assign(pars, pars + n_groups * kSirPars);
+    h->times.assign(time, time + n_time);
+    if (static_cast<int>(n_particles) > sir_filter_max_particles(h->fp64)) {
+      throw std::runtime_error("'n_particles' exceeds " +
+                               std::to_string(sir_filter_max_particles(h->fp64)) +
+                               " per group");
+    }
+    const size_t lds = (kScratchDoubles + n_particles) * sizeof(double);
+    if (lds > static_cast<size_t>(max_lds_per_block())) {
+      throw std::runtime_error("'n_particles' too large for one workgroup's LDS");
+    }
+    h->cfg = sir_filter_default_config(static_cast<int>(n_particles), h->fp64);
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
+    const size_t G = n_groups, P = n_particles;
+    h->d_times.alloc(n_time * sizeof(double));
+    to_device(h->d_times, time, n_time * sizeof(double), h->stream);
+    filter_upload_data(h.get(), data);
+    h->d_state.alloc(G * kSirState * P * h->real_size());
+    h->d_rng.alloc(G * 4 * (P + 2) * sizeof(uint64_t));
+    {
+      dbuf tmp(G * (P + 2) * 32);
+      expand_streams(seed_state, n_seed_streams, seed, G * (P + 2), tmp.get<uint64_t>(),
+                     h->stream);
+      hip_check(launch_rng_aos_to_soa(tmp.get<uint64_t>(), h->d_rng.get<uint64_t>(),
+                                      static_cast<int>(G), static_cast<int>(P + 2),
+                                      static_cast<int>(P + 2), h->stream),
+                "rng layout");
+      hip_check(hipStreamSynchronize(h->stream), "sync");
+    }
+    h->d_ll.alloc(G * sizeof(double));
+    h->d_time.alloc(G * sizeof(double));
+    h->d_slots.alloc(G * sizeof(filter_slot));
+    h->d_errors.alloc(G * P * sizeof(error_record));
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
+    h->d_counters.alloc(4 * sizeof(uint32_t));
+    hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajectories,
+                         const size_t* index_group, size_t n_index, double* ll_out, char* err,
+                         size_t errlen) {
+  try {
+    check_filter(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    const size_t G = h->G, P = h->P;
+    hipStream_t s = h->stream;
+    if (initial) {
+      std::vector<char> tmp(G * kSirState * P * h->real_size());
+      aos_to_soa(initial, tmp.data(), G, P, kSirState, h->fp64);
+      to_device(h->d_state, tmp.data(), tmp.size(), s);
+    }
+    if (save_trajectories && h->d_traj.bytes() == 0) {
+      h->d_traj.alloc(h->n_time * G * kSirState * P * h->real_size());
+    }
+    if (save_trajectories && h->d_ancestry.bytes() == 0) {
+      h->d_ancestry.alloc(h->n_time * G * P * sizeof(uint32_t));
+    }
+    std::vector<double> zeros(G, 0.0), t0(G, h->time_start);
+    to_device(h->d_ll, zeros.data(), G * sizeof(double), s);
+    to_device(h->d_time, t0.data(), G * sizeof(double), s);
+    hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+
+    std::vector<char> selected(G, 0);
+    for (size_t g : idx) selected[g] = 1;
+    std::vector<filter_slot> slots;
+    const uint32_t stride = static_cast<uint32_t>(h->n_groups_data);
+    for (size_t g = 0; g < G; ++g) {
+      // groups outside index_group are reset (filter.hpp:248-250) but not run
+      if (!selected[g] && initial) continue;
+      filter_slot sl{};
+      sir_kernel_pars(&h->pars[g * kSirPars], h->dt, sl.pars);
+      sl.times = h->d_times.get<double>();
+      const size_t goff = h->n_groups_data == 1 ? 0 : g;
+      sl.obs = h->d_obs.get<double>() + goff;
+      sl.lfact = h->d_lfact.get<double>() + goff;
+      sl.obs_stride = stride;
+      sl.n_particles = static_cast<uint32_t>(P);
+      sl.t_begin = 0;
+      sl.t_end = selected[g] ? static_cast<uint32_t>(h->n_time) : 0;
+      sl.flags = (initial ? 0u : SLOT_SET_INITIAL) | (save_trajectories ? SLOT_SAVE_TRAJ : 0u);
+      sl.group = static_cast<uint32_t>(g);
+      sl.dt = h->dt;
+      sl.state = h->d_state.get<char>() + g * kSirState * P * h->real_size();
+      sl.rng = h->d_rng.get<uint64_t>() + g * 4 * (P + 2);
+      sl.ll = h->d_ll.get<double>() + g;
+      sl.time = h->d_time.get<double>() + g;
+      sl.ancestry = save_trajectories ? h->d_ancestry.get<uint32_t>() + g * h->n_time * P
+                                      : nullptr;
+      sl.traj = save_trajectories
+                    ? h->d_traj.get<char>() + g * h->n_time * kSirState * P * h->real_size()
+                    : nullptr;
+      sl.errors = h->d_errors.get<error_record>() + g * P;
+      sl.error_count = h->d_counters.get<uint32_t>();
+      sl.n_fallback = h->d_counters.get<uint32_t>() + 1;
+      slots.push_back(sl);
+    }
+    to_device(h->d_slots, slots.data(), slots.size() * sizeof(filter_slot), s);
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    hip_check(launch_sir_filter(h->d_slots.get<filter_slot>(), static_cast<int>(slots.size()),
+                                static_cast<int>(P), h->fp64, h->cfg, s),
+              "sir filter kernel");
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+    uint32_t counters[4];
+    to_host(counters, h->d_counters, sizeof(counters), s);
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    h->last_ms = ms;
+    h->last_fallbacks = counters[1];
+    h->traj_valid = save_trajectories != 0;
+    h->ancestry_valid = save_trajectories != 0;
+    if (counters[0] > 0) {
+      std::vector<error_record> rec(G * P);
+      to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+      hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+      hip_check(hipStreamSynchronize(s), "sync");
+      throw std::runtime_error(error_report(rec, counters[0]));
+    }
+    std::vector<double> ll(G);
+    to_host(ll.data(), h->d_ll, G * sizeof(double), s);
+    for (size_t i = 0; i < idx.size(); ++i) ll_out[i] = ll[idx[i]];
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_update_pars(dust2_filter* h, const double* pars, const size_t* index_group,
+                                 size_t n_index, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    for (size_t i = 0; i < idx.size(); ++i) {
+      const double* row = pars + i * kSirPars;
+      check_sir_pars(row);
+      double* dst = &h->pars[idx[i] * kSirPars];
+      // I0, beta, gamma are updatable; N and exp_noise are constant
+      dst[1] = row[1];
+      dst[2] = row[2];
+      dst[3] = row[3];
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_last_state(dust2_filter* h, double* out, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    std::vector<char> tmp(h->d_state.bytes());
+    to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
+    soa_to_aos(tmp.data(), out, h->G, h->P, kSirState, h->fp64);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_last_trajectories(dust2_filter* h, double* out, char* err,
+                                       size_t errlen) {
+  try {
+    check_filter(h);
+    if (!h->traj_valid) {
+      throw std::runtime_error("Trajectories are not current; run with 'save_trajectories'");
+    }
+    // device: [G][n_time][S][P]
+    const size_t G = h->G, P = h->P, T = h->n_time;
+    std::vector<char> tmp(h->d_traj.bytes());
+    to_host(tmp.data(), h->d_traj, tmp.size(), h->stream);
+    for (size_t g = 0; g < G; ++g) {
+      for (size_t t = 0; t < T; ++t) {
+        for (size_t c = 0; c < kSirState; ++c) {
+          for (size_t p = 0; p < P; ++p) {
+            const size_t o = ((g * T + t) * kSirState + c) * P + p;
+            out[((t * G + g) * P + p) * kSirState + c] =
+                h->fp64 ? reinterpret_cast<const double*>(tmp.data())[o]
+                        : reinterpret_cast<const float*>(tmp.data())[o];
+          }
+        }
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_last_ancestry(dust2_filter* h, uint32_t* out, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (!h->ancestry_valid) {
+      throw std::runtime_error("Ancestry is not current; run with 'save_trajectories'");
+    }
+    const size_t G = h->G, P = h->P, T = h->n_time;
+    std::vector<uint32_t> tmp(G * T * P);
+    to_host(tmp.data(), h->d_ancestry, tmp.size() * sizeof(uint32_t), h->stream);
+    for (size_t g = 0; g < G; ++g) {
+      for (size_t t = 0; t < T; ++t) {
+        std::memcpy(out + (t * G + g) * P, tmp.data() + (g * T + t) * P, P * sizeof(uint32_t));
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    const size_t n = h->G * (h->P + 2);
+    dbuf tmp(n * 32);
+    hip_check(launch_rng_soa_to_aos(h->d_rng.get<uint64_t>(), tmp.get<uint64_t>(),
+                                    static_cast<int>(h->G), static_cast<int>(h->P + 2),
+                                    static_cast<int>(h->P + 2), h->stream),
+              "rng layout");
+    to_host(out, tmp, n * 32, h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_set_rng_state(dust2_filter* h, const uint64_t* state, char* err,
+                                   size_t errlen) {
+  try {
+    check_filter(h);
+    const size_t n = h->G * (h->P + 2);
+    dbuf tmp(n * 32);
+    to_device(tmp, state, n * 32, h->stream);
+    hip_check(launch_rng_aos_to_soa(tmp.get<uint64_t>(), h->d_rng.get<uint64_t>(),
+                                    static_cast<int>(h->G), static_cast<int>(h->P + 2),
+                                    static_cast<int>(h->P + 2), h->stream),
+              "rng layout");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_set_launch(dust2_filter* h, int block, int ppt, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    const launch_config c{block, ppt};
+    if (!sir_filter_config_valid(c, h->fp64)) {
+      throw std::runtime_error("unsupported launch shape");
+    }
+    if (static_cast<size_t>(block) * ppt < h->P) {
+      throw std::runtime_error("launch shape holds fewer than n_particles particles");
+    }
+    h->cfg = c;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    out[0] = h->last_fallbacks;
+    out[1] = h->last_ms;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_filter_sir_free(dust2_filter* h) { delete h; }
+
+}  // extern "C"
+
+// ============================================================================
+// System
+// ============================================================================
+struct dust2_system {
+  bool fp64 = true;
+  bool deterministic = false;
+  size_t G = 0, P = 0;
+  double time = 0, dt = 1;
+  std::vector<double> pars;  // G x 5
+  hipStream_t stream = nullptr;
+  dbuf d_state, d_tmp, d_rng, d_pars, d_errors, d_counters, d_obs, d_lfact, d_out, d_idx;
+  bool errors_pending = false;
+  size_t real_size() const { return fp64 ? 8 : 4; }
+  ~dust2_system() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void check_system(dust2_system* h, const char* action) {
+  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (system)");
+  if (h->errors_pending) {
+    throw std::runtime_error(std::string("Can't currently ") + action +
+                             " this system: errors are pending");
+  }
+}
+
+void system_upload_pars(dust2_system* h) {
+  std::vector<double> kp(h->G * 8);
+  for (size_t g = 0; g < h->G; ++g) sir_kernel_pars(&h->pars[g * kSirPars], h->dt, &kp[g * 8]);
+  to_device(h->d_pars, kp.data(), kp.size() * sizeof(double), h->stream);
+}
+
+void system_run(dust2_system* h, double time, bool set_initial) {
+  const double diff = std::max(0.0, time - h->time);
+  const size_t n_steps = static_cast<size_t>(std::round(diff / h->dt));
+  hipStream_t s = h->stream;
+  hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+  const uint32_t flags = (h->deterministic ? SLOT_DETERMINISTIC : 0u) |
+                         (set_initial ? SLOT_SET_INITIAL : 0u);
+  hip_check(launch_sir_run_to_time(h->d_state.get(), h->d_rng.get<uint64_t>(),
+                                   h->d_pars.get<double>(), static_cast<int>(h->G),
+                                   static_cast<int>(h->P), h->time, h->dt,
+                                   static_cast<int>(n_steps), flags, h->fp64,
+                                   h->d_errors.get<error_record>(),
+                                   h->d_counters.get<uint32_t>(), s),
+            "run_to_time kernel");
+  uint32_t count = 0;
+  to_host(&count, h->d_counters, sizeof(count), s);
+  if (count == 0) {
+    // on error the reference throws before advancing time
+    // (dust2/discrete/system.hpp:87-91)
+    h->time = h->time + n_steps * h->dt;
+  } else {
+    std::vector<error_record> rec(h->G * h->P);
+    to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->errors_pending = true;
+    throw std::runtime_error(error_report(rec, count));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particles, double time,
+                           double dt, const uint64_t* seed_state, size_t n_seed_streams,
+                           uint64_t seed, int deterministic, int precision, dust2_system** out,
+                           char* err, size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    check_dt(dt);
+    if (precision != DUST2_FP64 && precision != DUST2_FP32) {
+      throw std::runtime_error("'precision' must be DUST2_FP64 or DUST2_FP32");
+    }
+    for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * kSirPars);
+    auto h = std::make_unique<dust2_system>();
+    h->fp64 = precision == DUST2_FP64;
+    h->deterministic = deterministic != 0;
+    h->G = n_groups;
+    h->P = n_particles;
+    h->time = time;
+    h->dt = dt;
+    h->pars.assign(pars, pars + n_groups * kSirPars);
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t G = n_groups, P = n_particles;
+    h->d_state.alloc(G * kSirState * P * h->real_size());
+    h->d_tmp.alloc(G * kSirState * P * h->real_size());
+    h->d_rng.alloc(G * 4 * P * sizeof(uint64_t));
+    h->d_pars.alloc(G * 8 * sizeof(double));
+    h->d_errors.alloc(G * P * sizeof(error_record));
+    h->d_counters.alloc(4 * sizeof(uint32_t));
+    hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
+    {
+      dbuf tmp(G * P * 32);
+      expand_streams(seed_state, n_seed_streams, seed, G * P, tmp.get<uint64_t>(), h->stream);
+      hip_check(launch_rng_aos_to_soa(tmp.get<uint64_t>(), h->d_rng.get<uint64_t>(),
+                                      static_cast<int>(G), static_cast<int>(P),
+                                      static_cast<int>(P), h->stream),
+                "rng layout");
+      hip_check(hipStreamSynchronize(h->stream), "sync");
+    }
+    system_upload_pars(h.get());
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_run_to_time(dust2_system* h, double time, char* err, size_t errlen) {
+  try {
+    check_system(h, "run");
+    system_run(h, time, false);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_simulate(dust2_system* h, const double* times, size_t n_times,
+                              double* out, char* err, size_t errlen) {
+  try {
+    check_system(h, "simulate");
+    const size_t block = h->G * h->P * kSirState;
+    for (size_t i = 0; i < n_times; ++i) {
+      system_run(h, times[i], false);
+      std::vector<char> tmp(h->d_state.bytes());
+      to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
+      soa_to_aos(tmp.data(), out + i * block, h->G, h->P, kSirState, h->fp64);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_state(dust2_system* h, double* out, char* err, size_t errlen) {
+  try {
+    check_system(h, "get state from");
+    std::vector<char> tmp(h->d_state.bytes());
+    to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
+    soa_to_aos(tmp.data(), out, h->G, h->P, kSirState, h->fp64);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err, size_t errlen) {
+  try {
+    if (h == nullptr) check_system(h, "set state for");
+    std::vector<char> tmp(h->d_state.bytes());
+    aos_to_soa(state, tmp.data(), h->G, h->P, kSirState, h->fp64);
+    to_device(h->d_state, tmp.data(), tmp.size(), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    h->errors_pending = false;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_set_state_initial(dust2_system* h, char* err, size_t errlen) {
+  try {
+    if (h == nullptr) check_system(h, "set state for");
+    h->errors_pending = false;
+    // a zero-step run from the initial state writes initial(); the model's
+    // initial() draws no random numbers (sir.cpp:38-48)
+    system_run(h, h->time, true);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_time(dust2_system* h, double* out, char* err, size_t errlen) {
+  try {
+    check_system(h, "get time from");
+    *out = h->time;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_set_time(dust2_system* h, double time, char* err, size_t errlen) {
+  try {
+    check_system(h, "set time for");
+    h->time = time;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_update_pars(dust2_system* h, const double* pars, char* err, size_t errlen) {
+  try {
+    if (h == nullptr) check_system(h, "update pars for");
+    for (size_t g = 0; g < h->G; ++g) {
+      const double* row = pars + g * kSirPars;
+      check_sir_pars(row);
+      h->pars[g * kSirPars + 1] = row[1];
+      h->pars[g * kSirPars + 2] = row[2];
+      h->pars[g * kSirPars + 3] = row[3];
+    }
+    system_upload_pars(h);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_compare_data(dust2_system* h, const double* data, size_t n_groups_data,
+                                  double* out, char* err, size_t errlen) {
+  try {
+    check_system(h, "compare data for");
+    if (n_groups_data != 1 && n_groups_data != h->G) {
+      throw std::runtime_error("Expected 'data' to have length " + std::to_string(h->G) +
+                               ", but it had length " + std::to_string(n_groups_data));
+    }
+    std::vector<double> lf(n_groups_data);
+    for (size_t i = 0; i < n_groups_data; ++i) {
+      lf[i] = std::isnan(data[i]) ? 0.0 : std::lgamma(data[i] + 1);
+    }
+    hipStream_t s = h->stream;
+    h->d_obs.alloc(n_groups_data * sizeof(double));
+    h->d_lfact.alloc(n_groups_data * sizeof(double));
+    h->d_out.alloc(h->G * h->P * sizeof(double));
+    to_device(h->d_obs, data, n_groups_data * sizeof(double), s);
+    to_device(h->d_lfact, lf.data(), n_groups_data * sizeof(double), s);
+    hip_check(launch_sir_compare(h->d_state.get(), h->d_rng.get<uint64_t>(),
+                                 h->d_pars.get<double>(), h->d_obs.get<double>(),
+                                 h->d_lfact.get<double>(), n_groups_data == 1 ? 0 : 1,
+                                 static_cast<int>(h->G), static_cast<int>(h->P),
+                                 h->deterministic ? SLOT_DETERMINISTIC : 0u, h->fp64,
+                                 h->d_out.get<double>(), s),
+              "compare kernel");
+    to_host(out, h->d_out, h->G * h->P * sizeof(double), s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err, size_t errlen) {
+  try {
+    check_system(h, "reorder");
+    for (size_t i = 0; i < h->G * h->P; ++i) {
+      if (index[i] >= h->P) throw std::runtime_error("Invalid index for reorder");
+    }
+    hipStream_t s = h->stream;
+    h->d_idx.alloc(h->G * h->P * sizeof(uint32_t));
+    to_device(h->d_idx, index, h->G * h->P * sizeof(uint32_t), s);
+    hip_check(launch_reorder(h->d_state.get(), h->d_tmp.get(), h->d_idx.get<uint32_t>(),
+                             static_cast<int>(h->G), kSirState, static_cast<int>(h->P), h->fp64,
+                             s),
+              "reorder kernel");
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->d_state.swap(h->d_tmp);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen) {
+  try {
+    if (h == nullptr) check_system(h, "get rng state from");
+    const size_t n = h->G * h->P;
+    dbuf tmp(n * 32);
+    hip_check(launch_rng_soa_to_aos(h->d_rng.get<uint64_t>(), tmp.get<uint64_t>(),
+                                    static_cast<int>(h->G), static_cast<int>(h->P),
+                                    static_cast<int>(h->P), h->stream),
+              "rng layout");
+    to_host(out, tmp, n * 32, h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
+                                   size_t errlen) {
+  try {
+    if (h == nullptr) check_system(h, "set rng state for");
+    const size_t n = h->G * h->P;
+    dbuf tmp(n * 32);
+    to_device(tmp, state, n * 32, h->stream);
+    hip_check(launch_rng_aos_to_soa(tmp.get<uint64_t>(), h->d_rng.get<uint64_t>(),
+                                    static_cast<int>(h->G), static_cast<int>(h->P),
+                                    static_cast<int>(h->P), h->stream),
+              "rng layout");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_system_sir_free(dust2_system* h) { delete h; }
+
+// ============================================================================
+// Test entry points
+// ============================================================================
+int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* idx,
+                       double* out3, char* err, size_t errlen) {
+  try {
+    if (n < 1 || n > 2048) throw std::runtime_error("n must lie in [1, 2048]");
+    hipStream_t s = nullptr;
+    dbuf dw(n * sizeof(double)), di(n * sizeof(uint32_t)), dl(3 * sizeof(double));
+    to_device(dw, log_weights, n * sizeof(double), s);
+    hip_check(launch_resample_test(dw.get<double>(), static_cast<int>(n), u, di.get<uint32_t>(),
+                                   dl.get<double>(), s),
+              "resample kernel");
+    to_host(idx, di, n * sizeof(uint32_t), s);
+    to_host(out3, dl, 3 * sizeof(double), s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                          size_t n_streams, uint64_t* out, char* err, size_t errlen) {
+  try {
+    hipStream_t s = nullptr;
+    dbuf d(n_streams * 32);
+    expand_streams(seed_state, n_seed_streams, seed, n_streams, d.get<uint64_t>(), s);
+    to_host(out, d, n_streams * 32, s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams, size_t n_draws,
+                     const double* a, const double* b, int deterministic, double* out,
+                     uint32_t* err_out, char* err, size_t errlen) {
+  try {
+    if (kind < 0 || kind > 4) throw std::runtime_error("unknown sampler kind");
+    hipStream_t s = nullptr;
+    dbuf dr(n_streams * 32), da(n_streams * 8), db(n_streams * 8),
+        dout(n_streams * n_draws * 8), derr(n_streams * 4);
+    to_device(dr, rng, n_streams * 32, s);
+    to_device(da, a, n_streams * 8, s);
+    to_device(db, b, n_streams * 8, s);
+    hip_check(launch_sample_test(kind, precision == DUST2_FP64, dr.get<uint64_t>(),
+                                 static_cast<int>(n_streams), static_cast<int>(n_draws),
+                                 da.get<double>(), db.get<double>(), deterministic,
+                                 dout.get<double>(), derr.get<uint32_t>(), s),
+              "sample kernel");
+    to_host(out, dout, n_streams * n_draws * 8, s);
+    to_host(err_out, derr, n_streams * 4, s);
+    to_host(rng, dr, n_streams * 32, s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+}  // extern "C"

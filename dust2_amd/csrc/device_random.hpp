@@ -1,0 +1,302 @@
+// Device-side random number generation and samplers for the particle
+// engine (gfx950).  One xoshiro256+ stream per particle, held in registers
+// as four 64-bit words while a particle is being advanced.
+//
+// Replaces monty::random::{generator<double>, random_real, binomial,
+// exponential_rate, normal, poisson} and monty::density::poisson as called
+// from the reference models (inst/examples/sir.cpp:63-66,115-119;
+// dust2/filter.hpp:153).  Algorithms: xoshiro256+ (Blackman & Vigna),
+// Hormann (1993) BTRS for large-mean binomials, inversion by sequential
+// search for small means, inversion for the exponential, Box-Muller for the
+// normal, Hormann (1993) PTRS for large-mean Poisson.
+//
+// Every function is written for both double and float `real`.  The double
+// instantiation follows the arithmetic order of the FP64 CPU path term by
+// term (the library is compiled with -ffp-contract=off), so FP64 draws are
+// reproducible draw-for-draw up to last-place differences between device
+// and host transcendental functions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine.hpp"
+
+namespace dust2_amd {
+
+struct rng256 {
+  uint64_t s0, s1, s2, s3;
+};
+
+__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) {
+  return (x << k) | (x >> (64 - k));
+}
+
+__host__ __device__ __forceinline__ uint64_t rng_next(rng256& r) {
+  const uint64_t out = r.s0 + r.s3;
+  const uint64_t t = r.s1 << 17;
+  r.s2 ^= r.s0;
+  r.s3 ^= r.s1;
+  r.s1 ^= r.s2;
+  r.s0 ^= r.s3;
+  r.s2 ^= t;
+  r.s3 = rotl64(r.s3, 45);
+  return out;
+}
+
+template <typename real>
+__device__ __forceinline__ real random_real(rng256& r);
+
+// 53 high bits -> [0, 1) on the 2^-53 grid.
+template <>
+__device__ __forceinline__ double random_real<double>(rng256& r) {
+  return static_cast<double>(rng_next(r) >> 11) * 0x1.0p-53;
+}
+
+// 24 high bits -> [0, 1) on the 2^-24 grid.
+template <>
+__device__ __forceinline__ float random_real<float>(rng256& r) {
+  return static_cast<float>(static_cast<uint32_t>(rng_next(r) >> 40)) * 0x1.0p-24f;
+}
+
+__device__ __forceinline__ double d_log(double x) { return log(x); }
+__device__ __forceinline__ float d_log(float x) { return logf(x); }
+__device__ __forceinline__ double d_exp(double x) { return exp(x); }
+__device__ __forceinline__ float d_exp(float x) { return expf(x); }
+__device__ __forceinline__ double d_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float d_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double d_floor(double x) { return floor(x); }
+__device__ __forceinline__ float d_floor(float x) { return floorf(x); }
+__device__ __forceinline__ double d_fabs(double x) { return fabs(x); }
+__device__ __forceinline__ float d_fabs(float x) { return fabsf(x); }
+// 1 - exp(x) for x <= 0: the probability of at least one event at rate -x.
+// FP64 keeps the reference's expression (inst/examples/sir.cpp:63) so draws
+// reproduce; FP32 uses expm1, since 1 - expf(x) cancels catastrophically
+// for the small rates typical of a step.
+__device__ __forceinline__ double prob_from_rate(double x) { return 1 - exp(x); }
+__device__ __forceinline__ float prob_from_rate(float x) { return -expm1f(x); }
+
+// (1 - p)^n: FP64 as the reference's pow(q, n); FP32 through log1p so a
+// p near the float resolution of 1 - p keeps its accuracy.
+__device__ __forceinline__ double pow_one_minus(double q, double p, double n) {
+  (void)p;
+  return pow(q, n);
+}
+__device__ __forceinline__ float pow_one_minus(float q, float p, float n) {
+  (void)q;
+  return expf(n * log1pf(-p));
+}
+
+// Remainder of Stirling's series, log(k!) - [(k+1/2) log(k+1) - (k+1) +
+// log(2 pi)/2]: exact values for k <= 9 (computed from the definition to 60
+// digits), asymptotic series above.
+template <typename real>
+__device__ __forceinline__ real stirling_tail(real k) {
+  if (k <= 9) {
+    const int i = static_cast<int>(k);
+    // a switch keeps the table in scalar constants rather than scratch
+    switch (i) {
+      case 0: return static_cast<real>(0.08106146679532726);
+      case 1: return static_cast<real>(0.0413406959554093);
+      case 2: return static_cast<real>(0.02767792568499834);
+      case 3: return static_cast<real>(0.020790672103765093);
+      case 4: return static_cast<real>(0.016644691189821193);
+      case 5: return static_cast<real>(0.013876128823070748);
+      case 6: return static_cast<real>(0.01189670994589177);
+      case 7: return static_cast<real>(0.010411265261972096);
+      case 8: return static_cast<real>(0.009255462182712733);
+      default: return static_cast<real>(0.00833056343336287);
+    }
+  }
+  const real x = k + 1;
+  const real x2 = x * x;
+  return (static_cast<real>(1.0 / 12) -
+          (static_cast<real>(1.0 / 360) - static_cast<real>(1.0 / 1260) / x2) / x2) / x;
+}
+
+// Inversion by sequential search, n*p < 10, p <= 1/2:
+// pmf(k) = pmf(k-1) * (p/q) * (n+1-k)/k.  Restarts with a fresh uniform if
+// rounding lets the search run past n.
+template <typename real>
+__device__ real binomial_inversion(rng256& rng, real n, real p) {
+  const real q = 1 - p;
+  const real r = p / q;
+  const real g = r * (n + 1);
+  while (true) {
+    real u = random_real<real>(rng);
+    real f = pow_one_minus(q, p, n);
+    real k = 0;
+    bool ok = true;
+    while (u >= f) {
+      u -= f;
+      k += 1;
+      if (k > n) {
+        ok = false;
+        break;
+      }
+      f *= (g / k - r);
+    }
+    if (ok) {
+      return k;
+    }
+  }
+}
+
+// BTRS (Hormann 1993), p <= 1/2, n*p >= 10.  The constants needed only on
+// the rejection path (log(p/q), the mode term h, alpha) are computed the
+// first time a proposal falls outside the squeeze, and that path is always
+// evaluated in FP64: its log-factorial differences cancel terms of size
+// n*log(n), beyond float resolution for the population sizes modelled.
+template <typename real>
+__device__ real binomial_btrs(rng256& rng, real n, real p) {
+  const real q = 1 - p;
+  const real spq = d_sqrt(n * p * q);
+  const real b = static_cast<real>(1.15) + static_cast<real>(2.53) * spq;
+  const real a = static_cast<real>(-0.0873) + static_cast<real>(0.0248) * b +
+                 static_cast<real>(0.01) * p;
+  const real c = n * p + static_cast<real>(0.5);
+  const real v_r = static_cast<real>(0.92) - static_cast<real>(4.2) / b;
+  bool have_bound = false;
+  double alpha = 0, lpq = 0, m = 0, h = 0;
+  const double nd = n;
+  while (true) {
+    const real u = random_real<real>(rng) - static_cast<real>(0.5);
+    const real v = random_real<real>(rng);
+    const real us = static_cast<real>(0.5) - d_fabs(u);
+    const real k = d_floor((2 * a / us + b) * u + c);
+    if (us >= static_cast<real>(0.07) && v <= v_r) {
+      return k;
+    }
+    if (k < 0 || k > n) {
+      continue;
+    }
+    if (!have_bound) {
+      const double pd = p;
+      alpha = (2.83 + 5.1 / static_cast<double>(b)) * static_cast<double>(spq);
+      lpq = log(pd / static_cast<double>(q));
+      m = floor((nd + 1) * pd);
+      h = (m + 0.5) * log(m + 1) + stirling_tail(m) + (nd - m + 0.5) * log(nd - m + 1) +
+          stirling_tail(nd - m);
+      have_bound = true;
+    }
+    const double kd = k;
+    const double usd = us;
+    const double lv = log(static_cast<double>(v) * alpha /
+                          (static_cast<double>(a) / (usd * usd) + static_cast<double>(b)));
+    const double bound = h - (kd + 0.5) * log(kd + 1) - stirling_tail(kd) -
+                         (nd - kd + 0.5) * log(nd - kd + 1) - stirling_tail(nd - kd) +
+                         (kd - m) * lpq;
+    if (lv <= bound) {
+      return k;
+    }
+  }
+}
+
+// Binomial(n, p).  Returns ERR_BINOMIAL in *err (and 0) on invalid input,
+// as the reference throws; deterministic streams return n*p.
+template <typename real>
+__device__ __forceinline__ real binomial(rng256& rng, real n, real p, bool deterministic,
+                                         uint32_t& err) {
+  if (n < 0 || p < 0 || p > 1) {
+    err = ERR_BINOMIAL;
+    return 0;
+  }
+  if (deterministic) {
+    return n * p;
+  }
+  if (n == 0 || p == 0) {
+    return 0;
+  }
+  if (p == 1) {
+    return n;
+  }
+  const bool flip = p > static_cast<real>(0.5);
+  const real q = flip ? 1 - p : p;
+  const real k = (n * q >= 10) ? binomial_btrs<real>(rng, n, q)
+                               : binomial_inversion<real>(rng, n, q);
+  return flip ? n - k : k;
+}
+
+// Exp(rate) by inversion of 1 - U (exact, in (0, 1]).
+template <typename real>
+__device__ __forceinline__ real exponential_rate(rng256& rng, real rate, bool deterministic) {
+  if (deterministic) {
+    return 1 / rate;
+  }
+  return -d_log(1 - random_real<real>(rng)) / rate;
+}
+
+// Box-Muller normal.
+template <typename real>
+__device__ __forceinline__ real normal(rng256& rng, real mean, real sd, bool deterministic) {
+  if (deterministic) {
+    return mean;
+  }
+  real u1, u2;
+  do {
+    u1 = random_real<real>(rng);
+    u2 = random_real<real>(rng);
+  } while (u1 <= static_cast<real>(sizeof(real) == 8 ? 2.2250738585072014e-308 : 1.17549435e-38));
+  const real two_pi = static_cast<real>(2 * 3.14159265358979323846);
+  const real z = d_sqrt(-2 * d_log(u1)) * (sizeof(real) == 8 ? static_cast<real>(cos(static_cast<double>(two_pi * u2)))
+                                                               : static_cast<real>(cosf(static_cast<float>(two_pi * u2))));
+  return mean + sd * z;
+}
+
+// Poisson(lambda): multiplication below 10, PTRS above.
+template <typename real>
+__device__ real poisson(rng256& rng, real lambda, bool deterministic) {
+  if (deterministic) {
+    return lambda;
+  }
+  if (lambda == 0) {
+    return 0;
+  }
+  if (lambda < 10) {
+    const real l = d_exp(-lambda);
+    real k = 0;
+    real prod = random_real<real>(rng);
+    while (prod > l) {
+      k += 1;
+      prod *= random_real<real>(rng);
+    }
+    return k;
+  }
+  const real slam = d_sqrt(lambda);
+  const real loglam = d_log(lambda);
+  const real b = static_cast<real>(0.931) + static_cast<real>(2.53) * slam;
+  const real a = static_cast<real>(-0.059) + static_cast<real>(0.02483) * b;
+  const real inv_alpha = static_cast<real>(1.1239) + static_cast<real>(1.1328) / (b - static_cast<real>(3.4));
+  const real v_r = static_cast<real>(0.9277) - static_cast<real>(3.6224) / (b - 2);
+  while (true) {
+    const real u = random_real<real>(rng) - static_cast<real>(0.5);
+    const real v = random_real<real>(rng);
+    const real us = static_cast<real>(0.5) - d_fabs(u);
+    const real k = d_floor((2 * a / us + b) * u + lambda + static_cast<real>(0.43));
+    if (us >= static_cast<real>(0.07) && v <= v_r) {
+      return k;
+    }
+    if (k < 0 || (us < static_cast<real>(0.013) && v > us)) {
+      continue;
+    }
+    const real lg = sizeof(real) == 8 ? static_cast<real>(lgamma(static_cast<double>(k + 1)))
+                                      : static_cast<real>(lgammaf(static_cast<float>(k + 1)));
+    if (d_log(v) + d_log(inv_alpha) - d_log(a / (us * us) + b) <= -lambda + k * loglam - lg) {
+      return k;
+    }
+  }
+}
+
+// log Poisson pmf in FP64 with log(x!) supplied by the caller (it depends
+// on the data only and is computed once on the host).
+__device__ __forceinline__ double dpois_log(double x, double lambda, double lfact_x) {
+  if (x == 0 && lambda == 0) {
+    return 0;
+  }
+  if (lambda == 0) {
+    return -__builtin_inf();
+  }
+  return x * log(lambda) - lambda - lfact_x;
+}
+
+}  // namespace dust2_amd

@@ -1,0 +1,111 @@
+// Internal interface between the host runtime (api.cpp) and the kernel
+// launchers (*.hip).  Plain C++ types only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace dust2_amd {
+
+// Error codes recorded per particle; the host turns them into the
+// reference's messages.
+enum sampler_error : uint32_t {
+  ERR_NONE = 0,
+  ERR_BINOMIAL = 1,        // "Invalid call to binomial with n = .., p = .., q = .."
+  ERR_ODE_MAX_STEPS = 2,   // "too many steps"
+  ERR_ODE_STEP_SMALL = 3,  // "step too small"
+  ERR_ODE_VANISHED = 4,    // "step size vanished"
+  ERR_ODE_INIT = 5,        // "Initial step size was not finite"
+};
+
+// Doubles of LDS scratch every filter workgroup reserves ahead of its
+// P-length weight array (per-wave partials and broadcast slots).
+constexpr int kScratchDoubles = 64;
+
+enum slot_flags : uint32_t {
+  SLOT_SET_INITIAL = 1u,    // start from the model's initial state
+  SLOT_DETERMINISTIC = 2u,  // samplers return their means
+  SLOT_SAVE_TRAJ = 4u,      // write the post-reorder state at every time
+};
+
+// Per-particle error record (first error of a particle).
+struct error_record {
+  uint32_t code;
+  uint32_t interval;
+  double a;
+  double b;
+};
+
+// One parameter group's work in one launch of the filter kernel.  All
+// pointers are device pointers; one workgroup runs one slot.
+struct filter_slot {
+  double pars[8];          // model parameters (model-specific order)
+  const double* times;     // absolute data times, indexed by interval
+  const double* obs;       // observations, obs[t * obs_stride + i]
+  const double* lfact;     // log(obs!) in the same layout
+  uint32_t obs_stride;
+  uint32_t n_particles;
+  uint32_t t_begin;        // first interval run by this launch
+  uint32_t t_end;          // one past the last
+  uint32_t flags;
+  uint32_t group;          // group index, used in error records
+  double dt;
+  void* state;             // real SoA [n_state][n_particles]
+  uint64_t* rng;           // SoA [4][n_particles + 2]; streams 0, 1 = filter
+  double* ll;              // log-likelihood accumulator (in/out)
+  double* time;            // system time (in/out)
+  uint32_t* ancestry;      // [n_time][n_particles] or null
+  void* traj;              // real [n_time][n_state][n_particles] or null
+  error_record* errors;    // [n_particles]
+  uint32_t* error_count;   // shared counter
+  uint32_t* n_fallback;    // sequential-resample fallbacks (statistics)
+};
+
+struct launch_config {
+  int block;
+  int ppt;
+};
+
+// Persistent filter: every slot advances through intervals
+// [t_begin, t_end) with particle state held in registers.
+hipError_t launch_sir_filter(const filter_slot* d_slots, int n_slots, int max_particles,
+                             bool fp64, launch_config cfg, hipStream_t stream);
+
+// Largest particle count one workgroup can hold for this precision.
+int sir_filter_max_particles(bool fp64);
+launch_config sir_filter_default_config(int n_particles, bool fp64);
+bool sir_filter_config_valid(launch_config cfg, bool fp64);
+
+// ---- system (state resident in HBM between calls) -------------------------
+// state: real [G][S][P]; rng: [G][4][P]; pars: [G][8] doubles.
+hipError_t launch_sir_run_to_time(void* state, uint64_t* rng, const double* pars, int G,
+                                  int P, double time, double dt, int n_steps,
+                                  uint32_t flags, bool fp64, error_record* errors,
+                                  uint32_t* error_count, hipStream_t stream);
+hipError_t launch_sir_compare(const void* state, uint64_t* rng, const double* pars,
+                              const double* obs, const double* lfact, int obs_group_stride,
+                              int G, int P, uint32_t flags, bool fp64, double* out,
+                              hipStream_t stream);
+hipError_t launch_reorder(const void* state_in, void* state_out, const uint32_t* idx, int G,
+                          int S, int P, bool fp64, hipStream_t stream);
+
+// ---- RNG -------------------------------------------------------------------
+// out[n] (AoS, 4 words per stream) = jump^n(s0) for n < n_streams, using
+// the column-major GF(2) matrices J_i = jump^(2^i), i < n_mats.
+hipError_t launch_seed_streams(const uint64_t* d_s0, const uint64_t* d_mats, int n_mats,
+                               uint64_t n_streams, uint64_t* d_out, hipStream_t stream);
+// AoS raw streams <-> per-group SoA [G][4][stride] at column offset `first`.
+hipError_t launch_rng_aos_to_soa(const uint64_t* aos, uint64_t* soa, int G, int per_group,
+                                 int stride, hipStream_t stream);
+hipError_t launch_rng_soa_to_aos(const uint64_t* soa, uint64_t* aos, int G, int per_group,
+                                 int stride, hipStream_t stream);
+
+// ---- test entry points -----------------------------------------------------
+hipError_t launch_resample_test(const double* w, int n, double u, uint32_t* idx,
+                                double* ll_increment, hipStream_t stream);
+hipError_t launch_sample_test(int kind, bool fp64, uint64_t* rng_aos, int n_streams,
+                              int n_draws, const double* a, const double* b, int det,
+                              double* out, uint32_t* err, hipStream_t stream);
+
+}  // namespace dust2_amd

@@ -1,0 +1,401 @@
+"""Python host mirror of dust2's R interface for the accelerated path.
+
+Function names, argument meaning and error behaviour follow the reference's
+R API (R/interface.R, R/interface-filter.R, R/interface-unfilter.R,
+R/interface-likelihood.R); the work is done by the C-ABI library through
+``_lib``.  Arrays use the reference's memory order: a state array that R
+shapes [state, particle, group] is returned here as numpy shape
+(group, particle, state) over the same memory.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, dptr, f64, szptr, u32ptr, u64ptr
+
+FP64 = 0
+FP32 = 1
+
+
+class Generator:
+    """A model class compiled into the library (the analogue of an odin2 /
+    dust2 generator, inst/template/dust.R:1-12)."""
+
+    def __init__(self, name, time_type, parameters, defaults, constant, state, n_data=1):
+        self.name = name
+        self.time_type = time_type
+        self.parameters = parameters
+        self.defaults = defaults
+        self.constant = constant
+        self.state = state
+        self.n_data = n_data
+
+    @property
+    def n_state(self):
+        return len(self.state)
+
+    def __repr__(self):
+        return f"<dust_system_generator: {self.name} ({self.time_type})>"
+
+
+def sir():
+    """Stochastic SIR (inst/examples/sir.cpp:12-199)."""
+    return Generator(
+        "sir", "discrete", ["N", "I0", "beta", "gamma", "exp_noise"],
+        {"N": 1000.0, "I0": 10.0, "beta": 0.2, "gamma": 0.1, "exp_noise": 1e6},
+        {"N", "exp_noise"}, ["S", "I", "R", "cases_cumul", "cases_inc"])
+
+
+def sirode():
+    """Deterministic SIR ODE (inst/examples/sirode.cpp:9-102)."""
+    return Generator(
+        "sirode", "continuous", ["N", "I0", "beta", "gamma"],
+        {"N": 1000.0, "I0": 10.0, "beta": 0.2, "gamma": 0.1},
+        {"N"}, ["S", "I", "R", "cases_cumul", "cases_inc"])
+
+
+def _precision(p):
+    if p in ("fp64", "double", FP64):
+        return FP64
+    if p in ("fp32", "float", "single", FP32):
+        return FP32
+    raise ValueError("'precision' must be 'fp64' or 'fp32'")
+
+
+def pars_matrix(generator, pars, n_groups):
+    """Rows of model parameters, one per group; `pars` is a dict (shared by
+    all groups) or a list of n_groups dicts (R/interface.R:68-120)."""
+    if isinstance(pars, dict):
+        pars = [pars] * n_groups
+    if len(pars) != n_groups:
+        raise ValueError(f"Expected 'pars' to have length {n_groups}, but it had length "
+                         f"{len(pars)}")
+    rows = []
+    for p in pars:
+        unknown = set(p) - set(generator.parameters)
+        if unknown:
+            raise ValueError(f"Unknown parameters: {', '.join(sorted(unknown))}")
+        rows.append([float(p.get(k, generator.defaults[k])) for k in generator.parameters])
+    return np.ascontiguousarray(rows, dtype=np.float64)
+
+
+def pars_update(generator, pars, current):
+    """New parameter rows for an update: names absent from `pars` keep their
+    current values (update_shared, inst/examples/sir.cpp:89-93); constant
+    parameters cannot change."""
+    if isinstance(pars, dict):
+        pars = [pars] * current.shape[0]
+    if len(pars) != current.shape[0]:
+        raise ValueError(f"Expected 'pars' to have length {current.shape[0]}, but it had "
+                         f"length {len(pars)}")
+    rows = current.copy()
+    for i, p in enumerate(pars):
+        for k, v in p.items():
+            if k not in generator.parameters:
+                raise ValueError(f"Unknown parameters: {k}")
+            j = generator.parameters.index(k)
+            if k in generator.constant and float(v) != rows[i, j]:
+                raise ValueError(f"Parameter '{k}' is constant and cannot be updated")
+            rows[i, j] = float(v)
+    return np.ascontiguousarray(rows)
+
+
+def _seed_args(seed):
+    """seed: None (random), integer, or raw uint64 stream state (n x 4 words)."""
+    if seed is None:
+        seed = int(np.random.default_rng().integers(0, 2**31 - 1))
+    if isinstance(seed, (int, np.integer)):
+        return None, 0, C.c_uint64(int(seed) & (2**64 - 1))
+    raw = np.ascontiguousarray(seed, dtype=np.uint64).reshape(-1)
+    if raw.size % 4 != 0 or raw.size == 0:
+        raise ValueError("Expected raw seed to be a multiple of 4 uint64 words")
+    return raw, raw.size // 4, C.c_uint64(0)
+
+
+def _index_group(index_group, n_groups):
+    if index_group is None:
+        return None, 0, n_groups
+    idx = np.ascontiguousarray(index_group, dtype=np.uintp).reshape(-1)
+    return idx, idx.size, idx.size
+
+
+def prepare_data(data, n_groups, name="incidence"):
+    """Data as in R/data.R:114-175: a dict with 'time' (n_time) and the
+    observed column, shape (n_time,) shared or (n_time, n_groups)."""
+    time = f64(data["time"]).reshape(-1)
+    obs = f64(data[name])
+    if obs.ndim == 1:
+        if obs.shape[0] != time.shape[0]:
+            raise ValueError("Expected data columns to have the same length as 'time'")
+        n_groups_data = 1
+        obs = obs.reshape(-1, 1)
+    else:
+        if obs.shape != (time.shape[0], n_groups):
+            raise ValueError(f"Expected '{name}' to have shape ({time.shape[0]}, {n_groups})")
+        n_groups_data = n_groups
+    return time, np.ascontiguousarray(obs), n_groups_data
+
+
+# ---------------------------------------------------------------------------
+# Filter
+# ---------------------------------------------------------------------------
+class Filter:
+    """A particle filter object (R/interface-filter.R:37-97), initialised
+    lazily on the first likelihood run (R/interface-likelihood.R:48-119)."""
+
+    def __init__(self, generator, time_start, data, n_particles, n_groups, dt, seed,
+                 precision):
+        if generator.time_type != "discrete":
+            raise ValueError("Can't use 'dust_filter_create()' with continuous-time systems")
+        self.generator = generator
+        self.time_start = float(time_start)
+        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups)
+        self.n_particles = int(n_particles)
+        self.n_groups = int(n_groups)
+        self.dt = float(dt)
+        self.seed = seed
+        self.precision = _precision(precision)
+        self.ptr = None
+        self.launch = None
+
+    @property
+    def initialised(self):
+        return self.ptr is not None
+
+    def _alloc(self, pars):
+        rows = pars_matrix(self.generator, pars, self.n_groups)
+        self.rows = rows
+        raw, n_raw, seed = _seed_args(self.seed)
+        out = C.c_void_p()
+        call("dust2_filter_sir_alloc", dptr(rows), self.n_groups, self.n_particles,
+             self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),
+             self.n_groups_data, u64ptr(raw), n_raw, seed, self.precision, C.byref(out))
+        self.ptr = out
+        if self.launch is not None:
+            call("dust2_filter_sir_set_launch", self.ptr, *self.launch)
+
+    def set_launch(self, block, particles_per_thread):
+        self.launch = (int(block), int(particles_per_thread))
+        if self.ptr is not None:
+            call("dust2_filter_sir_set_launch", self.ptr, *self.launch)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib._lib is not None:
+            _lib.lib().dust2_filter_sir_free(self.ptr)
+            self.ptr = None
+
+
+def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=1,
+                       seed=None, precision="fp64"):
+    return Filter(generator, time_start, data, n_particles, n_groups, dt, seed, precision)
+
+
+def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_group=None):
+    """Run the filter (or unfilter) and return the log-likelihood per group
+    (a float for a single group), R/interface-likelihood.R:48-119."""
+    if isinstance(obj, Unfilter):
+        return obj.run(pars, index_group)
+    if not obj.initialised:
+        if pars is None:
+            raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+        obj._alloc(pars)
+    elif pars is not None:
+        idx, n_idx, _ = _index_group(index_group, obj.n_groups)
+        sel = np.arange(obj.n_groups) if idx is None else idx.astype(np.int64)
+        rows = pars_update(obj.generator, pars, obj.rows[sel])
+        call("dust2_filter_sir_update_pars", obj.ptr, dptr(rows), szptr(idx), n_idx)
+        obj.rows[sel] = rows
+    idx, n_idx, n_out = _index_group(index_group, obj.n_groups)
+    init = None
+    if initial is not None:
+        init = f64(initial).reshape(-1)
+        if init.size != obj.n_groups * obj.n_particles * obj.generator.n_state:
+            raise ValueError("Expected 'initial' to have n_groups * n_particles * n_state values")
+    ll = np.zeros(n_out)
+    call("dust2_filter_sir_run", obj.ptr, dptr(init), int(bool(save_trajectories)), szptr(idx),
+         n_idx, dptr(ll))
+    return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
+
+
+def dust_likelihood_last_state(obj):
+    out = np.zeros((obj.n_groups, obj.n_particles, obj.generator.n_state))
+    call("dust2_filter_sir_last_state", obj.ptr, dptr(out))
+    return out
+
+
+def dust_likelihood_last_trajectories(obj):
+    out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
+    call("dust2_filter_sir_last_trajectories", obj.ptr, dptr(out))
+    return out
+
+
+def dust_likelihood_last_ancestry(obj):
+    out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles), dtype=np.uint32)
+    call("dust2_filter_sir_last_ancestry", obj.ptr, u32ptr(out))
+    return out
+
+
+def dust_likelihood_rng_state(obj):
+    out = np.zeros(obj.n_groups * (obj.n_particles + 2) * 4, dtype=np.uint64)
+    call("dust2_filter_sir_rng_state", obj.ptr, u64ptr(out))
+    return out
+
+
+def dust_likelihood_set_rng_state(obj, state):
+    st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
+    if st.size != obj.n_groups * (obj.n_particles + 2) * 4:
+        raise ValueError("Incorrect length for 'rng_state'")
+    call("dust2_filter_sir_set_rng_state", obj.ptr, u64ptr(st))
+
+
+def dust_likelihood_last_stats(obj):
+    out = np.zeros(2)
+    call("dust2_filter_sir_last_stats", obj.ptr, dptr(out))
+    return {"n_fallback": int(out[0]), "kernel_ms": float(out[1])}
+
+
+# ---------------------------------------------------------------------------
+# System
+# ---------------------------------------------------------------------------
+class System:
+    def __init__(self, generator, pars, n_particles, n_groups, time, dt, seed, deterministic,
+                 precision):
+        if generator.name != "sir":
+            raise ValueError(f"No system kernels for '{generator.name}'")
+        self.generator = generator
+        self.n_particles = int(n_particles)
+        self.n_groups = int(n_groups)
+        rows = pars_matrix(generator, pars, self.n_groups)
+        self.rows = rows
+        raw, n_raw, s = _seed_args(seed)
+        out = C.c_void_p()
+        call("dust2_system_sir_alloc", dptr(rows), self.n_groups, self.n_particles,
+             float(time), float(dt), u64ptr(raw), n_raw, s, int(bool(deterministic)),
+             _precision(precision), C.byref(out))
+        self.ptr = out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib._lib is not None:
+            _lib.lib().dust2_system_sir_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def shape(self):
+        return (self.n_groups, self.n_particles, self.generator.n_state)
+
+
+def dust_system_create(generator, pars, n_particles, n_groups=1, time=0, dt=1, seed=None,
+                       deterministic=False, precision="fp64"):
+    return System(generator, pars, n_particles, n_groups, time, dt, seed, deterministic,
+                  precision)
+
+
+def dust_system_run_to_time(sys, time):
+    call("dust2_system_sir_run_to_time", sys.ptr, float(time))
+
+
+def dust_system_simulate(sys, times):
+    t = f64(times).reshape(-1)
+    out = np.zeros((t.size,) + sys.shape)
+    call("dust2_system_sir_simulate", sys.ptr, dptr(t), t.size, dptr(out))
+    return out
+
+
+def dust_system_state(sys):
+    out = np.zeros(sys.shape)
+    call("dust2_system_sir_state", sys.ptr, dptr(out))
+    return out
+
+
+def dust_system_set_state(sys, state):
+    st = np.broadcast_to(f64(state), sys.shape).copy()
+    call("dust2_system_sir_set_state", sys.ptr, dptr(st))
+
+
+def dust_system_set_state_initial(sys):
+    call("dust2_system_sir_set_state_initial", sys.ptr)
+
+
+def dust_system_time(sys):
+    out = np.zeros(1)
+    call("dust2_system_sir_time", sys.ptr, dptr(out))
+    return float(out[0])
+
+
+def dust_system_set_time(sys, time):
+    call("dust2_system_sir_set_time", sys.ptr, float(time))
+
+
+def dust_system_update_pars(sys, pars):
+    rows = pars_update(sys.generator, pars, sys.rows)
+    call("dust2_system_sir_update_pars", sys.ptr, dptr(rows))
+    sys.rows = rows
+
+
+def dust_system_compare_data(sys, data):
+    """data: one value (shared) or one per group; returns (group, particle)."""
+    d = f64(data).reshape(-1)
+    out = np.zeros((sys.n_groups, sys.n_particles))
+    call("dust2_system_sir_compare_data", sys.ptr, dptr(d), d.size, dptr(out))
+    return out
+
+
+def dust_system_reorder(sys, index):
+    idx = np.ascontiguousarray(np.broadcast_to(index, (sys.n_groups, sys.n_particles)),
+                               dtype=np.uint32)
+    call("dust2_system_sir_reorder", sys.ptr, u32ptr(idx))
+
+
+def dust_system_rng_state(sys):
+    out = np.zeros(sys.n_groups * sys.n_particles * 4, dtype=np.uint64)
+    call("dust2_system_sir_rng_state", sys.ptr, u64ptr(out))
+    return out
+
+
+def dust_system_set_rng_state(sys, state):
+    st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
+    call("dust2_system_sir_set_rng_state", sys.ptr, u64ptr(st))
+
+
+# ---------------------------------------------------------------------------
+# Unfilter (deterministic likelihood) -- defined in unfilter.py
+# ---------------------------------------------------------------------------
+class Unfilter:  # replaced by the implementation in unfilter.py
+    pass
+
+
+# ---------------------------------------------------------------------------
+# Kernel-level helpers used by the parity tests
+# ---------------------------------------------------------------------------
+def gpu_resample(log_weights, u):
+    w = f64(log_weights).reshape(-1)
+    idx = np.zeros(w.size, dtype=np.uint32)
+    out = np.zeros(3)
+    call("dust2_amd_resample", dptr(w), w.size, float(u), u32ptr(idx), dptr(out))
+    return idx, out
+
+
+def gpu_rng_streams(n_streams, seed=None, seed_state=None):
+    out = np.zeros(n_streams * 4, dtype=np.uint64)
+    raw = None if seed_state is None else np.ascontiguousarray(seed_state, dtype=np.uint64)
+    n_raw = 0 if raw is None else raw.size // 4
+    call("dust2_amd_rng_streams", u64ptr(raw), n_raw, C.c_uint64(0 if seed is None else seed),
+         n_streams, u64ptr(out))
+    return out.reshape(n_streams, 4)
+
+
+SAMPLER_KINDS = {"binomial": 0, "exponential": 1, "normal": 2, "poisson": 3, "real": 4}
+
+
+def gpu_sample(kind, rng, n_draws, a, b=None, precision="fp64", deterministic=False):
+    """Draw n_draws values per stream; rng (n_streams x 4) is advanced in place."""
+    rng = np.ascontiguousarray(rng, dtype=np.uint64)
+    n = rng.size // 4
+    a = np.broadcast_to(f64(a), (n,)).copy()
+    b = np.broadcast_to(f64(0.0 if b is None else b), (n,)).copy()
+    out = np.zeros((n, n_draws))
+    err = np.zeros(n, dtype=np.uint32)
+    call("dust2_amd_sample", SAMPLER_KINDS[kind], _precision(precision), u64ptr(rng), n,
+         n_draws, dptr(a), dptr(b), int(bool(deterministic)), dptr(out), u32ptr(err))
+    return out, err, rng

@@ -1,0 +1,162 @@
+/* dust2_amd: MI355X-native batched particle engine behind dust2's entry
+ * points.
+ *
+ * C ABI only: plain pointers, sizes and status codes.  Every function that
+ * can fail returns 0 on success and non-zero on failure, writing a
+ * NUL-terminated message (the reference's wording where one exists) into
+ * `err` (capacity `errlen`).  Handles own their device memory; they are not
+ * thread-safe individually, distinct handles may be driven concurrently.
+ *
+ * Conventions shared with the reference's R/cpp11 layer:
+ *   - parameters: one row of model parameters per group (pars[g * n_pars]);
+ *     sir:    N, I0, beta, gamma, exp_noise   (inst/examples/sir.cpp:18-24)
+ *     sirode: N, I0, beta, gamma              (inst/examples/sirode.cpp:15-20)
+ *   - data: data[t][g][j], n_groups_data == 1 means data shared by all
+ *     groups (dust2/discrete/system.hpp:267,273); NaN marks missing data.
+ *   - state in/out: [group][particle][state] (the memory order of the
+ *     reference's [state, particle, group] arrays, dust2/r/system.hpp:44-56).
+ *   - indices are 0-based (the C++ side of the reference,
+ *     dust2/r/helpers.hpp:279-306).
+ *   - RNG state: 4 x uint64 per stream (xoshiro256+), [stream][word]; filter
+ *     layout per group is [2 filter streams | P particle streams]
+ *     (dust2/r/filter.hpp:182-238).
+ *   - seed: when `seed_state` is non-NULL its first `n_seed_streams` streams
+ *     are used verbatim and later streams continue by jumping; otherwise the
+ *     integer `seed` seeds stream 0 and stream k is stream 0 jumped k times
+ *     (vignettes/design.Rmd:72-75).
+ */
+#ifndef DUST2_AMD_H
+#define DUST2_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DUST2_FP64 0
+#define DUST2_FP32 1
+
+typedef struct dust2_filter dust2_filter;
+typedef struct dust2_system dust2_system;
+typedef struct dust2_unfilter dust2_unfilter;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* 0 if a usable GPU is present and the kernels load; error text otherwise. */
+int dust2_amd_init(char* err, size_t errlen);
+const char* dust2_amd_version(void);
+
+/* ---- bootstrap particle filter (stochastic SIR) ---------------------------
+ * Replaces dust2_discrete_filter_alloc<sir> (dust2/r/discrete/filter.hpp:12-88)
+ * registered as dust2_filter_sir_alloc (inst/template/discrete/compare.cpp:1-9).
+ * `time` must be strictly increasing and after time_start
+ * (dust2/r/helpers.hpp:266-276); dt > 0 and integer or 1/integer
+ * (dust2/r/helpers.hpp:232-262).  `precision` is DUST2_FP64 (FP64 state and
+ * samplers) or DUST2_FP32 (FP32 state and samplers, FP64 log-weights,
+ * resampling sums and log-likelihood). */
+int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                           double time_start, const double* time, size_t n_time, double dt,
+                           const double* data, size_t n_groups_data,
+                           const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                           int precision, dust2_filter** out, char* err, size_t errlen);
+
+/* Replaces dust2_filter_run<T> (dust2/r/filter.hpp:21-52) / filter<T>::run
+ * (dust2/filter.hpp:120-182).  initial: NULL (model initial state) or
+ * [G][P][S].  index_group: NULL (all groups) or n_index group indices.
+ * ll_out receives one log-likelihood per selected group, in index order. */
+int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajectories,
+                         const size_t* index_group, size_t n_index, double* ll_out,
+                         char* err, size_t errlen);
+
+/* Replaces dust2_filter_update_pars<T> (dust2/r/filter.hpp:10-19); sir
+ * updates I0, beta and gamma (inst/examples/sir.cpp:89-93); N and
+ * exp_noise are constant.  pars: n_index rows of n_pars. */
+int dust2_filter_sir_update_pars(dust2_filter* h, const double* pars,
+                                 const size_t* index_group, size_t n_index,
+                                 char* err, size_t errlen);
+
+/* Replaces dust2_filter_last_state (dust2/r/filter.hpp): out [G][P][S]. */
+int dust2_filter_sir_last_state(dust2_filter* h, double* out, char* err, size_t errlen);
+
+/* Replaces dust2_filter_last_trajectories (dust2/r/filter.hpp:54-116), the
+ * raw per-time post-resampling states without ancestry tracing:
+ * out [n_time][G][P][S].  Requires save_trajectories on the last run. */
+int dust2_filter_sir_last_trajectories(dust2_filter* h, double* out, char* err,
+                                       size_t errlen);
+
+/* Ancestor indices of the last run: out [n_time][G][P] (0-based). */
+int dust2_filter_sir_last_ancestry(dust2_filter* h, uint32_t* out, char* err, size_t errlen);
+
+/* Replaces dust2_filter_rng_state / set_rng_state (dust2/r/filter.hpp:182-238):
+ * G*(P+2) streams, per group [2 filter | P particle]. */
+int dust2_filter_sir_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t errlen);
+int dust2_filter_sir_set_rng_state(dust2_filter* h, const uint64_t* state, char* err,
+                                   size_t errlen);
+
+/* Launch shape of the persistent kernel (threads per workgroup, particles
+ * per thread); results do not depend on it.  Chosen automatically unless
+ * set; invalid shapes are rejected. */
+int dust2_filter_sir_set_launch(dust2_filter* h, int block, int particles_per_thread,
+                                char* err, size_t errlen);
+
+/* Statistics of the last run: [0] resampling rounds that needed the
+ * sequential sum, [1] kernel milliseconds (device events). */
+int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
+
+void dust2_filter_sir_free(dust2_filter* h);
+
+/* ---- discrete-time system (stochastic SIR) --------------------------------
+ * Replaces dust2_discrete_alloc<sir> (dust2/r/discrete/system.hpp:12-54) and
+ * the dust2_system_sir_* methods (inst/template/system.cpp:1-54). */
+int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                           double time, double dt, const uint64_t* seed_state,
+                           size_t n_seed_streams, uint64_t seed, int deterministic,
+                           int precision, dust2_system** out, char* err, size_t errlen);
+int dust2_system_sir_run_to_time(dust2_system* h, double time, char* err, size_t errlen);
+int dust2_system_sir_simulate(dust2_system* h, const double* times, size_t n_times,
+                              double* out, char* err, size_t errlen);
+int dust2_system_sir_state(dust2_system* h, double* out, char* err, size_t errlen);
+int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err,
+                               size_t errlen);
+int dust2_system_sir_set_state_initial(dust2_system* h, char* err, size_t errlen);
+int dust2_system_sir_time(dust2_system* h, double* out, char* err, size_t errlen);
+int dust2_system_sir_set_time(dust2_system* h, double time, char* err, size_t errlen);
+int dust2_system_sir_update_pars(dust2_system* h, const double* pars, char* err,
+                                 size_t errlen);
+int dust2_system_sir_compare_data(dust2_system* h, const double* data, size_t n_groups_data,
+                                  double* out, char* err, size_t errlen);
+int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err,
+                             size_t errlen);
+int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen);
+int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
+                                   size_t errlen);
+void dust2_system_sir_free(dust2_system* h);
+
+/* ---- kernels exposed for parity tests --------------------------------------
+ * Systematic resampling of one weight vector on the GPU with the filter's
+ * own workgroup routine (replaces the reference test driver
+ * test_resample_weight / test_scale_log_weights, src/test.cpp:10-23).
+ * log_weights: n <= 2048 log-weights; idx: n ancestors; out3: [ll increment,
+ * resampled flag, sequential-fallback flag]. */
+int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* idx,
+                       double* out3, char* err, size_t errlen);
+
+/* n streams of raw xoshiro256+ state, the filter/system seeding path. */
+int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                          size_t n_streams, uint64_t* out, char* err, size_t errlen);
+
+/* Sampler draws on the GPU: kind 0 binomial(a=n, b=p), 1 exponential(a=rate),
+ * 2 normal(a=mean, b=sd), 3 poisson(a=lambda), 4 random_real.  n_draws per
+ * stream from each of n_streams streams (rng: [stream][4], updated);
+ * out [stream][draw]; err_out per stream (0 or 1 = invalid binomial). */
+int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams,
+                     size_t n_draws, const double* a, const double* b, int deterministic,
+                     double* out, uint32_t* err_out, char* err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

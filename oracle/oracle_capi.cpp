@@ -1,0 +1,390 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY (see oracle.hpp).  C entry points over
+// the CPU restatement, loaded by tests/ (ctypes), the smoke check and the
+// CPU-baseline leg of bench.py.  Never linked into the product.
+#include "oracle.hpp"
+
+#include <chrono>
+#include <cstring>
+
+using namespace oracle;
+
+namespace {
+
+int fail(const std::exception& e, char* err, size_t errlen) {
+  if (err && errlen) {
+    std::snprintf(err, errlen, "%s", e.what());
+  }
+  return 1;
+}
+
+std::vector<size_t> make_index(const size_t* index_group, size_t n_index,
+                               size_t n_groups) {
+  std::vector<size_t> ret;
+  if (index_group == nullptr) {
+    ret.resize(n_groups);
+    std::iota(ret.begin(), ret.end(), 0);
+  } else {
+    ret.assign(index_group, index_group + n_index);
+  }
+  return ret;
+}
+
+template <typename T>
+std::vector<typename T::shared_state> read_shared(const double* pars,
+                                                  size_t n_groups) {
+  std::vector<typename T::shared_state> ret;
+  for (size_t g = 0; g < n_groups; ++g) {
+    ret.push_back(T::build_shared(pars + g * T::n_pars));
+  }
+  return ret;
+}
+
+template <typename T>
+std::vector<typename T::data_type> read_data(const double* data, size_t n_rows) {
+  std::vector<typename T::data_type> ret;
+  for (size_t r = 0; r < n_rows; ++r) {
+    ret.push_back(T::build_data(data + r * T::n_data));
+  }
+  return ret;
+}
+
+// Expand the G*(P+2) filter streams from an integer seed or take them
+// verbatim, and split them [2 filter | P system] per group
+// (dust2/r/discrete/filter.hpp:63-74).
+void split_filter_streams(const std::vector<rng_state>& all, size_t G, size_t P,
+                          std::vector<uint64_t>& seed_filter,
+                          std::vector<uint64_t>& seed_system) {
+  seed_filter.clear();
+  seed_system.clear();
+  for (size_t g = 0; g < G; ++g) {
+    for (size_t j = 0; j < P + 2; ++j) {
+      const rng_state& s = all[g * (P + 2) + j];
+      auto& dst = j < 2 ? seed_filter : seed_system;
+      dst.insert(dst.end(), s.s, s.s + 4);
+    }
+  }
+}
+
+template <typename T>
+int filter_run_impl(const double* pars, size_t G, size_t P, double time_start,
+                    const double* time, size_t n_time, double dt,
+                    const double* data, size_t n_groups_data,
+                    const uint64_t* rng_in, uint64_t seed, int n_threads,
+                    const double* initial, const size_t* index_group,
+                    size_t n_index, double* ll_out, double* state_out,
+                    uint64_t* rng_out, uint32_t* ancestry, char* err,
+                    size_t errlen) {
+  try {
+    const size_t n_streams = G * (P + 2);
+    std::vector<uint64_t> seed_all;
+    if (rng_in) {
+      seed_all.assign(rng_in, rng_in + 4 * n_streams);
+    } else {
+      seed_all.resize(4);
+      seed_words(seed, seed_all.data());
+    }
+    const auto streams = prng(n_streams, seed_all, false);
+    std::vector<uint64_t> seed_filter, seed_system;
+    split_filter_streams(streams, G, P, seed_filter, seed_system);
+    discrete_system<T> sys(read_shared<T>(pars, G), time_start, dt, P,
+                           seed_system, false, n_threads);
+    filter<T> obj(sys, time_start, std::vector<double>(time, time + n_time),
+                  read_data<T>(data, n_time * n_groups_data), seed_filter);
+    if (initial) {
+      obj.sys.set_state(initial);
+    }
+    const auto idx = make_index(index_group, n_index, G);
+    obj.run(initial == nullptr, idx, ancestry);
+    for (size_t i = 0; i < G; ++i) ll_out[i] = obj.log_likelihood()[i];
+    if (state_out) {
+      std::copy(obj.sys.state().begin(), obj.sys.state().end(), state_out);
+    }
+    if (rng_out) {
+      for (size_t g = 0; g < G; ++g) {
+        for (size_t j = 0; j < P + 2; ++j) {
+          const rng_state& s = j < 2 ? obj.rng()[2 * g + j]
+                                     : obj.sys.rng()[g * P + (j - 2)];
+          std::memcpy(rng_out + 4 * (g * (P + 2) + j), s.s, 32);
+        }
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+template <typename T>
+int simulate_impl(const double* pars, size_t G, size_t P, double time_start,
+                  double dt, const double* times, size_t n_times,
+                  const uint64_t* rng_in, int deterministic, int n_threads,
+                  double* out, uint64_t* rng_out, char* err, size_t errlen) {
+  try {
+    std::vector<uint64_t> seed(rng_in, rng_in + 4 * G * P);
+    discrete_system<T> sys(read_shared<T>(pars, G), time_start, dt, P, seed,
+                           deterministic != 0, n_threads);
+    sys.set_state_initial();
+    std::vector<size_t> all(G);
+    std::iota(all.begin(), all.end(), 0);
+    const size_t stride = G * P * T::n_state;
+    for (size_t t = 0; t < n_times; ++t) {
+      sys.run_to_time(times[t], all);
+      std::copy(sys.state().begin(), sys.state().end(), out + t * stride);
+    }
+    if (rng_out) {
+      for (size_t k = 0; k < G * P; ++k) {
+        std::memcpy(rng_out + 4 * k, sys.rng()[k].s, 32);
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- RNG ------------------------------------------------------------------
+void oracle_seed_words(uint64_t seed, uint64_t* out4) { seed_words(seed, out4); }
+
+void oracle_prng(const uint64_t* seed, size_t n_seed, size_t n, uint64_t* out) {
+  std::vector<uint64_t> s(seed, seed + 4 * n_seed);
+  const auto streams = prng(n, s, false);
+  for (size_t i = 0; i < n; ++i) std::memcpy(out + 4 * i, streams[i].s, 32);
+}
+
+void oracle_jump(uint64_t* s4, int long_jump) {
+  rng_state r{{s4[0], s4[1], s4[2], s4[3]}, false};
+  if (long_jump) {
+    rng_long_jump(r);
+  } else {
+    rng_jump(r);
+  }
+  std::memcpy(s4, r.s, 32);
+}
+
+void oracle_rng_next(uint64_t* s4, size_t n, uint64_t* out) {
+  rng_state r{{s4[0], s4[1], s4[2], s4[3]}, false};
+  for (size_t i = 0; i < n; ++i) out[i] = rng_next(r);
+  std::memcpy(s4, r.s, 32);
+}
+
+void oracle_random_real(uint64_t* s4, size_t n, double* out) {
+  rng_state r{{s4[0], s4[1], s4[2], s4[3]}, false};
+  for (size_t i = 0; i < n; ++i) out[i] = random_real(r);
+  std::memcpy(s4, r.s, 32);
+}
+
+// kind: 0 binomial(a=n, b=p), 1 exponential_rate(a=rate), 2 normal(a=mean,
+// b=sd), 3 poisson(a=lambda).  Draws n_draws values from ONE stream,
+// with per-draw arguments.
+int oracle_sample(int kind, uint64_t* s4, int deterministic, size_t n_draws,
+                  const double* a, const double* b, double* out, char* err,
+                  size_t errlen) {
+  rng_state r{{s4[0], s4[1], s4[2], s4[3]}, deterministic != 0};
+  try {
+    for (size_t i = 0; i < n_draws; ++i) {
+      switch (kind) {
+        case 0: out[i] = binomial(r, a[i], b[i]); break;
+        case 1: out[i] = exponential_rate(r, a[i]); break;
+        case 2: out[i] = normal(r, a[i], b[i]); break;
+        default: out[i] = poisson(r, a[i]); break;
+      }
+    }
+  } catch (const std::exception& e) {
+    std::memcpy(s4, r.s, 32);
+    return fail(e, err, errlen);
+  }
+  std::memcpy(s4, r.s, 32);
+  return 0;
+}
+
+double oracle_dpois_log(double x, double lambda) { return dpois_log(x, lambda); }
+
+double oracle_scale_log_weights(size_t n, double* w) {
+  return scale_log_weights(n, w);
+}
+
+void oracle_resample_weight(size_t n, const double* w, double u, uint32_t* idx) {
+  resample_weight(n, w, u, idx);
+}
+
+// ---- filters --------------------------------------------------------------
+// pars: G x n_pars (sir: N, I0, beta, gamma, exp_noise); data: n_time x
+// n_groups_data x n_data; rng_in: G*(P+2) streams (NULL: expand `seed`);
+// state_out: AoS [(g*P + p)*S + i]; rng_out: G*(P+2) streams, filter
+// interleaved; ancestry: n_time x G x P (or NULL).
+int oracle_sir_filter_run(const double* pars, size_t G, size_t P,
+                          double time_start, const double* time, size_t n_time,
+                          double dt, const double* data, size_t n_groups_data,
+                          const uint64_t* rng_in, uint64_t seed, int n_threads,
+                          const double* initial, const size_t* index_group,
+                          size_t n_index, double* ll_out, double* state_out,
+                          uint64_t* rng_out, uint32_t* ancestry, char* err,
+                          size_t errlen) {
+  return filter_run_impl<sir>(pars, G, P, time_start, time, n_time, dt, data,
+                              n_groups_data, rng_in, seed, n_threads, initial,
+                              index_group, n_index, ll_out, state_out, rng_out,
+                              ancestry, err, errlen);
+}
+
+int oracle_seir16_filter_run(const double* pars, size_t G, size_t P,
+                             double time_start, const double* time,
+                             size_t n_time, double dt, const double* data,
+                             size_t n_groups_data, const uint64_t* rng_in,
+                             uint64_t seed, int n_threads, const double* initial,
+                             const size_t* index_group, size_t n_index,
+                             double* ll_out, double* state_out, uint64_t* rng_out,
+                             uint32_t* ancestry, char* err, size_t errlen) {
+  return filter_run_impl<seir16>(pars, G, P, time_start, time, n_time, dt, data,
+                                 n_groups_data, rng_in, seed, n_threads, initial,
+                                 index_group, n_index, ll_out, state_out,
+                                 rng_out, ancestry, err, errlen);
+}
+
+// System simulate (dust2/discrete/system.hpp:121-128): state at each time,
+// out: n_times x G x P x S.  rng_in: G*P streams.
+int oracle_sir_simulate(const double* pars, size_t G, size_t P,
+                        double time_start, double dt, const double* times,
+                        size_t n_times, const uint64_t* rng_in,
+                        int deterministic, int n_threads, double* out,
+                        uint64_t* rng_out, char* err, size_t errlen) {
+  return simulate_impl<sir>(pars, G, P, time_start, dt, times, n_times, rng_in,
+                            deterministic, n_threads, out, rng_out, err, errlen);
+}
+
+int oracle_seir16_simulate(const double* pars, size_t G, size_t P,
+                           double time_start, double dt, const double* times,
+                           size_t n_times, const uint64_t* rng_in,
+                           int deterministic, int n_threads, double* out,
+                           uint64_t* rng_out, char* err, size_t errlen) {
+  return simulate_impl<seir16>(pars, G, P, time_start, dt, times, n_times,
+                               rng_in, deterministic, n_threads, out, rng_out,
+                               err, errlen);
+}
+
+// ---- ODE unfilter ---------------------------------------------------------
+// control: [max_steps, atol, rtol, step_size_min, step_size_max].
+// stats_out (G*P x 4, or NULL): n_steps, n_steps_accepted,
+// n_steps_rejected, step_size after the run.
+int oracle_sirode_unfilter_run(const double* pars, size_t G, size_t P,
+                               double time_start, const double* time,
+                               size_t n_time, const double* data,
+                               size_t n_groups_data, const double* control,
+                               int n_threads, double* ll_out, double* state_out,
+                               double* stats_out, char* err, size_t errlen) {
+  try {
+    ode_control ctl;
+    ctl.max_steps = static_cast<size_t>(control[0]);
+    ctl.atol = control[1];
+    ctl.rtol = control[2];
+    ctl.step_size_min = std::max(control[3], std::numeric_limits<double>::epsilon());
+    ctl.step_size_max = control[4];
+    continuous_system<sirode> sys(read_shared<sirode>(pars, G), time_start, P, ctl,
+                                  n_threads);
+    const auto d = read_data<sirode>(data, n_time * n_groups_data);
+    unfilter_run(sys, time_start, std::vector<double>(time, time + n_time),
+                 d.data(), n_groups_data, ll_out);
+    if (state_out) {
+      std::copy(sys.state().begin(), sys.state().end(), state_out);
+    }
+    if (stats_out) {
+      for (size_t k = 0; k < G * P; ++k) {
+        const auto& in = sys.ode_internals_()[k];
+        stats_out[4 * k + 0] = in.n_steps;
+        stats_out[4 * k + 1] = in.n_steps_accepted;
+        stats_out[4 * k + 2] = in.n_steps_rejected;
+        stats_out[4 * k + 3] = in.step_size;
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// ---- deterministic discrete SIR unfilter + adjoint gradient ---------------
+// (dust2/unfilter.hpp:77-229, dust2/discrete/system.hpp:290-482).  P = 1;
+// data per group (n_time x G).  grad_out: G x 3 (beta, gamma, I0) or NULL.
+int oracle_sir_unfilter_run(const double* pars, size_t G, double time_start,
+                            const double* time, size_t n_time, double dt,
+                            const double* data, double* ll_out,
+                            double* grad_out, char* err, size_t errlen) {
+  try {
+    constexpr size_t S = sir::n_state;
+    constexpr size_t A = S + sir::n_gradient;
+    std::vector<uint64_t> seed(4);
+    seed_words(42, seed.data());
+    discrete_system<sir> sys(read_shared<sir>(pars, G), time_start, dt, 1, seed,
+                             true, 1);
+    const auto d = read_data<sir>(data, n_time * G);
+    std::vector<size_t> offset(n_time + 1, 0);
+    for (size_t i = 0; i < n_time; ++i) {
+      offset[i + 1] = std::round(std::max(0.0, time[i] - time_start) / dt) * G * S;
+    }
+    std::vector<double> history(offset.back() + G * S, 0.0);
+    sys.set_time(time_start);
+    sys.set_state_initial();
+    sys.run_to_time_history(time[n_time - 1], history.data());
+    std::fill(ll_out, ll_out + G, 0.0);
+    rng_state det{{0, 0, 0, 0}, true};
+    for (size_t i = 0; i < n_time; ++i) {
+      const double* st = history.data() + offset[i + 1];
+      for (size_t g = 0; g < G; ++g) {
+        ll_out[g] += sir::compare_data(time[i], st + g * S, d[i * G + g],
+                                       sys.shared()[g], det);
+      }
+    }
+    if (grad_out) {
+      std::vector<double> curr(A * G, 0.0), next(A * G, 0.0);
+      double* ac = curr.data();
+      double* an = next.data();
+      const size_t stride = G * S;
+      double tcur = time[n_time - 1];
+      for (size_t irev = 0; irev < n_time; ++irev) {
+        const size_t i = n_time - irev - 1;
+        const double time_i = i == 0 ? time_start : time[i - 1];
+        const double* state_i = history.data() + offset[i + 1];
+        for (size_t g = 0; g < G; ++g) {
+          sir::adjoint_compare_data(state_i + g * S, ac + g * A, d[i * G + g],
+                                    sys.shared()[g], an + g * A);
+        }
+        std::swap(ac, an);
+        const size_t n_steps = std::round(std::max(0.0, tcur - time_i) / dt);
+        for (size_t g = 0; g < G; ++g) {
+          double* c = ac + g * A;
+          double* n = an + g * A;
+          for (size_t s = 1; s <= n_steps; ++s) {
+            const double time_s = tcur - s * dt;
+            const double* st = state_i + g * S - s * stride;
+            sir::adjoint_update(dt, st, c, sys.shared()[g], n);
+            std::swap(c, n);
+            if (std::fmod(time_s, sir::zero_period) == 0) {
+              c[sir::zero_first] = 0;
+            }
+          }
+        }
+        if (n_steps % 2 == 1) {
+          std::swap(ac, an);
+        }
+        tcur = time_i;
+      }
+      for (size_t g = 0; g < G; ++g) {
+        sir::adjoint_initial(history.data() + g * S, ac + g * A, sys.shared()[g],
+                             an + g * A);
+      }
+      for (size_t g = 0; g < G; ++g) {
+        for (size_t j = 0; j < sir::n_gradient; ++j) {
+          grad_out[g * sir::n_gradient + j] = an[g * A + S + j];
+        }
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// Thin C wrapper over the parts of the reference engine that compile on
+// their own (no monty, no cpp11, no R): dust2/filter_details.hpp and
+// dust2/continuous/solver.hpp (+ control/history/events).  Built by
+// oracle/Makefile straight from the reference headers into oracle/_ref/;
+// used to pin the CPU restatement in oracle.hpp and to write the golden
+// fixtures under tests/golden/.  Nothing here is shipped.
+#include <dust2/continuous/solver.hpp>
+#include <dust2/filter_details.hpp>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+extern "C" {
+
+double ref_scale_log_weights(size_t n, double* w) {
+  std::vector<double> v(w, w + n);
+  const double ret = dust2::details::scale_log_weights<double>(n, v.begin());
+  std::copy(v.begin(), v.end(), w);
+  return ret;
+}
+
+void ref_resample_weight(size_t n, const double* w, double u, uint32_t* idx) {
+  std::vector<double> v(w, w + n);
+  std::vector<size_t> out(n);
+  dust2::details::resample_weight<double>(n, v.cbegin(), u, out.begin());
+  for (size_t i = 0; i < n; ++i) idx[i] = static_cast<uint32_t>(out[i]);
+}
+
+// The SIRODE model of inst/examples/sirode.cpp integrated by the
+// reference solver, driven the way dust_continuous + unfilter drive it
+// (dust2/continuous/system.hpp:80-106,444-482,531-542;
+// dust2/unfilter.hpp:42-72,162-186): initialise at time_start, then
+// solver.run from each data time to the next, zero_every = {{1, {4}}}.
+// pars: n_sets x [N, I0, beta, gamma]; data: n_time (shared).
+// out_state: n_sets x n_time x 5 (state at each data time);
+// out_stats: n_sets x 4 (n_steps, accepted, rejected, step_size).
+int ref_sirode_unfilter(const double* pars, size_t n_sets, double time_start,
+                        const double* time, size_t n_time, const double* data,
+                        const double* ctl_in, double* ll_out,
+                        double* out_state, double* out_stats, char* err,
+                        size_t errlen) {
+  using dust2::ode::internals;
+  using dust2::ode::solver;
+  const size_t n_var = 5;
+  dust2::ode::control<double> ctl(static_cast<size_t>(ctl_in[0]), ctl_in[1], ctl_in[2],
+                      std::max(ctl_in[3], std::numeric_limits<double>::epsilon()),
+                      ctl_in[4], std::vector<double>(), false, false);
+  for (size_t k = 0; k < n_sets; ++k) {
+    const double N = pars[4 * k + 0];
+    const double I0 = pars[4 * k + 1];
+    const double beta = pars[4 * k + 2];
+    const double gamma = pars[4 * k + 3];
+    auto rhs = [&](double, const double* y, double* dy) {
+      const double S = y[0];
+      const double I = y[1];
+      const double rate_SI = beta * S * I / N;
+      const double rate_IR = gamma * I;
+      dy[0] = -rate_SI;
+      dy[1] = rate_SI - rate_IR;
+      dy[2] = rate_IR;
+      dy[3] = rate_SI;
+      dy[4] = rate_SI;
+    };
+    solver<double> s(n_var, 0, ctl);
+    internals<double> in(n_var, 0, false);
+    dust2::zero_every_type<double> zero_every{{1, {4}}};
+    dust2::ode::events_type<double> events;
+    std::vector<double> y{N - I0, I0, 0, 0, 0};
+    double t = time_start;
+    double ll = 0;
+    try {
+      s.initialise(t, y.data(), in, rhs);
+      for (size_t i = 0; i < n_time; ++i) {
+        s.run(t, time[i], y.data(), zero_every, events, in, rhs);
+        t = time[i];
+        const double x = data[i];
+        if (!std::isnan(x)) {
+          const double lambda = y[4];
+          ll += (x == 0 && lambda == 0)
+                    ? 0
+                    : x * std::log(lambda) - lambda - std::lgamma(x + 1);
+        }
+        if (out_state) {
+          std::copy(y.begin(), y.end(), out_state + (k * n_time + i) * n_var);
+        }
+      }
+    } catch (const std::exception& e) {
+      if (err && errlen) std::snprintf(err, errlen, "%s", e.what());
+      return 1;
+    }
+    ll_out[k] = ll;
+    if (out_stats) {
+      out_stats[4 * k + 0] = in.n_steps;
+      out_stats[4 * k + 1] = in.n_steps_accepted;
+      out_stats[4 * k + 2] = in.n_steps_rejected;
+      out_stats[4 * k + 3] = in.step_size;
+    }
+  }
+  return 0;
+}
+
+// A generic logistic ODE through the reference solver (closed form
+// y = K / (1 + (K/y0 - 1) exp(-r t)), tests/testthat/helper-dust.R:68-73),
+// two independent variables with rates r1, r2, no zero_every.
+int ref_logistic(double r1, double r2, double K, double y0, const double* time,
+                 size_t n_time, const double* ctl_in, double* out) {
+  using namespace dust2::ode;
+  dust2::ode::control<double> ctl(static_cast<size_t>(ctl_in[0]), ctl_in[1], ctl_in[2],
+                      std::max(ctl_in[3], std::numeric_limits<double>::epsilon()),
+                      ctl_in[4], std::vector<double>(), false, false);
+  auto rhs = [&](double, const double* y, double* dy) {
+    dy[0] = r1 * y[0] * (1 - y[0] / K);
+    dy[1] = r2 * y[1] * (1 - y[1] / K);
+  };
+  solver<double> s(2, 0, ctl);
+  internals<double> in(2, 0, false);
+  dust2::zero_every_type<double> zero_every;
+  events_type<double> events;
+  std::vector<double> y{y0, y0};
+  double t = 0;
+  s.initialise(t, y.data(), in, rhs);
+  for (size_t i = 0; i < n_time; ++i) {
+    s.run(t, time[i], y.data(), zero_every, events, in, rhs);
+    t = time[i];
+    out[2 * i] = y[0];
+    out[2 * i + 1] = y[1];
+  }
+  return 0;
+}
+
+}  // extern "C"

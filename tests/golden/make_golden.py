@@ -1,0 +1,98 @@
+"""Writes the golden fixtures in this directory from the reference's own code.
+
+The parts of the reference engine that compile without monty/cpp11/R
+(dust2/filter_details.hpp, dust2/continuous/solver.hpp) are built in place
+by oracle/Makefile into oracle/_ref/libdust2ref.so; this script calls them on
+fixed, seeded inputs and stores inputs and outputs as JSON.  Run it where the
+reference tree exists:
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import oracle_lib as ol  # noqa: E402
+
+
+def weight_cases():
+    rng = np.random.default_rng(20240101)
+    cases = []
+    for n in [1, 2, 3, 7, 10, 64, 100, 257]:
+        for kind in ["normal", "spread", "nan", "neginf", "ties"]:
+            w = rng.normal(-50, 5 if kind != "spread" else 200, size=n)
+            if kind == "nan" and n > 2:
+                w[rng.integers(0, n, size=max(1, n // 5))] = np.nan
+            if kind == "neginf" and n > 2:
+                w[rng.integers(0, n, size=max(1, n // 3))] = -np.inf
+            if kind == "ties":
+                w = np.round(w / 10) * 10
+            for u in [0.0, float(rng.random()), 1 - 1e-8]:
+                cases.append((w.copy(), u))
+    # degenerate: every weight impossible, and a single surviving particle
+    cases.append((np.full(10, -np.inf), 0.3))
+    w = np.full(10, -np.inf)
+    w[6] = -3.0
+    cases.append((w, 0.7))
+    cases.append((np.full(10, np.nan), 0.5))
+    return cases
+
+
+def filter_details():
+    out = []
+    for w, u in weight_cases():
+        ll, scaled = ol.ref_scale_log_weights(w)
+        idx = ol.ref_resample_weight(scaled, u)
+        out.append({"w": w.tolist(), "u": u, "ll": ll, "scaled": scaled.tolist(),
+                    "idx": idx.tolist()})
+    return out
+
+
+def sirode_fixture():
+    rng = np.random.default_rng(42)
+    n_sets = 16
+    beta = rng.uniform(0.1, 0.5, n_sets)
+    gamma = rng.uniform(0.05, 0.2, n_sets)
+    pars = np.column_stack([np.full(n_sets, 1000.0), np.full(n_sets, 10.0), beta, gamma])
+    time = np.arange(1, 366, dtype=np.float64)
+    data = rng.poisson(8, size=365).astype(np.float64)
+    data[[5, 77, 200]] = np.nan
+    res = ol.ref_sirode(pars, 0.0, time, data)
+    # a second control setting with tight tolerances and a step-size cap
+    ctl2 = [100000, 1e-9, 1e-9, 0, 0.3]
+    res2 = ol.ref_sirode(pars[:4], 0.0, time[:60], data[:60], control=ctl2)
+    keep = [0, 1, 9, 29, 99, 199, 364]
+    return {
+        "pars": pars.tolist(), "time_start": 0.0, "time": time.tolist(),
+        "data": data.tolist(), "ll": res["ll"].tolist(), "state_index": keep,
+        "state": res["state"][:, keep, :].tolist(), "stats": res["stats"].tolist(),
+        "control2": ctl2, "ll2": res2["ll"].tolist(),
+        "stats2": res2["stats"].tolist(), "state2_final": res2["state"][:, -1, :].tolist(),
+    }
+
+
+def logistic_fixture():
+    time = np.array([0.5, 1, 2, 5, 10, 20, 30], dtype=np.float64)
+    out = ol.ref_logistic(0.1, 0.2, 100.0, 1.0, time)
+    return {"r1": 0.1, "r2": 0.2, "K": 100.0, "y0": 1.0, "time": time.tolist(),
+            "y": out.tolist()}
+
+
+def main():
+    if not ol.ref_available():
+        raise SystemExit("oracle/_ref/libdust2ref.so is not built (needs the reference tree)")
+    with open(os.path.join(HERE, "filter_details.json"), "w") as f:
+        json.dump(filter_details(), f)
+    with open(os.path.join(HERE, "sirode_dp5.json"), "w") as f:
+        json.dump(sirode_fixture(), f)
+    with open(os.path.join(HERE, "logistic_dp5.json"), "w") as f:
+        json.dump(logistic_fixture(), f)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,294 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
+built, to the reference's own compiled parts (oracle/_ref/libdust2ref.so).
+
+Test infrastructure only: the product never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libdust2ref.so")
+
+D = C.POINTER(C.c_double)
+U64 = C.POINTER(C.c_uint64)
+U32 = C.POINTER(C.c_uint32)
+SZ = C.POINTER(C.c_size_t)
+
+_oracle = None
+_ref = None
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(D)
+
+
+def _u64(a):
+    return None if a is None else a.ctypes.data_as(U64)
+
+
+def _u32(a):
+    return None if a is None else a.ctypes.data_as(U32)
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"], check=True)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        _oracle = C.CDLL(ORACLE_SO)
+        o = _oracle
+        o.oracle_dpois_log.restype = C.c_double
+        o.oracle_dpois_log.argtypes = [C.c_double, C.c_double]
+        o.oracle_scale_log_weights.restype = C.c_double
+        o.oracle_scale_log_weights.argtypes = [C.c_size_t, D]
+        o.oracle_resample_weight.argtypes = [C.c_size_t, D, C.c_double, U32]
+        o.oracle_seed_words.argtypes = [C.c_uint64, U64]
+        o.oracle_prng.argtypes = [U64, C.c_size_t, C.c_size_t, U64]
+        o.oracle_jump.argtypes = [U64, C.c_int]
+        o.oracle_rng_next.argtypes = [U64, C.c_size_t, U64]
+        o.oracle_random_real.argtypes = [U64, C.c_size_t, D]
+        o.oracle_sample.argtypes = [C.c_int, U64, C.c_int, C.c_size_t, D, D, D, C.c_char_p,
+                                    C.c_size_t]
+        filt = [D, C.c_size_t, C.c_size_t, C.c_double, D, C.c_size_t, C.c_double, D,
+                C.c_size_t, U64, C.c_uint64, C.c_int, D, SZ, C.c_size_t, D, D, U64, U32,
+                C.c_char_p, C.c_size_t]
+        o.oracle_sir_filter_run.argtypes = filt
+        o.oracle_seir16_filter_run.argtypes = filt
+        sim = [D, C.c_size_t, C.c_size_t, C.c_double, C.c_double, D, C.c_size_t, U64, C.c_int,
+               C.c_int, D, U64, C.c_char_p, C.c_size_t]
+        o.oracle_sir_simulate.argtypes = sim
+        o.oracle_seir16_simulate.argtypes = sim
+        o.oracle_sirode_unfilter_run.argtypes = [D, C.c_size_t, C.c_size_t, C.c_double, D,
+                                                 C.c_size_t, D, C.c_size_t, D, C.c_int, D, D,
+                                                 D, C.c_char_p, C.c_size_t]
+        o.oracle_sir_unfilter_run.argtypes = [D, C.c_size_t, C.c_double, D, C.c_size_t,
+                                              C.c_double, D, D, D, C.c_char_p, C.c_size_t]
+    return _oracle
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_SO)
+        _ref.ref_scale_log_weights.restype = C.c_double
+        _ref.ref_scale_log_weights.argtypes = [C.c_size_t, D]
+        _ref.ref_resample_weight.argtypes = [C.c_size_t, D, C.c_double, U32]
+        _ref.ref_sirode_unfilter.argtypes = [D, C.c_size_t, C.c_double, D, C.c_size_t, D, D, D,
+                                             D, D, C.c_char_p, C.c_size_t]
+        _ref.ref_logistic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D,
+                                      C.c_size_t, D, D]
+    return _ref
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc, buf):
+    if rc != 0:
+        raise OracleError(buf.value.decode())
+
+
+# ---- thin numpy wrappers ---------------------------------------------------
+def seed_words(seed):
+    out = np.zeros(4, dtype=np.uint64)
+    oracle().oracle_seed_words(C.c_uint64(seed), _u64(out))
+    return out
+
+
+def prng(n, seed=None, seed_state=None):
+    s = seed_words(seed) if seed_state is None else np.ascontiguousarray(seed_state,
+                                                                       dtype=np.uint64)
+    out = np.zeros(n * 4, dtype=np.uint64)
+    oracle().oracle_prng(_u64(s), s.size // 4, n, _u64(out))
+    return out.reshape(n, 4)
+
+
+def jump(state, long=False):
+    s = np.array(state, dtype=np.uint64).copy()
+    oracle().oracle_jump(_u64(s), int(long))
+    return s
+
+
+def rng_next(state, n):
+    s = np.array(state, dtype=np.uint64).copy()
+    out = np.zeros(n, dtype=np.uint64)
+    oracle().oracle_rng_next(_u64(s), n, _u64(out))
+    return out, s
+
+
+def random_real(state, n):
+    s = np.array(state, dtype=np.uint64).copy()
+    out = np.zeros(n)
+    oracle().oracle_random_real(_u64(s), n, _dp(out))
+    return out, s
+
+
+KINDS = {"binomial": 0, "exponential": 1, "normal": 2, "poisson": 3}
+
+
+def sample(kind, state, a, b=None, deterministic=False):
+    """Draw len(a) values from ONE stream with per-draw arguments."""
+    s = np.array(state, dtype=np.uint64).copy()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.zeros_like(a) if b is None else np.ascontiguousarray(
+        np.broadcast_to(b, a.shape), dtype=np.float64)
+    out = np.zeros(a.size)
+    buf = C.create_string_buffer(512)
+    rc = oracle().oracle_sample(KINDS[kind], _u64(s), int(deterministic), a.size, _dp(a), _dp(b),
+                                _dp(out), buf, 512)
+    _check(rc, buf)
+    return out, s
+
+
+def dpois_log(x, lam):
+    return oracle().oracle_dpois_log(float(x), float(lam))
+
+
+def scale_log_weights(w):
+    w = np.array(w, dtype=np.float64)
+    r = oracle().oracle_scale_log_weights(w.size, _dp(w))
+    return r, w
+
+
+def resample_weight(w, u):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    idx = np.zeros(w.size, dtype=np.uint32)
+    oracle().oracle_resample_weight(w.size, _dp(w), float(u), _u32(idx))
+    return idx
+
+
+def sir_filter(pars, n_particles, time_start, time, dt, data, rng=None, seed=42, n_threads=1,
+               initial=None, index_group=None, ancestry=False, model="sir"):
+    """pars: (G, n_pars); data: (n_time, G_data) (or (n_time, G_data, n_data)).
+    Returns dict(ll, state (G,P,S), rng (G*(P+2),4), ancestry)."""
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    P = n_particles
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    n_groups_data = data.shape[1]
+    S = 5 if model == "sir" else 80
+    ll = np.zeros(G)
+    state = np.zeros((G, P, S))
+    rng_out = np.zeros((G * (P + 2), 4), dtype=np.uint64)
+    anc = np.zeros((time.size, G, P), dtype=np.uint32) if ancestry else None
+    rng_in = None if rng is None else np.ascontiguousarray(rng, dtype=np.uint64)
+    init = None if initial is None else np.ascontiguousarray(initial, dtype=np.float64)
+    idx = None if index_group is None else np.ascontiguousarray(index_group, dtype=np.uintp)
+    buf = C.create_string_buffer(4096)
+    fn = oracle().oracle_sir_filter_run if model == "sir" else oracle().oracle_seir16_filter_run
+    rc = fn(_dp(pars), G, P, float(time_start), _dp(time), time.size, float(dt), _dp(data),
+            n_groups_data, _u64(rng_in), C.c_uint64(seed), n_threads, _dp(init),
+            None if idx is None else idx.ctypes.data_as(SZ), 0 if idx is None else idx.size,
+            _dp(ll), _dp(state), _u64(rng_out), _u32(anc), buf, 4096)
+    _check(rc, buf)
+    return {"ll": ll, "state": state, "rng": rng_out, "ancestry": anc}
+
+
+def sir_simulate(pars, n_particles, time_start, dt, times, rng, deterministic=False,
+                 n_threads=1, model="sir"):
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    P = n_particles
+    S = 5 if model == "sir" else 80
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    rng = np.ascontiguousarray(rng, dtype=np.uint64)
+    out = np.zeros((times.size, G, P, S))
+    rng_out = np.zeros((G * P, 4), dtype=np.uint64)
+    buf = C.create_string_buffer(4096)
+    fn = oracle().oracle_sir_simulate if model == "sir" else oracle().oracle_seir16_simulate
+    rc = fn(_dp(pars), G, P, float(time_start), float(dt), _dp(times), times.size, _u64(rng),
+            int(deterministic), n_threads, _dp(out), _u64(rng_out), buf, 4096)
+    _check(rc, buf)
+    return out, rng_out
+
+
+def sirode_unfilter(pars, n_particles, time_start, time, data, control=None, n_threads=1):
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    P = n_particles
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    n_groups_data = 1 if data.ndim == 1 else data.shape[1]
+    ctl = np.array([10000, 1e-6, 1e-6, 0, np.inf] if control is None else control,
+                   dtype=np.float64)
+    ll = np.zeros(G * P)
+    state = np.zeros((G, P, 5))
+    stats = np.zeros((G * P, 4))
+    buf = C.create_string_buffer(4096)
+    rc = oracle().oracle_sirode_unfilter_run(_dp(pars), G, P, float(time_start), _dp(time),
+                                             time.size, _dp(data), n_groups_data, _dp(ctl),
+                                             n_threads, _dp(ll), _dp(state), _dp(stats), buf,
+                                             4096)
+    _check(rc, buf)
+    return {"ll": ll, "state": state, "stats": stats}
+
+
+def sir_unfilter(pars, time_start, time, dt, data, gradient=True):
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    data = np.ascontiguousarray(np.broadcast_to(
+        np.asarray(data, dtype=np.float64).reshape(time.size, -1), (time.size, G)))
+    ll = np.zeros(G)
+    grad = np.zeros((G, 3)) if gradient else None
+    buf = C.create_string_buffer(4096)
+    rc = oracle().oracle_sir_unfilter_run(_dp(pars), G, float(time_start), _dp(time),
+                                          time.size, float(dt), _dp(data), _dp(ll), _dp(grad),
+                                          buf, 4096)
+    _check(rc, buf)
+    return ll, grad
+
+
+# ---- reference (compiled from /root/reference headers) ---------------------
+def ref_scale_log_weights(w):
+    w = np.array(w, dtype=np.float64)
+    r = ref().ref_scale_log_weights(w.size, _dp(w))
+    return r, w
+
+
+def ref_resample_weight(w, u):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    idx = np.zeros(w.size, dtype=np.uint32)
+    ref().ref_resample_weight(w.size, _dp(w), float(u), _u32(idx))
+    return idx
+
+
+def ref_sirode(pars, time_start, time, data, control=None):
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    n = pars.shape[0]
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    ctl = np.array([10000, 1e-6, 1e-6, 0, np.inf] if control is None else control,
+                   dtype=np.float64)
+    ll = np.zeros(n)
+    state = np.zeros((n, time.size, 5))
+    stats = np.zeros((n, 4))
+    buf = C.create_string_buffer(512)
+    rc = ref().ref_sirode_unfilter(_dp(pars), n, float(time_start), _dp(time), time.size,
+                                   _dp(data), _dp(ctl), _dp(ll), _dp(state), _dp(stats), buf, 512)
+    _check(rc, buf)
+    return {"ll": ll, "state": state, "stats": stats}
+
+
+def ref_logistic(r1, r2, K, y0, time, control=None):
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    ctl = np.array([10000, 1e-6, 1e-6, 0, np.inf] if control is None else control,
+                   dtype=np.float64)
+    out = np.zeros((time.size, 2))
+    ref().ref_logistic(r1, r2, K, y0, _dp(time), time.size, _dp(ctl), _dp(out))
+    return out
