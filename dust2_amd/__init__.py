@@ -10,6 +10,8 @@ from .interface import (  # noqa: F401
     Generator,
     dust_filter_create,
     dust_likelihood_last_ancestry,
+    dust_ode_control,
+    dust_unfilter_create,
     dust_likelihood_last_state,
     dust_likelihood_last_stats,
     dust_likelihood_last_trajectories,

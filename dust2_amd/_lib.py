@@ -54,6 +54,14 @@ _PROTOS = {
     "dust2_system_sir_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_system_sir_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_system_sir_free": (None, [VP]),
+    "dust2_unfilter_sirode_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
+                                              C.c_size_t, D, C.c_size_t, D, C.c_int,
+                                              C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_unfilter_sirode_run": (C.c_int, [VP, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_last_stats": (C.c_int, [VP, D, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_free": (None, [VP]),
     "dust2_amd_resample": (C.c_int, [D, C.c_size_t, C.c_double, U32, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,
                                         ERRLEN]),

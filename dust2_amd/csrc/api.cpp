@@ -1106,3 +1106,228 @@ int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams, s
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Unfilter (deterministic likelihood, SIR ODE, Dormand-Prince 5(4))
+// ============================================================================
+struct dust2_unfilter {
+  bool fp64 = true;
+  size_t G = 0, P = 0, n_time = 0, n_groups_data = 0;
+  double time_start = 0;
+  std::vector<double> pars;  // G x 4
+  ode_control_dev control{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active;
+  size_t real_size() const { return fp64 ? 8 : 4; }
+  ~dust2_unfilter() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+constexpr size_t kSirodePars = 4;
+
+void check_sirode_pars(const double* row) {
+  const char* names[] = {"N", "I0", "beta", "gamma"};
+  for (size_t i = 0; i < kSirodePars; ++i) {
+    if (!std::isfinite(row[i])) {
+      throw std::runtime_error(std::string("Parameter '") + names[i] + "' must be finite");
+    }
+  }
+}
+
+void check_unfilter(dust2_unfilter* h) {
+  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (unfilter)");
+}
+}  // namespace
+
+extern "C" {
+
+int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                                double time_start, const double* time, size_t n_time,
+                                const double* data, size_t n_groups_data, const double* control,
+                                int precision, dust2_unfilter** out, char* err, size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    check_time_sequence(time_start, time, n_time);
+    if (n_groups_data != 1 && n_groups_data != n_groups) {
+      throw std::runtime_error("Expected 'data' to have 1 or " + std::to_string(n_groups) +
+                               " groups, but it had " + std::to_string(n_groups_data));
+    }
+    if (precision != DUST2_FP64 && precision != DUST2_FP32) {
+      throw std::runtime_error("'precision' must be DUST2_FP64 or DUST2_FP32");
+    }
+    if (n_groups * n_particles > (size_t(1) << 31)) {
+      throw std::runtime_error("too many particles");
+    }
+    for (size_t g = 0; g < n_groups; ++g) check_sirode_pars(pars + g * kSirodePars);
+    auto h = std::make_unique<dust2_unfilter>();
+    h->fp64 = precision == DUST2_FP64;
+    h->G = n_groups;
+    h->P = n_particles;
+    h->n_time = n_time;
+    h->n_groups_data = n_groups_data;
+    h->time_start = time_start;
+    h->pars.assign(pars, pars + n_groups * kSirodePars);
+    // dust_ode_control() defaults (R/interface-continuous.R:51-79) with the
+    // C++ controller constants (dust2/continuous/control.hpp:11-24)
+    const double def[5] = {10000, 1e-6, 1e-6, 0, std::numeric_limits<double>::infinity()};
+    const double* c = control ? control : def;
+    if (!(c[0] >= 1) || !(c[1] > 0) || !(c[2] > 0) || !(c[3] >= 0) || !(c[4] >= 0)) {
+      throw std::runtime_error("Invalid ODE control");
+    }
+    h->control.max_steps = static_cast<uint32_t>(std::min(c[0], 4e9));
+    h->control.atol = c[1];
+    h->control.rtol = c[2];
+    h->control.step_size_min = std::max(c[3], std::numeric_limits<double>::epsilon());
+    h->control.step_size_max = c[4];
+    h->control.factor_safe = 0.9;
+    h->control.factor_min = 0.2;
+    h->control.factor_max = 10.0;
+    h->control.beta = 0.04;
+    h->control.constant = 0.2 - 0.04 * 0.75;
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
+    const size_t n_total = n_groups * n_particles;
+    const size_t nd = n_time * n_groups_data;
+    std::vector<double> lfact(nd);
+    for (size_t i = 0; i < nd; ++i) lfact[i] = std::isnan(data[i]) ? 0.0 : std::lgamma(data[i] + 1);
+    h->d_pars.alloc(h->pars.size() * sizeof(double));
+    h->d_times.alloc(n_time * sizeof(double));
+    h->d_obs.alloc(nd * sizeof(double));
+    h->d_lfact.alloc(nd * sizeof(double));
+    h->d_state.alloc(n_total * 5 * h->real_size());
+    h->d_ll.alloc(n_total * sizeof(double));
+    h->d_stats.alloc(n_total * 4 * sizeof(double));
+    h->d_errors.alloc(n_total * sizeof(error_record));
+    h->d_counters.alloc(4 * sizeof(uint32_t));
+    h->d_active.alloc(n_groups);
+    to_device(h->d_pars, h->pars.data(), h->pars.size() * sizeof(double), h->stream);
+    to_device(h->d_times, time, n_time * sizeof(double), h->stream);
+    to_device(h->d_obs, data, nd * sizeof(double), h->stream);
+    to_device(h->d_lfact, lfact.data(), nd * sizeof(double), h->stream);
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_run(dust2_unfilter* h, const size_t* index_group, size_t n_index,
+                              double* ll_out, char* err, size_t errlen) {
+  try {
+    check_unfilter(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    hipStream_t s = h->stream;
+    std::vector<uint8_t> active(h->G, 0);
+    for (size_t g : idx) active[g] = 1;
+    to_device(h->d_active, active.data(), h->G, s);
+    hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+    ode_launch a{};
+    a.pars = h->d_pars.get<double>();
+    a.active = h->d_active.get<uint8_t>();
+    a.n_particles = static_cast<int>(h->P);
+    a.n_total = static_cast<int>(h->G * h->P);
+    a.time_start = h->time_start;
+    a.times = h->d_times.get<double>();
+    a.n_time = static_cast<int>(h->n_time);
+    a.obs = h->d_obs.get<double>();
+    a.lfact = h->d_lfact.get<double>();
+    a.obs_stride = h->n_groups_data;
+    a.obs_group_stride = h->n_groups_data == 1 ? 0 : 1;
+    a.control = h->control;
+    a.state = h->d_state.get();
+    a.ll = h->d_ll.get<double>();
+    a.stats = h->d_stats.get<double>();
+    a.errors = h->d_errors.get<error_record>();
+    a.error_count = h->d_counters.get<uint32_t>();
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    hip_check(launch_dp5_unfilter(a, h->fp64, s), "dp5 unfilter kernel");
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+    uint32_t count = 0;
+    to_host(&count, h->d_counters, sizeof(count), s);
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    h->last_ms = ms;
+    if (count > 0) {
+      std::vector<error_record> rec(h->G * h->P);
+      to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+      hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+      hip_check(hipStreamSynchronize(s), "sync");
+      throw std::runtime_error(error_report(rec, count));
+    }
+    std::vector<double> ll(h->G * h->P);
+    to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), s);
+    for (size_t i = 0; i < idx.size(); ++i) {
+      std::memcpy(ll_out + i * h->P, ll.data() + idx[i] * h->P, h->P * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_update_pars(dust2_unfilter* h, const double* pars,
+                                      const size_t* index_group, size_t n_index, char* err,
+                                      size_t errlen) {
+  try {
+    check_unfilter(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    for (size_t i = 0; i < idx.size(); ++i) {
+      const double* row = pars + i * kSirodePars;
+      check_sirode_pars(row);
+      double* dst = &h->pars[idx[i] * kSirodePars];
+      dst[1] = row[1];
+      dst[2] = row[2];
+      dst[3] = row[3];
+    }
+    to_device(h->d_pars, h->pars.data(), h->pars.size() * sizeof(double), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_last_state(dust2_unfilter* h, double* out, char* err, size_t errlen) {
+  try {
+    check_unfilter(h);
+    const size_t n = h->G * h->P;
+    std::vector<char> tmp(h->d_state.bytes());
+    to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
+    for (size_t k = 0; k < n; ++k) {
+      for (size_t c = 0; c < 5; ++c) {
+        out[k * 5 + c] = h->fp64 ? reinterpret_cast<const double*>(tmp.data())[c * n + k]
+                                 : reinterpret_cast<const float*>(tmp.data())[c * n + k];
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* kernel_ms,
+                                     char* err, size_t errlen) {
+  try {
+    check_unfilter(h);
+    to_host(out, h->d_stats, h->G * h->P * 4 * sizeof(double), h->stream);
+    if (kernel_ms) *kernel_ms = h->last_ms;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_unfilter_sirode_free(dust2_unfilter* h) { delete h; }
+
+}  // extern "C"

@@ -192,6 +192,96 @@ __device__ real binomial_btrs(rng256& rng, real n, real p) {
   }
 }
 
+// ---- FP32 fast path ----------------------------------------------------------
+// The FP32 samplers have no reference draw sequence to reproduce, so they
+// use the hardware reciprocal/log/exp (1-2 ulp) and a rejection bound that
+// is free of cancellation, keeping the whole draw in FP32 registers.
+__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float f_log(float x) { return __logf(x); }
+
+// Inversion, n*p < 10, p <= 1/2.
+__device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
+  const float q = 1 - p;
+  const float r = p * f_rcp(q);
+  const float g = r * (n + 1);
+  const float f0 = __expf(n * log1pf(-p));
+  while (true) {
+    float u = random_real<float>(rng);
+    float f = f0;
+    float k = 0;
+    bool ok = true;
+    while (u >= f) {
+      u -= f;
+      k += 1;
+      if (k > n) {
+        ok = false;
+        break;
+      }
+      f *= g * f_rcp(k) - r;
+    }
+    if (ok) {
+      return k;
+    }
+  }
+}
+
+// BTRS with the acceptance test written as
+//   log pmf(k) - log pmf(m) = d*log(p(n-m+1) / (q(m+1)))
+//                             - (k+1/2) log1p(d/(m+1)) - (n-k+1/2) log1p(-d/(n-m+1))
+//                             + tail(m) + tail(n-m) - tail(k) - tail(n-k),  d = k - m,
+// the same quantity as the reference form (Stirling's formula for the
+// factorials) with the n*log(n)-sized terms cancelled analytically.
+__device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
+  const float q = 1 - p;
+  const float spq = __builtin_amdgcn_sqrtf(n * p * q);
+  const float b = 1.15f + 2.53f * spq;
+  const float a = -0.0873f + 0.0248f * b + 0.01f * p;
+  const float c = n * p + 0.5f;
+  const float rb = f_rcp(b);
+  const float v_r = 0.92f - 4.2f * rb;
+  const float two_a = 2 * a;
+  while (true) {
+    const float u = random_real<float>(rng) - 0.5f;
+    const float v = random_real<float>(rng);
+    const float us = 0.5f - fabsf(u);
+    const float k = floorf((two_a * f_rcp(us) + b) * u + c);
+    if (us >= 0.07f && v <= v_r) {
+      return k;
+    }
+    if (k < 0 || k > n) {
+      continue;
+    }
+    const float alpha = (2.83f + 5.1f * rb) * spq;
+    const float m = floorf((n + 1) * p);
+    const float d = k - m;
+    const float rm = f_rcp(m + 1);
+    const float rnm = f_rcp(n - m + 1);
+    const float L = f_log(p * (n - m + 1) * f_rcp(q * (m + 1)));
+    const float ratio = d * L - (k + 0.5f) * log1pf(d * rm) - (n - k + 0.5f) * log1pf(-d * rnm) +
+                        stirling_tail(m) + stirling_tail(n - m) - stirling_tail(k) -
+                        stirling_tail(n - k);
+    const float lv = f_log(v * alpha * f_rcp(a * f_rcp(us * us) + b));
+    if (lv <= ratio) {
+      return k;
+    }
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ real binomial_core(rng256& rng, real n, real q) {
+  return (n * q >= 10) ? binomial_btrs<real>(rng, n, q) : binomial_inversion<real>(rng, n, q);
+}
+
+// Called, not inlined, from the step loops: a single out-of-line copy keeps
+// the rejection loops' temporaries out of the caller's register budget.
+#ifndef DUST2_BINOMIAL_LINKAGE
+#define DUST2_BINOMIAL_LINKAGE __noinline__
+#endif
+template <>
+__device__ DUST2_BINOMIAL_LINKAGE float binomial_core<float>(rng256& rng, float n, float q) {
+  return (n * q >= 10) ? binomial_btrs_f32(rng, n, q) : binomial_inversion_f32(rng, n, q);
+}
+
 // Binomial(n, p).  Returns ERR_BINOMIAL in *err (and 0) on invalid input,
 // as the reference throws; deterministic streams return n*p.
 template <typename real>
@@ -212,8 +302,7 @@ __device__ __forceinline__ real binomial(rng256& rng, real n, real p, bool deter
   }
   const bool flip = p > static_cast<real>(0.5);
   const real q = flip ? 1 - p : p;
-  const real k = (n * q >= 10) ? binomial_btrs<real>(rng, n, q)
-                               : binomial_inversion<real>(rng, n, q);
+  const real k = binomial_core<real>(rng, n, q);
   return flip ? n - k : k;
 }
 

@@ -77,6 +77,37 @@ int sir_filter_max_particles(bool fp64);
 launch_config sir_filter_default_config(int n_particles, bool fp64);
 bool sir_filter_config_valid(launch_config cfg, bool fp64);
 
+// ---- ODE unfilter ----------------------------------------------------------
+// dust2/continuous/control.hpp:11-24 (step_size_min already clamped to
+// machine epsilon, dust2/r/continuous/control.hpp:15-17)
+struct ode_control_dev {
+  uint32_t max_steps;
+  double atol, rtol, step_size_min, step_size_max;
+  double factor_safe, factor_min, factor_max, beta, constant;
+};
+
+struct ode_launch {
+  const double* pars;        // [G][4]: N, I0, beta, gamma
+  const uint8_t* active;     // [G] groups to run, or null for all
+  int n_particles;
+  int n_total;               // G * P
+  double time_start;
+  const double* times;       // [n_time]
+  int n_time;
+  const double* obs;         // obs[t * obs_stride + g * obs_group_stride]
+  const double* lfact;       // same layout
+  size_t obs_stride;
+  size_t obs_group_stride;   // 0 for shared data, 1 per group
+  ode_control_dev control;
+  void* state;               // real [S][n_total]
+  double* ll;                // [n_total]
+  double* stats;             // [n_total][4] or null
+  error_record* errors;      // [n_total]
+  uint32_t* error_count;
+};
+
+hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream);
+
 // ---- system (state resident in HBM between calls) -------------------------
 // state: real [G][S][P]; rng: [G][4][P]; pars: [G][8] doubles.
 hipError_t launch_sir_run_to_time(void* state, uint64_t* rng, const double* pars, int G,

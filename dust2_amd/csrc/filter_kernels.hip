@@ -23,22 +23,16 @@ namespace dust2_amd {
 
 namespace {
 
-// Register-array access with a uniform index: a chain of uniform branches
-// keeps the arrays in VGPRs (a dynamic index would spill them to scratch).
+// The thread's particles are advanced one at a time in a rolled loop (one
+// inlined copy of the model).  Each iteration works on slot 0 of the
+// register arrays and then rotates them by one, so every array index is a
+// compile-time constant and the arrays stay in VGPRs; after PPT iterations
+// the original order is restored.
 template <int PPT, typename T>
-__device__ __forceinline__ void reg_load(T& dst, const T (&src)[PPT], int k) {
+__device__ __forceinline__ void reg_rotate(T (&a)[PPT], const T& last) {
 #pragma unroll
-  for (int kk = 0; kk < PPT; ++kk) {
-    if (k == kk) dst = src[kk];
-  }
-}
-
-template <int PPT, typename T>
-__device__ __forceinline__ void reg_store(T (&dst)[PPT], const T& src, int k) {
-#pragma unroll
-  for (int kk = 0; kk < PPT; ++kk) {
-    if (k == kk) dst[kk] = src;
-  }
+  for (int kk = 0; kk + 1 < PPT; ++kk) a[kk] = a[kk + 1];
+  a[PPT - 1] = last;
 }
 
 template <typename real>
@@ -116,21 +110,29 @@ __global__ void __launch_bounds__(BLOCK) sir_filter_kernel(const filter_slot* __
     const int n_steps = static_cast<int>(round(fmax(0.0, t_target - time) / dt));
     const double* obs = sd.obs + static_cast<size_t>(t) * sd.obs_stride;
     const double* lfact = sd.lfact + static_cast<size_t>(t) * sd.obs_stride;
+    // zero_every schedule of this interval, computed once: bit s set when
+    // fmod(time + s*dt, period) == 0 (dust2/discrete/system.hpp:411-419)
+    const bool zero_generic = n_steps > 64;
+    uint64_t zero_mask = 0;
+    if (!zero_generic) {
+      for (int s = 0; s < n_steps; ++s) {
+        if (fmod(time + s * dt, sir_model::zero_period) == 0) zero_mask |= uint64_t(1) << s;
+      }
+    }
 
     double w[PPT];
 #pragma unroll 1
     for (int k = 0; k < PPT; ++k) {
       const int p = p0 + k;
-      sir_particle<real> cur;
-      rng256 r;
-      reg_load<PPT>(cur, part, k);
-      reg_load<PPT>(r, rng, k);
+      sir_particle<real> cur = part[0];
+      rng256 r = rng[0];
       double wk = 0;
       if (p < P) {
         bool failed = (fail_mask >> k) & 1u;
         for (int s = 0; s < n_steps && !failed; ++s) {
-          const double time_s = time + s * dt;
-          if (fmod(time_s, sir_model::zero_period) == 0) {
+          const bool zero = zero_generic ? fmod(time + s * dt, sir_model::zero_period) == 0
+                                         : ((zero_mask >> s) & 1u) != 0;
+          if (zero) {
             cur.y[sir_model::zero_first] = 0;
           }
           uint32_t err = 0;
@@ -145,9 +147,9 @@ __global__ void __launch_bounds__(BLOCK) sir_filter_kernel(const filter_slot* __
         }
         wk = sir_model::compare<real>(cur.y, obs, lfact, sh, r, det);
       }
-      reg_store<PPT>(part, cur, k);
-      reg_store<PPT>(rng, r, k);
-      reg_store<PPT>(w, wk, k);
+      reg_rotate<PPT>(part, cur);
+      reg_rotate<PPT>(rng, r);
+      reg_rotate<PPT>(w, wk);
     }
     time = time + n_steps * dt;
 

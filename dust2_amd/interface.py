@@ -125,6 +125,8 @@ def prepare_data(data, n_groups, name="incidence"):
     observed column, shape (n_time,) shared or (n_time, n_groups)."""
     time = f64(data["time"]).reshape(-1)
     obs = f64(data[name])
+    if obs.ndim == 2 and obs.shape[1] == 1:
+        obs = obs[:, 0]
     if obs.ndim == 1:
         if obs.shape[0] != time.shape[0]:
             raise ValueError("Expected data columns to have the same length as 'time'")
@@ -359,10 +361,82 @@ def dust_system_set_rng_state(sys, state):
 
 
 # ---------------------------------------------------------------------------
-# Unfilter (deterministic likelihood) -- defined in unfilter.py
+# Unfilter (deterministic likelihood)
 # ---------------------------------------------------------------------------
-class Unfilter:  # replaced by the implementation in unfilter.py
-    pass
+def dust_ode_control(max_steps=10000, atol=1e-6, rtol=1e-6, step_size_min=0,
+                     step_size_max=np.inf):
+    """R/interface-continuous.R:51-79 (critical times, history and step-time
+    recording are not supported on this path)."""
+    return np.array([max_steps, atol, rtol, step_size_min, step_size_max], dtype=np.float64)
+
+
+class Unfilter:
+    """dust_unfilter_create (R/interface-unfilter.R:20-71) for the SIR ODE;
+    the likelihood of each particle is returned (P identical particles per
+    group give identical values)."""
+
+    def __init__(self, generator, time_start, data, n_particles, n_groups, ode_control,
+                 precision):
+        if generator.name != "sirode":
+            raise ValueError(f"No unfilter kernels for '{generator.name}'")
+        self.generator = generator
+        self.time_start = float(time_start)
+        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups)
+        self.n_particles = int(n_particles)
+        self.n_groups = int(n_groups)
+        self.control = dust_ode_control() if ode_control is None else f64(ode_control)
+        self.precision = _precision(precision)
+        self.ptr = None
+        self.rows = None
+
+    @property
+    def initialised(self):
+        return self.ptr is not None
+
+    def run(self, pars, index_group=None):
+        idx, n_idx, n_out = _index_group(index_group, self.n_groups)
+        if self.ptr is None:
+            if pars is None:
+                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+            self.rows = pars_matrix(self.generator, pars, self.n_groups)
+            out = C.c_void_p()
+            call("dust2_unfilter_sirode_alloc", dptr(self.rows), self.n_groups,
+                 self.n_particles, self.time_start, dptr(self.time), self.time.size,
+                 dptr(self.data), self.n_groups_data, dptr(self.control), self.precision,
+                 C.byref(out))
+            self.ptr = out
+        elif pars is not None:
+            sel = np.arange(self.n_groups) if idx is None else idx.astype(np.int64)
+            rows = pars_update(self.generator, pars, self.rows[sel])
+            call("dust2_unfilter_sirode_update_pars", self.ptr, dptr(rows), szptr(idx), n_idx)
+            self.rows[sel] = rows
+        ll = np.zeros((n_out, self.n_particles))
+        call("dust2_unfilter_sirode_run", self.ptr, szptr(idx), n_idx, dptr(ll))
+        if self.n_particles == 1:
+            ll = ll[:, 0]
+        return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1 and index_group is None) \
+            else ll
+
+    def last_state(self):
+        out = np.zeros((self.n_groups, self.n_particles, 5))
+        call("dust2_unfilter_sirode_last_state", self.ptr, dptr(out))
+        return out
+
+    def last_stats(self):
+        out = np.zeros((self.n_groups * self.n_particles, 4))
+        ms = np.zeros(1)
+        call("dust2_unfilter_sirode_last_stats", self.ptr, dptr(out), dptr(ms))
+        return out, float(ms[0])
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib._lib is not None:
+            _lib.lib().dust2_unfilter_sirode_free(self.ptr)
+            self.ptr = None
+
+
+def dust_unfilter_create(generator, time_start, data, n_particles=1, n_groups=1,
+                         ode_control=None, precision="fp64"):
+    return Unfilter(generator, time_start, data, n_particles, n_groups, ode_control, precision)
 
 
 # ---------------------------------------------------------------------------

@@ -134,6 +134,37 @@ int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char*
                                    size_t errlen);
 void dust2_system_sir_free(dust2_system* h);
 
+/* ---- deterministic likelihood of the SIR ODE (Dormand-Prince 5(4)) --------
+ * Replaces dust2_continuous_unfilter_alloc<sirode>
+ * (dust2/r/continuous/unfilter.hpp:14-58, registered as
+ * dust2_unfilter_sirode_alloc, inst/template/compare.cpp:6-29).
+ * control: NULL (dust_ode_control() defaults, R/interface-continuous.R:51-79)
+ * or [max_steps, atol, rtol, step_size_min, step_size_max]; step_size_min is
+ * clamped to machine epsilon (dust2/r/continuous/control.hpp:15-17).
+ * precision: DUST2_FP64, or DUST2_FP32 for FP32 state and stages with the
+ * error norm, step size and controller kept in FP64. */
+int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                                double time_start, const double* time, size_t n_time,
+                                const double* data, size_t n_groups_data, const double* control,
+                                int precision, dust2_unfilter** out, char* err, size_t errlen);
+
+/* Replaces dust2_unfilter_run<T> (dust2/r/unfilter.hpp:21-62) /
+ * unfilter<T>::run (dust2/unfilter.hpp:42-72): ll_out receives
+ * n_particles log-likelihoods per selected group. */
+int dust2_unfilter_sirode_run(dust2_unfilter* h, const size_t* index_group, size_t n_index,
+                              double* ll_out, char* err, size_t errlen);
+int dust2_unfilter_sirode_update_pars(dust2_unfilter* h, const double* pars,
+                                      const size_t* index_group, size_t n_index, char* err,
+                                      size_t errlen);
+/* out [G][P][S] */
+int dust2_unfilter_sirode_last_state(dust2_unfilter* h, double* out, char* err, size_t errlen);
+/* Solver counters of the last run, as dust_system_internals reports them
+ * (dust2/r/continuous/system.hpp:56-158): out [G*P][4] = n_steps,
+ * n_steps_accepted, n_steps_rejected, step_size; kernel_ms may be NULL. */
+int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* kernel_ms,
+                                     char* err, size_t errlen);
+void dust2_unfilter_sirode_free(dust2_unfilter* h);
+
 /* ---- kernels exposed for parity tests --------------------------------------
  * Systematic resampling of one weight vector on the GPU with the filter's
  * own workgroup routine (replaces the reference test driver

@@ -1,0 +1,104 @@
+"""GPU parity of the Dormand-Prince 5(4) unfilter (SIR ODE).
+
+FP64: log-likelihoods within 1e-10 relative of the reference solver's golden
+vectors (tests/golden/sirode_dp5.json, written from the reference's own
+dust2/continuous/solver.hpp), identical step counters, states within 1e-10.
+FP32 state with FP64 error norm and controller: states and log-likelihoods
+within the solver tolerance of the FP64 oracle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "sirode_dp5.json")
+
+
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def make_unfilter(lib, pars, time, data, P=1, control=None, precision="fp64"):
+    G = len(pars)
+    obj = lib.dust_unfilter_create(lib.sirode(), 0.0, {"time": time, "incidence": data},
+                                   n_particles=P, n_groups=G, ode_control=control,
+                                   precision=precision)
+    plist = [dict(zip(["N", "I0", "beta", "gamma"], p)) for p in pars]
+    ll = obj.run(plist)
+    return obj, np.atleast_1d(ll)
+
+
+def test_matches_reference_solver_golden(gpu_lib):
+    g = golden()
+    pars = np.array(g["pars"])
+    data = np.array(g["data"])
+    obj, ll = make_unfilter(gpu_lib, pars, g["time"], data)
+    np.testing.assert_allclose(ll, g["ll"], rtol=1e-10)
+    stats, _ = obj.last_stats()
+    np.testing.assert_array_equal(stats[:, :3], np.array(g["stats"])[:, :3])
+    np.testing.assert_allclose(stats[:, 3], np.array(g["stats"])[:, 3], rtol=1e-10)
+    final = np.array(g["state"])[:, -1, :]
+    np.testing.assert_allclose(obj.last_state()[:, 0, :], final, rtol=1e-10, atol=1e-10)
+
+
+def test_tight_control_matches_reference_solver(gpu_lib):
+    g = golden()
+    pars = np.array(g["pars"])[:4]
+    data = np.array(g["data"][:60])
+    obj, ll = make_unfilter(gpu_lib, pars, g["time"][:60], data,
+                            control=gpu_lib.dust_ode_control(*g["control2"]))
+    np.testing.assert_allclose(ll, g["ll2"], rtol=1e-10)
+    stats, _ = obj.last_stats()
+    np.testing.assert_array_equal(stats[:, :3], np.array(g["stats2"])[:, :3])
+
+
+def test_many_parameter_sets_match_oracle(gpu_lib):
+    rng = np.random.default_rng(3)
+    n = 4096
+    pars = np.column_stack([np.full(n, 1000.0), np.full(n, 10.0), rng.uniform(0.1, 0.5, n),
+                            rng.uniform(0.05, 0.2, n)])
+    time = np.arange(1.0, 366.0)
+    data = rng.poisson(6, 365).astype(float)
+    obj, ll = make_unfilter(gpu_lib, pars, time, data)
+    ref = ol.sirode_unfilter(pars, 1, 0.0, time, data, n_threads=8)
+    np.testing.assert_allclose(ll, ref["ll"], rtol=1e-10)
+    np.testing.assert_allclose(obj.last_state(), ref["state"], rtol=1e-10, atol=1e-9)
+
+
+def test_fp32_state_within_tolerance(gpu_lib):
+    rng = np.random.default_rng(4)
+    n = 2048
+    pars = np.column_stack([np.full(n, 1000.0), np.full(n, 10.0), rng.uniform(0.1, 0.5, n),
+                            rng.uniform(0.05, 0.2, n)])
+    time = np.arange(1.0, 366.0)
+    data = rng.poisson(6, 365).astype(float)
+    obj, ll = make_unfilter(gpu_lib, pars, time, data, precision="fp32")
+    ref = ol.sirode_unfilter(pars, 1, 0.0, time, data, n_threads=8)
+    # cumulative cases: relative error within 1e-4 (rtol 1e-6 per step,
+    # FP32 stage arithmetic); log-likelihood within 1e-4 relative
+    np.testing.assert_allclose(obj.last_state()[:, 0, 3], ref["state"][:, 0, 3], rtol=1e-4)
+    np.testing.assert_allclose(ll, ref["ll"], rtol=1e-4)
+
+
+def test_replicated_particles_identical(gpu_lib):
+    # test-unfilter.R:148-175
+    pars = np.array([[1000, 10, 0.3, 0.1], [1000, 10, 0.2, 0.1]])
+    time = np.arange(1.0, 31.0)
+    data = np.arange(30.0)
+    obj, ll = make_unfilter(gpu_lib, pars, time, data, P=4)
+    ll = ll.reshape(2, 4)
+    assert np.all(ll == ll[:, :1])
+
+
+def test_too_many_steps_error(gpu_lib):
+    # test-sirode.R:67-103
+    pars = np.array([[1000, 10, 0.3, 0.1]])
+    with pytest.raises(gpu_lib.DustError, match="too many steps"):
+        make_unfilter(gpu_lib, pars, np.arange(1.0, 11.0), np.ones(10),
+                      control=gpu_lib.dust_ode_control(max_steps=5))
