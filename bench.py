@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--n-obs", type=int, default=365)
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--launch", default=None, help="block,particles_per_thread")
+    ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "pipeline"])
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -98,13 +100,16 @@ def main():
                                  n_groups=G, dt=dt, seed=1, precision=args.precision)
     if args.launch:
         obj.set_launch(*[int(x) for x in args.launch.split(",")])
+    obj.set_engine(args.engine, graph=not args.no_graph)
     plist = [dict(zip(SIR_NAMES, p)) for p in pars]
+    # every call passes the parameters, as an MCMC driver does (the update
+    # is host-side bookkeeping)
     for _ in range(args.warmup):
         lib.dust_likelihood_run(obj, plist)
     kernel_ms = []
     t0 = _time.perf_counter()
     for _ in range(args.steps):
-        ll = lib.dust_likelihood_run(obj, None)
+        ll = lib.dust_likelihood_run(obj, plist)
         kernel_ms.append(lib.dust_likelihood_last_stats(obj)["kernel_ms"])
     elapsed = _time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
@@ -112,9 +117,26 @@ def main():
     psteps = G * P * n_steps
     value = psteps * args.steps / elapsed
     s_x = 4 if args.precision == "fp32" else 8
+    # algorithmic bytes per particle per observation: state gather-read and
+    # write, xoshiro256+ read and write, FP64 weight (lambda) write and read,
+    # u32 ancestor write and read
     b_obs = 2 * 5 * s_x + 2 * 32 + 2 * 8 + 2 * 4
-    b_launch = G * P * time.size * b_obs
-    k_ms = float(np.mean(kernel_ms))
+    stats = lib.dust_likelihood_last_stats(obj)
+    engine = stats["engine"]
+    if engine == "persistent":
+        kernel = "sir_filter_kernel"
+        k_ms = float(np.mean(kernel_ms))
+        b_launch = G * P * time.size * b_obs
+    else:
+        # the advance kernel moves the state, RNG, lambda write and
+        # ancestor read of one interval; timed per launch with events in
+        # one extra profiled run
+        kernel = "sir_advance_kernel"
+        obj.set_engine("pipeline", graph=False, profile=True)
+        lib.dust_likelihood_run(obj, plist)
+        k_ms = lib.dust_likelihood_last_stats(obj)["advance_ms"]
+        obj.set_engine(args.engine, graph=not args.no_graph)
+        b_launch = G * P * (2 * 5 * s_x + 2 * 32 + 8 + 4)
     achieved = b_launch / (k_ms * 1e-3) / 1e9
     out = {
         "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s",
@@ -124,12 +146,14 @@ def main():
         "data": "synthetic (seeded simulation of each group's model)",
         "config": {"workload": "batched stochastic SIR bootstrap filter",
                    "groups": G, "particles": P, "n_obs": int(time.size), "dt": dt,
-                   "precision": args.precision, "launch": obj.launch},
+                   "precision": args.precision, "launch": obj.launch, "engine": engine,
+                   "graph": not args.no_graph},
         "likelihood_evals_per_s": G * args.steps / elapsed,
+        "gpu_ms_per_step": float(np.mean(kernel_ms)),
         "mean_ll": float(np.mean(ll)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sir_filter_kernel", "kernel_ms": k_ms,
+                     "kernel": kernel, "kernel_ms": k_ms, "bytes_per_launch": b_launch,
                      "bytes_per_particle_obs": b_obs},
     }
     if not args.no_cpu:

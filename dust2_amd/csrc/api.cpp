@@ -349,17 +349,29 @@ struct dust2_filter {
   std::vector<double> times;
   hipStream_t stream = nullptr;
   launch_config cfg{0, 0};
+  int engine = DUST2_ENGINE_AUTO;
+  bool use_graph = true;
+  bool profile = false;
   dbuf d_times, d_obs, d_lfact, d_state, d_rng, d_ll, d_time, d_slots, d_errors, d_counters,
       d_ancestry, d_traj;
+  // interval pipeline
+  dbuf d_state_b, d_lambda, d_anc, d_sched_time, d_sched_steps, d_sched_zero, d_kpars, d_pslots,
+      d_pslots_init;
+  hipGraphExec_t graph = nullptr;
+  std::vector<uint64_t> graph_key;
   bool traj_valid = false;
   bool ancestry_valid = false;
   double last_ms = 0;
   double last_fallbacks = 0;
+  double last_advance_ms = 0;   // mean advance-kernel time (profile mode)
+  double last_resample_ms = 0;  // mean resample-kernel time (profile mode)
+  int last_engine = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   size_t real_size() const { return fp64 ? 8 : 4; }
 
   ~dust2_filter() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -412,6 +424,196 @@ void check_filter(dust2_filter* h) {
   if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (filter)");
 }
 
+// Step schedule of the data intervals, as dust_discrete::run_to_time
+// computes it (dust2/discrete/system.hpp:61-92): n_steps =
+// round(max(0, t - time)/dt), time += n_steps*dt, and the zero_every rule
+// fmod(time + s*dt, 1) == 0 evaluated once per step on the host.
+void filter_setup_pipeline(dust2_filter* h) {
+  if (h->d_sched_time.bytes()) return;
+  const size_t T = h->n_time, G = h->G, P = h->P;
+  std::vector<double> st(T);
+  std::vector<uint32_t> ns(T);
+  std::vector<uint64_t> zm(T);
+  double tcur = h->time_start;
+  for (size_t t = 0; t < T; ++t) {
+    const size_t n = static_cast<size_t>(std::round(std::max(0.0, h->times[t] - tcur) / h->dt));
+    st[t] = tcur;
+    ns[t] = static_cast<uint32_t>(n);
+    uint64_t m = 0;
+    for (size_t s = 0; s < std::min<size_t>(n, 64); ++s) {
+      if (std::fmod(tcur + s * h->dt, 1.0) == 0) m |= uint64_t(1) << s;
+    }
+    zm[t] = m;
+    tcur = tcur + n * h->dt;
+  }
+  h->d_sched_time.alloc(T * sizeof(double));
+  h->d_sched_steps.alloc(T * sizeof(uint32_t));
+  h->d_sched_zero.alloc(T * sizeof(uint64_t));
+  to_device(h->d_sched_time, st.data(), T * sizeof(double), h->stream);
+  to_device(h->d_sched_steps, ns.data(), T * sizeof(uint32_t), h->stream);
+  to_device(h->d_sched_zero, zm.data(), T * sizeof(uint64_t), h->stream);
+  h->d_state_b.alloc(G * kSirState * P * h->real_size());
+  h->d_lambda.alloc(G * P * sizeof(double));
+  h->d_anc.alloc(G * P * sizeof(uint32_t));
+  h->d_kpars.alloc(G * 8 * sizeof(double));
+  h->d_pslots.alloc(G * sizeof(pipe_slot));
+  h->d_pslots_init.alloc(G * sizeof(pipe_slot));
+  hip_check(hipStreamSynchronize(h->stream), "sync");
+}
+
+pipe_slot make_pipe_slot(dust2_filter* h, size_t g, bool save_traj) {
+  const size_t P = h->P;
+  const size_t rsz = h->real_size();
+  pipe_slot sl{};
+  sl.pars = h->d_kpars.get<double>() + 8 * g;
+  const size_t goff = h->n_groups_data == 1 ? 0 : g;
+  sl.obs = h->d_obs.get<double>() + goff;
+  sl.lfact = h->d_lfact.get<double>() + goff;
+  sl.obs_stride = static_cast<uint32_t>(h->n_groups_data);
+  sl.n_particles = static_cast<uint32_t>(P);
+  sl.flags = 0;
+  sl.group = static_cast<uint32_t>(g);
+  sl.sched_time = h->d_sched_time.get<double>();
+  sl.sched_steps = h->d_sched_steps.get<uint32_t>();
+  sl.sched_zero = h->d_sched_zero.get<uint64_t>();
+  sl.dt = h->dt;
+  sl.state_a = h->d_state.get<char>() + g * kSirState * P * rsz;
+  sl.state_b = h->d_state_b.get<char>() + g * kSirState * P * rsz;
+  sl.rng = h->d_rng.get<uint64_t>() + g * 4 * (P + 2);
+  sl.lambda = h->d_lambda.get<double>() + g * P;
+  sl.ancestors = h->d_anc.get<uint32_t>() + g * P;
+  sl.ll = h->d_ll.get<double>() + g;
+  sl.ancestry = save_traj ? h->d_ancestry.get<uint32_t>() + g * h->n_time * P : nullptr;
+  sl.traj = save_traj ? h->d_traj.get<char>() + g * h->n_time * kSirState * P * rsz : nullptr;
+  sl.errors = h->d_errors.get<error_record>() + g * P;
+  sl.error_count = h->d_counters.get<uint32_t>();
+  sl.n_fallback = h->d_counters.get<uint32_t>() + 1;
+  return sl;
+}
+
+// The launches of one pipeline run.  With `profile`, events around every
+// launch accumulate per-kernel times (not usable under graph capture).
+void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool save_traj,
+                       std::vector<hipEvent_t>* evs) {
+  hipStream_t s = h->stream;
+  const int P = static_cast<int>(h->P);
+  const pipe_slot* slots = h->d_pslots.get<pipe_slot>();
+  const int T = static_cast<int>(h->n_time);
+  auto mark = [&]() {
+    if (evs) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      hip_check(hipEventRecord(e, s), "hipEventRecord");
+      evs->push_back(e);
+    }
+  };
+  hip_check(launch_pipe_reset(slots, n_slots, s), "reset kernel");
+  int parity = 0;
+  for (int t = 0; t < T; ++t) {
+    uint32_t mode = PIPE_STEP;
+    if (t == 0) {
+      mode |= initial_given ? 0u : PIPE_INITIAL;
+    } else {
+      mode |= PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u);
+    }
+    mark();
+    hip_check(launch_sir_advance(slots, n_slots, P, t, parity, mode, h->fp64, s),
+              "advance kernel");
+    mark();
+    parity ^= 1;
+    hip_check(launch_pipe_resample(slots, n_slots, P, t, s), "resample kernel");
+    mark();
+  }
+  hip_check(launch_sir_advance(slots, n_slots, P, T, parity,
+                               PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u), h->fp64, s),
+            "gather kernel");
+}
+
+// Runs the filter through the interval pipeline; the final state is left
+// in d_state.
+void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_given,
+                  bool save_traj) {
+  filter_setup_pipeline(h);
+  hipStream_t s = h->stream;
+  const size_t G = h->G;
+  std::vector<double> kp(G * 8);
+  for (size_t g = 0; g < G; ++g) sir_kernel_pars(&h->pars[g * kSirPars], h->dt, &kp[g * 8]);
+  to_device(h->d_kpars, kp.data(), kp.size() * sizeof(double), s);
+  std::vector<pipe_slot> slots;
+  std::vector<char> selected(G, 0);
+  for (size_t g : idx) {
+    slots.push_back(make_pipe_slot(h, g, save_traj));
+    selected[g] = 1;
+  }
+  to_device(h->d_pslots, slots.data(), slots.size() * sizeof(pipe_slot), s);
+  hip_check(hipStreamSynchronize(s), "sync");
+  const int n_slots = static_cast<int>(slots.size());
+  const int T = static_cast<int>(h->n_time);
+
+  if (h->profile) {
+    std::vector<hipEvent_t> evs;
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    pipeline_launches(h, n_slots, initial_given, save_traj, &evs);
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+    hip_check(hipStreamSynchronize(s), "sync");
+    double adv = 0, res = 0;
+    for (int t = 0; t < T; ++t) {
+      float a = 0, b = 0;
+      hip_check(hipEventElapsedTime(&a, evs[3 * t], evs[3 * t + 1]), "elapsed");
+      hip_check(hipEventElapsedTime(&b, evs[3 * t + 1], evs[3 * t + 2]), "elapsed");
+      adv += a;
+      res += b;
+    }
+    for (auto e : evs) (void)hipEventDestroy(e);
+    h->last_advance_ms = adv / T;
+    h->last_resample_ms = res / T;
+  } else if (h->use_graph) {
+    const std::vector<uint64_t> key = {static_cast<uint64_t>(n_slots), h->P, h->n_time,
+                                       initial_given, save_traj, h->fp64};
+    if (!h->graph || key != h->graph_key) {
+      if (h->graph) {
+        hip_check(hipGraphExecDestroy(h->graph), "hipGraphExecDestroy");
+        h->graph = nullptr;
+      }
+      hipGraph_t g;
+      hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+      pipeline_launches(h, n_slots, initial_given, save_traj, nullptr);
+      hip_check(hipStreamEndCapture(s, &g), "end capture");
+      hip_check(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0), "graph instantiate");
+      hip_check(hipGraphDestroy(g), "hipGraphDestroy");
+      h->graph_key = key;
+    }
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    hip_check(hipGraphLaunch(h->graph, s), "hipGraphLaunch");
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+  } else {
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    pipeline_launches(h, n_slots, initial_given, save_traj, nullptr);
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+  }
+  // the final gather wrote buffer (T + 1) % 2 in the A/B naming
+  if ((T + 1) % 2 == 1) {
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->d_state.swap(h->d_state_b);
+  }
+  // groups outside index_group are reset to the initial state
+  // (dust2/filter.hpp:248-250) but not run
+  if (!initial_given) {
+    std::vector<pipe_slot> rest;
+    for (size_t g = 0; g < G; ++g) {
+      if (!selected[g]) rest.push_back(make_pipe_slot(h, g, false));
+    }
+    if (!rest.empty()) {
+      to_device(h->d_pslots_init, rest.data(), rest.size() * sizeof(pipe_slot), s);
+      hip_check(launch_sir_advance(h->d_pslots_init.get<pipe_slot>(),
+                                   static_cast<int>(rest.size()), static_cast<int>(h->P), 0, 1,
+                                   PIPE_INITIAL, h->fp64, s),
+                "initial kernel");
+      hip_check(hipStreamSynchronize(s), "sync");
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -458,14 +660,11 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
     h->dt = dt;
     h->pars.assign(pars, pars + n_groups * kSirPars);
     h->times.assign(time, time + n_time);
-    if (static_cast<int>(n_particles) > sir_filter_max_particles(h->fp64)) {
-      throw std::runtime_error("'n_particles' exceeds " +
-                               std::to_string(sir_filter_max_particles(h->fp64)) +
-                               " per group");
-    }
     const size_t lds = (kScratchDoubles + n_particles) * sizeof(double);
-    if (lds > static_cast<size_t>(max_lds_per_block())) {
-      throw std::runtime_error("'n_particles' too large for one workgroup's LDS");
+    if (lds > static_cast<size_t>(max_lds_per_block()) ||
+        pipe_resample_ppt(static_cast<int>(n_particles)) == 0) {
+      throw std::runtime_error("'n_particles' too large: one group's weights must fit one "
+                               "workgroup (at most 16384 particles)");
     }
     h->cfg = sir_filter_default_config(static_cast<int>(n_particles), h->fp64);
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -527,9 +726,18 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
     to_device(h->d_time, t0.data(), G * sizeof(double), s);
     hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
 
+    int engine = h->engine;
+    if (engine == DUST2_ENGINE_AUTO) {
+      engine = static_cast<int>(P) > sir_filter_max_particles(h->fp64) ? DUST2_ENGINE_PIPELINE
+                                                                       : DUST2_ENGINE_PERSISTENT;
+    }
+    h->last_engine = engine;
+    std::vector<filter_slot> slots;
+    if (engine == DUST2_ENGINE_PIPELINE) {
+      pipeline_run(h, idx, initial != nullptr, save_trajectories != 0);
+    } else {
     std::vector<char> selected(G, 0);
     for (size_t g : idx) selected[g] = 1;
-    std::vector<filter_slot> slots;
     const uint32_t stride = static_cast<uint32_t>(h->n_groups_data);
     for (size_t g = 0; g < G; ++g) {
       // groups outside index_group are reset (filter.hpp:248-250) but not run
@@ -567,6 +775,7 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
                                 static_cast<int>(P), h->fp64, h->cfg, s),
               "sir filter kernel");
     hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+    }
     uint32_t counters[4];
     to_host(counters, h->d_counters, sizeof(counters), s);
     float ms = 0;
@@ -728,6 +937,32 @@ int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t 
     check_filter(h);
     out[0] = h->last_fallbacks;
     out[1] = h->last_ms;
+    out[2] = h->last_engine;
+    out[3] = h->last_advance_ms;
+    out[4] = h->last_resample_ms;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_sir_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
+                                char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (engine != DUST2_ENGINE_AUTO && engine != DUST2_ENGINE_PERSISTENT &&
+        engine != DUST2_ENGINE_PIPELINE) {
+      throw std::runtime_error("unknown engine");
+    }
+    if (engine == DUST2_ENGINE_PERSISTENT &&
+        static_cast<int>(h->P) > sir_filter_max_particles(h->fp64)) {
+      throw std::runtime_error("the persistent engine holds at most " +
+                               std::to_string(sir_filter_max_particles(h->fp64)) +
+                               " particles per group");
+    }
+    h->engine = engine;
+    h->use_graph = use_graph != 0;
+    h->profile = profile != 0;
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

@@ -199,12 +199,51 @@ __device__ real binomial_btrs(rng256& rng, real n, real p) {
 __device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float f_log(float x) { return __logf(x); }
 
+// log(1 + x) accurate for small |x| from the hardware log (Goldberg 1991,
+// "What every computer scientist should know about floating-point
+// arithmetic", theorem 4): log(u) * x / (u - 1) with u = 1 + x rounded.
+__device__ __forceinline__ float f_log1p(float x) {
+  const float u = 1.0f + x;
+  const float d = u - 1.0f;
+  return d == 0.0f ? x : f_log(u) * (x * f_rcp(d));
+}
+
+// Two uniforms on the 2^-24 grid from one 64-bit output: bits 40..63 and
+// 16..39 (the low bits of xoshiro256+ are the weak ones and are unused).
+__device__ __forceinline__ void random_pair_f32(rng256& r, float& u, float& v) {
+  const uint64_t x = rng_next(r);
+  u = static_cast<float>(static_cast<uint32_t>(x >> 40)) * 0x1.0p-24f;
+  v = static_cast<float>(static_cast<uint32_t>(x >> 16) & 0xffffffu) * 0x1.0p-24f;
+}
+
+// Stirling-series remainder in FP32: exact values below 10 from a table,
+// the series 1/12 - 1/360 x^-2 + 1/1260 x^-4 over x = k + 1 above.
+__device__ __forceinline__ float stirling_tail_f32(float k) {
+  if (k <= 9) {
+    const int i = static_cast<int>(k);
+    float t = 0.08106146679532726f;
+    t = i == 1 ? 0.0413406959554093f : t;
+    t = i == 2 ? 0.02767792568499834f : t;
+    t = i == 3 ? 0.020790672103765093f : t;
+    t = i == 4 ? 0.016644691189821193f : t;
+    t = i == 5 ? 0.013876128823070748f : t;
+    t = i == 6 ? 0.01189670994589177f : t;
+    t = i == 7 ? 0.010411265261972096f : t;
+    t = i == 8 ? 0.009255462182712733f : t;
+    t = i == 9 ? 0.00833056343336287f : t;
+    return t;
+  }
+  const float r = f_rcp(k + 1);
+  const float r2 = r * r;
+  return (0.083333333f - (0.0027777778f - r2 * 0.00079365079f) * r2) * r;
+}
+
 // Inversion, n*p < 10, p <= 1/2.
 __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
   const float q = 1 - p;
   const float r = p * f_rcp(q);
   const float g = r * (n + 1);
-  const float f0 = __expf(n * log1pf(-p));
+  const float f0 = __expf(n * f_log1p(-p));
   while (true) {
     float u = random_real<float>(rng);
     float f = f0;
@@ -241,8 +280,9 @@ __device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
   const float v_r = 0.92f - 4.2f * rb;
   const float two_a = 2 * a;
   while (true) {
-    const float u = random_real<float>(rng) - 0.5f;
-    const float v = random_real<float>(rng);
+    float u, v;
+    random_pair_f32(rng, u, v);
+    u -= 0.5f;
     const float us = 0.5f - fabsf(u);
     const float k = floorf((two_a * f_rcp(us) + b) * u + c);
     if (us >= 0.07f && v <= v_r) {
@@ -257,9 +297,10 @@ __device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
     const float rm = f_rcp(m + 1);
     const float rnm = f_rcp(n - m + 1);
     const float L = f_log(p * (n - m + 1) * f_rcp(q * (m + 1)));
-    const float ratio = d * L - (k + 0.5f) * log1pf(d * rm) - (n - k + 0.5f) * log1pf(-d * rnm) +
-                        stirling_tail(m) + stirling_tail(n - m) - stirling_tail(k) -
-                        stirling_tail(n - k);
+    const float ratio = d * L - (k + 0.5f) * f_log1p(d * rm) -
+                        (n - k + 0.5f) * f_log1p(-d * rnm) + stirling_tail_f32(m) +
+                        stirling_tail_f32(n - m) - stirling_tail_f32(k) -
+                        stirling_tail_f32(n - k);
     const float lv = f_log(v * alpha * f_rcp(a * f_rcp(us * us) + b));
     if (lv <= ratio) {
       return k;

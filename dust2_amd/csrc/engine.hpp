@@ -77,6 +77,48 @@ int sir_filter_max_particles(bool fp64);
 launch_config sir_filter_default_config(int n_particles, bool fp64);
 bool sir_filter_config_valid(launch_config cfg, bool fp64);
 
+// ---- interval pipeline (state in HBM between intervals) -------------------
+enum pipe_mode : uint32_t {
+  PIPE_INITIAL = 1u,    // start from the model's initial state
+  PIPE_GATHER = 2u,     // load from the ancestor slot
+  PIPE_STEP = 4u,       // run the interval's steps and the compare draw
+  PIPE_TRAJ_PREV = 8u,  // save the loaded (post-reorder) state of interval t-1
+};
+
+// One parameter group's buffers for the pipeline kernels; one slot per
+// group, the same table for every launch of a run.
+struct pipe_slot {
+  const double* pars;          // model parameter block
+  const double* obs;           // obs[t * obs_stride]
+  const double* lfact;
+  uint32_t obs_stride;
+  uint32_t n_particles;
+  uint32_t flags;              // SLOT_DETERMINISTIC
+  uint32_t group;
+  const double* sched_time;    // system time at the start of interval t
+  const uint32_t* sched_steps; // steps in interval t
+  const uint64_t* sched_zero;  // bit s: zero_every applies at step s (s < 64)
+  double dt;
+  void* state_a;               // real [S][P], double buffered
+  void* state_b;
+  uint64_t* rng;               // [4][P + 2]
+  double* lambda;              // [P] Poisson means of the last interval
+  uint32_t* ancestors;         // [P] ancestors chosen by the last resample
+  double* ll;
+  uint32_t* ancestry;          // [n_time][P] or null
+  void* traj;                  // real [n_time][S][P] or null
+  error_record* errors;        // [P]
+  uint32_t* error_count;
+  uint32_t* n_fallback;
+};
+
+int pipe_resample_ppt(int max_particles);
+hipError_t launch_pipe_reset(const pipe_slot* d_slots, int n_slots, hipStream_t stream);
+hipError_t launch_sir_advance(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
+                              int parity, uint32_t mode, bool fp64, hipStream_t stream);
+hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
+                                hipStream_t stream);
+
 // ---- ODE unfilter ----------------------------------------------------------
 // dust2/continuous/control.hpp:11-24 (step_size_min already clamped to
 // machine epsilon, dust2/r/continuous/control.hpp:15-17)

@@ -273,8 +273,8 @@ launch_config sir_filter_default_config(int n_particles, bool) {
   if (n_particles <= 128) return {64, 2};
   if (n_particles <= 256) return {64, 4};
   if (n_particles <= 512) return {128, 4};
-  if (n_particles <= 1024) return {256, 4};
-  if (n_particles <= 2048) return {512, 4};
+  if (n_particles <= 1024) return {512, 2};
+  if (n_particles <= 2048) return {1024, 2};
   if (n_particles <= 4096) return {1024, 4};
   return {1024, 8};
 }

@@ -160,6 +160,7 @@ class Filter:
         self.precision = _precision(precision)
         self.ptr = None
         self.launch = None
+        self.engine = None
 
     @property
     def initialised(self):
@@ -176,11 +177,22 @@ class Filter:
         self.ptr = out
         if self.launch is not None:
             call("dust2_filter_sir_set_launch", self.ptr, *self.launch)
+        if self.engine is not None:
+            call("dust2_filter_sir_set_engine", self.ptr, *self.engine)
 
     def set_launch(self, block, particles_per_thread):
         self.launch = (int(block), int(particles_per_thread))
         if self.ptr is not None:
             call("dust2_filter_sir_set_launch", self.ptr, *self.launch)
+
+    ENGINES = {"auto": 0, "persistent": 1, "pipeline": 2}
+
+    def set_engine(self, engine="auto", graph=True, profile=False):
+        """Choose the filter engine ('auto', 'persistent', 'pipeline'),
+        HIP-graph replay of the pipeline and per-kernel profiling."""
+        self.engine = (self.ENGINES[engine], int(bool(graph)), int(bool(profile)))
+        if self.ptr is not None:
+            call("dust2_filter_sir_set_engine", self.ptr, *self.engine)
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and _lib._lib is not None:
@@ -252,9 +264,12 @@ def dust_likelihood_set_rng_state(obj, state):
 
 
 def dust_likelihood_last_stats(obj):
-    out = np.zeros(2)
+    out = np.zeros(5)
     call("dust2_filter_sir_last_stats", obj.ptr, dptr(out))
-    return {"n_fallback": int(out[0]), "kernel_ms": float(out[1])}
+    engines = {1: "persistent", 2: "pipeline"}
+    return {"n_fallback": int(out[0]), "kernel_ms": float(out[1]),
+            "engine": engines.get(int(out[2]), "none"), "advance_ms": float(out[3]),
+            "resample_ms": float(out[4])}
 
 
 # ---------------------------------------------------------------------------

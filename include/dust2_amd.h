@@ -38,6 +38,16 @@ extern "C" {
 #define DUST2_FP64 0
 #define DUST2_FP32 1
 
+/* Filter engines (results are identical; only speed differs):
+ *   PERSISTENT: one launch per run, each group's particles held in one
+ *               workgroup's registers for every interval;
+ *   PIPELINE:   per interval an advance launch (one thread per particle,
+ *               state in HBM) and a resample launch (one workgroup per
+ *               group), optionally replayed from a captured HIP graph. */
+#define DUST2_ENGINE_AUTO 0
+#define DUST2_ENGINE_PERSISTENT 1
+#define DUST2_ENGINE_PIPELINE 2
+
 typedef struct dust2_filter dust2_filter;
 typedef struct dust2_system dust2_system;
 typedef struct dust2_unfilter dust2_unfilter;
@@ -101,9 +111,17 @@ int dust2_filter_sir_set_rng_state(dust2_filter* h, const uint64_t* state, char*
 int dust2_filter_sir_set_launch(dust2_filter* h, int block, int particles_per_thread,
                                 char* err, size_t errlen);
 
-/* Statistics of the last run: [0] resampling rounds that needed the
- * sequential sum, [1] kernel milliseconds (device events). */
+/* Statistics of the last run (out[5]): [0] resampling rounds that needed the
+ * sequential sum, [1] GPU milliseconds of the run (device events), [2] the
+ * engine used, [3] / [4] mean advance / resample kernel milliseconds per
+ * interval (pipeline engine with profiling on). */
 int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
+
+/* Engine selection (DUST2_ENGINE_*), HIP-graph replay of the pipeline
+ * (default on) and per-kernel event profiling (default off, disables the
+ * graph).  No counterpart in the reference (its one engine is the CPU). */
+int dust2_filter_sir_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
+                                char* err, size_t errlen);
 
 void dust2_filter_sir_free(dust2_filter* h);
 

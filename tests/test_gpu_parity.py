@@ -119,23 +119,28 @@ def test_resample_bit_exact(gpu_lib):
 
 # ---------------------------------------------------------------------------
 def run_gpu_filter(lib, pars, P, time, dt, data, seed, precision="fp64", launch=None,
-                   save=False, n_groups=None):
+                   save=False, engine="auto", graph=True):
     G = len(pars)
     obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
                                  n_groups=G, dt=dt, seed=seed, precision=precision)
     if launch:
         obj.set_launch(*launch)
+    obj.set_engine(engine, graph=graph)
     plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
     ll = lib.dust_likelihood_run(obj, plist, save_trajectories=save)
     return obj, np.atleast_1d(ll)
 
 
+ENGINES = ["persistent", "pipeline"]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("G,P,n_time", [(1, 1000, 100), (3, 100, 30), (2, 257, 40)])
-def test_filter_fp64_matches_oracle(gpu_lib, G, P, n_time):
+def test_filter_fp64_matches_oracle(gpu_lib, G, P, n_time, engine):
     pars = np.array([[1000, 10, 0.2 + 0.05 * g, 0.1, 1e6] for g in range(G)])
     time = np.arange(1.0, n_time + 1)
     data = np.stack([sir_data(n_time, seed=42 + g) for g in range(G)], axis=1)
-    obj, ll = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 42, save=True)
+    obj, ll = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 42, save=True, engine=engine)
     ref = ol.sir_filter(pars, P, 0.0, time, 0.25, data, seed=42, ancestry=True, n_threads=4)
     np.testing.assert_allclose(ll, ref["ll"], rtol=1e-12)
     state = gpu_lib.dust_likelihood_last_state(obj)
@@ -144,6 +149,42 @@ def test_filter_fp64_matches_oracle(gpu_lib, G, P, n_time):
     assert np.array_equal(anc, ref["ancestry"])
     rng = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
     assert np.array_equal(rng, ref["rng"])
+    traj = gpu_lib.dust_likelihood_last_trajectories(obj)
+    assert np.array_equal(traj[-1], ref["state"])
+
+
+def test_pipeline_large_groups_match_oracle(gpu_lib):
+    # more particles than one workgroup holds in registers
+    pars = np.array([[5e5, 10, 0.4, 0.1, 1e6]])
+    time = np.arange(1.0, 11.0)
+    data = sir_data(10)
+    obj, ll = run_gpu_filter(gpu_lib, pars, 10000, time, 0.25, data, 8, engine="pipeline")
+    ref = ol.sir_filter(pars, 10000, 0.0, time, 0.25, data[:, None], seed=8, n_threads=8)
+    np.testing.assert_allclose(ll, ref["ll"], rtol=1e-12)
+    assert np.array_equal(gpu_lib.dust_likelihood_last_state(obj), ref["state"])
+
+
+def test_pipeline_graph_replays_match_eager(gpu_lib):
+    G, P = 4, 300
+    pars = np.array([[2000, 10, 0.25 + 0.05 * g, 0.1, 1e6] for g in range(G)])
+    time = np.arange(1.0, 31.0)
+    data = sir_data(30)
+    eager, ll_eager = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 13, engine="pipeline",
+                                     graph=False)
+    rng0 = ol.prng(G * (P + 2), seed=13).reshape(-1)
+    obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                     n_groups=G, dt=0.25, seed=13)
+    obj.set_engine("pipeline", graph=True)
+    plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
+    for rep in range(4):
+        if rep:
+            gpu_lib.dust_likelihood_set_rng_state(obj, rng0)
+        # eager work on the device between replays
+        gpu_lib.gpu_resample(np.random.default_rng(rep).normal(size=100), 0.5)
+        ll = gpu_lib.dust_likelihood_run(obj, plist)
+        assert np.array_equal(ll, ll_eager), rep
+        assert np.array_equal(gpu_lib.dust_likelihood_last_state(obj),
+                              gpu_lib.dust_likelihood_last_state(eager))
 
 
 def test_filter_launch_shape_invariance(gpu_lib):
@@ -159,32 +200,35 @@ def test_filter_launch_shape_invariance(gpu_lib):
         assert np.array_equal(st, results[0][1])
 
 
-def test_filter_missing_data_and_index_group(gpu_lib):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_filter_missing_data_and_index_group(gpu_lib, engine):
     G, P = 3, 64
     pars = np.array([[1000, 10, 0.15 + 0.05 * g, 0.1, 1e6] for g in range(G)])
     time = np.arange(1.0, 26.0)
     data = np.stack([sir_data(25, seed=g) for g in range(G)], axis=1)
     data[[2, 3, 17], :] = np.nan
     data[9, 1] = np.nan
-    obj, ll = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 99)
+    obj, ll = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 99, engine=engine)
     ref = ol.sir_filter(pars, P, 0.0, time, 0.25, data, seed=99)
     np.testing.assert_allclose(ll, ref["ll"], rtol=1e-12)
     # a subset of groups: selected groups reproduce the full run
     st = gpu_lib.dust_likelihood_rng_state(obj)
     obj2 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data}, P,
                                       n_groups=G, dt=0.25, seed=99)
+    obj2.set_engine(engine)
     plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
     ll_sub = gpu_lib.dust_likelihood_run(obj2, plist, index_group=[2, 0])
     np.testing.assert_allclose(ll_sub, ll[[2, 0]], rtol=1e-12)
     del st
 
 
-def test_filter_errors_reported(gpu_lib):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_filter_errors_reported(gpu_lib, engine):
     pars = np.array([[1000, 10, 0.1, -0.2, 1e6]])
     time = np.arange(1.0, 4.0)
     data = np.array([1.0, 2.0, 3.0])
     with pytest.raises(gpu_lib.DustError) as e:
-        run_gpu_filter(gpu_lib, pars, 10, time, 1.0, data, 42)
+        run_gpu_filter(gpu_lib, pars, 10, time, 1.0, data, 42, engine=engine)
     msg = str(e.value)
     assert msg.startswith("10 particles reported errors.")
     assert "  - 1: Invalid call to binomial with" in msg
@@ -197,7 +241,8 @@ def test_filter_fp32_statistically_matches_fp64_oracle(gpu_lib):
     time = np.arange(1.0, n_time + 1)
     data = sir_data(n_time)
     pars = np.array([SIR] * G)
-    _, ll32 = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 1, precision="fp32")
+    _, ll32 = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 1, precision="fp32",
+                             engine="pipeline")
     ref = ol.sir_filter(pars[:100], P, 0.0, time, 0.25, data[:, None], seed=2, n_threads=8)
     ll64 = ref["ll"]
     se = np.sqrt(ll32.var(ddof=1) / ll32.size + ll64.var(ddof=1) / ll64.size)
