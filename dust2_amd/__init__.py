@@ -34,6 +34,7 @@ from .interface import (  # noqa: F401
     gpu_resample,
     gpu_rng_streams,
     gpu_sample,
+    seir16,
     sir,
     sirode,
 )

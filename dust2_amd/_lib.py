@@ -28,17 +28,21 @@ _PROTOS = {
     "dust2_filter_sir_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D, C.c_size_t,
                                          C.c_double, D, C.c_size_t, U64, C.c_size_t, C.c_uint64,
                                          C.c_int, C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_filter_sir_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
-    "dust2_filter_sir_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
-    "dust2_filter_sir_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_sir_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_sir_last_ancestry": (C.c_int, [VP, U32, ERR, ERRLEN]),
-    "dust2_filter_sir_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
-    "dust2_filter_sir_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
-    "dust2_filter_sir_set_launch": (C.c_int, [VP, C.c_int, C.c_int, ERR, ERRLEN]),
-    "dust2_filter_sir_last_stats": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_sir_set_engine": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, ERR, ERRLEN]),
-    "dust2_filter_sir_free": (None, [VP]),
+    "dust2_filter_seir16_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
+                                            C.c_size_t, C.c_double, D, C.c_size_t, U64,
+                                            C.c_size_t, C.c_uint64, C.c_int, C.POINTER(VP), ERR,
+                                            ERRLEN]),
+    "dust2_filter_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_filter_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
+    "dust2_filter_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_last_ancestry": (C.c_int, [VP, U32, ERR, ERRLEN]),
+    "dust2_filter_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
+    "dust2_filter_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
+    "dust2_filter_set_launch": (C.c_int, [VP, C.c_int, C.c_int, ERR, ERRLEN]),
+    "dust2_filter_last_stats": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_set_engine": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, ERR, ERRLEN]),
+    "dust2_filter_free": (None, [VP]),
     "dust2_system_sir_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, C.c_double, U64,
                                          C.c_size_t, C.c_uint64, C.c_int, C.c_int,
                                          C.POINTER(VP), ERR, ERRLEN]),

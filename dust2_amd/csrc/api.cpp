@@ -342,6 +342,12 @@ void check_sir_pars(const double* row) {
 // Filter
 // ============================================================================
 struct dust2_filter {
+  int model = 0;             // 0: sir, 1: seir16
+  size_t S = 5;              // states per particle
+  size_t n_pars = 5;         // host parameter row length
+  size_t n_data = 1;         // data values per group per time
+  size_t kpars = 8;          // kernel parameter block (doubles)
+  double zero_period = 1;    // zero_every period of the model
   bool fp64 = true;
   size_t G = 0, P = 0, n_time = 0, n_groups_data = 0;
   double time_start = 0, dt = 1;
@@ -380,6 +386,54 @@ struct dust2_filter {
 
 namespace {
 
+// SEIR16 host parameter row: beta, sigma, gamma, exp_noise, I0, N[16],
+// C[16][16] (row-major).  Kernel block (doubles): [0] beta, [1] p_EI,
+// [2] p_IR (both as the FP32 value of 1 - exp(-rate*dt)), [3] exp_noise,
+// [4] I0, [5..20] N, [21 + 16a + b] beta * C[a][b] / N[b].
+constexpr size_t kSeirAge = 16;
+constexpr size_t kSeirPars = 5 + kSeirAge + kSeirAge * kSeirAge;
+constexpr size_t kSeirKpars = 5 + kSeirAge + kSeirAge * kSeirAge + 3;
+
+void seir_kernel_pars(const double* row, double dt, double* out) {
+  std::fill(out, out + kSeirKpars, 0.0);
+  const double beta = row[0];
+  out[0] = beta;
+  out[1] = static_cast<float>(-std::expm1(-row[1] * dt));
+  out[2] = static_cast<float>(-std::expm1(-row[2] * dt));
+  out[3] = row[3];
+  out[4] = row[4];
+  const double* N = row + 5;
+  const double* C = row + 5 + kSeirAge;
+  for (size_t a = 0; a < kSeirAge; ++a) {
+    out[5 + a] = N[a];
+    for (size_t b = 0; b < kSeirAge; ++b) {
+      out[5 + kSeirAge + a * kSeirAge + b] = beta * C[a * kSeirAge + b] / N[b];
+    }
+  }
+}
+
+void model_kernel_pars(const dust2_filter* h, const double* row, double* out) {
+  if (h->model == 1) {
+    seir_kernel_pars(row, h->dt, out);
+  } else {
+    sir_kernel_pars(row, h->dt, out);
+  }
+}
+
+hipError_t launch_advance(const dust2_filter* h, const pipe_slot* slots, int n_slots, int P,
+                          int t, int parity, uint32_t mode, hipStream_t s) {
+  if (h->model == 1) return launch_seir_advance(slots, n_slots, P, t, parity, mode, s);
+  return launch_sir_advance(slots, n_slots, P, t, parity, mode, h->fp64, s);
+}
+
+void check_finite_pars(const double* row, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(row[i])) {
+      throw std::runtime_error("Parameter " + std::to_string(i) + " must be finite");
+    }
+  }
+}
+
 void aos_to_soa(const double* src, void* dst, size_t G, size_t P, size_t S, bool fp64) {
   for (size_t g = 0; g < G; ++g) {
     for (size_t p = 0; p < P; ++p) {
@@ -409,7 +463,7 @@ void soa_to_aos(const void* src, double* dst, size_t G, size_t P, size_t S, bool
 }
 
 void filter_upload_data(dust2_filter* h, const double* data) {
-  const size_t n = h->n_time * h->n_groups_data;
+  const size_t n = h->n_time * h->n_groups_data * h->n_data;
   std::vector<double> lfact(n);
   for (size_t i = 0; i < n; ++i) {
     lfact[i] = std::isnan(data[i]) ? 0.0 : std::lgamma(data[i] + 1);
@@ -441,7 +495,7 @@ void filter_setup_pipeline(dust2_filter* h) {
     ns[t] = static_cast<uint32_t>(n);
     uint64_t m = 0;
     for (size_t s = 0; s < std::min<size_t>(n, 64); ++s) {
-      if (std::fmod(tcur + s * h->dt, 1.0) == 0) m |= uint64_t(1) << s;
+      if (std::fmod(tcur + s * h->dt, h->zero_period) == 0) m |= uint64_t(1) << s;
     }
     zm[t] = m;
     tcur = tcur + n * h->dt;
@@ -452,10 +506,10 @@ void filter_setup_pipeline(dust2_filter* h) {
   to_device(h->d_sched_time, st.data(), T * sizeof(double), h->stream);
   to_device(h->d_sched_steps, ns.data(), T * sizeof(uint32_t), h->stream);
   to_device(h->d_sched_zero, zm.data(), T * sizeof(uint64_t), h->stream);
-  h->d_state_b.alloc(G * kSirState * P * h->real_size());
+  h->d_state_b.alloc(G * h->S * P * h->real_size());
   h->d_lambda.alloc(G * P * sizeof(double));
   h->d_anc.alloc(G * P * sizeof(uint32_t));
-  h->d_kpars.alloc(G * 8 * sizeof(double));
+  h->d_kpars.alloc(G * h->kpars * sizeof(double));
   h->d_pslots.alloc(G * sizeof(pipe_slot));
   h->d_pslots_init.alloc(G * sizeof(pipe_slot));
   hip_check(hipStreamSynchronize(h->stream), "sync");
@@ -465,26 +519,26 @@ pipe_slot make_pipe_slot(dust2_filter* h, size_t g, bool save_traj) {
   const size_t P = h->P;
   const size_t rsz = h->real_size();
   pipe_slot sl{};
-  sl.pars = h->d_kpars.get<double>() + 8 * g;
-  const size_t goff = h->n_groups_data == 1 ? 0 : g;
+  sl.pars = h->d_kpars.get<double>() + h->kpars * g;
+  const size_t goff = (h->n_groups_data == 1 ? 0 : g) * h->n_data;
   sl.obs = h->d_obs.get<double>() + goff;
   sl.lfact = h->d_lfact.get<double>() + goff;
-  sl.obs_stride = static_cast<uint32_t>(h->n_groups_data);
+  sl.obs_stride = static_cast<uint32_t>(h->n_groups_data * h->n_data);
   sl.n_particles = static_cast<uint32_t>(P);
-  sl.flags = 0;
+  sl.flags = h->model == 1 ? SLOT_WEIGHT_IS_LL : 0u;
   sl.group = static_cast<uint32_t>(g);
   sl.sched_time = h->d_sched_time.get<double>();
   sl.sched_steps = h->d_sched_steps.get<uint32_t>();
   sl.sched_zero = h->d_sched_zero.get<uint64_t>();
   sl.dt = h->dt;
-  sl.state_a = h->d_state.get<char>() + g * kSirState * P * rsz;
-  sl.state_b = h->d_state_b.get<char>() + g * kSirState * P * rsz;
+  sl.state_a = h->d_state.get<char>() + g * h->S * P * rsz;
+  sl.state_b = h->d_state_b.get<char>() + g * h->S * P * rsz;
   sl.rng = h->d_rng.get<uint64_t>() + g * 4 * (P + 2);
   sl.lambda = h->d_lambda.get<double>() + g * P;
   sl.ancestors = h->d_anc.get<uint32_t>() + g * P;
   sl.ll = h->d_ll.get<double>() + g;
   sl.ancestry = save_traj ? h->d_ancestry.get<uint32_t>() + g * h->n_time * P : nullptr;
-  sl.traj = save_traj ? h->d_traj.get<char>() + g * h->n_time * kSirState * P * rsz : nullptr;
+  sl.traj = save_traj ? h->d_traj.get<char>() + g * h->n_time * h->S * P * rsz : nullptr;
   sl.errors = h->d_errors.get<error_record>() + g * P;
   sl.error_count = h->d_counters.get<uint32_t>();
   sl.n_fallback = h->d_counters.get<uint32_t>() + 1;
@@ -517,15 +571,15 @@ void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool sa
       mode |= PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u);
     }
     mark();
-    hip_check(launch_sir_advance(slots, n_slots, P, t, parity, mode, h->fp64, s),
+    hip_check(launch_advance(h, slots, n_slots, P, t, parity, mode, s),
               "advance kernel");
     mark();
     parity ^= 1;
     hip_check(launch_pipe_resample(slots, n_slots, P, t, s), "resample kernel");
     mark();
   }
-  hip_check(launch_sir_advance(slots, n_slots, P, T, parity,
-                               PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u), h->fp64, s),
+  hip_check(launch_advance(h, slots, n_slots, P, T, parity,
+                           PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u), s),
             "gather kernel");
 }
 
@@ -536,8 +590,8 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
   filter_setup_pipeline(h);
   hipStream_t s = h->stream;
   const size_t G = h->G;
-  std::vector<double> kp(G * 8);
-  for (size_t g = 0; g < G; ++g) sir_kernel_pars(&h->pars[g * kSirPars], h->dt, &kp[g * 8]);
+  std::vector<double> kp(G * h->kpars);
+  for (size_t g = 0; g < G; ++g) model_kernel_pars(h, &h->pars[g * h->n_pars], &kp[g * h->kpars]);
   to_device(h->d_kpars, kp.data(), kp.size() * sizeof(double), s);
   std::vector<pipe_slot> slots;
   std::vector<char> selected(G, 0);
@@ -605,9 +659,9 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
     }
     if (!rest.empty()) {
       to_device(h->d_pslots_init, rest.data(), rest.size() * sizeof(pipe_slot), s);
-      hip_check(launch_sir_advance(h->d_pslots_init.get<pipe_slot>(),
-                                   static_cast<int>(rest.size()), static_cast<int>(h->P), 0, 1,
-                                   PIPE_INITIAL, h->fp64, s),
+      hip_check(launch_advance(h, h->d_pslots_init.get<pipe_slot>(),
+                               static_cast<int>(rest.size()), static_cast<int>(h->P), 0, 1,
+                               PIPE_INITIAL, s),
                 "initial kernel");
       hip_check(hipStreamSynchronize(s), "sync");
     }
@@ -632,11 +686,11 @@ int dust2_amd_init(char* err, size_t errlen) {
 
 const char* dust2_amd_version(void) { return "0.1.0"; }
 
-int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
-                           double time_start, const double* time, size_t n_time, double dt,
-                           const double* data, size_t n_groups_data,
-                           const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
-                           int precision, dust2_filter** out, char* err, size_t errlen) {
+static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
+                             size_t n_particles, double time_start, const double* time,
+                             size_t n_time, double dt, const double* data, size_t n_groups_data,
+                             const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                             int precision, dust2_filter** out, char* err, size_t errlen) {
   std::unique_ptr<dust2_filter> h;
   try {
     check_sizes(n_groups, n_particles);
@@ -649,8 +703,22 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
     if (precision != DUST2_FP64 && precision != DUST2_FP32) {
       throw std::runtime_error("'precision' must be DUST2_FP64 or DUST2_FP32");
     }
-    for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * kSirPars);
     h = std::make_unique<dust2_filter>();
+    h->model = model;
+    if (model == 1) {
+      if (precision != DUST2_FP32) {
+        throw std::runtime_error("the seir16 model runs in DUST2_FP32 only");
+      }
+      h->S = 5 * kSeirAge;
+      h->n_pars = kSeirPars;
+      h->n_data = kSeirAge;
+      h->kpars = kSeirKpars;
+      h->zero_period = 7;
+      h->engine = DUST2_ENGINE_PIPELINE;
+      for (size_t g = 0; g < n_groups; ++g) check_finite_pars(pars + g * h->n_pars, h->n_pars);
+    } else {
+      for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * h->n_pars);
+    }
     h->fp64 = precision == DUST2_FP64;
     h->G = n_groups;
     h->P = n_particles;
@@ -658,7 +726,7 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
     h->n_groups_data = n_groups_data;
     h->time_start = time_start;
     h->dt = dt;
-    h->pars.assign(pars, pars + n_groups * kSirPars);
+    h->pars.assign(pars, pars + n_groups * h->n_pars);
     h->times.assign(time, time + n_time);
     const size_t lds = (kScratchDoubles + n_particles) * sizeof(double);
     if (lds > static_cast<size_t>(max_lds_per_block()) ||
@@ -674,7 +742,7 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
     h->d_times.alloc(n_time * sizeof(double));
     to_device(h->d_times, time, n_time * sizeof(double), h->stream);
     filter_upload_data(h.get(), data);
-    h->d_state.alloc(G * kSirState * P * h->real_size());
+    h->d_state.alloc(G * h->S * P * h->real_size());
     h->d_rng.alloc(G * 4 * (P + 2) * sizeof(uint64_t));
     {
       dbuf tmp(G * (P + 2) * 32);
@@ -702,7 +770,27 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
   }
 }
 
-int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajectories,
+int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                           double time_start, const double* time, size_t n_time, double dt,
+                           const double* data, size_t n_groups_data,
+                           const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                           int precision, dust2_filter** out, char* err, size_t errlen) {
+  return filter_alloc_impl(0, pars, n_groups, n_particles, time_start, time, n_time, dt, data,
+                           n_groups_data, seed_state, n_seed_streams, seed, precision, out, err,
+                           errlen);
+}
+
+int dust2_filter_seir16_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                              double time_start, const double* time, size_t n_time, double dt,
+                              const double* data, size_t n_groups_data,
+                              const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                              int precision, dust2_filter** out, char* err, size_t errlen) {
+  return filter_alloc_impl(1, pars, n_groups, n_particles, time_start, time, n_time, dt, data,
+                           n_groups_data, seed_state, n_seed_streams, seed, precision, out, err,
+                           errlen);
+}
+
+int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectories,
                          const size_t* index_group, size_t n_index, double* ll_out, char* err,
                          size_t errlen) {
   try {
@@ -711,12 +799,12 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
     const size_t G = h->G, P = h->P;
     hipStream_t s = h->stream;
     if (initial) {
-      std::vector<char> tmp(G * kSirState * P * h->real_size());
-      aos_to_soa(initial, tmp.data(), G, P, kSirState, h->fp64);
+      std::vector<char> tmp(G * h->S * P * h->real_size());
+      aos_to_soa(initial, tmp.data(), G, P, h->S, h->fp64);
       to_device(h->d_state, tmp.data(), tmp.size(), s);
     }
     if (save_trajectories && h->d_traj.bytes() == 0) {
-      h->d_traj.alloc(h->n_time * G * kSirState * P * h->real_size());
+      h->d_traj.alloc(h->n_time * G * h->S * P * h->real_size());
     }
     if (save_trajectories && h->d_ancestry.bytes() == 0) {
       h->d_ancestry.alloc(h->n_time * G * P * sizeof(uint32_t));
@@ -728,8 +816,9 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
 
     int engine = h->engine;
     if (engine == DUST2_ENGINE_AUTO) {
-      engine = static_cast<int>(P) > sir_filter_max_particles(h->fp64) ? DUST2_ENGINE_PIPELINE
-                                                                       : DUST2_ENGINE_PERSISTENT;
+      engine = (h->model != 0 || static_cast<int>(P) > sir_filter_max_particles(h->fp64))
+                   ? DUST2_ENGINE_PIPELINE
+                   : DUST2_ENGINE_PERSISTENT;
     }
     h->last_engine = engine;
     std::vector<filter_slot> slots;
@@ -743,7 +832,7 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
       // groups outside index_group are reset (filter.hpp:248-250) but not run
       if (!selected[g] && initial) continue;
       filter_slot sl{};
-      sir_kernel_pars(&h->pars[g * kSirPars], h->dt, sl.pars);
+      sir_kernel_pars(&h->pars[g * h->n_pars], h->dt, sl.pars);
       sl.times = h->d_times.get<double>();
       const size_t goff = h->n_groups_data == 1 ? 0 : g;
       sl.obs = h->d_obs.get<double>() + goff;
@@ -755,14 +844,14 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
       sl.flags = (initial ? 0u : SLOT_SET_INITIAL) | (save_trajectories ? SLOT_SAVE_TRAJ : 0u);
       sl.group = static_cast<uint32_t>(g);
       sl.dt = h->dt;
-      sl.state = h->d_state.get<char>() + g * kSirState * P * h->real_size();
+      sl.state = h->d_state.get<char>() + g * h->S * P * h->real_size();
       sl.rng = h->d_rng.get<uint64_t>() + g * 4 * (P + 2);
       sl.ll = h->d_ll.get<double>() + g;
       sl.time = h->d_time.get<double>() + g;
       sl.ancestry = save_trajectories ? h->d_ancestry.get<uint32_t>() + g * h->n_time * P
                                       : nullptr;
       sl.traj = save_trajectories
-                    ? h->d_traj.get<char>() + g * h->n_time * kSirState * P * h->real_size()
+                    ? h->d_traj.get<char>() + g * h->n_time * h->S * P * h->real_size()
                     : nullptr;
       sl.errors = h->d_errors.get<error_record>() + g * P;
       sl.error_count = h->d_counters.get<uint32_t>();
@@ -800,19 +889,29 @@ int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajec
   }
 }
 
-int dust2_filter_sir_update_pars(dust2_filter* h, const double* pars, const size_t* index_group,
+int dust2_filter_update_pars(dust2_filter* h, const double* pars, const size_t* index_group,
                                  size_t n_index, char* err, size_t errlen) {
   try {
     check_filter(h);
     const auto idx = check_index(index_group, n_index, h->G);
     for (size_t i = 0; i < idx.size(); ++i) {
-      const double* row = pars + i * kSirPars;
-      check_sir_pars(row);
-      double* dst = &h->pars[idx[i] * kSirPars];
-      // I0, beta, gamma are updatable; N and exp_noise are constant
-      dst[1] = row[1];
-      dst[2] = row[2];
-      dst[3] = row[3];
+      const double* row = pars + i * h->n_pars;
+      double* dst = &h->pars[idx[i] * h->n_pars];
+      if (h->model == 1) {
+        // seir16: beta, sigma, gamma, I0 are updatable; N, C, exp_noise constant
+        check_finite_pars(row, h->n_pars);
+        dst[0] = row[0];
+        dst[1] = row[1];
+        dst[2] = row[2];
+        dst[4] = row[4];
+      } else {
+        // sir: I0, beta, gamma are updatable; N and exp_noise are constant
+        // (inst/examples/sir.cpp:89-93)
+        check_sir_pars(row);
+        dst[1] = row[1];
+        dst[2] = row[2];
+        dst[3] = row[3];
+      }
     }
     return 0;
   } catch (const std::exception& e) {
@@ -820,19 +919,19 @@ int dust2_filter_sir_update_pars(dust2_filter* h, const double* pars, const size
   }
 }
 
-int dust2_filter_sir_last_state(dust2_filter* h, double* out, char* err, size_t errlen) {
+int dust2_filter_last_state(dust2_filter* h, double* out, char* err, size_t errlen) {
   try {
     check_filter(h);
     std::vector<char> tmp(h->d_state.bytes());
     to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
-    soa_to_aos(tmp.data(), out, h->G, h->P, kSirState, h->fp64);
+    soa_to_aos(tmp.data(), out, h->G, h->P, h->S, h->fp64);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
   }
 }
 
-int dust2_filter_sir_last_trajectories(dust2_filter* h, double* out, char* err,
+int dust2_filter_last_trajectories(dust2_filter* h, double* out, char* err,
                                        size_t errlen) {
   try {
     check_filter(h);
@@ -845,10 +944,10 @@ int dust2_filter_sir_last_trajectories(dust2_filter* h, double* out, char* err,
     to_host(tmp.data(), h->d_traj, tmp.size(), h->stream);
     for (size_t g = 0; g < G; ++g) {
       for (size_t t = 0; t < T; ++t) {
-        for (size_t c = 0; c < kSirState; ++c) {
+        for (size_t c = 0; c < h->S; ++c) {
           for (size_t p = 0; p < P; ++p) {
-            const size_t o = ((g * T + t) * kSirState + c) * P + p;
-            out[((t * G + g) * P + p) * kSirState + c] =
+            const size_t o = ((g * T + t) * h->S + c) * P + p;
+            out[((t * G + g) * P + p) * h->S + c] =
                 h->fp64 ? reinterpret_cast<const double*>(tmp.data())[o]
                         : reinterpret_cast<const float*>(tmp.data())[o];
           }
@@ -861,7 +960,7 @@ int dust2_filter_sir_last_trajectories(dust2_filter* h, double* out, char* err,
   }
 }
 
-int dust2_filter_sir_last_ancestry(dust2_filter* h, uint32_t* out, char* err, size_t errlen) {
+int dust2_filter_last_ancestry(dust2_filter* h, uint32_t* out, char* err, size_t errlen) {
   try {
     check_filter(h);
     if (!h->ancestry_valid) {
@@ -881,7 +980,7 @@ int dust2_filter_sir_last_ancestry(dust2_filter* h, uint32_t* out, char* err, si
   }
 }
 
-int dust2_filter_sir_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t errlen) {
+int dust2_filter_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t errlen) {
   try {
     check_filter(h);
     const size_t n = h->G * (h->P + 2);
@@ -897,7 +996,7 @@ int dust2_filter_sir_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t
   }
 }
 
-int dust2_filter_sir_set_rng_state(dust2_filter* h, const uint64_t* state, char* err,
+int dust2_filter_set_rng_state(dust2_filter* h, const uint64_t* state, char* err,
                                    size_t errlen) {
   try {
     check_filter(h);
@@ -915,7 +1014,7 @@ int dust2_filter_sir_set_rng_state(dust2_filter* h, const uint64_t* state, char*
   }
 }
 
-int dust2_filter_sir_set_launch(dust2_filter* h, int block, int ppt, char* err, size_t errlen) {
+int dust2_filter_set_launch(dust2_filter* h, int block, int ppt, char* err, size_t errlen) {
   try {
     check_filter(h);
     const launch_config c{block, ppt};
@@ -932,7 +1031,7 @@ int dust2_filter_sir_set_launch(dust2_filter* h, int block, int ppt, char* err, 
   }
 }
 
-int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t errlen) {
+int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errlen) {
   try {
     check_filter(h);
     out[0] = h->last_fallbacks;
@@ -946,13 +1045,16 @@ int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t 
   }
 }
 
-int dust2_filter_sir_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
+int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
                                 char* err, size_t errlen) {
   try {
     check_filter(h);
     if (engine != DUST2_ENGINE_AUTO && engine != DUST2_ENGINE_PERSISTENT &&
         engine != DUST2_ENGINE_PIPELINE) {
       throw std::runtime_error("unknown engine");
+    }
+    if (engine == DUST2_ENGINE_PERSISTENT && h->model != 0) {
+      throw std::runtime_error("the persistent engine runs the sir model only");
     }
     if (engine == DUST2_ENGINE_PERSISTENT &&
         static_cast<int>(h->P) > sir_filter_max_particles(h->fp64)) {
@@ -969,7 +1071,7 @@ int dust2_filter_sir_set_engine(dust2_filter* h, int engine, int use_graph, int 
   }
 }
 
-void dust2_filter_sir_free(dust2_filter* h) { delete h; }
+void dust2_filter_free(dust2_filter* h) { delete h; }
 
 }  // extern "C"
 

@@ -27,6 +27,7 @@ enum slot_flags : uint32_t {
   SLOT_SET_INITIAL = 1u,    // start from the model's initial state
   SLOT_DETERMINISTIC = 2u,  // samplers return their means
   SLOT_SAVE_TRAJ = 4u,      // write the post-reorder state at every time
+  SLOT_WEIGHT_IS_LL = 8u,   // pipeline: lambda[] already holds the log-weight
 };
 
 // Per-particle error record (first error of a particle).
@@ -118,6 +119,9 @@ hipError_t launch_sir_advance(const pipe_slot* d_slots, int n_slots, int max_par
                               int parity, uint32_t mode, bool fp64, hipStream_t stream);
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
                                 hipStream_t stream);
+// Age-structured SEIR (80 states, FP32), force of infection on MFMA.
+hipError_t launch_seir_advance(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
+                               int parity, uint32_t mode, hipStream_t stream);
 
 // ---- ODE unfilter ----------------------------------------------------------
 // dust2/continuous/control.hpp:11-24 (step_size_min already clamped to

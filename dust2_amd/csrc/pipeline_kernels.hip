@@ -126,11 +126,18 @@ __global__ void __launch_bounds__(BLOCK) resample_kernel(const pipe_slot* __rest
   const int p0 = tid * PPT;
   const double x = sd.obs[static_cast<size_t>(t) * sd.obs_stride];
   const double lf = sd.lfact[static_cast<size_t>(t) * sd.obs_stride];
+  const bool is_ll = (sd.flags & SLOT_WEIGHT_IS_LL) != 0;
   double w[PPT];
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     w[k] = 0;
-    if (p0 + k < P && !isnan(x)) w[k] = dpois_log(x, sd.lambda[p0 + k], lf);
+    if (p0 + k < P) {
+      if (is_ll) {
+        w[k] = sd.lambda[p0 + k];
+      } else if (!isnan(x)) {
+        w[k] = dpois_log(x, sd.lambda[p0 + k], lf);
+      }
+    }
   }
   const int rs = P + 2;
   rng256 frng{0, 0, 0, 0};

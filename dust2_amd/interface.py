@@ -22,7 +22,8 @@ class Generator:
     """A model class compiled into the library (the analogue of an odin2 /
     dust2 generator, inst/template/dust.R:1-12)."""
 
-    def __init__(self, name, time_type, parameters, defaults, constant, state, n_data=1):
+    def __init__(self, name, time_type, parameters, defaults, constant, state, n_data=1,
+                 sizes=None, precisions=("fp64", "fp32")):
         self.name = name
         self.time_type = time_type
         self.parameters = parameters
@@ -30,6 +31,15 @@ class Generator:
         self.constant = constant
         self.state = state
         self.n_data = n_data
+        self.sizes = sizes or {}
+        self.precisions = precisions
+
+    def size(self, name):
+        return self.sizes.get(name, 1)
+
+    @property
+    def n_pars(self):
+        return sum(self.size(k) for k in self.parameters)
 
     @property
     def n_state(self):
@@ -55,6 +65,23 @@ def sirode():
         {"N"}, ["S", "I", "R", "cases_cumul", "cases_inc"])
 
 
+def seir16():
+    """Age-structured stochastic SEIR, 16 age groups (no counterpart in the
+    reference; written to the same model contract).  N: population of each
+    age group (16); C: 16 x 16 contact matrix; I0 infectious seeded in every
+    age group; weekly incidence reset, Poisson observation of each age
+    group's incidence.  FP32 only."""
+    ages = range(16)
+    state = ([f"S{a}" for a in ages] + [f"E{a}" for a in ages] + [f"I{a}" for a in ages]
+             + [f"R{a}" for a in ages] + [f"inc{a}" for a in ages])
+    return Generator(
+        "seir16", "discrete", ["beta", "sigma", "gamma", "exp_noise", "I0", "N", "C"],
+        {"beta": 0.2, "sigma": 0.25, "gamma": 0.1, "exp_noise": 1e6, "I0": 10.0,
+         "N": np.full(16, 1e5), "C": np.full((16, 16), 1.0 / 16)},
+        {"N", "C", "exp_noise"}, state, n_data=16, sizes={"N": 16, "C": 256},
+        precisions=("fp32",))
+
+
 def _precision(p):
     if p in ("fp64", "double", FP64):
         return FP64
@@ -76,8 +103,23 @@ def pars_matrix(generator, pars, n_groups):
         unknown = set(p) - set(generator.parameters)
         if unknown:
             raise ValueError(f"Unknown parameters: {', '.join(sorted(unknown))}")
-        rows.append([float(p.get(k, generator.defaults[k])) for k in generator.parameters])
+        row = []
+        for k in generator.parameters:
+            v = np.asarray(p.get(k, generator.defaults[k]), dtype=np.float64).reshape(-1)
+            if v.size != generator.size(k):
+                raise ValueError(f"Expected '{k}' to have {generator.size(k)} values")
+            row.extend(v.tolist())
+        rows.append(row)
     return np.ascontiguousarray(rows, dtype=np.float64)
+
+
+def _par_offset(generator, name):
+    off = 0
+    for k in generator.parameters:
+        if k == name:
+            return off
+        off += generator.size(k)
+    raise ValueError(f"Unknown parameters: {name}")
 
 
 def pars_update(generator, pars, current):
@@ -92,12 +134,12 @@ def pars_update(generator, pars, current):
     rows = current.copy()
     for i, p in enumerate(pars):
         for k, v in p.items():
-            if k not in generator.parameters:
-                raise ValueError(f"Unknown parameters: {k}")
-            j = generator.parameters.index(k)
-            if k in generator.constant and float(v) != rows[i, j]:
+            j = _par_offset(generator, k)
+            n = generator.size(k)
+            v = np.asarray(v, dtype=np.float64).reshape(-1)
+            if k in generator.constant and not np.array_equal(v, rows[i, j:j + n]):
                 raise ValueError(f"Parameter '{k}' is constant and cannot be updated")
-            rows[i, j] = float(v)
+            rows[i, j:j + n] = v
     return np.ascontiguousarray(rows)
 
 
@@ -120,23 +162,27 @@ def _index_group(index_group, n_groups):
     return idx, idx.size, idx.size
 
 
-def prepare_data(data, n_groups, name="incidence"):
+def prepare_data(data, n_groups, name="incidence", n_data=1):
     """Data as in R/data.R:114-175: a dict with 'time' (n_time) and the
-    observed column, shape (n_time,) shared or (n_time, n_groups)."""
+    observed values, shape (n_time,) / (n_time, n_data) shared by all groups,
+    or (n_time, n_groups) / (n_time, n_groups, n_data)."""
     time = f64(data["time"]).reshape(-1)
+    nt = time.shape[0]
     obs = f64(data[name])
-    if obs.ndim == 2 and obs.shape[1] == 1:
-        obs = obs[:, 0]
-    if obs.ndim == 1:
-        if obs.shape[0] != time.shape[0]:
-            raise ValueError("Expected data columns to have the same length as 'time'")
-        n_groups_data = 1
-        obs = obs.reshape(-1, 1)
+    if n_data == 1:
+        if obs.ndim == 2 and obs.shape[1] == 1:
+            obs = obs[:, 0]
+        if obs.shape == (nt,):
+            return time, np.ascontiguousarray(obs.reshape(nt, 1)), 1
+        if obs.shape == (nt, n_groups):
+            return time, np.ascontiguousarray(obs), n_groups
     else:
-        if obs.shape != (time.shape[0], n_groups):
-            raise ValueError(f"Expected '{name}' to have shape ({time.shape[0]}, {n_groups})")
-        n_groups_data = n_groups
-    return time, np.ascontiguousarray(obs), n_groups_data
+        if obs.shape == (nt, n_data):
+            return time, np.ascontiguousarray(obs.reshape(nt, 1, n_data)), 1
+        if obs.shape == (nt, n_groups, n_data):
+            return time, np.ascontiguousarray(obs), n_groups
+    raise ValueError(f"Expected '{name}' to have {nt} rows of 1 or {n_groups} groups"
+                     + (f" x {n_data} values" if n_data > 1 else ""))
 
 
 # ---------------------------------------------------------------------------
@@ -152,11 +198,14 @@ class Filter:
             raise ValueError("Can't use 'dust_filter_create()' with continuous-time systems")
         self.generator = generator
         self.time_start = float(time_start)
-        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups)
+        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups,
+                                                                n_data=generator.n_data)
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
         self.dt = float(dt)
         self.seed = seed
+        if precision == "fp64" and "fp64" not in generator.precisions:
+            precision = generator.precisions[0]
         self.precision = _precision(precision)
         self.ptr = None
         self.launch = None
@@ -171,19 +220,20 @@ class Filter:
         self.rows = rows
         raw, n_raw, seed = _seed_args(self.seed)
         out = C.c_void_p()
-        call("dust2_filter_sir_alloc", dptr(rows), self.n_groups, self.n_particles,
+        call(f"dust2_filter_{self.generator.name}_alloc", dptr(rows), self.n_groups,
+             self.n_particles,
              self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),
              self.n_groups_data, u64ptr(raw), n_raw, seed, self.precision, C.byref(out))
         self.ptr = out
         if self.launch is not None:
-            call("dust2_filter_sir_set_launch", self.ptr, *self.launch)
+            call("dust2_filter_set_launch", self.ptr, *self.launch)
         if self.engine is not None:
-            call("dust2_filter_sir_set_engine", self.ptr, *self.engine)
+            call("dust2_filter_set_engine", self.ptr, *self.engine)
 
     def set_launch(self, block, particles_per_thread):
         self.launch = (int(block), int(particles_per_thread))
         if self.ptr is not None:
-            call("dust2_filter_sir_set_launch", self.ptr, *self.launch)
+            call("dust2_filter_set_launch", self.ptr, *self.launch)
 
     ENGINES = {"auto": 0, "persistent": 1, "pipeline": 2}
 
@@ -192,11 +242,11 @@ class Filter:
         HIP-graph replay of the pipeline and per-kernel profiling."""
         self.engine = (self.ENGINES[engine], int(bool(graph)), int(bool(profile)))
         if self.ptr is not None:
-            call("dust2_filter_sir_set_engine", self.ptr, *self.engine)
+            call("dust2_filter_set_engine", self.ptr, *self.engine)
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and _lib._lib is not None:
-            _lib.lib().dust2_filter_sir_free(self.ptr)
+            _lib.lib().dust2_filter_free(self.ptr)
             self.ptr = None
 
 
@@ -218,7 +268,7 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
         idx, n_idx, _ = _index_group(index_group, obj.n_groups)
         sel = np.arange(obj.n_groups) if idx is None else idx.astype(np.int64)
         rows = pars_update(obj.generator, pars, obj.rows[sel])
-        call("dust2_filter_sir_update_pars", obj.ptr, dptr(rows), szptr(idx), n_idx)
+        call("dust2_filter_update_pars", obj.ptr, dptr(rows), szptr(idx), n_idx)
         obj.rows[sel] = rows
     idx, n_idx, n_out = _index_group(index_group, obj.n_groups)
     init = None
@@ -227,32 +277,32 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
         if init.size != obj.n_groups * obj.n_particles * obj.generator.n_state:
             raise ValueError("Expected 'initial' to have n_groups * n_particles * n_state values")
     ll = np.zeros(n_out)
-    call("dust2_filter_sir_run", obj.ptr, dptr(init), int(bool(save_trajectories)), szptr(idx),
+    call("dust2_filter_run", obj.ptr, dptr(init), int(bool(save_trajectories)), szptr(idx),
          n_idx, dptr(ll))
     return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
 
 
 def dust_likelihood_last_state(obj):
     out = np.zeros((obj.n_groups, obj.n_particles, obj.generator.n_state))
-    call("dust2_filter_sir_last_state", obj.ptr, dptr(out))
+    call("dust2_filter_last_state", obj.ptr, dptr(out))
     return out
 
 
 def dust_likelihood_last_trajectories(obj):
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
-    call("dust2_filter_sir_last_trajectories", obj.ptr, dptr(out))
+    call("dust2_filter_last_trajectories", obj.ptr, dptr(out))
     return out
 
 
 def dust_likelihood_last_ancestry(obj):
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles), dtype=np.uint32)
-    call("dust2_filter_sir_last_ancestry", obj.ptr, u32ptr(out))
+    call("dust2_filter_last_ancestry", obj.ptr, u32ptr(out))
     return out
 
 
 def dust_likelihood_rng_state(obj):
     out = np.zeros(obj.n_groups * (obj.n_particles + 2) * 4, dtype=np.uint64)
-    call("dust2_filter_sir_rng_state", obj.ptr, u64ptr(out))
+    call("dust2_filter_rng_state", obj.ptr, u64ptr(out))
     return out
 
 
@@ -260,12 +310,12 @@ def dust_likelihood_set_rng_state(obj, state):
     st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
     if st.size != obj.n_groups * (obj.n_particles + 2) * 4:
         raise ValueError("Incorrect length for 'rng_state'")
-    call("dust2_filter_sir_set_rng_state", obj.ptr, u64ptr(st))
+    call("dust2_filter_set_rng_state", obj.ptr, u64ptr(st))
 
 
 def dust_likelihood_last_stats(obj):
     out = np.zeros(5)
-    call("dust2_filter_sir_last_stats", obj.ptr, dptr(out))
+    call("dust2_filter_last_stats", obj.ptr, dptr(out))
     engines = {1: "persistent", 2: "pipeline"}
     return {"n_fallback": int(out[0]), "kernel_ms": float(out[1]),
             "engine": engines.get(int(out[2]), "none"), "advance_ms": float(out[3]),

@@ -72,58 +72,70 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
                            const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                            int precision, dust2_filter** out, char* err, size_t errlen);
 
+/* Age-structured stochastic SEIR with 16 age groups (no reference model; the
+ * same generated-class contract).  Parameter row: beta, sigma, gamma,
+ * exp_noise, I0, N[16], C[16*16] (row-major contact matrix); data: 16
+ * weekly incidence values per group per time; state: S[16] E[16] I[16]
+ * R[16] inc[16].  DUST2_FP32 only; runs on the pipeline engine with the force
+ * of infection on the matrix cores.  All dust2_filter_* methods apply. */
+int dust2_filter_seir16_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                              double time_start, const double* time, size_t n_time, double dt,
+                              const double* data, size_t n_groups_data,
+                              const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                              int precision, dust2_filter** out, char* err, size_t errlen);
+
 /* Replaces dust2_filter_run<T> (dust2/r/filter.hpp:21-52) / filter<T>::run
  * (dust2/filter.hpp:120-182).  initial: NULL (model initial state) or
  * [G][P][S].  index_group: NULL (all groups) or n_index group indices.
  * ll_out receives one log-likelihood per selected group, in index order. */
-int dust2_filter_sir_run(dust2_filter* h, const double* initial, int save_trajectories,
+int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectories,
                          const size_t* index_group, size_t n_index, double* ll_out,
                          char* err, size_t errlen);
 
 /* Replaces dust2_filter_update_pars<T> (dust2/r/filter.hpp:10-19); sir
  * updates I0, beta and gamma (inst/examples/sir.cpp:89-93); N and
  * exp_noise are constant.  pars: n_index rows of n_pars. */
-int dust2_filter_sir_update_pars(dust2_filter* h, const double* pars,
+int dust2_filter_update_pars(dust2_filter* h, const double* pars,
                                  const size_t* index_group, size_t n_index,
                                  char* err, size_t errlen);
 
 /* Replaces dust2_filter_last_state (dust2/r/filter.hpp): out [G][P][S]. */
-int dust2_filter_sir_last_state(dust2_filter* h, double* out, char* err, size_t errlen);
+int dust2_filter_last_state(dust2_filter* h, double* out, char* err, size_t errlen);
 
 /* Replaces dust2_filter_last_trajectories (dust2/r/filter.hpp:54-116), the
  * raw per-time post-resampling states without ancestry tracing:
  * out [n_time][G][P][S].  Requires save_trajectories on the last run. */
-int dust2_filter_sir_last_trajectories(dust2_filter* h, double* out, char* err,
+int dust2_filter_last_trajectories(dust2_filter* h, double* out, char* err,
                                        size_t errlen);
 
 /* Ancestor indices of the last run: out [n_time][G][P] (0-based). */
-int dust2_filter_sir_last_ancestry(dust2_filter* h, uint32_t* out, char* err, size_t errlen);
+int dust2_filter_last_ancestry(dust2_filter* h, uint32_t* out, char* err, size_t errlen);
 
 /* Replaces dust2_filter_rng_state / set_rng_state (dust2/r/filter.hpp:182-238):
  * G*(P+2) streams, per group [2 filter | P particle]. */
-int dust2_filter_sir_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t errlen);
-int dust2_filter_sir_set_rng_state(dust2_filter* h, const uint64_t* state, char* err,
+int dust2_filter_rng_state(dust2_filter* h, uint64_t* out, char* err, size_t errlen);
+int dust2_filter_set_rng_state(dust2_filter* h, const uint64_t* state, char* err,
                                    size_t errlen);
 
 /* Launch shape of the persistent kernel (threads per workgroup, particles
  * per thread); results do not depend on it.  Chosen automatically unless
  * set; invalid shapes are rejected. */
-int dust2_filter_sir_set_launch(dust2_filter* h, int block, int particles_per_thread,
+int dust2_filter_set_launch(dust2_filter* h, int block, int particles_per_thread,
                                 char* err, size_t errlen);
 
 /* Statistics of the last run (out[5]): [0] resampling rounds that needed the
  * sequential sum, [1] GPU milliseconds of the run (device events), [2] the
  * engine used, [3] / [4] mean advance / resample kernel milliseconds per
  * interval (pipeline engine with profiling on). */
-int dust2_filter_sir_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
+int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
 
 /* Engine selection (DUST2_ENGINE_*), HIP-graph replay of the pipeline
  * (default on) and per-kernel event profiling (default off, disables the
  * graph).  No counterpart in the reference (its one engine is the CPU). */
-int dust2_filter_sir_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
+int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
                                 char* err, size_t errlen);
 
-void dust2_filter_sir_free(dust2_filter* h);
+void dust2_filter_free(dust2_filter* h);
 
 /* ---- discrete-time system (stochastic SIR) --------------------------------
  * Replaces dust2_discrete_alloc<sir> (dust2/r/discrete/system.hpp:12-54) and

@@ -24,7 +24,7 @@ def declared():
 def test_header_declares_entry_points():
     names = declared()
     assert "dust2_filter_sir_alloc" in names
-    assert "dust2_filter_sir_run" in names
+    assert "dust2_filter_run" in names
     assert len(names) > 20
 
 

@@ -1,0 +1,90 @@
+"""GPU checks of the age-structured SEIR16 filter (FP32, force of infection
+on MFMA).  No reference model exists: pinned by the CPU oracle's seir16
+(mean log-likelihood within 3 Monte Carlo standard errors), exact
+invariants (population conservation per age, weekly incidence sums) and a
+contact structure whose transpose would infect a different age group."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+AGES = 16
+
+
+def contact(seed=1):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(0, 2, (AGES, AGES))
+    C = (C + C.T) / 2
+    return C / np.max(np.abs(np.linalg.eigvals(C)))
+
+
+def seir_pars(beta=0.6, sigma=0.3, gamma=0.2, I0=5.0, N=None, C=None):
+    N = np.full(AGES, 2e4) if N is None else N
+    C = contact() if C is None else C
+    return {"beta": beta, "sigma": sigma, "gamma": gamma, "exp_noise": 1e6, "I0": I0,
+            "N": N, "C": C}
+
+
+def row(p):
+    return np.concatenate([[p["beta"], p["sigma"], p["gamma"], p["exp_noise"], p["I0"]],
+                           np.ravel(p["N"]), np.ravel(p["C"])])
+
+
+def run(lib, plist, P, time, data, seed, save=False):
+    obj = lib.dust_filter_create(lib.seir16(), 0.0, {"time": time, "incidence": data}, P,
+                                 n_groups=len(plist), dt=0.25, seed=seed, precision="fp32")
+    ll = lib.dust_likelihood_run(obj, plist, save_trajectories=save)
+    return obj, np.atleast_1d(ll)
+
+
+def test_population_conserved_and_incidence_sums(gpu_lib):
+    time = np.arange(7.0, 7.0 * 9, 7.0)
+    data = np.full((time.size, AGES), np.nan)   # no resampling: lineages intact
+    obj, _ = run(gpu_lib, [seir_pars()], 128, time, data, 3, save=True)
+    st = gpu_lib.dust_likelihood_last_state(obj)[0]          # (P, 80)
+    S, E, I, R = st[:, :16], st[:, 16:32], st[:, 32:48], st[:, 48:64]
+    np.testing.assert_array_equal(S + E + I + R, np.full_like(S, 2e4))
+    tr = gpu_lib.dust_likelihood_last_trajectories(obj)[:, 0]  # (T, P, 80)
+    inc = tr[:, :, 64:80].sum(axis=0)
+    np.testing.assert_array_equal(inc, (2e4 - 5.0) - S)
+    assert np.all(inc > 0)
+
+
+def test_force_of_infection_uses_contact_rows(gpu_lib):
+    # only age 3 is exposed, and only to infection in age 11; a transposed
+    # contraction would expose age 11 instead
+    C = np.zeros((AGES, AGES))
+    C[3, 11] = 1.0
+    time = np.array([7.0, 14.0])
+    data = np.full((2, AGES), np.nan)
+    obj, _ = run(gpu_lib, [seir_pars(beta=2.0, C=C, I0=500.0)], 64, time, data, 5)
+    st = gpu_lib.dust_likelihood_last_state(obj)[0]
+    S = st[:, :16]
+    assert np.all(S[:, 3] < 2e4 - 500.0)
+    others = np.delete(S, 3, axis=1)
+    np.testing.assert_array_equal(others, np.full_like(others, 2e4 - 500.0))
+
+
+def test_mean_log_likelihood_matches_fp64_oracle(gpu_lib):
+    G, P = 64, 256
+    time = np.arange(7.0, 7.0 * 7, 7.0)
+    p = seir_pars()
+    # observations: one oracle simulation of the model
+    sim, _ = ol.sir_simulate(row(p)[None, :], 1, 0.0, 0.25, time, ol.prng(1, seed=9),
+                             model="seir16")
+    data = sim[:, 0, 0, 64:80].copy()
+    _, ll_gpu = run(gpu_lib, [p] * G, P, time, data, 1)
+    ref = ol.sir_filter(np.tile(row(p), (24, 1)), P, 0.0, time, 0.25, data[:, None, :],
+                        seed=2, n_threads=16, model="seir16")
+    ll_cpu = ref["ll"]
+    se = np.sqrt(ll_gpu.var(ddof=1) / G + ll_cpu.var(ddof=1) / ll_cpu.size)
+    assert abs(ll_gpu.mean() - ll_cpu.mean()) < 3 * se + 1e-9, (ll_gpu.mean(), ll_cpu.mean(), se)
+
+
+def test_invalid_rate_reports_errors(gpu_lib):
+    time = np.array([7.0])
+    data = np.ones((1, AGES))
+    with pytest.raises(gpu_lib.DustError, match="particles reported errors"):
+        run(gpu_lib, [seir_pars(gamma=-0.5)], 64, time, data, 1)
