@@ -67,6 +67,14 @@ _PROTOS = {
     "dust2_unfilter_sirode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_stats": (C.c_int, [VP, D, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_free": (None, [VP]),
+    "dust2_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_server_submit": (C.c_int, [VP, C.c_int, D, C.c_size_t, C.c_double, D, C.c_size_t,
+                                      C.c_double, D, C.c_uint64, U64, ERR, ERRLEN]),
+    "dust2_server_step": (C.c_int, [VP, SZ, SZ, ERR, ERRLEN]),
+    "dust2_server_poll": (C.c_int, [VP, U64, D, C.POINTER(C.c_int), C.c_size_t, SZ, ERR,
+                                    ERRLEN]),
+    "dust2_server_memory": (C.c_int, [VP, SZ, SZ, ERR, ERRLEN]),
+    "dust2_server_free": (None, [VP]),
     "dust2_amd_resample": (C.c_int, [D, C.c_size_t, C.c_double, U32, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,
                                         ERRLEN]),

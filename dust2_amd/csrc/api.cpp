@@ -8,6 +8,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <deque>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -1666,5 +1669,526 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
 }
 
 void dust2_unfilter_sirode_free(dust2_unfilter* h) { delete h; }
+
+}  // extern "C"
+
+// ============================================================================
+// Serving: continuous batching of likelihood evaluations
+// ============================================================================
+namespace {
+
+// First-fit allocator over one device allocation (256-byte granules), so
+// admitting and retiring requests never calls hipMalloc/hipFree.
+class arena {
+ public:
+  void init(size_t bytes) {
+    mem_.alloc(bytes);
+    cap_ = bytes;
+    free_.clear();
+    free_[0] = bytes;
+    used_ = 0;
+  }
+  bool alloc(size_t bytes, size_t& off) {
+    bytes = (bytes + 255) & ~size_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        off = it->first;
+        const size_t rest = it->second - bytes;
+        free_.erase(it);
+        if (rest) free_[off + bytes] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  char* base() const { return mem_.get<char>(); }
+  size_t used() const { return used_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  dbuf mem_;
+  size_t cap_ = 0;
+  size_t used_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+struct srv_request {
+  uint64_t id = 0;
+  int model = 0;  // 0 sir, 1 seir16, 2 sirode
+  size_t P = 0;
+  double time_start = 0, dt = 1;
+  std::vector<double> pars, times, data;
+  uint64_t seed = 0;
+};
+
+struct srv_result {
+  uint64_t id;
+  double ll;
+  int status;
+  std::string message;
+};
+
+// Sub-buffer layout of one resident request inside its arena block.
+struct srv_layout {
+  size_t state_a, state_b, rng, lambda, anc, errors, ll, counters, sched_time, sched_steps,
+      sched_zero, obs, lfact, kpars, total;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+srv_layout srv_plan(size_t S, size_t P, size_t T, size_t n_data, size_t kpars) {
+  srv_layout L{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += align256(bytes);
+    return at;
+  };
+  L.state_a = take(S * P * 4);
+  L.state_b = take(S * P * 4);
+  L.rng = take(4 * (P + 2) * 8);
+  L.lambda = take(P * 8);
+  L.anc = take(P * 4);
+  L.errors = take(P * sizeof(error_record));
+  L.ll = take(8);
+  L.counters = take(8);
+  L.sched_time = take(T * 8);
+  L.sched_steps = take(T * 4);
+  L.sched_zero = take(T * 8);
+  L.obs = take(T * n_data * 8);
+  L.lfact = take(T * n_data * 8);
+  L.kpars = take(kpars * 8);
+  L.total = o;
+  return L;
+}
+
+struct srv_slot {
+  srv_request req;
+  size_t off = 0;
+  srv_layout L{};
+  size_t T = 0;
+  size_t done = 0;
+  pipe_slot ps{};
+};
+
+}  // namespace
+
+struct dust2_server {
+  static constexpr int kStreams = 4;
+  hipStream_t streams[kStreams] = {};
+  arena mem;
+  int chunk = 8;
+  uint64_t next_id = 1;
+  std::deque<srv_request> queue;
+  std::vector<std::unique_ptr<srv_slot>> active;
+  std::vector<srv_request> ode_queue;
+  std::vector<srv_result> results;
+  dbuf d_tables, d_status, d_ode_pars, d_ode_series, d_ode_lfact, d_ode_times, d_ode_off, d_ode_len,
+      d_ode_state, d_ode_ll, d_ode_errors, d_ode_counters;
+  size_t max_ode_per_tick = 65536;
+  ~dust2_server() {
+    for (auto s : streams) {
+      if (s) (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+namespace {
+
+size_t model_state(int model) { return model == 1 ? 5 * kSeirAge : 5; }
+size_t model_n_pars(int model) {
+  return model == 1 ? kSeirPars : (model == 2 ? kSirodePars : kSirPars);
+}
+size_t model_n_data(int model) { return model == 1 ? kSeirAge : 1; }
+size_t model_kpars(int model) { return model == 1 ? kSeirKpars : 8; }
+double model_zero_period(int model) { return model == 1 ? 7.0 : 1.0; }
+
+// Place a request in the arena and initialise its resident buffers.
+bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
+  const size_t S = model_state(rq.model), P = rq.P, T = rq.times.size();
+  const size_t nd = model_n_data(rq.model), kp = model_kpars(rq.model);
+  const srv_layout L = srv_plan(S, P, T, nd, kp);
+  size_t off = 0;
+  if (!sv->mem.alloc(L.total, off)) return false;
+  auto slot = std::make_unique<srv_slot>();
+  slot->req = rq;
+  slot->off = off;
+  slot->L = L;
+  slot->T = T;
+  char* b = sv->mem.base() + off;
+  // schedule (dust2/discrete/system.hpp:61-92) and data
+  std::vector<double> st(T), lf(T * nd);
+  std::vector<uint32_t> ns(T);
+  std::vector<uint64_t> zm(T);
+  double tcur = rq.time_start;
+  const double period = model_zero_period(rq.model);
+  for (size_t t = 0; t < T; ++t) {
+    const size_t n = static_cast<size_t>(std::round(std::max(0.0, rq.times[t] - tcur) / rq.dt));
+    st[t] = tcur;
+    ns[t] = static_cast<uint32_t>(n);
+    uint64_t m = 0;
+    for (size_t k = 0; k < std::min<size_t>(n, 64); ++k) {
+      if (std::fmod(tcur + k * rq.dt, period) == 0) m |= uint64_t(1) << k;
+    }
+    zm[t] = m;
+    tcur = tcur + n * rq.dt;
+  }
+  for (size_t i = 0; i < T * nd; ++i) {
+    lf[i] = std::isnan(rq.data[i]) ? 0.0 : std::lgamma(rq.data[i] + 1);
+  }
+  std::vector<double> kpars(kp, 0.0);
+  if (rq.model == 1) {
+    seir_kernel_pars(rq.pars.data(), rq.dt, kpars.data());
+  } else {
+    sir_kernel_pars(rq.pars.data(), rq.dt, kpars.data());
+  }
+  auto up = [&](size_t at, const void* src, size_t bytes) {
+    if (bytes) hip_check(hipMemcpyAsync(b + at, src, bytes, hipMemcpyHostToDevice, s), "upload");
+  };
+  up(L.sched_time, st.data(), T * 8);
+  up(L.sched_steps, ns.data(), T * 4);
+  up(L.sched_zero, zm.data(), T * 8);
+  up(L.obs, rq.data.data(), T * nd * 8);
+  up(L.lfact, lf.data(), T * nd * 8);
+  up(L.kpars, kpars.data(), kp * 8);
+  hip_check(hipMemsetAsync(b + L.ll, 0, 8, s), "memset");
+  hip_check(hipMemsetAsync(b + L.counters, 0, 8, s), "memset");
+  {
+    dbuf tmp((P + 2) * 32);
+    expand_streams(nullptr, 0, rq.seed, P + 2, tmp.get<uint64_t>(), s);
+    hip_check(launch_rng_aos_to_soa(tmp.get<uint64_t>(), reinterpret_cast<uint64_t*>(b + L.rng),
+                                    1, static_cast<int>(P + 2), static_cast<int>(P + 2), s),
+              "rng layout");
+    hip_check(hipStreamSynchronize(s), "sync");
+  }
+  pipe_slot& ps = slot->ps;
+  ps.pars = reinterpret_cast<const double*>(b + L.kpars);
+  ps.obs = reinterpret_cast<const double*>(b + L.obs);
+  ps.lfact = reinterpret_cast<const double*>(b + L.lfact);
+  ps.obs_stride = static_cast<uint32_t>(nd);
+  ps.n_particles = static_cast<uint32_t>(P);
+  ps.flags = SLOT_AUTO_MODE | (rq.model == 1 ? SLOT_WEIGHT_IS_LL : 0u);
+  ps.group = 0;
+  ps.sched_time = reinterpret_cast<const double*>(b + L.sched_time);
+  ps.sched_steps = reinterpret_cast<const uint32_t*>(b + L.sched_steps);
+  ps.sched_zero = reinterpret_cast<const uint64_t*>(b + L.sched_zero);
+  ps.dt = rq.dt;
+  ps.state_a = b + L.state_a;
+  ps.state_b = b + L.state_b;
+  ps.rng = reinterpret_cast<uint64_t*>(b + L.rng);
+  ps.lambda = reinterpret_cast<double*>(b + L.lambda);
+  ps.ancestors = reinterpret_cast<uint32_t*>(b + L.anc);
+  ps.ll = reinterpret_cast<double*>(b + L.ll);
+  ps.ancestry = nullptr;
+  ps.traj = nullptr;
+  ps.errors = reinterpret_cast<error_record*>(b + L.errors);
+  ps.error_count = reinterpret_cast<uint32_t*>(b + L.counters);
+  ps.n_fallback = reinterpret_cast<uint32_t*>(b + L.counters) + 1;
+  ps.t_offset = 0;
+  ps.n_time = static_cast<uint32_t>(T);
+  hip_check(hipMemsetAsync(b + L.errors, 0, P * sizeof(error_record), s), "memset");
+  sv->active.push_back(std::move(slot));
+  return true;
+}
+
+// All queued deterministic (ODE) requests in one launch, each with its own
+// series (one thread per request).
+void srv_run_ode(dust2_server* sv, hipStream_t s) {
+  if (sv->ode_queue.empty()) return;
+  const size_t n = std::min(sv->ode_queue.size(), sv->max_ode_per_tick);
+  std::vector<double> pars(n * kSirodePars), times, obs, lf;
+  std::vector<int64_t> off(n);
+  std::vector<int32_t> len(n);
+  for (size_t i = 0; i < n; ++i) {
+    const auto& rq = sv->ode_queue[i];
+    std::copy(rq.pars.begin(), rq.pars.end(), pars.begin() + i * kSirodePars);
+    off[i] = static_cast<int64_t>(times.size());
+    len[i] = static_cast<int32_t>(rq.times.size());
+    times.insert(times.end(), rq.times.begin(), rq.times.end());
+    obs.insert(obs.end(), rq.data.begin(), rq.data.end());
+    for (double x : rq.data) lf.push_back(std::isnan(x) ? 0.0 : std::lgamma(x + 1));
+  }
+  auto fit = [&](dbuf& d, size_t bytes) {
+    if (d.bytes() < bytes) d.alloc(std::max<size_t>(bytes, 256));
+  };
+  fit(sv->d_ode_pars, pars.size() * 8);
+  fit(sv->d_ode_times, times.size() * 8);
+  fit(sv->d_ode_series, obs.size() * 8);
+  fit(sv->d_ode_lfact, lf.size() * 8);
+  fit(sv->d_ode_off, n * 8);
+  fit(sv->d_ode_len, n * 4);
+  fit(sv->d_ode_state, n * 5 * 8);
+  fit(sv->d_ode_ll, n * 8);
+  fit(sv->d_ode_errors, n * sizeof(error_record));
+  fit(sv->d_ode_counters, 16);
+  to_device(sv->d_ode_pars, pars.data(), pars.size() * 8, s);
+  to_device(sv->d_ode_times, times.data(), times.size() * 8, s);
+  to_device(sv->d_ode_series, obs.data(), obs.size() * 8, s);
+  to_device(sv->d_ode_lfact, lf.data(), lf.size() * 8, s);
+  to_device(sv->d_ode_off, off.data(), n * 8, s);
+  to_device(sv->d_ode_len, len.data(), n * 4, s);
+  hip_check(hipMemsetAsync(sv->d_ode_errors.get(), 0, n * sizeof(error_record), s), "memset");
+  hip_check(hipMemsetAsync(sv->d_ode_counters.get(), 0, 16, s), "memset");
+  ode_launch a{};
+  a.pars = sv->d_ode_pars.get<double>();
+  a.n_particles = 1;
+  a.n_total = static_cast<int>(n);
+  a.time_start = sv->ode_queue[0].time_start;
+  a.times = sv->d_ode_times.get<double>();
+  a.obs = sv->d_ode_series.get<double>();
+  a.lfact = sv->d_ode_lfact.get<double>();
+  a.control = ode_control_dev{10000, 1e-6, 1e-6, std::numeric_limits<double>::epsilon(),
+                              std::numeric_limits<double>::infinity(), 0.9, 0.2, 10.0, 0.04,
+                              0.2 - 0.04 * 0.75};
+  a.state = sv->d_ode_state.get();
+  a.ll = sv->d_ode_ll.get<double>();
+  a.errors = sv->d_ode_errors.get<error_record>();
+  a.error_count = sv->d_ode_counters.get<uint32_t>();
+  a.series_offset = sv->d_ode_off.get<int64_t>();
+  a.series_len = sv->d_ode_len.get<int32_t>();
+  hip_check(launch_dp5_unfilter(a, true, s), "dp5 serving kernel");
+  std::vector<double> ll(n);
+  std::vector<error_record> rec(n);
+  to_host(ll.data(), sv->d_ode_ll, n * 8, s);
+  to_host(rec.data(), sv->d_ode_errors, n * sizeof(error_record), s);
+  for (size_t i = 0; i < n; ++i) {
+    srv_result r{sv->ode_queue[i].id, ll[i], rec[i].code ? 1 : 0,
+                 rec[i].code ? error_text(rec[i]) : std::string()};
+    sv->results.push_back(r);
+  }
+  sv->ode_queue.erase(sv->ode_queue.begin(), sv->ode_queue.begin() + n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dust2_server_create(size_t hbm_bytes, int chunk_intervals, dust2_server** out, char* err,
+                        size_t errlen) {
+  try {
+    if (chunk_intervals < 1) throw std::runtime_error("'chunk_intervals' must be at least 1");
+    if (hbm_bytes < (size_t(1) << 20)) throw std::runtime_error("'hbm_bytes' is too small");
+    auto sv = std::make_unique<dust2_server>();
+    sv->chunk = chunk_intervals;
+    for (auto& s : sv->streams) {
+      hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    sv->mem.init(hbm_bytes);
+    tables_for_current_device();
+    *out = sv.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t n_particles,
+                        double time_start, const double* time, size_t n_time, double dt,
+                        const double* data, uint64_t seed, uint64_t* id_out, char* err,
+                        size_t errlen) {
+  try {
+    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    if (model < 0 || model > 2) throw std::runtime_error("unknown model");
+    check_time_sequence(time_start, time, n_time);
+    srv_request rq;
+    rq.id = sv->next_id++;
+    rq.model = model;
+    rq.P = model == 2 ? 1 : n_particles;
+    rq.time_start = time_start;
+    rq.dt = dt;
+    rq.seed = seed;
+    const size_t np = model_n_pars(model);
+    rq.pars.assign(pars, pars + np);
+    rq.times.assign(time, time + n_time);
+    rq.data.assign(data, data + n_time * model_n_data(model));
+    if (model == 2) {
+      check_sirode_pars(pars);
+      sv->ode_queue.push_back(std::move(rq));
+    } else {
+      check_sizes(1, n_particles);
+      check_dt(dt);
+      if (pipe_resample_ppt(static_cast<int>(n_particles)) == 0) {
+        throw std::runtime_error("'n_particles' too large");
+      }
+      if (model == 0) {
+        check_sir_pars(pars);
+      } else {
+        check_finite_pars(pars, np);
+      }
+      sv->queue.push_back(std::move(rq));
+    }
+    *id_out = sv->next_id - 1;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// One scheduling tick: admit queued requests while their footprint fits,
+// advance every resident request by up to chunk_intervals data intervals
+// (requests of the same model and particle count share each launch;
+// distinct launch groups run on separate streams), run the queued ODE
+// requests, and retire finished requests.
+int dust2_server_step(dust2_server* sv, size_t* n_active, size_t* n_queued, char* err,
+                      size_t errlen) {
+  try {
+    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    hipStream_t s0 = sv->streams[0];
+    while (!sv->queue.empty()) {
+      if (!srv_admit(sv, sv->queue.front(), s0)) {
+        if (sv->active.empty()) {
+          throw std::runtime_error("request does not fit the server's memory budget");
+        }
+        break;
+      }
+      sv->queue.pop_front();
+    }
+    hip_check(hipStreamSynchronize(s0), "sync");
+    // launch groups: (model, particle count)
+    std::map<std::pair<int, size_t>, std::vector<srv_slot*>> groups;
+    for (auto& sl : sv->active) groups[{sl->req.model, sl->req.P}].push_back(sl.get());
+    std::vector<pipe_slot> table;
+    std::vector<std::tuple<int, size_t, size_t, size_t>> spans;  // model, P, first, count
+    for (auto& kv : groups) {
+      const size_t first = table.size();
+      for (auto* sl : kv.second) {
+        sl->ps.t_offset = static_cast<int32_t>(sl->done);
+        table.push_back(sl->ps);
+      }
+      spans.emplace_back(kv.first.first, kv.first.second, first, kv.second.size());
+    }
+    if (!table.empty()) {
+      if (sv->d_tables.bytes() < table.size() * sizeof(pipe_slot)) {
+        sv->d_tables.alloc(std::max<size_t>(table.size() * 2, 64) * sizeof(pipe_slot));
+      }
+      to_device(sv->d_tables, table.data(), table.size() * sizeof(pipe_slot), s0);
+      hip_check(hipStreamSynchronize(s0), "sync");
+      const pipe_slot* base = sv->d_tables.get<pipe_slot>();
+      int si = 0;
+      for (const auto& sp : spans) {
+        const int model = std::get<0>(sp);
+        const int P = static_cast<int>(std::get<1>(sp));
+        const pipe_slot* tab = base + std::get<2>(sp);
+        const int n = static_cast<int>(std::get<3>(sp));
+        hipStream_t s = sv->streams[(si++) % dust2_server::kStreams];
+        for (int k = 0; k < sv->chunk; ++k) {
+          if (model == 1) {
+            hip_check(launch_seir_advance(tab, n, P, k, 0, 0, s), "seir advance");
+          } else {
+            hip_check(launch_sir_advance(tab, n, P, k, 0, 0, false, s), "sir advance");
+          }
+          hip_check(launch_pipe_resample(tab, n, P, k, s), "resample");
+        }
+      }
+    }
+    srv_run_ode(sv, sv->streams[dust2_server::kStreams - 1]);
+    for (auto s : sv->streams) hip_check(hipStreamSynchronize(s), "sync");
+    // status of every resident request in one copy (the table is in the
+    // order of `groups`)
+    std::vector<double> status(2 * table.size());
+    if (!table.empty()) {
+      if (sv->d_status.bytes() < status.size() * 8) sv->d_status.alloc(status.size() * 16);
+      hip_check(launch_slot_status(sv->d_tables.get<pipe_slot>(), static_cast<int>(table.size()),
+                                   sv->d_status.get<double>(), s0),
+                "status kernel");
+      to_host(status.data(), sv->d_status, status.size() * 8, s0);
+    }
+    std::map<srv_slot*, size_t> pos;
+    {
+      size_t i = 0;
+      for (auto& kv : groups) {
+        for (auto* sl : kv.second) pos[sl] = i++;
+      }
+    }
+    // retire finished requests
+    std::vector<std::unique_ptr<srv_slot>> keep;
+    for (auto& sl : sv->active) {
+      sl->done += sv->chunk;
+      const size_t i = pos[sl.get()];
+      const uint32_t count = static_cast<uint32_t>(status[2 * i + 1]);
+      const char* b = sv->mem.base() + sl->off;
+      if (sl->done >= sl->T || count > 0) {
+        srv_result r{sl->req.id, status[2 * i], 0, std::string()};
+        if (count > 0) {
+          std::vector<error_record> rec(sl->req.P);
+          hip_check(hipMemcpy(rec.data(), b + sl->L.errors, rec.size() * sizeof(error_record),
+                              hipMemcpyDeviceToHost),
+                    "copy");
+          r.status = 1;
+          r.message = error_report(rec, count);
+        }
+        sv->results.push_back(std::move(r));
+        sv->mem.release(sl->off, sl->L.total);
+      } else {
+        keep.push_back(std::move(sl));
+      }
+    }
+    sv->active.swap(keep);
+    if (n_active) *n_active = sv->active.size();
+    if (n_queued) *n_queued = sv->queue.size() + sv->ode_queue.size();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_server_poll(dust2_server* sv, uint64_t* ids, double* ll, int* status, size_t max,
+                      size_t* n_out, char* err, size_t errlen) {
+  try {
+    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    const size_t n = std::min(max, sv->results.size());
+    for (size_t i = 0; i < n; ++i) {
+      ids[i] = sv->results[i].id;
+      ll[i] = sv->results[i].ll;
+      status[i] = sv->results[i].status;
+    }
+    if (n > 0 && err && errlen) {
+      // the first error message among the returned results, if any
+      for (size_t i = 0; i < n; ++i) {
+        if (sv->results[i].status) {
+          std::snprintf(err, errlen, "%s", sv->results[i].message.c_str());
+          break;
+        }
+      }
+    }
+    sv->results.erase(sv->results.begin(), sv->results.begin() + n);
+    *n_out = n;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_server_memory(dust2_server* sv, size_t* used, size_t* capacity, char* err,
+                        size_t errlen) {
+  try {
+    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    *used = sv->mem.used();
+    *capacity = sv->mem.capacity();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_server_free(dust2_server* sv) { delete sv; }
 
 }  // extern "C"

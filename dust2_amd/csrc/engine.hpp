@@ -28,6 +28,9 @@ enum slot_flags : uint32_t {
   SLOT_DETERMINISTIC = 2u,  // samplers return their means
   SLOT_SAVE_TRAJ = 4u,      // write the post-reorder state at every time
   SLOT_WEIGHT_IS_LL = 8u,   // pipeline: lambda[] already holds the log-weight
+  SLOT_AUTO_MODE = 16u,     // pipeline: the slot's own interval is t + t_offset;
+                            // mode and buffer parity follow from it, and
+                            // intervals past n_time do nothing (serving)
 };
 
 // Per-particle error record (first error of a particle).
@@ -111,10 +114,27 @@ struct pipe_slot {
   error_record* errors;        // [P]
   uint32_t* error_count;
   uint32_t* n_fallback;
+  int32_t t_offset;            // SLOT_AUTO_MODE: intervals already run
+  uint32_t n_time;             // SLOT_AUTO_MODE: intervals in the slot's series
 };
+
+// Interval, buffer parity and mode of a slot for launch interval t; false
+// when the slot has nothing to do in this launch.
+__host__ __device__ inline bool pipe_slot_interval(const pipe_slot& sd, int& t, int& parity,
+                                                   uint32_t& mode) {
+  if (!(sd.flags & SLOT_AUTO_MODE)) return true;
+  t += sd.t_offset;
+  if (t < 0 || static_cast<uint32_t>(t) >= sd.n_time) return false;
+  parity = t & 1;
+  mode = PIPE_STEP | (t == 0 ? PIPE_INITIAL : PIPE_GATHER);
+  return true;
+}
 
 int pipe_resample_ppt(int max_particles);
 hipError_t launch_pipe_reset(const pipe_slot* d_slots, int n_slots, hipStream_t stream);
+// out[2i] = slot i's log-likelihood, out[2i+1] = its error count
+hipError_t launch_slot_status(const pipe_slot* d_slots, int n_slots, double* out,
+                              hipStream_t stream);
 hipError_t launch_sir_advance(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
                               int parity, uint32_t mode, bool fp64, hipStream_t stream);
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
@@ -150,6 +170,11 @@ struct ode_launch {
   double* stats;             // [n_total][4] or null
   error_record* errors;      // [n_total]
   uint32_t* error_count;
+  // optional per-group data series (serving): group g reads times, obs and
+  // lfact from offset series_offset[g], series_len[g] values, one value per
+  // time; null for one series shared through obs_stride/obs_group_stride
+  const int64_t* series_offset;
+  const int32_t* series_len;
 };
 
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream);

@@ -264,10 +264,19 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
   ode_initialise<real>(s, sh, ctl, code);
   double time = args.time_start;
   double ll = 0;
-  const size_t dstride = args.obs_stride;
-  const size_t doff = args.obs_group_stride * static_cast<size_t>(g);
-  for (int t = 0; t < args.n_time && code == 0; ++t) {
-    const double t_end = args.times[t];
+  size_t dstride = args.obs_stride;
+  size_t doff = args.obs_group_stride * static_cast<size_t>(g);
+  const double* times = args.times;
+  int n_time = args.n_time;
+  if (args.series_offset) {
+    const int64_t off = args.series_offset[g];
+    times += off;
+    doff = static_cast<size_t>(off);
+    dstride = 1;
+    n_time = args.series_len[g];
+  }
+  for (int t = 0; t < n_time && code == 0; ++t) {
+    const double t_end = times[t];
     // solver::run (solver.hpp:231-262)
     double tt = time;
     while (tt < t_end && code == 0) {

@@ -46,6 +46,7 @@ __global__ void __launch_bounds__(kAdvBlock)
   const int P = static_cast<int>(sd.n_particles);
   const int p = blockIdx.x * kAdvBlock + threadIdx.x;
   if (p >= P) return;
+  if (!pipe_slot_interval(sd, t, parity, mode)) return;
   // the reference stops at the first interval with errors
   // (dust2/discrete/system.hpp:87): later intervals of the run do nothing
   if ((mode & PIPE_STEP) && __atomic_load_n(sd.error_count, __ATOMIC_RELAXED) != 0) return;
@@ -122,6 +123,11 @@ __global__ void __launch_bounds__(BLOCK) resample_kernel(const pipe_slot* __rest
   double* cum = smem + kScratchDoubles;
   const pipe_slot& sd = slots[blockIdx.x];
   const int P = static_cast<int>(sd.n_particles);
+  {
+    int parity = 0;
+    uint32_t mode = 0;
+    if (!pipe_slot_interval(sd, t, parity, mode)) return;
+  }
   const int tid = threadIdx.x;
   const int p0 = tid * PPT;
   const double x = sd.obs[static_cast<size_t>(t) * sd.obs_stride];
@@ -162,6 +168,17 @@ __global__ void __launch_bounds__(BLOCK) resample_kernel(const pipe_slot* __rest
   }
 }
 
+// Log-likelihood and error count of every slot into one array (one copy
+// back per scheduling tick instead of one per slot).
+__global__ void slot_status_kernel(const pipe_slot* __restrict__ slots, int n_slots,
+                                   double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_slots) {
+    out[2 * i] = slots[i].ll[0];
+    out[2 * i + 1] = static_cast<double>(slots[i].error_count[0]);
+  }
+}
+
 // Zero every slot's log-likelihood (a kernel, not a memset, so the whole
 // run can be captured in a graph).
 __global__ void reset_ll_kernel(const pipe_slot* __restrict__ slots, int n_slots) {
@@ -180,6 +197,14 @@ int pipe_resample_ppt(int max_particles) {
   if (max_particles <= 8192) return 8;
   if (max_particles <= 16384) return 16;
   return 0;
+}
+
+hipError_t launch_slot_status(const pipe_slot* d_slots, int n_slots, double* out,
+                              hipStream_t stream) {
+  if (n_slots == 0) return hipSuccess;
+  hipLaunchKernelGGL(slot_status_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, stream,
+                     d_slots, n_slots, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_pipe_reset(const pipe_slot* d_slots, int n_slots, hipStream_t stream) {

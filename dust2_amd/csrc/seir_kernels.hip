@@ -43,6 +43,7 @@ __global__ void __launch_bounds__(kWaveP)
   const int lane = threadIdx.x;
   const int p = blockIdx.x * kWaveP + lane;
   if (static_cast<int>(blockIdx.x) * kWaveP >= P) return;  // whole wavefront idle
+  if (!pipe_slot_interval(sd, t, parity, mode)) return;
   if ((mode & PIPE_STEP) && __atomic_load_n(sd.error_count, __ATOMIC_RELAXED) != 0) return;
   const bool active = p < P;
   const bool det = (sd.flags & SLOT_DETERMINISTIC) != 0;

@@ -195,6 +195,40 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
                                      char* err, size_t errlen);
 void dust2_unfilter_sirode_free(dust2_unfilter* h);
 
+/* ---- serving: continuous batching of likelihood evaluations ---------------
+ * No counterpart in the reference, whose MCMC driver evaluates one
+ * likelihood at a time (R/monty.R:188-204).  Each request is one parameter
+ * group's complete filter (sir, seir16: FP32) or ODE unfilter (sirode: FP64)
+ * run on its own series.  Requests are admitted when their device footprint
+ * fits the server's HBM budget (a first-fit arena, no allocation per
+ * request), stay resident between ticks, and every tick advances all
+ * resident requests by chunk_intervals data intervals; requests of the same
+ * model and particle count share launches, other launch groups run on
+ * separate streams.  A request's result depends only on its own inputs and
+ * seed: it is identical alone, in any mix and with the plain filter entry
+ * points on one group. */
+typedef struct dust2_server dust2_server;
+#define DUST2_MODEL_SIR 0
+#define DUST2_MODEL_SEIR16 1
+#define DUST2_MODEL_SIRODE 2
+int dust2_server_create(size_t hbm_bytes, int chunk_intervals, dust2_server** out, char* err,
+                        size_t errlen);
+/* pars: one parameter row of the model; data: n_time x n_data values. */
+int dust2_server_submit(dust2_server* s, int model, const double* pars, size_t n_particles,
+                        double time_start, const double* time, size_t n_time, double dt,
+                        const double* data, uint64_t seed, uint64_t* id_out, char* err,
+                        size_t errlen);
+/* One scheduling tick (blocking); reports resident and queued requests. */
+int dust2_server_step(dust2_server* s, size_t* n_active, size_t* n_queued, char* err,
+                      size_t errlen);
+/* Up to max finished results: id, log-likelihood, status (0 ok, 1 the
+ * request reported errors; `err` then holds the first message). */
+int dust2_server_poll(dust2_server* s, uint64_t* ids, double* ll, int* status, size_t max,
+                      size_t* n_out, char* err, size_t errlen);
+int dust2_server_memory(dust2_server* s, size_t* used, size_t* capacity, char* err,
+                        size_t errlen);
+void dust2_server_free(dust2_server* s);
+
 /* ---- kernels exposed for parity tests --------------------------------------
  * Systematic resampling of one weight vector on the GPU with the filter's
  * own workgroup routine (replaces the reference test driver

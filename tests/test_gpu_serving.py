@@ -1,0 +1,130 @@
+"""Continuous-batching server: every result equals the same request run
+alone, run with other chunk sizes, and run through the plain filter entry
+points (same kernels, same seed) -- bit for bit, FP32 filters and FP64 ODE
+unfilters; requests wait for memory rather than fail; a failing request
+reports its error without disturbing the others."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+SIR_NAMES = ["N", "I0", "beta", "gamma", "exp_noise"]
+
+
+def sir_request(rng, T):
+    pars = {"N": float(np.round(rng.uniform(1e3, 1e5))), "I0": 10.0,
+            "beta": rng.uniform(0.2, 0.5), "gamma": rng.uniform(0.05, 0.2), "exp_noise": 1e6}
+    time = np.arange(1.0, T + 1)
+    data = rng.poisson(20, T).astype(float)
+    return pars, {"time": time, "incidence": data}
+
+
+def seir_request(rng, T):
+    C = rng.uniform(0, 2, (16, 16))
+    C = (C + C.T) / 2
+    C /= np.max(np.abs(np.linalg.eigvals(C)))
+    pars = {"beta": rng.uniform(0.3, 0.8), "sigma": 0.3, "gamma": 0.2, "exp_noise": 1e6,
+            "I0": 5.0, "N": np.full(16, 2e4), "C": C}
+    time = np.arange(7.0, 7.0 * (T + 1), 7.0)
+    data = rng.poisson(30, (T, 16)).astype(float)
+    return pars, {"time": time, "incidence": data}
+
+
+def ode_request(rng, T):
+    pars = {"N": 1000.0, "I0": 10.0, "beta": rng.uniform(0.1, 0.5),
+            "gamma": rng.uniform(0.05, 0.2)}
+    time = np.arange(1.0, T + 1)
+    data = rng.poisson(6, T).astype(float)
+    return pars, {"time": time, "incidence": data}
+
+
+def make_mix(lib, seed=1, n=24):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        kind = i % 3
+        if kind == 0:
+            p, d = sir_request(rng, int(rng.integers(20, 60)))
+            reqs.append((lib.sir(), p, int(rng.choice([128, 512, 2048])), d, 0.25, 100 + i))
+        elif kind == 1:
+            p, d = seir_request(rng, int(rng.integers(3, 8)))
+            reqs.append((lib.seir16(), p, int(rng.choice([128, 512])), d, 0.25, 100 + i))
+        else:
+            p, d = ode_request(rng, int(rng.integers(30, 90)))
+            reqs.append((lib.sirode(), p, 1, d, 1.0, 100 + i))
+    return reqs
+
+
+def serve(lib, reqs, chunk=8, hbm=1 << 30):
+    from dust2_amd.serving import Server
+    sv = Server(hbm_bytes=hbm, chunk_intervals=chunk)
+    ids = [sv.submit(g, p, P, 0.0, d, dt=dt, seed=s) for (g, p, P, d, dt, s) in reqs]
+    res = {r[0]: r for r in sv.drain()}
+    assert sv.memory()[0] == 0
+    return [res[i] for i in ids]
+
+
+def test_results_identical_alone_in_mix_and_chunked(gpu_lib):
+    reqs = make_mix(gpu_lib)
+    mixed = serve(gpu_lib, reqs, chunk=8)
+    assert all(r[2] == 0 for r in mixed)
+    rechunked = serve(gpu_lib, reqs[::-1], chunk=3)[::-1]
+    for a, b in zip(mixed, rechunked):
+        assert a[1] == b[1]
+    for k in [0, 1, 2, 5]:
+        alone = serve(gpu_lib, [reqs[k]], chunk=50)
+        assert alone[0][1] == mixed[k][1]
+
+
+def test_server_matches_plain_entry_points(gpu_lib):
+    reqs = make_mix(gpu_lib, seed=2, n=6)
+    served = serve(gpu_lib, reqs)
+    for (g, p, P, d, dt, s), r in zip(reqs, served):
+        if g.name == "sirode":
+            obj = gpu_lib.dust_unfilter_create(g, 0.0, d, n_particles=1)
+            ll = obj.run(p)
+        else:
+            obj = gpu_lib.dust_filter_create(g, 0.0, d, P, dt=dt, seed=s, precision="fp32")
+            obj.set_engine("pipeline", graph=False)
+            ll = gpu_lib.dust_likelihood_run(obj, p)
+        assert r[1] == ll, (g.name, r[1], ll)
+
+
+def test_sir_requests_statistically_match_oracle(gpu_lib):
+    rng = np.random.default_rng(5)
+    p, d = sir_request(rng, 30)
+    reqs = [(gpu_lib.sir(), p, 512, d, 0.25, 1000 + i) for i in range(60)]
+    ll = np.array([r[1] for r in serve(gpu_lib, reqs)])
+    row = np.array([[p[k] for k in SIR_NAMES]] * 40)
+    ref = ol.sir_filter(row, 512, 0.0, d["time"], 0.25, d["incidence"][:, None], seed=3,
+                        n_threads=16)["ll"]
+    se = np.sqrt(ll.var(ddof=1) / ll.size + ref.var(ddof=1) / ref.size)
+    assert abs(ll.mean() - ref.mean()) < 3 * se
+
+
+def test_admission_waits_for_memory(gpu_lib):
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i in range(12):
+        p, d = sir_request(rng, 40)
+        reqs.append((gpu_lib.sir(), p, 2048, d, 0.25, i))
+    tight = serve(gpu_lib, reqs, chunk=5, hbm=2 << 20)   # room for ~2 requests
+    roomy = serve(gpu_lib, reqs, chunk=5, hbm=1 << 30)
+    assert [r[1] for r in tight] == [r[1] for r in roomy]
+
+
+def test_failing_request_is_isolated(gpu_lib):
+    from dust2_amd.serving import Server
+    rng = np.random.default_rng(4)
+    good_p, d = sir_request(rng, 20)
+    bad_p = dict(good_p, gamma=-0.5)
+    sv = Server(hbm_bytes=1 << 28, chunk_intervals=4)
+    a = sv.submit(gpu_lib.sir(), good_p, 128, 0.0, d, dt=0.25, seed=1)
+    b = sv.submit(gpu_lib.sir(), bad_p, 128, 0.0, d, dt=0.25, seed=2)
+    res = {r[0]: r for r in sv.drain()}
+    assert res[a][2] == 0 and np.isfinite(res[a][1])
+    assert res[b][2] == 1 and "particles reported errors" in res[b][3]
+    alone = serve(gpu_lib, [(gpu_lib.sir(), good_p, 128, d, 0.25, 1)], chunk=4)
+    assert alone[0][1] == res[a][1]
