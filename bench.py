@@ -1,15 +1,31 @@
-"""Benchmark: batched bootstrap particle filter on one MI355X.
+"""Benchmarks of the batched particle engine on one MI355X.
 
-Workload (default): 512 parameter groups x 2048 particles of the stochastic
-SIR model (beta ~ U(0.1, 0.5), gamma ~ U(0.05, 0.2), N ~ U(1e3, 1e6)
-integer, I0 = 10), dt = 0.25, 365 daily Poisson observations per group
-simulated from the model with a fixed seed, FP32 state and samplers with
-FP64 log-weights, resampling sums and likelihood.  One step = one complete
-filter run of every group (1.53e9 particle-steps).
+Default workload (the headline): 512 parameter groups x 2048 particles of
+the stochastic SIR model (beta ~ U(0.1, 0.5), gamma ~ U(0.05, 0.2),
+N ~ U(1e3, 1e6) integer, I0 = 10), dt = 0.25, 365 daily Poisson
+observations per group simulated from the model with a fixed seed, FP32
+state and samplers with FP64 log-weights, resampling sums and likelihood.
+One step = one complete filter run of every group (1.53e9 particle-steps).
 
-Prints one JSON line: particle-steps/s, the roofline figure of the filter
-kernel (algorithmic bytes per particle per observation, over the kernel's
-own event-timed duration) and the CPU oracle timed on a bounded sample.
+Other workloads (--workload):
+  sir-replicates  1000 replicate filters of the 1000-particle SIR problem
+                  (100 daily observations), FP64: evals/s and the mean and
+                  variance of the log-likelihood beside the FP64 CPU oracle
+  sirode          65,536 SIR ODE parameter sets, Dormand-Prince 5(4),
+                  rtol = atol = 1e-6, 365 daily observations, FP64 and FP32
+  seir16          128 groups x 8192 particles of the 16-age SEIR (80
+                  states), 200 weekly observations, FP32, batched vs one
+                  group per launch
+  serving         64 random-walk MCMC chains submitting likelihood
+                  evaluations (sir / seir16 / sirode, particles in
+                  {128, 512, 2048, 8192}, data length U{50..730}) at up to
+                  2000 requests/s: continuous batching vs one request at a
+                  time, evals/s and p50/p99 latency
+
+Each prints one JSON line.  `roofline` refers to the dominant kernel:
+algorithmic bytes per launch (state gather-read and write, xoshiro256+
+read and write, FP64 weight write and read, u32 ancestor write and read per
+particle per observation) over its event-timed duration.
 """
 import argparse
 import json
@@ -26,17 +42,29 @@ HBM_PEAK_GBS = 8000.0
 SIR_NAMES = ["N", "I0", "beta", "gamma", "exp_noise"]
 
 
-def make_workload(n_groups, n_time, seed=42):
+def cpu_threads():
+    return min(16, os.cpu_count() or 1)
+
+
+def oracle():
+    """The CPU oracle, used only for the cpu_baseline leg."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    return oracle_lib
+
+
+# ---------------------------------------------------------------------------
+# workload construction
+# ---------------------------------------------------------------------------
+def sir_groups(n_groups, seed=42):
     rng = np.random.default_rng(seed)
     beta = rng.uniform(0.1, 0.5, n_groups)
     gamma = rng.uniform(0.05, 0.2, n_groups)
     N = np.round(rng.uniform(1e3, 1e6, n_groups))
-    pars = np.column_stack([N, np.full(n_groups, 10.0), beta, gamma,
-                            np.full(n_groups, 1e6)])
-    return pars, np.arange(1.0, n_time + 1)
+    return np.column_stack([N, np.full(n_groups, 10.0), beta, gamma, np.full(n_groups, 1e6)])
 
 
-def simulate_data(lib, pars, time, dt, seed):
+def simulate_sir_data(lib, pars, time, dt, seed):
     """Daily incidence from one stochastic run of each group's own model
     (GPU system kernels)."""
     plist = [dict(zip(SIR_NAMES, p)) for p in pars]
@@ -46,26 +74,392 @@ def simulate_data(lib, pars, time, dt, seed):
     return np.ascontiguousarray(out[:, :, 0, 4])
 
 
-def cpu_baseline(pars, time, dt, data, n_particles, target_s=15.0):
-    """FP64 CPU oracle (OpenMP over group x particle, as dust2) on a bounded
-    sample of the same workload: as many groups as fit ~target_s."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol
-    threads = min(16, os.cpu_count() or 1)
-    g = 1
-    while True:
-        t0 = _time.perf_counter()
-        ol.sir_filter(pars[:g], n_particles, 0.0, time, dt, data[:, :g], seed=1,
-                      n_threads=threads)
-        el = _time.perf_counter() - t0
-        if el > target_s / 4 or g >= len(pars):
-            break
-        g = min(len(pars), max(g * 2, int(g * target_s / 4 / max(el, 1e-3))))
-    steps = g * n_particles * round((time[-1] - 0.0) / dt)
-    return {"value": steps / el, "unit": "particle-steps/s", "cores": threads,
+def contact_matrix(seed=7):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(0, 2, (16, 16))
+    C = (C + C.T) / 2
+    return C / np.max(np.abs(np.linalg.eigvals(C)))
+
+
+def seir_groups(n_groups, seed=43):
+    rng = np.random.default_rng(seed)
+    C = contact_matrix()
+    N = np.round(rng.uniform(1e4, 1e5, 16))
+    return [{"beta": rng.uniform(0.1, 0.5), "sigma": 0.25, "gamma": rng.uniform(0.05, 0.2),
+             "exp_noise": 1e6, "I0": 10.0, "N": N, "C": C} for _ in range(n_groups)]
+
+
+def simulate_seir_data(lib, plist, time, dt, seed):
+    """Weekly incidence of each age group from one stochastic run per group:
+    a one-particle filter with no observations saves its trajectory."""
+    nan = np.full((time.size, 16), np.nan)
+    obj = lib.dust_filter_create(lib.seir16(), 0.0, {"time": time, "incidence": nan}, 1,
+                                 n_groups=len(plist), dt=dt, seed=seed, precision="fp32")
+    lib.dust_likelihood_run(obj, plist, save_trajectories=True)
+    tr = lib.dust_likelihood_last_trajectories(obj)          # (T, G, 1, 80)
+    return np.ascontiguousarray(tr[:, :, 0, 64:80])            # (T, G, 16)
+
+
+def roofline(achieved, kernel, k_ms, b_launch, b_obs):
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+            "kernel_ms": k_ms, "bytes_per_launch": b_launch, "bytes_per_particle_obs": b_obs}
+
+
+def timed_runs(lib, obj, plist, steps, warmup):
+    for _ in range(warmup):
+        lib.dust_likelihood_run(obj, plist)
+    gpu_ms = []
+    t0 = _time.perf_counter()
+    for _ in range(steps):
+        ll = lib.dust_likelihood_run(obj, plist)
+        gpu_ms.append(lib.dust_likelihood_last_stats(obj)["kernel_ms"])
+    return np.atleast_1d(ll), _time.perf_counter() - t0, gpu_ms
+
+
+def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms):
+    b_obs = 2 * S * s_x + 2 * 32 + 2 * 8 + 2 * 4
+    if engine_used == "persistent":
+        b_launch = G * P * n_obs * b_obs
+        return roofline(b_launch / (run_ms * 1e-3) / 1e9, "sir_filter_kernel", run_ms, b_launch,
+                        b_obs)
+    # pipeline: the advance kernel moves state, RNG, weight write and
+    # ancestor read of one interval; timed per launch in one profiled run
+    obj.set_engine("pipeline", graph=False, profile=True)
+    lib.dust_likelihood_run(obj, plist)
+    k_ms = lib.dust_likelihood_last_stats(obj)["advance_ms"]
+    obj.set_engine("auto")
+    b_launch = G * P * (2 * S * s_x + 2 * 32 + 8 + 4)
+    name = "seir_advance_kernel" if S > 5 else "sir_advance_kernel"
+    return roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def bench_sir(lib, args):
+    dt = 0.25
+    G, P = args.groups, args.particles
+    pars = sir_groups(G)
+    time = np.arange(1.0, args.n_obs + 1)
+    data = simulate_sir_data(lib, pars, time, dt, seed=42)
+    obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                 n_groups=G, dt=dt, seed=1, precision=args.precision)
+    if args.launch:
+        obj.set_launch(*[int(x) for x in args.launch.split(",")])
+    obj.set_engine(args.engine, graph=not args.no_graph)
+    plist = [dict(zip(SIR_NAMES, p)) for p in pars]
+    # every call passes the parameters, as an MCMC driver does
+    ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
+    n_steps = int(round(time[-1] / dt))
+    value = G * P * n_steps * args.steps / elapsed
+    stats = lib.dust_likelihood_last_stats(obj)
+    s_x = 4 if args.precision == "fp32" else 8
+    rl = filter_roofline(lib, obj, plist, G, P, time.size, 5, s_x, stats["engine"],
+                         float(np.mean(gpu_ms)))
+    out = {
+        "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (seeded simulation of each group's model)",
+        "config": {"workload": "batched stochastic SIR bootstrap filter",
+                   "groups": G, "particles": P, "n_obs": int(time.size), "dt": dt,
+                   "precision": args.precision, "engine": stats["engine"],
+                   "launch": obj.launch},
+        "likelihood_evals_per_s": G * args.steps / elapsed,
+        "gpu_ms_per_step": float(np.mean(gpu_ms)),
+        "mean_ll": float(np.mean(ll)),
+        "roofline": rl,
+    }
+    if not args.no_cpu:
+        ol = oracle()
+        threads = cpu_threads()
+        g = 1
+        while True:
+            t0 = _time.perf_counter()
+            ol.sir_filter(pars[:g], P, 0.0, time, dt, data[:, :g], seed=1, n_threads=threads)
+            el = _time.perf_counter() - t0
+            if el > 4 or g >= G:
+                break
+            g = min(G, max(g * 2, int(g * 4.5 / max(el, 1e-3))))
+        out["cpu_baseline"] = {
+            "value": g * P * n_steps / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"{g} of {len(pars)} groups x {n_particles} particles x {time.size} obs, "
-                      f"FP64 oracle, {el:.1f} s"}
+            "sample": f"{g} of {G} groups x {P} particles x {time.size} obs, FP64 oracle "
+                      f"(OpenMP over group x particle), {el:.1f} s"}
+    return out
+
+
+def bench_sir_replicates(lib, args):
+    dt = 0.25
+    G, P = 1000, 1000
+    row = np.array([1000.0, 10.0, 0.2, 0.1, 1e6])
+    time = np.arange(1.0, 101.0)
+    data = simulate_sir_data(lib, row[None, :], time, dt, seed=42)[:, 0]
+    prec = args.precision if args.precision_set else "fp64"
+    obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                 n_groups=G, dt=dt, seed=1, precision=prec)
+    plist = [dict(zip(SIR_NAMES, row))] * G
+    ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
+    n_steps = 400
+    stats = lib.dust_likelihood_last_stats(obj)
+    s_x = 8 if prec == "fp64" else 4
+    rl = filter_roofline(lib, obj, plist, G, P, time.size, 5, s_x, stats["engine"],
+                         float(np.mean(gpu_ms)))
+    out = {"metric": "likelihood evals/s", "value": G * args.steps / elapsed,
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == "fp64" else "f32",
+           "data": "synthetic (100 daily observations simulated at seed 42)",
+           "config": {"workload": "1000 replicate SIR filters, 1000 particles", "groups": G,
+                      "particles": P, "n_obs": 100, "dt": dt, "precision": prec},
+           "particle_steps_per_s": G * P * n_steps * args.steps / elapsed,
+           "ll_mean": float(ll.mean()), "ll_var": float(ll.var(ddof=1)),
+           "ll_mc_se": float(ll.std(ddof=1) / np.sqrt(G)), "roofline": rl}
+    if not args.no_cpu:
+        ol = oracle()
+        threads = cpu_threads()
+        n_ref = 200
+        t0 = _time.perf_counter()
+        ref = ol.sir_filter(np.tile(row, (n_ref, 1)), P, 0.0, time, dt, data[:, None], seed=7,
+                            n_threads=threads)["ll"]
+        el = _time.perf_counter() - t0
+        se = float(np.sqrt(ll.var(ddof=1) / G + ref.var(ddof=1) / n_ref))
+        out["oracle_ll_mean"] = float(ref.mean())
+        out["delta_ll_in_mc_se"] = float((ll.mean() - ref.mean()) / se)
+        out["cpu_baseline"] = {"value": n_ref / el, "unit": "evals/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{n_ref} replicate filters, FP64 oracle, {el:.1f} s"}
+    return out
+
+
+def bench_sirode(lib, args):
+    n = 65536
+    rng = np.random.default_rng(42)
+    pars = np.column_stack([np.full(n, 1000.0), np.full(n, 10.0), rng.uniform(0.1, 0.5, n),
+                            rng.uniform(0.05, 0.2, n)])
+    time = np.arange(1.0, 366.0)
+    data = simulate_sir_data(lib, np.array([[1000.0, 10.0, 0.2, 0.1, 1e6]]), time, 0.25,
+                             seed=42)[:, 0]
+    plist = [dict(zip(["N", "I0", "beta", "gamma"], p)) for p in pars]
+    res = {}
+    for prec in ["fp64", "fp32"]:
+        obj = lib.dust_unfilter_create(lib.sirode(), 0.0, {"time": time, "incidence": data},
+                                       n_particles=1, n_groups=n, precision=prec)
+        for _ in range(args.warmup):
+            obj.run(plist)
+        t0 = _time.perf_counter()
+        ms = []
+        for _ in range(args.steps):
+            ll = obj.run(None)
+            ms.append(obj.last_stats()[1])
+        el = _time.perf_counter() - t0
+        stats, _ = obj.last_stats()
+        res[prec] = {"evals_per_s": n * args.steps / el, "kernel_ms": float(np.mean(ms)),
+                     "attempts_per_s": float(stats[:, 0].sum()) / (np.mean(ms) * 1e-3),
+                     "mean_steps": float(stats[:, 0].mean()), "ll": ll}
+    out = {"metric": "likelihood evals/s", "value": res["fp64"]["evals_per_s"],
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res["fp64"]["kernel_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SIR simulation)",
+           "config": {"workload": "SIR ODE unfilter, Dormand-Prince 5(4)", "parameter_sets": n,
+                      "n_obs": 365, "rtol": 1e-6, "atol": 1e-6},
+           "fp64": {k: v for k, v in res["fp64"].items() if k != "ll"},
+           "fp32_state": {k: v for k, v in res["fp32"].items() if k != "ll"},
+           "fp32_vs_fp64_max_rel_ll": float(np.max(np.abs(res["fp32"]["ll"] - res["fp64"]["ll"])
+                                                   / np.abs(res["fp64"]["ll"])))}
+    if not args.no_cpu:
+        ol = oracle()
+        threads = cpu_threads()
+        m = n
+        t0 = _time.perf_counter()
+        ref = ol.sirode_unfilter(pars[:m], 1, 0.0, time, data, n_threads=threads)
+        el = _time.perf_counter() - t0
+        out["fp64_vs_oracle_max_rel_ll"] = float(np.max(np.abs(res["fp64"]["ll"][:m] - ref["ll"])
+                                                        / np.abs(ref["ll"])))
+        out["cpu_baseline"] = {"value": m / el, "unit": "evals/s", "cores": threads,
+                               "kind": "port", "sample": f"{m} parameter sets, FP64 oracle, "
+                                                         f"{el:.1f} s"}
+    return out
+
+
+def bench_seir16(lib, args):
+    dt = 0.25
+    G, P = 128, 8192
+    n_obs = args.n_obs_seir
+    time = np.arange(7.0, 7.0 * (n_obs + 1), 7.0)
+    plist = seir_groups(G)
+    data = simulate_seir_data(lib, plist, time, dt, seed=42)
+    obj = lib.dust_filter_create(lib.seir16(), 0.0, {"time": time, "incidence": data}, P,
+                                 n_groups=G, dt=dt, seed=1, precision="fp32")
+    ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
+    n_steps = int(round(time[-1] / dt))
+    value = G * P * n_steps * args.steps / elapsed
+    rl = filter_roofline(lib, obj, plist, G, P, time.size, 80, 4, "pipeline",
+                         float(np.mean(gpu_ms)))
+    # plain path: one filter object per group, one group per run; each
+    # object is warmed by one run, then a second run is timed
+    n_plain = 4
+    plain_s = 0.0
+    for g in range(n_plain):
+        one = lib.dust_filter_create(lib.seir16(), 0.0,
+                                     {"time": time, "incidence": data[:, g, :]}, P, dt=dt,
+                                     seed=100 + g, precision="fp32")
+        lib.dust_likelihood_run(one, plist[g])
+        t0 = _time.perf_counter()
+        lib.dust_likelihood_run(one, plist[g])
+        plain_s += _time.perf_counter() - t0
+    plain_evals = n_plain / plain_s
+    batched_evals = G * args.steps / elapsed
+    out = {"metric": "particle-steps/s", "value": value, "unit": "particle-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded simulation of each group's model)",
+           "config": {"workload": "age-structured SEIR16 bootstrap filter (MFMA FOI)",
+                      "groups": G, "particles": P, "n_obs": n_obs, "dt": dt},
+           "likelihood_evals_per_s": batched_evals, "plain_evals_per_s": plain_evals,
+           "batched_over_plain": batched_evals / plain_evals,
+           "mean_ll": float(np.mean(ll)), "roofline": rl}
+    return out
+
+
+def bench_serving(lib, args):
+    from dust2_amd.serving import Server
+    n_chains = 64
+    rate = 2000.0
+    duration = args.serve_seconds
+    C = contact_matrix()
+    gens = {"sir": lib.sir(), "seir16": lib.seir16(), "sirode": lib.sirode()}
+
+    def make_chains(seed):
+        rng = np.random.default_rng(seed)
+        chains = []
+        for c in range(n_chains):
+            model = ["sir", "seir16", "sirode"][c % 3]
+            T = int(rng.integers(50, 731))
+            P = int(rng.choice([128, 512, 2048, 8192])) if model != "sirode" else 1
+            if model == "seir16":
+                time = np.arange(7.0, 7.0 * (T + 1), 7.0)
+                data = rng.poisson(30, (T, 16)).astype(float)
+            else:
+                time = np.arange(1.0, T + 1)
+                data = rng.poisson(30 if model == "sir" else 6, T).astype(float)
+            chains.append({"model": model, "T": T, "P": P, "data": {"time": time,
+                                                                   "incidence": data},
+                           "theta": np.array([0.3, 0.1]), "ll": None, "busy": False,
+                           "obj": None})
+        return chains
+
+    def pars(ch, theta):
+        beta, gamma = float(abs(theta[0])), float(abs(theta[1]))
+        if ch["model"] == "seir16":
+            return {"beta": beta, "sigma": 0.25, "gamma": gamma, "exp_noise": 1e6, "I0": 10.0,
+                    "N": np.full(16, 2e4), "C": C}
+        p = {"N": 1000.0, "I0": 10.0, "beta": beta, "gamma": gamma}
+        if ch["model"] == "sir":
+            p["exp_noise"] = 1e6
+        return p
+
+    def accept(rng, ch, prop, ll):
+        if ch["ll"] is None or np.log(rng.random()) < ll - ch["ll"]:
+            ch["theta"], ch["ll"] = prop, ll
+
+    def summary(lat, soj, done, el, n_arrived):
+        # lat: submission -> result; soj: Poisson arrival -> result, which
+        # grows without bound once arrivals outpace the service rate
+        lat = np.array(lat) if lat else np.array([np.nan])
+        soj = np.array(soj) if soj else np.array([np.nan])
+        return {"evals_per_s": done / el, "completed": done, "arrived": n_arrived,
+                "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "sojourn_p50_ms": float(np.percentile(soj, 50) * 1e3),
+                "sojourn_p99_ms": float(np.percentile(soj, 99) * 1e3)}
+
+    def run_batched():
+        # each arrival of the Poisson stream is one proposal of an idle chain
+        rng = np.random.default_rng(11)
+        chains = make_chains(5)
+        arrivals = np.cumsum(rng.exponential(1 / rate, int(rate * duration * 2) + 10))
+        sv = Server(hbm_bytes=64 << 30, chunk_intervals=args.chunk)
+        ai, done, lat, soj, pending = 0, 0, [], [], {}
+        t0 = _time.perf_counter()
+        while True:
+            now = _time.perf_counter() - t0
+            if now > duration:
+                break
+            while ai < arrivals.size and arrivals[ai] <= now:
+                idle = [c for c in chains if not c["busy"]]
+                if not idle:
+                    break
+                ch = idle[int(rng.integers(len(idle)))]
+                prop = ch["theta"] + rng.normal(0, 0.02, 2)
+                rid = sv.submit(gens[ch["model"]], pars(ch, prop), ch["P"], 0.0, ch["data"],
+                                dt=0.25, seed=int(rng.integers(0, 2**62)))
+                ch["busy"] = True
+                pending[rid] = (ch, prop, arrivals[ai], _time.perf_counter() - t0)
+                ai += 1
+            if not pending:
+                _time.sleep(1e-4)
+                continue
+            sv.step()
+            t_now = _time.perf_counter() - t0
+            for rid, ll, status, _ in sv.poll():
+                ch, prop, t_arr, t_sub = pending.pop(rid)
+                lat.append(t_now - t_sub)
+                soj.append(t_now - t_arr)
+                accept(rng, ch, prop, ll)
+                ch["busy"] = False
+                done += 1
+        return summary(lat, soj, done, _time.perf_counter() - t0, ai)
+
+    def run_plain():
+        # the plain path: one filter (or unfilter) object per chain, each
+        # request a synchronous dust_likelihood_run, one at a time
+        rng = np.random.default_rng(11)
+        chains = make_chains(5)
+        arrivals = np.cumsum(rng.exponential(1 / rate, int(rate * duration * 2) + 10))
+        ai, done, lat, soj = 0, 0, [], []
+        t0 = _time.perf_counter()
+        while True:
+            now = _time.perf_counter() - t0
+            if now > duration:
+                break
+            if arrivals[ai] > now:
+                _time.sleep(min(arrivals[ai] - now, 1e-3))
+                continue
+            ch = chains[int(rng.integers(n_chains))]
+            prop = ch["theta"] + rng.normal(0, 0.02, 2)
+            if ch["obj"] is None:
+                g = gens[ch["model"]]
+                if ch["model"] == "sirode":
+                    ch["obj"] = lib.dust_unfilter_create(g, 0.0, ch["data"])
+                else:
+                    ch["obj"] = lib.dust_filter_create(g, 0.0, ch["data"], ch["P"], dt=0.25,
+                                                       seed=int(rng.integers(0, 2**62)),
+                                                       precision="fp32")
+            t_sub = _time.perf_counter()
+            ll = lib.dust_likelihood_run(ch["obj"], pars(ch, prop))
+            lat.append(_time.perf_counter() - t_sub)
+            soj.append(_time.perf_counter() - t0 - arrivals[ai])
+            accept(rng, ch, prop, ll)
+            ai += 1
+            done += 1
+        return summary(lat, soj, done, _time.perf_counter() - t0,
+                       int(np.sum(arrivals <= duration)))
+
+    batched = run_batched()
+    plain = run_plain()
+    return {"metric": "likelihood evals/s", "value": batched["evals_per_s"], "unit": "evals/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": duration * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (random-walk MCMC proposals on seeded Poisson data)",
+            "config": {"workload": "continuous-batching likelihood server", "chains": n_chains,
+                       "arrival_rate_per_s": rate, "chunk_intervals": args.chunk,
+                       "seconds": duration},
+            "batched": batched, "plain_one_at_a_time": plain,
+            "batched_over_plain": batched["evals_per_s"] / max(plain["evals_per_s"], 1e-9)}
 
 
 def main():
@@ -73,92 +467,35 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="sir",
+                    choices=["sir", "sir-replicates", "sirode", "seir16", "serving"])
     ap.add_argument("--groups", type=int, default=512)
     ap.add_argument("--particles", type=int, default=2048)
     ap.add_argument("--n-obs", type=int, default=365)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--n-obs-seir", type=int, default=200)
+    ap.add_argument("--precision", default=None)
     ap.add_argument("--launch", default=None, help="block,particles_per_thread")
     ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "pipeline"])
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=16)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(groups shard across GPUs with no communication, "
-                                   "see DESIGN.md)"}))
+                                   "(parameter groups shard across GPUs with no "
+                                   "communication during a run)"}))
         sys.exit(2)
+    args.precision_set = args.precision is not None
+    if args.precision is None:
+        args.precision = "fp32"
 
     import dust2_amd as lib
     from dust2_amd._lib import call
     call("dust2_amd_init")
-
-    dt = 0.25
-    G, P = args.groups, args.particles
-    pars, time = make_workload(G, args.n_obs)
-    data = simulate_data(lib, pars, time, dt, seed=42)
-    obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
-                                 n_groups=G, dt=dt, seed=1, precision=args.precision)
-    if args.launch:
-        obj.set_launch(*[int(x) for x in args.launch.split(",")])
-    obj.set_engine(args.engine, graph=not args.no_graph)
-    plist = [dict(zip(SIR_NAMES, p)) for p in pars]
-    # every call passes the parameters, as an MCMC driver does (the update
-    # is host-side bookkeeping)
-    for _ in range(args.warmup):
-        lib.dust_likelihood_run(obj, plist)
-    kernel_ms = []
-    t0 = _time.perf_counter()
-    for _ in range(args.steps):
-        ll = lib.dust_likelihood_run(obj, plist)
-        kernel_ms.append(lib.dust_likelihood_last_stats(obj)["kernel_ms"])
-    elapsed = _time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    n_steps = int(round(time[-1] / dt))
-    psteps = G * P * n_steps
-    value = psteps * args.steps / elapsed
-    s_x = 4 if args.precision == "fp32" else 8
-    # algorithmic bytes per particle per observation: state gather-read and
-    # write, xoshiro256+ read and write, FP64 weight (lambda) write and read,
-    # u32 ancestor write and read
-    b_obs = 2 * 5 * s_x + 2 * 32 + 2 * 8 + 2 * 4
-    stats = lib.dust_likelihood_last_stats(obj)
-    engine = stats["engine"]
-    if engine == "persistent":
-        kernel = "sir_filter_kernel"
-        k_ms = float(np.mean(kernel_ms))
-        b_launch = G * P * time.size * b_obs
-    else:
-        # the advance kernel moves the state, RNG, lambda write and
-        # ancestor read of one interval; timed per launch with events in
-        # one extra profiled run
-        kernel = "sir_advance_kernel"
-        obj.set_engine("pipeline", graph=False, profile=True)
-        lib.dust_likelihood_run(obj, plist)
-        k_ms = lib.dust_likelihood_last_stats(obj)["advance_ms"]
-        obj.set_engine(args.engine, graph=not args.no_graph)
-        b_launch = G * P * (2 * 5 * s_x + 2 * 32 + 8 + 4)
-    achieved = b_launch / (k_ms * 1e-3) / 1e9
-    out = {
-        "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
-        "data": "synthetic (seeded simulation of each group's model)",
-        "config": {"workload": "batched stochastic SIR bootstrap filter",
-                   "groups": G, "particles": P, "n_obs": int(time.size), "dt": dt,
-                   "precision": args.precision, "launch": obj.launch, "engine": engine,
-                   "graph": not args.no_graph},
-        "likelihood_evals_per_s": G * args.steps / elapsed,
-        "gpu_ms_per_step": float(np.mean(kernel_ms)),
-        "mean_ll": float(np.mean(ll)),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": kernel, "kernel_ms": k_ms, "bytes_per_launch": b_launch,
-                     "bytes_per_particle_obs": b_obs},
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pars, time, dt, data, P)
-    print(json.dumps(out))
+    fn = {"sir": bench_sir, "sir-replicates": bench_sir_replicates, "sirode": bench_sirode,
+          "seir16": bench_seir16, "serving": bench_serving}[args.workload]
+    print(json.dumps(fn(lib, args)))
 
 
 if __name__ == "__main__":
