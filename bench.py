@@ -100,10 +100,26 @@ def simulate_seir_data(lib, plist, time, dt, seed):
     return np.ascontiguousarray(tr[:, :, 0, 64:80])            # (T, G, 16)
 
 
-def roofline(achieved, kernel, k_ms, b_launch, b_obs):
+def measured_traffic(key, kernel):
+    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+    profile of this workload and launch shape, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            tab = json.load(f)
+    except OSError:
+        return None
+    e = tab.get(key)
+    if e is None or not e["kernel"].startswith(kernel):
+        return None
+    return 1024 * (tab["fetch_scale"] * e["fetch_kib"] + tab["write_scale"] * e["write_kib"])
+
+
+def roofline(achieved, kernel, k_ms, b_launch, b_obs, traffic_key=None):
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
-            "kernel_ms": k_ms, "bytes_per_launch": b_launch, "bytes_per_particle_obs": b_obs}
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(traffic_key, kernel) if traffic_key else None,
+            "kernel": kernel, "kernel_ms": k_ms, "bytes_per_launch": b_launch,
+            "bytes_per_particle_obs": b_obs}
 
 
 def timed_runs(lib, obj, plist, steps, warmup):
@@ -117,12 +133,14 @@ def timed_runs(lib, obj, plist, steps, warmup):
     return np.atleast_1d(ll), _time.perf_counter() - t0, gpu_ms
 
 
-def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms):
+def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, key=None):
     b_obs = 2 * S * s_x + 2 * 32 + 2 * 8 + 2 * 4
     if engine_used == "persistent":
         b_launch = G * P * n_obs * b_obs
-        return roofline(b_launch / (run_ms * 1e-3) / 1e9, "sir_filter_kernel", run_ms, b_launch,
-                        b_obs)
+        kernel = "sir_filter_kernel<%s, %d, %d>" % (
+            ("float" if s_x == 4 else "double",) + lib.dust_likelihood_last_stats(obj)["launch"])
+        return roofline(b_launch / (run_ms * 1e-3) / 1e9, kernel, run_ms, b_launch, b_obs,
+                        key)
     # pipeline: the advance kernel moves state, RNG, weight write and
     # ancestor read of one interval; timed per launch in one profiled run
     obj.set_engine("pipeline", graph=False, profile=True)
@@ -156,7 +174,7 @@ def bench_sir(lib, args):
     stats = lib.dust_likelihood_last_stats(obj)
     s_x = 4 if args.precision == "fp32" else 8
     rl = filter_roofline(lib, obj, plist, G, P, time.size, 5, s_x, stats["engine"],
-                         float(np.mean(gpu_ms)))
+                         float(np.mean(gpu_ms)), key=f"sir:{G}x{P}:{args.precision}")
     out = {
         "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -206,7 +224,7 @@ def bench_sir_replicates(lib, args):
     stats = lib.dust_likelihood_last_stats(obj)
     s_x = 8 if prec == "fp64" else 4
     rl = filter_roofline(lib, obj, plist, G, P, time.size, 5, s_x, stats["engine"],
-                         float(np.mean(gpu_ms)))
+                         float(np.mean(gpu_ms)), key=f"sir-replicates:{G}x{P}:{prec}")
     out = {"metric": "likelihood evals/s", "value": G * args.steps / elapsed,
            "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,

@@ -1042,6 +1042,8 @@ int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errl
     out[2] = h->last_engine;
     out[3] = h->last_advance_ms;
     out[4] = h->last_resample_ms;
+    out[5] = h->cfg.block;
+    out[6] = h->cfg.ppt;
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

@@ -314,12 +314,12 @@ def dust_likelihood_set_rng_state(obj, state):
 
 
 def dust_likelihood_last_stats(obj):
-    out = np.zeros(5)
+    out = np.zeros(7)
     call("dust2_filter_last_stats", obj.ptr, dptr(out))
     engines = {1: "persistent", 2: "pipeline"}
     return {"n_fallback": int(out[0]), "kernel_ms": float(out[1]),
             "engine": engines.get(int(out[2]), "none"), "advance_ms": float(out[3]),
-            "resample_ms": float(out[4])}
+            "resample_ms": float(out[4]), "launch": (int(out[5]), int(out[6]))}
 
 
 # ---------------------------------------------------------------------------

@@ -123,10 +123,11 @@ int dust2_filter_set_rng_state(dust2_filter* h, const uint64_t* state, char* err
 int dust2_filter_set_launch(dust2_filter* h, int block, int particles_per_thread,
                                 char* err, size_t errlen);
 
-/* Statistics of the last run (out[5]): [0] resampling rounds that needed the
+/* Statistics of the last run (out[7]): [0] resampling rounds that needed the
  * sequential sum, [1] GPU milliseconds of the run (device events), [2] the
  * engine used, [3] / [4] mean advance / resample kernel milliseconds per
- * interval (pipeline engine with profiling on). */
+ * interval (pipeline engine with profiling on), [5] / [6] the persistent
+ * engine's threads per workgroup and particles per thread. */
 int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
 
 /* Engine selection (DUST2_ENGINE_*), HIP-graph replay of the pipeline
