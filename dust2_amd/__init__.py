@@ -12,9 +12,11 @@ from .interface import (  # noqa: F401
     dust_likelihood_last_ancestry,
     dust_ode_control,
     dust_unfilter_create,
+    dust_likelihood_last_gradient,
     dust_likelihood_last_state,
     dust_likelihood_last_stats,
     dust_likelihood_last_trajectories,
+    dust_likelihood_last_trajectories_raw,
     dust_likelihood_rng_state,
     dust_likelihood_run,
     dust_likelihood_set_rng_state,
@@ -32,6 +34,7 @@ from .interface import (  # noqa: F401
     dust_system_time,
     dust_system_update_pars,
     gpu_resample,
+    gpu_trajectory_lineage,
     gpu_rng_streams,
     gpu_sample,
     seir16,

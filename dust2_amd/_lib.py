@@ -35,7 +35,8 @@ _PROTOS = {
     "dust2_filter_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_filter_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_filter_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_last_trajectories": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
+    "dust2_filter_last_trajectories_raw": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_last_ancestry": (C.c_int, [VP, U32, ERR, ERRLEN]),
     "dust2_filter_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
@@ -67,6 +68,14 @@ _PROTOS = {
     "dust2_unfilter_sirode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_stats": (C.c_int, [VP, D, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_free": (None, [VP]),
+    "dust2_unfilter_sir_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
+                                           C.c_size_t, C.c_double, D, C.c_size_t,
+                                           C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_unfilter_sir_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sir_last_gradient": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_sir_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
+    "dust2_unfilter_sir_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_sir_free": (None, [VP]),
     "dust2_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(VP), ERR, ERRLEN]),
     "dust2_server_submit": (C.c_int, [VP, C.c_int, D, C.c_size_t, C.c_double, D, C.c_size_t,
                                       C.c_double, D, C.c_uint64, U64, ERR, ERRLEN]),
@@ -76,6 +85,8 @@ _PROTOS = {
     "dust2_server_memory": (C.c_int, [VP, SZ, SZ, ERR, ERRLEN]),
     "dust2_server_free": (None, [VP]),
     "dust2_amd_resample": (C.c_int, [D, C.c_size_t, C.c_double, U32, D, ERR, ERRLEN]),
+    "dust2_amd_trajectory_lineage": (C.c_int, [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
+                                               C.c_size_t, SZ, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,
                                         ERRLEN]),
     "dust2_amd_sample": (C.c_int, [C.c_int, C.c_int, U64, C.c_size_t, C.c_size_t, D, D, C.c_int,

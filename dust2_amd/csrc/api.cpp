@@ -370,6 +370,9 @@ struct dust2_filter {
   std::vector<uint64_t> graph_key;
   bool traj_valid = false;
   bool ancestry_valid = false;
+  std::vector<size_t> last_index;        // groups of the last run
+  std::vector<uint8_t> select_valid;     // per group: a random particle is chosen
+  dbuf d_select, d_groups, d_traj_out;   // lineage export
   double last_ms = 0;
   double last_fallbacks = 0;
   double last_advance_ms = 0;   // mean advance-kernel time (profile mode)
@@ -876,6 +879,10 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     h->last_fallbacks = counters[1];
     h->traj_valid = save_trajectories != 0;
     h->ancestry_valid = save_trajectories != 0;
+    h->last_index = idx;
+    // a new run forgets the chosen particles of its groups (filter.hpp:162-175)
+    if (h->select_valid.size() != G) h->select_valid.assign(G, 0);
+    for (size_t g : idx) h->select_valid[g] = 0;
     if (counters[0] > 0) {
       std::vector<error_record> rec(G * P);
       to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
@@ -934,12 +941,69 @@ int dust2_filter_last_state(dust2_filter* h, double* out, char* err, size_t errl
   }
 }
 
-int dust2_filter_last_trajectories(dust2_filter* h, double* out, char* err,
+namespace {
+// Groups of the last run as u32 on the device; chosen particles for them
+// (drawn once per run, filter.hpp:130-138) when `select` is set.
+const uint32_t* lineage_groups(dust2_filter* h, bool select) {
+  std::vector<uint32_t> gs(h->last_index.begin(), h->last_index.end());
+  if (h->d_groups.bytes() < h->G * sizeof(uint32_t)) h->d_groups.alloc(h->G * sizeof(uint32_t));
+  to_device(h->d_groups, gs.data(), gs.size() * sizeof(uint32_t), h->stream);
+  if (select) {
+    if (h->d_select.bytes() < h->G * sizeof(uint32_t)) {
+      h->d_select.alloc(h->G * sizeof(uint32_t));
+    }
+    std::vector<uint32_t> todo;
+    for (size_t g : h->last_index) {
+      if (!h->select_valid[g]) todo.push_back(static_cast<uint32_t>(g));
+    }
+    if (!todo.empty()) {
+      dbuf d_todo;
+      d_todo.alloc(todo.size() * sizeof(uint32_t));
+      to_device(d_todo, todo.data(), todo.size() * sizeof(uint32_t), h->stream);
+      hip_check(launch_select_particle(h->d_rng.get<uint64_t>(), d_todo.get<uint32_t>(),
+                                       static_cast<int>(todo.size()), static_cast<int>(h->P),
+                                       h->d_select.get<uint32_t>(), h->stream),
+                "select particle kernel");
+      hip_check(hipStreamSynchronize(h->stream), "sync");
+      for (uint32_t g : todo) h->select_valid[g] = 1;
+    }
+  }
+  return h->d_groups.get<uint32_t>();
+}
+}  // namespace
+
+int dust2_filter_last_trajectories(dust2_filter* h, int select_random_particle, double* out,
+                                   char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (!h->traj_valid) {
+      throw std::runtime_error("Trajectories are not current");
+    }
+    const bool select = select_random_particle != 0;
+    const uint32_t* groups = lineage_groups(h, select);
+    const size_t n_out = h->last_index.size();
+    const size_t p_out = select ? 1 : h->P;
+    const size_t bytes = h->n_time * n_out * p_out * h->S * sizeof(double);
+    if (h->d_traj_out.bytes() < bytes) h->d_traj_out.alloc(bytes);
+    hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(), groups,
+                             static_cast<int>(n_out), static_cast<int>(h->n_time),
+                             static_cast<int>(h->S), static_cast<int>(h->P),
+                             select ? h->d_select.get<uint32_t>() : nullptr, h->fp64,
+                             h->d_traj_out.get<double>(), h->stream),
+              "lineage kernel");
+    to_host(out, h->d_traj_out, bytes, h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_last_trajectories_raw(dust2_filter* h, double* out, char* err,
                                        size_t errlen) {
   try {
     check_filter(h);
     if (!h->traj_valid) {
-      throw std::runtime_error("Trajectories are not current; run with 'save_trajectories'");
+      throw std::runtime_error("Trajectories are not current");
     }
     // device: [G][n_time][S][P]
     const size_t G = h->G, P = h->P, T = h->n_time;
@@ -1409,6 +1473,59 @@ int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* 
   }
 }
 
+int dust2_amd_trajectory_lineage(const double* raw, const uint32_t* ancestry, size_t n_time,
+                                 size_t n_groups, size_t n_particles, size_t n_state,
+                                 const size_t* select, double* out, char* err, size_t errlen) {
+  try {
+    if (n_time == 0 || n_groups == 0 || n_particles == 0 || n_state == 0) {
+      throw std::runtime_error("sizes must be positive");
+    }
+    const size_t T = n_time, G = n_groups, P = n_particles, S = n_state;
+    // caller layout [t][g][p][s] -> device [g][t][s][p]; ancestry [t][g][p] -> [g][t][p]
+    std::vector<double> traj(T * G * P * S);
+    std::vector<uint32_t> anc(ancestry ? T * G * P : 0);
+    for (size_t t = 0; t < T; ++t) {
+      for (size_t g = 0; g < G; ++g) {
+        for (size_t p = 0; p < P; ++p) {
+          for (size_t c = 0; c < S; ++c) {
+            traj[((g * T + t) * S + c) * P + p] = raw[((t * G + g) * P + p) * S + c];
+          }
+          if (ancestry) {
+            const uint32_t a = ancestry[(t * G + g) * P + p];
+            if (a >= P) throw std::runtime_error("ancestor index out of range");
+            anc[(g * T + t) * P + p] = a;
+          }
+        }
+      }
+    }
+    std::vector<uint32_t> groups(G), sel(G);
+    for (size_t g = 0; g < G; ++g) {
+      groups[g] = static_cast<uint32_t>(g);
+      if (select) {
+        if (select[g] >= P) throw std::runtime_error("selected particle out of range");
+        sel[g] = static_cast<uint32_t>(select[g]);
+      }
+    }
+    hipStream_t s = nullptr;
+    const size_t p_out = select ? 1 : P;
+    dbuf dt(traj.size() * sizeof(double)), da(std::max<size_t>(anc.size(), 1) * 4),
+        dg(G * 4), ds(G * 4), dout(T * G * p_out * S * sizeof(double));
+    to_device(dt, traj.data(), traj.size() * sizeof(double), s);
+    if (ancestry) to_device(da, anc.data(), anc.size() * 4, s);
+    to_device(dg, groups.data(), G * 4, s);
+    to_device(ds, sel.data(), G * 4, s);
+    hip_check(launch_lineage(dt.get(), ancestry ? da.get<uint32_t>() : nullptr,
+                             dg.get<uint32_t>(), static_cast<int>(G), static_cast<int>(T),
+                             static_cast<int>(S), static_cast<int>(P),
+                             select ? ds.get<uint32_t>() : nullptr, true, dout.get<double>(), s),
+              "lineage kernel");
+    to_host(out, dout, T * G * p_out * S * sizeof(double), s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                           size_t n_streams, uint64_t* out, char* err, size_t errlen) {
   try {
@@ -1671,6 +1788,273 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
 }
 
 void dust2_unfilter_sirode_free(dust2_unfilter* h) { delete h; }
+
+}  // extern "C"
+
+// ============================================================================
+// Deterministic discrete-time SIR unfilter with the adjoint gradient
+// ============================================================================
+struct dust2_unfilter_sir {
+  size_t G = 0, P = 0, n_time = 0, n_groups_data = 0, total_steps = 0;
+  double time_start = 0, dt = 1;
+  std::vector<double> pars;           // G x 5 (host rows)
+  std::vector<double> times;
+  std::vector<uint8_t> grad_current;  // per group: gradient of the last run is valid
+  std::vector<size_t> last_index;     // groups of the last run
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  dbuf d_pars, d_sched_time, d_sched_steps, d_obs, d_lfact, d_state, d_ll, d_errors,
+      d_counters, d_active, d_hist, d_inc, d_grad;
+  ~dust2_unfilter_sir() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+void check_unfilter_sir(dust2_unfilter_sir* h) {
+  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (unfilter)");
+}
+
+// N, I0, beta, gamma, exp_noise -> N, I0, beta, gamma, exp_noise,
+// 1 - exp(-gamma dt), exp(-gamma dt), 0
+void sir_adjoint_pars(const double* row, double dt, double* out) {
+  for (int i = 0; i < 5; ++i) out[i] = row[i];
+  out[5] = 1 - std::exp(-row[3] * dt);
+  out[6] = std::exp(-row[3] * dt);
+  out[7] = 0;
+}
+
+void upload_sir_adjoint_pars(dust2_unfilter_sir* h) {
+  std::vector<double> kp(h->G * 8);
+  for (size_t g = 0; g < h->G; ++g) sir_adjoint_pars(&h->pars[g * kSirPars], h->dt, &kp[g * 8]);
+  to_device(h->d_pars, kp.data(), kp.size() * sizeof(double), h->stream);
+}
+}  // namespace
+
+extern "C" {
+
+int dust2_unfilter_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                             double time_start, const double* time, size_t n_time, double dt,
+                             const double* data, size_t n_groups_data,
+                             dust2_unfilter_sir** out, char* err, size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    check_dt(dt);
+    check_time_sequence(time_start, time, n_time);
+    if (n_groups_data != 1 && n_groups_data != n_groups) {
+      throw std::runtime_error("Expected 'data' to have 1 or " + std::to_string(n_groups) +
+                               " groups, but it had " + std::to_string(n_groups_data));
+    }
+    if (n_groups * n_particles > (size_t(1) << 31)) {
+      throw std::runtime_error("too many particles");
+    }
+    for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * kSirPars);
+    auto h = std::make_unique<dust2_unfilter_sir>();
+    h->G = n_groups;
+    h->P = n_particles;
+    h->n_time = n_time;
+    h->n_groups_data = n_groups_data;
+    h->time_start = time_start;
+    h->dt = dt;
+    h->pars.assign(pars, pars + n_groups * kSirPars);
+    h->times.assign(time, time + n_time);
+    h->grad_current.assign(n_groups, 0);
+    // interval schedule of the system's run_to_time calls
+    // (dust2/discrete/system.hpp:61-92): steps = round(max(0, t - time) / dt),
+    // time advanced by steps * dt
+    std::vector<double> st(n_time);
+    std::vector<uint32_t> ns(n_time);
+    double tcur = time_start;
+    for (size_t t = 0; t < n_time; ++t) {
+      const double n = std::round(std::max(0.0, time[t] - tcur) / dt);
+      st[t] = tcur;
+      ns[t] = static_cast<uint32_t>(n);
+      tcur = tcur + n * dt;
+      h->total_steps += ns[t];
+    }
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
+    const size_t n_total = n_groups * n_particles;
+    const size_t nd = n_time * n_groups_data;
+    std::vector<double> lfact(nd);
+    for (size_t i = 0; i < nd; ++i) lfact[i] = std::isnan(data[i]) ? 0.0 : std::lgamma(data[i] + 1);
+    h->d_pars.alloc(n_groups * 8 * sizeof(double));
+    h->d_sched_time.alloc(n_time * sizeof(double));
+    h->d_sched_steps.alloc(n_time * sizeof(uint32_t));
+    h->d_obs.alloc(nd * sizeof(double));
+    h->d_lfact.alloc(nd * sizeof(double));
+    h->d_state.alloc(n_total * 5 * sizeof(double));
+    h->d_ll.alloc(n_total * sizeof(double));
+    h->d_errors.alloc(n_total * sizeof(error_record));
+    h->d_counters.alloc(4 * sizeof(uint32_t));
+    h->d_active.alloc(n_groups);
+    upload_sir_adjoint_pars(h.get());
+    to_device(h->d_sched_time, st.data(), n_time * sizeof(double), h->stream);
+    to_device(h->d_sched_steps, ns.data(), n_time * sizeof(uint32_t), h->stream);
+    to_device(h->d_obs, data, nd * sizeof(double), h->stream);
+    to_device(h->d_lfact, lfact.data(), nd * sizeof(double), h->stream);
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adjoint,
+                           const size_t* index_group, size_t n_index, double* ll_out,
+                           char* err, size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    hipStream_t s = h->stream;
+    const size_t n_total = h->G * h->P;
+    std::vector<uint8_t> active(h->G, 0);
+    for (size_t g : idx) active[g] = 1;
+    to_device(h->d_active, active.data(), h->G, s);
+    if (initial != nullptr) {
+      // [G][P][S] -> [S][G*P]
+      std::vector<double> tmp(n_total * 5);
+      for (size_t k = 0; k < n_total; ++k) {
+        for (size_t c = 0; c < 5; ++c) tmp[c * n_total + k] = initial[k * 5 + c];
+      }
+      to_device(h->d_state, tmp.data(), tmp.size() * sizeof(double), s);
+    }
+    if (adjoint) {
+      // history of S and I at every step and incidence at every data time
+      // (dust2/adjoint_data.hpp:28-46), allocated on first use
+      const size_t hist_bytes = std::max<size_t>(h->total_steps, 1) * 2 * n_total * sizeof(double);
+      if (h->d_hist.bytes() != hist_bytes) {
+        h->d_hist.alloc(hist_bytes);
+        h->d_inc.alloc(h->n_time * n_total * sizeof(double));
+        h->d_grad.alloc(n_total * 3 * sizeof(double));
+      }
+    }
+    hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+    sir_unfilter_launch a{};
+    a.pars = h->d_pars.get<double>();
+    a.active = h->d_active.get<uint8_t>();
+    a.n_particles = static_cast<int>(h->P);
+    a.n_total = static_cast<int>(n_total);
+    a.set_initial = initial == nullptr ? 1 : 0;
+    a.dt = h->dt;
+    a.n_time = static_cast<int>(h->n_time);
+    a.sched_time = h->d_sched_time.get<double>();
+    a.sched_steps = h->d_sched_steps.get<uint32_t>();
+    a.obs = h->d_obs.get<double>();
+    a.lfact = h->d_lfact.get<double>();
+    a.obs_stride = h->n_groups_data;
+    a.obs_group_stride = h->n_groups_data == 1 ? 0 : 1;
+    a.state = h->d_state.get<double>();
+    a.ll = h->d_ll.get<double>();
+    a.hist = adjoint ? h->d_hist.get<double>() : nullptr;
+    a.inc = adjoint ? h->d_inc.get<double>() : nullptr;
+    a.grad = adjoint ? h->d_grad.get<double>() : nullptr;
+    a.errors = h->d_errors.get<error_record>();
+    a.error_count = h->d_counters.get<uint32_t>();
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    hip_check(launch_sir_unfilter(a, s), "sir unfilter kernel");
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+    uint32_t count = 0;
+    to_host(&count, h->d_counters, sizeof(count), s);
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    h->last_ms = ms;
+    for (size_t g : idx) h->grad_current[g] = 0;
+    if (count > 0) {
+      std::vector<error_record> rec(n_total);
+      to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+      hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+      hip_check(hipStreamSynchronize(s), "sync");
+      throw std::runtime_error(error_report(rec, count));
+    }
+    if (adjoint) {
+      for (size_t g : idx) h->grad_current[g] = 1;
+    }
+    h->last_index = idx;
+    std::vector<double> ll(n_total);
+    to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), s);
+    for (size_t i = 0; i < idx.size(); ++i) {
+      std::memcpy(ll_out + i * h->P, ll.data() + idx[i] * h->P, h->P * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_last_gradient(dust2_unfilter_sir* h, double* out, char* err,
+                                     size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    bool any = false;
+    for (size_t g : h->last_index) any = any || h->grad_current[g];
+    if (!any) {
+      throw std::runtime_error("System was not run with 'adjoint = TRUE'");
+    }
+    if (h->P > 1) {
+      throw std::runtime_error("n_particles > 1 not yet supported; see mrc-5406");
+    }
+    std::vector<double> grad(h->G * 3);
+    to_host(grad.data(), h->d_grad, grad.size() * sizeof(double), h->stream);
+    for (size_t i = 0; i < h->last_index.size(); ++i) {
+      std::memcpy(out + 3 * i, grad.data() + 3 * h->last_index[i], 3 * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_update_pars(dust2_unfilter_sir* h, const double* pars,
+                                   const size_t* index_group, size_t n_index, char* err,
+                                   size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    for (size_t i = 0; i < idx.size(); ++i) {
+      const double* row = pars + i * kSirPars;
+      check_sir_pars(row);
+      double* dst = &h->pars[idx[i] * kSirPars];
+      // I0, beta and gamma update; N and exp_noise are constant
+      // (inst/examples/sir.cpp:89-93)
+      dst[1] = row[1];
+      dst[2] = row[2];
+      dst[3] = row[3];
+      h->grad_current[idx[i]] = 0;
+    }
+    upload_sir_adjoint_pars(h);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_last_state(dust2_unfilter_sir* h, double* out, char* err,
+                                  size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    const size_t n = h->G * h->P;
+    std::vector<double> tmp(n * 5);
+    to_host(tmp.data(), h->d_state, tmp.size() * sizeof(double), h->stream);
+    for (size_t k = 0; k < n; ++k) {
+      for (size_t c = 0; c < 5; ++c) out[k * 5 + c] = tmp[c * n + k];
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_unfilter_sir_free(dust2_unfilter_sir* h) { delete h; }
 
 }  // extern "C"
 

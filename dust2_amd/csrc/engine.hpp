@@ -179,6 +179,44 @@ struct ode_launch {
 
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream);
 
+// ---- deterministic discrete SIR unfilter and adjoint -------------------------
+struct sir_unfilter_launch {
+  const double* pars;           // [G][8]: N, I0, beta, gamma, exp_noise, p_IR, e_IR, -
+  const uint8_t* active;        // [G] groups to run, or null for all
+  int n_particles;
+  int n_total;                  // G * P
+  int set_initial;              // 1: model initial state, 0: start from `state`
+  double dt;
+  int n_time;
+  const double* sched_time;     // [n_time] time at the start of interval i
+  const uint32_t* sched_steps;  // [n_time] steps in interval i
+  const double* obs;            // obs[i * obs_stride + g * obs_group_stride]
+  const double* lfact;          // same layout
+  size_t obs_stride;
+  size_t obs_group_stride;
+  double* state;                // [S][n_total] in/out
+  double* ll;                   // [n_total]
+  double* hist;                 // [total steps][2][n_total] or null (no gradient)
+  double* inc;                  // [n_time][n_total] or null
+  double* grad;                 // [n_total][3] or null
+  error_record* errors;         // [n_total]
+  uint32_t* error_count;
+};
+
+hipError_t launch_sir_unfilter(const sir_unfilter_launch& args, hipStream_t stream);
+
+// ---- trajectories ------------------------------------------------------------
+// Lineage export: traj real [G][T][S][P], ancestry [G][T][P] or null,
+// groups[n_groups_out], select [G] chosen particle or null (all particles);
+// out double [T][n_groups_out][P_out][S].
+hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint32_t* groups,
+                          int n_groups_out, int T, int S, int P, const uint32_t* select,
+                          bool fp64, double* out, hipStream_t stream);
+// select[g] = floor(u * P), u from filter stream 2g + 1 of the [G][4][P + 2]
+// stream blocks, for the n listed groups.
+hipError_t launch_select_particle(uint64_t* rng, const uint32_t* groups, int n, int P,
+                                  uint32_t* select, hipStream_t stream);
+
 // ---- system (state resident in HBM between calls) -------------------------
 // state: real [G][S][P]; rng: [G][4][P]; pars: [G][8] doubles.
 hipError_t launch_sir_run_to_time(void* state, uint64_t* rng, const double* pars, int G,

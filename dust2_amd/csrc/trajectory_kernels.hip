@@ -1,0 +1,95 @@
+// Trajectory export following the particle lineage, and the random choice
+// of one particle per group.
+//
+// Replaces trajectories<T>::export_state with reorder = true
+// (dust2/trajectories.hpp:128-189, reorder_state_group_ / reorder_state_single_
+// at :313-343) and filter<T>::select_random_particle (dust2/filter.hpp:130-138).
+//
+// The filter saves, at every data time t, the post-resampling state
+// [t][S][P] of each group and the ancestor index used by that resampling
+// [t][P].  The exported trajectory of final particle k is the state of k at
+// the last time, then of ancestor(k) at the time before, and so on back to
+// the first time: one thread per exported particle walks the ancestry
+// backwards, gathering one state vector per time.
+#include <hip/hip_runtime.h>
+
+#include "device_random.hpp"
+#include "engine.hpp"
+
+namespace dust2_amd {
+
+namespace {
+
+// traj: real [G][T][S][P]; ancestry: [G][T][P] (null: no resampling ever);
+// groups: the n_groups_out groups to export; select: [G] chosen particle
+// per group, or null for every particle.  out: double
+// [T][n_groups_out][P_out][S], P_out = select ? 1 : P.
+template <typename real>
+__global__ void __launch_bounds__(256)
+    lineage_kernel(const real* __restrict__ traj, const uint32_t* __restrict__ ancestry,
+                   const uint32_t* __restrict__ groups, int n_groups_out, int T, int S, int P,
+                   const uint32_t* __restrict__ select, double* __restrict__ out) {
+  const int p_out = select ? 1 : P;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_groups_out * p_out) return;
+  const int j = i / p_out;          // output group
+  const int k = i - j * p_out;      // output particle
+  const size_t g = groups[j];
+  uint32_t idx = select ? select[g] : static_cast<uint32_t>(k);
+  const real* tg = traj + g * static_cast<size_t>(T) * S * P;
+  const uint32_t* ag = ancestry ? ancestry + g * static_cast<size_t>(T) * P : nullptr;
+  for (int t = T - 1; t >= 0; --t) {
+    const real* ts = tg + static_cast<size_t>(t) * S * P;
+    double* o = out + ((static_cast<size_t>(t) * n_groups_out + j) * p_out + k) * S;
+    for (int c = 0; c < S; ++c) o[c] = static_cast<double>(ts[static_cast<size_t>(c) * P + idx]);
+    if (ag) idx = ag[static_cast<size_t>(t) * P + idx];
+  }
+}
+
+// One draw from filter stream 2g + 1 (column 1 of the group's [4][P + 2]
+// stream block) per listed group: particle floor(u * P).
+__global__ void select_particle_kernel(uint64_t* __restrict__ rng, const uint32_t* groups,
+                                       int n, int P, uint32_t* __restrict__ select) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const size_t g = groups[j];
+  const size_t stride = static_cast<size_t>(P) + 2;
+  uint64_t* r = rng + g * 4 * stride + 1;
+  rng256 s{r[0], r[stride], r[2 * stride], r[3 * stride]};
+  const double u = random_real<double>(s);
+  r[0] = s.s0;
+  r[stride] = s.s1;
+  r[2 * stride] = s.s2;
+  r[3 * stride] = s.s3;
+  select[g] = static_cast<uint32_t>(floor(u * P));
+}
+
+}  // namespace
+
+hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint32_t* groups,
+                          int n_groups_out, int T, int S, int P, const uint32_t* select,
+                          bool fp64, double* out, hipStream_t stream) {
+  const int n = n_groups_out * (select ? 1 : P);
+  const int block = 256;
+  const dim3 grid((n + block - 1) / block);
+  if (fp64) {
+    hipLaunchKernelGGL(lineage_kernel<double>, grid, dim3(block), 0, stream,
+                       static_cast<const double*>(traj), ancestry, groups, n_groups_out, T, S,
+                       P, select, out);
+  } else {
+    hipLaunchKernelGGL(lineage_kernel<float>, grid, dim3(block), 0, stream,
+                       static_cast<const float*>(traj), ancestry, groups, n_groups_out, T, S,
+                       P, select, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_select_particle(uint64_t* rng, const uint32_t* groups, int n, int P,
+                                  uint32_t* select, hipStream_t stream) {
+  const int block = 64;
+  hipLaunchKernelGGL(select_particle_kernel, dim3((n + block - 1) / block), dim3(block), 0,
+                     stream, rng, groups, n, P, select);
+  return hipGetLastError();
+}
+
+}  // namespace dust2_amd

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import call, dptr, f64, szptr, u32ptr, u64ptr
+from ._lib import DustError, call, dptr, f64, szptr, u32ptr, u64ptr
 
 FP64 = 0
 FP32 = 1
@@ -210,6 +210,7 @@ class Filter:
         self.ptr = None
         self.launch = None
         self.engine = None
+        self.last_index = None
 
     @property
     def initialised(self):
@@ -255,11 +256,12 @@ def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=
     return Filter(generator, time_start, data, n_particles, n_groups, dt, seed, precision)
 
 
-def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_group=None):
+def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_group=None,
+                        adjoint=False):
     """Run the filter (or unfilter) and return the log-likelihood per group
     (a float for a single group), R/interface-likelihood.R:48-119."""
     if isinstance(obj, Unfilter):
-        return obj.run(pars, index_group)
+        return obj.run(pars, index_group, initial=initial, adjoint=adjoint)
     if not obj.initialised:
         if pars is None:
             raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
@@ -277,6 +279,7 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
         if init.size != obj.n_groups * obj.n_particles * obj.generator.n_state:
             raise ValueError("Expected 'initial' to have n_groups * n_particles * n_state values")
     ll = np.zeros(n_out)
+    obj.last_index = index_group
     call("dust2_filter_run", obj.ptr, dptr(init), int(bool(save_trajectories)), szptr(idx),
          n_idx, dptr(ll))
     return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
@@ -288,9 +291,25 @@ def dust_likelihood_last_state(obj):
     return out
 
 
-def dust_likelihood_last_trajectories(obj):
+def dust_likelihood_last_trajectories(obj, select_random_particle=False):
+    """R/interface-likelihood.R dust_likelihood_last_trajectories: each
+    particle's trajectory follows its ancestry back from the last time
+    (dust2/trajectories.hpp:128-189); (n_time, n_groups, n_particles or 1,
+    n_state), groups of the last run."""
+    if obj.ptr is None:
+        raise DustError("Trajectories are not current")
+    n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
+    p_out = 1 if select_random_particle else obj.n_particles
+    out = np.zeros((obj.time.size, n_groups, p_out, obj.generator.n_state))
+    call("dust2_filter_last_trajectories", obj.ptr, int(bool(select_random_particle)),
+         dptr(out))
+    return out
+
+
+def dust_likelihood_last_trajectories_raw(obj):
+    """The saved post-resampling states, not following the ancestry."""
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
-    call("dust2_filter_last_trajectories", obj.ptr, dptr(out))
+    call("dust2_filter_last_trajectories_raw", obj.ptr, dptr(out))
     return out
 
 
@@ -436,55 +455,105 @@ def dust_ode_control(max_steps=10000, atol=1e-6, rtol=1e-6, step_size_min=0,
 
 
 class Unfilter:
-    """dust_unfilter_create (R/interface-unfilter.R:20-71) for the SIR ODE;
-    the likelihood of each particle is returned (P identical particles per
-    group give identical values)."""
+    """dust_unfilter_create (R/interface-unfilter.R:20-71): the deterministic
+    likelihood.  For the SIR ODE it integrates with Dormand-Prince 5(4); for
+    the discrete-time SIR it runs the stochastic model with every sampler
+    returning its mean, and can compute the adjoint gradient.  The
+    likelihood of each particle is returned (P identical particles per group
+    give identical values)."""
 
-    def __init__(self, generator, time_start, data, n_particles, n_groups, ode_control,
+    def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control,
                  precision):
-        if generator.name != "sirode":
+        if generator.name == "sirode":
+            if dt is not None:
+                raise ValueError("Can't use 'dt' with continuous-time systems")
+        elif generator.name == "sir":
+            if ode_control is not None:
+                raise ValueError("Can't use 'ode_control' with discrete-time systems")
+            if _precision(precision) != FP64:
+                raise ValueError("The discrete unfilter runs in FP64 only")
+        else:
             raise ValueError(f"No unfilter kernels for '{generator.name}'")
         self.generator = generator
+        self.continuous = generator.name == "sirode"
         self.time_start = float(time_start)
         self.time, self.data, self.n_groups_data = prepare_data(data, n_groups)
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
+        self.dt = 1.0 if dt is None else float(dt)
         self.control = dust_ode_control() if ode_control is None else f64(ode_control)
         self.precision = _precision(precision)
         self.ptr = None
         self.rows = None
+        self.last_index = None
 
     @property
     def initialised(self):
         return self.ptr is not None
 
-    def run(self, pars, index_group=None):
-        idx, n_idx, n_out = _index_group(index_group, self.n_groups)
-        if self.ptr is None:
-            if pars is None:
-                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
-            self.rows = pars_matrix(self.generator, pars, self.n_groups)
-            out = C.c_void_p()
+    def _prefix(self):
+        return "dust2_unfilter_sirode" if self.continuous else "dust2_unfilter_sir"
+
+    def _alloc(self, pars):
+        self.rows = pars_matrix(self.generator, pars, self.n_groups)
+        out = C.c_void_p()
+        if self.continuous:
             call("dust2_unfilter_sirode_alloc", dptr(self.rows), self.n_groups,
                  self.n_particles, self.time_start, dptr(self.time), self.time.size,
                  dptr(self.data), self.n_groups_data, dptr(self.control), self.precision,
                  C.byref(out))
-            self.ptr = out
+        else:
+            call("dust2_unfilter_sir_alloc", dptr(self.rows), self.n_groups, self.n_particles,
+                 self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),
+                 self.n_groups_data, C.byref(out))
+        self.ptr = out
+
+    def run(self, pars, index_group=None, initial=None, adjoint=False):
+        idx, n_idx, n_out = _index_group(index_group, self.n_groups)
+        if self.ptr is None:
+            if pars is None:
+                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+            self._alloc(pars)
         elif pars is not None:
             sel = np.arange(self.n_groups) if idx is None else idx.astype(np.int64)
             rows = pars_update(self.generator, pars, self.rows[sel])
-            call("dust2_unfilter_sirode_update_pars", self.ptr, dptr(rows), szptr(idx), n_idx)
+            call(self._prefix() + "_update_pars", self.ptr, dptr(rows), szptr(idx), n_idx)
             self.rows[sel] = rows
         ll = np.zeros((n_out, self.n_particles))
-        call("dust2_unfilter_sirode_run", self.ptr, szptr(idx), n_idx, dptr(ll))
+        if self.continuous:
+            if adjoint:
+                raise ValueError("The ODE unfilter has no adjoint; use the discrete model")
+            if initial is not None:
+                raise ValueError("'initial' is not supported for the ODE unfilter")
+            call("dust2_unfilter_sirode_run", self.ptr, szptr(idx), n_idx, dptr(ll))
+        else:
+            init = None
+            if initial is not None:
+                init = np.ascontiguousarray(np.broadcast_to(
+                    f64(initial), (self.n_groups, self.n_particles, 5)))
+            call("dust2_unfilter_sir_run", self.ptr, dptr(init), int(bool(adjoint)),
+                 szptr(idx), n_idx, dptr(ll))
+        self.last_index = index_group
         if self.n_particles == 1:
             ll = ll[:, 0]
         return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1 and index_group is None) \
             else ll
 
+    def last_gradient(self):
+        """dust_likelihood_last_gradient (R/interface-likelihood.R): d ll /
+        d (beta, gamma, I0) per group of the last run."""
+        if self.continuous:
+            raise ValueError("The ODE unfilter has no adjoint")
+        if self.ptr is None:
+            raise ValueError("System was not run with 'adjoint = TRUE'")
+        n = self.n_groups if self.last_index is None else np.size(self.last_index)
+        out = np.zeros((n, 3))
+        call("dust2_unfilter_sir_last_gradient", self.ptr, dptr(out))
+        return out[0] if (self.n_groups == 1 and self.last_index is None) else out
+
     def last_state(self):
         out = np.zeros((self.n_groups, self.n_particles, 5))
-        call("dust2_unfilter_sirode_last_state", self.ptr, dptr(out))
+        call(self._prefix() + "_last_state", self.ptr, dptr(out))
         return out
 
     def last_stats(self):
@@ -495,13 +564,18 @@ class Unfilter:
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and _lib._lib is not None:
-            _lib.lib().dust2_unfilter_sirode_free(self.ptr)
+            getattr(_lib.lib(), self._prefix() + "_free")(self.ptr)
             self.ptr = None
 
 
-def dust_unfilter_create(generator, time_start, data, n_particles=1, n_groups=1,
+def dust_unfilter_create(generator, time_start, data, n_particles=1, n_groups=1, dt=None,
                          ode_control=None, precision="fp64"):
-    return Unfilter(generator, time_start, data, n_particles, n_groups, ode_control, precision)
+    return Unfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control,
+                    precision)
+
+
+def dust_likelihood_last_gradient(obj):
+    return obj.last_gradient()
 
 
 # ---------------------------------------------------------------------------
@@ -513,6 +587,19 @@ def gpu_resample(log_weights, u):
     out = np.zeros(3)
     call("dust2_amd_resample", dptr(w), w.size, float(u), u32ptr(idx), dptr(out))
     return idx, out
+
+
+def gpu_trajectory_lineage(raw, ancestry=None, select=None):
+    """The lineage-export kernel on given states: raw (T, G, P, S),
+    ancestry (T, G, P) or None, select (G,) or None."""
+    raw = f64(raw)
+    T, G, P, S = raw.shape
+    anc = None if ancestry is None else np.ascontiguousarray(ancestry, dtype=np.uint32)
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.uintp)
+    out = np.zeros((T, G, 1 if sel is not None else P, S))
+    call("dust2_amd_trajectory_lineage", dptr(raw), u32ptr(anc), T, G, P, S, szptr(sel),
+         dptr(out))
+    return out
 
 
 def gpu_rng_streams(n_streams, seed=None, seed_state=None):

@@ -51,6 +51,7 @@ extern "C" {
 typedef struct dust2_filter dust2_filter;
 typedef struct dust2_system dust2_system;
 typedef struct dust2_unfilter dust2_unfilter;
+typedef struct dust2_unfilter_sir dust2_unfilter_sir;
 
 /* ---- library ------------------------------------------------------------ */
 
@@ -102,10 +103,19 @@ int dust2_filter_update_pars(dust2_filter* h, const double* pars,
 /* Replaces dust2_filter_last_state (dust2/r/filter.hpp): out [G][P][S]. */
 int dust2_filter_last_state(dust2_filter* h, double* out, char* err, size_t errlen);
 
-/* Replaces dust2_filter_last_trajectories (dust2/r/filter.hpp:54-116), the
- * raw per-time post-resampling states without ancestry tracing:
- * out [n_time][G][P][S].  Requires save_trajectories on the last run. */
-int dust2_filter_last_trajectories(dust2_filter* h, double* out, char* err,
+/* Replaces dust2_filter_last_trajectories (dust2/r/filter.hpp:54-95) with
+ * trajectories<T>::export_state(reorder = true) (dust2/trajectories.hpp:128-189):
+ * each exported particle's trajectory follows its ancestry back from the
+ * last time.  select_random_particle != 0 exports one particle per group,
+ * chosen once per run from filter stream 2g + 1 (filter<T>::select_random_particle,
+ * dust2/filter.hpp:130-138).  out [n_time][n_groups of the last run][P or 1][S].
+ * Requires save_trajectories on the last run ("Trajectories are not current"). */
+int dust2_filter_last_trajectories(dust2_filter* h, int select_random_particle, double* out,
+                                   char* err, size_t errlen);
+
+/* The saved post-resampling states as stored, without following the
+ * ancestry: out [n_time][G][P][S].  No counterpart in the reference. */
+int dust2_filter_last_trajectories_raw(dust2_filter* h, double* out, char* err,
                                        size_t errlen);
 
 /* Ancestor indices of the last run: out [n_time][G][P] (0-based). */
@@ -196,6 +206,41 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
                                      char* err, size_t errlen);
 void dust2_unfilter_sirode_free(dust2_unfilter* h);
 
+/* ---- deterministic discrete-time SIR unfilter with the adjoint ----------
+ * Replaces dust2_discrete_unfilter_alloc<sir> (dust2/r/discrete/unfilter.hpp:12-58),
+ * registered as dust2_unfilter_sir_alloc (inst/template/discrete/compare.cpp).
+ * Samplers return their means (the reference's deterministic streams); FP64.
+ * Parameters as dust2_filter_sir_alloc; data[t][g] with n_groups_data 1 or G. */
+int dust2_unfilter_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                             double time_start, const double* time, size_t n_time, double dt,
+                             const double* data, size_t n_groups_data,
+                             dust2_unfilter_sir** out, char* err, size_t errlen);
+
+/* Replaces dust2_unfilter_run<T> (dust2/r/unfilter.hpp:21-62) ->
+ * unfilter<T>::run / run_adjoint (dust2/unfilter.hpp:42-114).  initial: NULL
+ * (model initial state) or [G][P][S].  adjoint != 0 also computes the
+ * gradient of the log-likelihood with respect to (beta, gamma, I0).
+ * ll_out: [n_index][P], one log-likelihood per particle. */
+int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adjoint,
+                           const size_t* index_group, size_t n_index, double* ll_out,
+                           char* err, size_t errlen);
+
+/* Replaces dust2_discrete_unfilter_last_gradient (dust2/r/unfilter.hpp:153-179):
+ * out [n_index of the last run][3] = d ll / d (beta, gamma, I0).  Errors as the
+ * reference when the last run had no adjoint or P > 1. */
+int dust2_unfilter_sir_last_gradient(dust2_unfilter_sir* h, double* out, char* err,
+                                     size_t errlen);
+
+/* Replaces dust2_unfilter_update_pars<T> (dust2/r/unfilter.hpp:10-19): I0,
+ * beta and gamma update, N and exp_noise are constant. */
+int dust2_unfilter_sir_update_pars(dust2_unfilter_sir* h, const double* pars,
+                                   const size_t* index_group, size_t n_index, char* err,
+                                   size_t errlen);
+/* out [G][P][S] */
+int dust2_unfilter_sir_last_state(dust2_unfilter_sir* h, double* out, char* err,
+                                  size_t errlen);
+void dust2_unfilter_sir_free(dust2_unfilter_sir* h);
+
 /* ---- serving: continuous batching of likelihood evaluations ---------------
  * No counterpart in the reference, whose MCMC driver evaluates one
  * likelihood at a time (R/monty.R:188-204).  Each request is one parameter
@@ -239,6 +284,13 @@ void dust2_server_free(dust2_server* s);
 int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* idx,
                        double* out3, char* err, size_t errlen);
 
+/* Lineage export of given saved states (the kernel behind
+ * dust2_filter_last_trajectories): raw [n_time][G][P][S], ancestry
+ * [n_time][G][P] or NULL, select [G] or NULL; out as
+ * dust2_filter_last_trajectories.  For the tests. */
+int dust2_amd_trajectory_lineage(const double* raw, const uint32_t* ancestry, size_t n_time,
+                                 size_t n_groups, size_t n_particles, size_t n_state,
+                                 const size_t* select, double* out, char* err, size_t errlen);
 /* n streams of raw xoshiro256+ state, the filter/system seeding path. */
 int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                           size_t n_streams, uint64_t* out, char* err, size_t errlen);

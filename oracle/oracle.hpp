@@ -441,14 +441,18 @@ struct sir {
   }
 
   // Discrete adjoint (sir.cpp:126-198).  n_adjoint = 5 states + 3
-  // parameters (beta, gamma, I0).  The gamma term uses the derivative of
-  // p_IR = 1 - exp(-gamma dt), dt*exp(-gamma dt); the reference writes
-  // exp(-gamma) (sir.cpp:175), which is the same expression at dt = 1.
+  // parameters (beta, gamma, I0).  Two terms differ from the reference's
+  // hand-written adjoint, which the finite-difference checks in the tests
+  // settle: the gamma term uses the derivative of p_IR = 1 - exp(-gamma dt),
+  // dt*exp(-gamma dt), where the reference writes exp(-gamma) (sir.cpp:175;
+  // equal at dt = 1); and the initial condition S0 = N - I0 contributes
+  // -adj_S to the I0 gradient, which the reference's adjoint_initial
+  // (sir.cpp:126-141) leaves out.
   static constexpr size_t n_gradient = 3;
   static void adjoint_initial(const double*, const double* adj, const shared_state&,
                               double* adj_next) {
     for (size_t i = 0; i < 8; ++i) adj_next[i] = adj[i];
-    adj_next[7] = adj[7] + adj[1];
+    adj_next[7] = adj[7] + adj[1] - adj[0];
   }
   static void adjoint_update(double dt, const double* y, const double* adj,
                              const shared_state& sh, double* an) {

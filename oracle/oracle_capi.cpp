@@ -387,4 +387,30 @@ int oracle_sir_unfilter_run(const double* pars, size_t G, double time_start,
   }
 }
 
+// Trajectory export with reorder (dust2/trajectories.hpp:128-189, using
+// reorder_state_group_ / reorder_state_single_ at :313-343): walking back
+// from the last time, write the state of each tracked particle, then
+// replace the particle by its ancestor at that time.  raw [T][G][P][S],
+// ancestry [T][G][P] (NULL: never reordered), select [G] (NULL: every
+// particle); out [T][G][P or 1][S].
+void oracle_trajectory_lineage(const double* raw, const uint32_t* ancestry, size_t T,
+                               size_t G, size_t P, size_t S, const size_t* select,
+                               double* out) {
+  const size_t p_out = select ? 1 : P;
+  std::vector<size_t> index(G * p_out);
+  for (size_t g = 0; g < G; ++g) {
+    for (size_t k = 0; k < p_out; ++k) index[g * p_out + k] = select ? select[g] : k;
+  }
+  for (size_t irev = 0; irev < T; ++irev) {
+    const size_t t = T - irev - 1;
+    for (size_t g = 0; g < G; ++g) {
+      for (size_t k = 0; k < p_out; ++k) {
+        size_t& i = index[g * p_out + k];
+        std::copy_n(raw + ((t * G + g) * P + i) * S, S, out + ((t * G + g) * p_out + k) * S);
+        if (ancestry) i = ancestry[(t * G + g) * P + i];
+      }
+    }
+  }
+}
+
 }  // extern "C"

@@ -51,6 +51,9 @@ def oracle():
         o.oracle_scale_log_weights.restype = C.c_double
         o.oracle_scale_log_weights.argtypes = [C.c_size_t, D]
         o.oracle_resample_weight.argtypes = [C.c_size_t, D, C.c_double, U32]
+        o.oracle_trajectory_lineage.argtypes = [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
+                                                C.c_size_t, SZ, D]
+        o.oracle_trajectory_lineage.restype = None
         o.oracle_seed_words.argtypes = [C.c_uint64, U64]
         o.oracle_prng.argtypes = [U64, C.c_size_t, C.c_size_t, U64]
         o.oracle_jump.argtypes = [U64, C.c_int]
@@ -236,6 +239,19 @@ def sirode_unfilter(pars, n_particles, time_start, time, data, control=None, n_t
                                              4096)
     _check(rc, buf)
     return {"ll": ll, "state": state, "stats": stats}
+
+
+def trajectory_lineage(raw, ancestry=None, select=None):
+    """raw (T, G, P, S); ancestry (T, G, P) or None; select (G,) or None."""
+    raw = np.ascontiguousarray(raw, dtype=np.float64)
+    T, G, P, S = raw.shape
+    anc = None if ancestry is None else np.ascontiguousarray(ancestry, dtype=np.uint32)
+    sel = None if select is None else np.ascontiguousarray(select, dtype=np.uintp)
+    out = np.zeros((T, G, 1 if sel is not None else P, S))
+    oracle().oracle_trajectory_lineage(_dp(raw), _u32(anc), T, G, P, S,
+                                       None if sel is None else sel.ctypes.data_as(SZ),
+                                       _dp(out))
+    return out
 
 
 def sir_unfilter(pars, time_start, time, dt, data, gradient=True):

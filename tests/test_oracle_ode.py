@@ -73,3 +73,43 @@ def test_unfilter_deterministic_sir_closed_form():
                 t += 1
             expect += stats.poisson.logpmf(data[i, g], inc + 1e-6)
         assert ll[g] == pytest.approx(expect, rel=1e-12)
+
+
+def _fd_gradient(pars, time, dt, data, rel=1e-6):
+    # central differences in (beta, gamma, I0), N held fixed
+    out = np.zeros((pars.shape[0], 3))
+    for j, col in enumerate([2, 3, 1]):
+        h = rel * pars[:, col]
+        up, dn = pars.copy(), pars.copy()
+        up[:, col] += h
+        dn[:, col] -= h
+        lu, _ = ol.sir_unfilter(up, 0.0, time, dt, data, gradient=False)
+        ld, _ = ol.sir_unfilter(dn, 0.0, time, dt, data, gradient=False)
+        out[:, j] = (lu - ld) / (2 * h)
+    return out
+
+
+@pytest.mark.parametrize("times,dt", [([4.0, 8.0, 12.0, 16.0], 1.0),    # test-unfilter-adjoint.R:14-27
+                                      ([3.0, 6.0, 9.0, 12.0], 1.0),     # odd step counts, :120-133
+                                      ([4.0, 8.0, 12.0, 16.0], 0.25)])
+def test_adjoint_gradient_matches_finite_differences(times, dt):
+    pars = np.array([[1000, 10, 0.1, 0.2, 1e6], [5000, 25, 0.35, 0.12, 1e6],
+                     [1000, 10, 0.3, 0.05, 1e6]], dtype=float)
+    time = np.array(times)
+    data = np.tile(np.arange(1.0, 5.0)[:, None], (1, 3))
+    ll, grad = ol.sir_unfilter(pars, 0.0, time, dt, data)
+    ll2, _ = ol.sir_unfilter(pars, 0.0, time, dt, data, gradient=False)
+    assert np.array_equal(ll, ll2)  # the adjoint run gives the same likelihood
+    fd = _fd_gradient(pars, time, dt, data)
+    np.testing.assert_allclose(grad, fd, rtol=1e-5)
+
+
+def test_adjoint_gradient_with_missing_data_and_long_series():
+    rng = np.random.default_rng(3)
+    pars = np.array([[1e4, 10, 0.25, 0.1, 1e6]])
+    time = np.arange(1.0, 61.0)
+    data = rng.poisson(20, (60, 1)).astype(float)
+    data[[3, 17, 40], 0] = np.nan
+    _, grad = ol.sir_unfilter(pars, 0.0, time, 0.25, data)
+    fd = _fd_gradient(pars, time, 0.25, data)
+    np.testing.assert_allclose(grad, fd, rtol=1e-5)
