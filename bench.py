@@ -146,10 +146,14 @@ def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, k
     obj.set_engine("pipeline", graph=False, profile=True)
     lib.dust_likelihood_run(obj, plist)
     k_ms = lib.dust_likelihood_last_stats(obj)["advance_ms"]
-    obj.set_engine("auto")
+    obj.set_engine("pipeline")
     b_launch = G * P * (2 * S * s_x + 2 * 32 + 8 + 4)
     name = "seir_advance_kernel" if S > 5 else "sir_advance_kernel"
-    return roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs)
+    out = roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs)
+    # the whole run (advance + resample launches) against the same bytes
+    out["run_ms"] = run_ms
+    out["run_achieved"] = G * P * n_obs * b_obs / (run_ms * 1e-3) / 1e9
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -163,10 +167,15 @@ def bench_sir(lib, args):
     data = simulate_sir_data(lib, pars, time, dt, seed=42)
     obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
                                  n_groups=G, dt=dt, seed=1, precision=args.precision)
+    plist = [dict(zip(SIR_NAMES, p)) for p in pars]
+    tuned = None
     if args.launch:
         obj.set_launch(*[int(x) for x in args.launch.split(",")])
     obj.set_engine(args.engine, graph=not args.no_graph)
-    plist = [dict(zip(SIR_NAMES, p)) for p in pars]
+    if args.engine == "auto" and not args.launch and not args.no_tune:
+        # untimed: the autotuner runs each engine / launch shape twice and
+        # restores the RNG streams, then keeps the fastest
+        tuned = obj.autotune(plist)
     # every call passes the parameters, as an MCMC driver does
     ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
     n_steps = int(round(time[-1] / dt))
@@ -184,7 +193,8 @@ def bench_sir(lib, args):
         "config": {"workload": "batched stochastic SIR bootstrap filter",
                    "groups": G, "particles": P, "n_obs": int(time.size), "dt": dt,
                    "precision": args.precision, "engine": stats["engine"],
-                   "launch": obj.launch},
+                   "launch": stats["launch"] if stats["engine"] == "persistent" else None},
+        "autotune": tuned,
         "likelihood_evals_per_s": G * args.steps / elapsed,
         "gpu_ms_per_step": float(np.mean(gpu_ms)),
         "mean_ll": float(np.mean(ll)),
@@ -219,6 +229,7 @@ def bench_sir_replicates(lib, args):
     obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
                                  n_groups=G, dt=dt, seed=1, precision=prec)
     plist = [dict(zip(SIR_NAMES, row))] * G
+    tuned = None if args.no_tune else obj.autotune(plist)
     ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
     n_steps = 400
     stats = lib.dust_likelihood_last_stats(obj)
@@ -232,6 +243,7 @@ def bench_sir_replicates(lib, args):
            "data": "synthetic (100 daily observations simulated at seed 42)",
            "config": {"workload": "1000 replicate SIR filters, 1000 particles", "groups": G,
                       "particles": P, "n_obs": 100, "dt": dt, "precision": prec},
+           "engine": stats["engine"], "autotune": tuned,
            "particle_steps_per_s": G * P * n_steps * args.steps / elapsed,
            "ll_mean": float(ll.mean()), "ll_var": float(ll.var(ddof=1)),
            "ll_mc_se": float(ll.std(ddof=1) / np.sqrt(G)), "roofline": rl}
@@ -495,6 +507,7 @@ def main():
     ap.add_argument("--launch", default=None, help="block,particles_per_thread")
     ap.add_argument("--engine", default="auto", choices=["auto", "persistent", "pipeline"])
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the engine autotuner")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=16)
     ap.add_argument("--serve-seconds", type=float, default=15.0)

@@ -1140,6 +1140,93 @@ int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int prof
   }
 }
 
+int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_index,
+                          double* report, size_t report_rows, size_t* n_report, char* err,
+                          size_t errlen) {
+  try {
+    check_filter(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    const int P = static_cast<int>(h->P);
+    // candidates: the persistent kernel at every launch shape that holds the
+    // group without more than doubling it, and the interval pipeline
+    std::vector<std::pair<int, launch_config>> cand;
+    if (h->model == 0 && P <= sir_filter_max_particles(h->fp64)) {
+      static const int shapes[][2] = {{64, 1},   {64, 2},   {64, 4},   {128, 2}, {128, 4},
+                                      {256, 2},  {256, 4},  {256, 8},  {512, 2}, {512, 4},
+                                      {512, 8},  {1024, 2}, {1024, 4}, {1024, 8}};
+      for (const auto& sh : shapes) {
+        const int cap = sh[0] * sh[1];
+        if (cap >= P && cap < 2 * P + 64) {
+          cand.push_back({DUST2_ENGINE_PERSISTENT, launch_config{sh[0], sh[1]}});
+        }
+      }
+    }
+    cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg});
+    // tuning runs must leave the object as they found it: the particle and
+    // filter streams and the state are restored after every run
+    hipStream_t s = h->stream;
+    dbuf rng0(h->d_rng.bytes()), state0(h->d_state.bytes());
+    hip_check(hipMemcpyAsync(rng0.get(), h->d_rng.get(), h->d_rng.bytes(),
+                             hipMemcpyDeviceToDevice, s), "copy");
+    hip_check(hipMemcpyAsync(state0.get(), h->d_state.get(), h->d_state.bytes(),
+                             hipMemcpyDeviceToDevice, s), "copy");
+    const int engine0 = h->engine;
+    const launch_config cfg0 = h->cfg;
+    const bool graph0 = h->use_graph, profile0 = h->profile;
+    std::vector<double> ll(idx.size());
+    std::vector<double> ms(cand.size(), 0.0);
+    std::string failure;
+    for (size_t i = 0; i < cand.size() && failure.empty(); ++i) {
+      h->engine = cand[i].first;
+      h->cfg = cand[i].second;
+      h->use_graph = true;
+      h->profile = false;
+      for (int rep = 0; rep < 2; ++rep) {
+        char buf[4096];
+        const int rc = dust2_filter_run(h, nullptr, 0, idx.data(), idx.size(), ll.data(), buf,
+                                        sizeof buf);
+        hip_check(hipMemcpyAsync(h->d_rng.get(), rng0.get(), h->d_rng.bytes(),
+                                 hipMemcpyDeviceToDevice, s), "copy");
+        if (rc != 0) {
+          failure = buf;
+          break;
+        }
+        ms[i] = h->last_ms;  // the second (warm) run counts
+      }
+    }
+    hip_check(hipMemcpyAsync(h->d_state.get(), state0.get(), h->d_state.bytes(),
+                             hipMemcpyDeviceToDevice, s), "copy");
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->use_graph = graph0;
+    h->profile = profile0;
+    h->traj_valid = false;
+    h->ancestry_valid = false;
+    if (!failure.empty()) {
+      h->engine = engine0;
+      h->cfg = cfg0;
+      throw std::runtime_error(failure);
+    }
+    size_t best = 0;
+    for (size_t i = 1; i < cand.size(); ++i) {
+      if (ms[i] < ms[best]) best = i;
+    }
+    h->engine = cand[best].first;
+    if (h->engine == DUST2_ENGINE_PERSISTENT) h->cfg = cand[best].second;
+    else h->cfg = cfg0;
+    const size_t n = std::min(report_rows, cand.size());
+    for (size_t i = 0; report && i < n; ++i) {
+      report[4 * i + 0] = cand[i].first;
+      report[4 * i + 1] = cand[i].first == DUST2_ENGINE_PERSISTENT ? cand[i].second.block : 0;
+      report[4 * i + 2] = cand[i].first == DUST2_ENGINE_PERSISTENT ? cand[i].second.ppt : 0;
+      report[4 * i + 3] = ms[i];
+    }
+    if (n_report) *n_report = cand.size();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 void dust2_filter_free(dust2_filter* h) { delete h; }
 
 }  // extern "C"

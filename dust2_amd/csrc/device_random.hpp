@@ -239,11 +239,20 @@ __device__ __forceinline__ float stirling_tail_f32(float k) {
 }
 
 // Inversion, n*p < 10, p <= 1/2.
+//
+// The search restarts with a fresh uniform once k passes min(n, 110).  With
+// mean below 10 the float pmf underflows to zero before k = 100, after
+// which `u >= f` can no longer fail: a uniform left above the float sum of
+// the pmf (probability about 1e-7 per draw, from rounding) would otherwise
+// walk all the way to k = n before restarting, up to 1e6 iterations for one
+// lane that the rest of its wave waits on.  Restarting at 110 instead gives
+// the same draw sequence.
 __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
   const float q = 1 - p;
   const float r = p * f_rcp(q);
   const float g = r * (n + 1);
   const float f0 = __expf(n * f_log1p(-p));
+  const float k_max = fminf(n, 110.0f);
   while (true) {
     float u = random_real<float>(rng);
     float f = f0;
@@ -252,7 +261,7 @@ __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
     while (u >= f) {
       u -= f;
       k += 1;
-      if (k > n) {
+      if (k > k_max) {
         ok = false;
         break;
       }

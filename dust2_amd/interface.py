@@ -132,6 +132,17 @@ def pars_update(generator, pars, current):
         raise ValueError(f"Expected 'pars' to have length {current.shape[0]}, but it had "
                          f"length {len(pars)}")
     rows = current.copy()
+    keys = tuple(pars[0]) if len(pars) else ()
+    if all(tuple(p) == keys for p in pars):
+        # every row names the same parameters: one array per name
+        for k in keys:
+            j = _par_offset(generator, k)
+            n = generator.size(k)
+            v = np.array([p[k] for p in pars], dtype=np.float64).reshape(len(pars), n)
+            if k in generator.constant and not np.array_equal(v, rows[:, j:j + n]):
+                raise ValueError(f"Parameter '{k}' is constant and cannot be updated")
+            rows[:, j:j + n] = v
+        return np.ascontiguousarray(rows)
     for i, p in enumerate(pars):
         for k, v in p.items():
             j = _par_offset(generator, k)
@@ -244,6 +255,32 @@ class Filter:
         self.engine = (self.ENGINES[engine], int(bool(graph)), int(bool(profile)))
         if self.ptr is not None:
             call("dust2_filter_set_engine", self.ptr, *self.engine)
+
+    def autotune(self, pars=None, index_group=None):
+        """Time every engine / launch shape on this workload and keep the
+        fastest; returns the candidates as dicts (engine, block, ppt, ms)."""
+        if self.ptr is None:
+            if pars is None:
+                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+            self._alloc(pars)
+        elif pars is not None:
+            idx, n_idx, _ = _index_group(index_group, self.n_groups)
+            sel = np.arange(self.n_groups) if idx is None else idx.astype(np.int64)
+            rows = pars_update(self.generator, pars, self.rows[sel])
+            call("dust2_filter_update_pars", self.ptr, dptr(rows), szptr(idx), n_idx)
+            self.rows[sel] = rows
+        idx, n_idx, _ = _index_group(index_group, self.n_groups)
+        rep = np.zeros((32, 4))
+        n = np.zeros(1, dtype=np.uintp)
+        call("dust2_filter_autotune", self.ptr, szptr(idx), n_idx, dptr(rep), 32, szptr(n))
+        names = {1: "persistent", 2: "pipeline"}
+        out = [{"engine": names[int(r[0])], "block": int(r[1]), "ppt": int(r[2]),
+                "ms": float(r[3])} for r in rep[:int(n[0])]]
+        best = min(out, key=lambda r: r["ms"])
+        self.engine = (self.ENGINES[best["engine"]], 1, 0)
+        if best["engine"] == "persistent":
+            self.launch = (best["block"], best["ppt"])
+        return out
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and _lib._lib is not None:

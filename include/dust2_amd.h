@@ -146,6 +146,19 @@ int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errl
 int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
                                 char* err, size_t errlen);
 
+/* Autotuner (no counterpart in the reference): times a run of the groups in
+ * index_group (NULL: all) with every candidate engine and launch shape -- the
+ * persistent kernel at each threads x particles-per-thread shape that holds
+ * the group, and the interval pipeline -- and keeps the fastest.  Results do
+ * not depend on the choice; the RNG streams and state are restored, so the
+ * object is left as it was apart from the engine choice (trajectories of
+ * earlier runs are no longer current).  report (may be NULL): up to
+ * report_rows rows of [engine, block, particles_per_thread, milliseconds];
+ * *n_report receives the number of candidates. */
+int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_index,
+                          double* report, size_t report_rows, size_t* n_report, char* err,
+                          size_t errlen);
+
 void dust2_filter_free(dust2_filter* h);
 
 /* ---- discrete-time system (stochastic SIR) --------------------------------

@@ -300,3 +300,23 @@ def test_system_errors_lock_until_reset(gpu_lib):
     gpu_lib.dust_system_update_pars(sys, {"gamma": 0.2})
     gpu_lib.dust_system_set_state_initial(sys)
     assert gpu_lib.dust_system_time(sys) == 0
+
+
+def test_autotune_keeps_results_and_streams(gpu_lib):
+    G, P = 3, 500
+    pars = np.array([[2000, 10, 0.25 + 0.05 * g, 0.1, 1e6] for g in range(G)])
+    time = np.arange(1.0, 21.0)
+    data = np.stack([sir_data(20, seed=7 + g) for g in range(G)], axis=1)
+    plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
+    ref = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                     n_groups=G, dt=0.25, seed=11)
+    ll_ref = gpu_lib.dust_likelihood_run(ref, plist)
+    obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                     n_groups=G, dt=0.25, seed=11)
+    report = obj.autotune(plist)
+    assert len(report) >= 2 and all(r["ms"] > 0 for r in report)
+    assert {r["engine"] for r in report} == {"persistent", "pipeline"}
+    ll = gpu_lib.dust_likelihood_run(obj, None)
+    np.testing.assert_array_equal(ll, ll_ref)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_rng_state(obj),
+                                  gpu_lib.dust_likelihood_rng_state(ref))
