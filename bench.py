@@ -149,7 +149,7 @@ def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, k
     obj.set_engine("pipeline")
     b_launch = G * P * (2 * S * s_x + 2 * 32 + 8 + 4)
     name = "seir_advance_kernel" if S > 5 else "sir_advance_kernel"
-    out = roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs)
+    out = roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs, key)
     # the whole run (advance + resample launches) against the same bytes
     out["run_ms"] = run_ms
     out["run_achieved"] = G * P * n_obs * b_obs / (run_ms * 1e-3) / 1e9
@@ -327,7 +327,7 @@ def bench_seir16(lib, args):
     n_steps = int(round(time[-1] / dt))
     value = G * P * n_steps * args.steps / elapsed
     rl = filter_roofline(lib, obj, plist, G, P, time.size, 80, 4, "pipeline",
-                         float(np.mean(gpu_ms)))
+                         float(np.mean(gpu_ms)), key=f"seir16:{G}x{P}:fp32")
     # plain path: one filter object per group, one group per run; each
     # object is warmed by one run, then a second run is timed
     n_plain = 4

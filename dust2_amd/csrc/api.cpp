@@ -368,6 +368,12 @@ struct dust2_filter {
       d_pslots_init;
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
+  // the pipeline splits the groups into `pipe_chains` independent launch
+  // chains on their own streams, so one chain's resample overlaps another
+  // chain's advance
+  int pipe_chains = 2;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool traj_valid = false;
   bool ancestry_valid = false;
   std::vector<size_t> last_index;        // groups of the last run
@@ -386,6 +392,9 @@ struct dust2_filter {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -553,40 +562,69 @@ pipe_slot make_pipe_slot(dust2_filter* h, size_t g, bool save_traj) {
 
 // The launches of one pipeline run.  With `profile`, events around every
 // launch accumulate per-kernel times (not usable under graph capture).
+// Number of launch chains for n_slots groups: chain c runs slots
+// [c * n / chains, (c + 1) * n / chains).  Profiling uses one chain so that
+// every launch can be timed.
+int pipeline_chains(const dust2_filter* h, int n_slots) {
+  if (h->profile || n_slots < 2) return 1;
+  return std::min(h->pipe_chains, 2);
+}
+
+void chain_range(int n_slots, int chains, int c, int& first, int& count) {
+  first = static_cast<int>(static_cast<long>(n_slots) * c / chains);
+  count = static_cast<int>(static_cast<long>(n_slots) * (c + 1) / chains) - first;
+}
+
+// The launches of one pipeline run.  With `profile`, events around every
+// launch accumulate per-kernel times (not usable under graph capture).
 void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool save_traj,
                        std::vector<hipEvent_t>* evs) {
-  hipStream_t s = h->stream;
   const int P = static_cast<int>(h->P);
-  const pipe_slot* slots = h->d_pslots.get<pipe_slot>();
   const int T = static_cast<int>(h->n_time);
-  auto mark = [&]() {
-    if (evs) {
-      hipEvent_t e;
-      hip_check(hipEventCreate(&e), "hipEventCreate");
-      hip_check(hipEventRecord(e, s), "hipEventRecord");
-      evs->push_back(e);
-    }
-  };
-  hip_check(launch_pipe_reset(slots, n_slots, s), "reset kernel");
-  int parity = 0;
-  for (int t = 0; t < T; ++t) {
-    uint32_t mode = PIPE_STEP;
-    if (t == 0) {
-      mode |= initial_given ? 0u : PIPE_INITIAL;
-    } else {
-      mode |= PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u);
-    }
-    mark();
-    hip_check(launch_advance(h, slots, n_slots, P, t, parity, mode, s),
-              "advance kernel");
-    mark();
-    parity ^= 1;
-    hip_check(launch_pipe_resample(slots, n_slots, P, t, s), "resample kernel");
-    mark();
+  const int chains = pipeline_chains(h, n_slots);
+  if (chains > 1) {
+    // fork: the second chain starts after everything queued before it
+    hip_check(hipEventRecord(h->ev_fork, h->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(h->stream2, h->ev_fork, 0), "hipStreamWaitEvent");
   }
-  hip_check(launch_advance(h, slots, n_slots, P, T, parity,
-                           PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u), s),
-            "gather kernel");
+  for (int c = 0; c < chains; ++c) {
+    hipStream_t s = c == 0 ? h->stream : h->stream2;
+    int first = 0, count = 0;
+    chain_range(n_slots, chains, c, first, count);
+    const pipe_slot* slots = h->d_pslots.get<pipe_slot>() + first;
+    auto mark = [&]() {
+      if (evs) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventRecord(e, s), "hipEventRecord");
+        evs->push_back(e);
+      }
+    };
+    hip_check(launch_pipe_reset(slots, count, s), "reset kernel");
+    int parity = 0;
+    for (int t = 0; t < T; ++t) {
+      uint32_t mode = PIPE_STEP;
+      if (t == 0) {
+        mode |= initial_given ? 0u : PIPE_INITIAL;
+      } else {
+        mode |= PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u);
+      }
+      mark();
+      hip_check(launch_advance(h, slots, count, P, t, parity, mode, s), "advance kernel");
+      mark();
+      parity ^= 1;
+      hip_check(launch_pipe_resample(slots, count, P, t, s), "resample kernel");
+      mark();
+    }
+    hip_check(launch_advance(h, slots, count, P, T, parity,
+                             PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u), s),
+              "gather kernel");
+  }
+  if (chains > 1) {
+    // join
+    hip_check(hipEventRecord(h->ev_join, h->stream2), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(h->stream, h->ev_join, 0), "hipStreamWaitEvent");
+  }
 }
 
 // Runs the filter through the interval pipeline; the final state is left
@@ -604,6 +642,18 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
   for (size_t g : idx) {
     slots.push_back(make_pipe_slot(h, g, save_traj));
     selected[g] = 1;
+  }
+  {
+    // each chain counts its own errors (counter 0 or 2) so that a failing
+    // chain stops only itself; the report keeps the earliest interval
+    const int chains = pipeline_chains(h, static_cast<int>(slots.size()));
+    for (int c = 1; c < chains; ++c) {
+      int first = 0, count = 0;
+      chain_range(static_cast<int>(slots.size()), chains, c, first, count);
+      for (int i = first; i < first + count; ++i) {
+        slots[i].error_count = h->d_counters.get<uint32_t>() + 2;
+      }
+    }
   }
   to_device(h->d_pslots, slots.data(), slots.size() * sizeof(pipe_slot), s);
   hip_check(hipStreamSynchronize(s), "sync");
@@ -629,7 +679,8 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
     h->last_resample_ms = res / T;
   } else if (h->use_graph) {
     const std::vector<uint64_t> key = {static_cast<uint64_t>(n_slots), h->P, h->n_time,
-                                       initial_given, save_traj, h->fp64};
+                                       initial_given, save_traj, h->fp64,
+                                       static_cast<uint64_t>(pipeline_chains(h, n_slots))};
     if (!h->graph || key != h->graph_key) {
       if (h->graph) {
         hip_check(hipGraphExecDestroy(h->graph), "hipGraphExecDestroy");
@@ -742,6 +793,9 @@ static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
     }
     h->cfg = sir_filter_default_config(static_cast<int>(n_particles), h->fp64);
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
     const size_t G = n_groups, P = n_particles;
@@ -883,12 +937,24 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     // a new run forgets the chosen particles of its groups (filter.hpp:162-175)
     if (h->select_valid.size() != G) h->select_valid.assign(G, 0);
     for (size_t g : idx) h->select_valid[g] = 0;
-    if (counters[0] > 0) {
+    if (counters[0] + counters[2] > 0) {
       std::vector<error_record> rec(G * P);
       to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
       hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
       hip_check(hipStreamSynchronize(s), "sync");
-      throw std::runtime_error(error_report(rec, counters[0]));
+      // the run stops at the first interval with errors
+      // (dust2/discrete/system.hpp:87): independent launch chains may have
+      // gone on to fail later, so keep only the earliest interval's errors
+      uint32_t first = UINT32_MAX;
+      for (const auto& r : rec) {
+        if (r.code != ERR_NONE) first = std::min(first, r.interval);
+      }
+      size_t count = 0;
+      for (auto& r : rec) {
+        if (r.code != ERR_NONE && r.interval != first) r.code = ERR_NONE;
+        if (r.code != ERR_NONE) ++count;
+      }
+      throw std::runtime_error(error_report(rec, count));
     }
     std::vector<double> ll(G);
     to_host(ll.data(), h->d_ll, G * sizeof(double), s);
@@ -1149,7 +1215,12 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
     const int P = static_cast<int>(h->P);
     // candidates: the persistent kernel at every launch shape that holds the
     // group without more than doubling it, and the interval pipeline
-    std::vector<std::pair<int, launch_config>> cand;
+    struct candidate {
+      int engine;
+      launch_config cfg;
+      int chains;
+    };
+    std::vector<candidate> cand;
     if (h->model == 0 && P <= sir_filter_max_particles(h->fp64)) {
       static const int shapes[][2] = {{64, 1},   {64, 2},   {64, 4},   {128, 2}, {128, 4},
                                       {256, 2},  {256, 4},  {256, 8},  {512, 2}, {512, 4},
@@ -1157,11 +1228,12 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       for (const auto& sh : shapes) {
         const int cap = sh[0] * sh[1];
         if (cap >= P && cap < 2 * P + 64) {
-          cand.push_back({DUST2_ENGINE_PERSISTENT, launch_config{sh[0], sh[1]}});
+          cand.push_back({DUST2_ENGINE_PERSISTENT, launch_config{sh[0], sh[1]}, 1});
         }
       }
     }
-    cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg});
+    cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, 1});
+    if (idx.size() > 1) cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, 2});
     // tuning runs must leave the object as they found it: the particle and
     // filter streams and the state are restored after every run
     hipStream_t s = h->stream;
@@ -1173,12 +1245,14 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
     const int engine0 = h->engine;
     const launch_config cfg0 = h->cfg;
     const bool graph0 = h->use_graph, profile0 = h->profile;
+    const int chains0 = h->pipe_chains;
     std::vector<double> ll(idx.size());
     std::vector<double> ms(cand.size(), 0.0);
     std::string failure;
     for (size_t i = 0; i < cand.size() && failure.empty(); ++i) {
-      h->engine = cand[i].first;
-      h->cfg = cand[i].second;
+      h->engine = cand[i].engine;
+      h->cfg = cand[i].cfg;
+      h->pipe_chains = cand[i].chains;
       h->use_graph = true;
       h->profile = false;
       for (int rep = 0; rep < 2; ++rep) {
@@ -1204,20 +1278,22 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
     if (!failure.empty()) {
       h->engine = engine0;
       h->cfg = cfg0;
+      h->pipe_chains = chains0;
       throw std::runtime_error(failure);
     }
     size_t best = 0;
     for (size_t i = 1; i < cand.size(); ++i) {
       if (ms[i] < ms[best]) best = i;
     }
-    h->engine = cand[best].first;
-    if (h->engine == DUST2_ENGINE_PERSISTENT) h->cfg = cand[best].second;
-    else h->cfg = cfg0;
+    h->engine = cand[best].engine;
+    h->cfg = h->engine == DUST2_ENGINE_PERSISTENT ? cand[best].cfg : cfg0;
+    h->pipe_chains = h->engine == DUST2_ENGINE_PIPELINE ? cand[best].chains : chains0;
     const size_t n = std::min(report_rows, cand.size());
     for (size_t i = 0; report && i < n; ++i) {
-      report[4 * i + 0] = cand[i].first;
-      report[4 * i + 1] = cand[i].first == DUST2_ENGINE_PERSISTENT ? cand[i].second.block : 0;
-      report[4 * i + 2] = cand[i].first == DUST2_ENGINE_PERSISTENT ? cand[i].second.ppt : 0;
+      const bool pers = cand[i].engine == DUST2_ENGINE_PERSISTENT;
+      report[4 * i + 0] = cand[i].engine;
+      report[4 * i + 1] = pers ? cand[i].cfg.block : 0;
+      report[4 * i + 2] = pers ? cand[i].cfg.ppt : cand[i].chains;
       report[4 * i + 3] = ms[i];
     }
     if (n_report) *n_report = cand.size();

@@ -274,8 +274,13 @@ class Filter:
         n = np.zeros(1, dtype=np.uintp)
         call("dust2_filter_autotune", self.ptr, szptr(idx), n_idx, dptr(rep), 32, szptr(n))
         names = {1: "persistent", 2: "pipeline"}
-        out = [{"engine": names[int(r[0])], "block": int(r[1]), "ppt": int(r[2]),
-                "ms": float(r[3])} for r in rep[:int(n[0])]]
+        out = []
+        for r in rep[:int(n[0])]:
+            e = names[int(r[0])]
+            if e == "persistent":
+                out.append({"engine": e, "block": int(r[1]), "ppt": int(r[2]), "ms": float(r[3])})
+            else:
+                out.append({"engine": e, "chains": int(r[2]), "ms": float(r[3])})
         best = min(out, key=lambda r: r["ms"])
         self.engine = (self.ENGINES[best["engine"]], 1, 0)
         if best["engine"] == "persistent":

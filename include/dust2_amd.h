@@ -152,9 +152,11 @@ int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int prof
  * the group, and the interval pipeline -- and keeps the fastest.  Results do
  * not depend on the choice; the RNG streams and state are restored, so the
  * object is left as it was apart from the engine choice (trajectories of
- * earlier runs are no longer current).  report (may be NULL): up to
- * report_rows rows of [engine, block, particles_per_thread, milliseconds];
- * *n_report receives the number of candidates. */
+ * earlier runs are no longer current).  The pipeline is tried with one
+ * launch chain and with the groups split into two chains on two streams.
+ * report (may be NULL): up to report_rows rows of [engine, block,
+ * particles per thread (persistent) or launch chains (pipeline),
+ * milliseconds]; *n_report receives the number of candidates. */
 int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_index,
                           double* report, size_t report_rows, size_t* n_report, char* err,
                           size_t errlen);
