@@ -323,13 +323,28 @@ __device__ __forceinline__ real binomial_core(rng256& rng, real n, real q) {
 }
 
 // Called, not inlined, from the step loops: a single out-of-line copy keeps
-// the rejection loops' temporaries out of the caller's register budget.
+// the rejection loops' temporaries out of the caller's register budget.  The
+// stream travels by value in and out (registers), not through a pointer,
+// which would put it in scratch memory around every call.
 #ifndef DUST2_BINOMIAL_LINKAGE
 #define DUST2_BINOMIAL_LINKAGE __noinline__
 #endif
+struct binomial_draw_f32 {
+  rng256 rng;
+  float k;
+};
+__device__ DUST2_BINOMIAL_LINKAGE binomial_draw_f32 binomial_core_f32(rng256 rng, float n,
+                                                                      float q) {
+  binomial_draw_f32 d;
+  d.k = (n * q >= 10) ? binomial_btrs_f32(rng, n, q) : binomial_inversion_f32(rng, n, q);
+  d.rng = rng;
+  return d;
+}
 template <>
-__device__ DUST2_BINOMIAL_LINKAGE float binomial_core<float>(rng256& rng, float n, float q) {
-  return (n * q >= 10) ? binomial_btrs_f32(rng, n, q) : binomial_inversion_f32(rng, n, q);
+__device__ __forceinline__ float binomial_core<float>(rng256& rng, float n, float q) {
+  const binomial_draw_f32 d = binomial_core_f32(rng, n, q);
+  rng = d.rng;
+  return d.k;
 }
 
 // Binomial(n, p).  Returns ERR_BINOMIAL in *err (and 0) on invalid input,

@@ -23,18 +23,6 @@ namespace dust2_amd {
 
 namespace {
 
-// The thread's particles are advanced one at a time in a rolled loop (one
-// inlined copy of the model).  Each iteration works on slot 0 of the
-// register arrays and then rotates them by one, so every array index is a
-// compile-time constant and the arrays stay in VGPRs; after PPT iterations
-// the original order is restored.
-template <int PPT, typename T>
-__device__ __forceinline__ void reg_rotate(T (&a)[PPT], const T& last) {
-#pragma unroll
-  for (int kk = 0; kk + 1 < PPT; ++kk) a[kk] = a[kk + 1];
-  a[PPT - 1] = last;
-}
-
 template <typename real>
 struct sir_particle {
   real y[5];
@@ -120,36 +108,37 @@ __global__ void __launch_bounds__(BLOCK) sir_filter_kernel(const filter_slot* __
       }
     }
 
+    // the thread's particles advance step by step together, so the
+    // independent sampler chains of its PPT particles interleave
     double w[PPT];
-#pragma unroll 1
-    for (int k = 0; k < PPT; ++k) {
-      const int p = p0 + k;
-      sir_particle<real> cur = part[0];
-      rng256 r = rng[0];
-      double wk = 0;
-      if (p < P) {
-        bool failed = (fail_mask >> k) & 1u;
-        for (int s = 0; s < n_steps && !failed; ++s) {
-          const bool zero = zero_generic ? fmod(time + s * dt, sir_model::zero_period) == 0
-                                         : ((zero_mask >> s) & 1u) != 0;
-          if (zero) {
-            cur.y[sir_model::zero_first] = 0;
-          }
-          uint32_t err = 0;
-          real ea = 0, eb = 0;
-          sir_model::update<real>(dt_r, cur.y, sh, r, det, err, ea, eb);
-          if (err) {
-            failed = true;
-            fail_mask |= 1u << k;
-            sd.errors[p] = error_record{err, t, static_cast<double>(ea), static_cast<double>(eb)};
-            atomicAdd(sd.error_count, 1u);
-          }
+    bool live[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) live[k] = (p0 + k < P) && !((fail_mask >> k) & 1u);
+    for (int s = 0; s < n_steps; ++s) {
+      const bool zero = zero_generic ? fmod(time + s * dt, sir_model::zero_period) == 0
+                                     : ((zero_mask >> s) & 1u) != 0;
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        if (!live[k]) continue;
+        if (zero) {
+          part[k].y[sir_model::zero_first] = 0;
         }
-        wk = sir_model::compare<real>(cur.y, obs, lfact, sh, r, det);
+        uint32_t err = 0;
+        real ea = 0, eb = 0;
+        sir_model::update<real>(dt_r, part[k].y, sh, rng[k], det, err, ea, eb);
+        if (err) {
+          live[k] = false;
+          fail_mask |= 1u << k;
+          sd.errors[p0 + k] =
+              error_record{err, t, static_cast<double>(ea), static_cast<double>(eb)};
+          atomicAdd(sd.error_count, 1u);
+        }
       }
-      reg_rotate<PPT>(part, cur);
-      reg_rotate<PPT>(rng, r);
-      reg_rotate<PPT>(w, wk);
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+      w[k] = (p0 + k < P) ? sir_model::compare<real>(part[k].y, obs, lfact, sh, rng[k], det)
+                          : 0.0;
     }
     time = time + n_steps * dt;
 
