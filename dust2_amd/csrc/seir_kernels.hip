@@ -12,9 +12,11 @@
 //   A operand: lane l holds A[l % 16][4*kk + l / 16]
 //   B operand: lane l holds I[4*kk + l / 16][16*tile + l % 16]
 //   result:    lane l, element i holds lambda[4*(l / 16) + i][16*tile + l % 16]
-// Each wavefront keeps its 64 particles' states in LDS, one column per
-// lane ([state][lane], bank-conflict free), which also provides the
-// transposes the MFMA operands need.
+// Each lane keeps its particle's 80 states in registers (the age loop is
+// unrolled, so every index is a compile-time constant); LDS holds only the
+// wavefront's 16 x 64 infectious counts and forces of infection, which give
+// the MFMA operands their transposed layouts.  8 KB of LDS per wavefront
+// leaves the occupancy to the register budget.
 #include <hip/hip_runtime.h>
 
 #include "device_random.hpp"
@@ -34,10 +36,14 @@ constexpr int kWaveP = 64;
 constexpr int kParEI = 1, kParIR = 2, kParNoise = 3, kParI0 = 4, kParN = 5,
               kParA = 5 + kAge;
 
-__global__ void __launch_bounds__(kWaveP)
+#ifndef DUST2_SEIR_WAVES_PER_SIMD
+#define DUST2_SEIR_WAVES_PER_SIMD 3
+#endif
+// the register budget for DUST2_SEIR_WAVES_PER_SIMD resident wavefronts
+__global__ void __launch_bounds__(kWaveP, DUST2_SEIR_WAVES_PER_SIMD)
     seir_advance_kernel(const pipe_slot* __restrict__ slots, int t, int parity, uint32_t mode) {
-  __shared__ float st[kSeirState * kWaveP];
-  __shared__ float foi[kAge * kWaveP];
+  __shared__ float xi[kAge * kWaveP];   // I[a][lane]
+  __shared__ float foi[kAge * kWaveP];  // lambda[a][lane]
   const pipe_slot& sd = slots[blockIdx.y];
   const int P = static_cast<int>(sd.n_particles);
   const int lane = threadIdx.x;
@@ -51,25 +57,42 @@ __global__ void __launch_bounds__(kWaveP)
   const float* state_in = static_cast<const float*>(parity ? sd.state_b : sd.state_a);
   float* state_out = static_cast<float*>(parity ? sd.state_a : sd.state_b);
 
+  float S[kAge], E[kAge], I[kAge], R[kAge], inc[kAge];
   if (mode & PIPE_INITIAL) {
     const float I0 = static_cast<float>(pr[kParI0]);
+#pragma unroll
     for (int a = 0; a < kAge; ++a) {
       const float N = static_cast<float>(pr[kParN + a]);
-      st[(a)*kWaveP + lane] = active ? N - I0 : 0.f;
-      st[(kAge + a) * kWaveP + lane] = 0.f;
-      st[(2 * kAge + a) * kWaveP + lane] = active ? I0 : 0.f;
-      st[(3 * kAge + a) * kWaveP + lane] = 0.f;
-      st[(4 * kAge + a) * kWaveP + lane] = 0.f;
+      S[a] = active ? N - I0 : 0.f;
+      E[a] = 0.f;
+      I[a] = active ? I0 : 0.f;
+      R[a] = 0.f;
+      inc[a] = 0.f;
     }
   } else {
     const int src = active ? ((mode & PIPE_GATHER) ? static_cast<int>(sd.ancestors[p]) : p) : 0;
-    for (int c = 0; c < kSeirState; ++c) {
-      st[c * kWaveP + lane] = active ? state_in[static_cast<size_t>(c) * P + src] : 0.f;
+    const float* x = state_in + src;
+    const size_t Pz = static_cast<size_t>(P);
+#pragma unroll
+    for (int a = 0; a < kAge; ++a) {
+      S[a] = active ? x[a * Pz] : 0.f;
+      E[a] = active ? x[(kAge + a) * Pz] : 0.f;
+      I[a] = active ? x[(2 * kAge + a) * Pz] : 0.f;
+      R[a] = active ? x[(3 * kAge + a) * Pz] : 0.f;
+      inc[a] = active ? x[(4 * kAge + a) * Pz] : 0.f;
     }
   }
   if ((mode & PIPE_TRAJ_PREV) && sd.traj && active) {
-    float* tr = static_cast<float*>(sd.traj) + static_cast<size_t>(t - 1) * kSeirState * P;
-    for (int c = 0; c < kSeirState; ++c) tr[static_cast<size_t>(c) * P + p] = st[c * kWaveP + lane];
+    float* tr = static_cast<float*>(sd.traj) + static_cast<size_t>(t - 1) * kSeirState * P + p;
+    const size_t Pz = static_cast<size_t>(P);
+#pragma unroll
+    for (int a = 0; a < kAge; ++a) {
+      tr[a * Pz] = S[a];
+      tr[(kAge + a) * Pz] = E[a];
+      tr[(2 * kAge + a) * Pz] = I[a];
+      tr[(3 * kAge + a) * Pz] = R[a];
+      tr[(4 * kAge + a) * Pz] = inc[a];
+    }
   }
 
   if (mode & PIPE_STEP) {
@@ -95,8 +118,11 @@ __global__ void __launch_bounds__(kWaveP)
       const bool zero = s < 64 ? ((zero_mask >> s) & 1u) != 0
                                : fmod(sd.sched_time[t] + s * sd.dt, 7.0) == 0;
       if (zero) {
-        for (int a = 0; a < kAge; ++a) st[(4 * kAge + a) * kWaveP + lane] = 0.f;
+#pragma unroll
+        for (int a = 0; a < kAge; ++a) inc[a] = 0.f;
       }
+#pragma unroll
+      for (int a = 0; a < kAge; ++a) xi[a * kWaveP + lane] = I[a];
       __syncthreads();
       // force of infection on the matrix cores (all 64 lanes take part)
 #pragma unroll
@@ -104,7 +130,7 @@ __global__ void __launch_bounds__(kWaveP)
         v4f d = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-          const float b = st[(2 * kAge + 4 * kk + lane / 16) * kWaveP + tile * 16 + lane % 16];
+          const float b = xi[(4 * kk + lane / 16) * kWaveP + tile * 16 + lane % 16];
           d = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kk], b, d, 0, 0, 0);
         }
 #pragma unroll
@@ -114,32 +140,30 @@ __global__ void __launch_bounds__(kWaveP)
       }
       __syncthreads();
       if (active && !failed) {
-#pragma unroll 1
+#pragma unroll
         for (int a = 0; a < kAge; ++a) {
-          const float S = st[a * kWaveP + lane];
-          const float E = st[(kAge + a) * kWaveP + lane];
-          const float I = st[(2 * kAge + a) * kWaveP + lane];
+          if (failed) continue;
           const float lam = foi[a * kWaveP + lane];
           const float p_SE = prob_from_rate(-lam * dt);
           uint32_t err = 0;
           float bad_n = 0, bad_p = 0;
-          const float n_SE = binomial<float>(r, S, p_SE, det, err);
-          if (err) { bad_n = S; bad_p = p_SE; }
-          const float n_EI = err ? 0.f : binomial<float>(r, E, p_EI, det, err);
-          if (err && bad_n == 0 && bad_p == 0) { bad_n = E; bad_p = p_EI; }
-          const float n_IR = err ? 0.f : binomial<float>(r, I, p_IR, det, err);
-          if (err && bad_n == 0 && bad_p == 0) { bad_n = I; bad_p = p_IR; }
+          const float n_SE = binomial<float>(r, S[a], p_SE, det, err);
+          if (err) { bad_n = S[a]; bad_p = p_SE; }
+          const float n_EI = err ? 0.f : binomial<float>(r, E[a], p_EI, det, err);
+          if (err && bad_n == 0 && bad_p == 0) { bad_n = E[a]; bad_p = p_EI; }
+          const float n_IR = err ? 0.f : binomial<float>(r, I[a], p_IR, det, err);
+          if (err && bad_n == 0 && bad_p == 0) { bad_n = I[a]; bad_p = p_IR; }
           if (err) {
             sd.errors[p] = error_record{err, static_cast<uint32_t>(t), bad_n, bad_p};
             atomicAdd(sd.error_count, 1u);
             failed = true;
-            break;
+            continue;
           }
-          st[a * kWaveP + lane] = S - n_SE;
-          st[(kAge + a) * kWaveP + lane] = E + n_SE - n_EI;
-          st[(2 * kAge + a) * kWaveP + lane] = I + n_EI - n_IR;
-          st[(3 * kAge + a) * kWaveP + lane] += n_IR;
-          st[(4 * kAge + a) * kWaveP + lane] += n_SE;
+          S[a] = S[a] - n_SE;
+          E[a] = E[a] + n_SE - n_EI;
+          I[a] = I[a] + n_EI - n_IR;
+          R[a] += n_IR;
+          inc[a] += n_SE;
         }
       }
     }
@@ -151,11 +175,12 @@ __global__ void __launch_bounds__(kWaveP)
       const double* lf = sd.lfact + static_cast<size_t>(t) * sd.obs_stride;
       const float rate = static_cast<float>(pr[kParNoise]);
       double w = 0;
+#pragma unroll
       for (int a = 0; a < kAge; ++a) {
         const double x = obs[a];
         if (isnan(x)) continue;
         const double noise = static_cast<double>(exponential_rate<float>(r, rate, det));
-        const double lambda = static_cast<double>(st[(4 * kAge + a) * kWaveP + lane]) + noise;
+        const double lambda = static_cast<double>(inc[a]) + noise;
         w += dpois_log(x, lambda, lf[a]);
       }
       sd.lambda[p] = failed ? -__builtin_inf() : w;
@@ -166,8 +191,15 @@ __global__ void __launch_bounds__(kWaveP)
     }
   }
   if (active) {
-    for (int c = 0; c < kSeirState; ++c) {
-      state_out[static_cast<size_t>(c) * P + p] = st[c * kWaveP + lane];
+    float* y = state_out + p;
+    const size_t Pz = static_cast<size_t>(P);
+#pragma unroll
+    for (int a = 0; a < kAge; ++a) {
+      y[a * Pz] = S[a];
+      y[(kAge + a) * Pz] = E[a];
+      y[(2 * kAge + a) * Pz] = I[a];
+      y[(3 * kAge + a) * Pz] = R[a];
+      y[(4 * kAge + a) * Pz] = inc[a];
     }
   }
 }
