@@ -323,6 +323,8 @@ def bench_seir16(lib, args):
     data = simulate_seir_data(lib, plist, time, dt, seed=42)
     obj = lib.dust_filter_create(lib.seir16(), 0.0, {"time": time, "incidence": data}, P,
                                  n_groups=G, dt=dt, seed=1, precision="fp32")
+    # untimed: one and two launch chains, twice each, streams restored
+    tuned = None if args.no_tune else obj.autotune(plist)
     ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
     n_steps = int(round(time[-1] / dt))
     value = G * P * n_steps * args.steps / elapsed
@@ -350,8 +352,29 @@ def bench_seir16(lib, args):
            "config": {"workload": "age-structured SEIR16 bootstrap filter (MFMA FOI)",
                       "groups": G, "particles": P, "n_obs": n_obs, "dt": dt},
            "likelihood_evals_per_s": batched_evals, "plain_evals_per_s": plain_evals,
-           "batched_over_plain": batched_evals / plain_evals,
+           "batched_over_plain": batched_evals / plain_evals, "autotune": tuned,
            "mean_ll": float(np.mean(ll)), "roofline": rl}
+    if not args.no_cpu:
+        ol = oracle()
+        threads = cpu_threads()
+        rows = np.array([np.concatenate([[q["beta"], q["sigma"], q["gamma"], q["exp_noise"],
+                                          q["I0"]], q["N"], np.ravel(q["C"])])
+                         for q in plist[:1]])
+        n = 10
+        while True:
+            t0 = _time.perf_counter()
+            ol.sir_filter(rows, P, 0.0, time[:n], dt, data[:n, :1, :], seed=1,
+                          n_threads=threads, model="seir16")
+            el = _time.perf_counter() - t0
+            if el > 3 or n >= time.size:
+                break
+            n = min(time.size, n * 2)
+        steps = int(round(time[n - 1] / dt))
+        out["cpu_baseline"] = {
+            "value": P * steps / el, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 of {G} groups x {P} particles x {n} of {time.size} weekly obs, "
+                      f"FP64 oracle, {el:.1f} s"}
     return out
 
 
