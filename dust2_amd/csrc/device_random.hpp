@@ -56,7 +56,11 @@ __device__ __forceinline__ double random_real<double>(rng256& r) {
 // 24 high bits -> [0, 1) on the 2^-24 grid.
 template <>
 __device__ __forceinline__ float random_real<float>(rng256& r) {
-  return static_cast<float>(static_cast<uint32_t>(rng_next(r) >> 40)) * 0x1.0p-24f;
+  // bits 40..63 kept in place in the high word (exact in float): the same
+  // value as (x >> 40) * 2^-24, but a 32-bit conversion, where the shifted
+  // form folds into a 64-bit to float conversion sequence
+  return static_cast<float>(static_cast<uint32_t>(rng_next(r) >> 32) & 0xffffff00u) *
+         0x1.0p-32f;
 }
 
 __device__ __forceinline__ double d_log(double x) { return log(x); }
@@ -212,7 +216,7 @@ __device__ __forceinline__ float f_log1p(float x) {
 // 16..39 (the low bits of xoshiro256+ are the weak ones and are unused).
 __device__ __forceinline__ void random_pair_f32(rng256& r, float& u, float& v) {
   const uint64_t x = rng_next(r);
-  u = static_cast<float>(static_cast<uint32_t>(x >> 40)) * 0x1.0p-24f;
+  u = static_cast<float>(static_cast<uint32_t>(x >> 32) & 0xffffff00u) * 0x1.0p-32f;
   v = static_cast<float>(static_cast<uint32_t>(x >> 16) & 0xffffffu) * 0x1.0p-24f;
 }
 
