@@ -289,9 +289,20 @@ def bench_sirode(lib, args):
         res[prec] = {"evals_per_s": n * args.steps / el, "kernel_ms": float(np.mean(ms)),
                      "attempts_per_s": float(stats[:, 0].sum()) / (np.mean(ms) * 1e-3),
                      "mean_steps": float(stats[:, 0].mean()), "ll": ll}
+    # likelihood and gradient (beta, gamma, I0) by forward sensitivities, FP64
+    gobj = lib.dust_unfilter_create(lib.sirode(), 0.0, {"time": time, "incidence": data},
+                                    n_particles=1, n_groups=n)
+    gobj.run(plist, adjoint=True)
+    t0 = _time.perf_counter()
+    for _ in range(args.steps):
+        gobj.run(None, adjoint=True)
+    grad_el = _time.perf_counter() - t0
+    grad = gobj.last_gradient()
     out = {"metric": "likelihood evals/s", "value": res["fp64"]["evals_per_s"],
            "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": res["fp64"]["kernel_ms"], "higher_is_better": True, "scaling": "weak",
+           "gradient_evals_per_s": n * args.steps / grad_el,
+           "gradient_kernel_ms": gobj.last_stats()[1],
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SIR simulation)",
            "config": {"workload": "SIR ODE unfilter, Dormand-Prince 5(4)", "parameter_sets": n,
                       "n_obs": 365, "rtol": 1e-6, "atol": 1e-6},
@@ -308,6 +319,10 @@ def bench_sirode(lib, args):
         el = _time.perf_counter() - t0
         out["fp64_vs_oracle_max_rel_ll"] = float(np.max(np.abs(res["fp64"]["ll"][:m] - ref["ll"])
                                                         / np.abs(ref["ll"])))
+        k = 4096
+        _, ref_grad = ol.sirode_gradient(pars[:k], 0.0, time, data, n_threads=threads)
+        out["gradient_vs_oracle_max_rel"] = float(np.max(np.abs(grad[:k] - ref_grad)
+                                                         / np.abs(ref_grad)))
         out["cpu_baseline"] = {"value": m / el, "unit": "evals/s", "cores": threads,
                                "kind": "port", "sample": f"{m} parameter sets, FP64 oracle, "
                                                          f"{el:.1f} s"}

@@ -1741,7 +1741,10 @@ struct dust2_unfilter {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
-  dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active;
+  dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active,
+      d_grad;
+  std::vector<uint8_t> grad_current;  // per group: the gradient of the last run is valid
+  std::vector<size_t> last_index;     // groups of the last run
   size_t real_size() const { return fp64 ? 8 : 4; }
   ~dust2_unfilter() {
     if (ev0) (void)hipEventDestroy(ev0);
@@ -1795,6 +1798,7 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
     h->n_groups_data = n_groups_data;
     h->time_start = time_start;
     h->pars.assign(pars, pars + n_groups * kSirodePars);
+    h->grad_current.assign(n_groups, 0);
     // dust_ode_control() defaults (R/interface-continuous.R:51-79) with the
     // C++ controller constants (dust2/continuous/control.hpp:11-24)
     const double def[5] = {10000, 1e-6, 1e-6, 0, std::numeric_limits<double>::infinity()};
@@ -1844,53 +1848,120 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
   }
 }
 
+}  // extern "C"
+
+namespace {
+// One launch of the DP5 unfilter over the groups in idx; sens_param >= 0
+// runs the forward sensitivity of that parameter into d_grad as well.
+void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_param) {
+  hipStream_t s = h->stream;
+  std::vector<uint8_t> active(h->G, 0);
+  for (size_t g : idx) active[g] = 1;
+  to_device(h->d_active, active.data(), h->G, s);
+  hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+  ode_launch a{};
+  a.sens_param = sens_param;
+  a.grad = sens_param >= 0 ? h->d_grad.get<double>() : nullptr;
+  a.pars = h->d_pars.get<double>();
+  a.active = h->d_active.get<uint8_t>();
+  a.n_particles = static_cast<int>(h->P);
+  a.n_total = static_cast<int>(h->G * h->P);
+  a.time_start = h->time_start;
+  a.times = h->d_times.get<double>();
+  a.n_time = static_cast<int>(h->n_time);
+  a.obs = h->d_obs.get<double>();
+  a.lfact = h->d_lfact.get<double>();
+  a.obs_stride = h->n_groups_data;
+  a.obs_group_stride = h->n_groups_data == 1 ? 0 : 1;
+  a.control = h->control;
+  a.state = h->d_state.get();
+  a.ll = h->d_ll.get<double>();
+  a.stats = h->d_stats.get<double>();
+  a.errors = h->d_errors.get<error_record>();
+  a.error_count = h->d_counters.get<uint32_t>();
+  hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+  hip_check(launch_dp5_unfilter(a, h->fp64, s), "dp5 unfilter kernel");
+  hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+  uint32_t count = 0;
+  to_host(&count, h->d_counters, sizeof(count), s);
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+  h->last_ms = ms;
+  if (count > 0) {
+    std::vector<error_record> rec(h->G * h->P);
+    to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+    hip_check(hipStreamSynchronize(s), "sync");
+    throw std::runtime_error(error_report(rec, count));
+  }
+}
+
+void sirode_copy_ll(dust2_unfilter* h, const std::vector<size_t>& idx, double* ll_out) {
+  std::vector<double> ll(h->G * h->P);
+  to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), h->stream);
+  for (size_t i = 0; i < idx.size(); ++i) {
+    std::memcpy(ll_out + i * h->P, ll.data() + idx[i] * h->P, h->P * sizeof(double));
+  }
+}
+}  // namespace
+
+extern "C" {
+
 int dust2_unfilter_sirode_run(dust2_unfilter* h, const size_t* index_group, size_t n_index,
                               double* ll_out, char* err, size_t errlen) {
   try {
     check_unfilter(h);
     const auto idx = check_index(index_group, n_index, h->G);
-    hipStream_t s = h->stream;
-    std::vector<uint8_t> active(h->G, 0);
-    for (size_t g : idx) active[g] = 1;
-    to_device(h->d_active, active.data(), h->G, s);
-    hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
-    ode_launch a{};
-    a.pars = h->d_pars.get<double>();
-    a.active = h->d_active.get<uint8_t>();
-    a.n_particles = static_cast<int>(h->P);
-    a.n_total = static_cast<int>(h->G * h->P);
-    a.time_start = h->time_start;
-    a.times = h->d_times.get<double>();
-    a.n_time = static_cast<int>(h->n_time);
-    a.obs = h->d_obs.get<double>();
-    a.lfact = h->d_lfact.get<double>();
-    a.obs_stride = h->n_groups_data;
-    a.obs_group_stride = h->n_groups_data == 1 ? 0 : 1;
-    a.control = h->control;
-    a.state = h->d_state.get();
-    a.ll = h->d_ll.get<double>();
-    a.stats = h->d_stats.get<double>();
-    a.errors = h->d_errors.get<error_record>();
-    a.error_count = h->d_counters.get<uint32_t>();
-    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-    hip_check(launch_dp5_unfilter(a, h->fp64, s), "dp5 unfilter kernel");
-    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
-    uint32_t count = 0;
-    to_host(&count, h->d_counters, sizeof(count), s);
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
-    h->last_ms = ms;
-    if (count > 0) {
-      std::vector<error_record> rec(h->G * h->P);
-      to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
-      hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
-      hip_check(hipStreamSynchronize(s), "sync");
-      throw std::runtime_error(error_report(rec, count));
+    for (size_t g : idx) h->grad_current[g] = 0;
+    sirode_launch(h, idx, -1);
+    sirode_copy_ll(h, idx, ll_out);
+    h->last_index = idx;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, const size_t* index_group,
+                                       size_t n_index, double* ll_out, char* err,
+                                       size_t errlen) {
+  try {
+    check_unfilter(h);
+    if (!h->fp64) {
+      throw std::runtime_error("The ODE gradient is computed in FP64 only");
     }
-    std::vector<double> ll(h->G * h->P);
-    to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), s);
-    for (size_t i = 0; i < idx.size(); ++i) {
-      std::memcpy(ll_out + i * h->P, ll.data() + idx[i] * h->P, h->P * sizeof(double));
+    const auto idx = check_index(index_group, n_index, h->G);
+    for (size_t g : idx) h->grad_current[g] = 0;
+    if (h->d_grad.bytes() == 0) h->d_grad.alloc(h->G * h->P * 3 * sizeof(double));
+    double total_ms = 0;
+    for (int param = 0; param < 3; ++param) {
+      sirode_launch(h, idx, param);
+      total_ms += h->last_ms;
+    }
+    h->last_ms = total_ms;
+    sirode_copy_ll(h, idx, ll_out);
+    for (size_t g : idx) h->grad_current[g] = 1;
+    h->last_index = idx;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_last_gradient(dust2_unfilter* h, double* out, char* err,
+                                        size_t errlen) {
+  try {
+    check_unfilter(h);
+    bool any = false;
+    for (size_t g : h->last_index) any = any || h->grad_current[g];
+    if (!any) {
+      throw std::runtime_error("System was not run with 'adjoint = TRUE'");
+    }
+    std::vector<double> grad(h->G * h->P * 3);
+    to_host(grad.data(), h->d_grad, grad.size() * sizeof(double), h->stream);
+    for (size_t i = 0; i < h->last_index.size(); ++i) {
+      std::memcpy(out + 3 * h->P * i, grad.data() + 3 * h->P * h->last_index[i],
+                  3 * h->P * sizeof(double));
     }
     return 0;
   } catch (const std::exception& e) {
@@ -1911,6 +1982,7 @@ int dust2_unfilter_sirode_update_pars(dust2_unfilter* h, const double* pars,
       dst[1] = row[1];
       dst[2] = row[2];
       dst[3] = row[3];
+      h->grad_current[idx[i]] = 0;
     }
     to_device(h->d_pars, h->pars.data(), h->pars.size() * sizeof(double), h->stream);
     hip_check(hipStreamSynchronize(h->stream), "sync");
@@ -2497,6 +2569,7 @@ void srv_run_ode(dust2_server* sv, hipStream_t s) {
   hip_check(hipMemsetAsync(sv->d_ode_errors.get(), 0, n * sizeof(error_record), s), "memset");
   hip_check(hipMemsetAsync(sv->d_ode_counters.get(), 0, 16, s), "memset");
   ode_launch a{};
+  a.sens_param = -1;
   a.pars = sv->d_ode_pars.get<double>();
   a.n_particles = 1;
   a.n_total = static_cast<int>(n);

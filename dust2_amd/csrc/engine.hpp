@@ -175,6 +175,10 @@ struct ode_launch {
   // time; null for one series shared through obs_stride/obs_group_stride
   const int64_t* series_offset;
   const int32_t* series_len;
+  // forward sensitivity run (FP64): parameter 0 beta, 1 gamma, 2 I0, or -1
+  // for none; grad[3 * particle + sens_param] receives d ll / d theta
+  int sens_param;
+  double* grad;
 };
 
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream);

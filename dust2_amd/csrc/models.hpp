@@ -123,6 +123,35 @@ struct sirode_model {
     dy[3] = rate_SI;
     dy[4] = rate_SI;
   }
+
+  // Forward sensitivity s = dy/dtheta (theta: 0 beta, 1 gamma, 2 I0):
+  // ds/dt = J(y) s + df/dtheta.
+  template <typename real>
+  __device__ __forceinline__ static void rhs_sens(const real* y, const real* s,
+                                                  const shared<real>& sh, int param,
+                                                  real* ds) {
+    const real S = y[0];
+    const real I = y[1];
+    real d_SI = sh.beta * (s[0] * I + S * s[1]) / sh.N;
+    real d_IR = sh.gamma * s[1];
+    if (param == 0) d_SI += S * I / sh.N;
+    if (param == 1) d_IR += I;
+    ds[0] = -d_SI;
+    ds[1] = d_SI - d_IR;
+    ds[2] = d_IR;
+    ds[3] = d_SI;
+    ds[4] = d_SI;
+  }
+
+  // dy0/dtheta of the initial state
+  template <typename real>
+  __device__ __forceinline__ static void initial_sens(int param, real* s) {
+    for (int i = 0; i < 5; ++i) s[i] = 0;
+    if (param == 2) {
+      s[0] = -1;
+      s[1] = 1;
+    }
+  }
 };
 
 }  // namespace dust2_amd

@@ -53,17 +53,34 @@ struct ode_state {
   double t0, h;  // last accepted step
   real c1, c2, c3, c4, c5;  // dense output of variable zero_first
   uint32_t n_steps, n_accepted, n_rejected;
+  // forward sensitivity run: s = dy/dtheta, its derivative and the dense
+  // output of its reset component
+  real sy[NV];
+  real dsydt[NV];
+  real sc1, sc2, sc3, sc4, sc5;
+  int param;
 };
 
 template <typename real>
-__device__ __forceinline__ real interp(const ode_state<real>& s, double t) {
-  const double u = (t - s.t0) / s.h;
+__device__ __forceinline__ real interp5(double t0, double h, real c1, real c2, real c3, real c4,
+                                        real c5, double t) {
+  const double u = (t - t0) / h;
   const double v = 1 - u;
-  return static_cast<real>(static_cast<double>(s.c1) +
-                           u * (static_cast<double>(s.c2) +
-                                v * (static_cast<double>(s.c3) +
-                                     u * (static_cast<double>(s.c4) +
-                                          v * static_cast<double>(s.c5)))));
+  return static_cast<real>(static_cast<double>(c1) +
+                           u * (static_cast<double>(c2) +
+                                v * (static_cast<double>(c3) +
+                                     u * (static_cast<double>(c4) +
+                                          v * static_cast<double>(c5)))));
+}
+
+template <typename real>
+__device__ __forceinline__ real interp(const ode_state<real>& s, double t) {
+  return interp5(s.t0, s.h, s.c1, s.c2, s.c3, s.c4, s.c5, t);
+}
+
+template <typename real>
+__device__ __forceinline__ real interp_sens(const ode_state<real>& s, double t) {
+  return interp5(s.t0, s.h, s.sc1, s.sc2, s.sc3, s.sc4, s.sc5, t);
 }
 
 template <typename real>
@@ -74,46 +91,93 @@ __device__ __forceinline__ void rhs(const real* y, const sirode_model::shared<re
 
 // One attempted step of size h from (t, y) with k1 = dydt; returns the
 // FP64 error norm and leaves the 5th-order solution in y_next, the FSAL
-// derivative in k2 and the dense-output coefficient in c5.
-template <typename real>
+// derivative in k2 and the dense-output coefficient in c5.  With SENS the
+// sensitivity s = dy/dtheta takes the same stages (its stage derivatives
+// evaluated at the state's stage values) into sy_next / ks2 / sc5; it does
+// not enter the error norm, so the step sequence is the plain run's.
+template <typename real, bool SENS>
 __device__ __forceinline__ double try_step(double t, double h, const ode_state<real>& s,
                                            const sirode_model::shared<real>& sh,
                                            const ode_control_dev& ctl, real* y_next, real* k2,
-                                           real& c5_out) {
+                                           real& c5_out, real* sy_next, real* ks2,
+                                           real& sc5_out) {
   using namespace dp;
   const real* y = s.y;
   const real* k1 = s.dydt;
+  const real* sy = s.sy;
+  const real* q1 = s.dsydt;
   const real hr = static_cast<real>(h);
   real k3[NV], k4[NV], k5[NV], k6[NV], ys[NV];
+  real q3[NV], q4[NV], q5[NV], q6[NV], qs[NV];
   (void)t;
   for (int i = 0; i < NV; ++i) y_next[i] = y[i] + hr * static_cast<real>(A21) * k1[i];
   rhs<real>(y_next, sh, k2);
+  if constexpr (SENS) {
+    for (int i = 0; i < NV; ++i) sy_next[i] = sy[i] + hr * static_cast<real>(A21) * q1[i];
+    sirode_model::rhs_sens<real>(y_next, sy_next, sh, s.param, ks2);
+  }
   for (int i = 0; i < NV; ++i)
     y_next[i] = y[i] + hr * (static_cast<real>(A31) * k1[i] + static_cast<real>(A32) * k2[i]);
   rhs<real>(y_next, sh, k3);
+  if constexpr (SENS) {
+    for (int i = 0; i < NV; ++i)
+      sy_next[i] = sy[i] + hr * (static_cast<real>(A31) * q1[i] + static_cast<real>(A32) * ks2[i]);
+    sirode_model::rhs_sens<real>(y_next, sy_next, sh, s.param, q3);
+  }
   for (int i = 0; i < NV; ++i)
     y_next[i] = y[i] + hr * (static_cast<real>(A41) * k1[i] + static_cast<real>(A42) * k2[i] +
                              static_cast<real>(A43) * k3[i]);
   rhs<real>(y_next, sh, k4);
+  if constexpr (SENS) {
+    for (int i = 0; i < NV; ++i)
+      sy_next[i] = sy[i] + hr * (static_cast<real>(A41) * q1[i] + static_cast<real>(A42) * ks2[i] +
+                                 static_cast<real>(A43) * q3[i]);
+    sirode_model::rhs_sens<real>(y_next, sy_next, sh, s.param, q4);
+  }
   for (int i = 0; i < NV; ++i)
     y_next[i] = y[i] + hr * (static_cast<real>(A51) * k1[i] + static_cast<real>(A52) * k2[i] +
                              static_cast<real>(A53) * k3[i] + static_cast<real>(A54) * k4[i]);
   rhs<real>(y_next, sh, k5);
+  if constexpr (SENS) {
+    for (int i = 0; i < NV; ++i)
+      sy_next[i] = sy[i] + hr * (static_cast<real>(A51) * q1[i] + static_cast<real>(A52) * ks2[i] +
+                                 static_cast<real>(A53) * q3[i] + static_cast<real>(A54) * q4[i]);
+    sirode_model::rhs_sens<real>(y_next, sy_next, sh, s.param, q5);
+  }
   for (int i = 0; i < NV; ++i)
     ys[i] = y[i] + hr * (static_cast<real>(A61) * k1[i] + static_cast<real>(A62) * k2[i] +
                          static_cast<real>(A63) * k3[i] + static_cast<real>(A64) * k4[i] +
                          static_cast<real>(A65) * k5[i]);
   rhs<real>(ys, sh, k6);
+  if constexpr (SENS) {
+    for (int i = 0; i < NV; ++i)
+      qs[i] = sy[i] + hr * (static_cast<real>(A61) * q1[i] + static_cast<real>(A62) * ks2[i] +
+                            static_cast<real>(A63) * q3[i] + static_cast<real>(A64) * q4[i] +
+                            static_cast<real>(A65) * q5[i]);
+    sirode_model::rhs_sens<real>(ys, qs, sh, s.param, q6);
+  }
   for (int i = 0; i < NV; ++i)
     y_next[i] = y[i] + hr * (static_cast<real>(A71) * k1[i] + static_cast<real>(A73) * k3[i] +
                              static_cast<real>(A74) * k4[i] + static_cast<real>(A75) * k5[i] +
                              static_cast<real>(A76) * k6[i]);
   rhs<real>(y_next, sh, k2);
+  if constexpr (SENS) {
+    for (int i = 0; i < NV; ++i)
+      sy_next[i] = sy[i] + hr * (static_cast<real>(A71) * q1[i] + static_cast<real>(A73) * q3[i] +
+                                 static_cast<real>(A74) * q4[i] + static_cast<real>(A75) * q5[i] +
+                                 static_cast<real>(A76) * q6[i]);
+    sirode_model::rhs_sens<real>(y_next, sy_next, sh, s.param, ks2);
+  }
   {
     const int i = sirode_model::zero_first;
     c5_out = hr * (static_cast<real>(D1) * k1[i] + static_cast<real>(D3) * k3[i] +
                    static_cast<real>(D4) * k4[i] + static_cast<real>(D5) * k5[i] +
                    static_cast<real>(D6) * k6[i] + static_cast<real>(D7) * k2[i]);
+    if constexpr (SENS) {
+      sc5_out = hr * (static_cast<real>(D1) * q1[i] + static_cast<real>(D3) * q3[i] +
+                      static_cast<real>(D4) * q4[i] + static_cast<real>(D5) * q5[i] +
+                      static_cast<real>(D6) * q6[i] + static_cast<real>(D7) * ks2[i]);
+    }
   }
   double err = 0;
   for (int i = 0; i < NV; ++i) {
@@ -130,7 +194,7 @@ __device__ __forceinline__ double try_step(double t, double h, const ode_state<r
 }
 
 // solver::step: returns the new time, or NaN with *code set on failure.
-template <typename real>
+template <typename real, bool SENS>
 __device__ double ode_step(double t, double t_end, ode_state<real>& s,
                            const sirode_model::shared<real>& sh, const ode_control_dev& ctl,
                            uint32_t& code) {
@@ -140,6 +204,7 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
   bool truncated = false;
   double h = s.step_size;
   real y_next[NV], k2[NV];
+  real sy_next[NV], ks2[NV];
   while (true) {
     if (s.n_steps > ctl.max_steps) {
       code = ERR_ODE_MAX_STEPS;
@@ -157,8 +222,9 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
     if (truncated) {
       h = t_end - t;
     }
-    real c5;
-    const double err = try_step<real>(t, h, s, sh, ctl, y_next, k2, c5);
+    real c5, sc5 = 0;
+    const double err =
+        try_step<real, SENS>(t, h, s, sh, ctl, y_next, k2, c5, sy_next, ks2, sc5);
     s.n_steps++;
     const double fac11 = pow(err, ctl.constant);
     if (err <= 1) {
@@ -171,6 +237,19 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
       s.c3 = bspl;
       s.c4 = -static_cast<real>(h) * k2[z] + ydiff - bspl;
       s.c5 = c5;
+      if constexpr (SENS) {
+        const real sdiff = sy_next[z] - s.sy[z];
+        const real sbspl = static_cast<real>(h) * s.dsydt[z] - sdiff;
+        s.sc1 = s.sy[z];
+        s.sc2 = sdiff;
+        s.sc3 = sbspl;
+        s.sc4 = -static_cast<real>(h) * ks2[z] + sdiff - sbspl;
+        s.sc5 = sc5;
+        for (int i = 0; i < NV; ++i) {
+          s.dsydt[i] = ks2[i];
+          s.sy[i] = sy_next[i];
+        }
+      }
       s.t0 = t;
       s.h = h;
       for (int i = 0; i < NV; ++i) {
@@ -239,7 +318,7 @@ __device__ void ode_initialise(ode_state<real>& s, const sirode_model::shared<re
   s.step_size = h;
 }
 
-template <typename real>
+template <typename real, bool SENS>
 __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= args.n_total) return;
@@ -262,6 +341,13 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
   s.c1 = s.c2 = s.c3 = s.c4 = s.c5 = 0;
   uint32_t code = 0;
   ode_initialise<real>(s, sh, ctl, code);
+  double dll = 0;
+  if constexpr (SENS) {
+    s.param = args.sens_param;
+    sirode_model::initial_sens<real>(s.param, s.sy);
+    sirode_model::rhs_sens<real>(s.y, s.sy, sh, s.param, s.dsydt);
+    s.sc1 = s.sc2 = s.sc3 = s.sc4 = s.sc5 = 0;
+  }
   double time = args.time_start;
   double ll = 0;
   size_t dstride = args.obs_stride;
@@ -289,12 +375,14 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
           const double t_reset = n * period;
           if (t_reset == tt) {
             s.y[sirode_model::zero_first] = 0;
+            if constexpr (SENS) s.sy[sirode_model::zero_first] = 0;
           } else {
             s.y[sirode_model::zero_first] -= interp(s, t_reset);
+            if constexpr (SENS) s.sy[sirode_model::zero_first] -= interp_sens(s, t_reset);
           }
         }
       }
-      tt = ode_step<real>(tt, t_end, s, sh, ctl, code);
+      tt = ode_step<real, SENS>(tt, t_end, s, sh, ctl, code);
     }
     // apply_zero_every_final (solver.hpp:468-492)
     if (code == 0 && s.h != 0 && s.h > sirode_model::zero_period) {
@@ -303,6 +391,7 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
       const int m = static_cast<int>(ceil(s.t0 / period));
       if (n > m) {
         s.y[sirode_model::zero_first] -= interp(s, (n - 1) * period);
+        if constexpr (SENS) s.sy[sirode_model::zero_first] -= interp_sens(s, (n - 1) * period);
       }
     }
     time = t_end;
@@ -319,6 +408,10 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
         v = x * log(lambda) - lambda - args.lfact[t * dstride + doff];
       }
       ll += v;
+      if constexpr (SENS) {
+        // d/dlambda log Poisson(x; lambda) = x / lambda - 1
+        dll += (x == 0 ? -1.0 : x / lambda - 1) * static_cast<double>(s.sy[4]);
+      }
     }
   }
   args.ll[i] = ll;
@@ -328,6 +421,9 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
     args.stats[4 * static_cast<size_t>(i) + 1] = s.n_accepted;
     args.stats[4 * static_cast<size_t>(i) + 2] = s.n_rejected;
     args.stats[4 * static_cast<size_t>(i) + 3] = s.step_size;
+  }
+  if constexpr (SENS) {
+    args.grad[3 * static_cast<size_t>(i) + args.sens_param] = dll;
   }
   if (code) {
     args.errors[i] = error_record{code, 0u, 0.0, 0.0};
@@ -340,10 +436,16 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream) {
   if (args.n_total == 0) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((args.n_total + 255) / 256);
-  if (fp64) {
-    hipLaunchKernelGGL(dp5_unfilter_kernel<double>, dim3(blocks), dim3(256), 0, stream, args);
+  if (args.sens_param >= 0) {
+    if (!fp64 || args.grad == nullptr || args.sens_param > 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((dp5_unfilter_kernel<double, true>), dim3(blocks), dim3(256), 0, stream,
+                       args);
+  } else if (fp64) {
+    hipLaunchKernelGGL((dp5_unfilter_kernel<double, false>), dim3(blocks), dim3(256), 0,
+                       stream, args);
   } else {
-    hipLaunchKernelGGL(dp5_unfilter_kernel<float>, dim3(blocks), dim3(256), 0, stream, args);
+    hipLaunchKernelGGL((dp5_unfilter_kernel<float, false>), dim3(blocks), dim3(256), 0,
+                       stream, args);
   }
   return hipGetLastError();
 }

@@ -563,11 +563,11 @@ class Unfilter:
             self.rows[sel] = rows
         ll = np.zeros((n_out, self.n_particles))
         if self.continuous:
-            if adjoint:
-                raise ValueError("The ODE unfilter has no adjoint; use the discrete model")
             if initial is not None:
                 raise ValueError("'initial' is not supported for the ODE unfilter")
-            call("dust2_unfilter_sirode_run", self.ptr, szptr(idx), n_idx, dptr(ll))
+            # adjoint = True: the gradient by forward sensitivities
+            name = "dust2_unfilter_sirode_run_gradient" if adjoint else "dust2_unfilter_sirode_run"
+            call(name, self.ptr, szptr(idx), n_idx, dptr(ll))
         else:
             init = None
             if initial is not None:
@@ -584,13 +584,16 @@ class Unfilter:
     def last_gradient(self):
         """dust_likelihood_last_gradient (R/interface-likelihood.R): d ll /
         d (beta, gamma, I0) per group of the last run."""
-        if self.continuous:
-            raise ValueError("The ODE unfilter has no adjoint")
         if self.ptr is None:
             raise ValueError("System was not run with 'adjoint = TRUE'")
         n = self.n_groups if self.last_index is None else np.size(self.last_index)
-        out = np.zeros((n, 3))
-        call("dust2_unfilter_sir_last_gradient", self.ptr, dptr(out))
+        if self.continuous:
+            out = np.zeros((n, self.n_particles, 3))
+            call("dust2_unfilter_sirode_last_gradient", self.ptr, dptr(out))
+            out = out[:, 0] if self.n_particles == 1 else out
+        else:
+            out = np.zeros((n, 3))
+            call("dust2_unfilter_sir_last_gradient", self.ptr, dptr(out))
         return out[0] if (self.n_groups == 1 and self.last_index is None) else out
 
     def last_state(self):

@@ -219,6 +219,18 @@ int dust2_unfilter_sirode_last_state(dust2_unfilter* h, double* out, char* err, 
  * n_steps_accepted, n_steps_rejected, step_size; kernel_ms may be NULL. */
 int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* kernel_ms,
                                      char* err, size_t errlen);
+/* Likelihood and its gradient with respect to (beta, gamma, I0) by forward
+ * sensitivities of the ODE, integrated with the same steps as the state (the
+ * sensitivities do not enter the error control), so ll_out is exactly the
+ * plain run's.  No counterpart in the reference, whose only gradient is the
+ * discrete adjoint (dust2/unfilter.hpp:77-229); the interface mirrors that
+ * one.  FP64 only. */
+int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, const size_t* index_group,
+                                       size_t n_index, double* ll_out, char* err,
+                                       size_t errlen);
+/* out [n_index of the last run][P][3] = d ll / d (beta, gamma, I0). */
+int dust2_unfilter_sirode_last_gradient(dust2_unfilter* h, double* out, char* err,
+                                        size_t errlen);
 void dust2_unfilter_sirode_free(dust2_unfilter* h);
 
 /* ---- deterministic discrete-time SIR unfilter with the adjoint ----------

@@ -44,6 +44,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -532,6 +533,58 @@ struct sirode {
   }
 };
 
+// Forward sensitivities of the SIR ODE with respect to one parameter
+// (beta, gamma or I0): the state is augmented with s = dy/dtheta, which
+// obeys ds/dt = J(y) s + df/dtheta.  No counterpart in the reference (its
+// only gradient is the discrete adjoint).  Layout [S, I, R, cum, sS, sI,
+// sR, scum, inc, sinc] keeps the two reset variables contiguous; only the
+// five model states enter the error norm and the initial step, so the
+// step sequence, the states and the likelihood are those of `sirode`.
+struct sirode_sens {
+  static constexpr size_t n_state = 10;
+  struct shared_state {
+    sirode::shared_state base;
+    int param;  // 0: beta, 1: gamma, 2: I0
+  };
+  static constexpr double zero_period = 1;
+  static constexpr size_t zero_first = 8;
+  static constexpr size_t zero_count = 2;
+  static bool error_component(size_t i) { return i < 4 || i == 8; }
+  static constexpr size_t n_error = 5;
+  static void initial(const shared_state& sh, double* y) {
+    for (size_t i = 0; i < n_state; ++i) y[i] = 0;
+    y[0] = sh.base.N - sh.base.I0;
+    y[1] = sh.base.I0;
+    if (sh.param == 2) {
+      y[4] = -1;
+      y[5] = 1;
+    }
+  }
+  static void rhs(double, const double* y, const shared_state& sh, double* dy) {
+    const auto& b = sh.base;
+    const double S = y[0];
+    const double I = y[1];
+    const double rate_SI = b.beta * S * I / b.N;
+    const double rate_IR = b.gamma * I;
+    dy[0] = -rate_SI;
+    dy[1] = rate_SI - rate_IR;
+    dy[2] = rate_IR;
+    dy[3] = rate_SI;
+    dy[8] = rate_SI;
+    const double sS = y[4];
+    const double sI = y[5];
+    double d_SI = b.beta * (sS * I + S * sI) / b.N;
+    double d_IR = b.gamma * sI;
+    if (sh.param == 0) d_SI += S * I / b.N;
+    if (sh.param == 1) d_IR += I;
+    dy[4] = -d_SI;
+    dy[5] = d_SI - d_IR;
+    dy[6] = d_IR;
+    dy[7] = d_SI;
+    dy[9] = d_SI;
+  }
+};
+
 // Age-structured stochastic SEIR with 16 age groups.  No counterpart exists
 // in the reference; written to the same contract.  State layout:
 // S[0:16] E[16:32] I[32:48] R[48:64] inc[64:80]; incidence is reset every
@@ -985,10 +1038,32 @@ constexpr double D1 = -12715105075.0 / 11282082432.0,
                  D7 = 69997945.0 / 29380423.0;
 }  // namespace dp5
 
+// Models may restrict the error norm and the initial step to some state
+// components (error_component(i), n_error); by default all take part.
+template <typename U, typename = void>
+struct has_error_mask : std::false_type {};
+template <typename U>
+struct has_error_mask<U, std::void_t<decltype(U::error_component(0))>> : std::true_type {};
+
 template <typename T>
 class ode_solver {
  public:
   static constexpr size_t N = T::n_state;
+  static bool in_error(size_t i) {
+    if constexpr (has_error_mask<T>::value) {
+      return T::error_component(i);
+    } else {
+      (void)i;
+      return true;
+    }
+  }
+  static constexpr size_t n_error() {
+    if constexpr (has_error_mask<T>::value) {
+      return T::n_error;
+    } else {
+      return N;
+    }
+  }
   using shared_state = typename T::shared_state;
   using internals = ode_internals<N>;
   ode_control control;
@@ -1027,12 +1102,13 @@ class ode_solver {
                     E6 * k6_[i] + E7 * k2_[i]);
     double err = 0;
     for (size_t i = 0; i < N; ++i) {
+      if (!in_error(i)) continue;
       const double sk =
           k4_[i] / (control.atol +
                     control.rtol * std::max(std::fabs(y[i]), std::fabs(y_next_[i])));
       err += sk * sk;
     }
-    return std::sqrt(err / N);
+    return std::sqrt(err / n_error());
   }
 
   double step(double t, double t_end, double* y, internals& in,
@@ -1161,6 +1237,7 @@ class ode_solver {
     double norm_f = 0;
     double norm_y = 0;
     for (size_t i = 0; i < N; ++i) {
+      if (!in_error(i)) continue;
       const double sk = control.atol + control.rtol * std::fabs(y[i]);
       norm_f += (f0[i] / sk) * (f0[i] / sk);
       norm_y += (y[i] / sk) * (y[i] / sk);
@@ -1173,6 +1250,7 @@ class ode_solver {
     T::rhs(t + h, y1, sh, f1);
     double der2 = 0;
     for (size_t i = 0; i < N; ++i) {
+      if (!in_error(i)) continue;
       const double sk = control.atol + control.rtol * std::fabs(y[i]);
       der2 += ((f1[i] - f0[i]) / sk) * ((f1[i] - f0[i]) / sk);
     }

@@ -413,4 +413,63 @@ void oracle_trajectory_lineage(const double* raw, const uint32_t* ancestry, size
   }
 }
 
+// Log-likelihood of the SIR ODE unfilter and its gradient with respect to
+// (beta, gamma, I0) by forward sensitivities (sirode_sens): one augmented
+// integration per parameter; the likelihood is that of the plain unfilter.
+// d/dlambda log Poisson(x; lambda) = x / lambda - 1.
+int oracle_sirode_unfilter_gradient(const double* pars, size_t G, double time_start,
+                                    const double* time, size_t n_time, const double* data,
+                                    size_t n_groups_data, const double* control_in,
+                                    int n_threads, double* ll_out, double* grad_out,
+                                    char* err, size_t errlen) {
+  try {
+    ode_control ctl;
+    if (control_in) {
+      ctl.max_steps = static_cast<size_t>(control_in[0]);
+      ctl.atol = control_in[1];
+      ctl.rtol = control_in[2];
+      ctl.step_size_min = std::max(control_in[3], std::numeric_limits<double>::epsilon());
+      ctl.step_size_max = control_in[4];
+    }
+    std::string failure;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(n_threads) collapse(2)
+#endif
+    for (size_t g = 0; g < G; ++g) {
+      for (int param = 0; param < 3; ++param) {
+        try {
+          sirode_sens::shared_state sh{sirode::build_shared(pars + g * sirode::n_pars), param};
+          double y[sirode_sens::n_state];
+          sirode_sens::initial(sh, y);
+          ode_internals<sirode_sens::n_state> in;
+          ode_solver<sirode_sens> solver(ctl);
+          solver.initialise(time_start, y, in, sh);
+          double t = time_start;
+          double ll = 0, dll = 0;
+          for (size_t i = 0; i < n_time; ++i) {
+            solver.run(t, time[i], y, in, sh);
+            t = time[i];
+            const double x = data[i * n_groups_data + (n_groups_data == 1 ? 0 : g)];
+            if (std::isnan(x)) continue;
+            const double lambda = y[8];
+            ll += dpois_log(x, lambda);
+            dll += (x == 0 ? -1.0 : x / lambda - 1) * y[9];
+          }
+          if (param == 0) ll_out[g] = ll;
+          grad_out[3 * g + param] = dll;
+        } catch (const std::exception& e) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+          failure = e.what();
+        }
+      }
+    }
+    if (!failure.empty()) throw std::runtime_error(failure);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 }  // extern "C"

@@ -51,6 +51,9 @@ def oracle():
         o.oracle_scale_log_weights.restype = C.c_double
         o.oracle_scale_log_weights.argtypes = [C.c_size_t, D]
         o.oracle_resample_weight.argtypes = [C.c_size_t, D, C.c_double, U32]
+        o.oracle_sirode_unfilter_gradient.argtypes = [D, C.c_size_t, C.c_double, D, C.c_size_t,
+                                                      D, C.c_size_t, D, C.c_int, D, D,
+                                                      C.c_char_p, C.c_size_t]
         o.oracle_trajectory_lineage.argtypes = [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
                                                 C.c_size_t, SZ, D]
         o.oracle_trajectory_lineage.restype = None
@@ -252,6 +255,27 @@ def trajectory_lineage(raw, ancestry=None, select=None):
                                        None if sel is None else sel.ctypes.data_as(SZ),
                                        _dp(out))
     return out
+
+
+def sirode_gradient(pars, time_start, time, data, control=None, n_threads=1):
+    """Log-likelihood and d ll / d (beta, gamma, I0) of the SIR ODE
+    unfilter by forward sensitivities; pars (G, 4), data (n_time,) or
+    (n_time, G)."""
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    n_groups_data = 1 if data.ndim == 1 else data.shape[1]
+    ctl = None if control is None else np.ascontiguousarray(control, dtype=np.float64)
+    ll = np.zeros(G)
+    grad = np.zeros((G, 3))
+    buf = C.create_string_buffer(4096)
+    rc = oracle().oracle_sirode_unfilter_gradient(_dp(pars), G, float(time_start), _dp(time),
+                                                  time.size, _dp(data), n_groups_data,
+                                                  _dp(ctl), n_threads, _dp(ll), _dp(grad),
+                                                  buf, 4096)
+    _check(rc, buf)
+    return ll, grad
 
 
 def sir_unfilter(pars, time_start, time, dt, data, gradient=True):

@@ -102,3 +102,49 @@ def test_too_many_steps_error(gpu_lib):
     with pytest.raises(gpu_lib.DustError, match="too many steps"):
         make_unfilter(gpu_lib, pars, np.arange(1.0, 11.0), np.ones(10),
                       control=gpu_lib.dust_ode_control(max_steps=5))
+
+
+def test_gradient_matches_oracle_and_keeps_likelihood(gpu_lib):
+    rng = np.random.default_rng(21)
+    G = 256
+    pars = np.column_stack([np.full(G, 1000.0), rng.uniform(5, 20, G),
+                            rng.uniform(0.1, 0.5, G), rng.uniform(0.05, 0.2, G)])
+    time = np.arange(1.0, 121.0)
+    data = rng.poisson(8, (120, G)).astype(float)
+    data[[7, 50], :] = np.nan
+    plist = [dict(zip(["N", "I0", "beta", "gamma"], p)) for p in pars]
+    obj = gpu_lib.dust_unfilter_create(gpu_lib.sirode(), 0.0, {"time": time, "incidence": data},
+                                       n_particles=1, n_groups=G)
+    ll_plain = obj.run(plist)
+    with pytest.raises(gpu_lib.DustError, match="System was not run with 'adjoint = TRUE'"):
+        obj.last_gradient()
+    ll = gpu_lib.dust_likelihood_run(obj, None, adjoint=True)
+    grad = gpu_lib.dust_likelihood_last_gradient(obj)
+    assert np.array_equal(ll, ll_plain)       # same steps: the plain likelihood
+    ref_ll, ref_grad = ol.sirode_gradient(pars, 0.0, time, data, n_threads=8)
+    np.testing.assert_allclose(ll, ref_ll, rtol=1e-10)
+    np.testing.assert_allclose(grad, ref_grad, rtol=1e-10, atol=1e-10 * np.abs(ref_grad).max())
+
+
+def test_gradient_matches_central_differences(gpu_lib):
+    rng = np.random.default_rng(22)
+    G = 32
+    pars = np.column_stack([np.full(G, 1000.0), rng.uniform(5, 20, G),
+                            rng.uniform(0.1, 0.5, G), rng.uniform(0.05, 0.2, G)])
+    time = np.arange(1.0, 61.0)
+    data = rng.poisson(8, 60).astype(float)
+    obj = gpu_lib.dust_unfilter_create(gpu_lib.sirode(), 0.0, {"time": time, "incidence": data},
+                                       n_particles=1, n_groups=G)
+    names = ["N", "I0", "beta", "gamma"]
+    gpu_lib.dust_likelihood_run(obj, [dict(zip(names, p)) for p in pars], adjoint=True)
+    grad = obj.last_gradient()
+    fd = np.zeros_like(grad)
+    for j, col in enumerate([2, 3, 1]):
+        h = 1e-6 * pars[:, col]
+        up, dn = pars.copy(), pars.copy()
+        up[:, col] += h
+        dn[:, col] -= h
+        lu = obj.run([dict(zip(names, p)) for p in up])
+        ld = obj.run([dict(zip(names, p)) for p in dn])
+        fd[:, j] = (lu - ld) / (2 * h)
+    np.testing.assert_allclose(grad, fd, rtol=1e-6)
