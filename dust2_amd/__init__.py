@@ -17,6 +17,7 @@ from .interface import (  # noqa: F401
     dust_likelihood_last_stats,
     dust_likelihood_last_trajectories,
     dust_likelihood_last_trajectories_raw,
+    dust_likelihood_last_snapshots,
     dust_likelihood_rng_state,
     dust_likelihood_run,
     dust_likelihood_set_rng_state,

@@ -37,6 +37,9 @@ _PROTOS = {
     "dust2_filter_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_last_trajectories": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
     "dust2_filter_last_trajectories_raw": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_set_save_snapshots": (C.c_int, [VP, D, C.c_size_t, ERR, ERRLEN]),
+    "dust2_filter_last_snapshots": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
+    "dust2_filter_last_state_selected": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_last_ancestry": (C.c_int, [VP, U32, ERR, ERRLEN]),
     "dust2_filter_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),

@@ -377,6 +377,9 @@ struct dust2_filter {
   bool traj_valid = false;
   bool ancestry_valid = false;
   std::vector<size_t> last_index;        // groups of the last run
+  std::vector<double> next_snapshots;    // snapshot times requested for the next run
+  std::vector<int32_t> snap_slot;        // per data time: snapshot slot or -1
+  size_t n_snap = 0;                     // snapshots of the last run
   std::vector<uint8_t> select_valid;     // per group: a random particle is chosen
   dbuf d_select, d_groups, d_traj_out;   // lineage export
   double last_ms = 0;
@@ -850,11 +853,16 @@ int dust2_filter_seir16_alloc(const double* pars, size_t n_groups, size_t n_part
                            errlen);
 }
 
-int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectories,
+int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectories_in,
                          const size_t* index_group, size_t n_index, double* ll_out, char* err,
                          size_t errlen) {
   try {
     check_filter(h);
+    // snapshots need the saved states and ancestry of every time, as the
+    // reference's filter does (dust2/filter.hpp:47-49)
+    std::vector<double> snapshots;
+    snapshots.swap(h->next_snapshots);
+    const int save_trajectories = (save_trajectories_in || !snapshots.empty()) ? 1 : 0;
     const auto idx = check_index(index_group, n_index, h->G);
     const size_t G = h->G, P = h->P;
     hipStream_t s = h->stream;
@@ -931,8 +939,13 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     h->last_ms = ms;
     h->last_fallbacks = counters[1];
-    h->traj_valid = save_trajectories != 0;
+    h->traj_valid = save_trajectories_in != 0;
     h->ancestry_valid = save_trajectories != 0;
+    h->snap_slot.assign(h->n_time, -1);
+    for (size_t i = 0, j = 0; i < h->n_time && j < snapshots.size(); ++i) {
+      if (h->times[i] == snapshots[j]) h->snap_slot[i] = static_cast<int32_t>(j++);
+    }
+    h->n_snap = snapshots.size();
     h->last_index = idx;
     // a new run forgets the chosen particles of its groups (filter.hpp:162-175)
     if (h->select_valid.size() != G) h->select_valid.assign(G, 0);
@@ -1054,8 +1067,8 @@ int dust2_filter_last_trajectories(dust2_filter* h, int select_random_particle, 
     hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(), groups,
                              static_cast<int>(n_out), static_cast<int>(h->n_time),
                              static_cast<int>(h->S), static_cast<int>(h->P),
-                             select ? h->d_select.get<uint32_t>() : nullptr, h->fp64,
-                             h->d_traj_out.get<double>(), h->stream),
+                             select ? h->d_select.get<uint32_t>() : nullptr, nullptr,
+                             h->fp64, h->d_traj_out.get<double>(), h->stream),
               "lineage kernel");
     to_host(out, h->d_traj_out, bytes, h->stream);
     return 0;
@@ -1085,6 +1098,83 @@ int dust2_filter_last_trajectories_raw(dust2_filter* h, double* out, char* err,
                         : reinterpret_cast<const float*>(tmp.data())[o];
           }
         }
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_set_save_snapshots(dust2_filter* h, const double* times, size_t n,
+                                    char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    std::vector<double> snaps(times, times + n);
+    for (size_t i = 1; i < n; ++i) {
+      if (!(snaps[i] > snaps[i - 1])) {
+        throw std::runtime_error("'save_snapshots' must be strictly increasing");
+      }
+    }
+    for (double t : snaps) {
+      if (std::find(h->times.begin(), h->times.end(), t) == h->times.end()) {
+        throw std::runtime_error("All elements of 'save_snapshots' must be found in 'time'");
+      }
+    }
+    h->next_snapshots = snaps;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_last_snapshots(dust2_filter* h, int select_random_particle, double* out,
+                                char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (!h->ancestry_valid || h->n_snap == 0) {
+      throw std::runtime_error("Snapshots are not current");
+    }
+    const bool select = select_random_particle != 0;
+    const uint32_t* groups = lineage_groups(h, select);
+    const size_t n_out = h->last_index.size();
+    const size_t p_out = select ? 1 : h->P;
+    const size_t bytes = h->n_snap * n_out * p_out * h->S * sizeof(double);
+    if (h->d_traj_out.bytes() < bytes) h->d_traj_out.alloc(bytes);
+    dbuf d_slot(h->n_time * sizeof(int32_t));
+    to_device(d_slot, h->snap_slot.data(), h->n_time * sizeof(int32_t), h->stream);
+    hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(), groups,
+                             static_cast<int>(n_out), static_cast<int>(h->n_time),
+                             static_cast<int>(h->S), static_cast<int>(h->P),
+                             select ? h->d_select.get<uint32_t>() : nullptr,
+                             d_slot.get<int32_t>(), h->fp64, h->d_traj_out.get<double>(),
+                             h->stream),
+              "lineage kernel");
+    to_host(out, h->d_traj_out, bytes, h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_last_state_selected(dust2_filter* h, double* out, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (h->last_index.empty()) {
+      throw std::runtime_error("State is not current");
+    }
+    lineage_groups(h, true);
+    std::vector<uint32_t> sel(h->G);
+    to_host(sel.data(), h->d_select, h->G * sizeof(uint32_t), h->stream);
+    std::vector<char> tmp(h->d_state.bytes());
+    to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
+    const size_t P = h->P, S = h->S;
+    for (size_t i = 0; i < h->last_index.size(); ++i) {
+      const size_t g = h->last_index[i];
+      for (size_t c = 0; c < S; ++c) {
+        const size_t o = (g * S + c) * P + sel[g];
+        out[i * S + c] = h->fp64 ? reinterpret_cast<const double*>(tmp.data())[o]
+                                 : reinterpret_cast<const float*>(tmp.data())[o];
       }
     }
     return 0;
@@ -1680,7 +1770,8 @@ int dust2_amd_trajectory_lineage(const double* raw, const uint32_t* ancestry, si
     hip_check(launch_lineage(dt.get(), ancestry ? da.get<uint32_t>() : nullptr,
                              dg.get<uint32_t>(), static_cast<int>(G), static_cast<int>(T),
                              static_cast<int>(S), static_cast<int>(P),
-                             select ? ds.get<uint32_t>() : nullptr, true, dout.get<double>(), s),
+                             select ? ds.get<uint32_t>() : nullptr, nullptr, true,
+                             dout.get<double>(), s),
               "lineage kernel");
     to_host(out, dout, T * G * p_out * S * sizeof(double), s);
     return 0;

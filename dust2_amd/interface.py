@@ -222,6 +222,7 @@ class Filter:
         self.launch = None
         self.engine = None
         self.last_index = None
+        self.last_snapshots = None
 
     @property
     def initialised(self):
@@ -299,7 +300,7 @@ def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=
 
 
 def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_group=None,
-                        adjoint=False):
+                        adjoint=False, save_snapshots=None):
     """Run the filter (or unfilter) and return the log-likelihood per group
     (a float for a single group), R/interface-likelihood.R:48-119."""
     if isinstance(obj, Unfilter):
@@ -322,14 +323,40 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
             raise ValueError("Expected 'initial' to have n_groups * n_particles * n_state values")
     ll = np.zeros(n_out)
     obj.last_index = index_group
+    obj.last_snapshots = None
+    if save_snapshots is not None and np.size(save_snapshots) > 0:
+        snaps = f64(save_snapshots).reshape(-1)
+        call("dust2_filter_set_save_snapshots", obj.ptr, dptr(snaps), snaps.size)
+        obj.last_snapshots = snaps
     call("dust2_filter_run", obj.ptr, dptr(init), int(bool(save_trajectories)), szptr(idx),
          n_idx, dptr(ll))
     return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
 
 
-def dust_likelihood_last_state(obj):
+def dust_likelihood_last_state(obj, select_random_particle=False):
+    """Final state (n_groups, n_particles, n_state); with
+    select_random_particle, the chosen particle of each group of the last
+    run (n_groups, n_state)."""
+    if select_random_particle:
+        n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
+        out = np.zeros((n_groups, obj.generator.n_state))
+        call("dust2_filter_last_state_selected", obj.ptr, dptr(out))
+        return out
     out = np.zeros((obj.n_groups, obj.n_particles, obj.generator.n_state))
     call("dust2_filter_last_state", obj.ptr, dptr(out))
+    return out
+
+
+def dust_likelihood_last_snapshots(obj, select_random_particle=False):
+    """The whole state at the snapshot times of the last run, following each
+    exported particle's ancestry: (n_snapshots, n_groups, n_particles or 1,
+    n_state)."""
+    if obj.ptr is None or getattr(obj, "last_snapshots", None) is None:
+        raise DustError("Snapshots are not current")
+    n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
+    p_out = 1 if select_random_particle else obj.n_particles
+    out = np.zeros((obj.last_snapshots.size, n_groups, p_out, obj.generator.n_state))
+    call("dust2_filter_last_snapshots", obj.ptr, int(bool(select_random_particle)), dptr(out))
     return out
 
 

@@ -113,6 +113,26 @@ int dust2_filter_last_state(dust2_filter* h, double* out, char* err, size_t errl
 int dust2_filter_last_trajectories(dust2_filter* h, int select_random_particle, double* out,
                                    char* err, size_t errlen);
 
+/* Snapshot times for the next dust2_filter_run (dust_likelihood_run's
+ * save_snapshots, R/interface-likelihood.R:83-93): strictly increasing, each
+ * one of the data times ("All elements of 'save_snapshots' must be found in
+ * 'time'").  The run then keeps every time's state and ancestry. */
+int dust2_filter_set_save_snapshots(dust2_filter* h, const double* times, size_t n,
+                                    char* err, size_t errlen);
+
+/* Replaces dust2_filter_last_snapshots (dust2/r/filter.hpp:97-139) with
+ * trajectories<T>::export_snapshots(reorder = true) (dust2/trajectories.hpp:
+ * 191-283): the whole state at the snapshot times along each exported
+ * particle's ancestry.  out [n_snapshots][n_groups of the last run][P or 1][S];
+ * "Snapshots are not current" if the last run saved none. */
+int dust2_filter_last_snapshots(dust2_filter* h, int select_random_particle, double* out,
+                                char* err, size_t errlen);
+
+/* dust2_filter_last_state with select_random_particle (dust2/r/filter.hpp:
+ * 141-180): the state of the chosen particle (the same choice as the
+ * trajectory export) of each group of the last run: out [n_groups][S]. */
+int dust2_filter_last_state_selected(dust2_filter* h, double* out, char* err, size_t errlen);
+
 /* The saved post-resampling states as stored, without following the
  * ancestry: out [n_time][G][P][S].  No counterpart in the reference. */
 int dust2_filter_last_trajectories_raw(dust2_filter* h, double* out, char* err,

@@ -100,3 +100,30 @@ def test_index_group_exports_selected_groups(gpu_lib):
     assert np.all(np.isfinite(ll))
     assert np.all(sub[..., 0] + sub[..., 1] + sub[..., 2] == 1e4)
     assert full.shape == (12, 4, 64, 5)
+
+
+def test_snapshots_follow_ancestry(gpu_lib):
+    G, P, T = 3, 150, 20
+    obj, plist = filter_with_trajectories(gpu_lib, G=G, P=P, n_time=T)
+    snaps = np.array([3.0, 10.0, 17.0])
+    gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True, save_snapshots=snaps)
+    h = gpu_lib.dust_likelihood_last_trajectories(obj)
+    s = gpu_lib.dust_likelihood_last_snapshots(obj)
+    assert s.shape == (3, G, P, 5)
+    np.testing.assert_array_equal(s, h[snaps.astype(int) - 1])
+    s1 = gpu_lib.dust_likelihood_last_snapshots(obj, select_random_particle=True)
+    h1 = gpu_lib.dust_likelihood_last_trajectories(obj, select_random_particle=True)
+    np.testing.assert_array_equal(s1, h1[snaps.astype(int) - 1])
+    # the chosen particle is shared with last_state(select_random_particle)
+    st = gpu_lib.dust_likelihood_last_state(obj, select_random_particle=True)
+    np.testing.assert_array_equal(st, h1[-1, :, 0])
+    # snapshots alone: no trajectories, and the next run forgets them
+    gpu_lib.dust_likelihood_run(obj, None, save_snapshots=[5.0])
+    with pytest.raises(gpu_lib.DustError, match="Trajectories are not current"):
+        gpu_lib.dust_likelihood_last_trajectories(obj)
+    assert gpu_lib.dust_likelihood_last_snapshots(obj).shape == (1, G, P, 5)
+    gpu_lib.dust_likelihood_run(obj, None)
+    with pytest.raises(gpu_lib.DustError, match="Snapshots are not current"):
+        gpu_lib.dust_likelihood_last_snapshots(obj)
+    with pytest.raises(gpu_lib.DustError, match="must be found in 'time'"):
+        gpu_lib.dust_likelihood_run(obj, None, save_snapshots=[2.5])
