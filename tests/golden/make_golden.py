@@ -76,6 +76,29 @@ def sirode_fixture():
     }
 
 
+def sirode_weekly_fixture():
+    """Weekly data with the daily incidence reset: solver steps span day
+    boundaries, so the reset subtracts the dense-output value at the
+    boundary inside a step (dust2/continuous/solver.hpp:437-492)."""
+    rng = np.random.default_rng(7)
+    n_sets = 8
+    beta = rng.uniform(0.1, 0.5, n_sets)
+    gamma = rng.uniform(0.05, 0.2, n_sets)
+    pars = np.column_stack([np.full(n_sets, 1000.0), np.full(n_sets, 10.0), beta, gamma])
+    time = np.arange(7, 7 * 53, 7, dtype=np.float64)
+    data = rng.poisson(6, size=time.size).astype(np.float64)
+    res = ol.ref_sirode(pars, 0.0, time, data)
+    ctl2 = [100000, 1e-4, 1e-4, 0, np.inf]
+    res2 = ol.ref_sirode(pars, 0.0, time, data, control=ctl2)
+    return {
+        "pars": pars.tolist(), "time_start": 0.0, "time": time.tolist(),
+        "data": data.tolist(), "ll": res["ll"].tolist(), "stats": res["stats"].tolist(),
+        "state": res["state"].tolist(), "control2": [100000, 1e-4, 1e-4, 0, 1e300],
+        "ll2": res2["ll"].tolist(), "stats2": res2["stats"].tolist(),
+        "state2": res2["state"].tolist(),
+    }
+
+
 def logistic_fixture():
     time = np.array([0.5, 1, 2, 5, 10, 20, 30], dtype=np.float64)
     out = ol.ref_logistic(0.1, 0.2, 100.0, 1.0, time)
@@ -90,6 +113,8 @@ def main():
         json.dump(filter_details(), f)
     with open(os.path.join(HERE, "sirode_dp5.json"), "w") as f:
         json.dump(sirode_fixture(), f)
+    with open(os.path.join(HERE, "sirode_weekly_dp5.json"), "w") as f:
+        json.dump(sirode_weekly_fixture(), f)
     with open(os.path.join(HERE, "logistic_dp5.json"), "w") as f:
         json.dump(logistic_fixture(), f)
 

@@ -148,3 +148,20 @@ def test_gradient_matches_central_differences(gpu_lib):
         ld = obj.run([dict(zip(names, p)) for p in dn])
         fd[:, j] = (lu - ld) / (2 * h)
     np.testing.assert_allclose(grad, fd, rtol=1e-6)
+
+
+@pytest.mark.parametrize("which", ["default", "loose"])
+def test_weekly_data_matches_reference_solver(gpu_lib, which):
+    # steps spanning several daily resets (reset by the dense output at the
+    # boundary, final correction after the last step), reference goldens
+    with open(os.path.join(os.path.dirname(__file__), "golden", "sirode_weekly_dp5.json")) as f:
+        g = json.load(f)
+    pars = np.array(g["pars"])
+    control = None if which == "default" else gpu_lib.dust_ode_control(*g["control2"])
+    key = "" if which == "default" else "2"
+    obj, ll = make_unfilter(gpu_lib, pars, g["time"], np.array(g["data"]), control=control)
+    np.testing.assert_allclose(ll, g["ll" + key], rtol=1e-10)
+    stats, _ = obj.last_stats()
+    np.testing.assert_array_equal(stats[:, :3], np.array(g["stats" + key])[:, :3])
+    np.testing.assert_allclose(obj.last_state()[:, 0, :], np.array(g["state" + key])[:, -1, :],
+                               rtol=1e-10, atol=1e-9)

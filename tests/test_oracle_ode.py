@@ -143,3 +143,18 @@ def test_ode_sensitivity_gradient_matches_finite_differences(control):
     ref = ol.sirode_unfilter(pars, 1, 0.0, time, data, control=ctl)["ll"]
     assert np.array_equal(ll, ref)
     np.testing.assert_allclose(grad, _ode_fd(pars, time, data, control), rtol=1e-6)
+
+
+def test_sirode_weekly_data_matches_reference_solver():
+    # steps spanning several daily resets: the reset subtracts the dense
+    # output at the boundary, and after the last step the final correction
+    g = load("sirode_weekly_dp5.json")
+    pars = np.array(g["pars"])
+    for ll_key, stats_key, state_key, ctl in [("ll", "stats", "state", None),
+                                              ("ll2", "stats2", "state2", g["control2"])]:
+        res = ol.sirode_unfilter(pars, 1, g["time_start"], g["time"], np.array(g["data"]),
+                                 control=ctl)
+        assert res["ll"].tolist() == g[ll_key]
+        assert res["stats"].tolist() == g[stats_key]
+        assert np.array_equal(res["state"][:, 0, :], np.array(g[state_key])[:, -1, :])
+        assert np.max(res["stats"][:, 3]) > 1  # steps longer than the reset period
