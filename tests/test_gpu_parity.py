@@ -320,3 +320,54 @@ def test_autotune_keeps_results_and_streams(gpu_lib):
     np.testing.assert_array_equal(ll, ll_ref)
     np.testing.assert_array_equal(gpu_lib.dust_likelihood_rng_state(obj),
                                   gpu_lib.dust_likelihood_rng_state(ref))
+
+
+NAMES = ["N", "I0", "beta", "gamma", "exp_noise"]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_filter_initial_state_rng_state_and_parameter_change(gpu_lib, engine):
+    # test-filter.R:207-307: manual initial state, parameters changed between
+    # runs (the streams carry on), and streams set before a run
+    G, P, T = 2, 300, 25
+    pars = np.array([[2000, 10, 0.3, 0.1, 1e6], [5000, 20, 0.25, 0.12, 1e6]])
+    time = np.arange(1.0, T + 1)
+    data = np.stack([sir_data(T, seed=3 + g) for g in range(G)], axis=1)
+    obj, ll1 = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 9, engine=engine)
+    rng1 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    ref1 = ol.sir_filter(pars, P, 0.0, time, 0.25, data, seed=9, n_threads=4)
+    np.testing.assert_allclose(ll1, ref1["ll"], rtol=1e-12)
+    # second run with new beta continues the streams
+    pars2 = pars.copy()
+    pars2[:, 2] = [0.35, 0.2]
+    ll2 = gpu_lib.dust_likelihood_run(obj, [{"beta": b} for b in pars2[:, 2]])
+    ref2 = ol.sir_filter(pars2, P, 0.0, time, 0.25, data, rng=rng1, n_threads=4)
+    np.testing.assert_allclose(ll2, ref2["ll"], rtol=1e-12)
+    # running does not touch filter stream 2g+1 (test-filter.R:118-148)
+    rng2 = gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4)
+    assert np.array_equal(rng2[:, 1], rng1.reshape(G, P + 2, 4)[:, 1])
+    # a manual initial state
+    rng_init = np.random.default_rng(5)
+    init = np.zeros((G, P, 5))
+    init[:, :, 1] = rng_init.integers(5, 50, (G, P))
+    init[:, :, 0] = pars2[:, None, 0] - init[:, :, 1]
+    rng3 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    ll3 = gpu_lib.dust_likelihood_run(obj, None, initial=init)
+    ref3 = ol.sir_filter(pars2, P, 0.0, time, 0.25, data, rng=rng3, initial=init, n_threads=4)
+    np.testing.assert_allclose(ll3, ref3["ll"], rtol=1e-12)
+    assert np.array_equal(gpu_lib.dust_likelihood_last_state(obj), ref3["state"])
+    # streams set before a run (test-filter.R:249-276)
+    gpu_lib.dust_likelihood_set_rng_state(obj, rng1)
+    ll4 = gpu_lib.dust_likelihood_run(obj, None)
+    np.testing.assert_array_equal(ll4, ll2)
+
+
+def test_shared_data_equals_replicated_data(gpu_lib):
+    # test-filter.R:747-772
+    G, P, T = 3, 200, 20
+    pars = np.array([[2000, 10, 0.3 + 0.02 * g, 0.1, 1e6] for g in range(G)])
+    time = np.arange(1.0, T + 1)
+    d = sir_data(T, seed=11)
+    _, ll_shared = run_gpu_filter(gpu_lib, pars, P, time, 0.25, d, 4)
+    _, ll_rep = run_gpu_filter(gpu_lib, pars, P, time, 0.25, np.tile(d[:, None], (1, G)), 4)
+    np.testing.assert_array_equal(ll_shared, ll_rep)
