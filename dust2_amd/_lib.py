@@ -67,17 +67,21 @@ _PROTOS = {
     "dust2_unfilter_sirode_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                               C.c_size_t, D, C.c_size_t, D, C.c_int,
                                               C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_unfilter_sirode_run": (C.c_int, [VP, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_run": (C.c_int, [VP, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_stats": (C.c_int, [VP, D, D, ERR, ERRLEN]),
-    "dust2_unfilter_sirode_run_gradient": (C.c_int, [VP, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_run_gradient": (C.c_int, [VP, C.c_int, SZ, C.c_size_t, D, ERR,
+                                                     ERRLEN]),
     "dust2_unfilter_sirode_last_gradient": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_free": (None, [VP]),
     "dust2_unfilter_sir_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                            C.c_size_t, C.c_double, D, C.c_size_t,
                                            C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_unfilter_sir_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sir_run": (C.c_int, [VP, D, C.c_int, C.c_int, SZ, C.c_size_t, D, ERR,
+                                         ERRLEN]),
+    "dust2_unfilter_sir_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sir_last_gradient": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sir_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_unfilter_sir_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),

@@ -76,6 +76,10 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
     const size_t o = static_cast<size_t>(i) * a.obs_stride + g * a.obs_group_stride;
     ll += sir_model::compare<double>(y, a.obs + o, a.lfact + o, sh, det_rng, true);
     if (a.inc != nullptr) a.inc[i * NT + k] = y[4];
+    if (a.traj != nullptr) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) a.traj[(static_cast<size_t>(i) * 5 + c) * NT + k] = y[c];
+    }
   }
 #pragma unroll
   for (int c = 0; c < 5; ++c) a.state[c * NT + k] = y[c];

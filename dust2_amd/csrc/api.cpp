@@ -1833,7 +1833,8 @@ struct dust2_unfilter {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active,
-      d_grad;
+      d_grad, d_traj;
+  bool traj_valid = false;
   std::vector<uint8_t> grad_current;  // per group: the gradient of the last run is valid
   std::vector<size_t> last_index;     // groups of the last run
   size_t real_size() const { return fp64 ? 8 : 4; }
@@ -1944,7 +1945,8 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
 namespace {
 // One launch of the DP5 unfilter over the groups in idx; sens_param >= 0
 // runs the forward sensitivity of that parameter into d_grad as well.
-void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_param) {
+void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_param,
+                   bool save_traj) {
   hipStream_t s = h->stream;
   std::vector<uint8_t> active(h->G, 0);
   for (size_t g : idx) active[g] = 1;
@@ -1953,6 +1955,10 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
   ode_launch a{};
   a.sens_param = sens_param;
   a.grad = sens_param >= 0 ? h->d_grad.get<double>() : nullptr;
+  if (save_traj && h->d_traj.bytes() == 0) {
+    h->d_traj.alloc(h->n_time * 5 * h->G * h->P * h->real_size());
+  }
+  a.traj = save_traj ? h->d_traj.get() : nullptr;
   a.pars = h->d_pars.get<double>();
   a.active = h->d_active.get<uint8_t>();
   a.n_particles = static_cast<int>(h->P);
@@ -1998,13 +2004,16 @@ void sirode_copy_ll(dust2_unfilter* h, const std::vector<size_t>& idx, double* l
 
 extern "C" {
 
-int dust2_unfilter_sirode_run(dust2_unfilter* h, const size_t* index_group, size_t n_index,
-                              double* ll_out, char* err, size_t errlen) {
+int dust2_unfilter_sirode_run(dust2_unfilter* h, int save_trajectories,
+                              const size_t* index_group, size_t n_index, double* ll_out,
+                              char* err, size_t errlen) {
   try {
     check_unfilter(h);
     const auto idx = check_index(index_group, n_index, h->G);
     for (size_t g : idx) h->grad_current[g] = 0;
-    sirode_launch(h, idx, -1);
+    h->traj_valid = false;
+    sirode_launch(h, idx, -1, save_trajectories != 0);
+    h->traj_valid = save_trajectories != 0;
     sirode_copy_ll(h, idx, ll_out);
     h->last_index = idx;
     return 0;
@@ -2013,9 +2022,9 @@ int dust2_unfilter_sirode_run(dust2_unfilter* h, const size_t* index_group, size
   }
 }
 
-int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, const size_t* index_group,
-                                       size_t n_index, double* ll_out, char* err,
-                                       size_t errlen) {
+int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, int save_trajectories,
+                                       const size_t* index_group, size_t n_index,
+                                       double* ll_out, char* err, size_t errlen) {
   try {
     check_unfilter(h);
     if (!h->fp64) {
@@ -2025,10 +2034,12 @@ int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, const size_t* index_gr
     for (size_t g : idx) h->grad_current[g] = 0;
     if (h->d_grad.bytes() == 0) h->d_grad.alloc(h->G * h->P * 3 * sizeof(double));
     double total_ms = 0;
+    h->traj_valid = false;
     for (int param = 0; param < 3; ++param) {
-      sirode_launch(h, idx, param);
+      sirode_launch(h, idx, param, param == 0 && save_trajectories != 0);
       total_ms += h->last_ms;
     }
+    h->traj_valid = save_trajectories != 0;
     h->last_ms = total_ms;
     sirode_copy_ll(h, idx, ll_out);
     for (size_t g : idx) h->grad_current[g] = 1;
@@ -2053,6 +2064,34 @@ int dust2_unfilter_sirode_last_gradient(dust2_unfilter* h, double* out, char* er
     for (size_t i = 0; i < h->last_index.size(); ++i) {
       std::memcpy(out + 3 * h->P * i, grad.data() + 3 * h->P * h->last_index[i],
                   3 * h->P * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_last_trajectories(dust2_unfilter* h, double* out, char* err,
+                                            size_t errlen) {
+  try {
+    check_unfilter(h);
+    if (!h->traj_valid) throw std::runtime_error("Trajectories are not current");
+    const size_t NT = h->G * h->P, T = h->n_time;
+    std::vector<char> tmp(T * 5 * NT * h->real_size());
+    to_host(tmp.data(), h->d_traj, tmp.size(), h->stream);
+    // device [t][S][G*P] -> out [t][groups of the last run][P][S]
+    for (size_t t = 0; t < T; ++t) {
+      for (size_t i = 0; i < h->last_index.size(); ++i) {
+        for (size_t p = 0; p < h->P; ++p) {
+          const size_t k = h->last_index[i] * h->P + p;
+          for (size_t c = 0; c < 5; ++c) {
+            const size_t o = (t * 5 + c) * NT + k;
+            out[((t * h->last_index.size() + i) * h->P + p) * 5 + c] =
+                h->fp64 ? reinterpret_cast<const double*>(tmp.data())[o]
+                        : reinterpret_cast<const float*>(tmp.data())[o];
+          }
+        }
+      }
     }
     return 0;
   } catch (const std::exception& e) {
@@ -2131,7 +2170,8 @@ struct dust2_unfilter_sir {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   dbuf d_pars, d_sched_time, d_sched_steps, d_obs, d_lfact, d_state, d_ll, d_errors,
-      d_counters, d_active, d_hist, d_inc, d_grad;
+      d_counters, d_active, d_hist, d_inc, d_grad, d_traj;
+  bool traj_valid = false;
   ~dust2_unfilter_sir() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -2234,9 +2274,9 @@ int dust2_unfilter_sir_alloc(const double* pars, size_t n_groups, size_t n_parti
   }
 }
 
-int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adjoint,
-                           const size_t* index_group, size_t n_index, double* ll_out,
-                           char* err, size_t errlen) {
+int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int save_trajectories,
+                           int adjoint, const size_t* index_group, size_t n_index,
+                           double* ll_out, char* err, size_t errlen) {
   try {
     check_unfilter_sir(h);
     const auto idx = check_index(index_group, n_index, h->G);
@@ -2263,8 +2303,12 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adj
         h->d_grad.alloc(n_total * 3 * sizeof(double));
       }
     }
+    if (save_trajectories && h->d_traj.bytes() == 0) {
+      h->d_traj.alloc(h->n_time * 5 * n_total * sizeof(double));
+    }
     hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
     sir_unfilter_launch a{};
+    a.traj = save_trajectories ? h->d_traj.get<double>() : nullptr;
     a.pars = h->d_pars.get<double>();
     a.active = h->d_active.get<uint8_t>();
     a.n_particles = static_cast<int>(h->P);
@@ -2294,6 +2338,7 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adj
     hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     h->last_ms = ms;
     for (size_t g : idx) h->grad_current[g] = 0;
+    h->traj_valid = false;
     if (count > 0) {
       std::vector<error_record> rec(n_total);
       to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
@@ -2304,6 +2349,7 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adj
     if (adjoint) {
       for (size_t g : idx) h->grad_current[g] = 1;
     }
+    h->traj_valid = save_trajectories != 0;
     h->last_index = idx;
     std::vector<double> ll(n_total);
     to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), s);
@@ -2332,6 +2378,30 @@ int dust2_unfilter_sir_last_gradient(dust2_unfilter_sir* h, double* out, char* e
     to_host(grad.data(), h->d_grad, grad.size() * sizeof(double), h->stream);
     for (size_t i = 0; i < h->last_index.size(); ++i) {
       std::memcpy(out + 3 * i, grad.data() + 3 * h->last_index[i], 3 * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_last_trajectories(dust2_unfilter_sir* h, double* out, char* err,
+                                         size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    if (!h->traj_valid) throw std::runtime_error("Trajectories are not current");
+    const size_t NT = h->G * h->P, T = h->n_time;
+    std::vector<double> tmp(T * 5 * NT);
+    to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->stream);
+    for (size_t t = 0; t < T; ++t) {
+      for (size_t i = 0; i < h->last_index.size(); ++i) {
+        for (size_t p = 0; p < h->P; ++p) {
+          const size_t k = h->last_index[i] * h->P + p;
+          for (size_t c = 0; c < 5; ++c) {
+            out[((t * h->last_index.size() + i) * h->P + p) * 5 + c] = tmp[(t * 5 + c) * NT + k];
+          }
+        }
+      }
     }
     return 0;
   } catch (const std::exception& e) {

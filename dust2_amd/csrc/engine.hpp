@@ -179,6 +179,8 @@ struct ode_launch {
   // for none; grad[3 * particle + sens_param] receives d ll / d theta
   int sens_param;
   double* grad;
+  // state at each data time, real [n_time][S][n_total], or null
+  void* traj;
 };
 
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream);
@@ -203,6 +205,7 @@ struct sir_unfilter_launch {
   double* hist;                 // [total steps][2][n_total] or null (no gradient)
   double* inc;                  // [n_time][n_total] or null
   double* grad;                 // [n_total][3] or null
+  double* traj;                 // [n_time][S][n_total] state at each data time, or null
   error_record* errors;         // [n_total]
   uint32_t* error_count;
 };

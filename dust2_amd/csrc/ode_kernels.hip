@@ -395,6 +395,10 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
       }
     }
     time = t_end;
+    if (args.traj != nullptr) {
+      real* tr = static_cast<real*>(args.traj) + static_cast<size_t>(t) * NV * args.n_total;
+      for (int k = 0; k < NV; ++k) tr[static_cast<size_t>(k) * args.n_total + i] = s.y[k];
+    }
     // compare: log Poisson(obs; cases_inc) (sirode.cpp:89-101)
     const double x = args.obs[t * dstride + doff];
     if (!isnan(x)) {

@@ -304,7 +304,8 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
     """Run the filter (or unfilter) and return the log-likelihood per group
     (a float for a single group), R/interface-likelihood.R:48-119."""
     if isinstance(obj, Unfilter):
-        return obj.run(pars, index_group, initial=initial, adjoint=adjoint)
+        return obj.run(pars, index_group, initial=initial, adjoint=adjoint,
+                       save_trajectories=save_trajectories)
     if not obj.initialised:
         if pars is None:
             raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
@@ -361,10 +362,13 @@ def dust_likelihood_last_snapshots(obj, select_random_particle=False):
 
 
 def dust_likelihood_last_trajectories(obj, select_random_particle=False):
-    """R/interface-likelihood.R dust_likelihood_last_trajectories: each
-    particle's trajectory follows its ancestry back from the last time
-    (dust2/trajectories.hpp:128-189); (n_time, n_groups, n_particles or 1,
-    n_state), groups of the last run."""
+    """dust_likelihood_last_trajectories (R/interface-likelihood.R): for a
+    filter each particle's trajectory follows its ancestry back from the
+    last time (dust2/trajectories.hpp:128-189); for an unfilter the state at
+    every data time.  (n_time, n_groups of the last run, n_particles or 1,
+    n_state)."""
+    if isinstance(obj, Unfilter):
+        return obj.last_trajectories()
     if obj.ptr is None:
         raise DustError("Trajectories are not current")
     n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
@@ -577,7 +581,8 @@ class Unfilter:
                  self.n_groups_data, C.byref(out))
         self.ptr = out
 
-    def run(self, pars, index_group=None, initial=None, adjoint=False):
+    def run(self, pars, index_group=None, initial=None, adjoint=False,
+            save_trajectories=False):
         idx, n_idx, n_out = _index_group(index_group, self.n_groups)
         if self.ptr is None:
             if pars is None:
@@ -594,14 +599,14 @@ class Unfilter:
                 raise ValueError("'initial' is not supported for the ODE unfilter")
             # adjoint = True: the gradient by forward sensitivities
             name = "dust2_unfilter_sirode_run_gradient" if adjoint else "dust2_unfilter_sirode_run"
-            call(name, self.ptr, szptr(idx), n_idx, dptr(ll))
+            call(name, self.ptr, int(bool(save_trajectories)), szptr(idx), n_idx, dptr(ll))
         else:
             init = None
             if initial is not None:
                 init = np.ascontiguousarray(np.broadcast_to(
                     f64(initial), (self.n_groups, self.n_particles, 5)))
-            call("dust2_unfilter_sir_run", self.ptr, dptr(init), int(bool(adjoint)),
-                 szptr(idx), n_idx, dptr(ll))
+            call("dust2_unfilter_sir_run", self.ptr, dptr(init), int(bool(save_trajectories)),
+                 int(bool(adjoint)), szptr(idx), n_idx, dptr(ll))
         self.last_index = index_group
         if self.n_particles == 1:
             ll = ll[:, 0]
@@ -626,6 +631,16 @@ class Unfilter:
     def last_state(self):
         out = np.zeros((self.n_groups, self.n_particles, 5))
         call(self._prefix() + "_last_state", self.ptr, dptr(out))
+        return out
+
+    def last_trajectories(self):
+        """State at every data time of the last run (no reordering):
+        (n_time, n_groups of the last run, n_particles, n_state)."""
+        if self.ptr is None:
+            raise DustError("Trajectories are not current")
+        n = self.n_groups if self.last_index is None else np.size(self.last_index)
+        out = np.zeros((self.time.size, n, self.n_particles, 5))
+        call(self._prefix() + "_last_trajectories", self.ptr, dptr(out))
         return out
 
     def last_stats(self):

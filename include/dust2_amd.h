@@ -226,9 +226,16 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
 
 /* Replaces dust2_unfilter_run<T> (dust2/r/unfilter.hpp:21-62) /
  * unfilter<T>::run (dust2/unfilter.hpp:42-72): ll_out receives
- * n_particles log-likelihoods per selected group. */
-int dust2_unfilter_sirode_run(dust2_unfilter* h, const size_t* index_group, size_t n_index,
-                              double* ll_out, char* err, size_t errlen);
+ * n_particles log-likelihoods per selected group.  save_trajectories keeps
+ * the state at every data time (trajectories_.add, dust2/unfilter.hpp:66). */
+int dust2_unfilter_sirode_run(dust2_unfilter* h, int save_trajectories,
+                              const size_t* index_group, size_t n_index, double* ll_out,
+                              char* err, size_t errlen);
+
+/* Replaces dust2_unfilter_last_trajectories (dust2/r/unfilter.hpp:64-96),
+ * no reordering: out [n_time][n_groups of the last run][P][S]. */
+int dust2_unfilter_sirode_last_trajectories(dust2_unfilter* h, double* out, char* err,
+                                            size_t errlen);
 int dust2_unfilter_sirode_update_pars(dust2_unfilter* h, const double* pars,
                                       const size_t* index_group, size_t n_index, char* err,
                                       size_t errlen);
@@ -245,9 +252,9 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
  * plain run's.  No counterpart in the reference, whose only gradient is the
  * discrete adjoint (dust2/unfilter.hpp:77-229); the interface mirrors that
  * one.  FP64 only. */
-int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, const size_t* index_group,
-                                       size_t n_index, double* ll_out, char* err,
-                                       size_t errlen);
+int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, int save_trajectories,
+                                       const size_t* index_group, size_t n_index,
+                                       double* ll_out, char* err, size_t errlen);
 /* out [n_index of the last run][P][3] = d ll / d (beta, gamma, I0). */
 int dust2_unfilter_sirode_last_gradient(dust2_unfilter* h, double* out, char* err,
                                         size_t errlen);
@@ -268,9 +275,15 @@ int dust2_unfilter_sir_alloc(const double* pars, size_t n_groups, size_t n_parti
  * (model initial state) or [G][P][S].  adjoint != 0 also computes the
  * gradient of the log-likelihood with respect to (beta, gamma, I0).
  * ll_out: [n_index][P], one log-likelihood per particle. */
-int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int adjoint,
-                           const size_t* index_group, size_t n_index, double* ll_out,
-                           char* err, size_t errlen);
+int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int save_trajectories,
+                           int adjoint, const size_t* index_group, size_t n_index,
+                           double* ll_out, char* err, size_t errlen);
+
+/* Replaces dust2_unfilter_last_trajectories (dust2/r/unfilter.hpp:64-96):
+ * the state at every data time, out [n_time][n_groups of the last run][P][S]
+ * (identical with and without the adjoint). */
+int dust2_unfilter_sir_last_trajectories(dust2_unfilter_sir* h, double* out, char* err,
+                                         size_t errlen);
 
 /* Replaces dust2_discrete_unfilter_last_gradient (dust2/r/unfilter.hpp:153-179):
  * out [n_index of the last run][3] = d ll / d (beta, gamma, I0).  Errors as the

@@ -124,3 +124,30 @@ def test_gradient_errors_and_initial_state(gpu_lib):
     # invalid parameters report the reference's binomial error
     with pytest.raises(gpu_lib.DustError, match="Invalid call to binomial"):
         gpu_lib.dust_likelihood_run(obj, [{"beta": -0.5}, {"beta": 0.2}])
+
+
+def test_unfilter_trajectories_with_and_without_adjoint(gpu_lib):
+    # test-unfilter-adjoint.R:100-117 and test-unfilter.R:46-75
+    pars = random_pars(3, 12)
+    time = np.arange(1.0, 31.0)
+    data = np.random.default_rng(13).poisson(10, (30, 3)).astype(float)
+    obj = make(gpu_lib, pars, time, 0.25, data)
+    with pytest.raises(gpu_lib.DustError, match="Trajectories are not current"):
+        gpu_lib.dust_likelihood_run(obj, plist(pars))
+        gpu_lib.dust_likelihood_last_trajectories(obj)
+    ll1 = gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True)
+    h1 = gpu_lib.dust_likelihood_last_trajectories(obj)
+    ll2 = gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True, adjoint=True)
+    h2 = gpu_lib.dust_likelihood_last_trajectories(obj)
+    np.testing.assert_array_equal(ll1, ll2)
+    np.testing.assert_array_equal(h1, h2)
+    assert h1.shape == (30, 3, 1, 5)
+    np.testing.assert_array_equal(h1[-1], obj.last_state())
+    # the likelihood is the sum of the Poisson terms at the saved states
+    from scipy import stats
+    expect = stats.poisson.logpmf(data, h1[:, :, 0, 4] + 1e-6).sum(axis=0)
+    np.testing.assert_allclose(ll1, expect, rtol=1e-12)
+    sub = gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True, index_group=[2])
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_last_trajectories(obj)[:, 0],
+                                  h1[:, 2])
+    assert sub.shape == (1,)

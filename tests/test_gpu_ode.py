@@ -165,3 +165,23 @@ def test_weekly_data_matches_reference_solver(gpu_lib, which):
     np.testing.assert_array_equal(stats[:, :3], np.array(g["stats" + key])[:, :3])
     np.testing.assert_allclose(obj.last_state()[:, 0, :], np.array(g["state" + key])[:, -1, :],
                                rtol=1e-10, atol=1e-9)
+
+
+def test_ode_unfilter_trajectories(gpu_lib):
+    g = golden()
+    pars = np.array(g["pars"])[:3]
+    data = np.array(g["data"])
+    obj = gpu_lib.dust_unfilter_create(gpu_lib.sirode(), 0.0,
+                                       {"time": g["time"], "incidence": data},
+                                       n_particles=1, n_groups=3)
+    names = ["N", "I0", "beta", "gamma"]
+    ll = gpu_lib.dust_likelihood_run(obj, [dict(zip(names, p)) for p in pars],
+                                     save_trajectories=True)
+    h = gpu_lib.dust_likelihood_last_trajectories(obj)
+    assert h.shape == (365, 3, 1, 5)
+    ref = np.array(g["state"])[:3]                     # (3, len(state_index), 5)
+    np.testing.assert_allclose(h[g["state_index"], :, 0, :].transpose(1, 0, 2), ref,
+                               rtol=1e-10, atol=1e-9)
+    ll2 = gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True, adjoint=True)
+    np.testing.assert_array_equal(ll, ll2)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_last_trajectories(obj), h)
