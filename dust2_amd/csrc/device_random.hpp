@@ -200,8 +200,12 @@ __device__ real binomial_btrs(rng256& rng, real n, real p) {
 // The FP32 samplers have no reference draw sequence to reproduce, so they
 // use the hardware reciprocal/log/exp (1-2 ulp) and a rejection bound that
 // is free of cancellation, keeping the whole draw in FP32 registers.
+// Natural log from the hardware base-2 log (v_log_f32) times ln 2 in one
+// rounding: every argument below is a normal number or zero.
 __device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float f_log(float x) { return __logf(x); }
+__device__ __forceinline__ float f_log(float x) {
+  return __builtin_amdgcn_logf(x) * 0.693147181f;
+}
 
 // log(1 + x) accurate for small |x| from the hardware log (Goldberg 1991,
 // "What every computer scientist should know about floating-point
@@ -220,26 +224,19 @@ __device__ __forceinline__ void random_pair_f32(rng256& r, float& u, float& v) {
   v = static_cast<float>(static_cast<uint32_t>(x >> 16) & 0xffffffu) * 0x1.0p-24f;
 }
 
-// Stirling-series remainder in FP32: exact values below 10 from a table,
-// the series 1/12 - 1/360 x^-2 + 1/1260 x^-4 over x = k + 1 above.
+// Stirling-series remainder in FP32, log(k!) - (k + 1/2) log(k + 1) + (k + 1)
+// - log(2 pi) / 2: exact values for k < 4, the series 1/12 - 1/360 x^-2 +
+// 1/1260 x^-4 over x = k + 1 above (truncation below 8e-9 from k = 4),
+// without a branch so that lanes on either side run together.
 __device__ __forceinline__ float stirling_tail_f32(float k) {
-  if (k <= 9) {
-    const int i = static_cast<int>(k);
-    float t = 0.08106146679532726f;
-    t = i == 1 ? 0.0413406959554093f : t;
-    t = i == 2 ? 0.02767792568499834f : t;
-    t = i == 3 ? 0.020790672103765093f : t;
-    t = i == 4 ? 0.016644691189821193f : t;
-    t = i == 5 ? 0.013876128823070748f : t;
-    t = i == 6 ? 0.01189670994589177f : t;
-    t = i == 7 ? 0.010411265261972096f : t;
-    t = i == 8 ? 0.009255462182712733f : t;
-    t = i == 9 ? 0.00833056343336287f : t;
-    return t;
-  }
   const float r = f_rcp(k + 1);
   const float r2 = r * r;
-  return (0.083333333f - (0.0027777778f - r2 * 0.00079365079f) * r2) * r;
+  float t = (0.083333333f - (0.0027777778f - r2 * 0.00079365079f) * r2) * r;
+  t = k == 3 ? 0.020790672103765093f : t;
+  t = k == 2 ? 0.02767792568499834f : t;
+  t = k == 1 ? 0.0413406959554093f : t;
+  t = k == 0 ? 0.08106146679532726f : t;
+  return t;
 }
 
 // Inversion, n*p < 10, p <= 1/2.
@@ -261,17 +258,13 @@ __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
     float u = random_real<float>(rng);
     float f = f0;
     float k = 0;
-    bool ok = true;
-    while (u >= f) {
+    // one exit test per step; leaving with k > k_max restarts the search
+    while (u >= f && k <= k_max) {
       u -= f;
       k += 1;
-      if (k > k_max) {
-        ok = false;
-        break;
-      }
       f *= g * f_rcp(k) - r;
     }
-    if (ok) {
+    if (k <= k_max) {
       return k;
     }
   }

@@ -73,6 +73,44 @@ def test_samplers_fp64_draw_for_draw(gpu_lib, kind, a, b):
     assert mismatch <= n_streams // 200, f"{mismatch} of {n_streams} streams diverged"
 
 
+def chisq_pvalue(draws, pmf, support):
+    from scipy import stats
+    obs = np.array([(draws == k).sum() for k in support], dtype=float)
+    exp = pmf(support) * draws.size
+    keep = exp >= 5
+    obs = np.append(obs[keep], draws.size - obs[keep].sum())
+    exp = np.append(exp[keep], draws.size - exp[keep].sum())
+    if exp[-1] < 5:
+        obs[-2] += obs[-1]
+        exp[-2] += exp[-1]
+        obs, exp = obs[:-1], exp[:-1]
+    return stats.chisquare(obs, exp * obs.sum() / exp.sum()).pvalue
+
+
+@pytest.mark.parametrize("n,p", [(20, 0.1), (9, 0.5), (5, 0.93), (25, 0.4), (20, 0.5),
+                                 (40, 0.3), (1000, 0.3), (60, 0.8), (1_000_000, 0.001),
+                                 (200, 0.05), (1e6, 2e-5), (5e5, 0.5)])
+def test_binomial_fp32_distribution(gpu_lib, n, p):
+    """The FP32 sampler has no draw sequence to follow: its draws are held to
+    the exact binomial pmf (chi-square) and mean.  Cases cover inversion
+    (n p < 10), BTRS near its threshold where the Stirling remainders of k
+    and n - k take their tabulated values, large n, and p > 1/2."""
+    from scipy import stats
+    rng = ol.prng(1000, seed=int(11 + n * p))
+    out, err, _ = gpu_lib.gpu_sample("binomial", rng, 200, float(n), float(p),
+                                     precision="fp32")
+    assert not err.any()
+    draws = out.reshape(-1)
+    assert np.all(draws == np.round(draws))
+    assert draws.min() >= 0 and draws.max() <= n
+    mean = n * p
+    sd = np.sqrt(n * p * (1 - p))
+    lo, hi = int(max(0, mean - 8 * sd)), int(min(n, mean + 8 * sd)) + 1
+    support = np.arange(lo, hi)
+    assert chisq_pvalue(draws, lambda k: stats.binom.pmf(k, n, p), support) > 1e-4
+    assert abs(draws.mean() - mean) < 5 * sd / np.sqrt(draws.size)
+
+
 def test_binomial_invalid_reports(gpu_lib):
     rng = ol.prng(2, seed=1)
     out, err, _ = gpu_lib.gpu_sample("binomial", rng, 1, [10.0, 10.0], [0.5, -0.2])
