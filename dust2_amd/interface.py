@@ -7,6 +7,7 @@ R/interface-likelihood.R); the work is done by the C-ABI library through
 shapes [state, particle, group] is returned here as numpy shape
 (group, particle, state) over the same memory.
 """
+import copy
 import ctypes as C
 
 import numpy as np
@@ -214,6 +215,10 @@ class Filter:
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
         self.dt = float(dt)
+        # a seed drawn at creation, so that the state reported before the
+        # first run is the one the run starts from
+        if seed is None:
+            seed = int(np.random.default_rng().integers(0, 2**31 - 1))
         self.seed = seed
         if precision == "fp64" and "fp64" not in generator.precisions:
             precision = generator.precisions[0]
@@ -334,6 +339,20 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
     return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
 
 
+def dust_likelihood_copy(obj, seed=None):
+    """R/interface-likelihood.R:156-169: an independent copy of a filter or
+    unfilter with the same settings.  The copy is uninitialised (its device
+    state is allocated on its first run) and a filter copy draws from the
+    streams of `seed` (a fresh seed when None), not from the parent's."""
+    dst = copy.copy(obj)
+    dst.ptr = None
+    dst.last_index = None
+    if isinstance(obj, Filter):
+        dst.last_snapshots = None
+        dst.seed = int(np.random.default_rng().integers(0, 2**31 - 1)) if seed is None else seed
+    return dst
+
+
 def dust_likelihood_last_state(obj, select_random_particle=False):
     """Final state (n_groups, n_particles, n_state); with
     select_random_particle, the chosen particle of each group of the last
@@ -393,15 +412,29 @@ def dust_likelihood_last_ancestry(obj):
 
 
 def dust_likelihood_rng_state(obj):
-    out = np.zeros(obj.n_groups * (obj.n_particles + 2) * 4, dtype=np.uint64)
+    """R/interface-likelihood.R:281-288: before the first run, the state the
+    run will start from (the seed expanded to n_groups * (n_particles + 2)
+    streams, dust2/r/discrete/filter.hpp:63-74)."""
+    n = obj.n_groups * (obj.n_particles + 2)
+    if obj.ptr is None:
+        raw, n_raw, seed = _seed_args(obj.seed)
+        out = np.zeros(n * 4, dtype=np.uint64)
+        call("dust2_amd_rng_streams", u64ptr(raw), n_raw, seed, n, u64ptr(out))
+        return out
+    out = np.zeros(n * 4, dtype=np.uint64)
     call("dust2_filter_rng_state", obj.ptr, u64ptr(out))
     return out
 
 
 def dust_likelihood_set_rng_state(obj, state):
+    """R/interface-likelihood.R:297-306: before the first run the state
+    replaces the seed."""
     st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
     if st.size != obj.n_groups * (obj.n_particles + 2) * 4:
         raise ValueError("Incorrect length for 'rng_state'")
+    if obj.ptr is None:
+        obj.seed = st.reshape(-1, 4).copy()
+        return
     call("dust2_filter_set_rng_state", obj.ptr, u64ptr(st))
 
 

@@ -1,0 +1,132 @@
+"""Parity at the benchmark workloads' full sizes, through properties that do
+not depend on size.
+
+* Batched SIR filter, 512 groups x 2048 particles x 365 daily observations
+  (dt = 0.25): the FP64 run is compared draw-for-draw with the oracle on
+  sampled groups (each group is an independent filter over its own block of
+  RNG streams, exported from the device before the run); the FP32 run is
+  compared with the FP64 run group by group (paired differences centred on
+  zero); every final state conserves the population exactly.
+* SIR ODE unfilter, 65,536 parameter sets x 365 observations: every
+  log-likelihood and final state against the oracle.
+* Age-structured SEIR16 filter, 128 groups x 8192 particles x 200 weekly
+  observations: population conserved exactly in every age group of every
+  particle, incidence non-negative, finite log-likelihoods.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+SIR_NAMES = ["N", "I0", "beta", "gamma", "exp_noise"]
+
+
+def sir_groups(G, seed=42):
+    rng = np.random.default_rng(seed)
+    beta = rng.uniform(0.1, 0.5, G)
+    gamma = rng.uniform(0.05, 0.2, G)
+    N = np.round(rng.uniform(1e3, 1e6, G))
+    return np.column_stack([N, np.full(G, 10.0), beta, gamma, np.full(G, 1e6)])
+
+
+def sir_data(pars, time, seed):
+    out, _ = ol.sir_simulate(pars, 1, 0.0, 0.25, time, ol.prng(len(pars), seed=seed),
+                             n_threads=8)
+    return np.ascontiguousarray(out[:, :, 0, 4])             # (n_time, G)
+
+
+@pytest.fixture(scope="module")
+def sir_c2(gpu_lib):
+    G, P = 512, 2048
+    time = np.arange(1.0, 366.0)
+    pars = sir_groups(G)
+    data = sir_data(pars, time, 7)
+    plist = [dict(zip(SIR_NAMES, p)) for p in pars]
+    out = {"pars": pars, "data": data, "time": time, "P": P}
+    for prec in ["fp64", "fp32"]:
+        obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data},
+                                         P, n_groups=G, dt=0.25, seed=11, precision=prec)
+        rng0 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4).copy()
+        ll = np.atleast_1d(gpu_lib.dust_likelihood_run(obj, plist))
+        out[prec] = {"ll": ll, "rng0": rng0, "state": gpu_lib.dust_likelihood_last_state(obj),
+                     "engine": gpu_lib.dust_likelihood_last_stats(obj)["engine"]}
+    return out
+
+
+@pytest.mark.parametrize("g", [0, 137, 511])
+def test_sir_full_size_fp64_groups_match_oracle(sir_c2, g):
+    P = sir_c2["P"]
+    r = sir_c2["fp64"]
+    block = r["rng0"][g * (P + 2):(g + 1) * (P + 2)]
+    ref = ol.sir_filter(sir_c2["pars"][g:g + 1], P, 0.0, sir_c2["time"], 0.25,
+                        sir_c2["data"][:, g:g + 1], rng=block, n_threads=8)
+    np.testing.assert_allclose(r["ll"][g], ref["ll"][0], rtol=1e-12)
+    assert np.array_equal(r["state"][g], ref["state"][0])
+
+
+def test_sir_full_size_population_conserved(sir_c2):
+    N = sir_c2["pars"][:, 0]
+    for prec in ["fp64", "fp32"]:
+        st = sir_c2[prec]["state"]                            # (G, P, 5)
+        assert np.all(st[:, :, :3] >= 0)
+        assert np.array_equal(st[:, :, 0] + st[:, :, 1] + st[:, :, 2],
+                              np.broadcast_to(N[:, None], st.shape[:2]))
+        assert np.all(st[:, :, 3] >= st[:, :, 4])
+        assert np.all(np.isfinite(sir_c2[prec]["ll"]))
+
+
+def test_sir_full_size_fp32_paired_with_fp64(sir_c2):
+    # the two precisions draw different random numbers; over 512 groups the
+    # paired differences of the log-likelihood estimates centre on zero
+    d = sir_c2["fp32"]["ll"] - sir_c2["fp64"]["ll"]
+    se = d.std(ddof=1) / np.sqrt(d.size)
+    assert abs(d.mean()) < 4 * se, (d.mean(), se)
+
+
+def test_sirode_full_size_matches_oracle(gpu_lib):
+    rng = np.random.default_rng(5)
+    n = 65536
+    pars = np.column_stack([np.full(n, 1000.0), np.full(n, 10.0), rng.uniform(0.1, 0.5, n),
+                            rng.uniform(0.05, 0.2, n)])
+    time = np.arange(1.0, 366.0)
+    data = rng.poisson(6, 365).astype(float)
+    obj = gpu_lib.dust_unfilter_create(gpu_lib.sirode(), 0.0, {"time": time, "incidence": data},
+                                       n_particles=1, n_groups=n)
+    ll = np.atleast_1d(obj.run([dict(zip(["N", "I0", "beta", "gamma"], p)) for p in pars]))
+    ref = ol.sirode_unfilter(pars, 1, 0.0, time, data, n_threads=8)
+    np.testing.assert_allclose(ll, ref["ll"], rtol=1e-10)
+    np.testing.assert_allclose(obj.last_state(), ref["state"], rtol=1e-10, atol=1e-9)
+
+
+def contact_matrix(seed=7):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(0, 2, (16, 16))
+    C = (C + C.T) / 2
+    return C / np.max(np.abs(np.linalg.eigvals(C)))
+
+
+def test_seir16_full_size_invariants(gpu_lib):
+    G, P = 128, 8192
+    rng = np.random.default_rng(43)
+    C = contact_matrix()
+    N = np.round(rng.uniform(1e4, 1e5, 16))
+    plist = [{"beta": rng.uniform(0.1, 0.5), "sigma": 0.25, "gamma": rng.uniform(0.05, 0.2),
+              "exp_noise": 1e6, "I0": 10.0, "N": N, "C": C} for _ in range(G)]
+    time = 7.0 * np.arange(1, 201)
+    # data: one stochastic run of each group (a one-particle filter with no
+    # observations saves its trajectory)
+    nan = np.full((time.size, 16), np.nan)
+    sim = gpu_lib.dust_filter_create(gpu_lib.seir16(), 0.0, {"time": time, "incidence": nan}, 1,
+                                     n_groups=G, dt=0.25, seed=1, precision="fp32")
+    gpu_lib.dust_likelihood_run(sim, plist, save_trajectories=True)
+    data = np.ascontiguousarray(gpu_lib.dust_likelihood_last_trajectories(sim)[:, :, 0, 64:80])
+    obj = gpu_lib.dust_filter_create(gpu_lib.seir16(), 0.0, {"time": time, "incidence": data},
+                                     P, n_groups=G, dt=0.25, seed=2, precision="fp32")
+    ll = np.atleast_1d(gpu_lib.dust_likelihood_run(obj, plist))
+    assert ll.shape == (G,) and np.all(np.isfinite(ll))
+    st = gpu_lib.dust_likelihood_last_state(obj)             # (G, P, 80)
+    pop = st[:, :, 0:16] + st[:, :, 16:32] + st[:, :, 32:48] + st[:, :, 48:64]
+    assert np.array_equal(pop, np.broadcast_to(N, pop.shape))
+    assert np.all(st[:, :, :64] >= 0) and np.all(st[:, :, 64:80] >= 0)

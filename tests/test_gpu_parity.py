@@ -409,3 +409,89 @@ def test_shared_data_equals_replicated_data(gpu_lib):
     _, ll_shared = run_gpu_filter(gpu_lib, pars, P, time, 0.25, d, 4)
     _, ll_rep = run_gpu_filter(gpu_lib, pars, P, time, 0.25, np.tile(d[:, None], (1, G)), 4)
     np.testing.assert_array_equal(ll_shared, ll_rep)
+
+
+# ---------------------------------------------------------------------------
+# Random number state before and after runs (tests/testthat/test-filter.R)
+def sir_pars(beta):
+    return {"beta": beta, "gamma": 0.2, "N": 1000, "I0": 10, "exp_noise": 1e6}
+
+
+SMALL = {"time": np.array([4.0, 8.0, 12.0, 16.0]), "incidence": np.arange(1.0, 5.0)}
+
+
+def test_running_filter_keeps_second_filter_stream(gpu_lib):
+    # test-filter.R:118-147
+    P = 100
+    obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, SMALL, P, dt=1, seed=42)
+    s0 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    assert s0.shape == (P + 2, 4)
+    np.testing.assert_array_equal(s0, ol.prng(P + 2, seed=42))
+    gpu_lib.dust_likelihood_run(obj, sir_pars(0.1))
+    s1 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    np.testing.assert_array_equal(s1[1], s0[1])
+    gpu_lib.dust_likelihood_run(obj, sir_pars(0.1))
+    s2 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    np.testing.assert_array_equal(s2[1], s0[1])
+    gpu_lib.dust_likelihood_last_state(obj, select_random_particle=True)
+    s3 = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    _, after = ol.random_real(s1[1], 1)
+    np.testing.assert_array_equal(s3[1], after)
+    np.testing.assert_array_equal(np.delete(s3, 1, axis=0), np.delete(s2, 1, axis=0))
+
+
+def test_nested_filter_groups_equal_single_filters(gpu_lib):
+    # test-filter.R:150-204: group g of a grouped filter is the filter whose
+    # seed is the first stream of group g's block
+    P = 100
+    data = {"time": SMALL["time"], "incidence": np.column_stack([np.arange(1.0, 5.0),
+                                                                 np.arange(2.0, 6.0)])}
+    pars = [sir_pars(0.1), sir_pars(0.2)]
+    obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, P, n_groups=2, seed=42)
+    s = gpu_lib.dust_likelihood_rng_state(obj).reshape(-1, 4)
+    np.testing.assert_array_equal(s, ol.prng(2 * (P + 2), seed=42))
+    res = np.array([gpu_lib.dust_likelihood_run(obj, pars if i == 0 else None)
+                    for i in range(5)])
+    for g in range(2):
+        one = {"time": SMALL["time"], "incidence": data["incidence"][:, g]}
+        seed = 42 if g == 0 else s[P + 2:P + 3]
+        obj1 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, one, P, seed=seed)
+        np.testing.assert_array_equal(gpu_lib.dust_likelihood_rng_state(obj1).reshape(-1, 4),
+                                      s[g * (P + 2):(g + 1) * (P + 2)])
+        res1 = np.array([gpu_lib.dust_likelihood_run(obj1, pars[g] if i == 0 else None)
+                         for i in range(5)]).reshape(-1)
+        np.testing.assert_array_equal(res1, res[:, g])
+
+
+@pytest.mark.parametrize("after_run", [False, True])
+def test_set_rng_state_before_and_after_running(gpu_lib, after_run):
+    # test-filter.R:249-307
+    data = {"time": SMALL["time"], "incidence": np.column_stack([np.arange(1.0, 5.0),
+                                                                 np.arange(2.0, 6.0)])}
+    pars = [sir_pars(0.1), sir_pars(0.2)]
+    obj1 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, 100, n_groups=2, seed=42)
+    obj2 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, 100, n_groups=2, seed=43)
+    if after_run:
+        assert not np.array_equal(gpu_lib.dust_likelihood_run(obj1, pars),
+                                  gpu_lib.dust_likelihood_run(obj2, pars))
+    s1 = gpu_lib.dust_likelihood_rng_state(obj1)
+    s2 = gpu_lib.dust_likelihood_rng_state(obj2)
+    assert not np.array_equal(s1, s2)
+    gpu_lib.dust_likelihood_set_rng_state(obj2, s1)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_rng_state(obj2), s1)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_run(obj1, pars),
+                                  gpu_lib.dust_likelihood_run(obj2, pars))
+
+
+def test_copy_is_independent(gpu_lib):
+    # test-filter.R:310-340
+    obj1 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, SMALL, 100, seed=42)
+    obj2 = gpu_lib.dust_likelihood_copy(obj1)
+    obj3 = gpu_lib.dust_likelihood_copy(obj1, seed=42)
+    s1 = gpu_lib.dust_likelihood_rng_state(obj1)
+    assert not np.array_equal(gpu_lib.dust_likelihood_rng_state(obj2), s1)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_rng_state(obj3), s1)
+    ll1 = gpu_lib.dust_likelihood_run(obj1, sir_pars(0.1))
+    ll2 = gpu_lib.dust_likelihood_run(obj2, sir_pars(0.1))
+    ll3 = gpu_lib.dust_likelihood_run(obj3, sir_pars(0.1))
+    assert ll1 == ll3 and ll1 != ll2
