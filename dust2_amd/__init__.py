@@ -10,6 +10,7 @@ from .interface import (  # noqa: F401
     Generator,
     dust_filter_create,
     dust_likelihood_copy,
+    dust_likelihood_ensure_initialised,
     dust_likelihood_last_ancestry,
     dust_ode_control,
     dust_unfilter_create,

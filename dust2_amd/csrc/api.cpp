@@ -1113,7 +1113,7 @@ int dust2_filter_set_save_snapshots(dust2_filter* h, const double* times, size_t
     std::vector<double> snaps(times, times + n);
     for (size_t i = 1; i < n; ++i) {
       if (!(snaps[i] > snaps[i - 1])) {
-        throw std::runtime_error("'save_snapshots' must be strictly increasing");
+        throw std::runtime_error("Values in 'save_snapshots' must be increasing");
       }
     }
     for (double t : snaps) {

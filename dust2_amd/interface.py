@@ -167,6 +167,24 @@ def _seed_args(seed):
     return raw, raw.size // 4, C.c_uint64(0)
 
 
+def check_index(index, max_value, name, unique=True):
+    """R/filter-support.R:73-91 with 0-based indices: every element in
+    [0, max_value) and, when asked, distinct."""
+    if index is None:
+        return None
+    idx = np.asarray(index).reshape(-1)
+    if idx.size and (not np.all(np.equal(np.mod(idx, 1), 0))):
+        raise ValueError(f"'{name}' must be integer-like")
+    idx = idx.astype(np.int64)
+    if np.any(idx < 0):
+        raise ValueError(f"All elements of '{name}' must be at least 0")
+    if np.any(idx >= max_value):
+        raise ValueError(f"All elements of '{name}' must be at most {max_value - 1}")
+    if unique and np.unique(idx).size != idx.size:
+        raise ValueError(f"All elements of '{name}' must be distinct")
+    return idx
+
+
 def _index_group(index_group, n_groups):
     if index_group is None:
         return None, 0, n_groups
@@ -207,7 +225,7 @@ class Filter:
     def __init__(self, generator, time_start, data, n_particles, n_groups, dt, seed,
                  precision):
         if generator.time_type != "discrete":
-            raise ValueError("Can't use 'dust_filter_create()' with continuous-time systems")
+            raise ValueError("Can't use 'dust_filter_create()' with continuous-time models")
         self.generator = generator
         self.time_start = float(time_start)
         self.time, self.data, self.n_groups_data = prepare_data(data, n_groups,
@@ -227,6 +245,7 @@ class Filter:
         self.launch = None
         self.engine = None
         self.last_index = None
+        self.last_index_state = None
         self.last_snapshots = None
 
     @property
@@ -305,15 +324,25 @@ def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=
 
 
 def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_group=None,
-                        adjoint=False, save_snapshots=None):
+                        adjoint=False, save_snapshots=None, index_state=None):
     """Run the filter (or unfilter) and return the log-likelihood per group
-    (a float for a single group), R/interface-likelihood.R:48-119."""
-    if isinstance(obj, Unfilter):
-        return obj.run(pars, index_group, initial=initial, adjoint=adjoint,
-                       save_trajectories=save_trajectories)
-    if not obj.initialised:
+    (a float for a single group), R/interface-likelihood.R:48-119.
+    index_state (0-based) selects the states kept in the trajectories;
+    snapshots and the final state keep every state."""
+    index_group = check_index(index_group, obj.n_groups, "index_group")
+    if obj.ptr is None:
         if pars is None:
             raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+        if index_group is not None:
+            raise ValueError("'index_group' must be NULL, as 'obj' is not initialised")
+    index_state = check_index(index_state, obj.generator.n_state, "index_state")
+    if isinstance(obj, Unfilter):
+        ll = obj.run(pars, index_group, initial=initial, adjoint=adjoint,
+                     save_trajectories=save_trajectories)
+        obj.last_index_state = index_state
+        return ll
+    obj.last_index_state = index_state
+    if not obj.initialised:
         obj._alloc(pars)
     elif pars is not None:
         idx, n_idx, _ = _index_group(index_group, obj.n_groups)
@@ -339,6 +368,17 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
     return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
 
 
+def dust_likelihood_ensure_initialised(obj, pars):
+    """R/interface-likelihood.R:121-139: allocate the object's device state
+    with `pars` unless it already exists; True when it was allocated now."""
+    if obj.ptr is not None:
+        return False
+    if pars is None:
+        raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+    obj._alloc(pars)
+    return True
+
+
 def dust_likelihood_copy(obj, seed=None):
     """R/interface-likelihood.R:156-169: an independent copy of a filter or
     unfilter with the same settings.  The copy is uninitialised (its device
@@ -347,6 +387,7 @@ def dust_likelihood_copy(obj, seed=None):
     dst = copy.copy(obj)
     dst.ptr = None
     dst.last_index = None
+    dst.last_index_state = None
     if isinstance(obj, Filter):
         dst.last_snapshots = None
         dst.seed = int(np.random.default_rng().integers(0, 2**31 - 1)) if seed is None else seed
@@ -387,15 +428,17 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
     every data time.  (n_time, n_groups of the last run, n_particles or 1,
     n_state)."""
     if isinstance(obj, Unfilter):
-        return obj.last_trajectories()
-    if obj.ptr is None:
-        raise DustError("Trajectories are not current")
-    n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
-    p_out = 1 if select_random_particle else obj.n_particles
-    out = np.zeros((obj.time.size, n_groups, p_out, obj.generator.n_state))
-    call("dust2_filter_last_trajectories", obj.ptr, int(bool(select_random_particle)),
-         dptr(out))
-    return out
+        out = obj.last_trajectories()
+    else:
+        if obj.ptr is None:
+            raise DustError("Trajectories are not current")
+        n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
+        p_out = 1 if select_random_particle else obj.n_particles
+        out = np.zeros((obj.time.size, n_groups, p_out, obj.generator.n_state))
+        call("dust2_filter_last_trajectories", obj.ptr, int(bool(select_random_particle)),
+             dptr(out))
+    index_state = getattr(obj, "last_index_state", None)
+    return out if index_state is None else np.ascontiguousarray(out[..., index_state])
 
 
 def dust_likelihood_last_trajectories_raw(obj):
@@ -592,6 +635,7 @@ class Unfilter:
         self.ptr = None
         self.rows = None
         self.last_index = None
+        self.last_index_state = None
 
     @property
     def initialised(self):

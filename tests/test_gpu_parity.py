@@ -255,8 +255,17 @@ def test_filter_missing_data_and_index_group(gpu_lib, engine):
                                       n_groups=G, dt=0.25, seed=99)
     obj2.set_engine(engine)
     plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
-    ll_sub = gpu_lib.dust_likelihood_run(obj2, plist, index_group=[2, 0])
+    # test-filter.R:39-51: an uninitialised object runs every group
+    with pytest.raises(ValueError, match="'index_group' must be NULL, as 'obj' is not"):
+        gpu_lib.dust_likelihood_run(obj2, plist[:1], index_group=[0])
+    assert gpu_lib.dust_likelihood_ensure_initialised(obj2, plist)
+    assert not gpu_lib.dust_likelihood_ensure_initialised(obj2, plist)
+    ll_sub = gpu_lib.dust_likelihood_run(obj2, [plist[2], plist[0]], index_group=[2, 0])
     np.testing.assert_allclose(ll_sub, ll[[2, 0]], rtol=1e-12)
+    with pytest.raises(ValueError, match="must be distinct"):
+        gpu_lib.dust_likelihood_run(obj2, None, index_group=[1, 1])
+    with pytest.raises(ValueError, match="must be at most 2"):
+        gpu_lib.dust_likelihood_run(obj2, None, index_group=[3])
     del st
 
 
@@ -495,3 +504,39 @@ def test_copy_is_independent(gpu_lib):
     ll2 = gpu_lib.dust_likelihood_run(obj2, sir_pars(0.1))
     ll3 = gpu_lib.dust_likelihood_run(obj3, sir_pars(0.1))
     assert ll1 == ll3 and ll1 != ll2
+
+
+def test_partial_trajectories_by_index_state(gpu_lib):
+    # test-filter.R:92-116 and :360-395
+    pars = [sir_pars(0.1), sir_pars(0.2)]
+    data = {"time": SMALL["time"], "incidence": np.arange(1.0, 9.0).reshape(2, 4).T}
+    obj1 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, 100, n_groups=2, seed=42)
+    obj2 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, 100, n_groups=2, seed=42)
+    ll1 = gpu_lib.dust_likelihood_run(obj1, pars, save_trajectories=True)
+    h1 = gpu_lib.dust_likelihood_last_trajectories(obj1)
+    ll2 = gpu_lib.dust_likelihood_run(obj2, pars, save_trajectories=True, index_state=[1, 3])
+    h2 = gpu_lib.dust_likelihood_last_trajectories(obj2)
+    np.testing.assert_array_equal(ll1, ll2)
+    assert h1.shape == (4, 2, 100, 5) and h2.shape == (4, 2, 100, 2)
+    np.testing.assert_array_equal(h2, h1[..., [1, 3]])
+    # per group, with a state subset
+    obj3 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, 100, n_groups=2, seed=42)
+    obj4 = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, data, 100, n_groups=2, seed=42)
+    assert gpu_lib.dust_likelihood_ensure_initialised(obj3, pars)
+    assert gpu_lib.dust_likelihood_ensure_initialised(obj4, pars)
+    la = gpu_lib.dust_likelihood_run(obj3, pars, index_state=[0, 2, 4], save_trajectories=True)
+    ha = gpu_lib.dust_likelihood_last_trajectories(obj3)
+    for g in range(2):
+        lb = gpu_lib.dust_likelihood_run(obj4, [pars[g]], index_state=[0, 2, 4], index_group=[g],
+                                         save_trajectories=True)
+        np.testing.assert_array_equal(lb, la[[g]])
+        np.testing.assert_array_equal(gpu_lib.dust_likelihood_last_trajectories(obj4),
+                                      ha[:, [g]])
+    # snapshots and the final state keep every state
+    gpu_lib.dust_likelihood_run(obj1, None, save_snapshots=[8.0], index_state=[4])
+    assert gpu_lib.dust_likelihood_last_snapshots(obj1).shape == (1, 2, 100, 5)
+    assert gpu_lib.dust_likelihood_last_state(obj1).shape == (2, 100, 5)
+    with pytest.raises(gpu_lib.DustError, match="must be increasing"):
+        gpu_lib.dust_likelihood_run(obj1, None, save_snapshots=[12.0, 4.0])
+    with pytest.raises(ValueError, match="continuous-time models"):
+        gpu_lib.dust_filter_create(gpu_lib.sirode(), 0.0, SMALL, 10)
