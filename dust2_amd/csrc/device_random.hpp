@@ -75,10 +75,18 @@ __device__ __forceinline__ double d_fabs(double x) { return fabs(x); }
 __device__ __forceinline__ float d_fabs(float x) { return fabsf(x); }
 // 1 - exp(x) for x <= 0: the probability of at least one event at rate -x.
 // FP64 keeps the reference's expression (inst/examples/sir.cpp:63) so draws
-// reproduce; FP32 uses expm1, since 1 - expf(x) cancels catastrophically
-// for the small rates typical of a step.
+// reproduce.  FP32 avoids the cancellation of 1 - expf(x) at the small
+// rates typical of a step: the Taylor series through x^6 for |x| < 1/16
+// (truncation below 1e-11 relative), the hardware exp2 above, where
+// 1 - exp(x) >= 0.06 keeps its relative accuracy to about 1e-6.
 __device__ __forceinline__ double prob_from_rate(double x) { return 1 - exp(x); }
-__device__ __forceinline__ float prob_from_rate(float x) { return -expm1f(x); }
+__device__ __forceinline__ float prob_from_rate(float x) {
+  const float series =
+      -x * (1.0f + x * (0.5f + x * (0.16666667f + x * (0.041666668f +
+                                                      x * (0.008333334f + x * 0.0013888889f)))));
+  const float direct = 1.0f - __builtin_amdgcn_exp2f(x * 1.44269504f);
+  return x > -0.0625f ? series : direct;
+}
 
 // (1 - p)^n: FP64 as the reference's pow(q, n); FP32 through log1p so a
 // p near the float resolution of 1 - p keeps its accuracy.
@@ -239,6 +247,15 @@ __device__ __forceinline__ float stirling_tail_f32(float k) {
   return t;
 }
 
+// 1/j for j = 0..112 (entry 0 unused), correctly rounded.
+struct inv_int_table {
+  float v[113];
+  constexpr inv_int_table() : v() {
+    for (int j = 1; j < 113; ++j) v[j] = 1.0f / static_cast<float>(j);
+  }
+};
+__constant__ constexpr inv_int_table kInvInt{};
+
 // Inversion, n*p < 10, p <= 1/2.
 //
 // The search restarts with a fresh uniform once k passes min(n, 110).  With
@@ -258,11 +275,17 @@ __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
     float u = random_real<float>(rng);
     float f = f0;
     float k = 0;
-    // one exit test per step; leaving with k > k_max restarts the search
-    while (u >= f && k <= k_max) {
+    // One exit test per step; leaving with k > k_max restarts the search.
+    // Every lane still searching is at step j together, so 1/j is one
+    // table entry for the whole wavefront (a scalar load, issued a step
+    // ahead; no per-lane reciprocal); j <= k_max + 1 <= 111.
+    float inv_next = kInvInt.v[1];
+    for (int j = 1; u >= f && k <= k_max; ++j) {
+      const float inv_j = inv_next;
+      inv_next = kInvInt.v[j + 1];
       u -= f;
-      k += 1;
-      f *= g * f_rcp(k) - r;
+      k = static_cast<float>(j);
+      f *= g * inv_j - r;
     }
     if (k <= k_max) {
       return k;
@@ -307,7 +330,8 @@ __device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
                         (n - k + 0.5f) * f_log1p(-d * rnm) + stirling_tail_f32(m) +
                         stirling_tail_f32(n - m) - stirling_tail_f32(k) -
                         stirling_tail_f32(n - k);
-    const float lv = f_log(v * alpha * f_rcp(a * f_rcp(us * us) + b));
+    const float us2 = us * us;
+    const float lv = f_log(v * alpha * us2 * f_rcp(a + b * us2));
     if (lv <= ratio) {
       return k;
     }

@@ -39,14 +39,24 @@ struct sir_model {
     y[4] = 0;
   }
 
+  // -beta * I / N * dt.  FP64 in the reference's association,
+  // ((-beta * I) / N) * dt; FP32 as -I times the step-invariant
+  // beta * dt / N, with no division per step.
+  __device__ __forceinline__ static double rate_SI(const shared<double>& sh, double I,
+                                                   double dt) {
+    return -sh.beta * I / sh.N * dt;
+  }
+  __device__ __forceinline__ static float rate_SI(const shared<float>& sh, float I, float dt) {
+    return -I * (sh.beta * (dt * __builtin_amdgcn_rcpf(sh.N)));
+  }
+
   template <typename real>
   __device__ __forceinline__ static void update(real dt, real* y, const shared<real>& sh,
                                                 rng256& rng, bool det, uint32_t& err,
                                                 real& err_a, real& err_b) {
     const real S = y[0];
     const real I = y[1];
-    // same association as the reference: ((-beta * I) / N) * dt
-    const real p_SI = prob_from_rate(-sh.beta * I / sh.N * dt);
+    const real p_SI = prob_from_rate(rate_SI(sh, I, dt));
     const real n_SI = binomial<real>(rng, S, p_SI, det, err);
     if (err) {
       err_a = S;
