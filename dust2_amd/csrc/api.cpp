@@ -368,12 +368,15 @@ struct dust2_filter {
       d_pslots_init;
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
-  // the pipeline splits the groups into `pipe_chains` independent launch
-  // chains on their own streams, so one chain's resample overlaps another
-  // chain's advance
+  // the pipeline splits the groups into `pipe_chains` (at most kMaxChains)
+  // independent launch chains on their own streams, so one chain's
+  // resample overlaps another chain's advance; chain c > 0 runs on
+  // chain_stream[c - 1] and counts its errors in counter 1 + c
+  static constexpr int kMaxChains = 4;
   int pipe_chains = 2;
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t chain_stream[kMaxChains - 1] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kMaxChains - 1] = {};
   bool traj_valid = false;
   bool ancestry_valid = false;
   std::vector<size_t> last_index;        // groups of the last run
@@ -396,8 +399,10 @@ struct dust2_filter {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_join) (void)hipEventDestroy(ev_join);
-    if (stream2) (void)hipStreamDestroy(stream2);
+    for (int c = 0; c < kMaxChains - 1; ++c) {
+      if (ev_join[c]) (void)hipEventDestroy(ev_join[c]);
+      if (chain_stream[c]) (void)hipStreamDestroy(chain_stream[c]);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -570,7 +575,7 @@ pipe_slot make_pipe_slot(dust2_filter* h, size_t g, bool save_traj) {
 // every launch can be timed.
 int pipeline_chains(const dust2_filter* h, int n_slots) {
   if (h->profile || n_slots < 2) return 1;
-  return std::min(h->pipe_chains, 2);
+  return std::max(1, std::min({h->pipe_chains, dust2_filter::kMaxChains, n_slots}));
 }
 
 void chain_range(int n_slots, int chains, int c, int& first, int& count) {
@@ -586,12 +591,14 @@ void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool sa
   const int T = static_cast<int>(h->n_time);
   const int chains = pipeline_chains(h, n_slots);
   if (chains > 1) {
-    // fork: the second chain starts after everything queued before it
+    // fork: the other chains start after everything queued before them
     hip_check(hipEventRecord(h->ev_fork, h->stream), "hipEventRecord");
-    hip_check(hipStreamWaitEvent(h->stream2, h->ev_fork, 0), "hipStreamWaitEvent");
+    for (int c = 1; c < chains; ++c) {
+      hip_check(hipStreamWaitEvent(h->chain_stream[c - 1], h->ev_fork, 0), "hipStreamWaitEvent");
+    }
   }
   for (int c = 0; c < chains; ++c) {
-    hipStream_t s = c == 0 ? h->stream : h->stream2;
+    hipStream_t s = c == 0 ? h->stream : h->chain_stream[c - 1];
     int first = 0, count = 0;
     chain_range(n_slots, chains, c, first, count);
     const pipe_slot* slots = h->d_pslots.get<pipe_slot>() + first;
@@ -623,10 +630,10 @@ void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool sa
                              PIPE_GATHER | (save_traj ? PIPE_TRAJ_PREV : 0u), s),
               "gather kernel");
   }
-  if (chains > 1) {
+  for (int c = 1; c < chains; ++c) {
     // join
-    hip_check(hipEventRecord(h->ev_join, h->stream2), "hipEventRecord");
-    hip_check(hipStreamWaitEvent(h->stream, h->ev_join, 0), "hipStreamWaitEvent");
+    hip_check(hipEventRecord(h->ev_join[c - 1], h->chain_stream[c - 1]), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(h->stream, h->ev_join[c - 1], 0), "hipStreamWaitEvent");
   }
 }
 
@@ -647,14 +654,15 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
     selected[g] = 1;
   }
   {
-    // each chain counts its own errors (counter 0 or 2) so that a failing
-    // chain stops only itself; the report keeps the earliest interval
+    // each chain counts its own errors (counter 0, or 1 + c for chain c)
+    // so that a failing chain stops only itself; the report keeps the
+    // earliest interval
     const int chains = pipeline_chains(h, static_cast<int>(slots.size()));
     for (int c = 1; c < chains; ++c) {
       int first = 0, count = 0;
       chain_range(static_cast<int>(slots.size()), chains, c, first, count);
       for (int i = first; i < first + count; ++i) {
-        slots[i].error_count = h->d_counters.get<uint32_t>() + 2;
+        slots[i].error_count = h->d_counters.get<uint32_t>() + 1 + c;
       }
     }
   }
@@ -796,9 +804,12 @@ static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
     }
     h->cfg = sir_filter_default_config(static_cast<int>(n_particles), h->fp64);
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
-    hip_check(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming), "hipEventCreate");
+    for (int c = 0; c < dust2_filter::kMaxChains - 1; ++c) {
+      hip_check(hipStreamCreateWithFlags(&h->chain_stream[c], hipStreamNonBlocking),
+                "hipStreamCreate");
+      hip_check(hipEventCreateWithFlags(&h->ev_join[c], hipEventDisableTiming), "hipEventCreate");
+    }
     hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
     const size_t G = n_groups, P = n_particles;
@@ -822,7 +833,8 @@ static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
     h->d_slots.alloc(G * sizeof(filter_slot));
     h->d_errors.alloc(G * P * sizeof(error_record));
     hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
-    h->d_counters.alloc(4 * sizeof(uint32_t));
+    // error counts (chain 0 and 2..4), fallback count (1)
+    h->d_counters.alloc(8 * sizeof(uint32_t));
     hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), h->stream), "memset");
     hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
     hip_check(hipStreamSynchronize(h->stream), "sync");
@@ -933,7 +945,7 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
               "sir filter kernel");
     hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
     }
-    uint32_t counters[4];
+    uint32_t counters[8];
     to_host(counters, h->d_counters, sizeof(counters), s);
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
@@ -950,7 +962,7 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     // a new run forgets the chosen particles of its groups (filter.hpp:162-175)
     if (h->select_valid.size() != G) h->select_valid.assign(G, 0);
     for (size_t g : idx) h->select_valid[g] = 0;
-    if (counters[0] + counters[2] > 0) {
+    if (counters[0] + counters[2] + counters[3] + counters[4] > 0) {
       std::vector<error_record> rec(G * P);
       to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
       hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
@@ -1270,6 +1282,20 @@ int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errl
   }
 }
 
+int dust2_filter_set_pipeline_chains(dust2_filter* h, int chains, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (chains < 1 || chains > dust2_filter::kMaxChains) {
+      throw std::runtime_error("'chains' must be between 1 and " +
+                               std::to_string(dust2_filter::kMaxChains));
+    }
+    h->pipe_chains = chains;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
                                 char* err, size_t errlen) {
   try {
@@ -1322,8 +1348,11 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
         }
       }
     }
-    cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, 1});
-    if (idx.size() > 1) cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, 2});
+    for (int chains = 1; chains <= dust2_filter::kMaxChains; ++chains) {
+      if (chains == 1 || idx.size() >= static_cast<size_t>(chains)) {
+        cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, chains});
+      }
+    }
     // tuning runs must leave the object as they found it: the particle and
     // filter streams and the state are restored after every run
     hipStream_t s = h->stream;

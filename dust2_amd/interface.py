@@ -244,6 +244,7 @@ class Filter:
         self.ptr = None
         self.launch = None
         self.engine = None
+        self.chains = None
         self.last_index = None
         self.last_index_state = None
         self.last_snapshots = None
@@ -266,6 +267,8 @@ class Filter:
             call("dust2_filter_set_launch", self.ptr, *self.launch)
         if self.engine is not None:
             call("dust2_filter_set_engine", self.ptr, *self.engine)
+        if self.chains is not None:
+            call("dust2_filter_set_pipeline_chains", self.ptr, self.chains)
 
     def set_launch(self, block, particles_per_thread):
         self.launch = (int(block), int(particles_per_thread))
@@ -274,12 +277,17 @@ class Filter:
 
     ENGINES = {"auto": 0, "persistent": 1, "pipeline": 2}
 
-    def set_engine(self, engine="auto", graph=True, profile=False):
+    def set_engine(self, engine="auto", graph=True, profile=False, chains=None):
         """Choose the filter engine ('auto', 'persistent', 'pipeline'),
-        HIP-graph replay of the pipeline and per-kernel profiling."""
+        HIP-graph replay of the pipeline, per-kernel profiling and the
+        pipeline's number of launch chains (1-4)."""
         self.engine = (self.ENGINES[engine], int(bool(graph)), int(bool(profile)))
+        if chains is not None:
+            self.chains = int(chains)
         if self.ptr is not None:
             call("dust2_filter_set_engine", self.ptr, *self.engine)
+            if self.chains is not None:
+                call("dust2_filter_set_pipeline_chains", self.ptr, self.chains)
 
     def autotune(self, pars=None, index_group=None):
         """Time every engine / launch shape on this workload and keep the

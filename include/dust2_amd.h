@@ -160,6 +160,13 @@ int dust2_filter_set_launch(dust2_filter* h, int block, int particles_per_thread
  * engine's threads per workgroup and particles per thread. */
 int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
 
+/* Number of independent launch chains (1-4, each on its own stream) the
+ * pipeline engine splits the groups into, so that one chain's resampling
+ * overlaps another chain's advance (default 2; dust2_filter_autotune
+ * chooses it by measurement).  Results do not depend on it.  No
+ * counterpart in the reference. */
+int dust2_filter_set_pipeline_chains(dust2_filter* h, int chains, char* err, size_t errlen);
+
 /* Engine selection (DUST2_ENGINE_*), HIP-graph replay of the pipeline
  * (default on) and per-kernel event profiling (default off, disables the
  * graph).  No counterpart in the reference (its one engine is the CPU). */

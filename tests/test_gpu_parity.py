@@ -540,3 +540,38 @@ def test_partial_trajectories_by_index_state(gpu_lib):
         gpu_lib.dust_likelihood_run(obj1, None, save_snapshots=[12.0, 4.0])
     with pytest.raises(ValueError, match="continuous-time models"):
         gpu_lib.dust_filter_create(gpu_lib.sirode(), 0.0, SMALL, 10)
+
+
+def test_pipeline_launch_chains_give_identical_results(gpu_lib):
+    # the groups split into 1-4 launch chains on their own streams: same
+    # results, and errors in groups of different chains are reported from
+    # the earliest interval (dust2/discrete/system.hpp:87)
+    G, P = 7, 200
+    pars = np.array([[2000, 10, 0.2 + 0.04 * g, 0.1, 1e6] for g in range(G)])
+    time = np.arange(1.0, 31.0)
+    data = sir_data(30)
+    out = []
+    for chains in [1, 2, 3, 4]:
+        obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data},
+                                         P, n_groups=G, dt=0.25, seed=21)
+        obj.set_engine("pipeline", chains=chains)
+        plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
+        ll = gpu_lib.dust_likelihood_run(obj, plist, save_trajectories=True)
+        out.append((ll, gpu_lib.dust_likelihood_last_state(obj),
+                    gpu_lib.dust_likelihood_last_trajectories(obj),
+                    gpu_lib.dust_likelihood_rng_state(obj)))
+        # groups 1 and 6 (different chains for 2-4 chains) fail; only the
+        # particles of the earliest failing interval are reported
+        bad = [dict(p) for p in plist]
+        bad[1]["gamma"] = -0.5
+        bad[6]["gamma"] = -0.5
+        with pytest.raises(gpu_lib.DustError) as e:
+            gpu_lib.dust_likelihood_run(obj, bad)
+        assert str(e.value).startswith(f"{2 * P} particles reported errors."), chains
+    for ll, st, tr, rng in out[1:]:
+        np.testing.assert_array_equal(ll, out[0][0])
+        np.testing.assert_array_equal(st, out[0][1])
+        np.testing.assert_array_equal(tr, out[0][2])
+        np.testing.assert_array_equal(rng, out[0][3])
+    with pytest.raises(gpu_lib.DustError, match="between 1 and 4"):
+        obj.set_engine("pipeline", chains=5)
