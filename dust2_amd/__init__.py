@@ -36,6 +36,7 @@ from .interface import (  # noqa: F401
     dust_system_state,
     dust_system_time,
     dust_system_update_pars,
+    gpu_interpolate,
     gpu_resample,
     gpu_trajectory_lineage,
     gpu_rng_streams,

@@ -98,6 +98,8 @@ _PROTOS = {
     "dust2_amd_resample": (C.c_int, [D, C.c_size_t, C.c_double, U32, D, ERR, ERRLEN]),
     "dust2_amd_trajectory_lineage": (C.c_int, [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
                                                C.c_size_t, SZ, D, ERR, ERRLEN]),
+    "dust2_amd_interpolate": (C.c_int, [C.c_int, D, C.c_size_t, D, C.c_size_t, C.c_size_t, D,
+                                        C.c_size_t, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,
                                         ERRLEN]),
     "dust2_amd_sample": (C.c_int, [C.c_int, C.c_int, U64, C.c_size_t, C.c_size_t, D, D, C.c_int,

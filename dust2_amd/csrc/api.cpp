@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "interpolate.hpp"
 
 using namespace dust2_amd;
 
@@ -1803,6 +1804,60 @@ int dust2_amd_trajectory_lineage(const double* raw, const uint32_t* ancestry, si
                              dout.get<double>(), s),
               "lineage kernel");
     to_host(out, dout, T * G * p_out * S * sizeof(double), s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_amd_interpolate(int kind, const double* t, size_t n_t, const double* y,
+                          size_t n_y, size_t n_series, const double* z, size_t n_z, double* out,
+                          char* err, size_t errlen) {
+  try {
+    if (kind != INTERP_CONSTANT && kind != INTERP_LINEAR && kind != INTERP_SPLINE) {
+      throw std::runtime_error("unknown interpolation kind");
+    }
+    if (n_series == 0 || n_y % n_series != 0) {
+      throw std::runtime_error("'y' must hold n_series values per time");
+    }
+    interp_validate(t, n_t, n_y / n_series, "t", "y");
+    if (n_t == 0 || (kind != INTERP_CONSTANT && n_t < 2)) {
+      throw std::runtime_error("Interpolation needs at least " +
+                               std::string(kind == INTERP_CONSTANT ? "one time" : "two times"));
+    }
+    const std::vector<double> k = kind == INTERP_SPLINE
+                                      ? interp_spline_slopes(t, y, n_t, n_series)
+                                      : std::vector<double>(n_y, 0.0);
+    hipStream_t s = nullptr;
+    dbuf dt(n_t * 8), dy(n_y * 8), dk(n_y * 8), dz(std::max<size_t>(n_z, 1) * 8),
+        dout(std::max<size_t>(n_z * n_series, 1) * 8), derr(std::max<size_t>(n_z, 1) * 4);
+    to_device(dt, t, n_t * 8, s);
+    to_device(dy, y, n_y * 8, s);
+    to_device(dk, k.data(), n_y * 8, s);
+    if (n_z) to_device(dz, z, n_z * 8, s);
+    hip_check(launch_interpolate(kind, dt.get<double>(), dy.get<double>(), dk.get<double>(),
+                                 static_cast<int>(n_t), static_cast<int>(n_series),
+                                 dz.get<double>(), static_cast<int>(n_z), dout.get<double>(),
+                                 derr.get<int>(), s),
+              "interpolate kernel");
+    std::vector<int> e(n_z);
+    if (n_z) {
+      to_host(out, dout, n_z * n_series * 8, s);
+      to_host(e.data(), derr, n_z * 4, s);
+    }
+    for (size_t q = 0; q < n_z; ++q) {
+      if (e[q] == INTERP_OK) continue;
+      // dust2/interpolate/interpolate.hpp:20-40, stream formatting
+      std::ostringstream msg;
+      if (e[q] == INTERP_BEFORE) {
+        msg << "Tried to interpolate at time = " << z[q] << ", which is " << t[0] - z[q]
+            << " before the first time (" << t[0] << ")";
+      } else {
+        msg << "Tried to interpolate at time = " << z[q] << ", which is "
+            << z[q] - t[n_t - 1] << " after the last time (" << t[n_t - 1] << ")";
+      }
+      throw std::runtime_error(msg.str());
+    }
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

@@ -225,6 +225,12 @@ hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint
 hipError_t launch_select_particle(uint64_t* rng, const uint32_t* groups, int n, int P,
                                   uint32_t* select, hipStream_t stream);
 
+// out[q * n_series + j]: series j of the interpolator (kind, t[n], y, k) at
+// z[q]; err[q]: 0 or the interp_error of z[q].
+hipError_t launch_interpolate(int kind, const double* t, const double* y, const double* k, int n,
+                              int n_series, const double* z, int n_z, double* out, int* err,
+                              hipStream_t stream);
+
 // ---- system (state resident in HBM between calls) -------------------------
 // state: real [G][S][P]; rng: [G][4][P]; pars: [G][8] doubles.
 hipError_t launch_sir_run_to_time(void* state, uint64_t* rng, const double* pars, int G,

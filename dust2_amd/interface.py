@@ -774,6 +774,28 @@ def gpu_trajectory_lineage(raw, ancestry=None, select=None):
     return out
 
 
+INTERP_KINDS = {"constant": 0, "linear": 1, "spline": 2}
+
+
+def gpu_interpolate(kind, t, y, z):
+    """Evaluate a piecewise constant / linear / natural cubic spline
+    interpolator on the device (dust2/interpolate/interpolate.hpp): t
+    (n_t,) strictly increasing, y (n_t,) or (n_t, n_series), z (n_z,).
+    Returns (n_z,) or (n_z, n_series)."""
+    t = f64(t).reshape(-1)
+    y = f64(y)
+    series = y.ndim > 1
+    y2 = y.reshape(y.shape[0], -1) if y.ndim else y.reshape(1, 1)
+    n_series = y2.shape[1]
+    yv = np.ascontiguousarray(y2).reshape(-1)
+    z = f64(z).reshape(-1)
+    out = np.zeros(max(z.size * n_series, 1))
+    call("dust2_amd_interpolate", INTERP_KINDS[kind], dptr(t), t.size, dptr(yv), yv.size,
+         n_series, dptr(z), z.size, dptr(out))
+    out = out[:z.size * n_series].reshape(z.size, n_series)
+    return out if series else out[:, 0]
+
+
 def gpu_rng_streams(n_streams, seed=None, seed_state=None):
     out = np.zeros(n_streams * 4, dtype=np.uint64)
     raw = None if seed_state is None else np.ascontiguousarray(seed_state, dtype=np.uint64)

@@ -358,6 +358,17 @@ int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* 
 int dust2_amd_trajectory_lineage(const double* raw, const uint32_t* ancestry, size_t n_time,
                                  size_t n_groups, size_t n_particles, size_t n_state,
                                  const size_t* select, double* out, char* err, size_t errlen);
+/* Evaluates an interpolator of time-varying inputs on the device: kind
+ * 0 piecewise constant, 1 piecewise linear, 2 natural cubic spline; times
+ * t[n_t] strictly increasing, values y[n_t * n_series] (y[i * n_series + j]
+ * for time i of series j); out[n_z * n_series] at the times z.  Replaces
+ * dust2::interpolate::Interpolate{Constant,Linear,Spline}{,Array}::eval
+ * (dust2/interpolate/interpolate.hpp:69-268, spline slopes
+ * dust2/interpolate/spline.hpp:17-90), with the reference's errors for
+ * times outside the knots.  For the tests and for model authors. */
+int dust2_amd_interpolate(int kind, const double* t, size_t n_t, const double* y,
+                          size_t n_y, size_t n_series, const double* z, size_t n_z, double* out,
+                          char* err, size_t errlen);
 /* n streams of raw xoshiro256+ state, the filter/system seeding path. */
 int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                           size_t n_streams, uint64_t* out, char* err, size_t errlen);

@@ -1416,4 +1416,85 @@ void unfilter_run(Sys& sys, double time_start, const std::vector<double>& time,
   }
 }
 
+// ---------------------------------------------------------------------------
+// dust2/interpolate/interpolate.hpp:14-269 and spline.hpp:10-90: piecewise
+// constant, piecewise linear and natural cubic spline interpolation.  The
+// interval is found by a left-to-right scan (the reference bisects); the
+// arithmetic of each value follows the reference so that results agree
+// bit for bit.  y[i * n_series + j]: time i, series j.
+// ---------------------------------------------------------------------------
+namespace interp {
+
+enum { constant = 0, linear = 1, spline = 2 };
+
+// Slopes of the natural cubic spline through every series
+// (calculate_a, calculate_b, solve_tridiagonal).
+inline std::vector<double> spline_slopes(const std::vector<double>& t,
+                                         const std::vector<double>& y, size_t n_series) {
+  const size_t n = t.size();
+  std::vector<double> k(y.size());
+  for (size_t j = 0; j < n_series; ++j) {
+    std::vector<double> a0(n), a1(n), a2(n), b(n);
+    a1[0] = 2 / (t[1] - t[0]);
+    a2[0] = 1 / (t[1] - t[0]);
+    b[0] = 3 * (y[n_series + j] - y[j]) / ((t[1] - t[0]) * (t[1] - t[0]));
+    for (size_t i = 1; i < n - 1; ++i) {
+      const double h0 = t[i] - t[i - 1], h1 = t[i + 1] - t[i];
+      a0[i] = 1 / h0;
+      a1[i] = 2 * (1 / h0 + 1 / h1);
+      a2[i] = 1 / h1;
+      const double d0 = y[i * n_series + j] - y[(i - 1) * n_series + j];
+      const double d1 = y[(i + 1) * n_series + j] - y[i * n_series + j];
+      b[i] = 3 * (d0 / (h0 * h0) + d1 / (h1 * h1));
+    }
+    const double hl = t[n - 1] - t[n - 2];
+    a0[n - 1] = 1 / hl;
+    a1[n - 1] = 2 / hl;
+    b[n - 1] = 3 * (y[(n - 1) * n_series + j] - y[(n - 2) * n_series + j]) / (hl * hl);
+    for (size_t i = 1; i < n; ++i) {
+      const double fac = a0[i] / a1[i - 1];
+      a1[i] -= fac * a2[i - 1];
+      b[i] -= fac * b[i - 1];
+    }
+    b[n - 1] /= a1[n - 1];
+    for (size_t ii = 1; ii < n; ++ii) {
+      const size_t i = n - 1 - ii;
+      b[i] = (b[i] - a2[i] * b[i + 1]) / a1[i];
+    }
+    for (size_t i = 0; i < n; ++i) k[i * n_series + j] = b[i];
+  }
+  return k;
+}
+
+// 0: ok, 1: before the first time, 2: after the last time.
+inline int eval(int kind, const std::vector<double>& t, const std::vector<double>& y,
+                const std::vector<double>& k, size_t n_series, double z, double* out) {
+  const size_t n = t.size();
+  if (!(z >= t[0])) return 1;
+  if (z > t[n - 1] && kind != constant) return 2;
+  size_t i = 0;
+  while (i + 1 < n && t[i + 1] <= z) ++i;
+  for (size_t j = 0; j < n_series; ++j) {
+    const double y0 = y[i * n_series + j];
+    if (kind == constant || i == n - 1) {
+      out[j] = y0;
+      continue;
+    }
+    const double t0 = t[i], t1 = t[i + 1], y1 = y[(i + 1) * n_series + j];
+    if (kind == linear) {
+      out[j] = n_series == 1 ? y0 + (y1 - y0) * (z - t0) / (t1 - t0)
+                             : y0 + (y1 - y0) * ((z - t0) / (t1 - t0));
+      continue;
+    }
+    const double k0 = k[i * n_series + j], k1 = k[(i + 1) * n_series + j];
+    const double r = (z - t0) / (t1 - t0);
+    const double a = k0 * (t1 - t0) - (y1 - y0);
+    const double b = -k1 * (t1 - t0) + (y1 - y0);
+    out[j] = (1 - r) * y0 + r * y1 + r * (1 - r) * (a * (1 - r) + b * r);
+  }
+  return 0;
+}
+
+}  // namespace interp
+
 }  // namespace oracle

@@ -472,4 +472,24 @@ int oracle_sirode_unfilter_gradient(const double* pars, size_t G, double time_st
   }
 }
 
+// Interpolation (oracle::interp): values out[n_z * n_series]; returns 0, or
+// 1 / 2 for the first time z that lies before / after the knots (its index
+// in *bad).
+int oracle_interpolate(int kind, const double* t, size_t n_t, const double* y,
+                       size_t n_series, const double* z, size_t n_z, double* out,
+                       size_t* bad) {
+  const std::vector<double> tv(t, t + n_t), yv(y, y + n_t * n_series);
+  const std::vector<double> k = kind == oracle::interp::spline
+                                    ? oracle::interp::spline_slopes(tv, yv, n_series)
+                                    : std::vector<double>(yv.size(), 0.0);
+  for (size_t q = 0; q < n_z; ++q) {
+    const int rc = oracle::interp::eval(kind, tv, yv, k, n_series, z[q], out + q * n_series);
+    if (rc != 0) {
+      *bad = q;
+      return rc;
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -1,13 +1,15 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY.
 //
 // Thin C wrapper over the parts of the reference engine that compile on
-// their own (no monty, no cpp11, no R): dust2/filter_details.hpp and
-// dust2/continuous/solver.hpp (+ control/history/events).  Built by
+// their own (no monty, no cpp11, no R): dust2/filter_details.hpp,
+// dust2/continuous/solver.hpp (+ control/history/events) and
+// dust2/interpolate/interpolate.hpp.  Built by
 // oracle/Makefile straight from the reference headers into oracle/_ref/;
 // used to pin the CPU restatement in oracle.hpp and to write the golden
 // fixtures under tests/golden/.  Nothing here is shipped.
 #include <dust2/continuous/solver.hpp>
 #include <dust2/filter_details.hpp>
+#include <dust2/interpolate/interpolate.hpp>
 
 #include <cstdint>
 #include <cstdio>
@@ -129,6 +131,36 @@ int ref_logistic(double r1, double r2, double K, double y0, const double* time,
     out[2 * i + 1] = y[1];
   }
   return 0;
+}
+
+// The reference's interpolators: scalar classes for one series, array
+// classes (dimensions n_series) for several; values out[n_z * n_series].
+// Returns 0, or 1 with the error text in err.
+int ref_interpolate(int kind, const double* t, size_t n_t, const double* y, size_t n_series,
+                    const double* z, size_t n_z, double* out, char* err, size_t errlen) {
+  try {
+    const std::vector<double> tv(t, t + n_t), yv(y, y + n_t * n_series);
+    namespace di = dust2::interpolate;
+    if (n_series == 1) {
+      for (size_t q = 0; q < n_z; ++q) {
+        if (kind == 0) out[q] = di::InterpolateConstant<double>(tv, yv, "t", "y").eval(z[q]);
+        if (kind == 1) out[q] = di::InterpolateLinear<double>(tv, yv, "t", "y").eval(z[q]);
+        if (kind == 2) out[q] = di::InterpolateSpline<double>(tv, yv, "t", "y").eval(z[q]);
+      }
+    } else {
+      dust2::array::dimensions<1> dim{n_series};
+      for (size_t q = 0; q < n_z; ++q) {
+        double* o = out + q * n_series;
+        if (kind == 0) di::InterpolateConstantArray<double, 1>(tv, yv, dim, "t", "y").eval(z[q], o);
+        if (kind == 1) di::InterpolateLinearArray<double, 1>(tv, yv, dim, "t", "y").eval(z[q], o);
+        if (kind == 2) di::InterpolateSplineArray<double, 1>(tv, yv, dim, "t", "y").eval(z[q], o);
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    std::snprintf(err, errlen, "%s", e.what());
+    return 1;
+  }
 }
 
 }  // extern "C"

@@ -57,6 +57,8 @@ def oracle():
         o.oracle_trajectory_lineage.argtypes = [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
                                                 C.c_size_t, SZ, D]
         o.oracle_trajectory_lineage.restype = None
+        o.oracle_interpolate.argtypes = [C.c_int, D, C.c_size_t, D, C.c_size_t, D, C.c_size_t,
+                                         D, SZ]
         o.oracle_seed_words.argtypes = [C.c_uint64, U64]
         o.oracle_prng.argtypes = [U64, C.c_size_t, C.c_size_t, U64]
         o.oracle_jump.argtypes = [U64, C.c_int]
@@ -96,6 +98,8 @@ def ref():
                                              D, D, C.c_char_p, C.c_size_t]
         _ref.ref_logistic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D,
                                       C.c_size_t, D, D]
+        _ref.ref_interpolate.argtypes = [C.c_int, D, C.c_size_t, D, C.c_size_t, D, C.c_size_t,
+                                         D, C.c_char_p, C.c_size_t]
     return _ref
 
 
@@ -332,3 +336,41 @@ def ref_logistic(r1, r2, K, y0, time, control=None):
     out = np.zeros((time.size, 2))
     ref().ref_logistic(r1, r2, K, y0, _dp(time), time.size, _dp(ctl), _dp(out))
     return out
+
+
+INTERP_KINDS = {"constant": 0, "linear": 1, "spline": 2}
+
+
+def _interp_args(t, y, z):
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    y = np.asarray(y, dtype=np.float64)
+    n_series = 1 if y.ndim == 1 else y.shape[1]
+    yv = np.ascontiguousarray(y.reshape(-1))
+    z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+    return t, yv, n_series, z, y.ndim > 1
+
+
+def interpolate(kind, t, y, z):
+    """Oracle interpolation; raises OracleError naming the first time
+    outside the knots ("before" / "after")."""
+    t, yv, n_series, z, series = _interp_args(t, y, z)
+    out = np.zeros(max(z.size * n_series, 1))
+    bad = np.zeros(1, dtype=np.uintp)
+    rc = oracle().oracle_interpolate(INTERP_KINDS[kind], _dp(t), t.size, _dp(yv), n_series,
+                                     _dp(z), z.size, _dp(out), bad.ctypes.data_as(SZ))
+    if rc:
+        raise OracleError(("before" if rc == 1 else "after") + f" at {z[bad[0]]}")
+    out = out[:z.size * n_series].reshape(z.size, n_series)
+    return out if series else out[:, 0]
+
+
+def ref_interpolate(kind, t, y, z):
+    """The reference's own interpolators (oracle/_ref)."""
+    t, yv, n_series, z, series = _interp_args(t, y, z)
+    out = np.zeros(max(z.size * n_series, 1))
+    buf = C.create_string_buffer(512)
+    rc = ref().ref_interpolate(INTERP_KINDS[kind], _dp(t), t.size, _dp(yv), n_series, _dp(z),
+                               z.size, _dp(out), buf, 512)
+    _check(rc, buf)
+    out = out[:z.size * n_series].reshape(z.size, n_series)
+    return out if series else out[:, 0]
