@@ -43,6 +43,7 @@ from .interface import (  # noqa: F401
     gpu_sample,
     seir16,
     sir,
+    sir_time_varying,
     sirode,
 )
 

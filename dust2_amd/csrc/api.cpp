@@ -286,6 +286,16 @@ std::string error_text(const error_record& r) {
     case ERR_ODE_STEP_SMALL: return "step too small";
     case ERR_ODE_VANISHED: return "step size vanished";
     case ERR_ODE_INIT: return "Initial step size was not finite";
+    case ERR_INTERP_BEFORE:
+    case ERR_INTERP_AFTER: {
+      // dust2/interpolate/interpolate.hpp:20-40; a = time, b = first or last knot
+      std::ostringstream msg;
+      msg << "Tried to interpolate at time = " << r.a << ", which is "
+          << (r.code == ERR_INTERP_BEFORE ? r.b - r.a : r.a - r.b)
+          << (r.code == ERR_INTERP_BEFORE ? " before the first time (" : " after the last time (")
+          << r.b << ")";
+      return msg.str();
+    }
     default: return "unknown error";
   }
 }
@@ -385,6 +395,11 @@ struct dust2_filter {
   std::vector<int32_t> snap_slot;        // per data time: snapshot slot or -1
   size_t n_snap = 0;                     // snapshots of the last run
   std::vector<uint8_t> select_valid;     // per group: a random particle is chosen
+  // beta interpolated in time (SIR, pipeline engine): kind, knots shared by
+  // the groups, values and spline slopes per group (or shared)
+  int beta_kind = -1;
+  size_t beta_n = 0, beta_groups = 0;
+  dbuf d_beta_t, d_beta_y, d_beta_k;
   dbuf d_select, d_groups, d_traj_out;   // lineage export
   double last_ms = 0;
   double last_fallbacks = 0;
@@ -566,6 +581,14 @@ pipe_slot make_pipe_slot(dust2_filter* h, size_t g, bool save_traj) {
   sl.errors = h->d_errors.get<error_record>() + g * P;
   sl.error_count = h->d_counters.get<uint32_t>();
   sl.n_fallback = h->d_counters.get<uint32_t>() + 1;
+  if (h->beta_n > 0) {
+    const size_t off = h->beta_groups == 1 ? 0 : g * h->beta_n;
+    sl.beta_t = h->d_beta_t.get<double>();
+    sl.beta_y = h->d_beta_y.get<double>() + off;
+    sl.beta_k = h->d_beta_k.get<double>() + off;
+    sl.beta_n = static_cast<int32_t>(h->beta_n);
+    sl.beta_kind = h->beta_kind;
+  }
   return sl;
 }
 
@@ -897,9 +920,13 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
 
     int engine = h->engine;
     if (engine == DUST2_ENGINE_AUTO) {
-      engine = (h->model != 0 || static_cast<int>(P) > sir_filter_max_particles(h->fp64))
+      engine = (h->model != 0 || h->beta_n > 0 ||
+                static_cast<int>(P) > sir_filter_max_particles(h->fp64))
                    ? DUST2_ENGINE_PIPELINE
                    : DUST2_ENGINE_PERSISTENT;
+    }
+    if (engine == DUST2_ENGINE_PERSISTENT && h->beta_n > 0) {
+      throw std::runtime_error("an interpolated beta runs on the pipeline engine only");
     }
     h->last_engine = engine;
     std::vector<filter_slot> slots;
@@ -1283,6 +1310,54 @@ int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errl
   }
 }
 
+int dust2_filter_set_beta_schedule(dust2_filter* h, int kind, const double* t, size_t n_t,
+                                   const double* y, size_t n_groups_y, char* err,
+                                   size_t errlen) {
+  try {
+    check_filter(h);
+    if (kind < 0) {
+      h->beta_kind = -1;
+      h->beta_n = 0;
+      h->beta_groups = 0;
+      return 0;
+    }
+    if (h->model != 0) throw std::runtime_error("an interpolated beta is a SIR model input");
+    if (kind != INTERP_CONSTANT && kind != INTERP_LINEAR && kind != INTERP_SPLINE) {
+      throw std::runtime_error("unknown interpolation kind");
+    }
+    if (n_groups_y != 1 && n_groups_y != h->G) {
+      throw std::runtime_error("'beta_y' must be shared by all groups or given for each");
+    }
+    interp_validate(t, n_t, n_t, "beta_t", "beta_y");
+    if (n_t == 0 || (kind != INTERP_CONSTANT && n_t < 2)) {
+      throw std::runtime_error("'beta_t' needs at least " +
+                               std::string(kind == INTERP_CONSTANT ? "one time" : "two times"));
+    }
+    // slopes per group: y is [n_groups_y][n_t], one series each
+    std::vector<double> k(n_groups_y * n_t, 0.0);
+    if (kind == INTERP_SPLINE) {
+      for (size_t g = 0; g < n_groups_y; ++g) {
+        const std::vector<double> kg = interp_spline_slopes(t, y + g * n_t, n_t, 1);
+        std::copy(kg.begin(), kg.end(), k.begin() + g * n_t);
+      }
+    }
+    hipStream_t s = h->stream;
+    h->d_beta_t.alloc(n_t * sizeof(double));
+    h->d_beta_y.alloc(n_groups_y * n_t * sizeof(double));
+    h->d_beta_k.alloc(n_groups_y * n_t * sizeof(double));
+    to_device(h->d_beta_t, t, n_t * sizeof(double), s);
+    to_device(h->d_beta_y, y, n_groups_y * n_t * sizeof(double), s);
+    to_device(h->d_beta_k, k.data(), k.size() * sizeof(double), s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->beta_kind = kind;
+    h->beta_n = n_t;
+    h->beta_groups = n_groups_y;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 int dust2_filter_set_pipeline_chains(dust2_filter* h, int chains, char* err, size_t errlen) {
   try {
     check_filter(h);
@@ -1338,7 +1413,7 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       int chains;
     };
     std::vector<candidate> cand;
-    if (h->model == 0 && P <= sir_filter_max_particles(h->fp64)) {
+    if (h->model == 0 && h->beta_n == 0 && P <= sir_filter_max_particles(h->fp64)) {
       static const int shapes[][2] = {{64, 1},   {64, 2},   {64, 4},   {128, 2}, {128, 4},
                                       {256, 2},  {256, 4},  {256, 8},  {512, 2}, {512, 4},
                                       {512, 8},  {1024, 2}, {1024, 4}, {1024, 8}};

@@ -17,6 +17,8 @@ enum sampler_error : uint32_t {
   ERR_ODE_STEP_SMALL = 3,  // "step too small"
   ERR_ODE_VANISHED = 4,    // "step size vanished"
   ERR_ODE_INIT = 5,        // "Initial step size was not finite"
+  ERR_INTERP_BEFORE = 6,   // "Tried to interpolate at time = .., which is .. before .."
+  ERR_INTERP_AFTER = 7,    // "Tried to interpolate at time = .., which is .. after .."
 };
 
 // Doubles of LDS scratch every filter workgroup reserves ahead of its
@@ -116,6 +118,14 @@ struct pipe_slot {
   uint32_t* n_fallback;
   int32_t t_offset;            // SLOT_AUTO_MODE: intervals already run
   uint32_t n_time;             // SLOT_AUTO_MODE: intervals in the slot's series
+  // SIR with beta interpolated in time (interp_eval at each step's start
+  // time): knots beta_t[beta_n], values beta_y, spline slopes beta_k;
+  // beta_n = 0 keeps the constant beta of the parameter block
+  const double* beta_t;
+  const double* beta_y;
+  const double* beta_k;
+  int32_t beta_n;
+  int32_t beta_kind;
 };
 
 // Interval, buffer parity and mode of a slot for launch interval t; false

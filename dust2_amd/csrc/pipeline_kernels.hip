@@ -19,6 +19,7 @@
 #include "block_resample.hpp"
 #include "device_random.hpp"
 #include "engine.hpp"
+#include "interpolate.hpp"
 #include "models.hpp"
 
 namespace dust2_amd {
@@ -83,7 +84,24 @@ __global__ void __launch_bounds__(kAdvBlock)
       if (zero) y[sir_model::zero_first] = 0;
       uint32_t err = 0;
       real ea = 0, eb = 0;
-      sir_model::update<real>(dt_r, y, sh, r, det, err, ea, eb);
+      sir_model::shared<real> sh_s = sh;
+      if (sd.beta_n > 0) {
+        // beta at the step's start time, as the step's update sees it
+        const double time_s = sd.sched_time[t] + s * sd.dt;
+        int ie = INTERP_OK;
+        const double b = interp_eval(sd.beta_kind, sd.beta_t, sd.beta_y, sd.beta_k, sd.beta_n,
+                                     1, 0, time_s, ie);
+        if (ie != INTERP_OK) {
+          const double knot = ie == INTERP_BEFORE ? sd.beta_t[0] : sd.beta_t[sd.beta_n - 1];
+          sd.errors[p] = error_record{ie == INTERP_BEFORE ? ERR_INTERP_BEFORE : ERR_INTERP_AFTER,
+                                      static_cast<uint32_t>(t), time_s, knot};
+          atomicAdd(sd.error_count, 1u);
+          failed = true;
+          break;
+        }
+        sh_s.beta = static_cast<real>(b);
+      }
+      sir_model::update<real>(dt_r, y, sh_s, r, det, err, ea, eb);
       if (err) {
         sd.errors[p] = error_record{err, static_cast<uint32_t>(t), static_cast<double>(ea),
                                     static_cast<double>(eb)};

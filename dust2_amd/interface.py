@@ -24,7 +24,10 @@ class Generator:
     dust2 generator, inst/template/dust.R:1-12)."""
 
     def __init__(self, name, time_type, parameters, defaults, constant, state, n_data=1,
-                 sizes=None, precisions=("fp64", "fp32")):
+                 sizes=None, precisions=("fp64", "fp32"), interpolate_beta=None):
+        # interpolate_beta: None, or the interpolation kind of a beta given
+        # as knots 'beta_t' and values 'beta_y' (array parameters)
+        self.interpolate_beta = interpolate_beta
         self.name = name
         self.time_type = time_type
         self.parameters = parameters
@@ -56,6 +59,69 @@ def sir():
         "sir", "discrete", ["N", "I0", "beta", "gamma", "exp_noise"],
         {"N": 1000.0, "I0": 10.0, "beta": 0.2, "gamma": 0.1, "exp_noise": 1e6},
         {"N", "exp_noise"}, ["S", "I", "R", "cases_cumul", "cases_inc"])
+
+
+def sir_time_varying(kind="linear"):
+    """The stochastic SIR with beta interpolated in time ('constant',
+    'linear' or 'spline'): the model odin2 generates for
+    `beta <- interpolate(beta_t, beta_y, kind)` in inst/examples/sir.cpp,
+    with array parameters 'beta_t' (knot times, shared by all groups) and
+    'beta_y' (values, per group); each step uses beta at its start time
+    (dust2/interpolate/interpolate.hpp).  Filters only, pipeline engine."""
+    if kind not in INTERP_KINDS:
+        raise ValueError(f"Unknown interpolation kind '{kind}'")
+    return Generator(
+        "sir", "discrete", ["N", "I0", "beta", "gamma", "exp_noise"],
+        {"N": 1000.0, "I0": 10.0, "beta": 0.0, "gamma": 0.1, "exp_noise": 1e6},
+        {"N", "exp_noise"}, ["S", "I", "R", "cases_cumul", "cases_inc"],
+        interpolate_beta=kind)
+
+
+def _split_beta(generator, pars):
+    """Scalar parameters and the (beta_t, beta_y per entry) arrays of a
+    generator with an interpolated beta."""
+    if generator.interpolate_beta is None or pars is None:
+        return pars, None
+    single = isinstance(pars, dict)
+    plist = [pars] if single else list(pars)
+    scal, arrays = [], []
+    for p in plist:
+        q = dict(p)
+        bt, by = q.pop("beta_t", None), q.pop("beta_y", None)
+        if (bt is None) != (by is None):
+            raise ValueError("'beta_t' and 'beta_y' must be given together")
+        scal.append(q)
+        arrays.append(None if bt is None else (f64(bt).reshape(-1), f64(by).reshape(-1)))
+    return (scal[0] if single else scal), arrays
+
+
+def _set_beta(obj, arrays, groups):
+    """Store the interpolated beta of `groups` (None: all) and upload it."""
+    if arrays is None or all(a is None for a in arrays):
+        return
+    G = obj.n_groups
+    groups = list(range(G)) if groups is None else [int(g) for g in groups]
+    if len(arrays) == 1 and len(groups) > 1:
+        arrays = arrays * len(groups)
+    for g, a in zip(groups, arrays):
+        if a is None:
+            continue
+        bt, by = a
+        if obj.beta_t is None:
+            obj.beta_t = bt.copy()
+            obj.beta_y = np.full((G, bt.size), np.nan)
+        if bt.size != obj.beta_t.size or not np.array_equal(bt, obj.beta_t):
+            raise ValueError("'beta_t' must be the same for every group")
+        if by.size != bt.size:
+            raise ValueError("Time variable 'beta_t' and interpolation target 'beta_y' must "
+                             f"have the same length, but do not ({bt.size} vs {by.size})")
+        obj.beta_y[g] = by
+    if np.isnan(obj.beta_y).any():
+        raise ValueError("'beta_t' and 'beta_y' must be given for every group")
+    shared = bool(np.all(obj.beta_y == obj.beta_y[:1]))
+    y = np.ascontiguousarray(obj.beta_y[:1] if shared else obj.beta_y)
+    call("dust2_filter_set_beta_schedule", obj.ptr, INTERP_KINDS[obj.generator.interpolate_beta],
+         dptr(obj.beta_t), obj.beta_t.size, dptr(y), y.shape[0])
 
 
 def sirode():
@@ -245,6 +311,8 @@ class Filter:
         self.launch = None
         self.engine = None
         self.chains = None
+        self.beta_t = None
+        self.beta_y = None
         self.last_index = None
         self.last_index_state = None
         self.last_snapshots = None
@@ -254,6 +322,10 @@ class Filter:
         return self.ptr is not None
 
     def _alloc(self, pars):
+        pars, arrays = _split_beta(self.generator, pars)
+        if self.generator.interpolate_beta is not None and (
+                arrays is None or any(a is None for a in arrays)):
+            raise ValueError("'beta_t' and 'beta_y' are required by an interpolated beta")
         rows = pars_matrix(self.generator, pars, self.n_groups)
         self.rows = rows
         raw, n_raw, seed = _seed_args(self.seed)
@@ -263,6 +335,7 @@ class Filter:
              self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),
              self.n_groups_data, u64ptr(raw), n_raw, seed, self.precision, C.byref(out))
         self.ptr = out
+        _set_beta(self, arrays, None)
         if self.launch is not None:
             call("dust2_filter_set_launch", self.ptr, *self.launch)
         if self.engine is not None:
@@ -355,9 +428,11 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
     elif pars is not None:
         idx, n_idx, _ = _index_group(index_group, obj.n_groups)
         sel = np.arange(obj.n_groups) if idx is None else idx.astype(np.int64)
+        pars, arrays = _split_beta(obj.generator, pars)
         rows = pars_update(obj.generator, pars, obj.rows[sel])
         call("dust2_filter_update_pars", obj.ptr, dptr(rows), szptr(idx), n_idx)
         obj.rows[sel] = rows
+        _set_beta(obj, arrays, sel)
     idx, n_idx, n_out = _index_group(index_group, obj.n_groups)
     init = None
     if initial is not None:

@@ -160,6 +160,19 @@ int dust2_filter_set_launch(dust2_filter* h, int block, int particles_per_thread
  * engine's threads per workgroup and particles per thread. */
 int dust2_filter_last_stats(dust2_filter* h, double* out, char* err, size_t errlen);
 
+/* SIR with beta interpolated in time: from the next run each step uses
+ * beta at its start time from the interpolator (kind 0 piecewise constant,
+ * 1 linear, 2 natural cubic spline) through knots t[n_t], values
+ * y[n_groups_y][n_t] (n_groups_y 1: shared, or one row per group), in place
+ * of the constant beta; kind < 0 removes it.  The model odin2 generates for
+ * `beta <- interpolate(beta_t, beta_y, kind)` in inst/examples/sir.cpp's
+ * update, with dust2::interpolate (dust2/interpolate/interpolate.hpp:69-268)
+ * and its errors for a step time outside the knots, reported per particle.
+ * Runs on the pipeline engine. */
+int dust2_filter_set_beta_schedule(dust2_filter* h, int kind, const double* t, size_t n_t,
+                                   const double* y, size_t n_groups_y, char* err,
+                                   size_t errlen);
+
 /* Number of independent launch chains (1-4, each on its own stream) the
  * pipeline engine splits the groups into, so that one chain's resampling
  * overlaps another chain's advance (default 2; dust2_filter_autotune

@@ -1497,4 +1497,72 @@ inline int eval(int kind, const std::vector<double>& t, const std::vector<double
 
 }  // namespace interp
 
+// ---------------------------------------------------------------------------
+// The SIR of inst/examples/sir.cpp with beta interpolated in time, the form
+// odin2 generates for `beta <- interpolate(beta_t, beta_y, kind)`: update
+// evaluates beta at the step's start time (dust2::interpolate, its errors
+// thrown as the reference's messages).  Parameter row: N, I0, (unused),
+// gamma, exp_noise, kind, n_knots, beta_t[64], beta_y[64].
+// ---------------------------------------------------------------------------
+struct sir_tv {
+  static constexpr size_t max_knots = 64;
+  static constexpr size_t n_state = 5;
+  static constexpr size_t n_pars = 7 + 2 * max_knots;
+  struct shared_state {
+    double N, I0, gamma, exp_noise;
+    int kind;
+    std::vector<double> t, y, k;
+  };
+  using data_type = sir::data_type;
+  static shared_state build_shared(const double* p) {
+    shared_state sh;
+    sh.N = p[0];
+    sh.I0 = p[1];
+    sh.gamma = p[3];
+    sh.exp_noise = p[4];
+    sh.kind = static_cast<int>(p[5]);
+    const size_t n = static_cast<size_t>(p[6]);
+    sh.t.assign(p + 7, p + 7 + n);
+    sh.y.assign(p + 7 + max_knots, p + 7 + max_knots + n);
+    sh.k = sh.kind == interp::spline ? interp::spline_slopes(sh.t, sh.y, 1)
+                                     : std::vector<double>(n, 0.0);
+    return sh;
+  }
+  static constexpr size_t n_data = 1;
+  static data_type build_data(const double* d) { return sir::build_data(d); }
+  static constexpr double zero_period = sir::zero_period;
+  static constexpr size_t zero_first = sir::zero_first;
+  static constexpr size_t zero_count = sir::zero_count;
+
+  static double beta(double time, const shared_state& sh) {
+    double b = 0;
+    const int rc = interp::eval(sh.kind, sh.t, sh.y, sh.k, 1, time, &b);
+    if (rc != 0) {
+      std::ostringstream msg;
+      if (rc == 1) {
+        msg << "Tried to interpolate at time = " << time << ", which is " << sh.t[0] - time
+            << " before the first time (" << sh.t[0] << ")";
+      } else {
+        msg << "Tried to interpolate at time = " << time << ", which is "
+            << time - sh.t.back() << " after the last time (" << sh.t.back() << ")";
+      }
+      throw std::runtime_error(msg.str());
+    }
+    return b;
+  }
+  static void initial(double time, const shared_state& sh, rng_state& rng, double* y) {
+    sir::initial(time, sir::shared_state{sh.N, sh.I0, 0, sh.gamma, sh.exp_noise}, rng, y);
+  }
+  static void update(double time, double dt, const double* y, const shared_state& sh,
+                     rng_state& rng, double* yn) {
+    const sir::shared_state s{sh.N, sh.I0, beta(time, sh), sh.gamma, sh.exp_noise};
+    sir::update(time, dt, y, s, rng, yn);
+  }
+  static double compare_data(double time, const double* y, const data_type& d,
+                             const shared_state& sh, rng_state& rng) {
+    return sir::compare_data(time, y, d, sir::shared_state{sh.N, sh.I0, 0, sh.gamma,
+                                                           sh.exp_noise}, rng);
+  }
+};
+
 }  // namespace oracle

@@ -230,6 +230,17 @@ int oracle_sir_filter_run(const double* pars, size_t G, size_t P,
                               ancestry, err, errlen);
 }
 
+int oracle_sir_tv_filter_run(const double* pars, size_t G, size_t P, double time_start,
+                             const double* time, size_t n_time, double dt, const double* data,
+                             size_t n_groups_data, const uint64_t* rng_in, uint64_t seed,
+                             int n_threads, const double* initial, const size_t* index_group,
+                             size_t n_index, double* ll_out, double* state_out,
+                             uint64_t* rng_out, uint32_t* ancestry, char* err, size_t errlen) {
+  return filter_run_impl<sir_tv>(pars, G, P, time_start, time, n_time, dt, data, n_groups_data,
+                                 rng_in, seed, n_threads, initial, index_group, n_index, ll_out,
+                                 state_out, rng_out, ancestry, err, errlen);
+}
+
 int oracle_seir16_filter_run(const double* pars, size_t G, size_t P,
                              double time_start, const double* time,
                              size_t n_time, double dt, const double* data,
@@ -253,6 +264,14 @@ int oracle_sir_simulate(const double* pars, size_t G, size_t P,
                         uint64_t* rng_out, char* err, size_t errlen) {
   return simulate_impl<sir>(pars, G, P, time_start, dt, times, n_times, rng_in,
                             deterministic, n_threads, out, rng_out, err, errlen);
+}
+
+int oracle_sir_tv_simulate(const double* pars, size_t G, size_t P, double time_start,
+                           double dt, const double* times, size_t n_times,
+                           const uint64_t* rng_in, int deterministic, int n_threads,
+                           double* out, uint64_t* rng_out, char* err, size_t errlen) {
+  return simulate_impl<sir_tv>(pars, G, P, time_start, dt, times, n_times, rng_in,
+                               deterministic, n_threads, out, rng_out, err, errlen);
 }
 
 int oracle_seir16_simulate(const double* pars, size_t G, size_t P,

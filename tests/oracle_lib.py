@@ -71,10 +71,12 @@ def oracle():
                 C.c_char_p, C.c_size_t]
         o.oracle_sir_filter_run.argtypes = filt
         o.oracle_seir16_filter_run.argtypes = filt
+        o.oracle_sir_tv_filter_run.argtypes = filt
         sim = [D, C.c_size_t, C.c_size_t, C.c_double, C.c_double, D, C.c_size_t, U64, C.c_int,
                C.c_int, D, U64, C.c_char_p, C.c_size_t]
         o.oracle_sir_simulate.argtypes = sim
         o.oracle_seir16_simulate.argtypes = sim
+        o.oracle_sir_tv_simulate.argtypes = sim
         o.oracle_sirode_unfilter_run.argtypes = [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                                  C.c_size_t, D, C.c_size_t, D, C.c_int, D, D,
                                                  D, C.c_char_p, C.c_size_t]
@@ -191,7 +193,7 @@ def sir_filter(pars, n_particles, time_start, time, dt, data, rng=None, seed=42,
     time = np.ascontiguousarray(time, dtype=np.float64)
     data = np.ascontiguousarray(data, dtype=np.float64)
     n_groups_data = data.shape[1]
-    S = 5 if model == "sir" else 80
+    S = 80 if model == "seir16" else 5
     ll = np.zeros(G)
     state = np.zeros((G, P, S))
     rng_out = np.zeros((G * (P + 2), 4), dtype=np.uint64)
@@ -200,7 +202,8 @@ def sir_filter(pars, n_particles, time_start, time, dt, data, rng=None, seed=42,
     init = None if initial is None else np.ascontiguousarray(initial, dtype=np.float64)
     idx = None if index_group is None else np.ascontiguousarray(index_group, dtype=np.uintp)
     buf = C.create_string_buffer(4096)
-    fn = oracle().oracle_sir_filter_run if model == "sir" else oracle().oracle_seir16_filter_run
+    fn = {"sir": oracle().oracle_sir_filter_run, "seir16": oracle().oracle_seir16_filter_run,
+          "sir_tv": oracle().oracle_sir_tv_filter_run}[model]
     rc = fn(_dp(pars), G, P, float(time_start), _dp(time), time.size, float(dt), _dp(data),
             n_groups_data, _u64(rng_in), C.c_uint64(seed), n_threads, _dp(init),
             None if idx is None else idx.ctypes.data_as(SZ), 0 if idx is None else idx.size,
@@ -214,13 +217,14 @@ def sir_simulate(pars, n_particles, time_start, dt, times, rng, deterministic=Fa
     pars = np.ascontiguousarray(pars, dtype=np.float64)
     G = pars.shape[0]
     P = n_particles
-    S = 5 if model == "sir" else 80
+    S = 80 if model == "seir16" else 5
     times = np.ascontiguousarray(times, dtype=np.float64)
     rng = np.ascontiguousarray(rng, dtype=np.uint64)
     out = np.zeros((times.size, G, P, S))
     rng_out = np.zeros((G * P, 4), dtype=np.uint64)
     buf = C.create_string_buffer(4096)
-    fn = oracle().oracle_sir_simulate if model == "sir" else oracle().oracle_seir16_simulate
+    fn = {"sir": oracle().oracle_sir_simulate, "seir16": oracle().oracle_seir16_simulate,
+          "sir_tv": oracle().oracle_sir_tv_simulate}[model]
     rc = fn(_dp(pars), G, P, float(time_start), float(dt), _dp(times), times.size, _u64(rng),
             int(deterministic), n_threads, _dp(out), _u64(rng_out), buf, 4096)
     _check(rc, buf)
@@ -374,3 +378,19 @@ def ref_interpolate(kind, t, y, z):
     _check(rc, buf)
     out = out[:z.size * n_series].reshape(z.size, n_series)
     return out if series else out[:, 0]
+
+
+def sir_tv_rows(pars, kind, beta_t, beta_y):
+    """Oracle parameter rows of the SIR with interpolated beta: pars (G, 5)
+    SIR rows (beta unused), beta_t (n,), beta_y (G, n) or (n,)."""
+    kinds = {"constant": 0, "linear": 1, "spline": 2}
+    pars = np.atleast_2d(np.asarray(pars, dtype=np.float64))
+    beta_y = np.atleast_2d(np.asarray(beta_y, dtype=np.float64))
+    G, n = pars.shape[0], len(beta_t)
+    rows = np.zeros((G, 7 + 2 * 64))
+    rows[:, :5] = pars
+    rows[:, 5] = kinds[kind]
+    rows[:, 6] = n
+    rows[:, 7:7 + n] = beta_t
+    rows[:, 7 + 64:7 + 64 + n] = np.broadcast_to(beta_y, (G, n))
+    return rows

@@ -88,3 +88,43 @@ def test_oracle_equals_reference_build(kind, n_series):
     z = np.concatenate([t, rng.uniform(t[0], t[-1], 300)])
     np.testing.assert_array_equal(ol.interpolate(kind, t, y, z),
                                   ol.ref_interpolate(kind, t, y, z))
+
+
+# ---------------------------------------------------------------------------
+# The SIR with beta interpolated in time (oracle model sir_tv)
+SIR = [1000.0, 10.0, 0.0, 0.1, 1e6]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_sir_tv_flat_schedule_is_the_plain_sir(kind):
+    # a schedule that is 0.3 everywhere reproduces the SIR with beta = 0.3
+    # draw for draw, whatever the interpolation
+    time = np.arange(1.0, 31.0)
+    data = np.arange(30.0)[:, None]
+    beta_t = np.array([0.0, 7.0, 15.0, 40.0])
+    rows = ol.sir_tv_rows([SIR], kind, beta_t, np.full(4, 0.3))
+    plain = np.array([[1000.0, 10.0, 0.3, 0.1, 1e6]])
+    a = ol.sir_filter(rows, 50, 0.0, time, 0.25, data, seed=3, model="sir_tv")
+    b = ol.sir_filter(plain, 50, 0.0, time, 0.25, data, seed=3)
+    np.testing.assert_array_equal(a["ll"], b["ll"])
+    np.testing.assert_array_equal(a["state"], b["state"])
+
+
+def test_sir_tv_uses_beta_at_step_start_time():
+    # beta drops to 0 from time 10: no infection in a step starting at or
+    # after 10, so S at time 30 equals S at time 10 in every particle
+    rows = ol.sir_tv_rows([[1000.0, 10.0, 0.0, 0.05, 1e6]], "constant",
+                          np.array([0.0, 10.0]), np.array([0.8, 0.0]))
+    out, _ = ol.sir_simulate(rows, 40, 0.0, 0.25, np.array([10.0, 30.0]),
+                             ol.prng(40, seed=5), model="sir_tv")
+    np.testing.assert_array_equal(out[1, 0, :, 0], out[0, 0, :, 0])
+    assert np.all(out[0, 0, :, 0] < 990)
+
+
+def test_sir_tv_reports_times_outside_the_knots():
+    rows = ol.sir_tv_rows([SIR], "linear", np.array([0.0, 9.0]), np.array([0.2, 0.3]))
+    with pytest.raises(ol.OracleError,
+                       match=r"Tried to interpolate at time = 9.25, which is 0.25 after the "
+                             r"last time \(9\)"):
+        ol.sir_filter(rows, 10, 0.0, np.arange(1.0, 11.0), 0.25, np.ones((10, 1)), seed=1,
+                      model="sir_tv")
