@@ -25,6 +25,7 @@ from .interface import (  # noqa: F401
     dust_likelihood_set_rng_state,
     dust_system_compare_data,
     dust_system_create,
+    dust_system_internals,
     dust_system_reorder,
     dust_system_rng_state,
     dust_system_run_to_time,

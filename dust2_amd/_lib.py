@@ -100,6 +100,22 @@ _PROTOS = {
     "dust2_amd_resample": (C.c_int, [D, C.c_size_t, C.c_double, U32, D, ERR, ERRLEN]),
     "dust2_amd_trajectory_lineage": (C.c_int, [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
                                                C.c_size_t, SZ, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_set_critical_times": (C.c_int, [VP, D, C.c_size_t, ERR, ERRLEN]),
+    "dust2_system_sirode_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D, D,
+                                            C.c_size_t, C.c_size_t, C.POINTER(VP), ERR,
+                                            ERRLEN]),
+    "dust2_system_sirode_run_to_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
+    "dust2_system_sirode_simulate": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_system_sirode_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_sirode_set_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_sirode_set_state_initial": (C.c_int, [VP, ERR, ERRLEN]),
+    "dust2_system_sirode_time": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_sirode_set_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
+    "dust2_system_sirode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_sirode_compare_data": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_system_sirode_reorder": (C.c_int, [VP, U32, ERR, ERRLEN]),
+    "dust2_system_sirode_internals": (C.c_int, [VP, D, D, SZ, ERR, ERRLEN]),
+    "dust2_system_sirode_free": (None, [VP]),
     "dust2_amd_interpolate": (C.c_int, [C.c_int, D, C.c_size_t, D, C.c_size_t, C.c_size_t, D,
                                         C.c_size_t, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,

@@ -1620,6 +1620,13 @@ int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
 int dust2_system_sir_run_to_time(dust2_system* h, double time, char* err, size_t errlen) {
   try {
     check_system(h, "run");
+    if (time < h->time) {
+      // dust2/r/system.hpp:24-27
+      char buf[256];
+      std::snprintf(buf, sizeof buf, "Can't run to time %f, system already at time %f", time,
+                    h->time);
+      throw std::runtime_error(buf);
+    }
     system_run(h, time, false);
     return 0;
   } catch (const std::exception& e) {
@@ -1991,6 +1998,7 @@ struct dust2_unfilter {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
+  dbuf d_critical;
   dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active,
       d_grad, d_traj;
   bool traj_valid = false;
@@ -2006,6 +2014,45 @@ struct dust2_unfilter {
 
 namespace {
 constexpr size_t kSirodePars = 4;
+
+// dust_ode_control() values (R/interface-continuous.R:51-79: max_steps,
+// atol, rtol, step_size_min, step_size_max; null for the defaults) with
+// the C++ controller constants (dust2/continuous/control.hpp:11-24).
+ode_control_dev ode_control_from(const double* control) {
+  const double def[5] = {10000, 1e-6, 1e-6, 0, std::numeric_limits<double>::infinity()};
+  const double* c = control ? control : def;
+  if (!(c[0] >= 1) || !(c[1] > 0) || !(c[2] > 0) || !(c[3] >= 0) || !(c[4] >= 0)) {
+    throw std::runtime_error("Invalid ODE control");
+  }
+  ode_control_dev ctl{};
+  ctl.max_steps = static_cast<uint32_t>(std::min(c[0], 4e9));
+  ctl.atol = c[1];
+  ctl.rtol = c[2];
+  ctl.step_size_min = std::max(c[3], std::numeric_limits<double>::epsilon());
+  ctl.step_size_max = c[4];
+  ctl.factor_safe = 0.9;
+  ctl.factor_min = 0.2;
+  ctl.factor_max = 10.0;
+  ctl.beta = 0.04;
+  ctl.constant = 0.2 - 0.04 * 0.75;
+  return ctl;
+}
+
+// control.critical_times (dust2/continuous/control.hpp:26-39): the times
+// must increase; kept on the device for the kernels.
+void set_critical_times(ode_control_dev& ctl, dbuf& d, const double* times, size_t n,
+                        hipStream_t s) {
+  for (size_t i = 1; i < n; ++i) {
+    if (!(times[i] > times[i - 1])) {
+      throw std::runtime_error("'critical_times' must be strictly increasing");
+    }
+  }
+  d.alloc(std::max<size_t>(n, 1) * sizeof(double));
+  if (n) to_device(d, times, n * sizeof(double), s);
+  hip_check(hipStreamSynchronize(s), "sync");
+  ctl.critical_times = n ? d.get<double>() : nullptr;
+  ctl.n_critical = static_cast<int>(n);
+}
 
 void check_sirode_pars(const double* row) {
   const char* names[] = {"N", "I0", "beta", "gamma"};
@@ -2050,23 +2097,7 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
     h->time_start = time_start;
     h->pars.assign(pars, pars + n_groups * kSirodePars);
     h->grad_current.assign(n_groups, 0);
-    // dust_ode_control() defaults (R/interface-continuous.R:51-79) with the
-    // C++ controller constants (dust2/continuous/control.hpp:11-24)
-    const double def[5] = {10000, 1e-6, 1e-6, 0, std::numeric_limits<double>::infinity()};
-    const double* c = control ? control : def;
-    if (!(c[0] >= 1) || !(c[1] > 0) || !(c[2] > 0) || !(c[3] >= 0) || !(c[4] >= 0)) {
-      throw std::runtime_error("Invalid ODE control");
-    }
-    h->control.max_steps = static_cast<uint32_t>(std::min(c[0], 4e9));
-    h->control.atol = c[1];
-    h->control.rtol = c[2];
-    h->control.step_size_min = std::max(c[3], std::numeric_limits<double>::epsilon());
-    h->control.step_size_max = c[4];
-    h->control.factor_safe = 0.9;
-    h->control.factor_min = 0.2;
-    h->control.factor_max = 10.0;
-    h->control.beta = 0.04;
-    h->control.constant = 0.2 - 0.04 * 0.75;
+    h->control = ode_control_from(control);
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
@@ -2312,6 +2343,357 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
 }
 
 void dust2_unfilter_sirode_free(dust2_unfilter* h) { delete h; }
+
+int dust2_unfilter_sirode_set_critical_times(dust2_unfilter* h, const double* times, size_t n,
+                                             char* err, size_t errlen) {
+  try {
+    check_unfilter(h);
+    set_critical_times(h->control, h->d_critical, times, n, h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Continuous-time system: the SIR ODE (dust2/continuous/system.hpp)
+// ============================================================================
+struct dust2_ode_system {
+  size_t G = 0, P = 0;
+  double time = 0;
+  std::vector<double> pars;           // G x 4
+  std::vector<uint8_t> reinit;        // per group: the solver must initialise
+  ode_control_dev control{};
+  uint32_t record_cap = 0;            // step times kept per particle (0: off)
+  hipStream_t stream = nullptr;
+  dbuf d_pars, d_state, d_internals, d_reinit, d_targets, d_out, d_errors, d_counters,
+      d_critical, d_step_times, d_step_count, d_tmp;
+  bool errors_pending = false;
+  ~dust2_ode_system() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void check_ode_system(dust2_ode_system* h, const char* action) {
+  if (h == nullptr) {
+    throw std::runtime_error("Pointer has been serialised, cannot continue safely (system)");
+  }
+  if (h->errors_pending) {
+    throw std::runtime_error(std::string("Can't currently ") + action +
+                             " this system: errors are pending");
+  }
+}
+
+// Integrates every particle from h->time through `targets`, writing the
+// state after each target to out ([n][G][P][S]) when given.
+void ode_system_run(dust2_ode_system* h, const double* targets, size_t n, double* out) {
+  hipStream_t s = h->stream;
+  const size_t n_total = h->G * h->P;
+  h->d_targets.alloc(std::max<size_t>(n, 1) * sizeof(double));
+  if (n) to_device(h->d_targets, targets, n * sizeof(double), s);
+  to_device(h->d_reinit, h->reinit.data(), h->G, s);
+  if (out) h->d_out.alloc(std::max<size_t>(n, 1) * kSirState * n_total * sizeof(double));
+  hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+  ode_sys_launch a{};
+  a.pars = h->d_pars.get<double>();
+  a.n_particles = static_cast<int>(h->P);
+  a.n_total = static_cast<int>(n_total);
+  a.reinit = h->d_reinit.get<uint8_t>();
+  a.time = h->time;
+  a.targets = h->d_targets.get<double>();
+  a.n_targets = static_cast<int>(n);
+  a.control = h->control;
+  a.state = h->d_state.get<double>();
+  a.internals = h->d_internals.get<ode_internals>();
+  a.out = out ? h->d_out.get<double>() : nullptr;
+  a.record = ode_record{h->record_cap ? h->d_step_times.get<double>() : nullptr,
+                        h->d_step_count.get<uint32_t>(), h->record_cap};
+  a.errors = h->d_errors.get<error_record>();
+  a.error_count = h->d_counters.get<uint32_t>();
+  hip_check(launch_dp5_system(a, s), "ode system kernel");
+  uint32_t count = 0;
+  to_host(&count, h->d_counters, sizeof(count), s);
+  std::fill(h->reinit.begin(), h->reinit.end(), 0);
+  if (count != 0) {
+    std::vector<error_record> rec(n_total);
+    to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->errors_pending = true;
+    throw std::runtime_error(error_report(rec, count));
+  }
+  if (n) h->time = targets[n - 1];
+  if (out) {
+    const size_t block = kSirState * n_total;
+    std::vector<double> tmp(n * block);
+    to_host(tmp.data(), h->d_out, tmp.size() * sizeof(double), s);
+    for (size_t q = 0; q < n; ++q) {
+      soa_to_aos(tmp.data() + q * block, out + q * block, 1, n_total, kSirState, true);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dust2_system_sirode_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                              double time, const double* control, const double* critical_times,
+                              size_t n_critical, size_t record_step_times,
+                              dust2_ode_system** out, char* err, size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    for (size_t g = 0; g < n_groups; ++g) check_sirode_pars(pars + g * kSirodePars);
+    auto h = std::make_unique<dust2_ode_system>();
+    h->G = n_groups;
+    h->P = n_particles;
+    h->time = time;
+    h->pars.assign(pars, pars + n_groups * kSirodePars);
+    h->reinit.assign(n_groups, 1);
+    h->control = ode_control_from(control);
+    h->record_cap = static_cast<uint32_t>(std::min<size_t>(record_step_times, 1u << 20));
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t n_total = n_groups * n_particles;
+    h->d_pars.alloc(n_groups * kSirodePars * sizeof(double));
+    to_device(h->d_pars, pars, n_groups * kSirodePars * sizeof(double), h->stream);
+    h->d_state.alloc(kSirState * n_total * sizeof(double));
+    h->d_internals.alloc(n_total * sizeof(ode_internals));
+    h->d_reinit.alloc(n_groups);
+    h->d_errors.alloc(n_total * sizeof(error_record));
+    h->d_counters.alloc(4 * sizeof(uint32_t));
+    h->d_step_count.alloc(n_total * sizeof(uint32_t));
+    if (h->record_cap) h->d_step_times.alloc(n_total * h->record_cap * sizeof(double));
+    hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_internals.get(), 0, h->d_internals.bytes(), h->stream),
+              "memset");
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_step_count.get(), 0, h->d_step_count.bytes(), h->stream),
+              "memset");
+    set_critical_times(h->control, h->d_critical, critical_times, n_critical, h->stream);
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_run_to_time(dust2_ode_system* h, double time, char* err,
+                                    size_t errlen) {
+  try {
+    check_ode_system(h, "run");
+    if (time < h->time) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf, "Can't run to time %f, system already at time %f", time,
+                    h->time);
+      throw std::runtime_error(buf);
+    }
+    ode_system_run(h, &time, 1, nullptr);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_simulate(dust2_ode_system* h, const double* times, size_t n_times,
+                                 double* out, char* err, size_t errlen) {
+  try {
+    check_ode_system(h, "simulate");
+    double prev = h->time;
+    for (size_t i = 0; i < n_times; ++i) {
+      if (times[i] < prev) {
+        throw std::runtime_error("Expected 'times' to be increasing, starting at the "
+                                 "system's time");
+      }
+      prev = times[i];
+    }
+    ode_system_run(h, times, n_times, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_state(dust2_ode_system* h, double* out, char* err, size_t errlen) {
+  try {
+    check_ode_system(h, "get state from");
+    std::vector<double> tmp(kSirState * h->G * h->P);
+    to_host(tmp.data(), h->d_state, tmp.size() * sizeof(double), h->stream);
+    soa_to_aos(tmp.data(), out, 1, h->G * h->P, kSirState, true);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_set_state(dust2_ode_system* h, const double* state, char* err,
+                                  size_t errlen) {
+  try {
+    if (h == nullptr) check_ode_system(h, "set state for");
+    std::vector<double> tmp(kSirState * h->G * h->P);
+    aos_to_soa(state, tmp.data(), 1, h->G * h->P, kSirState, true);
+    to_device(h->d_state, tmp.data(), tmp.size() * sizeof(double), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    std::fill(h->reinit.begin(), h->reinit.end(), 1);
+    h->errors_pending = false;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_set_state_initial(dust2_ode_system* h, char* err, size_t errlen) {
+  try {
+    if (h == nullptr) check_ode_system(h, "set state for");
+    // sirode.cpp:35-45: S = N - I0, I = I0, the rest 0
+    std::vector<double> st(h->G * h->P * kSirState, 0.0);
+    for (size_t g = 0; g < h->G; ++g) {
+      for (size_t p = 0; p < h->P; ++p) {
+        double* y = st.data() + (g * h->P + p) * kSirState;
+        y[0] = h->pars[g * kSirodePars + 0] - h->pars[g * kSirodePars + 1];
+        y[1] = h->pars[g * kSirodePars + 1];
+      }
+    }
+    h->errors_pending = false;
+    return dust2_system_sirode_set_state(h, st.data(), err, errlen);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_time(dust2_ode_system* h, double* out, char* err, size_t errlen) {
+  try {
+    check_ode_system(h, "get time from");
+    *out = h->time;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_set_time(dust2_ode_system* h, double time, char* err, size_t errlen) {
+  try {
+    check_ode_system(h, "set time for");
+    h->time = time;
+    std::fill(h->reinit.begin(), h->reinit.end(), 1);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_update_pars(dust2_ode_system* h, const double* pars, char* err,
+                                    size_t errlen) {
+  try {
+    if (h == nullptr) check_ode_system(h, "update pars for");
+    for (size_t g = 0; g < h->G; ++g) check_sirode_pars(pars + g * kSirodePars);
+    h->pars.assign(pars, pars + h->G * kSirodePars);
+    to_device(h->d_pars, pars, h->pars.size() * sizeof(double), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    // update_shared marks the group's solver for initialisation
+    // (dust2/continuous/system.hpp:256-265)
+    std::fill(h->reinit.begin(), h->reinit.end(), 1);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_compare_data(dust2_ode_system* h, const double* data,
+                                     size_t n_groups_data, double* out, char* err,
+                                     size_t errlen) {
+  try {
+    check_ode_system(h, "compare data for");
+    if (n_groups_data != 1 && n_groups_data != h->G) {
+      throw std::runtime_error("Expected data for one group or every group");
+    }
+    std::vector<double> st(h->G * h->P * kSirState);
+    char buf[512];
+    if (dust2_system_sirode_state(h, st.data(), buf, sizeof buf) != 0) {
+      throw std::runtime_error(buf);
+    }
+    // sirode.cpp:89-101: log Poisson(incidence; cases_inc), no draw
+    for (size_t g = 0; g < h->G; ++g) {
+      const double x = data[n_groups_data == 1 ? 0 : g];
+      for (size_t p = 0; p < h->P; ++p) {
+        const double lambda = st[(g * h->P + p) * kSirState + 4];
+        double v = 0;
+        if (!std::isnan(x)) {
+          v = (x == 0 && lambda == 0) ? 0
+              : lambda == 0           ? -std::numeric_limits<double>::infinity()
+                                      : x * std::log(lambda) - lambda - std::lgamma(x + 1);
+        }
+        out[g * h->P + p] = v;
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_reorder(dust2_ode_system* h, const uint32_t* index, char* err,
+                                size_t errlen) {
+  try {
+    check_ode_system(h, "reorder");
+    const size_t n_total = h->G * h->P;
+    for (size_t k = 0; k < n_total; ++k) {
+      if (index[k] >= h->P) throw std::runtime_error("reorder index out of range");
+    }
+    // the state and the solver internals move together
+    // (dust2/continuous/system.hpp:172-199)
+    std::vector<double> st(kSirState * n_total), st2(kSirState * n_total);
+    std::vector<ode_internals> in(n_total), in2(n_total);
+    to_host(st.data(), h->d_state, st.size() * sizeof(double), h->stream);
+    to_host(in.data(), h->d_internals, in.size() * sizeof(ode_internals), h->stream);
+    for (size_t g = 0; g < h->G; ++g) {
+      for (size_t p = 0; p < h->P; ++p) {
+        const size_t to = g * h->P + p, from = g * h->P + index[to];
+        for (size_t c = 0; c < kSirState; ++c) st2[c * n_total + to] = st[c * n_total + from];
+        in2[to] = in[from];
+      }
+    }
+    to_device(h->d_state, st2.data(), st2.size() * sizeof(double), h->stream);
+    to_device(h->d_internals, in2.data(), in2.size() * sizeof(ode_internals), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_sirode_internals(dust2_ode_system* h, double* stats, double* step_times,
+                                  size_t* n_step_times, char* err, size_t errlen) {
+  try {
+    check_ode_system(h, "get internals from");
+    const size_t n_total = h->G * h->P;
+    std::vector<ode_internals> in(n_total);
+    to_host(in.data(), h->d_internals, in.size() * sizeof(ode_internals), h->stream);
+    for (size_t k = 0; k < n_total; ++k) {
+      stats[4 * k + 0] = in[k].n_steps;
+      stats[4 * k + 1] = in[k].n_accepted;
+      stats[4 * k + 2] = in[k].n_rejected;
+      stats[4 * k + 3] = in[k].step_size;
+    }
+    if (n_step_times) {
+      std::vector<uint32_t> cnt(n_total);
+      to_host(cnt.data(), h->d_step_count, cnt.size() * sizeof(uint32_t), h->stream);
+      for (size_t k = 0; k < n_total; ++k) n_step_times[k] = h->record_cap ? cnt[k] : 0;
+      if (step_times && h->record_cap) {
+        to_host(step_times, h->d_step_times, n_total * h->record_cap * sizeof(double),
+                h->stream);
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_system_sirode_free(dust2_ode_system* h) { delete h; }
 
 }  // extern "C"
 

@@ -160,6 +160,30 @@ struct ode_control_dev {
   uint32_t max_steps;
   double atol, rtol, step_size_min, step_size_max;
   double factor_safe, factor_min, factor_max, beta, constant;
+  // times the solver must stop at, increasing (dust2/continuous/control.hpp
+  // critical_times), device memory, or null
+  const double* critical_times;
+  int n_critical;
+};
+
+// Solver internals of one particle of a continuous-time system kept in HBM
+// between calls (dust2/continuous/solver.hpp:34-73 internals: derivative,
+// last step's dense output of the reset variable, controller state and
+// counters).
+struct ode_internals {
+  double dydt[5];
+  double step_size, error, t0, h;
+  double c1, c2, c3, c4, c5;
+  uint32_t n_steps, n_accepted, n_rejected, pad;
+};
+
+// Step times of one particle (control debug_record_step_times): the time
+// of the initialisation and of the end of every accepted step, the first
+// `cap` kept; count keeps the total.
+struct ode_record {
+  double* times;
+  uint32_t* count;
+  uint32_t cap;
 };
 
 struct ode_launch {
@@ -194,6 +218,27 @@ struct ode_launch {
 };
 
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream);
+
+// ---- continuous-time system (SIR ODE, FP64) ---------------------------------
+// Runs every particle from `time` through the target times in turn; the
+// state after target q goes to out[q][S][n_total] when out is given.
+struct ode_sys_launch {
+  const double* pars;          // [G][4]: N, I0, beta, gamma
+  int n_particles;
+  int n_total;
+  const uint8_t* reinit;       // [G]: initialise the solver first
+  double time;                 // system time at the start
+  const double* targets;       // [n_targets]
+  int n_targets;
+  ode_control_dev control;
+  double* state;               // [S][n_total]
+  ode_internals* internals;    // [n_total]
+  double* out;                 // [n_targets][S][n_total] or null
+  ode_record record;           // step times, or times = null
+  error_record* errors;        // [n_total]
+  uint32_t* error_count;
+};
+hipError_t launch_dp5_system(const ode_sys_launch& args, hipStream_t stream);
 
 // ---- deterministic discrete SIR unfilter and adjoint -------------------------
 struct sir_unfilter_launch {

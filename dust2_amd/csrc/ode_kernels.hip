@@ -276,6 +276,64 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
   }
 }
 
+__device__ __forceinline__ void record_step(const ode_record& rec, int i, double t) {
+  if (rec.times == nullptr) return;
+  const uint32_t n = rec.count[i];
+  if (n < rec.cap) rec.times[static_cast<size_t>(i) * rec.cap + n] = t;
+  rec.count[i] = n + 1;
+}
+
+// solver::run (solver.hpp:231-262): integrate from t to t_end, stopping at
+// every critical time in between, applying the reset of the incidence
+// variable at each step start that crossed a reset boundary
+// (apply_zero_every, solver.hpp:437-465) and after the last step
+// (apply_zero_every_final, solver.hpp:468-492).  Returns the time reached.
+template <typename real, bool SENS>
+__device__ double ode_run(double t, double t_end, ode_state<real>& s,
+                          const sirode_model::shared<real>& sh, const ode_control_dev& ctl,
+                          uint32_t& code, const ode_record& rec, int i) {
+  // first critical time after t (std::upper_bound)
+  int ic = 0;
+  while (ic < ctl.n_critical && ctl.critical_times[ic] <= t) ++ic;
+  double t_end_i =
+      (ic == ctl.n_critical || ctl.critical_times[ic] >= t_end) ? t_end : ctl.critical_times[ic];
+  while (t < t_end && code == 0) {
+    if (s.h != 0) {
+      const double period = sirode_model::zero_period;
+      const int n = static_cast<int>(floor(t / period));
+      const int m = static_cast<int>(floor(s.t0 / period));
+      if (n > m) {
+        const double t_reset = n * period;
+        if (t_reset == t) {
+          s.y[sirode_model::zero_first] = 0;
+          if constexpr (SENS) s.sy[sirode_model::zero_first] = 0;
+        } else {
+          s.y[sirode_model::zero_first] -= interp(s, t_reset);
+          if constexpr (SENS) s.sy[sirode_model::zero_first] -= interp_sens(s, t_reset);
+        }
+      }
+    }
+    t = ode_step<real, SENS>(t, t_end_i, s, sh, ctl, code);
+    if (code == 0) record_step(rec, i, t);
+    if (t >= t_end_i && t < t_end) {
+      ++ic;
+      t_end_i = (ic == ctl.n_critical || ctl.critical_times[ic] >= t_end)
+                    ? t_end
+                    : ctl.critical_times[ic];
+    }
+  }
+  if (code == 0 && s.h != 0 && s.h > sirode_model::zero_period) {
+    const double period = sirode_model::zero_period;
+    const int n = static_cast<int>(ceil(t / period));
+    const int m = static_cast<int>(ceil(s.t0 / period));
+    if (n > m) {
+      s.y[sirode_model::zero_first] -= interp(s, (n - 1) * period);
+      if constexpr (SENS) s.sy[sirode_model::zero_first] -= interp_sens(s, (n - 1) * period);
+    }
+  }
+  return t;
+}
+
 // solver::initialise (solver.hpp:264-324), Hairer's starting step.
 template <typename real>
 __device__ void ode_initialise(ode_state<real>& s, const sirode_model::shared<real>& sh,
@@ -361,39 +419,10 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
     dstride = 1;
     n_time = args.series_len[g];
   }
+  const ode_record no_record{nullptr, nullptr, 0};
   for (int t = 0; t < n_time && code == 0; ++t) {
     const double t_end = times[t];
-    // solver::run (solver.hpp:231-262)
-    double tt = time;
-    while (tt < t_end && code == 0) {
-      // apply_zero_every (solver.hpp:437-465)
-      if (s.h != 0) {
-        const double period = sirode_model::zero_period;
-        const int n = static_cast<int>(floor(tt / period));
-        const int m = static_cast<int>(floor(s.t0 / period));
-        if (n > m) {
-          const double t_reset = n * period;
-          if (t_reset == tt) {
-            s.y[sirode_model::zero_first] = 0;
-            if constexpr (SENS) s.sy[sirode_model::zero_first] = 0;
-          } else {
-            s.y[sirode_model::zero_first] -= interp(s, t_reset);
-            if constexpr (SENS) s.sy[sirode_model::zero_first] -= interp_sens(s, t_reset);
-          }
-        }
-      }
-      tt = ode_step<real, SENS>(tt, t_end, s, sh, ctl, code);
-    }
-    // apply_zero_every_final (solver.hpp:468-492)
-    if (code == 0 && s.h != 0 && s.h > sirode_model::zero_period) {
-      const double period = sirode_model::zero_period;
-      const int n = static_cast<int>(ceil(tt / period));
-      const int m = static_cast<int>(ceil(s.t0 / period));
-      if (n > m) {
-        s.y[sirode_model::zero_first] -= interp(s, (n - 1) * period);
-        if constexpr (SENS) s.sy[sirode_model::zero_first] -= interp_sens(s, (n - 1) * period);
-      }
-    }
+    ode_run<real, SENS>(time, t_end, s, sh, ctl, code, no_record, i);
     time = t_end;
     if (args.traj != nullptr) {
       real* tr = static_cast<real*>(args.traj) + static_cast<size_t>(t) * NV * args.n_total;
@@ -435,7 +464,86 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
   }
 }
 
+// The continuous-time system (dust2/continuous/system.hpp:80-106,
+// 444-482, 531-542): the solver internals of each particle persist in HBM
+// between calls; a group whose state, time or parameters changed
+// initialises its solver first (initialise_solver_), recording the
+// starting time among the step times as solver::initialise does.
+__global__ void __launch_bounds__(256) dp5_system_kernel(ode_sys_launch a) {
+  static_assert(NV == 5, "ode_internals holds five variables");
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_total) return;
+  const int g = i / a.n_particles;
+  const double* pr = a.pars + 4 * g;
+  sirode_model::shared<double> sh;
+  sh.N = pr[0];
+  sh.I0 = pr[1];
+  sh.beta = pr[2];
+  sh.gamma = pr[3];
+  const size_t NT = static_cast<size_t>(a.n_total);
+  ode_state<double> s;
+  for (int k = 0; k < NV; ++k) s.y[k] = a.state[k * NT + i];
+  uint32_t code = 0;
+  ode_internals& in = a.internals[i];
+  if (a.reinit[g]) {
+    s.t0 = 0;
+    s.h = 0;
+    s.c1 = s.c2 = s.c3 = s.c4 = s.c5 = 0;
+    if (a.record.times) a.record.count[i] = 0;
+    record_step(a.record, i, a.time);
+    ode_initialise<double>(s, sh, a.control, code);
+  } else {
+    for (int k = 0; k < NV; ++k) s.dydt[k] = in.dydt[k];
+    s.step_size = in.step_size;
+    s.error = in.error;
+    s.t0 = in.t0;
+    s.h = in.h;
+    s.c1 = in.c1;
+    s.c2 = in.c2;
+    s.c3 = in.c3;
+    s.c4 = in.c4;
+    s.c5 = in.c5;
+    s.n_steps = in.n_steps;
+    s.n_accepted = in.n_accepted;
+    s.n_rejected = in.n_rejected;
+  }
+  double time = a.time;
+  for (int q = 0; q < a.n_targets && code == 0; ++q) {
+    ode_run<double, false>(time, a.targets[q], s, sh, a.control, code, a.record, i);
+    time = a.targets[q];
+    if (a.out) {
+      double* o = a.out + static_cast<size_t>(q) * NV * NT;
+      for (int k = 0; k < NV; ++k) o[k * NT + i] = s.y[k];
+    }
+  }
+  for (int k = 0; k < NV; ++k) a.state[k * NT + i] = s.y[k];
+  for (int k = 0; k < NV; ++k) in.dydt[k] = s.dydt[k];
+  in.step_size = s.step_size;
+  in.error = s.error;
+  in.t0 = s.t0;
+  in.h = s.h;
+  in.c1 = s.c1;
+  in.c2 = s.c2;
+  in.c3 = s.c3;
+  in.c4 = s.c4;
+  in.c5 = s.c5;
+  in.n_steps = s.n_steps;
+  in.n_accepted = s.n_accepted;
+  in.n_rejected = s.n_rejected;
+  if (code) {
+    a.errors[i] = error_record{code, 0u, 0.0, 0.0};
+    atomicAdd(a.error_count, 1u);
+  }
+}
+
 }  // namespace
+
+hipError_t launch_dp5_system(const ode_sys_launch& args, hipStream_t stream) {
+  if (args.n_total == 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((args.n_total + 255) / 256);
+  hipLaunchKernelGGL(dp5_system_kernel, dim3(blocks), dim3(256), 0, stream, args);
+  return hipGetLastError();
+}
 
 hipError_t launch_dp5_unfilter(const ode_launch& args, bool fp64, hipStream_t stream) {
   if (args.n_total == 0) return hipSuccess;

@@ -603,51 +603,93 @@ class System:
         return (self.n_groups, self.n_particles, self.generator.n_state)
 
 
-def dust_system_create(generator, pars, n_particles, n_groups=1, time=0, dt=1, seed=None,
-                       deterministic=False, precision="fp64"):
-    return System(generator, pars, n_particles, n_groups, time, dt, seed, deterministic,
-                  precision)
+class ContinuousSystem:
+    """A continuous-time system (dust_system_create for an ODE model,
+    R/interface.R:60-150; dust2/continuous/system.hpp): FP64, solver
+    internals kept on the device between calls."""
+    prefix = "dust2_system_sirode"
+
+    def __init__(self, generator, pars, n_particles, n_groups, time, ode_control):
+        if generator.name != "sirode":
+            raise ValueError(f"No continuous system kernels for '{generator.name}'")
+        self.generator = generator
+        self.n_particles = int(n_particles)
+        self.n_groups = int(n_groups)
+        self.rows = pars_matrix(generator, pars, self.n_groups)
+        self.ode_control = _ode_control(ode_control)
+        ctl = self.ode_control
+        self.record = 100000 if ctl.debug_record_step_times else 0
+        out = C.c_void_p()
+        call("dust2_system_sirode_alloc", dptr(self.rows), self.n_groups, self.n_particles,
+             float(time), dptr(ctl.values), dptr(ctl.critical_times), ctl.critical_times.size,
+             self.record, C.byref(out))
+        self.ptr = out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib._lib is not None:
+            _lib.lib().dust2_system_sirode_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def shape(self):
+        return (self.n_groups, self.n_particles, self.generator.n_state)
+
+
+def dust_system_create(generator, pars, n_particles, n_groups=1, time=0, dt=None, seed=None,
+                       deterministic=False, precision="fp64", ode_control=None):
+    if generator.time_type == "continuous":
+        if dt is not None:
+            raise ValueError("Can't use 'dt' with continuous-time systems")
+        return ContinuousSystem(generator, pars, n_particles, n_groups, time, ode_control)
+    if ode_control is not None:
+        raise ValueError("Can't use 'ode_control' with discrete-time systems")
+    return System(generator, pars, n_particles, n_groups, time, 1 if dt is None else dt, seed,
+                  deterministic, precision)
+
+
+def _prefix(sys):
+    return getattr(sys, "prefix", "dust2_system_sir")
 
 
 def dust_system_run_to_time(sys, time):
-    call("dust2_system_sir_run_to_time", sys.ptr, float(time))
+    call(_prefix(sys) + "_run_to_time", sys.ptr, float(time))
 
 
 def dust_system_simulate(sys, times):
     t = f64(times).reshape(-1)
     out = np.zeros((t.size,) + sys.shape)
-    call("dust2_system_sir_simulate", sys.ptr, dptr(t), t.size, dptr(out))
+    call(_prefix(sys) + "_simulate", sys.ptr, dptr(t), t.size, dptr(out))
     return out
 
 
 def dust_system_state(sys):
     out = np.zeros(sys.shape)
-    call("dust2_system_sir_state", sys.ptr, dptr(out))
+    call(_prefix(sys) + "_state", sys.ptr, dptr(out))
     return out
 
 
 def dust_system_set_state(sys, state):
     st = np.broadcast_to(f64(state), sys.shape).copy()
-    call("dust2_system_sir_set_state", sys.ptr, dptr(st))
+    call(_prefix(sys) + "_set_state", sys.ptr, dptr(st))
 
 
 def dust_system_set_state_initial(sys):
-    call("dust2_system_sir_set_state_initial", sys.ptr)
+    call(_prefix(sys) + "_set_state_initial", sys.ptr)
 
 
 def dust_system_time(sys):
     out = np.zeros(1)
-    call("dust2_system_sir_time", sys.ptr, dptr(out))
+    call(_prefix(sys) + "_time", sys.ptr, dptr(out))
     return float(out[0])
 
 
 def dust_system_set_time(sys, time):
-    call("dust2_system_sir_set_time", sys.ptr, float(time))
+    call(_prefix(sys) + "_set_time", sys.ptr, float(time))
 
 
 def dust_system_update_pars(sys, pars):
     rows = pars_update(sys.generator, pars, sys.rows)
-    call("dust2_system_sir_update_pars", sys.ptr, dptr(rows))
+    call(_prefix(sys) + "_update_pars", sys.ptr, dptr(rows))
     sys.rows = rows
 
 
@@ -655,14 +697,38 @@ def dust_system_compare_data(sys, data):
     """data: one value (shared) or one per group; returns (group, particle)."""
     d = f64(data).reshape(-1)
     out = np.zeros((sys.n_groups, sys.n_particles))
-    call("dust2_system_sir_compare_data", sys.ptr, dptr(d), d.size, dptr(out))
+    call(_prefix(sys) + "_compare_data", sys.ptr, dptr(d), d.size, dptr(out))
     return out
 
 
 def dust_system_reorder(sys, index):
     idx = np.ascontiguousarray(np.broadcast_to(index, (sys.n_groups, sys.n_particles)),
                                dtype=np.uint32)
-    call("dust2_system_sir_reorder", sys.ptr, u32ptr(idx))
+    call(_prefix(sys) + "_reorder", sys.ptr, u32ptr(idx))
+
+
+def dust_system_internals(sys):
+    """dust_system_internals for a continuous-time system: per particle
+    (group-major) n_steps, n_steps_accepted, n_steps_rejected, step_size
+    and, with debug_record_step_times, the step times."""
+    if not isinstance(sys, ContinuousSystem):
+        raise ValueError("Internals are only available for continuous-time systems")
+    n = sys.n_groups * sys.n_particles
+    stats = np.zeros((n, 4))
+    counts = np.zeros(n, dtype=np.uintp)
+    times = np.zeros(n * max(sys.record, 1))
+    call("dust2_system_sirode_internals", sys.ptr, dptr(stats),
+         dptr(times) if sys.record else None, szptr(counts))
+    out = []
+    for k in range(n):
+        row = {"group": k // sys.n_particles, "particle": k % sys.n_particles,
+               "n_steps": int(stats[k, 0]), "n_steps_accepted": int(stats[k, 1]),
+               "n_steps_rejected": int(stats[k, 2]), "step_size": stats[k, 3]}
+        if sys.record:
+            c = min(int(counts[k]), sys.record)
+            row["step_times"] = times[k * sys.record:k * sys.record + c].copy()
+        out.append(row)
+    return out
 
 
 def dust_system_rng_state(sys):
@@ -679,11 +745,34 @@ def dust_system_set_rng_state(sys, state):
 # ---------------------------------------------------------------------------
 # Unfilter (deterministic likelihood)
 # ---------------------------------------------------------------------------
+class OdeControl:
+    """dust_ode_control (R/interface-continuous.R:51-79): the five solver
+    settings (as an array: max_steps, atol, rtol, step_size_min,
+    step_size_max), the times the solver must stop at and whether systems
+    record their step times (history saving is not supported)."""
+
+    def __init__(self, values, critical_times=None, debug_record_step_times=False):
+        self.values = f64(values).reshape(-1)
+        self.critical_times = (np.zeros(0) if critical_times is None
+                               else np.ascontiguousarray(f64(critical_times).reshape(-1)))
+        if np.any(np.diff(self.critical_times) <= 0):
+            raise ValueError("'critical_times' must be strictly increasing")
+        self.debug_record_step_times = bool(debug_record_step_times)
+
+    def __array__(self, dtype=None, copy=None):
+        return self.values if dtype is None else self.values.astype(dtype)
+
+
 def dust_ode_control(max_steps=10000, atol=1e-6, rtol=1e-6, step_size_min=0,
-                     step_size_max=np.inf):
-    """R/interface-continuous.R:51-79 (critical times, history and step-time
-    recording are not supported on this path)."""
-    return np.array([max_steps, atol, rtol, step_size_min, step_size_max], dtype=np.float64)
+                     step_size_max=np.inf, critical_times=None, debug_record_step_times=False):
+    return OdeControl([max_steps, atol, rtol, step_size_min, step_size_max], critical_times,
+                      debug_record_step_times)
+
+
+def _ode_control(ctl):
+    if ctl is None:
+        return dust_ode_control()
+    return ctl if isinstance(ctl, OdeControl) else OdeControl(ctl)
 
 
 class Unfilter:
@@ -713,7 +802,8 @@ class Unfilter:
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
         self.dt = 1.0 if dt is None else float(dt)
-        self.control = dust_ode_control() if ode_control is None else f64(ode_control)
+        self.ode_control = _ode_control(ode_control)
+        self.control = self.ode_control.values
         self.precision = _precision(precision)
         self.ptr = None
         self.rows = None
@@ -735,6 +825,9 @@ class Unfilter:
                  self.n_particles, self.time_start, dptr(self.time), self.time.size,
                  dptr(self.data), self.n_groups_data, dptr(self.control), self.precision,
                  C.byref(out))
+            ct = self.ode_control.critical_times
+            if ct.size:
+                call("dust2_unfilter_sirode_set_critical_times", out, dptr(ct), ct.size)
         else:
             call("dust2_unfilter_sir_alloc", dptr(self.rows), self.n_groups, self.n_particles,
                  self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),

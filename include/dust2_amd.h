@@ -52,6 +52,7 @@ typedef struct dust2_filter dust2_filter;
 typedef struct dust2_system dust2_system;
 typedef struct dust2_unfilter dust2_unfilter;
 typedef struct dust2_unfilter_sir dust2_unfilter_sir;
+typedef struct dust2_ode_system dust2_ode_system;
 
 /* ---- library ------------------------------------------------------------ */
 
@@ -279,6 +280,51 @@ int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, int save_trajectories,
 int dust2_unfilter_sirode_last_gradient(dust2_unfilter* h, double* out, char* err,
                                         size_t errlen);
 void dust2_unfilter_sirode_free(dust2_unfilter* h);
+/* control.critical_times (dust2/continuous/control.hpp:26-39) for the ODE
+ * unfilter: the solver stops at each of these times (strictly increasing)
+ * as solver::run does (dust2/continuous/solver.hpp:243-259). */
+int dust2_unfilter_sirode_set_critical_times(dust2_unfilter* h, const double* times, size_t n,
+                                             char* err, size_t errlen);
+
+/* ---- continuous-time system (SIR ODE, FP64) --------------------------------
+ * Replaces dust2_continuous_alloc<sirode> (dust2/r/continuous/system.hpp)
+ * and the dust2_system_sirode_* methods (inst/template/system.cpp:1-54)
+ * over dust_continuous (dust2/continuous/system.hpp:32-330): state
+ * [G][P][5] as the other entry points; the solver internals of every
+ * particle stay on the device between calls and are re-initialised after
+ * set_state, set_state_initial, set_time and update_pars.  control: the
+ * five dust_ode_control() values or NULL; critical_times[n_critical];
+ * record_step_times: 0, or the number of step times kept per particle
+ * (control debug_record_step_times).  Errors lock the system as the
+ * reference does (dust2/r/system.hpp:9-16). */
+int dust2_system_sirode_alloc(const double* pars, size_t n_groups, size_t n_particles,
+                              double time, const double* control, const double* critical_times,
+                              size_t n_critical, size_t record_step_times,
+                              dust2_ode_system** out, char* err, size_t errlen);
+int dust2_system_sirode_run_to_time(dust2_ode_system* h, double time, char* err,
+                                    size_t errlen);
+int dust2_system_sirode_simulate(dust2_ode_system* h, const double* times, size_t n_times,
+                                 double* out, char* err, size_t errlen);
+int dust2_system_sirode_state(dust2_ode_system* h, double* out, char* err, size_t errlen);
+int dust2_system_sirode_set_state(dust2_ode_system* h, const double* state, char* err,
+                                  size_t errlen);
+int dust2_system_sirode_set_state_initial(dust2_ode_system* h, char* err, size_t errlen);
+int dust2_system_sirode_time(dust2_ode_system* h, double* out, char* err, size_t errlen);
+int dust2_system_sirode_set_time(dust2_ode_system* h, double time, char* err, size_t errlen);
+int dust2_system_sirode_update_pars(dust2_ode_system* h, const double* pars, char* err,
+                                    size_t errlen);
+int dust2_system_sirode_compare_data(dust2_ode_system* h, const double* data,
+                                     size_t n_groups_data, double* out, char* err,
+                                     size_t errlen);
+int dust2_system_sirode_reorder(dust2_ode_system* h, const uint32_t* index, char* err,
+                                size_t errlen);
+/* dust_system_internals (dust2/r/continuous/system.hpp, solver internals
+ * dust2/continuous/solver.hpp:34-73): stats[G*P][4] = n_steps,
+ * n_steps_accepted, n_steps_rejected, step_size; with recording,
+ * step_times[G*P][record_step_times] and their counts n_step_times[G*P]. */
+int dust2_system_sirode_internals(dust2_ode_system* h, double* stats, double* step_times,
+                                  size_t* n_step_times, char* err, size_t errlen);
+void dust2_system_sirode_free(dust2_ode_system* h);
 
 /* ---- deterministic discrete-time SIR unfilter with the adjoint ----------
  * Replaces dust2_discrete_unfilter_alloc<sir> (dust2/r/discrete/unfilter.hpp:12-58),

@@ -104,6 +104,60 @@ int ref_sirode_unfilter(const double* pars, size_t n_sets, double time_start,
   return 0;
 }
 
+// The SIRODE model of one particle through the reference solver as a
+// continuous-time system drives it (dust2/continuous/system.hpp:80-106,
+// 444-482, 531-542): initialise at time_start, then solver.run to each
+// target in turn, with control.critical_times and
+// debug_record_step_times.  state at each target: out_state[n][5]; stats
+// [4]; step times up to cap into step_times, their number in *n_steps.
+int ref_sirode_system(const double* pars, double time_start, const double* targets,
+                      size_t n_targets, const double* ctl_in, const double* critical,
+                      size_t n_critical, double* out_state, double* out_stats,
+                      double* step_times, size_t cap, size_t* n_steps, char* err,
+                      size_t errlen) {
+  using dust2::ode::internals;
+  using dust2::ode::solver;
+  const size_t n_var = 5;
+  dust2::ode::control<double> ctl(static_cast<size_t>(ctl_in[0]), ctl_in[1], ctl_in[2],
+                      std::max(ctl_in[3], std::numeric_limits<double>::epsilon()),
+                      ctl_in[4], std::vector<double>(critical, critical + n_critical), false,
+                      true);
+  const double N = pars[0], I0 = pars[1], beta = pars[2], gamma = pars[3];
+  auto rhs = [&](double, const double* y, double* dy) {
+    const double rate_SI = beta * y[0] * y[1] / N;
+    const double rate_IR = gamma * y[1];
+    dy[0] = -rate_SI;
+    dy[1] = rate_SI - rate_IR;
+    dy[2] = rate_IR;
+    dy[3] = rate_SI;
+    dy[4] = rate_SI;
+  };
+  solver<double> s(n_var, 0, ctl);
+  internals<double> in(n_var, 0, false);
+  dust2::zero_every_type<double> zero_every{{1, {4}}};
+  dust2::ode::events_type<double> events;
+  std::vector<double> y{N - I0, I0, 0, 0, 0};
+  double t = time_start;
+  try {
+    s.initialise(t, y.data(), in, rhs);
+    for (size_t i = 0; i < n_targets; ++i) {
+      s.run(t, targets[i], y.data(), zero_every, events, in, rhs);
+      t = targets[i];
+      std::copy(y.begin(), y.end(), out_state + i * n_var);
+    }
+  } catch (const std::exception& e) {
+    if (err && errlen) std::snprintf(err, errlen, "%s", e.what());
+    return 1;
+  }
+  out_stats[0] = in.n_steps;
+  out_stats[1] = in.n_steps_accepted;
+  out_stats[2] = in.n_steps_rejected;
+  out_stats[3] = in.step_size;
+  *n_steps = in.step_times.size();
+  for (size_t i = 0; i < std::min(cap, in.step_times.size()); ++i) step_times[i] = in.step_times[i];
+  return 0;
+}
+
 // A generic logistic ODE through the reference solver (closed form
 // y = K / (1 + (K/y0 - 1) exp(-r t)), tests/testthat/helper-dust.R:68-73),
 // two independent variables with rates r1, r2, no zero_every.

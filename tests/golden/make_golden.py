@@ -1,7 +1,8 @@
 """Writes the golden fixtures in this directory from the reference's own code.
 
 The parts of the reference engine that compile without monty/cpp11/R
-(dust2/filter_details.hpp, dust2/continuous/solver.hpp) are built in place
+(dust2/filter_details.hpp, dust2/continuous/solver.hpp,
+dust2/interpolate/interpolate.hpp) are built in place
 by oracle/Makefile into oracle/_ref/libdust2ref.so; this script calls them on
 fixed, seeded inputs and stores inputs and outputs as JSON.  Run it where the
 reference tree exists:
@@ -106,6 +107,33 @@ def logistic_fixture():
             "y": out.tolist()}
 
 
+def sirode_system_fixture():
+    """One SIRODE particle through the reference solver driven as a
+    continuous-time system: whole runs to several targets in turn, with and
+    without critical times and a maximum step size, with the recorded step
+    times."""
+    cases = []
+    specs = [
+        ("simulate_daily", [1000, 10, 0.2, 0.1], np.arange(1.0, 11.0), None, []),
+        ("step_size_max", [1000, 10, 0.2, 0.1], np.arange(0.0, 11.0),
+         [10000, 1e-6, 1e-6, 0, 0.8], []),
+        ("critical_times", [1000, 10, 0.2, 0.1], [50.0], None, [10.0, 20.0, 30.0]),
+        ("no_critical_times", [1000, 10, 0.2, 0.1], [50.0], None, []),
+        ("pieces", [5000, 20, 0.5, 0.1], [5.0, 12.0, 30.0, 31.5, 60.0], None, [25.0, 40.0]),
+        ("tight", [1000, 10, 0.3, 0.15], np.arange(2.0, 40.0, 3.0),
+         [10000, 1e-9, 1e-9, 0, np.inf], [7.5]),
+    ]
+    for name, pars, targets, control, critical in specs:
+        r = ol.ref_sirode_system(pars, 0.0, targets, control, critical)
+        cases.append({"name": name, "pars": list(map(float, pars)),
+                      "targets": list(map(float, targets)),
+                      "control": None if control is None else [float(c) for c in control],
+                      "critical_times": list(map(float, critical)),
+                      "state": r["state"].tolist(), "stats": r["stats"].tolist(),
+                      "step_times": r["step_times"].tolist()})
+    return {"cases": cases}
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref/libdust2ref.so is not built (needs the reference tree)")
@@ -117,6 +145,8 @@ def main():
         json.dump(sirode_weekly_fixture(), f)
     with open(os.path.join(HERE, "logistic_dp5.json"), "w") as f:
         json.dump(logistic_fixture(), f)
+    with open(os.path.join(HERE, "sirode_system_dp5.json"), "w") as f:
+        json.dump(sirode_system_fixture(), f)
 
 
 if __name__ == "__main__":

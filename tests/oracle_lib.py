@@ -100,6 +100,8 @@ def ref():
                                              D, D, C.c_char_p, C.c_size_t]
         _ref.ref_logistic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D,
                                       C.c_size_t, D, D]
+        _ref.ref_sirode_system.argtypes = [D, C.c_double, D, C.c_size_t, D, D, C.c_size_t, D,
+                                           D, D, C.c_size_t, SZ, C.c_char_p, C.c_size_t]
         _ref.ref_interpolate.argtypes = [C.c_int, D, C.c_size_t, D, C.c_size_t, D, C.c_size_t,
                                          D, C.c_char_p, C.c_size_t]
     return _ref
@@ -394,3 +396,26 @@ def sir_tv_rows(pars, kind, beta_t, beta_y):
     rows[:, 7:7 + n] = beta_t
     rows[:, 7 + 64:7 + 64 + n] = np.broadcast_to(beta_y, (G, n))
     return rows
+
+
+def ref_sirode_system(pars, time_start, targets, control=None, critical_times=()):
+    """One SIRODE particle through the reference solver driven as a
+    continuous-time system (oracle/_ref): states at the targets, the
+    solver counters and the recorded step times."""
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1)
+    ctl = np.array([10000, 1e-6, 1e-6, 0, np.inf] if control is None else control,
+                   dtype=np.float64)
+    crit = np.ascontiguousarray(critical_times, dtype=np.float64).reshape(-1)
+    state = np.zeros((targets.size, 5))
+    stats = np.zeros(4)
+    cap = 200000
+    times = np.zeros(cap)
+    n = np.zeros(1, dtype=np.uintp)
+    buf = C.create_string_buffer(512)
+    rc = ref().ref_sirode_system(_dp(pars), float(time_start), _dp(targets), targets.size,
+                                 _dp(ctl), _dp(crit) if crit.size else None, crit.size,
+                                 _dp(state), _dp(stats), _dp(times), cap,
+                                 n.ctypes.data_as(SZ), buf, 512)
+    _check(rc, buf)
+    return {"state": state, "stats": stats, "step_times": times[:int(n[0])].copy()}
