@@ -477,10 +477,21 @@ def dust_likelihood_copy(obj, seed=None):
     return dst
 
 
+def _filter_only(obj, what):
+    if not isinstance(obj, Filter):
+        raise ValueError(f"'{what}' is only available for particle filters")
+
+
 def dust_likelihood_last_state(obj, select_random_particle=False):
     """Final state (n_groups, n_particles, n_state); with
     select_random_particle, the chosen particle of each group of the last
-    run (n_groups, n_state)."""
+    run (n_groups, n_state).  For an unfilter every particle is the same
+    deterministic run, so the selected particle is the first."""
+    if isinstance(obj, Unfilter):
+        if obj.ptr is None:
+            raise DustError("System has not been run")
+        st = obj.last_state()
+        return st[:, 0, :] if select_random_particle else st
     if select_random_particle:
         n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
         out = np.zeros((n_groups, obj.generator.n_state))
@@ -495,7 +506,8 @@ def dust_likelihood_last_snapshots(obj, select_random_particle=False):
     """The whole state at the snapshot times of the last run, following each
     exported particle's ancestry: (n_snapshots, n_groups, n_particles or 1,
     n_state)."""
-    if obj.ptr is None or getattr(obj, "last_snapshots", None) is None:
+    if isinstance(obj, Unfilter) or obj.ptr is None or \
+            getattr(obj, "last_snapshots", None) is None:
         raise DustError("Snapshots are not current")
     n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
     p_out = 1 if select_random_particle else obj.n_particles
@@ -526,12 +538,14 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
 
 def dust_likelihood_last_trajectories_raw(obj):
     """The saved post-resampling states, not following the ancestry."""
+    _filter_only(obj, "last_trajectories_raw")
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
     call("dust2_filter_last_trajectories_raw", obj.ptr, dptr(out))
     return out
 
 
 def dust_likelihood_last_ancestry(obj):
+    _filter_only(obj, "last_ancestry")
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles), dtype=np.uint32)
     call("dust2_filter_last_ancestry", obj.ptr, u32ptr(out))
     return out
@@ -541,6 +555,7 @@ def dust_likelihood_rng_state(obj):
     """R/interface-likelihood.R:281-288: before the first run, the state the
     run will start from (the seed expanded to n_groups * (n_particles + 2)
     streams, dust2/r/discrete/filter.hpp:63-74)."""
+    _filter_only(obj, "rng_state")
     n = obj.n_groups * (obj.n_particles + 2)
     if obj.ptr is None:
         raw, n_raw, seed = _seed_args(obj.seed)
@@ -555,6 +570,7 @@ def dust_likelihood_rng_state(obj):
 def dust_likelihood_set_rng_state(obj, state):
     """R/interface-likelihood.R:297-306: before the first run the state
     replaces the seed."""
+    _filter_only(obj, "set_rng_state")
     st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
     if st.size != obj.n_groups * (obj.n_particles + 2) * 4:
         raise ValueError("Incorrect length for 'rng_state'")
@@ -565,6 +581,7 @@ def dust_likelihood_set_rng_state(obj, state):
 
 
 def dust_likelihood_last_stats(obj):
+    _filter_only(obj, "last_stats")
     out = np.zeros(7)
     call("dust2_filter_last_stats", obj.ptr, dptr(out))
     engines = {1: "persistent", 2: "pipeline"}
