@@ -355,7 +355,34 @@ void check_sir_pars(const double* row) {
 // ============================================================================
 // Filter
 // ============================================================================
+// Every handle begins with a tag naming its type, so that an entry point
+// handed another type of handle (through an untyped foreign-function
+// pointer) reports it instead of reading the wrong object.
+enum handle_tag : uint32_t {
+  TAG_FILTER = 0x64326631u,
+  TAG_SYSTEM = 0x64327331u,
+  TAG_UNFILTER = 0x64327531u,
+  TAG_UNFILTER_SIR = 0x64327532u,
+  TAG_ODE_SYSTEM = 0x64327332u,
+  TAG_SERVER = 0x64327631u,
+};
+
+template <typename H>
+bool handle_ok(const H* h, uint32_t tag) {
+  return h != nullptr && h->tag == tag;
+}
+
+template <typename H>
+void check_handle(const H* h, uint32_t tag, const char* what) {
+  if (h == nullptr) {
+    throw std::runtime_error(std::string("Pointer has been serialised, cannot continue safely (") +
+                             what + ")");
+  }
+  if (h->tag != tag) throw std::runtime_error(std::string("Not a ") + what + " handle");
+}
+
 struct dust2_filter {
+  uint32_t tag = TAG_FILTER;
   int model = 0;             // 0: sir, 1: seir16
   size_t S = 5;              // states per particle
   size_t n_pars = 5;         // host parameter row length
@@ -513,9 +540,7 @@ void filter_upload_data(dust2_filter* h, const double* data) {
   to_device(h->d_lfact, lfact.data(), n * sizeof(double), h->stream);
 }
 
-void check_filter(dust2_filter* h) {
-  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (filter)");
-}
+void check_filter(dust2_filter* h) { check_handle(h, TAG_FILTER, "filter"); }
 
 // Step schedule of the data intervals, as dust_discrete::run_to_time
 // computes it (dust2/discrete/system.hpp:61-92): n_steps =
@@ -1498,7 +1523,9 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
   }
 }
 
-void dust2_filter_free(dust2_filter* h) { delete h; }
+void dust2_filter_free(dust2_filter* h) {
+  if (handle_ok(h, TAG_FILTER)) delete h;
+}
 
 }  // extern "C"
 
@@ -1506,6 +1533,7 @@ void dust2_filter_free(dust2_filter* h) { delete h; }
 // System
 // ============================================================================
 struct dust2_system {
+  uint32_t tag = TAG_SYSTEM;
   bool fp64 = true;
   bool deterministic = false;
   size_t G = 0, P = 0;
@@ -1523,7 +1551,7 @@ struct dust2_system {
 namespace {
 
 void check_system(dust2_system* h, const char* action) {
-  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (system)");
+  check_handle(h, TAG_SYSTEM, "system");
   if (h->errors_pending) {
     throw std::runtime_error(std::string("Can't currently ") + action +
                              " this system: errors are pending");
@@ -1665,7 +1693,7 @@ int dust2_system_sir_state(dust2_system* h, double* out, char* err, size_t errle
 
 int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err, size_t errlen) {
   try {
-    if (h == nullptr) check_system(h, "set state for");
+    check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     std::vector<char> tmp(h->d_state.bytes());
     aos_to_soa(state, tmp.data(), h->G, h->P, kSirState, h->fp64);
     to_device(h->d_state, tmp.data(), tmp.size(), h->stream);
@@ -1679,7 +1707,7 @@ int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err, 
 
 int dust2_system_sir_set_state_initial(dust2_system* h, char* err, size_t errlen) {
   try {
-    if (h == nullptr) check_system(h, "set state for");
+    check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     h->errors_pending = false;
     // a zero-step run from the initial state writes initial(); the model's
     // initial() draws no random numbers (sir.cpp:38-48)
@@ -1712,7 +1740,7 @@ int dust2_system_sir_set_time(dust2_system* h, double time, char* err, size_t er
 
 int dust2_system_sir_update_pars(dust2_system* h, const double* pars, char* err, size_t errlen) {
   try {
-    if (h == nullptr) check_system(h, "update pars for");
+    check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     for (size_t g = 0; g < h->G; ++g) {
       const double* row = pars + g * kSirPars;
       check_sir_pars(row);
@@ -1783,7 +1811,7 @@ int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err, 
 
 int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen) {
   try {
-    if (h == nullptr) check_system(h, "get rng state from");
+    check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     const size_t n = h->G * h->P;
     dbuf tmp(n * 32);
     hip_check(launch_rng_soa_to_aos(h->d_rng.get<uint64_t>(), tmp.get<uint64_t>(),
@@ -1800,7 +1828,7 @@ int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t
 int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
                                    size_t errlen) {
   try {
-    if (h == nullptr) check_system(h, "set rng state for");
+    check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     const size_t n = h->G * h->P;
     dbuf tmp(n * 32);
     to_device(tmp, state, n * 32, h->stream);
@@ -1815,7 +1843,9 @@ int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char*
   }
 }
 
-void dust2_system_sir_free(dust2_system* h) { delete h; }
+void dust2_system_sir_free(dust2_system* h) {
+  if (handle_ok(h, TAG_SYSTEM)) delete h;
+}
 
 // ============================================================================
 // Test entry points
@@ -1990,6 +2020,7 @@ int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams, s
 // Unfilter (deterministic likelihood, SIR ODE, Dormand-Prince 5(4))
 // ============================================================================
 struct dust2_unfilter {
+  uint32_t tag = TAG_UNFILTER;
   bool fp64 = true;
   size_t G = 0, P = 0, n_time = 0, n_groups_data = 0;
   double time_start = 0;
@@ -2063,9 +2094,7 @@ void check_sirode_pars(const double* row) {
   }
 }
 
-void check_unfilter(dust2_unfilter* h) {
-  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (unfilter)");
-}
+void check_unfilter(dust2_unfilter* h) { check_handle(h, TAG_UNFILTER, "unfilter"); }
 }  // namespace
 
 extern "C" {
@@ -2342,7 +2371,9 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
   }
 }
 
-void dust2_unfilter_sirode_free(dust2_unfilter* h) { delete h; }
+void dust2_unfilter_sirode_free(dust2_unfilter* h) {
+  if (handle_ok(h, TAG_UNFILTER)) delete h;
+}
 
 int dust2_unfilter_sirode_set_critical_times(dust2_unfilter* h, const double* times, size_t n,
                                              char* err, size_t errlen) {
@@ -2361,6 +2392,7 @@ int dust2_unfilter_sirode_set_critical_times(dust2_unfilter* h, const double* ti
 // Continuous-time system: the SIR ODE (dust2/continuous/system.hpp)
 // ============================================================================
 struct dust2_ode_system {
+  uint32_t tag = TAG_ODE_SYSTEM;
   size_t G = 0, P = 0;
   double time = 0;
   std::vector<double> pars;           // G x 4
@@ -2379,9 +2411,7 @@ struct dust2_ode_system {
 namespace {
 
 void check_ode_system(dust2_ode_system* h, const char* action) {
-  if (h == nullptr) {
-    throw std::runtime_error("Pointer has been serialised, cannot continue safely (system)");
-  }
+  check_handle(h, TAG_ODE_SYSTEM, "system");
   if (h->errors_pending) {
     throw std::runtime_error(std::string("Can't currently ") + action +
                              " this system: errors are pending");
@@ -2532,7 +2562,7 @@ int dust2_system_sirode_state(dust2_ode_system* h, double* out, char* err, size_
 int dust2_system_sirode_set_state(dust2_ode_system* h, const double* state, char* err,
                                   size_t errlen) {
   try {
-    if (h == nullptr) check_ode_system(h, "set state for");
+    check_handle(h, TAG_ODE_SYSTEM, "system");  // allowed while errors are pending
     std::vector<double> tmp(kSirState * h->G * h->P);
     aos_to_soa(state, tmp.data(), 1, h->G * h->P, kSirState, true);
     to_device(h->d_state, tmp.data(), tmp.size() * sizeof(double), h->stream);
@@ -2547,7 +2577,7 @@ int dust2_system_sirode_set_state(dust2_ode_system* h, const double* state, char
 
 int dust2_system_sirode_set_state_initial(dust2_ode_system* h, char* err, size_t errlen) {
   try {
-    if (h == nullptr) check_ode_system(h, "set state for");
+    check_handle(h, TAG_ODE_SYSTEM, "system");  // allowed while errors are pending
     // sirode.cpp:35-45: S = N - I0, I = I0, the rest 0
     std::vector<double> st(h->G * h->P * kSirState, 0.0);
     for (size_t g = 0; g < h->G; ++g) {
@@ -2588,7 +2618,7 @@ int dust2_system_sirode_set_time(dust2_ode_system* h, double time, char* err, si
 int dust2_system_sirode_update_pars(dust2_ode_system* h, const double* pars, char* err,
                                     size_t errlen) {
   try {
-    if (h == nullptr) check_ode_system(h, "update pars for");
+    check_handle(h, TAG_ODE_SYSTEM, "system");  // allowed while errors are pending
     for (size_t g = 0; g < h->G; ++g) check_sirode_pars(pars + g * kSirodePars);
     h->pars.assign(pars, pars + h->G * kSirodePars);
     to_device(h->d_pars, pars, h->pars.size() * sizeof(double), h->stream);
@@ -2693,7 +2723,9 @@ int dust2_system_sirode_internals(dust2_ode_system* h, double* stats, double* st
   }
 }
 
-void dust2_system_sirode_free(dust2_ode_system* h) { delete h; }
+void dust2_system_sirode_free(dust2_ode_system* h) {
+  if (handle_ok(h, TAG_ODE_SYSTEM)) delete h;
+}
 
 }  // extern "C"
 
@@ -2701,6 +2733,7 @@ void dust2_system_sirode_free(dust2_ode_system* h) { delete h; }
 // Deterministic discrete-time SIR unfilter with the adjoint gradient
 // ============================================================================
 struct dust2_unfilter_sir {
+  uint32_t tag = TAG_UNFILTER_SIR;
   size_t G = 0, P = 0, n_time = 0, n_groups_data = 0, total_steps = 0;
   double time_start = 0, dt = 1;
   std::vector<double> pars;           // G x 5 (host rows)
@@ -2722,7 +2755,7 @@ struct dust2_unfilter_sir {
 
 namespace {
 void check_unfilter_sir(dust2_unfilter_sir* h) {
-  if (h == nullptr) throw std::runtime_error("Pointer has been serialised, cannot continue safely (unfilter)");
+  check_handle(h, TAG_UNFILTER_SIR, "unfilter");
 }
 
 // N, I0, beta, gamma, exp_noise -> N, I0, beta, gamma, exp_noise,
@@ -2991,7 +3024,9 @@ int dust2_unfilter_sir_last_state(dust2_unfilter_sir* h, double* out, char* err,
   }
 }
 
-void dust2_unfilter_sir_free(dust2_unfilter_sir* h) { delete h; }
+void dust2_unfilter_sir_free(dust2_unfilter_sir* h) {
+  if (handle_ok(h, TAG_UNFILTER_SIR)) delete h;
+}
 
 }  // extern "C"
 
@@ -3115,6 +3150,7 @@ struct srv_slot {
 }  // namespace
 
 struct dust2_server {
+  uint32_t tag = TAG_SERVER;
   static constexpr int kStreams = 4;
   hipStream_t streams[kStreams] = {};
   arena mem;
@@ -3329,7 +3365,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
                         const double* data, uint64_t seed, uint64_t* id_out, char* err,
                         size_t errlen) {
   try {
-    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    check_handle(sv, TAG_SERVER, "server");
     if (model < 0 || model > 2) throw std::runtime_error("unknown model");
     check_time_sequence(time_start, time, n_time);
     srv_request rq;
@@ -3374,7 +3410,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
 int dust2_server_step(dust2_server* sv, size_t* n_active, size_t* n_queued, char* err,
                       size_t errlen) {
   try {
-    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    check_handle(sv, TAG_SERVER, "server");
     hipStream_t s0 = sv->streams[0];
     while (!sv->queue.empty()) {
       if (!srv_admit(sv, sv->queue.front(), s0)) {
@@ -3477,7 +3513,7 @@ int dust2_server_step(dust2_server* sv, size_t* n_active, size_t* n_queued, char
 int dust2_server_poll(dust2_server* sv, uint64_t* ids, double* ll, int* status, size_t max,
                       size_t* n_out, char* err, size_t errlen) {
   try {
-    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    check_handle(sv, TAG_SERVER, "server");
     const size_t n = std::min(max, sv->results.size());
     for (size_t i = 0; i < n; ++i) {
       ids[i] = sv->results[i].id;
@@ -3504,7 +3540,7 @@ int dust2_server_poll(dust2_server* sv, uint64_t* ids, double* ll, int* status, 
 int dust2_server_memory(dust2_server* sv, size_t* used, size_t* capacity, char* err,
                         size_t errlen) {
   try {
-    if (sv == nullptr) throw std::runtime_error("server is not initialised");
+    check_handle(sv, TAG_SERVER, "server");
     *used = sv->mem.used();
     *capacity = sv->mem.capacity();
     return 0;
@@ -3513,6 +3549,8 @@ int dust2_server_memory(dust2_server* sv, size_t* used, size_t* capacity, char* 
   }
 }
 
-void dust2_server_free(dust2_server* sv) { delete sv; }
+void dust2_server_free(dust2_server* sv) {
+  if (handle_ok(sv, TAG_SERVER)) delete sv;
+}
 
 }  // extern "C"

@@ -113,6 +113,10 @@ def test_trajectories_subsets_and_final_state(gpu_lib):
             fn(obj)
     with pytest.raises(gpu_lib.DustError, match="Snapshots are not current"):
         gpu_lib.dust_likelihood_last_snapshots(obj)
+    # the C ABI refuses a handle of another type
+    from dust2_amd._lib import call, dptr
+    with pytest.raises(gpu_lib.DustError, match="Not a filter handle"):
+        call("dust2_filter_last_state", obj.ptr, dptr(np.zeros(5)))
     # index_state: trajectories keep the chosen states, the final state all
     gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True, index_state=[1, 3])
     np.testing.assert_array_equal(gpu_lib.dust_likelihood_last_trajectories(obj),
