@@ -116,6 +116,23 @@ _PROTOS = {
     "dust2_system_sirode_reorder": (C.c_int, [VP, U32, ERR, ERRLEN]),
     "dust2_system_sirode_internals": (C.c_int, [VP, D, D, SZ, ERR, ERRLEN]),
     "dust2_system_sirode_free": (None, [VP]),
+    "dust2_system_ode_model": (C.c_int, [C.c_char_p, SZ, ERR, ERRLEN]),
+    "dust2_system_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double, D, D,
+                                         C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                         C.c_size_t, C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_system_ode_run_to_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
+    "dust2_system_ode_simulate": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_system_ode_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_ode_set_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_ode_set_state_initial": (C.c_int, [VP, ERR, ERRLEN]),
+    "dust2_system_ode_time": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_ode_set_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
+    "dust2_system_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_ode_reorder": (C.c_int, [VP, U32, ERR, ERRLEN]),
+    "dust2_system_ode_internals": (C.c_int, [VP, D, D, SZ, ERR, ERRLEN]),
+    "dust2_system_ode_events": (C.c_int, [VP, SZ, D, ERR, ERRLEN]),
+    "dust2_system_ode_history": (C.c_int, [VP, SZ, D, ERR, ERRLEN]),
+    "dust2_system_ode_free": (None, [VP]),
     "dust2_amd_interpolate": (C.c_int, [C.c_int, D, C.c_size_t, D, C.c_size_t, C.c_size_t, D,
                                         C.c_size_t, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,

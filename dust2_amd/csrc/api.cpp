@@ -23,6 +23,8 @@
 
 #include "engine.hpp"
 #include "interpolate.hpp"
+#include "ode_general.hpp"
+#include "ode_models.hpp"
 
 using namespace dust2_amd;
 
@@ -286,6 +288,13 @@ std::string error_text(const error_record& r) {
     case ERR_ODE_STEP_SMALL: return "step too small";
     case ERR_ODE_VANISHED: return "step size vanished";
     case ERR_ODE_INIT: return "Initial step size was not finite";
+    case ERR_ODE_HISTORY_LOCATE: return "Failed to locate history element";
+    case ERR_ODE_HISTORY_SHORT: {
+      std::ostringstream msg;
+      msg << "History too short for a delayed value at time " << r.a
+          << ": raise 'history_cap' (steps kept per particle)";
+      return msg.str();
+    }
     case ERR_INTERP_BEFORE:
     case ERR_INTERP_AFTER: {
       // dust2/interpolate/interpolate.hpp:20-40; a = time, b = first or last knot
@@ -364,6 +373,7 @@ enum handle_tag : uint32_t {
   TAG_UNFILTER = 0x64327531u,
   TAG_UNFILTER_SIR = 0x64327532u,
   TAG_ODE_SYSTEM = 0x64327332u,
+  TAG_ODE_GENERAL = 0x64327333u,
   TAG_SERVER = 0x64327631u,
 };
 
@@ -2725,6 +2735,475 @@ int dust2_system_sirode_internals(dust2_ode_system* h, double* stats, double* st
 
 void dust2_system_sirode_free(dust2_ode_system* h) {
   if (handle_ok(h, TAG_ODE_SYSTEM)) delete h;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// General continuous-time systems: events, carried variables, delays
+// (dust2/continuous/system.hpp over the models of ode_models.hpp)
+// ============================================================================
+struct dust2_ode_general {
+  uint32_t tag = TAG_ODE_GENERAL;
+  int model = 0;
+  odem::model_info info{};
+  size_t G = 0, P = 0, S = 0;          // S = n_var + n_special + n_output
+  int words = 0, hist_words = 0;
+  double time = 0;
+  std::vector<double> pars;            // G x n_pars
+  std::vector<uint8_t> reinit;
+  ode_control_dev control{};
+  uint32_t record_cap = 0, event_cap = 0, history_cap = 0;
+  bool save_history = false;
+  hipStream_t stream = nullptr;
+  dbuf d_pars, d_state, d_internals, d_reinit, d_targets, d_out, d_errors, d_counters,
+      d_critical, d_step_times, d_step_count, d_events, d_history;
+  bool errors_pending = false;
+  ~dust2_ode_general() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+const char* const kGeneralModels[odem::MODEL_COUNT] = {"bounce",       "change",   "eventzero",
+                                                       "specialevent", "sirdelay", "logdelay"};
+
+int general_model_id(const char* name) {
+  for (int i = 0; i < odem::MODEL_COUNT; ++i) {
+    if (name && std::strcmp(name, kGeneralModels[i]) == 0) return i;
+  }
+  throw std::runtime_error(std::string("Unknown continuous-time model '") + (name ? name : "") +
+                           "'");
+}
+
+void check_general_pars(int model, const double* row) {
+  if (model == odem::MODEL_CHANGE) {
+    const double n = row[1];
+    if (!(n >= 0 && n <= odem::change::max_changes && std::floor(n) == n)) {
+      throw std::runtime_error("'n' must be an integer between 0 and " +
+                               std::to_string(odem::change::max_changes));
+    }
+  }
+  if (model == odem::MODEL_LOGDELAY && !(row[2] > 0)) {
+    throw std::runtime_error("'tau' must be positive");
+  }
+}
+
+void check_general(dust2_ode_general* h, const char* action) {
+  check_handle(h, TAG_ODE_GENERAL, "system");
+  if (h->errors_pending) {
+    throw std::runtime_error(std::string("Can't currently ") + action +
+                             " this system: errors are pending");
+  }
+}
+
+void general_run(dust2_ode_general* h, const double* targets, size_t n, double* out) {
+  hipStream_t s = h->stream;
+  const size_t n_total = h->G * h->P;
+  h->d_targets.alloc(std::max<size_t>(n, 1) * sizeof(double));
+  if (n) to_device(h->d_targets, targets, n * sizeof(double), s);
+  to_device(h->d_reinit, h->reinit.data(), h->G, s);
+  if (out) h->d_out.alloc(std::max<size_t>(n, 1) * h->S * n_total * sizeof(double));
+  hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
+  gen_launch a{};
+  a.model = h->model;
+  a.pars = h->d_pars.get<double>();
+  a.n_pars = h->info.n_pars;
+  a.n_particles = static_cast<int>(h->P);
+  a.n_total = static_cast<int>(n_total);
+  a.reinit = h->d_reinit.get<uint8_t>();
+  a.time = h->time;
+  a.targets = h->d_targets.get<double>();
+  a.n_targets = static_cast<int>(n);
+  a.control = h->control;
+  a.state = h->d_state.get<double>();
+  a.internals = h->d_internals.get<double>();
+  a.words = h->words;
+  a.out = out ? h->d_out.get<double>() : nullptr;
+  a.record = ode_record{h->record_cap ? h->d_step_times.get<double>() : nullptr,
+                        h->d_step_count.get<uint32_t>(), h->record_cap};
+  a.events = h->event_cap ? h->d_events.get<double>() : nullptr;
+  a.event_cap = h->event_cap;
+  a.history = h->history_cap ? h->d_history.get<double>() : nullptr;
+  a.history_cap = h->history_cap;
+  a.save_history = h->save_history ? 1 : 0;
+  a.errors = h->d_errors.get<error_record>();
+  a.error_count = h->d_counters.get<uint32_t>();
+  hip_check(launch_general_system(a, s), "general ODE system kernel");
+  uint32_t count = 0;
+  to_host(&count, h->d_counters, sizeof(count), s);
+  std::fill(h->reinit.begin(), h->reinit.end(), 0);
+  if (count != 0) {
+    std::vector<error_record> rec(n_total);
+    to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
+    hip_check(hipStreamSynchronize(s), "sync");
+    h->errors_pending = true;
+    throw std::runtime_error(error_report(rec, count));
+  }
+  if (n) h->time = targets[n - 1];
+  if (out) {
+    const size_t block = h->S * n_total;
+    std::vector<double> tmp(n * block);
+    to_host(tmp.data(), h->d_out, tmp.size() * sizeof(double), s);
+    for (size_t q = 0; q < n; ++q) {
+      soa_to_aos(tmp.data() + q * block, out + q * block, 1, n_total, h->S, true);
+    }
+  }
+}
+
+// Per-particle rows of a device array of n_total rows of `row` bytes,
+// permuted within groups: row to <- row (group start + index[to]).
+void permute_rows(dbuf& d, size_t G, size_t P, size_t row, const uint32_t* index, hipStream_t s) {
+  if (row == 0 || d.get() == nullptr) return;
+  const size_t n_total = G * P;
+  std::vector<unsigned char> a(n_total * row), b(n_total * row);
+  to_host(a.data(), d, a.size(), s);
+  for (size_t g = 0; g < G; ++g) {
+    for (size_t p = 0; p < P; ++p) {
+      const size_t to = g * P + p, from = g * P + index[to];
+      std::memcpy(b.data() + to * row, a.data() + from * row, row);
+    }
+  }
+  to_device(d, b.data(), b.size(), s);
+  hip_check(hipStreamSynchronize(s), "sync");
+}
+
+}  // namespace
+
+extern "C" {
+
+int dust2_system_ode_model(const char* model, size_t* sizes, char* err, size_t errlen) {
+  try {
+    const int id = general_model_id(model);
+    const odem::model_info m = odem::info(id);
+    sizes[0] = static_cast<size_t>(m.n_var);
+    sizes[1] = static_cast<size_t>(m.n_special);
+    sizes[2] = static_cast<size_t>(m.n_output);
+    sizes[3] = static_cast<size_t>(m.n_pars);
+    sizes[4] = static_cast<size_t>(m.max_events);
+    sizes[5] = static_cast<size_t>(m.has_delays ? m.n_hist : m.n_var + m.n_special);
+    sizes[6] = m.has_delays ? 1u : 0u;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_alloc(const char* model, const double* pars, size_t n_groups,
+                           size_t n_particles, double time, const double* control,
+                           const double* critical_times, size_t n_critical,
+                           size_t record_step_times, size_t save_history, size_t history_cap,
+                           size_t event_cap, dust2_ode_general** out, char* err,
+                           size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    auto h = std::make_unique<dust2_ode_general>();
+    h->model = general_model_id(model);
+    h->info = odem::info(h->model);
+    const size_t np = static_cast<size_t>(h->info.n_pars);
+    for (size_t g = 0; g < n_groups; ++g) check_general_pars(h->model, pars + g * np);
+    h->G = n_groups;
+    h->P = n_particles;
+    h->S = static_cast<size_t>(h->info.n_var + h->info.n_special + h->info.n_output);
+    h->words = gen_words(h->model);
+    h->hist_words = gen_history_words(h->model);
+    h->time = time;
+    h->pars.assign(pars, pars + n_groups * np);
+    h->reinit.assign(n_groups, 1);
+    h->control = ode_control_from(control);
+    h->record_cap = static_cast<uint32_t>(std::min<size_t>(record_step_times, 1u << 20));
+    h->event_cap =
+        h->info.max_events > 0 ? static_cast<uint32_t>(std::min<size_t>(event_cap, 1u << 20)) : 0;
+    // a model with delays always keeps its history (system.hpp:414-440)
+    h->save_history = save_history != 0 || h->info.has_delays;
+    h->history_cap =
+        h->save_history ? static_cast<uint32_t>(std::min<size_t>(std::max<size_t>(history_cap, 1),
+                                                                 1u << 20))
+                        : 0;
+    const size_t n_total = n_groups * n_particles;
+    if (static_cast<double>(n_total) * h->history_cap * h->hist_words * 8 > 64e9) {
+      throw std::runtime_error("The history ring would exceed 64 GB: lower 'history_cap'");
+    }
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    h->d_pars.alloc(std::max<size_t>(h->pars.size(), 1) * sizeof(double));
+    to_device(h->d_pars, pars, h->pars.size() * sizeof(double), h->stream);
+    h->d_state.alloc(h->S * n_total * sizeof(double));
+    h->d_internals.alloc(n_total * h->words * sizeof(double));
+    h->d_reinit.alloc(n_groups);
+    h->d_errors.alloc(n_total * sizeof(error_record));
+    h->d_counters.alloc(4 * sizeof(uint32_t));
+    h->d_step_count.alloc(n_total * sizeof(uint32_t));
+    if (h->record_cap) h->d_step_times.alloc(n_total * h->record_cap * sizeof(double));
+    if (h->event_cap) h->d_events.alloc(n_total * h->event_cap * 3 * sizeof(double));
+    if (h->history_cap) {
+      h->d_history.alloc(n_total * h->history_cap * h->hist_words * sizeof(double));
+    }
+    hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_internals.get(), 0, h->d_internals.bytes(), h->stream),
+              "memset");
+    hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
+    hip_check(hipMemsetAsync(h->d_step_count.get(), 0, h->d_step_count.bytes(), h->stream),
+              "memset");
+    set_critical_times(h->control, h->d_critical, critical_times, n_critical, h->stream);
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_run_to_time(dust2_ode_general* h, double time, char* err, size_t errlen) {
+  try {
+    check_general(h, "run");
+    if (time < h->time) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf, "Can't run to time %f, system already at time %f", time,
+                    h->time);
+      throw std::runtime_error(buf);
+    }
+    general_run(h, &time, 1, nullptr);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_simulate(dust2_ode_general* h, const double* times, size_t n_times,
+                              double* out, char* err, size_t errlen) {
+  try {
+    check_general(h, "simulate");
+    double prev = h->time;
+    for (size_t i = 0; i < n_times; ++i) {
+      if (times[i] < prev) {
+        throw std::runtime_error("Expected 'times' to be increasing, starting at the "
+                                 "system's time");
+      }
+      prev = times[i];
+    }
+    general_run(h, times, n_times, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_state(dust2_ode_general* h, double* out, char* err, size_t errlen) {
+  try {
+    check_general(h, "get state from");
+    std::vector<double> tmp(h->S * h->G * h->P);
+    to_host(tmp.data(), h->d_state, tmp.size() * sizeof(double), h->stream);
+    soa_to_aos(tmp.data(), out, 1, h->G * h->P, h->S, true);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_set_state(dust2_ode_general* h, const double* state, char* err,
+                               size_t errlen) {
+  try {
+    check_handle(h, TAG_ODE_GENERAL, "system");  // allowed while errors are pending
+    std::vector<double> tmp(h->S * h->G * h->P);
+    aos_to_soa(state, tmp.data(), 1, h->G * h->P, h->S, true);
+    to_device(h->d_state, tmp.data(), tmp.size() * sizeof(double), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    std::fill(h->reinit.begin(), h->reinit.end(), 1);
+    h->errors_pending = false;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_set_state_initial(dust2_ode_general* h, char* err, size_t errlen) {
+  try {
+    check_handle(h, TAG_ODE_GENERAL, "system");  // allowed while errors are pending
+    std::vector<double> st(h->G * h->P * h->S, 0.0);
+    const size_t np = static_cast<size_t>(h->info.n_pars);
+    for (size_t g = 0; g < h->G; ++g) {
+      const double* p = h->pars.data() + g * np;
+      for (size_t q = 0; q < h->P; ++q) {
+        double* y = st.data() + (g * h->P + q) * h->S;
+        switch (h->model) {
+          case odem::MODEL_BOUNCE: odem::bounce::initial(h->time, p, y); break;
+          case odem::MODEL_CHANGE: odem::change::initial(h->time, p, y); break;
+          case odem::MODEL_EVENTZERO: odem::eventzero::initial(h->time, p, y); break;
+          case odem::MODEL_SPECIALEVENT: odem::specialevent::initial(h->time, p, y); break;
+          case odem::MODEL_SIRDELAY: odem::sirdelay::initial(h->time, p, y); break;
+          case odem::MODEL_LOGDELAY: odem::logdelay::initial(h->time, p, y); break;
+          default: break;
+        }
+      }
+    }
+    h->errors_pending = false;
+    return dust2_system_ode_set_state(h, st.data(), err, errlen);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_time(dust2_ode_general* h, double* out, char* err, size_t errlen) {
+  try {
+    check_general(h, "get time from");
+    *out = h->time;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_set_time(dust2_ode_general* h, double time, char* err, size_t errlen) {
+  try {
+    check_general(h, "set time for");
+    h->time = time;
+    std::fill(h->reinit.begin(), h->reinit.end(), 1);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_update_pars(dust2_ode_general* h, const double* pars, char* err,
+                                 size_t errlen) {
+  try {
+    check_handle(h, TAG_ODE_GENERAL, "system");  // allowed while errors are pending
+    const size_t np = static_cast<size_t>(h->info.n_pars);
+    for (size_t g = 0; g < h->G; ++g) check_general_pars(h->model, pars + g * np);
+    h->pars.assign(pars, pars + h->G * np);
+    to_device(h->d_pars, pars, h->pars.size() * sizeof(double), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    // update_shared marks the group's solver for initialisation
+    // (dust2/continuous/system.hpp:256-265)
+    std::fill(h->reinit.begin(), h->reinit.end(), 1);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_reorder(dust2_ode_general* h, const uint32_t* index, char* err,
+                             size_t errlen) {
+  try {
+    check_general(h, "reorder");
+    const size_t n_total = h->G * h->P;
+    for (size_t k = 0; k < n_total; ++k) {
+      if (index[k] >= h->P) throw std::runtime_error("reorder index out of range");
+    }
+    // state and every piece of solver internals move together
+    // (dust2/continuous/system.hpp:172-199)
+    std::vector<double> st(h->S * n_total), st2(h->S * n_total);
+    to_host(st.data(), h->d_state, st.size() * sizeof(double), h->stream);
+    for (size_t g = 0; g < h->G; ++g) {
+      for (size_t p = 0; p < h->P; ++p) {
+        const size_t to = g * h->P + p, from = g * h->P + index[to];
+        for (size_t c = 0; c < h->S; ++c) st2[c * n_total + to] = st[c * n_total + from];
+      }
+    }
+    to_device(h->d_state, st2.data(), st2.size() * sizeof(double), h->stream);
+    permute_rows(h->d_internals, h->G, h->P, h->words * sizeof(double), index, h->stream);
+    permute_rows(h->d_step_count, h->G, h->P, sizeof(uint32_t), index, h->stream);
+    permute_rows(h->d_step_times, h->G, h->P, h->record_cap * sizeof(double), index, h->stream);
+    permute_rows(h->d_events, h->G, h->P, h->event_cap * 3 * sizeof(double), index, h->stream);
+    permute_rows(h->d_history, h->G, h->P,
+                 static_cast<size_t>(h->history_cap) * h->hist_words * sizeof(double), index,
+                 h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_internals(dust2_ode_general* h, double* stats, double* step_times,
+                               size_t* n_step_times, char* err, size_t errlen) {
+  try {
+    check_general(h, "get internals from");
+    const size_t n_total = h->G * h->P;
+    const odem::model_info& m = h->info;
+    const int ny = m.n_var + m.n_special;
+    const gen_layout L = make_gen_layout(m.n_var, ny, m.max_events > 0 ? m.max_events : 1,
+                                         m.has_delays ? m.n_hist : ny);
+    std::vector<double> w(n_total * h->words);
+    to_host(w.data(), h->d_internals, w.size() * sizeof(double), h->stream);
+    for (size_t k = 0; k < n_total; ++k) {
+      const double* r = w.data() + k * h->words;
+      stats[4 * k + 0] = r[L.n_steps];
+      stats[4 * k + 1] = r[L.n_accepted];
+      stats[4 * k + 2] = r[L.n_rejected];
+      stats[4 * k + 3] = r[L.step_size];
+    }
+    if (n_step_times) {
+      std::vector<uint32_t> cnt(n_total);
+      to_host(cnt.data(), h->d_step_count, cnt.size() * sizeof(uint32_t), h->stream);
+      for (size_t k = 0; k < n_total; ++k) n_step_times[k] = h->record_cap ? cnt[k] : 0;
+      if (step_times && h->record_cap) {
+        to_host(step_times, h->d_step_times, n_total * h->record_cap * sizeof(double),
+                h->stream);
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_events(dust2_ode_general* h, size_t* counts, double* records, char* err,
+                            size_t errlen) {
+  try {
+    check_general(h, "get internals from");
+    const size_t n_total = h->G * h->P;
+    const odem::model_info& m = h->info;
+    const int ny = m.n_var + m.n_special;
+    const gen_layout L = make_gen_layout(m.n_var, ny, m.max_events > 0 ? m.max_events : 1,
+                                         m.has_delays ? m.n_hist : ny);
+    std::vector<double> w(n_total * h->words);
+    to_host(w.data(), h->d_internals, w.size() * sizeof(double), h->stream);
+    for (size_t k = 0; k < n_total; ++k) {
+      counts[k] = static_cast<size_t>(w[k * h->words + L.event_count]);
+    }
+    if (records && h->event_cap) {
+      to_host(records, h->d_events, n_total * h->event_cap * 3 * sizeof(double), h->stream);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_history(dust2_ode_general* h, size_t* counts, double* entries, char* err,
+                             size_t errlen) {
+  try {
+    check_general(h, "get internals from");
+    const size_t n_total = h->G * h->P;
+    const odem::model_info& m = h->info;
+    const int ny = m.n_var + m.n_special;
+    const gen_layout L = make_gen_layout(m.n_var, ny, m.max_events > 0 ? m.max_events : 1,
+                                         m.has_delays ? m.n_hist : ny);
+    std::vector<double> w(n_total * h->words);
+    to_host(w.data(), h->d_internals, w.size() * sizeof(double), h->stream);
+    const size_t cap = h->history_cap, hw = static_cast<size_t>(h->hist_words);
+    std::vector<double> ring(cap ? n_total * cap * hw : 0);
+    if (cap) to_host(ring.data(), h->d_history, ring.size() * sizeof(double), h->stream);
+    for (size_t k = 0; k < n_total; ++k) {
+      const size_t count = static_cast<size_t>(w[k * h->words + L.hist_count]);
+      counts[k] = count;
+      if (!entries || cap == 0) continue;
+      // the kept steps, oldest first
+      const size_t kept = std::min(count, cap), first = count - kept;
+      for (size_t j = 0; j < kept; ++j) {
+        const double* src = ring.data() + (k * cap + (first + j) % cap) * hw;
+        std::copy(src, src + hw, entries + (k * cap + j) * hw);
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_system_ode_free(dust2_ode_general* h) {
+  if (handle_ok(h, TAG_ODE_GENERAL)) delete h;
 }
 
 }  // extern "C"

@@ -19,6 +19,8 @@ enum sampler_error : uint32_t {
   ERR_ODE_INIT = 5,        // "Initial step size was not finite"
   ERR_INTERP_BEFORE = 6,   // "Tried to interpolate at time = .., which is .. before .."
   ERR_INTERP_AFTER = 7,    // "Tried to interpolate at time = .., which is .. after .."
+  ERR_ODE_HISTORY_LOCATE = 8,  // "Failed to locate history element"
+  ERR_ODE_HISTORY_SHORT = 9,   // a delayed value in a step that left the history ring
 };
 
 // Doubles of LDS scratch every filter workgroup reserves ahead of its

@@ -24,10 +24,16 @@ class Generator:
     dust2 generator, inst/template/dust.R:1-12)."""
 
     def __init__(self, name, time_type, parameters, defaults, constant, state, n_data=1,
-                 sizes=None, precisions=("fp64", "fp32"), interpolate_beta=None):
+                 sizes=None, precisions=("fp64", "fp32"), interpolate_beta=None,
+                 normalise=None, events=None):
         # interpolate_beta: None, or the interpolation kind of a beta given
         # as knots 'beta_t' and values 'beta_y' (array parameters)
         self.interpolate_beta = interpolate_beta
+        # normalise: maps one group's parameter dict to the parameter
+        # layout (array lengths fixed); events: names of the events of a
+        # row (a function of the row)
+        self.normalise = normalise
+        self.events = events
         self.name = name
         self.time_type = time_type
         self.parameters = parameters
@@ -149,6 +155,95 @@ def seir16():
         precisions=("fp32",))
 
 
+# ---- continuous-time models with events, carried variables and delays ----
+# (dust2_amd/csrc/ode_models.hpp; the reference's example systems)
+MAX_CHANGES = 16
+
+
+def bounce():
+    """A ball bouncing with damping: one root event (tests/testthat/examples/
+    event.cpp).  'g' is fixed at 9.8 there; here it is a parameter with that
+    default."""
+    return Generator("bounce", "continuous", ["g", "height", "velocity", "damp"],
+                     {"g": 9.8, "height": 0.0, "velocity": 10.0, "damp": 0.9},
+                     {"g", "height", "velocity"}, ["height", "velocity"],
+                     precisions=("fp64",), events=lambda row: ["bounce"])
+
+
+def _change_normalise(p):
+    # build_shared (tests/testthat/examples/event-time.cpp): r, n, then
+    # t_change and delta of length n (padded to the fixed layout)
+    out = dict(p)
+    if "r" not in p:
+        raise ValueError("Expected a value for 'r'")
+    n = int(p.get("n", np.size(p.get("t_change", []))))
+    tc = f64(p.get("t_change", np.zeros(0))).reshape(-1)
+    dl = f64(p.get("delta", np.zeros(0))).reshape(-1)
+    if tc.size != n or dl.size != n:
+        raise ValueError(f"Expected 't_change' and 'delta' to have length {n}")
+    if n > MAX_CHANGES:
+        raise ValueError(f"At most {MAX_CHANGES} changes are supported")
+    out["n"] = float(n)
+    out["t_change"] = np.concatenate([tc, np.full(MAX_CHANGES - n, np.nan)])
+    out["delta"] = np.concatenate([dl, np.zeros(MAX_CHANGES - n)])
+    return out
+
+
+def change():
+    """Linear growth with jumps at scheduled times: one event per change
+    time (tests/testthat/examples/event-time.cpp)."""
+    return Generator("change", "continuous", ["r", "n", "t_change", "delta"],
+                     {"r": np.nan, "n": 0.0, "t_change": np.full(MAX_CHANGES, np.nan),
+                      "delta": np.zeros(MAX_CHANGES)},
+                     {"r", "n", "t_change", "delta"}, ["y"],
+                     sizes={"t_change": MAX_CHANGES, "delta": MAX_CHANGES},
+                     precisions=("fp64",), normalise=_change_normalise,
+                     events=lambda row: [f"event-{i + 1}" for i in range(int(row[1]))])
+
+
+def eventzero():
+    """Two accumulating variables, the second reset every period_z, with
+    events that only record a crossing (tests/testthat/examples/
+    event-zero.cpp)."""
+    return Generator("eventzero", "continuous", ["r", "value_y", "value_z", "period_z"],
+                     {"r": 1.0, "value_y": 9999.0, "value_z": 9999.0, "period_z": 1.0},
+                     {"r", "value_y", "value_z", "period_z"}, ["y", "z"],
+                     precisions=("fp64",), events=lambda row: ["y", "z"])
+
+
+def specialevent():
+    """Growth at r0 until an event at time_activate sets the carried
+    variable 'active', after which the rate is r1 (tests/testthat/examples/
+    specialevent.cpp)."""
+    return Generator("specialevent", "continuous", ["r0", "r1", "time_activate"],
+                     {"r0": 1.0, "r1": -1.0, "time_activate": np.nan},
+                     {"r0", "r1", "time_activate"}, ["y", "active", "end"],
+                     precisions=("fp64",),
+                     events=lambda row: [] if np.isnan(row[2]) else ["time_activate"])
+
+
+def sirdelay():
+    """The SIR ODE with incidence as the cumulative count less its value one
+    time unit earlier, read through a delay (tests/testthat/examples/
+    delay.cpp)."""
+    return Generator("sirdelay", "continuous", ["N", "I0", "beta", "gamma"],
+                     {"N": 1000.0, "I0": 10.0, "beta": 0.2, "gamma": 0.1},
+                     {"N"}, ["S", "I", "R", "cases_cumul", "cases_inc"],
+                     precisions=("fp64",), events=lambda row: [])
+
+
+def logdelay():
+    """The delayed logistic equation dy/dt = r y(t) (1 - y(t - tau) / K),
+    whose derivative reads a delayed value (no reference model; exercises
+    delays in the rhs, dust2/continuous/system.hpp:371-387)."""
+    return Generator("logdelay", "continuous", ["r", "K", "tau", "y0"],
+                     {"r": 1.0, "K": 100.0, "tau": 1.0, "y0": 10.0},
+                     {"tau"}, ["y"], precisions=("fp64",), events=lambda row: [])
+
+
+GENERAL_MODELS = ("bounce", "change", "eventzero", "specialevent", "sirdelay", "logdelay")
+
+
 def _precision(p):
     if p in ("fp64", "double", FP64):
         return FP64
@@ -167,6 +262,8 @@ def pars_matrix(generator, pars, n_groups):
                          f"{len(pars)}")
     rows = []
     for p in pars:
+        if generator.normalise is not None:
+            p = generator.normalise(p)
         unknown = set(p) - set(generator.parameters)
         if unknown:
             raise ValueError(f"Unknown parameters: {', '.join(sorted(unknown))}")
@@ -652,11 +749,62 @@ class ContinuousSystem:
         return (self.n_groups, self.n_particles, self.generator.n_state)
 
 
-def dust_system_create(generator, pars, n_particles, n_groups=1, time=0, dt=None, seed=None,
-                       deterministic=False, precision="fp64", ode_control=None):
+class GeneralSystem:
+    """A continuous-time system of a model with events, carried variables,
+    resets or delays (dust_system_create, R/interface.R:60-150, over
+    dust2/continuous/system.hpp): FP64; solver internals, the event log and
+    the history ring stay on the device between calls."""
+    prefix = "dust2_system_ode"
+
+    def __init__(self, generator, pars, n_particles, n_groups, time, ode_control,
+                 history_cap=None, event_cap=1024):
+        self.generator = generator
+        self.n_particles = int(n_particles)
+        self.n_groups = int(n_groups)
+        sizes = np.zeros(7, dtype=np.uintp)
+        call("dust2_system_ode_model", generator.name.encode(), szptr(sizes))
+        (self.n_var, self.n_special, self.n_output, n_pars, self.max_events,
+         self.n_hist, has_delays) = (int(v) for v in sizes)
+        self.has_delays = bool(has_delays)
+        self.rows = pars_matrix(generator, pars, self.n_groups)
+        if self.rows.shape[1] != n_pars:
+            raise ValueError(f"Parameter layout of '{generator.name}' does not match the library")
+        self.ode_control = _ode_control(ode_control)
+        ctl = self.ode_control
+        self.record = 100000 if ctl.debug_record_step_times else 0
+        self.save_history = bool(ctl.save_history) or self.has_delays
+        if history_cap is None:
+            history_cap = 100000 if ctl.save_history else 4096
+        self.history_cap = int(history_cap) if self.save_history else 0
+        self.event_cap = int(event_cap) if self.max_events else 0
+        out = C.c_void_p()
+        call("dust2_system_ode_alloc", generator.name.encode(), dptr(self.rows), self.n_groups,
+             self.n_particles, float(time), dptr(ctl.values), dptr(ctl.critical_times),
+             ctl.critical_times.size, self.record, int(ctl.save_history), self.history_cap,
+             self.event_cap, C.byref(out))
+        self.ptr = out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib._lib is not None:
+            _lib.lib().dust2_system_ode_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def shape(self):
+        return (self.n_groups, self.n_particles, self.generator.n_state)
+
+
+def dust_system_create(generator, pars=None, n_particles=1, n_groups=1, time=0, dt=None,
+                       seed=None, deterministic=False, precision="fp64", ode_control=None,
+                       **kwargs):
+    if pars is None:
+        pars = {}
     if generator.time_type == "continuous":
         if dt is not None:
             raise ValueError("Can't use 'dt' with continuous-time systems")
+        if generator.name in GENERAL_MODELS:
+            return GeneralSystem(generator, pars, n_particles, n_groups, time, ode_control,
+                                 **kwargs)
         return ContinuousSystem(generator, pars, n_particles, n_groups, time, ode_control)
     if ode_control is not None:
         raise ValueError("Can't use 'ode_control' with discrete-time systems")
@@ -724,18 +872,34 @@ def dust_system_reorder(sys, index):
     call(_prefix(sys) + "_reorder", sys.ptr, u32ptr(idx))
 
 
-def dust_system_internals(sys):
-    """dust_system_internals for a continuous-time system: per particle
-    (group-major) n_steps, n_steps_accepted, n_steps_rejected, step_size
-    and, with debug_record_step_times, the step times."""
-    if not isinstance(sys, ContinuousSystem):
+def dust_system_internals(sys, include_history=False):
+    """dust_system_internals for a continuous-time system (dust2/r/continuous/
+    system.hpp:56-160): per particle (group-major) n_steps,
+    n_steps_accepted, n_steps_rejected, step_size; with
+    debug_record_step_times, the step times; for models with events, the
+    events found (name, time, 1-based index, sign); with include_history,
+    the saved history (t0, t1, h and coefficients [variable, 5, step])."""
+    if not isinstance(sys, (ContinuousSystem, GeneralSystem)):
         raise ValueError("Internals are only available for continuous-time systems")
     n = sys.n_groups * sys.n_particles
     stats = np.zeros((n, 4))
     counts = np.zeros(n, dtype=np.uintp)
     times = np.zeros(n * max(sys.record, 1))
-    call("dust2_system_sirode_internals", sys.ptr, dptr(stats),
+    call(sys.prefix + "_internals", sys.ptr, dptr(stats),
          dptr(times) if sys.record else None, szptr(counts))
+    events = history = None
+    if isinstance(sys, GeneralSystem):
+        if sys.event_cap:
+            ev_n = np.zeros(n, dtype=np.uintp)
+            ev = np.zeros(n * sys.event_cap * 3)
+            call("dust2_system_ode_events", sys.ptr, szptr(ev_n), dptr(ev))
+            events = (ev_n, ev.reshape(n, sys.event_cap, 3))
+        if include_history and sys.history_cap:
+            h_n = np.zeros(n, dtype=np.uintp)
+            hw = 3 + 5 * sys.n_hist
+            hist = np.zeros(n * sys.history_cap * hw)
+            call("dust2_system_ode_history", sys.ptr, szptr(h_n), dptr(hist))
+            history = (h_n, hist.reshape(n, sys.history_cap, hw))
     out = []
     for k in range(n):
         row = {"group": k // sys.n_particles, "particle": k % sys.n_particles,
@@ -744,6 +908,24 @@ def dust_system_internals(sys):
         if sys.record:
             c = min(int(counts[k]), sys.record)
             row["step_times"] = times[k * sys.record:k * sys.record + c].copy()
+        if events is not None:
+            names = sys.generator.events(sys.rows[k // sys.n_particles])
+            c = int(events[0][k])
+            if c > sys.event_cap:
+                raise RuntimeError(f"{c} events found but only {sys.event_cap} kept; "
+                                   "raise 'event_cap'")
+            r = events[1][k, :c]
+            idx = r[:, 1].astype(int)
+            row["events"] = {"name": [names[i] for i in idx], "time": r[:, 0].copy(),
+                             "index": idx + 1, "sign": r[:, 2].copy()}
+        if history is not None:
+            c = int(history[0][k])
+            kept = min(c, sys.history_cap)
+            e = history[1][k, :kept]
+            H = sys.n_hist
+            row["history"] = {"t0": e[:, 0].copy(), "t1": e[:, 1].copy(), "h": e[:, 2].copy(),
+                              "coefficients": e[:, 3:].reshape(kept, 5, H).transpose(2, 1, 0),
+                              "n_saved": c}
         out.append(row)
     return out
 
@@ -765,11 +947,14 @@ def dust_system_set_rng_state(sys, state):
 class OdeControl:
     """dust_ode_control (R/interface-continuous.R:51-79): the five solver
     settings (as an array: max_steps, atol, rtol, step_size_min,
-    step_size_max), the times the solver must stop at and whether systems
-    record their step times (history saving is not supported)."""
+    step_size_max), the times the solver must stop at, whether systems
+    record their step times and whether they save the history of their
+    steps (models with delays always do)."""
 
-    def __init__(self, values, critical_times=None, debug_record_step_times=False):
+    def __init__(self, values, critical_times=None, debug_record_step_times=False,
+                 save_history=False):
         self.values = f64(values).reshape(-1)
+        self.save_history = bool(save_history)
         self.critical_times = (np.zeros(0) if critical_times is None
                                else np.ascontiguousarray(f64(critical_times).reshape(-1)))
         if np.any(np.diff(self.critical_times) <= 0):
@@ -781,9 +966,10 @@ class OdeControl:
 
 
 def dust_ode_control(max_steps=10000, atol=1e-6, rtol=1e-6, step_size_min=0,
-                     step_size_max=np.inf, critical_times=None, debug_record_step_times=False):
+                     step_size_max=np.inf, critical_times=None, debug_record_step_times=False,
+                     save_history=False):
     return OdeControl([max_steps, atol, rtol, step_size_min, step_size_max], critical_times,
-                      debug_record_step_times)
+                      debug_record_step_times, save_history)
 
 
 def _ode_control(ctl):

@@ -53,6 +53,7 @@ typedef struct dust2_system dust2_system;
 typedef struct dust2_unfilter dust2_unfilter;
 typedef struct dust2_unfilter_sir dust2_unfilter_sir;
 typedef struct dust2_ode_system dust2_ode_system;
+typedef struct dust2_ode_general dust2_ode_general;
 
 /* ---- library ------------------------------------------------------------ */
 
@@ -325,6 +326,67 @@ int dust2_system_sirode_reorder(dust2_ode_system* h, const uint32_t* index, char
 int dust2_system_sirode_internals(dust2_ode_system* h, double* stats, double* step_times,
                                   size_t* n_step_times, char* err, size_t errlen);
 void dust2_system_sirode_free(dust2_ode_system* h);
+
+/* ---- general continuous-time systems: events, carried variables, delays --
+ * One set of entry points for the continuous-time models of the reference's
+ * tests and examples that use events, special (carried) variables, periodic
+ * resets and delays, chosen by name at alloc: "bounce"
+ * (tests/testthat/examples/event.cpp), "change" (event-time.cpp),
+ * "eventzero" (event-zero.cpp), "specialevent" (specialevent.cpp),
+ * "sirdelay" (delay.cpp) and "logdelay" (a delayed logistic equation whose
+ * rhs reads the delayed state).  They replace the per-model
+ * dust2_system_<name>_* registrations that inst/template/system.cpp:1-54
+ * generates over dust_continuous (dust2/continuous/system.hpp:32-585) with
+ * solver::step / run / apply_events (dust2/continuous/solver.hpp:152-262,
+ * 359-435), history and delays (dust2/continuous/history.hpp,
+ * delays.hpp); an R shim maps dust2_system_bounce_alloc(...) to
+ * dust2_system_ode_alloc("bounce", ...).  FP64.
+ *
+ * sizes[7] from dust2_system_ode_model: n_var, n_special, n_output, n_pars,
+ * max_events, history variables, has_delays.  State [G][P][S] with
+ * S = n_var + n_special + n_output (the reference's packing order);
+ * pars[G][n_pars]. */
+int dust2_system_ode_model(const char* model, size_t* sizes, char* err, size_t errlen);
+/* control: the five dust_ode_control() values or NULL; critical_times;
+ * record_step_times: step times kept per particle (0: off); save_history
+ * (control save_history; always on for a model with delays) with
+ * history_cap steps kept per particle, the most recent; event_cap: events
+ * kept per particle (the first).  Errors lock the system as the reference
+ * does (dust2/r/system.hpp:9-16). */
+int dust2_system_ode_alloc(const char* model, const double* pars, size_t n_groups,
+                           size_t n_particles, double time, const double* control,
+                           const double* critical_times, size_t n_critical,
+                           size_t record_step_times, size_t save_history, size_t history_cap,
+                           size_t event_cap, dust2_ode_general** out, char* err,
+                           size_t errlen);
+int dust2_system_ode_run_to_time(dust2_ode_general* h, double time, char* err, size_t errlen);
+int dust2_system_ode_simulate(dust2_ode_general* h, const double* times, size_t n_times,
+                              double* out, char* err, size_t errlen);
+int dust2_system_ode_state(dust2_ode_general* h, double* out, char* err, size_t errlen);
+int dust2_system_ode_set_state(dust2_ode_general* h, const double* state, char* err,
+                               size_t errlen);
+int dust2_system_ode_set_state_initial(dust2_ode_general* h, char* err, size_t errlen);
+int dust2_system_ode_time(dust2_ode_general* h, double* out, char* err, size_t errlen);
+int dust2_system_ode_set_time(dust2_ode_general* h, double time, char* err, size_t errlen);
+int dust2_system_ode_update_pars(dust2_ode_general* h, const double* pars, char* err,
+                                 size_t errlen);
+int dust2_system_ode_reorder(dust2_ode_general* h, const uint32_t* index, char* err,
+                             size_t errlen);
+/* dust_system_internals (dust2/r/continuous/system.hpp:56-160):
+ * stats[G*P][4] = n_steps, n_steps_accepted, n_steps_rejected, step_size;
+ * step_times[G*P][record_step_times] with their counts. */
+int dust2_system_ode_internals(dust2_ode_general* h, double* stats, double* step_times,
+                               size_t* n_step_times, char* err, size_t errlen);
+/* the events found (internals$events): counts[G*P] and
+ * records[G*P][event_cap][3] = time, event index (0-based), sign. */
+int dust2_system_ode_events(dust2_ode_general* h, size_t* counts, double* records, char* err,
+                            size_t errlen);
+/* the history (internals$history): counts[G*P] of steps saved and, oldest
+ * first, entries[G*P][history_cap][3 + 5 H] = t0, t1, h, then the
+ * coefficients c1..c5 of the H history variables. */
+int dust2_system_ode_history(dust2_ode_general* h, size_t* counts, double* entries, char* err,
+                             size_t errlen);
+void dust2_system_ode_free(dust2_ode_general* h);
 
 /* ---- deterministic discrete-time SIR unfilter with the adjoint ----------
  * Replaces dust2_discrete_unfilter_alloc<sir> (dust2/r/discrete/unfilter.hpp:12-58),

@@ -2,16 +2,20 @@
 //
 // Thin C wrapper over the parts of the reference engine that compile on
 // their own (no monty, no cpp11, no R): dust2/filter_details.hpp,
-// dust2/continuous/solver.hpp (+ control/history/events) and
+// dust2/continuous/solver.hpp (+ control/history/events/delays and the
+// root finder inst/include/lostturnip.hpp) and
 // dust2/interpolate/interpolate.hpp.  Built by
 // oracle/Makefile straight from the reference headers into oracle/_ref/;
 // used to pin the CPU restatement in oracle.hpp and to write the golden
 // fixtures under tests/golden/.  Nothing here is shipped.
+#include <dust2/continuous/delays.hpp>
 #include <dust2/continuous/solver.hpp>
 #include <dust2/filter_details.hpp>
 #include <dust2/interpolate/interpolate.hpp>
 
 #include <cstdint>
+#include <functional>
+#include <string>
 #include <cstdio>
 #include <vector>
 
@@ -215,6 +219,199 @@ int ref_interpolate(int kind, const double* t, size_t n_t, const double* y, size
     std::snprintf(err, errlen, "%s", e.what());
     return 1;
   }
+}
+
+// One particle of a continuous-time model with events, special variables,
+// resets or delays, driven through the reference solver the way
+// dust_continuous drives it (dust2/continuous/system.hpp:80-106 run_to_time,
+// :371-410 rhs_ / output_, :414-482 initialise_delays_ /
+// initialise_solver_): initialise and initialise_events at time_start, then
+// solver.run to each target in turn, output variables from the delayed
+// values after each.  The models restate the reference's example systems
+// (tests/testthat/examples/event.cpp, event-time.cpp, event-zero.cpp,
+// specialevent.cpp, delay.cpp) plus a delayed logistic equation; model ids
+// and parameter rows as dust2_amd/csrc/ode_models.hpp.
+//
+// Outputs: out_state[n_targets][S]; stats[4]; step times (cap, *n_steps);
+// events[cap_ev][3] = time, index, sign (*n_events); history entries
+// [cap_h][3 + 5 H] = t0, t1, h, c1..c5 of the H history variables, oldest
+// first (*n_hist).  y0: the initial state, or null for the model's own.
+int ref_ode_general(int model, const double* p, const double* y0, double time_start,
+                    const double* targets, size_t n_targets, const double* ctl_in,
+                    const double* critical, size_t n_critical, int save_history,
+                    double* out_state, double* out_stats, double* step_times, size_t cap,
+                    size_t* n_steps, double* events, size_t cap_ev, size_t* n_events,
+                    double* hist, size_t cap_h, size_t* n_hist, char* err, size_t errlen) {
+  using dust2::ode::event;
+  using dust2::ode::events_type;
+  using dust2::ode::internals;
+  using dust2::ode::root_type;
+  using dust2::ode::solver;
+  size_t n_var = 0, n_special = 0, n_output = 0;
+  std::vector<double> y;
+  events_type<double> ev;
+  dust2::zero_every_type<double> zero_every;
+  std::function<void(double, const double*, double*)> rhs_plain;
+  bool rhs_delays = false, has_delays = false;
+  std::vector<dust2::ode::delays<double>> dl;
+  switch (model) {
+    case 0: {  // bounce: g, height, velocity, damp
+      n_var = 2;
+      y = {p[1], p[2]};
+      rhs_plain = [p](double, const double* s, double* d) {
+        d[0] = s[1];
+        d[1] = -p[0];
+      };
+      ev.push_back(event<double>("bounce", 0, [p](double, double, double* s) {
+        s[0] = 0;
+        s[1] = -p[3] * s[1];
+      }));
+      break;
+    }
+    case 1: {  // change: r, n, t_change[16], delta[16]
+      n_var = 1;
+      y = {0};
+      rhs_plain = [p](double, const double*, double* d) { d[0] = p[0]; };
+      for (int i = 0; i < static_cast<int>(p[1]); ++i) {
+        ev.push_back(event<double>(
+            "event-" + std::to_string(i + 1), std::vector<size_t>{},
+            [p, i](double t, const double*) { return t - p[2 + i]; },
+            [p, i](double, double, double* s) { s[0] += p[2 + 16 + i]; }));
+      }
+      break;
+    }
+    case 2: {  // eventzero: r, value_y, value_z, period_z
+      n_var = 2;
+      y = {0, 1};
+      rhs_plain = [p](double t, const double*, double* d) {
+        d[0] = t / p[0];
+        d[1] = t / p[0];
+      };
+      zero_every.push_back({p[3], {1}});
+      auto none = [](double, double, double*) {};
+      ev.push_back(event<double>("y", std::vector<size_t>{0},
+                                 [p](double, const double* s) { return s[0] - p[1]; }, none,
+                                 root_type::increase));
+      ev.push_back(event<double>("z", std::vector<size_t>{1},
+                                 [p](double, const double* s) { return s[0] - p[2]; }, none,
+                                 root_type::increase));
+      break;
+    }
+    case 3: {  // specialevent: r0, r1, time_activate
+      n_var = 1;
+      n_special = 2;
+      y = {0, 0, 0};
+      rhs_plain = [p](double, const double* s, double* d) { d[0] = s[1] != 0 ? p[1] : p[0]; };
+      if (!std::isnan(p[2])) {
+        ev.push_back(event<double>(
+            "time_activate", std::vector<size_t>{},
+            [p](double t, const double*) { return t - p[2]; },
+            [](double, double, double* s) { s[1] = 1; }));
+      }
+      break;
+    }
+    case 4: {  // sirdelay: N, I0, beta, gamma; output y4 = y3 - y3(t - 1)
+      n_var = 4;
+      n_output = 1;
+      y = {p[0] - p[1], p[1], 0, 0, 0};
+      rhs_plain = [p](double, const double* s, double* d) {
+        const double rate_SI = p[2] * s[0] * s[1] / p[0];
+        const double rate_IR = p[3] * s[1];
+        d[0] = -rate_SI;
+        d[1] = rate_SI - rate_IR;
+        d[2] = rate_IR;
+        d[3] = rate_SI;
+      };
+      dl.push_back(dust2::ode::delays<double>({{1, {{3, 1}}}}));
+      has_delays = true;
+      break;
+    }
+    case 5: {  // logdelay: r, K, tau, y0
+      n_var = 1;
+      y = {p[3]};
+      dl.push_back(dust2::ode::delays<double>({{p[2], {{0, 1}}}}));
+      has_delays = true;
+      rhs_delays = true;
+      break;
+    }
+    default:
+      std::snprintf(err, errlen, "unknown model");
+      return 1;
+  }
+  if (y0) std::copy(y0, y0 + y.size(), y.begin());
+  const size_t S = y.size();
+  dust2::ode::control<double> ctl(static_cast<size_t>(ctl_in[0]), ctl_in[1], ctl_in[2],
+                                  std::max(ctl_in[3], std::numeric_limits<double>::epsilon()),
+                                  ctl_in[4], std::vector<double>(critical, critical + n_critical),
+                                  save_history != 0, true);
+  if (has_delays) {
+    ctl.step_size_max = dl[0].step_size_max(ctl.step_size_max, rhs_delays);
+    ctl.save_history = true;
+  }
+  solver<double> s(n_var, n_special, ctl);
+  internals<double> in(n_var, n_special, ctl.save_history);
+  dust2::ode::delay_result_type<double> res;
+  if (has_delays) {
+    in.history_values.set_index(dl[0].index());
+    res = dl[0].result();
+  }
+  std::function<void(double, const double*, double*)> rhs;
+  if (rhs_delays) {
+    rhs = [&, p](double t, const double* st, double* d) {
+      dl[0].eval(t, in.history_values, res);
+      d[0] = p[0] * st[0] * (1 - res[0].data[0] / p[1]);
+    };
+  } else {
+    rhs = rhs_plain;
+  }
+  double t = time_start;
+  try {
+    s.initialise(t, y.data(), in, rhs);
+    s.initialise_events(ev, in);
+    for (size_t i = 0; i < n_targets; ++i) {
+      s.run(t, targets[i], y.data(), zero_every, ev, in, rhs);
+      t = targets[i];
+      if (n_output > 0) {
+        dl[0].eval(t, in.history_values, res);
+        y[4] = y[3] - res[0].data[0];
+      }
+      std::copy(y.begin(), y.end(), out_state + i * S);
+    }
+  } catch (const std::exception& e) {
+    if (err && errlen) std::snprintf(err, errlen, "%s", e.what());
+    return 1;
+  }
+  out_stats[0] = in.n_steps;
+  out_stats[1] = in.n_steps_accepted;
+  out_stats[2] = in.n_steps_rejected;
+  out_stats[3] = in.step_size;
+  *n_steps = in.step_times.size();
+  for (size_t i = 0; i < std::min(cap, in.step_times.size()); ++i) step_times[i] = in.step_times[i];
+  *n_events = in.events.size();
+  for (size_t i = 0; i < std::min(cap_ev, in.events.size()); ++i) {
+    events[3 * i + 0] = in.events[i].time;
+    events[3 * i + 1] = static_cast<double>(in.events[i].index);
+    events[3 * i + 2] = in.events[i].sign;
+  }
+  const auto& hd = in.history_values.data();
+  *n_hist = hd.size();
+  const size_t H = in.history_values.n_state();
+  const size_t W = 3 + 5 * H;
+  for (size_t i = 0; i < std::min(cap_h, hd.size()); ++i) {
+    double* e = hist + i * W;
+    e[0] = hd[i].t0;
+    e[1] = hd[i].t1;
+    e[2] = hd[i].h;
+    for (size_t j = 0; j < H; ++j) {
+      e[3 + j] = hd[i].c1[j];
+      const bool ode = j < hd[i].c2.size();
+      e[3 + H + j] = ode ? hd[i].c2[j] : 0;
+      e[3 + 2 * H + j] = ode ? hd[i].c3[j] : 0;
+      e[3 + 3 * H + j] = ode ? hd[i].c4[j] : 0;
+      e[3 + 4 * H + j] = ode ? hd[i].c5[j] : 0;
+    }
+  }
+  return 0;
 }
 
 }  // extern "C"

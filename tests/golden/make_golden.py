@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import oracle_lib as ol  # noqa: E402
 
@@ -134,6 +135,59 @@ def sirode_system_fixture():
     return {"cases": cases}
 
 
+GENERAL_CASES = [
+    # name, model, user parameters, targets, control, critical times, save_history
+    ("bounce_default", "bounce", {}, np.linspace(0, 6, 60), None, [], True),
+    ("bounce_damped", "bounce", {"damp": 0.7, "velocity": 5.0, "height": 2.0},
+     np.linspace(0, 8, 41), None, [], True),
+    ("bounce_critical", "bounce", {}, [6.0], None, [1.0, 2.5, 4.0], False),
+    ("bounce_step_max", "bounce", {"damp": 0.8}, np.arange(0.0, 7.0), [10000, 1e-6, 1e-6, 0, 0.3],
+     [], False),
+    ("change_times", "change", {"r": 0.2, "t_change": [2.0, 5.1234, 7.0],
+                                "delta": [0.3, -1.2, 0.8]}, np.linspace(0, 10, 101), None, [],
+     True),
+    ("change_coincident", "change", {"r": 0.2, "t_change": [2.0, 2.0, 3.0],
+                                     "delta": [1.0, 3.0, 5.0]}, np.linspace(0, 10, 101), None,
+     [], True),
+    ("eventzero_y", "eventzero", {"value_y": 6.3456}, [100.0], None, [], True),
+    ("eventzero_both", "eventzero", {"value_y": 30.0, "value_z": 1.5, "period_z": 2.0},
+     np.arange(0.0, 20.0, 0.5), None, [], False),
+    ("specialevent_none", "specialevent", {}, np.linspace(0, 10, 101), None, [], False),
+    ("specialevent_at_3", "specialevent", {"time_activate": 3.0}, np.linspace(0, 10, 101), None,
+     [], True),
+    ("sirdelay_default", "sirdelay", {}, np.arange(0.0, 101.0, 4.0), None, [], False),
+    ("sirdelay_fast", "sirdelay", {"beta": 0.4, "gamma": 0.15, "I0": 5.0},
+     np.arange(0.0, 60.5, 0.5), None, [], False),
+    ("logdelay_oscillating", "logdelay", {"r": 1.8}, np.arange(0.0, 31.0), None, [], False),
+    ("logdelay_tight", "logdelay", {"r": 1.2, "tau": 2.0, "K": 50.0, "y0": 5.0},
+     np.arange(0.0, 40.0, 2.0), [10000, 1e-8, 1e-8, 0, np.inf], [5.0, 11.0], False),
+]
+
+
+def ode_general_fixture():
+    """Continuous-time models with events, special variables, resets and
+    delays through the reference's own solver, events, history and delays
+    (oracle/_ref): states at the targets, counters, step times, events and
+    history."""
+    import dust2_amd
+    from dust2_amd.interface import pars_matrix
+    cases = []
+    for name, model, pars, targets, control, critical, save_history in GENERAL_CASES:
+        row = pars_matrix(getattr(dust2_amd, model)(), pars, 1)[0]
+        r = ol.ref_ode_general(model, row, targets, control=control, critical_times=critical,
+                               save_history=save_history)
+        cases.append({"name": name, "model": model,
+                      "pars": {k: np.asarray(v).tolist() for k, v in pars.items()},
+                      "row": row.tolist(), "targets": list(map(float, targets)),
+                      "control": None if control is None else [float(c) for c in control],
+                      "critical_times": list(map(float, critical)),
+                      "save_history": save_history,
+                      "state": r["state"].tolist(), "stats": r["stats"].tolist(),
+                      "step_times": r["step_times"].tolist(), "events": r["events"].tolist(),
+                      "history": r["history"].tolist() if save_history else None})
+    return {"cases": cases}
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref/libdust2ref.so is not built (needs the reference tree)")
@@ -147,6 +201,8 @@ def main():
         json.dump(logistic_fixture(), f)
     with open(os.path.join(HERE, "sirode_system_dp5.json"), "w") as f:
         json.dump(sirode_system_fixture(), f)
+    with open(os.path.join(HERE, "ode_general_dp5.json"), "w") as f:
+        json.dump(ode_general_fixture(), f)
 
 
 if __name__ == "__main__":

@@ -419,3 +419,46 @@ def ref_sirode_system(pars, time_start, targets, control=None, critical_times=()
                                  n.ctypes.data_as(SZ), buf, 512)
     _check(rc, buf)
     return {"state": state, "stats": stats, "step_times": times[:int(n[0])].copy()}
+
+
+GENERAL_IDS = {"bounce": 0, "change": 1, "eventzero": 2, "specialevent": 3, "sirdelay": 4,
+               "logdelay": 5}
+
+
+def ref_ode_general(model, row, targets, y0=None, control=None, critical_times=(),
+                    save_history=False, time_start=0.0):
+    """One particle of a continuous-time model with events, special
+    variables, resets or delays through the reference solver (oracle/_ref):
+    states at the targets, counters, step times, events (time, 0-based
+    index, sign) and the saved history (t0, t1, h, coefficients)."""
+    row = np.ascontiguousarray(row, dtype=np.float64)
+    targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1)
+    ctl = np.array([10000, 1e-6, 1e-6, 0, np.inf] if control is None else control,
+                   dtype=np.float64)
+    crit = np.ascontiguousarray(critical_times, dtype=np.float64).reshape(-1)
+    S = {"bounce": 2, "change": 1, "eventzero": 2, "specialevent": 3, "sirdelay": 5,
+         "logdelay": 1}[model]
+    H = {"bounce": 2, "change": 1, "eventzero": 2, "specialevent": 3, "sirdelay": 1,
+         "logdelay": 1}[model]
+    y0a = None if y0 is None else np.ascontiguousarray(y0, dtype=np.float64)
+    state = np.zeros((targets.size, S))
+    stats = np.zeros(4)
+    cap, cap_ev, cap_h = 200000, 10000, 200000
+    times = np.zeros(cap)
+    ev = np.zeros(cap_ev * 3)
+    hist = np.zeros(cap_h * (3 + 5 * H))
+    n = np.zeros(3, dtype=np.uintp)
+    buf = C.create_string_buffer(512)
+    szp = [n[i:i + 1].ctypes.data_as(SZ) for i in range(3)]
+    rc = ref().ref_ode_general(
+        GENERAL_IDS[model], _dp(row), _dp(y0a) if y0a is not None else None,
+        C.c_double(time_start), _dp(targets), C.c_size_t(targets.size), _dp(ctl),
+        _dp(crit) if crit.size else None, C.c_size_t(crit.size), C.c_int(int(save_history)),
+        _dp(state), _dp(stats), _dp(times), C.c_size_t(cap), szp[0], _dp(ev),
+        C.c_size_t(cap_ev), szp[1], _dp(hist), C.c_size_t(cap_h), szp[2], buf, 512)
+    _check(rc, buf)
+    n_ev, n_h = int(n[1]), int(n[2])
+    return {"state": state, "stats": stats, "step_times": times[:int(n[0])].copy(),
+            "events": ev[:3 * n_ev].reshape(n_ev, 3).copy(),
+            "history": hist[:n_h * (3 + 5 * H)].reshape(n_h, 3 + 5 * H).copy()}
+
