@@ -485,10 +485,17 @@ __global__ void __launch_bounds__(256) dp5_system_kernel(ode_sys_launch a) {
   for (int k = 0; k < NV; ++k) s.y[k] = a.state[k * NT + i];
   uint32_t code = 0;
   ode_internals& in = a.internals[i];
+  // the last step's dense output survives initialisation (internals::reset,
+  // solver.hpp:62-70, leaves it), so a reset boundary crossed before a
+  // set_state still applies at the next run's start
+  s.t0 = in.t0;
+  s.h = in.h;
+  s.c1 = in.c1;
+  s.c2 = in.c2;
+  s.c3 = in.c3;
+  s.c4 = in.c4;
+  s.c5 = in.c5;
   if (a.reinit[g]) {
-    s.t0 = 0;
-    s.h = 0;
-    s.c1 = s.c2 = s.c3 = s.c4 = s.c5 = 0;
     if (a.record.times) a.record.count[i] = 0;
     record_step(a.record, i, a.time);
     ode_initialise<double>(s, sh, a.control, code);
@@ -496,13 +503,6 @@ __global__ void __launch_bounds__(256) dp5_system_kernel(ode_sys_launch a) {
     for (int k = 0; k < NV; ++k) s.dydt[k] = in.dydt[k];
     s.step_size = in.step_size;
     s.error = in.error;
-    s.t0 = in.t0;
-    s.h = in.h;
-    s.c1 = in.c1;
-    s.c2 = in.c2;
-    s.c3 = in.c3;
-    s.c4 = in.c4;
-    s.c5 = in.c5;
     s.n_steps = in.n_steps;
     s.n_accepted = in.n_accepted;
     s.n_rejected = in.n_rejected;

@@ -112,13 +112,14 @@ int ref_sirode_unfilter(const double* pars, size_t n_sets, double time_start,
 // continuous-time system drives it (dust2/continuous/system.hpp:80-106,
 // 444-482, 531-542): initialise at time_start, then solver.run to each
 // target in turn, with control.critical_times and
-// debug_record_step_times.  state at each target: out_state[n][5]; stats
+// debug_record_step_times; after target restart_after (if < n_targets) the
+// solver is initialised again, as set_state / update_pars make it.  state at each target: out_state[n][5]; stats
 // [4]; step times up to cap into step_times, their number in *n_steps.
 int ref_sirode_system(const double* pars, double time_start, const double* targets,
                       size_t n_targets, const double* ctl_in, const double* critical,
-                      size_t n_critical, double* out_state, double* out_stats,
-                      double* step_times, size_t cap, size_t* n_steps, char* err,
-                      size_t errlen) {
+                      size_t n_critical, size_t restart_after, double* out_state,
+                      double* out_stats, double* step_times, size_t cap, size_t* n_steps,
+                      char* err, size_t errlen) {
   using dust2::ode::internals;
   using dust2::ode::solver;
   const size_t n_var = 5;
@@ -148,6 +149,11 @@ int ref_sirode_system(const double* pars, double time_start, const double* targe
       s.run(t, targets[i], y.data(), zero_every, events, in, rhs);
       t = targets[i];
       std::copy(y.begin(), y.end(), out_state + i * n_var);
+      if (i == restart_after) {
+        // set_state / update_pars mark the solver for initialisation
+        // (dust2/continuous/system.hpp:444-482): initialise again from here
+        s.initialise(t, y.data(), in, rhs);
+      }
     }
   } catch (const std::exception& e) {
     if (err && errlen) std::snprintf(err, errlen, "%s", e.what());

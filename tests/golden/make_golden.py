@@ -123,10 +123,14 @@ def sirode_system_fixture():
         ("pieces", [5000, 20, 0.5, 0.1], [5.0, 12.0, 30.0, 31.5, 60.0], None, [25.0, 40.0]),
         ("tight", [1000, 10, 0.3, 0.15], np.arange(2.0, 40.0, 3.0),
          [10000, 1e-9, 1e-9, 0, np.inf], [7.5]),
+        # the state is set again after 12 (re-initialising the solver): the
+        # incidence is still reset at the start of the next run
+        ("restart", [1000, 10, 0.2, 0.1], [5.0, 12.0, 12.5, 13.0, 20.0], None, []),
     ]
     for name, pars, targets, control, critical in specs:
-        r = ol.ref_sirode_system(pars, 0.0, targets, control, critical)
-        cases.append({"name": name, "pars": list(map(float, pars)),
+        restart = 1 if name == "restart" else None
+        r = ol.ref_sirode_system(pars, 0.0, targets, control, critical, restart_after=restart)
+        cases.append({"name": name, "pars": list(map(float, pars)), "restart_after": restart,
                       "targets": list(map(float, targets)),
                       "control": None if control is None else [float(c) for c in control],
                       "critical_times": list(map(float, critical)),

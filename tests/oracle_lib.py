@@ -100,8 +100,9 @@ def ref():
                                              D, D, C.c_char_p, C.c_size_t]
         _ref.ref_logistic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D,
                                       C.c_size_t, D, D]
-        _ref.ref_sirode_system.argtypes = [D, C.c_double, D, C.c_size_t, D, D, C.c_size_t, D,
-                                           D, D, C.c_size_t, SZ, C.c_char_p, C.c_size_t]
+        _ref.ref_sirode_system.argtypes = [D, C.c_double, D, C.c_size_t, D, D, C.c_size_t,
+                                           C.c_size_t, D, D, D, C.c_size_t, SZ, C.c_char_p,
+                                           C.c_size_t]
         _ref.ref_interpolate.argtypes = [C.c_int, D, C.c_size_t, D, C.c_size_t, D, C.c_size_t,
                                          D, C.c_char_p, C.c_size_t]
     return _ref
@@ -398,7 +399,8 @@ def sir_tv_rows(pars, kind, beta_t, beta_y):
     return rows
 
 
-def ref_sirode_system(pars, time_start, targets, control=None, critical_times=()):
+def ref_sirode_system(pars, time_start, targets, control=None, critical_times=(),
+                      restart_after=None):
     """One SIRODE particle through the reference solver driven as a
     continuous-time system (oracle/_ref): states at the targets, the
     solver counters and the recorded step times."""
@@ -415,6 +417,8 @@ def ref_sirode_system(pars, time_start, targets, control=None, critical_times=()
     buf = C.create_string_buffer(512)
     rc = ref().ref_sirode_system(_dp(pars), float(time_start), _dp(targets), targets.size,
                                  _dp(ctl), _dp(crit) if crit.size else None, crit.size,
+                                 C.c_size_t(targets.size if restart_after is None
+                                            else restart_after),
                                  _dp(state), _dp(stats), _dp(times), cap,
                                  n.ctypes.data_as(SZ), buf, 512)
     _check(rc, buf)
