@@ -117,9 +117,13 @@ _PROTOS = {
     "dust2_system_sirode_internals": (C.c_int, [VP, D, D, SZ, ERR, ERRLEN]),
     "dust2_system_sirode_free": (None, [VP]),
     "dust2_system_ode_model": (C.c_int, [C.c_char_p, SZ, ERR, ERRLEN]),
-    "dust2_system_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double, D, D,
-                                         C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
-                                         C.c_size_t, C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_system_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double,
+                                         C.c_double, D, D, C.c_size_t, C.c_size_t, C.c_size_t,
+                                         C.c_size_t, C.c_size_t, U64, C.c_size_t, C.c_uint64,
+                                         C.c_int, C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_system_ode_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
+    "dust2_system_ode_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
+    "dust2_system_ode_compare_data": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_system_ode_run_to_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
     "dust2_system_ode_simulate": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_system_ode_state": (C.c_int, [VP, D, ERR, ERRLEN]),

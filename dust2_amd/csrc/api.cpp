@@ -2755,9 +2755,11 @@ struct dust2_ode_general {
   ode_control_dev control{};
   uint32_t record_cap = 0, event_cap = 0, history_cap = 0;
   bool save_history = false;
+  double dt = 0;                       // mixed time: update interval (0: none)
+  bool deterministic = false;
   hipStream_t stream = nullptr;
   dbuf d_pars, d_state, d_internals, d_reinit, d_targets, d_out, d_errors, d_counters,
-      d_critical, d_step_times, d_step_count, d_events, d_history;
+      d_critical, d_step_times, d_step_count, d_events, d_history, d_rng, d_data, d_ll;
   bool errors_pending = false;
   ~dust2_ode_general() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -2766,8 +2768,9 @@ struct dust2_ode_general {
 
 namespace {
 
-const char* const kGeneralModels[odem::MODEL_COUNT] = {"bounce",       "change",   "eventzero",
-                                                       "specialevent", "sirdelay", "logdelay"};
+const char* const kGeneralModels[odem::MODEL_COUNT] = {
+    "bounce", "change", "eventzero", "specialevent", "sirdelay", "logdelay", "special",
+    "malaria"};
 
 int general_model_id(const char* name) {
   for (int i = 0; i < odem::MODEL_COUNT; ++i) {
@@ -2778,6 +2781,10 @@ int general_model_id(const char* name) {
 }
 
 void check_general_pars(int model, const double* row) {
+  if (model == odem::MODEL_MALARIA && row[3] != odem::malaria::n_rates) {
+    throw std::runtime_error("'n_rates' must be " + std::to_string(odem::malaria::n_rates) +
+                             " (the exposed-vector stages are fixed at build time)");
+  }
   if (model == odem::MODEL_CHANGE) {
     const double n = row[1];
     if (!(n >= 0 && n <= odem::change::max_changes && std::floor(n) == n)) {
@@ -2828,6 +2835,9 @@ void general_run(dust2_ode_general* h, const double* targets, size_t n, double* 
   a.history = h->history_cap ? h->d_history.get<double>() : nullptr;
   a.history_cap = h->history_cap;
   a.save_history = h->save_history ? 1 : 0;
+  a.dt = h->dt;
+  a.rng = h->info.mixed ? h->d_rng.get<uint64_t>() : nullptr;
+  a.deterministic = h->deterministic ? 1 : 0;
   a.errors = h->d_errors.get<error_record>();
   a.error_count = h->d_counters.get<uint32_t>();
   hip_check(launch_general_system(a, s), "general ODE system kernel");
@@ -2885,6 +2895,8 @@ int dust2_system_ode_model(const char* model, size_t* sizes, char* err, size_t e
     sizes[4] = static_cast<size_t>(m.max_events);
     sizes[5] = static_cast<size_t>(m.has_delays ? m.n_hist : m.n_var + m.n_special);
     sizes[6] = m.has_delays ? 1u : 0u;
+    sizes[7] = m.mixed ? 1u : 0u;
+    sizes[8] = static_cast<size_t>(m.n_data);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -2892,16 +2904,24 @@ int dust2_system_ode_model(const char* model, size_t* sizes, char* err, size_t e
 }
 
 int dust2_system_ode_alloc(const char* model, const double* pars, size_t n_groups,
-                           size_t n_particles, double time, const double* control,
+                           size_t n_particles, double time, double dt, const double* control,
                            const double* critical_times, size_t n_critical,
                            size_t record_step_times, size_t save_history, size_t history_cap,
-                           size_t event_cap, dust2_ode_general** out, char* err,
+                           size_t event_cap, const uint64_t* seed_state, size_t n_seed_streams,
+                           uint64_t seed, int deterministic, dust2_ode_general** out, char* err,
                            size_t errlen) {
   try {
     check_sizes(n_groups, n_particles);
     auto h = std::make_unique<dust2_ode_general>();
     h->model = general_model_id(model);
     h->info = odem::info(h->model);
+    if (dt != 0) {
+      // dt is for mixed-time models only (R/interface.R:600-620)
+      if (!h->info.mixed) throw std::runtime_error("Can't use 'dt' with continuous-time systems");
+      check_dt(dt);
+    }
+    h->dt = dt;
+    h->deterministic = deterministic != 0;
     const size_t np = static_cast<size_t>(h->info.n_pars);
     for (size_t g = 0; g < n_groups; ++g) check_general_pars(h->model, pars + g * np);
     h->G = n_groups;
@@ -2946,8 +2966,70 @@ int dust2_system_ode_alloc(const char* model, const double* pars, size_t n_group
     hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
     hip_check(hipMemsetAsync(h->d_step_count.get(), 0, h->d_step_count.bytes(), h->stream),
               "memset");
+    if (h->info.mixed) {
+      // one stream per particle, as the system's prng (system.hpp:361)
+      h->d_rng.alloc(n_total * 4 * sizeof(uint64_t));
+      expand_streams(seed_state, n_seed_streams, seed, n_total, h->d_rng.get<uint64_t>(),
+                     h->stream);
+    }
     set_critical_times(h->control, h->d_critical, critical_times, n_critical, h->stream);
     *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_rng_state(dust2_ode_general* h, uint64_t* out, char* err, size_t errlen) {
+  try {
+    check_handle(h, TAG_ODE_GENERAL, "system");
+    if (!h->info.mixed) throw std::runtime_error("This system has no random number streams");
+    to_host(out, h->d_rng, h->G * h->P * 4 * sizeof(uint64_t), h->stream);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_set_rng_state(dust2_ode_general* h, const uint64_t* state, char* err,
+                                   size_t errlen) {
+  try {
+    check_handle(h, TAG_ODE_GENERAL, "system");
+    if (!h->info.mixed) throw std::runtime_error("This system has no random number streams");
+    to_device(h->d_rng, state, h->G * h->P * 4 * sizeof(uint64_t), h->stream);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_system_ode_compare_data(dust2_ode_general* h, const double* data,
+                                  size_t n_groups_data, double* out, char* err, size_t errlen) {
+  try {
+    check_general(h, "compare data for");
+    if (h->info.n_data == 0) throw std::runtime_error("This model has no compare_data");
+    if (n_groups_data != 1 && n_groups_data != h->G) {
+      throw std::runtime_error("Expected data for one group or every group");
+    }
+    const size_t n_total = h->G * h->P;
+    const size_t nd = n_groups_data * static_cast<size_t>(h->info.n_data);
+    h->d_data.alloc(nd * sizeof(double));
+    to_device(h->d_data, data, nd * sizeof(double), h->stream);
+    h->d_ll.alloc(n_total * sizeof(double));
+    gen_compare_launch a{};
+    a.model = h->model;
+    a.pars = h->d_pars.get<double>();
+    a.n_pars = h->info.n_pars;
+    a.n_particles = static_cast<int>(h->P);
+    a.n_total = static_cast<int>(n_total);
+    a.time = h->time;
+    a.state = h->d_state.get<double>();
+    a.data = h->d_data.get<double>();
+    a.shared_data = n_groups_data == 1 ? 1 : 0;
+    a.out = h->d_ll.get<double>();
+    hip_check(launch_general_compare(a, h->stream), "compare kernel");
+    to_host(out, h->d_ll, n_total * sizeof(double), h->stream);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -3033,6 +3115,8 @@ int dust2_system_ode_set_state_initial(dust2_ode_general* h, char* err, size_t e
           case odem::MODEL_SPECIALEVENT: odem::specialevent::initial(h->time, p, y); break;
           case odem::MODEL_SIRDELAY: odem::sirdelay::initial(h->time, p, y); break;
           case odem::MODEL_LOGDELAY: odem::logdelay::initial(h->time, p, y); break;
+          case odem::MODEL_SPECIAL: odem::special::initial(h->time, p, y); break;
+          case odem::MODEL_MALARIA: odem::malaria::initial(h->time, p, y); break;
           default: break;
         }
       }

@@ -75,12 +75,29 @@ struct gen_launch {
   double* history;           // [n_total][history_cap][3 + 5 NH] or null
   uint32_t history_cap;
   int save_history;          // keep the history ring (always with delays)
+  double dt;                 // mixed-time update interval, 0 for none
+  const uint64_t* rng;       // [n_total][4] system streams (mixed-time models)
+  int deterministic;         // updates take the mean of their draws
   error_record* errors;      // [n_total]
   uint32_t* error_count;
+};
+
+struct gen_compare_launch {
+  int model;
+  const double* pars;        // [G][n_pars]
+  int n_pars;
+  int n_particles;
+  int n_total;
+  double time;
+  const double* state;       // [S][n_total]
+  const double* data;        // [G or 1][n_data]
+  int shared_data;
+  double* out;               // [n_total]
 };
 
 int gen_words(int model);
 int gen_history_words(int model);
 hipError_t launch_general_system(const gen_launch& args, hipStream_t stream);
+hipError_t launch_general_compare(const gen_compare_launch& args, hipStream_t stream);
 
 }  // namespace dust2_amd

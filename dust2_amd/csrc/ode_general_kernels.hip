@@ -612,7 +612,54 @@ __device__ void initialise(gctx& c, double t, gstate<M>& s, const ode_control_de
   h = fmax(h, ctl.step_size_min * 100);
   h = fmin(h, ctl.step_size_max);
   s.step_size = h;
-  for (int e = 0; e < gstate<M>::ME; ++e) s.last_event[e] = -__builtin_inf();
+}
+
+// internals::reset then solver::initialise at t (solver.hpp:62-70,
+// 264-324): the step-time log restarts at t and the history at the
+// current state.
+template <class M>
+__device__ void reinitialise(gctx& c, double t, gstate<M>& s, const ode_control_dev& ctl,
+                             const glogs& lg, double* hist_init) {
+  constexpr int NH = hist_words<M>::NH;
+  if (lg.rec.times) lg.rec.count[lg.i] = 0;
+  record_time(lg.rec, lg.i, t);
+  for (int j = 0; j < NH; ++j) hist_init[j] = s.y[hist_var<M>(j)];
+  s.hist_count = 0;
+  s.hist_first_t0 = 0;
+  c.hist_count = 0;
+  c.hist_first_t0 = 0;
+  initialise<M>(c, t, s, ctl);
+}
+
+// Mixed time (dust2/continuous/system.hpp:544-585, run_to_time1 with
+// dt > 0), restated step for step: n = (time - time_) / dt steps; step k
+// integrates to time_ + k dt (k < n - 1; the last to `time`), then applies
+// the model's update with the system's RNG stream as it stands (the
+// reference passes the stream by value, so updates do not advance it) and
+// re-initialises the solver at time_ if the update changed the state.
+template <class M>
+__device__ void run_mixed(gctx& c, double time0, double time, double dt, gstate<M>& s,
+                          const ode_control_dev& ctl, const glogs& lg, double* hist_init,
+                          const rng256& rng, bool det) {
+  constexpr int NY = gstate<M>::NY;
+  const uint64_t n_steps = static_cast<uint64_t>((time - time0) / dt);
+  double t1 = time0;
+  for (uint64_t k = 0; k < n_steps && *c.code == 0; ++k) {
+    const double t0 = t1;
+    t1 = (k == n_steps - 1) ? time : time0 + static_cast<double>(k) * dt;
+    run<M>(c, t0, t1, s, ctl, lg);
+    if (*c.code) return;
+    double y_next[NY];
+    for (int i = 0; i < NY; ++i) y_next[i] = s.y[i];
+    rng256 r = rng;
+    M::update(time, dt, s.y, c.p, r, det, y_next);
+    bool same = true;
+    for (int i = 0; i < NY; ++i) {
+      same = same && (y_next[i] == s.y[i]);
+      s.y[i] = y_next[i];
+    }
+    if (!same) reinitialise<M>(c, time0, s, ctl, lg, hist_init);
+  }
 }
 
 template <class M>
@@ -723,24 +770,28 @@ __global__ void __launch_bounds__(256) general_system_kernel(gen_launch a) {
   for (int k = 0; k < NY; ++k) s.y[k] = y_full[k];
   load<M>(s, w);
   if (a.reinit[g]) {
-    if (a.record.times) a.record.count[i] = 0;
-    record_time(a.record, i, a.time);
-    // history reset (history.hpp:123-132)
-    for (int j = 0; j < NH; ++j) {
-      const int v = hist_var<M>(j);
-      w[L.hist_init + j] = s.y[v];
-    }
-    s.hist_count = 0;
-    s.hist_first_t0 = 0;
-    c.hist_count = 0;
-    c.hist_first_t0 = 0;
-    initialise<M>(c, a.time, s, ctl);
+    // initialise_solver_ (system.hpp:444-482): initialise, then
+    // initialise_events
+    reinitialise<M>(c, a.time, s, ctl, lg, w + L.hist_init);
+    for (int e = 0; e < gstate<M>::ME; ++e) s.last_event[e] = -__builtin_inf();
   }
   c.hist_count = s.hist_count;
   c.hist_first_t0 = s.hist_first_t0;
+  rng256 rng{};
+  if constexpr (M::mixed) {
+    if (a.rng) {
+      const uint64_t* r = a.rng + 4 * static_cast<size_t>(i);
+      rng = rng256{r[0], r[1], r[2], r[3]};
+    }
+  }
   double time = a.time;
   for (int q = 0; q < a.n_targets && code == 0; ++q) {
-    run<M>(c, time, a.targets[q], s, ctl, lg);
+    if (M::mixed && a.dt > 0) {
+      run_mixed<M>(c, time, a.targets[q], a.dt, s, ctl, lg, w + L.hist_init, rng,
+                   a.deterministic != 0);
+    } else {
+      run<M>(c, time, a.targets[q], s, ctl, lg);
+    }
     time = a.targets[q];
     for (int k = 0; k < NY; ++k) y_full[k] = s.y[k];
     compute_output<M>(c, time, y_full);
@@ -756,6 +807,21 @@ __global__ void __launch_bounds__(256) general_system_kernel(gen_launch a) {
     a.errors[i] = error_record{code, 0u, time, 0.0};
     atomicAdd(a.error_count, 1u);
   }
+}
+
+// compare_data (dust2/continuous/system.hpp:287-312): the model's log
+// density of one data row (per group, or shared) at each particle's state.
+template <class M>
+__global__ void __launch_bounds__(256) general_compare_kernel(gen_compare_launch a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_total) return;
+  const int g = i / a.n_particles;
+  constexpr int NS = M::n_var + M::n_special + M::n_output;
+  const size_t NT = static_cast<size_t>(a.n_total);
+  double y[NS];
+  for (int k = 0; k < NS; ++k) y[k] = a.state[k * NT + i];
+  const double* d = a.data + static_cast<size_t>(a.shared_data ? 0 : g) * M::n_data;
+  a.out[i] = M::compare(a.time, y, d, a.pars + static_cast<size_t>(g) * a.n_pars);
 }
 
 template <class M>
@@ -780,6 +846,18 @@ int gen_history_words(int model) {
   return 3 + 5 * nh;
 }
 
+hipError_t launch_general_compare(const gen_compare_launch& a, hipStream_t stream) {
+  if (a.n_total == 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((a.n_total + 255) / 256);
+  switch (a.model) {
+    case odem::MODEL_MALARIA:
+      hipLaunchKernelGGL(general_compare_kernel<odem::malaria>, dim3(blocks), dim3(256), 0,
+                         stream, a);
+      return hipGetLastError();
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_general_system(const gen_launch& a, hipStream_t stream) {
   if (a.n_total == 0) return hipSuccess;
   if (a.words != gen_words(a.model)) return hipErrorInvalidValue;
@@ -790,6 +868,8 @@ hipError_t launch_general_system(const gen_launch& a, hipStream_t stream) {
     case odem::MODEL_SPECIALEVENT: return launch_model<odem::specialevent>(a, stream);
     case odem::MODEL_SIRDELAY: return launch_model<odem::sirdelay>(a, stream);
     case odem::MODEL_LOGDELAY: return launch_model<odem::logdelay>(a, stream);
+    case odem::MODEL_SPECIAL: return launch_model<odem::special>(a, stream);
+    case odem::MODEL_MALARIA: return launch_model<odem::malaria>(a, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -241,7 +241,49 @@ def logdelay():
                      {"tau"}, ["y"], precisions=("fp64",), events=lambda row: [])
 
 
-GENERAL_MODELS = ("bounce", "change", "eventzero", "specialevent", "sirdelay", "logdelay")
+def special():
+    """Mixed time: a is the integral of a carried rate b, and with dt every
+    update moves b by a normal step of sd (tests/testthat/examples/
+    special.cpp)."""
+    return Generator("special", "mixed", ["sd"], {"sd": 1.0}, {"sd"}, ["a", "b"],
+                     precisions=("fp64",), events=lambda row: [])
+
+
+MALARIA_RATES = 15
+
+
+def _malaria_normalise(p):
+    # build_shared (inst/examples/malaria.cpp:160-176): bh, bv and the
+    # vector death rate mu = -log(0.9) are fixed
+    out = dict(p)
+    if int(out.get("n_rates", MALARIA_RATES)) != MALARIA_RATES:
+        raise ValueError(f"'n_rates' must be {MALARIA_RATES} (fixed at build time)")
+    out["n_rates"] = float(MALARIA_RATES)
+    out.setdefault("bh", 0.05)
+    out.setdefault("bv", 0.05)
+    out.setdefault("mu", -np.log(0.9))
+    return out
+
+
+def malaria():
+    """Mixed time: host-vector malaria whose vector birth rate beta follows a
+    log-normal random walk with every dt, compared to binomial prevalence
+    data (inst/examples/malaria.cpp)."""
+    state = (["Sh", "Ih", "Sv", "Iv"] + [f"Ev[{i + 1}]" for i in range(MALARIA_RATES)]
+             + ["beta"])
+    return Generator("malaria", "mixed",
+                     ["a", "bh", "bv", "n_rates", "mu", "r", "tau", "initial_Ih",
+                      "initial_Iv", "initial_Sv", "beta_volatility"],
+                     {"a": 1.0 / 3.0, "bh": 0.05, "bv": 0.05, "n_rates": 15.0,
+                      "mu": -np.log(0.9), "r": 0.01, "tau": 12.0, "initial_Ih": 0.8,
+                      "initial_Iv": 18.0, "initial_Sv": 100.0, "beta_volatility": 0.5},
+                     {"bh", "bv", "n_rates", "mu", "initial_Ih", "initial_Iv", "initial_Sv",
+                      "beta_volatility"}, state, n_data=2, precisions=("fp64",),
+                     normalise=_malaria_normalise, events=lambda row: [])
+
+
+GENERAL_MODELS = ("bounce", "change", "eventzero", "specialevent", "sirdelay", "logdelay",
+                  "special", "malaria")
 
 
 def _precision(p):
@@ -756,16 +798,18 @@ class GeneralSystem:
     the history ring stay on the device between calls."""
     prefix = "dust2_system_ode"
 
-    def __init__(self, generator, pars, n_particles, n_groups, time, ode_control,
-                 history_cap=None, event_cap=1024):
+    def __init__(self, generator, pars, n_particles, n_groups, time, ode_control, dt=None,
+                 seed=None, deterministic=False, history_cap=None, event_cap=1024):
         self.generator = generator
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
-        sizes = np.zeros(7, dtype=np.uintp)
+        sizes = np.zeros(9, dtype=np.uintp)
         call("dust2_system_ode_model", generator.name.encode(), szptr(sizes))
         (self.n_var, self.n_special, self.n_output, n_pars, self.max_events,
-         self.n_hist, has_delays) = (int(v) for v in sizes)
+         self.n_hist, has_delays, mixed, self.n_data) = (int(v) for v in sizes)
         self.has_delays = bool(has_delays)
+        self.mixed = bool(mixed)
+        self.dt = 0.0 if dt is None else float(dt)
         self.rows = pars_matrix(generator, pars, self.n_groups)
         if self.rows.shape[1] != n_pars:
             raise ValueError(f"Parameter layout of '{generator.name}' does not match the library")
@@ -777,11 +821,13 @@ class GeneralSystem:
             history_cap = 100000 if ctl.save_history else 4096
         self.history_cap = int(history_cap) if self.save_history else 0
         self.event_cap = int(event_cap) if self.max_events else 0
+        raw, n_raw, s = _seed_args(seed)
         out = C.c_void_p()
         call("dust2_system_ode_alloc", generator.name.encode(), dptr(self.rows), self.n_groups,
-             self.n_particles, float(time), dptr(ctl.values), dptr(ctl.critical_times),
-             ctl.critical_times.size, self.record, int(ctl.save_history), self.history_cap,
-             self.event_cap, C.byref(out))
+             self.n_particles, float(time), self.dt, dptr(ctl.values),
+             dptr(ctl.critical_times), ctl.critical_times.size, self.record,
+             int(ctl.save_history), self.history_cap, self.event_cap, u64ptr(raw), n_raw, s,
+             int(bool(deterministic)), C.byref(out))
         self.ptr = out
 
     def __del__(self):
@@ -799,6 +845,11 @@ def dust_system_create(generator, pars=None, n_particles=1, n_groups=1, time=0, 
                        **kwargs):
     if pars is None:
         pars = {}
+    if generator.time_type == "mixed":
+        # dt given: the model's update runs every dt; none: continuous only
+        # (R/interface.R:600-620)
+        return GeneralSystem(generator, pars, n_particles, n_groups, time, ode_control, dt=dt,
+                             seed=seed, deterministic=deterministic, **kwargs)
     if generator.time_type == "continuous":
         if dt is not None:
             raise ValueError("Can't use 'dt' with continuous-time systems")
@@ -859,7 +910,19 @@ def dust_system_update_pars(sys, pars):
 
 
 def dust_system_compare_data(sys, data):
-    """data: one value (shared) or one per group; returns (group, particle)."""
+    """data: one value (shared) or one per group; returns (group, particle).
+    For a model with several data fields (malaria: tested, positive), a dict
+    of them or an array [group][field]."""
+    if isinstance(sys, GeneralSystem):
+        if isinstance(data, dict):
+            names = ["tested", "positive"]
+            d = np.column_stack([np.atleast_1d(f64(data.get(k, np.nan))) for k in names])
+        else:
+            d = np.atleast_2d(f64(data))
+        d = np.ascontiguousarray(d)
+        out = np.zeros((sys.n_groups, sys.n_particles))
+        call("dust2_system_ode_compare_data", sys.ptr, dptr(d), d.shape[0], dptr(out))
+        return out
     d = f64(data).reshape(-1)
     out = np.zeros((sys.n_groups, sys.n_particles))
     call(_prefix(sys) + "_compare_data", sys.ptr, dptr(d), d.size, dptr(out))
@@ -932,13 +995,17 @@ def dust_system_internals(sys, include_history=False):
 
 def dust_system_rng_state(sys):
     out = np.zeros(sys.n_groups * sys.n_particles * 4, dtype=np.uint64)
-    call("dust2_system_sir_rng_state", sys.ptr, u64ptr(out))
+    name = "dust2_system_ode_rng_state" if isinstance(sys, GeneralSystem) else \
+        "dust2_system_sir_rng_state"
+    call(name, sys.ptr, u64ptr(out))
     return out
 
 
 def dust_system_set_rng_state(sys, state):
     st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
-    call("dust2_system_sir_set_rng_state", sys.ptr, u64ptr(st))
+    name = "dust2_system_ode_set_rng_state" if isinstance(sys, GeneralSystem) else \
+        "dust2_system_sir_set_rng_state"
+    call(name, sys.ptr, u64ptr(st))
 
 
 # ---------------------------------------------------------------------------

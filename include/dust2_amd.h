@@ -333,8 +333,10 @@ void dust2_system_sirode_free(dust2_ode_system* h);
  * resets and delays, chosen by name at alloc: "bounce"
  * (tests/testthat/examples/event.cpp), "change" (event-time.cpp),
  * "eventzero" (event-zero.cpp), "specialevent" (specialevent.cpp),
- * "sirdelay" (delay.cpp) and "logdelay" (a delayed logistic equation whose
- * rhs reads the delayed state).  They replace the per-model
+ * "sirdelay" (delay.cpp), "logdelay" (a delayed logistic equation whose
+ * rhs reads the delayed state) and the mixed-time models "special"
+ * (special.cpp) and "malaria" (inst/examples/malaria.cpp), which with
+ * dt > 0 apply their discrete update every dt (system.hpp:544-585).  They replace the per-model
  * dust2_system_<name>_* registrations that inst/template/system.cpp:1-54
  * generates over dust_continuous (dust2/continuous/system.hpp:32-585) with
  * solver::step / run / apply_events (dust2/continuous/solver.hpp:152-262,
@@ -342,23 +344,37 @@ void dust2_system_sirode_free(dust2_ode_system* h);
  * delays.hpp); an R shim maps dust2_system_bounce_alloc(...) to
  * dust2_system_ode_alloc("bounce", ...).  FP64.
  *
- * sizes[7] from dust2_system_ode_model: n_var, n_special, n_output, n_pars,
- * max_events, history variables, has_delays.  State [G][P][S] with
+ * sizes[9] from dust2_system_ode_model: n_var, n_special, n_output, n_pars,
+ * max_events, history variables, has_delays, mixed, n_data.  State [G][P][S] with
  * S = n_var + n_special + n_output (the reference's packing order);
  * pars[G][n_pars]. */
 int dust2_system_ode_model(const char* model, size_t* sizes, char* err, size_t errlen);
-/* control: the five dust_ode_control() values or NULL; critical_times;
+/* dt: 0, or for a mixed-time model the update interval (an integer or the
+ * inverse of one); seed_state / seed / deterministic as
+ * dust2_system_sir_alloc, for the streams of mixed-time updates.
+ * control: the five dust_ode_control() values or NULL; critical_times;
  * record_step_times: step times kept per particle (0: off); save_history
  * (control save_history; always on for a model with delays) with
  * history_cap steps kept per particle, the most recent; event_cap: events
  * kept per particle (the first).  Errors lock the system as the reference
  * does (dust2/r/system.hpp:9-16). */
 int dust2_system_ode_alloc(const char* model, const double* pars, size_t n_groups,
-                           size_t n_particles, double time, const double* control,
+                           size_t n_particles, double time, double dt, const double* control,
                            const double* critical_times, size_t n_critical,
                            size_t record_step_times, size_t save_history, size_t history_cap,
-                           size_t event_cap, dust2_ode_general** out, char* err,
+                           size_t event_cap, const uint64_t* seed_state, size_t n_seed_streams,
+                           uint64_t seed, int deterministic, dust2_ode_general** out, char* err,
                            size_t errlen);
+/* the particles' random number streams ([G*P][4] words, stream k for
+ * particle k) of a mixed-time system (dust2/r/system.hpp rng_state /
+ * set_rng_state) */
+int dust2_system_ode_rng_state(dust2_ode_general* h, uint64_t* out, char* err, size_t errlen);
+int dust2_system_ode_set_rng_state(dust2_ode_general* h, const uint64_t* state, char* err,
+                                   size_t errlen);
+/* compare_data (dust2/continuous/system.hpp:287-312) for a model with data
+ * (malaria: tested, positive): data[n_groups_data][n_data], out[G*P]. */
+int dust2_system_ode_compare_data(dust2_ode_general* h, const double* data,
+                                  size_t n_groups_data, double* out, char* err, size_t errlen);
 int dust2_system_ode_run_to_time(dust2_ode_general* h, double time, char* err, size_t errlen);
 int dust2_system_ode_simulate(dust2_ode_general* h, const double* times, size_t n_times,
                               double* out, char* err, size_t errlen);

@@ -13,6 +13,8 @@
 #include <dust2/filter_details.hpp>
 #include <dust2/interpolate/interpolate.hpp>
 
+#include "oracle.hpp"  // xoshiro256+ streams and the normal sampler (monty is absent)
+
 #include <cstdint>
 #include <functional>
 #include <string>
@@ -417,6 +419,113 @@ int ref_ode_general(int model, const double* p, const double* y0, double time_st
       e[3 + 4 * H + j] = ode ? hd[i].c5[j] : 0;
     }
   }
+  return 0;
+}
+
+// A mixed-time model (the reference's special and malaria examples,
+// tests/testthat/examples/special.cpp, inst/examples/malaria.cpp) through the
+// reference solver, with the system's mixed-time loop restated from
+// dust2/continuous/system.hpp:544-585 (run_to_time1 with dt > 0) and the
+// model's update drawing from a copy of the particle's stream (the
+// reference passes the stream by value).  Model ids and parameter rows as
+// dust2_amd/csrc/ode_models.hpp; updates use the oracle's normal sampler
+// (monty is not in the reference tree).  rng: the particle's stream, 4
+// words.  dt = 0: continuous time only.
+int ref_ode_mixed(int model, const double* p, const double* y0, double time_start, double dt,
+                  const double* targets, size_t n_targets, const uint64_t* rng,
+                  int deterministic, double* out_state, double* out_stats, double* step_times,
+                  size_t cap, size_t* n_steps, char* err, size_t errlen) {
+  using dust2::ode::internals;
+  using dust2::ode::solver;
+  size_t n_var = 0;
+  std::vector<double> y;
+  std::function<void(double, const double*, double*)> rhs;
+  std::function<void(const double*, oracle::rng_state&, double*)> update;
+  if (model == 6) {  // special: sd
+    n_var = 1;
+    y = {0, 0};
+    rhs = [](double, const double* s, double* d) { d[0] = s[1]; };
+    update = [p](const double* s, oracle::rng_state& r, double* next) {
+      next[1] = oracle::normal(r, s[1], p[0]);
+    };
+  } else if (model == 7) {  // malaria
+    const int n_rates = static_cast<int>(p[3]);
+    n_var = 4 + n_rates;
+    y.assign(n_var + 1, 0.0);
+    y[0] = 1 - p[7];
+    y[1] = p[7];
+    y[2] = p[9];
+    y[3] = p[8];
+    y[4 + n_rates] = p[4];
+    rhs = [p, n_rates](double, const double* s, double* d) {
+      const double a = p[0], bh = p[1], bv = p[2], n = p[3], mu = p[4], r = p[5], tau = p[6];
+      const double Sh = s[0], Ih = s[1], Sv = s[2], Iv = s[3];
+      const double* Ev = s + 4;
+      const double beta = s[4 + n_rates];
+      const double foi_h = a * bh * Iv;
+      const double foi_v = a * bv * Ih;
+      d[4] = foi_v * Sv - (n / tau) * Ev[0] - mu * Ev[0];
+      for (int i = 1; i < n_rates; ++i) {
+        d[4 + i] = (n / tau) * Ev[i - 1] - (n / tau) * Ev[i] - mu * Ev[i];
+      }
+      d[0] = -foi_h * Sh + r * Ih;
+      d[1] = foi_h * Sh - r * Ih;
+      d[2] = beta * p[9] - foi_v * Sv - mu * Sv;
+      d[3] = (n / tau) * Ev[n_rates - 1] - mu * Iv;
+    };
+    update = [p, n_rates](const double* s, oracle::rng_state& r, double* next) {
+      const int i = 4 + n_rates;
+      next[i] = s[i] * std::exp(oracle::normal(r, 0, p[10]));
+    };
+  } else {
+    std::snprintf(err, errlen, "unknown model");
+    return 1;
+  }
+  if (y0) std::copy(y0, y0 + y.size(), y.begin());
+  const size_t n_special = 1, NY = y.size();
+  dust2::ode::control<double> ctl;
+  ctl.debug_record_step_times = true;
+  solver<double> s(n_var, n_special, ctl);
+  internals<double> in(n_var, n_special, false);
+  dust2::zero_every_type<double> zero_every;
+  dust2::ode::events_type<double> events;
+  oracle::rng_state stream{{rng[0], rng[1], rng[2], rng[3]}, deterministic != 0};
+  double time_ = time_start;
+  try {
+    s.initialise(time_, y.data(), in, rhs);
+    s.initialise_events(events, in);
+    for (size_t q = 0; q < n_targets; ++q) {
+      const double time = targets[q];
+      if (dt == 0) {
+        s.run(time_, time, y.data(), zero_every, events, in, rhs);
+      } else {
+        const size_t steps = (time - time_) / dt;
+        double t1 = time_;
+        for (size_t k = 0; k < steps; ++k) {
+          const double t0 = t1;
+          t1 = (k == steps - 1) ? time : time_ + k * dt;
+          s.run(t0, t1, y.data(), zero_every, events, in, rhs);
+          std::vector<double> y_other(y.begin(), y.begin() + NY);
+          oracle::rng_state r = stream;
+          update(y.data(), r, y_other.data());
+          const bool same = std::equal(y.begin(), y.begin() + NY, y_other.begin());
+          std::copy(y_other.begin(), y_other.end(), y.begin());
+          if (!same) s.initialise(time_, y.data(), in, rhs);
+        }
+      }
+      time_ = time;
+      std::copy(y.begin(), y.end(), out_state + q * NY);
+    }
+  } catch (const std::exception& e) {
+    if (err && errlen) std::snprintf(err, errlen, "%s", e.what());
+    return 1;
+  }
+  out_stats[0] = in.n_steps;
+  out_stats[1] = in.n_steps_accepted;
+  out_stats[2] = in.n_steps_rejected;
+  out_stats[3] = in.step_size;
+  *n_steps = in.step_times.size();
+  for (size_t i = 0; i < std::min(cap, in.step_times.size()); ++i) step_times[i] = in.step_times[i];
   return 0;
 }
 

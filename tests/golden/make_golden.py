@@ -192,6 +192,44 @@ def ode_general_fixture():
     return {"cases": cases}
 
 
+MIXED_CASES = [
+    # name, model, user parameters, initial states per particle (None: the
+    # model's), targets, dt, deterministic
+    ("special_dt1", "special", {}, [[0.0, -1.0], [2.0, 0.0], [4.0, 1.0]], np.arange(0.0, 6.0),
+     1.0, False),
+    ("special_quarter", "special", {"sd": 0.3}, [[0.0, 0.5], [1.0, -0.2]],
+     [0.0, 1.0, 2.5, 3.0, 3.1, 5.0], 0.25, False),
+    ("special_deterministic", "special", {"sd": 2.0}, [[1.0, 0.7]], np.arange(0.0, 4.0), 0.5,
+     True),
+    ("malaria_dt1", "malaria", {}, None, np.arange(0.0, 21.0), 1.0, False),
+    ("malaria_continuous", "malaria", {"a": 0.4}, None, np.arange(0.0, 21.0), 0.0, False),
+]
+
+
+def ode_mixed_fixture():
+    """Mixed-time models through the reference solver with the system's
+    mixed-time loop restated (oracle/_ref): per particle, its stream, the
+    states at the targets, counters and step times."""
+    import dust2_amd
+    from dust2_amd.interface import pars_matrix
+    cases = []
+    for name, model, pars, y0, targets, dt, det in MIXED_CASES:
+        row = pars_matrix(getattr(dust2_amd, model)(), pars, 1)[0]
+        n = len(y0) if y0 is not None else 3
+        streams = ol.prng(n, seed=17)
+        parts = []
+        for k in range(n):
+            r = ol.ref_ode_mixed(model, row, targets, streams[k],
+                                 y0=None if y0 is None else y0[k], dt=dt, deterministic=det)
+            parts.append({"rng": [int(w) for w in streams[k]], "state": r["state"].tolist(),
+                          "stats": r["stats"].tolist(), "step_times": r["step_times"].tolist()})
+        cases.append({"name": name, "model": model,
+                      "pars": {k: float(v) for k, v in pars.items()}, "row": row.tolist(),
+                      "y0": y0, "targets": list(map(float, targets)), "dt": dt,
+                      "deterministic": det, "particles": parts})
+    return {"cases": cases}
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref/libdust2ref.so is not built (needs the reference tree)")
@@ -207,6 +245,8 @@ def main():
         json.dump(sirode_system_fixture(), f)
     with open(os.path.join(HERE, "ode_general_dp5.json"), "w") as f:
         json.dump(ode_general_fixture(), f)
+    with open(os.path.join(HERE, "ode_mixed_dp5.json"), "w") as f:
+        json.dump(ode_mixed_fixture(), f)
 
 
 if __name__ == "__main__":

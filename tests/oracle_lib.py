@@ -466,3 +466,29 @@ def ref_ode_general(model, row, targets, y0=None, control=None, critical_times=(
             "events": ev[:3 * n_ev].reshape(n_ev, 3).copy(),
             "history": hist[:n_h * (3 + 5 * H)].reshape(n_h, 3 + 5 * H).copy()}
 
+
+def ref_ode_mixed(model, row, targets, rng, y0=None, dt=1.0, deterministic=False,
+                  time_start=0.0):
+    """One particle of a mixed-time model (special, malaria) through the
+    reference solver with the mixed-time loop of the continuous system
+    restated (oracle/_ref); rng: the particle's stream (4 words)."""
+    ids = {"special": 6, "malaria": 7}
+    S = {"special": 2, "malaria": 20}[model]
+    row = np.ascontiguousarray(row, dtype=np.float64)
+    targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1)
+    r = np.ascontiguousarray(rng, dtype=np.uint64).reshape(4)
+    y0a = None if y0 is None else np.ascontiguousarray(y0, dtype=np.float64)
+    state = np.zeros((targets.size, S))
+    stats = np.zeros(4)
+    cap = 200000
+    times = np.zeros(cap)
+    n = np.zeros(1, dtype=np.uintp)
+    buf = C.create_string_buffer(512)
+    rc = ref().ref_ode_mixed(
+        C.c_int(ids[model]), _dp(row), _dp(y0a) if y0a is not None else None,
+        C.c_double(time_start), C.c_double(dt), _dp(targets), C.c_size_t(targets.size),
+        _u64(r), C.c_int(int(deterministic)), _dp(state), _dp(stats), _dp(times),
+        C.c_size_t(cap), n.ctypes.data_as(SZ), buf, 512)
+    _check(rc, buf)
+    return {"state": state, "stats": stats, "step_times": times[:int(n[0])].copy()}
+

@@ -1,0 +1,128 @@
+"""Pins the golden vectors of the mixed-time models
+(tests/golden/ode_mixed_dp5.json, written by the reference solver with the
+continuous system's mixed-time loop, dust2/continuous/system.hpp:544-585,
+restated): closed forms of the special model, the update draws against
+the oracle's normal sampler on each particle's stream, the malaria ODE
+against an independent integration and, where the reference tree is built
+into oracle/_ref, a fresh run value for value."""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+import oracle_lib as ol
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ode_mixed_dp5.json")
+
+
+def cases():
+    with open(GOLDEN) as f:
+        return {c["name"]: c for c in json.load(f)["cases"]}
+
+
+def test_special_updates_are_normal_draws_of_the_stream():
+    # special.cpp: b_next = normal(b, sd).  The update takes a copy of the
+    # particle's stream, so every update adds the same first normal draw of
+    # that stream; a integrates b exactly (da/dt = b, constant per step)
+    for name in ["special_dt1", "special_quarter"]:
+        c = cases()[name]
+        sd = c["pars"].get("sd", 1.0)
+        for k, p in enumerate(c["particles"]):
+            y = np.array(p["state"])
+            z, _ = ol.sample("normal", p["rng"], [0.0], [sd])
+            steps = np.diff(y[:, 1])
+            moved = steps != 0
+            np.testing.assert_allclose(steps[moved], z[0] * np.round(steps[moved] / z[0]),
+                                       rtol=1e-12)
+            if name == "special_dt1":
+                # one update per unit call: b moves by z every call, and a
+                # gains the b it held over the interval
+                np.testing.assert_allclose(steps, z[0], rtol=1e-12)
+                np.testing.assert_allclose(np.diff(y[:, 0]), y[:-1, 1], rtol=1e-10, atol=1e-12)
+
+
+def test_special_update_schedule():
+    # a call from time_ to time takes n = (time - time_) / dt steps: the
+    # first integrates to time_ (nothing), step k to time_ + k dt and the
+    # last to time, an update after each; so a call of n >= 1 steps moves b
+    # by n draws and integrates a over the whole interval with the b held
+    # at each step start
+    c = cases()["special_quarter"]
+    t = np.array(c["targets"])
+    for p in c["particles"]:
+        y = np.array(p["state"])
+        z, _ = ol.sample("normal", p["rng"], [0.0], [0.3])
+        for q in range(1, t.size):
+            n = int((t[q] - t[q - 1]) / 0.25)
+            assert y[q, 1] == pytest.approx(y[q - 1, 1] + n * z[0], rel=1e-12, abs=1e-12)
+            if n >= 1:
+                # the b values in force: b, b+z, ..., over step ends
+                # time_, time_ + dt, ..., time_ + (n-2) dt, time
+                ends = [t[q - 1] + k * 0.25 for k in range(n - 1)] + [t[q]]
+                starts = [t[q - 1]] + ends[:-1]
+                b = y[q - 1, 1] + z[0] * np.arange(n)
+                want = y[q - 1, 0] + np.sum(b * (np.array(ends) - np.array(starts)))
+                assert y[q, 0] == pytest.approx(want, rel=1e-10, abs=1e-12)
+            else:
+                assert y[q, 0] == y[q - 1, 0]
+
+
+def test_special_deterministic_keeps_the_rate():
+    c = cases()["special_deterministic"]
+    y = np.array(c["particles"][0]["state"])
+    t = np.array(c["targets"])
+    np.testing.assert_array_equal(y[:, 1], 0.7)
+    np.testing.assert_allclose(y[:, 0], 1.0 + 0.7 * t, rtol=1e-12)
+
+
+def malaria_rhs(p):
+    n = 15
+
+    def f(_, s):
+        Sh, Ih, Sv, Iv = s[:4]
+        Ev = s[4:4 + n]
+        beta = s[4 + n]
+        foi_h = p["a"] * 0.05 * Iv
+        foi_v = p["a"] * 0.05 * Ih
+        mu = -np.log(0.9)
+        d = np.zeros(n + 5)
+        d[4] = foi_v * Sv - (n / p["tau"]) * Ev[0] - mu * Ev[0]
+        d[5:4 + n] = (n / p["tau"]) * Ev[:-1] - (n / p["tau"]) * Ev[1:] - mu * Ev[1:]
+        d[0] = -foi_h * Sh + p["r"] * Ih
+        d[1] = foi_h * Sh - p["r"] * Ih
+        d[2] = beta * 100 - foi_v * Sv - mu * Sv
+        d[3] = (n / p["tau"]) * Ev[-1] - mu * Iv
+        return d
+    return f
+
+
+def test_malaria_against_independent_integration():
+    for name in ["malaria_continuous", "malaria_dt1"]:
+        c = cases()[name]
+        p = dict({"a": 1 / 3, "r": 0.01, "tau": 12.0}, **c["pars"])
+        t = np.array(c["targets"])
+        for part in c["particles"]:
+            y = np.array(part["state"])
+            # between targets the ODE runs with the beta in force over the
+            # interval (before that interval's update)
+            for q in range(1, t.size):
+                y0 = y[q - 1].copy()
+                sol = solve_ivp(malaria_rhs(p), (t[q - 1], t[q]), y0, rtol=1e-11, atol=1e-12,
+                                method="DOP853")
+                np.testing.assert_allclose(y[q, :19], sol.y[:19, -1], rtol=1e-5, atol=1e-7)
+            if name == "malaria_continuous":
+                np.testing.assert_allclose(y[:, 19], -np.log(0.9), rtol=1e-15)
+
+
+@pytest.mark.skipif(not ol.ref_available(), reason="reference tree not built into oracle/_ref")
+@pytest.mark.parametrize("name", sorted(cases()))
+def test_golden_is_the_reference_run(name):
+    c = cases()[name]
+    for k, p in enumerate(c["particles"]):
+        r = ol.ref_ode_mixed(c["model"], c["row"], c["targets"], np.array(p["rng"], np.uint64),
+                             y0=None if c["y0"] is None else c["y0"][k], dt=c["dt"],
+                             deterministic=c["deterministic"])
+        np.testing.assert_array_equal(r["state"], p["state"])
+        np.testing.assert_array_equal(r["stats"], p["stats"])
