@@ -374,6 +374,7 @@ enum handle_tag : uint32_t {
   TAG_UNFILTER_SIR = 0x64327532u,
   TAG_ODE_SYSTEM = 0x64327332u,
   TAG_ODE_GENERAL = 0x64327333u,
+  TAG_FILTER_ODE = 0x64326632u,
   TAG_SERVER = 0x64327631u,
 };
 
@@ -3288,6 +3289,273 @@ int dust2_system_ode_history(dust2_ode_general* h, size_t* counts, double* entri
 
 void dust2_system_ode_free(dust2_ode_general* h) {
   if (handle_ok(h, TAG_ODE_GENERAL)) delete h;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Particle filter over a general continuous / mixed-time model
+// (dust2/filter.hpp:42-175 over dust_continuous)
+// ============================================================================
+struct dust2_filter_ode {
+  uint32_t tag = TAG_FILTER_ODE;
+  dust2_ode_general* sys = nullptr;   // owned
+  size_t G = 0, P = 0, n_time = 0, n_groups_data = 0;
+  double time_start = 0;
+  std::vector<double> times;
+  dbuf d_data, d_frng, d_ll, d_ll_step, d_idx, d_state2, d_words2, d_traj, d_ancestry,
+      d_fallback, d_groups, d_out;
+  bool traj_current = false;
+  ~dust2_filter_ode() {
+    if (sys) dust2_system_ode_free(sys);
+  }
+};
+
+namespace {
+
+void check_filter_ode(dust2_filter_ode* h) { check_handle(h, TAG_FILTER_ODE, "filter"); }
+
+}  // namespace
+
+extern "C" {
+
+int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_groups,
+                           size_t n_particles, double time_start, const double* time,
+                           size_t n_time, const double* data, size_t n_groups_data, double dt,
+                           const double* control, const uint64_t* seed_state,
+                           size_t n_seed_streams, uint64_t seed, dust2_filter_ode** out,
+                           char* err, size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    const int id = general_model_id(model);
+    const odem::model_info info = odem::info(id);
+    if (info.n_data == 0) throw std::runtime_error("This model has no compare_data");
+    if (info.has_delays) throw std::runtime_error("Filters over models with delays are not built");
+    if (!info.mixed) {
+      throw std::runtime_error("Can't use 'dust_filter_create()' with continuous-time models");
+    }
+    if (n_particles > 8192) throw std::runtime_error("At most 8192 particles per group");
+    if ((kScratchDoubles + n_particles) * sizeof(double) >
+        static_cast<size_t>(max_lds_per_block())) {
+      throw std::runtime_error("Too many particles per group for the resampling workgroup");
+    }
+    check_time_sequence(time_start, time, n_time);
+    if (n_groups_data != 1 && n_groups_data != n_groups) {
+      throw std::runtime_error("Expected data for one group or every group");
+    }
+    auto h = std::make_unique<dust2_filter_ode>();
+    h->G = n_groups;
+    h->P = n_particles;
+    h->n_time = n_time;
+    h->n_groups_data = n_groups_data;
+    h->time_start = time_start;
+    h->times.assign(time, time + n_time);
+    const size_t G = n_groups, P = n_particles;
+    // streams [2 filter | P particle] per group (dust2/r/discrete/filter.hpp:63-74)
+    std::vector<uint64_t> all(G * (P + 2) * 4), part(G * P * 4), filt(G * 8);
+    {
+      hipStream_t s = nullptr;
+      hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+      dbuf tmp(all.size() * sizeof(uint64_t));
+      expand_streams(seed_state, n_seed_streams, seed, G * (P + 2), tmp.get<uint64_t>(), s);
+      to_host(all.data(), tmp, all.size() * sizeof(uint64_t), s);
+      (void)hipStreamDestroy(s);
+    }
+    for (size_t g = 0; g < G; ++g) {
+      const uint64_t* blk = all.data() + g * (P + 2) * 4;
+      std::copy(blk, blk + 8, filt.data() + g * 8);
+      std::copy(blk + 8, blk + (P + 2) * 4, part.data() + g * P * 4);
+    }
+    char buf[1024];
+    if (dust2_system_ode_alloc(model, pars, n_groups, n_particles, time_start, dt, control,
+                               nullptr, 0, 0, 0, 0, 0, part.data(), G * P, 0, 0, &h->sys, buf,
+                               sizeof buf) != 0) {
+      throw std::runtime_error(buf);
+    }
+    hipStream_t s = h->sys->stream;
+    const size_t nd = n_time * n_groups_data * static_cast<size_t>(info.n_data);
+    h->d_data.alloc(nd * sizeof(double));
+    to_device(h->d_data, data, nd * sizeof(double), s);
+    h->d_frng.alloc(filt.size() * sizeof(uint64_t));
+    to_device(h->d_frng, filt.data(), filt.size() * sizeof(uint64_t), s);
+    h->d_ll.alloc(G * sizeof(double));
+    h->d_ll_step.alloc(G * P * sizeof(double));
+    h->d_idx.alloc(G * P * sizeof(uint32_t));
+    h->d_state2.alloc(h->sys->d_state.bytes());
+    h->d_words2.alloc(h->sys->d_internals.bytes());
+    h->d_fallback.alloc(sizeof(uint32_t));
+    hip_check(hipStreamSynchronize(s), "sync");
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// filter<T>::run (dust2/filter.hpp:42-104) with reset (:156-175): every
+// group from time_start (initial: [G][P][S] or null for the model's own),
+// per data time run_to_time, compare_data, log-sum-exp, one uniform per
+// group and systematic resampling, then reorder.  ll_out[G].
+int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories,
+                         double* ll_out, char* err, size_t errlen) {
+  try {
+    check_filter_ode(h);
+    dust2_ode_general* sys = h->sys;
+    hipStream_t s = sys->stream;
+    const size_t G = h->G, P = h->P, n_total = G * P;
+    char buf[1024];
+    // reset: set_time(time_start), then the initial state
+    sys->errors_pending = false;
+    sys->time = h->time_start;
+    std::fill(sys->reinit.begin(), sys->reinit.end(), 1);
+    const int rc = initial ? dust2_system_ode_set_state(sys, initial, buf, sizeof buf)
+                           : dust2_system_ode_set_state_initial(sys, buf, sizeof buf);
+    if (rc != 0) throw std::runtime_error(buf);
+    hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), s), "memset");
+    hip_check(hipMemsetAsync(h->d_fallback.get(), 0, h->d_fallback.bytes(), s), "memset");
+    h->traj_current = false;
+    if (save_trajectories) {
+      h->d_traj.alloc(G * h->n_time * sys->S * P * sizeof(double));
+      h->d_ancestry.alloc(G * h->n_time * P * sizeof(uint32_t));
+    }
+    const int nd = sys->info.n_data;
+    for (size_t t = 0; t < h->n_time; ++t) {
+      general_run(sys, &h->times[t], 1, nullptr);
+      gen_compare_launch c{};
+      c.model = sys->model;
+      c.pars = sys->d_pars.get<double>();
+      c.n_pars = sys->info.n_pars;
+      c.n_particles = static_cast<int>(P);
+      c.n_total = static_cast<int>(n_total);
+      c.time = sys->time;
+      c.state = sys->d_state.get<double>();
+      c.data = h->d_data.get<double>() + t * h->n_groups_data * nd;
+      c.shared_data = h->n_groups_data == 1 ? 1 : 0;
+      c.out = h->d_ll_step.get<double>();
+      hip_check(launch_general_compare(c, s), "compare kernel");
+      gen_resample_launch r{};
+      r.ll_step = h->d_ll_step.get<double>();
+      r.filter_rng = h->d_frng.get<uint64_t>();
+      r.ll = h->d_ll.get<double>();
+      r.idx = h->d_idx.get<uint32_t>();
+      r.n_groups = static_cast<int>(G);
+      r.n_particles = static_cast<int>(P);
+      r.fallback_count = h->d_fallback.get<uint32_t>();
+      hip_check(launch_general_resample(r, s), "resample kernel");
+      gen_reorder_launch o{};
+      o.idx = h->d_idx.get<uint32_t>();
+      o.state_in = sys->d_state.get<double>();
+      o.state_out = h->d_state2.get<double>();
+      o.S = static_cast<int>(sys->S);
+      o.words_in = sys->d_internals.get<double>();
+      o.words_out = h->d_words2.get<double>();
+      o.words = sys->words;
+      o.n_particles = static_cast<int>(P);
+      o.n_total = static_cast<int>(n_total);
+      o.traj = save_trajectories ? h->d_traj.get<double>() : nullptr;
+      o.ancestry = save_trajectories ? h->d_ancestry.get<uint32_t>() : nullptr;
+      o.n_time = static_cast<int>(h->n_time);
+      o.t = static_cast<int>(t);
+      hip_check(launch_general_reorder(o, s), "reorder kernel");
+      sys->d_state.swap(h->d_state2);
+      sys->d_internals.swap(h->d_words2);
+    }
+    to_host(ll_out, h->d_ll, G * sizeof(double), s);
+    h->traj_current = save_trajectories != 0;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_ode_update_pars(dust2_filter_ode* h, const double* pars, char* err,
+                                 size_t errlen) {
+  try {
+    check_filter_ode(h);
+    return dust2_system_ode_update_pars(h->sys, pars, err, errlen);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, size_t errlen) {
+  try {
+    check_filter_ode(h);
+    h->sys->errors_pending = false;
+    return dust2_system_ode_state(h->sys, out, err, errlen);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// trajectories following the ancestry (dust2/trajectories.hpp:128-189):
+// out [n_time][G][P][S]
+int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
+                                       size_t errlen) {
+  try {
+    check_filter_ode(h);
+    if (!h->traj_current) throw std::runtime_error("Trajectories are not current");
+    hipStream_t s = h->sys->stream;
+    const size_t G = h->G, P = h->P, S = h->sys->S, T = h->n_time;
+    std::vector<uint32_t> groups(G);
+    for (size_t g = 0; g < G; ++g) groups[g] = static_cast<uint32_t>(g);
+    h->d_groups.alloc(G * sizeof(uint32_t));
+    to_device(h->d_groups, groups.data(), G * sizeof(uint32_t), s);
+    h->d_out.alloc(T * G * P * S * sizeof(double));
+    hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(),
+                             h->d_groups.get<uint32_t>(), static_cast<int>(G),
+                             static_cast<int>(T), static_cast<int>(S), static_cast<int>(P),
+                             nullptr, nullptr, true, h->d_out.get<double>(), s),
+              "lineage kernel");
+    to_host(out, h->d_out, T * G * P * S * sizeof(double), s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// the streams in the reference's export order: per group the 2 filter
+// streams then the P particle streams (dust2/r/filter.hpp:182-207)
+int dust2_filter_ode_rng_state(dust2_filter_ode* h, uint64_t* out, char* err, size_t errlen) {
+  try {
+    check_filter_ode(h);
+    const size_t G = h->G, P = h->P;
+    std::vector<uint64_t> filt(G * 8), part(G * P * 4);
+    to_host(filt.data(), h->d_frng, filt.size() * sizeof(uint64_t), h->sys->stream);
+    to_host(part.data(), h->sys->d_rng, part.size() * sizeof(uint64_t), h->sys->stream);
+    for (size_t g = 0; g < G; ++g) {
+      uint64_t* blk = out + g * (P + 2) * 4;
+      std::copy(filt.data() + g * 8, filt.data() + g * 8 + 8, blk);
+      std::copy(part.data() + g * P * 4, part.data() + (g + 1) * P * 4, blk + 8);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_ode_set_rng_state(dust2_filter_ode* h, const uint64_t* state, char* err,
+                                   size_t errlen) {
+  try {
+    check_filter_ode(h);
+    const size_t G = h->G, P = h->P;
+    std::vector<uint64_t> filt(G * 8), part(G * P * 4);
+    for (size_t g = 0; g < G; ++g) {
+      const uint64_t* blk = state + g * (P + 2) * 4;
+      std::copy(blk, blk + 8, filt.data() + g * 8);
+      std::copy(blk + 8, blk + (P + 2) * 4, part.data() + g * P * 4);
+    }
+    to_device(h->d_frng, filt.data(), filt.size() * sizeof(uint64_t), h->sys->stream);
+    to_device(h->sys->d_rng, part.data(), part.size() * sizeof(uint64_t), h->sys->stream);
+    hip_check(hipStreamSynchronize(h->sys->stream), "sync");
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_filter_ode_free(dust2_filter_ode* h) {
+  if (handle_ok(h, TAG_FILTER_ODE)) delete h;
 }
 
 }  // extern "C"

@@ -95,9 +95,37 @@ struct gen_compare_launch {
   double* out;               // [n_total]
 };
 
+struct gen_resample_launch {
+  const double* ll_step;     // [G][P] log-weights
+  uint64_t* filter_rng;      // [G][2][4]: streams 2g (resampling) and 2g + 1
+  double* ll;                // [G] accumulated log-likelihood
+  uint32_t* idx;             // [G][P] ancestors
+  int n_groups;
+  int n_particles;           // at most 8192
+  uint32_t* fallback_count;
+};
+
+struct gen_reorder_launch {
+  const uint32_t* idx;       // [G][P] ancestors within the group
+  const double* state_in;    // [S][n_total]
+  double* state_out;
+  int S;
+  const double* words_in;    // [n_total][words] solver internals
+  double* words_out;
+  int words;
+  int n_particles;
+  int n_total;
+  double* traj;              // [G][n_time][S][P] or null
+  uint32_t* ancestry;        // [G][n_time][P] or null
+  int n_time;
+  int t;
+};
+
 int gen_words(int model);
 int gen_history_words(int model);
 hipError_t launch_general_system(const gen_launch& args, hipStream_t stream);
 hipError_t launch_general_compare(const gen_compare_launch& args, hipStream_t stream);
+hipError_t launch_general_resample(const gen_resample_launch& args, hipStream_t stream);
+hipError_t launch_general_reorder(const gen_reorder_launch& args, hipStream_t stream);
 
 }  // namespace dust2_amd

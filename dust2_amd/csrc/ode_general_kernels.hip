@@ -18,6 +18,7 @@
 // the history ring are separate per-particle arrays.
 #include <hip/hip_runtime.h>
 
+#include "block_resample.hpp"
 #include "engine.hpp"
 #include "ode_general.hpp"
 #include "ode_models.hpp"
@@ -824,6 +825,71 @@ __global__ void __launch_bounds__(256) general_compare_kernel(gen_compare_launch
   a.out[i] = M::compare(a.time, y, d, a.pars + static_cast<size_t>(g) * a.n_pars);
 }
 
+// One group per workgroup: log-sum-exp of the particles' log-weights added
+// to the group's log-likelihood, one uniform from the group's filter stream
+// and systematic resampling, indices exact (block_resample.hpp); no
+// weights (all exactly 0) leave the identity (dust2/filter.hpp:56-80).
+template <int PPT>
+__global__ void __launch_bounds__(256) general_resample_kernel(gen_resample_launch a) {
+  extern __shared__ double lds[];
+  double* scratch = lds;
+  double* cum = lds + kScratchDoubles;
+  const int g = blockIdx.x;
+  const int P = a.n_particles;
+  const int p0 = threadIdx.x * PPT;
+  double w[PPT];
+  int idx[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    w[k] = p0 + k < P ? a.ll_step[static_cast<size_t>(g) * P + p0 + k] : 0.0;
+  }
+  uint64_t* fr = a.filter_rng + 8 * static_cast<size_t>(g);  // stream 2g
+  auto u_draw = [&]() {
+    rng256 r{fr[0], fr[1], fr[2], fr[3]};
+    const double u = random_real<double>(r);
+    fr[0] = r.s0;
+    fr[1] = r.s1;
+    fr[2] = r.s2;
+    fr[3] = r.s3;
+    return u;
+  };
+  const resample_result res = block_resample<256, PPT>(w, idx, P, cum, scratch, u_draw);
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    if (p0 + k < P) a.idx[static_cast<size_t>(g) * P + p0 + k] = static_cast<uint32_t>(idx[k]);
+  }
+  if (threadIdx.x == 0) {
+    a.ll[g] += res.ll_increment;
+    if (res.n_fallback) atomicAdd(a.fallback_count, 1u);
+  }
+}
+
+// reorder (dust2/continuous/system.hpp:172-199): state and solver internals
+// of slot i come from particle idx[i] of the same group; the particle
+// streams stay put.  With traj, the reordered state and the ancestors are
+// also saved for time slot t ([G][T][S][P], [G][T][P]).
+__global__ void __launch_bounds__(256) general_reorder_kernel(gen_reorder_launch a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_total) return;
+  const int P = a.n_particles;
+  const int g = i / P, p = i - g * P;
+  const size_t NT = static_cast<size_t>(a.n_total);
+  const uint32_t from = static_cast<uint32_t>(g) * P + a.idx[i];
+  for (int k = 0; k < a.S; ++k) {
+    const double v = a.state_in[k * NT + from];
+    a.state_out[k * NT + i] = v;
+    if (a.traj) {
+      a.traj[((static_cast<size_t>(g) * a.n_time + a.t) * a.S + k) * P + p] = v;
+    }
+  }
+  const double* src = a.words_in + static_cast<size_t>(from) * a.words;
+  double* dst = a.words_out + static_cast<size_t>(i) * a.words;
+  for (int k = 0; k < a.words; ++k) dst[k] = src[k];
+  if (a.ancestry) {
+    a.ancestry[(static_cast<size_t>(g) * a.n_time + a.t) * P + p] = a.idx[i];
+  }
+}
+
 template <class M>
 hipError_t launch_model(const gen_launch& a, hipStream_t stream) {
   const unsigned blocks = static_cast<unsigned>((a.n_total + 255) / 256);
@@ -856,6 +922,36 @@ hipError_t launch_general_compare(const gen_compare_launch& a, hipStream_t strea
       return hipGetLastError();
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_general_resample(const gen_resample_launch& a, hipStream_t stream) {
+  if (a.n_groups == 0) return hipSuccess;
+  const int P = a.n_particles;
+  const size_t lds = (kScratchDoubles + static_cast<size_t>(P)) * sizeof(double);
+  const dim3 grid(a.n_groups), block(256);
+  if (P <= 256) {
+    hipLaunchKernelGGL(general_resample_kernel<1>, grid, block, lds, stream, a);
+  } else if (P <= 512) {
+    hipLaunchKernelGGL(general_resample_kernel<2>, grid, block, lds, stream, a);
+  } else if (P <= 1024) {
+    hipLaunchKernelGGL(general_resample_kernel<4>, grid, block, lds, stream, a);
+  } else if (P <= 2048) {
+    hipLaunchKernelGGL(general_resample_kernel<8>, grid, block, lds, stream, a);
+  } else if (P <= 4096) {
+    hipLaunchKernelGGL(general_resample_kernel<16>, grid, block, lds, stream, a);
+  } else if (P <= 8192) {
+    hipLaunchKernelGGL(general_resample_kernel<32>, grid, block, lds, stream, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_general_reorder(const gen_reorder_launch& a, hipStream_t stream) {
+  if (a.n_total == 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((a.n_total + 255) / 256);
+  hipLaunchKernelGGL(general_reorder_kernel, dim3(blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_general_system(const gen_launch& a, hipStream_t stream) {

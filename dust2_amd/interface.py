@@ -538,8 +538,89 @@ class Filter:
             self.ptr = None
 
 
+class ModelFilter:
+    """A particle filter over a mixed-time model (the general ODE engine):
+    dust_filter_create for a model run in continuous time with discrete
+    updates (R/interface-filter.R:37-97, tests/testthat/test-filter.R:705-722),
+    over dust2/filter.hpp.  Data fields per model (malaria: tested,
+    positive); FP64."""
+
+    DATA_FIELDS = {"malaria": ["tested", "positive"]}
+
+    def __init__(self, generator, time_start, data, n_particles, n_groups, dt, seed,
+                 ode_control=None):
+        self.generator = generator
+        self.time_start = float(time_start)
+        self.n_particles = int(n_particles)
+        self.n_groups = int(n_groups)
+        self.dt = 0.0 if dt is None else float(dt)
+        fields = self.DATA_FIELDS.get(generator.name)
+        if fields is None:
+            raise ValueError(f"'{generator.name}' has no compare_data")
+        self.time = np.ascontiguousarray(f64(data["time"]).reshape(-1))
+        cols = [np.asarray(f64(data[k])) for k in fields]
+        cols = [c.reshape(self.time.size, -1) for c in cols]
+        self.n_groups_data = cols[0].shape[1]
+        if self.n_groups_data not in (1, self.n_groups):
+            raise ValueError("Expected data for one group or every group")
+        # [time][group][field]
+        self.data = np.ascontiguousarray(np.stack(cols, axis=2))
+        if seed is None:
+            seed = int(np.random.default_rng().integers(0, 2**31 - 1))
+        self.seed = seed
+        self.ode_control = _ode_control(ode_control)
+        self.ptr = None
+        self.rows = None
+        self.last_index = None
+        self.last_index_state = None
+        self.traj_current = False
+
+    @property
+    def initialised(self):
+        return self.ptr is not None
+
+    def _alloc(self, pars):
+        self.rows = pars_matrix(self.generator, pars, self.n_groups)
+        raw, n_raw, seed = _seed_args(self.seed)
+        out = C.c_void_p()
+        call("dust2_filter_ode_alloc", self.generator.name.encode(), dptr(self.rows),
+             self.n_groups, self.n_particles, self.time_start, dptr(self.time), self.time.size,
+             dptr(self.data), self.n_groups_data, self.dt, dptr(self.ode_control.values),
+             u64ptr(raw), n_raw, seed, C.byref(out))
+        self.ptr = out
+
+    def run(self, pars, index_group=None, initial=None, save_trajectories=False):
+        if index_group is not None:
+            raise ValueError("'index_group' is not supported by this filter")
+        if self.ptr is None:
+            if pars is None:
+                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+            self._alloc(pars)
+        elif pars is not None:
+            rows = pars_update(self.generator, pars, self.rows)
+            call("dust2_filter_ode_update_pars", self.ptr, dptr(rows))
+            self.rows = rows
+        init = None
+        if initial is not None:
+            init = np.broadcast_to(
+                f64(initial), (self.n_groups, self.n_particles, self.generator.n_state)).copy()
+        ll = np.zeros(self.n_groups)
+        call("dust2_filter_ode_run", self.ptr, dptr(init), int(bool(save_trajectories)),
+             dptr(ll))
+        self.traj_current = bool(save_trajectories)
+        return float(ll[0]) if self.n_groups == 1 else ll
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib._lib is not None:
+            _lib.lib().dust2_filter_ode_free(self.ptr)
+            self.ptr = None
+
+
 def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=1,
-                       seed=None, precision="fp64"):
+                       seed=None, precision="fp64", ode_control=None):
+    if generator.time_type == "mixed":
+        return ModelFilter(generator, time_start, data, n_particles, n_groups, dt, seed,
+                           ode_control)
     return Filter(generator, time_start, data, n_particles, n_groups, dt, seed, precision)
 
 
@@ -556,6 +637,9 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
         if index_group is not None:
             raise ValueError("'index_group' must be NULL, as 'obj' is not initialised")
     index_state = check_index(index_state, obj.generator.n_state, "index_state")
+    if isinstance(obj, ModelFilter):
+        obj.last_index_state = index_state
+        return obj.run(pars, index_group, initial=initial, save_trajectories=save_trajectories)
     if isinstance(obj, Unfilter):
         ll = obj.run(pars, index_group, initial=initial, adjoint=adjoint,
                      save_trajectories=save_trajectories)
@@ -617,7 +701,7 @@ def dust_likelihood_copy(obj, seed=None):
 
 
 def _filter_only(obj, what):
-    if not isinstance(obj, Filter):
+    if not isinstance(obj, (Filter, ModelFilter)):
         raise ValueError(f"'{what}' is only available for particle filters")
 
 
@@ -631,6 +715,14 @@ def dust_likelihood_last_state(obj, select_random_particle=False):
             raise DustError("System has not been run")
         st = obj.last_state()
         return st[:, 0, :] if select_random_particle else st
+    if isinstance(obj, ModelFilter):
+        if obj.ptr is None:
+            raise DustError("System has not been run")
+        if select_random_particle:
+            raise ValueError("'select_random_particle' is not supported by this filter")
+        out = np.zeros((obj.n_groups, obj.n_particles, obj.generator.n_state))
+        call("dust2_filter_ode_last_state", obj.ptr, dptr(out))
+        return out
     if select_random_particle:
         n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
         out = np.zeros((n_groups, obj.generator.n_state))
@@ -663,6 +755,13 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
     n_state)."""
     if isinstance(obj, Unfilter):
         out = obj.last_trajectories()
+    elif isinstance(obj, ModelFilter):
+        if obj.ptr is None or not obj.traj_current:
+            raise DustError("Trajectories are not current")
+        if select_random_particle:
+            raise ValueError("'select_random_particle' is not supported by this filter")
+        out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
+        call("dust2_filter_ode_last_trajectories", obj.ptr, dptr(out))
     else:
         if obj.ptr is None:
             raise DustError("Trajectories are not current")
@@ -702,7 +801,9 @@ def dust_likelihood_rng_state(obj):
         call("dust2_amd_rng_streams", u64ptr(raw), n_raw, seed, n, u64ptr(out))
         return out
     out = np.zeros(n * 4, dtype=np.uint64)
-    call("dust2_filter_rng_state", obj.ptr, u64ptr(out))
+    name = "dust2_filter_ode_rng_state" if isinstance(obj, ModelFilter) else \
+        "dust2_filter_rng_state"
+    call(name, obj.ptr, u64ptr(out))
     return out
 
 
@@ -716,7 +817,9 @@ def dust_likelihood_set_rng_state(obj, state):
     if obj.ptr is None:
         obj.seed = st.reshape(-1, 4).copy()
         return
-    call("dust2_filter_set_rng_state", obj.ptr, u64ptr(st))
+    name = "dust2_filter_ode_set_rng_state" if isinstance(obj, ModelFilter) else \
+        "dust2_filter_set_rng_state"
+    call(name, obj.ptr, u64ptr(st))
 
 
 def dust_likelihood_last_stats(obj):

@@ -54,6 +54,7 @@ typedef struct dust2_unfilter dust2_unfilter;
 typedef struct dust2_unfilter_sir dust2_unfilter_sir;
 typedef struct dust2_ode_system dust2_ode_system;
 typedef struct dust2_ode_general dust2_ode_general;
+typedef struct dust2_filter_ode dust2_filter_ode;
 
 /* ---- library ------------------------------------------------------------ */
 
@@ -403,6 +404,38 @@ int dust2_system_ode_events(dust2_ode_general* h, size_t* counts, double* record
 int dust2_system_ode_history(dust2_ode_general* h, size_t* counts, double* entries, char* err,
                              size_t errlen);
 void dust2_system_ode_free(dust2_ode_general* h);
+
+/* ---- particle filter over a mixed-time model ------------------------------
+ * dust_filter_create for a model of the general continuous-time engine with
+ * a discrete update and data (malaria), replacing the filter registrations
+ * of inst/template/compare.cpp:31-64 over filter<dust_continuous<T>>
+ * (dust2/filter.hpp:42-175): per data time run_to_time, compare_data, the
+ * log-sum-exp and systematic resampling of each group (exact indices) and
+ * the reorder of state and solver internals.  Streams [2 filter | P
+ * particle] per group as dust2_filter_sir_alloc (dust2/r/discrete/
+ * filter.hpp:63-74).  data[n_time][n_groups_data][n_data]; dt: the update
+ * interval (0: none); control: dust_ode_control() values or NULL.  FP64;
+ * at most 8192 particles per group. */
+int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_groups,
+                           size_t n_particles, double time_start, const double* time,
+                           size_t n_time, const double* data, size_t n_groups_data, double dt,
+                           const double* control, const uint64_t* seed_state,
+                           size_t n_seed_streams, uint64_t seed, dust2_filter_ode** out,
+                           char* err, size_t errlen);
+/* initial: [G][P][S] or NULL (the model's initial state); ll_out[G] */
+int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories,
+                         double* ll_out, char* err, size_t errlen);
+int dust2_filter_ode_update_pars(dust2_filter_ode* h, const double* pars, char* err,
+                                 size_t errlen);
+int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, size_t errlen);
+/* out[n_time][G][P][S], each particle's lineage (dust2/trajectories.hpp:128-189) */
+int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
+                                       size_t errlen);
+/* [G][P + 2][4] words: per group the 2 filter streams, then the particles' */
+int dust2_filter_ode_rng_state(dust2_filter_ode* h, uint64_t* out, char* err, size_t errlen);
+int dust2_filter_ode_set_rng_state(dust2_filter_ode* h, const uint64_t* state, char* err,
+                                   size_t errlen);
+void dust2_filter_ode_free(dust2_filter_ode* h);
 
 /* ---- deterministic discrete-time SIR unfilter with the adjoint ----------
  * Replaces dust2_discrete_unfilter_alloc<sir> (dust2/r/discrete/unfilter.hpp:12-58),

@@ -529,4 +529,130 @@ int ref_ode_mixed(int model, const double* p, const double* y0, double time_star
   return 0;
 }
 
+// One group of the malaria particle filter (tests/testthat/test-filter.R:
+// 705-722) end to end: P particles through the reference solver with the
+// mixed-time loop restated as ref_ode_mixed does, compare_data as
+// inst/examples/malaria.cpp:176-188 (binomial log density), then the
+// reference's own scale_log_weights and resample_weight
+// (dust2/filter_details.hpp) with one uniform from the group's filter
+// stream, and the reorder of state and solver internals
+// (dust2/filter.hpp:42-104, dust2/continuous/system.hpp:172-199).
+// part_rng[P][4]; filt_rng[4] (stream 2g, advanced in place);
+// data[n_time][2] = tested, positive; out: *ll, final_state[P][S].
+int ref_malaria_filter(const double* p, size_t P, double time_start, const double* times,
+                       size_t n_time, const double* data, double dt, const uint64_t* part_rng,
+                       uint64_t* filt_rng, double* ll_out, double* final_state, char* err,
+                       size_t errlen) {
+  using dust2::ode::internals;
+  using dust2::ode::solver;
+  const int n_rates = static_cast<int>(p[3]);
+  const size_t n_var = 4 + n_rates, NY = n_var + 1;
+  auto rhs = [p, n_rates](double, const double* s, double* d) {
+    const double a = p[0], bh = p[1], bv = p[2], n = p[3], mu = p[4], r = p[5], tau = p[6];
+    const double Sh = s[0], Ih = s[1], Sv = s[2], Iv = s[3];
+    const double* Ev = s + 4;
+    const double beta = s[4 + n_rates];
+    const double foi_h = a * bh * Iv;
+    const double foi_v = a * bv * Ih;
+    d[4] = foi_v * Sv - (n / tau) * Ev[0] - mu * Ev[0];
+    for (int i = 1; i < n_rates; ++i) {
+      d[4 + i] = (n / tau) * Ev[i - 1] - (n / tau) * Ev[i] - mu * Ev[i];
+    }
+    d[0] = -foi_h * Sh + r * Ih;
+    d[1] = foi_h * Sh - r * Ih;
+    d[2] = beta * p[9] - foi_v * Sv - mu * Sv;
+    d[3] = (n / tau) * Ev[n_rates - 1] - mu * Iv;
+  };
+  std::vector<double> y0(NY, 0.0);
+  y0[0] = 1 - p[7];
+  y0[1] = p[7];
+  y0[2] = p[9];
+  y0[3] = p[8];
+  y0[4 + n_rates] = p[4];
+  dust2::ode::control<double> ctl;
+  solver<double> s(n_var, 1, ctl);
+  std::vector<internals<double>> in(P, internals<double>(n_var, 1, false));
+  std::vector<std::vector<double>> y(P, y0);
+  dust2::zero_every_type<double> zero_every;
+  dust2::ode::events_type<double> events;
+  oracle::rng_state fr{{filt_rng[0], filt_rng[1], filt_rng[2], filt_rng[3]}, false};
+  double ll = 0;
+  double time_ = time_start;
+  try {
+    for (size_t k = 0; k < P; ++k) {
+      s.initialise(time_, y[k].data(), in[k], rhs);
+      s.initialise_events(events, in[k]);
+    }
+    std::vector<double> w(P);
+    std::vector<size_t> idx(P);
+    for (size_t q = 0; q < n_time; ++q) {
+      const double time = times[q];
+      for (size_t k = 0; k < P; ++k) {
+        if (dt == 0) {
+          s.run(time_, time, y[k].data(), zero_every, events, in[k], rhs);
+          continue;
+        }
+        const size_t steps = (time - time_) / dt;
+        double t1 = time_;
+        for (size_t j = 0; j < steps; ++j) {
+          const double t0 = t1;
+          t1 = (j == steps - 1) ? time : time_ + j * dt;
+          s.run(t0, t1, y[k].data(), zero_every, events, in[k], rhs);
+          std::vector<double> y_other(y[k]);
+          oracle::rng_state r{{part_rng[4 * k], part_rng[4 * k + 1], part_rng[4 * k + 2],
+                               part_rng[4 * k + 3]},
+                              false};
+          const int i = 4 + n_rates;
+          y_other[i] = y[k][i] * std::exp(oracle::normal(r, 0, p[10]));
+          const bool same = std::equal(y[k].begin(), y[k].end(), y_other.begin());
+          y[k] = y_other;
+          if (!same) s.initialise(time_, y[k].data(), in[k], rhs);
+        }
+      }
+      time_ = time;
+      const double tested = data[2 * q], positive = data[2 * q + 1];
+      bool any = false;
+      for (size_t k = 0; k < P; ++k) {
+        double v = 0;
+        if (!std::isnan(tested) && !std::isnan(positive)) {
+          const double prob = y[k][1];
+          if (prob == 0) {
+            v = positive == 0 ? 0 : -std::numeric_limits<double>::infinity();
+          } else if (prob == 1) {
+            v = positive == tested ? 0 : -std::numeric_limits<double>::infinity();
+          } else {
+            v = std::lgamma(tested + 1) - std::lgamma(positive + 1) -
+                std::lgamma(tested - positive + 1) + positive * std::log(prob) +
+                (tested - positive) * std::log1p(-prob);
+          }
+        }
+        w[k] = v;
+        any = any || v != 0;
+      }
+      if (any) {
+        ll += dust2::details::scale_log_weights<double>(P, w.begin());
+        const double u = oracle::random_real(fr);
+        dust2::details::resample_weight(P, w.begin(), u, idx.begin());
+      } else {
+        for (size_t k = 0; k < P; ++k) idx[k] = k;
+      }
+      std::vector<std::vector<double>> y2(P);
+      std::vector<internals<double>> in2(in);
+      for (size_t k = 0; k < P; ++k) {
+        y2[k] = y[idx[k]];
+        in2[k] = in[idx[k]];
+      }
+      y.swap(y2);
+      in.swap(in2);
+    }
+  } catch (const std::exception& e) {
+    if (err && errlen) std::snprintf(err, errlen, "%s", e.what());
+    return 1;
+  }
+  *ll_out = ll;
+  for (size_t k = 0; k < P; ++k) std::copy(y[k].begin(), y[k].end(), final_state + k * NY);
+  std::copy(fr.s, fr.s + 4, filt_rng);
+  return 0;
+}
+
 }  // extern "C"

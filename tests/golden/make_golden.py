@@ -230,6 +230,46 @@ def ode_mixed_fixture():
     return {"cases": cases}
 
 
+def malaria_filter_fixture():
+    """The malaria particle filter (tests/testthat/test-filter.R:705-722)
+    through the reference solver and resampling (oracle/_ref): streams in
+    the filter's layout ([2 filter | P particle] per group), the
+    log-likelihood of consecutive runs and the final states."""
+    import dust2_amd
+    from dust2_amd.interface import pars_matrix
+    specs = [
+        ("reference_test", [{}], 100, [4.0, 8.0, 12.0, 16.0],
+         [[[2, 1]], [[4, 2]], [[6, 3]], [[8, 3]]], 1.0, 3),
+        ("two_groups_missing", [{}, {"a": 0.5, "r": 0.02}], 64,
+         [2.0, 4.0, 6.0, 8.0, 10.0, 12.0],
+         [[[5, 1], [5, 2]], [[5, 2], [5, 3]], [[np.nan, np.nan], [np.nan, np.nan]],
+          [[6, 2], [6, 4]], [[6, 3], [6, 3]], [[8, 3], [8, 5]]], 1.0, 2),
+        ("no_updates", [{}], 40, [5.0, 10.0], [[[10, 3]], [[10, 4]]], 0.0, 1),
+    ]
+    cases = []
+    for name, pars, P, times, data, dt, n_runs in specs:
+        G = len(pars)
+        rows = pars_matrix(dust2_amd.malaria(), pars, G)
+        streams = ol.prng(G * (P + 2), seed=42).reshape(G, P + 2, 4)
+        data = np.array(data, dtype=float)  # [time][group][field]
+        runs = []
+        filt = streams[:, :2].copy()
+        for _ in range(n_runs):
+            lls, finals = [], []
+            for g in range(G):
+                ll, fin, fr = ol.ref_malaria_filter(rows[g], P, 0.0, times, data[:, g], dt,
+                                                    streams[g, 2:], filt[g, 0])
+                filt[g, 0] = fr
+                lls.append(ll)
+                finals.append(fin.tolist())
+            runs.append({"ll": lls, "state": finals})
+        cases.append({"name": name, "pars": pars, "P": P, "times": times,
+                      "data": np.where(np.isnan(data), None, data).tolist(), "dt": dt,
+                      "streams": [[int(w) for w in st] for st in streams.reshape(-1, 4)],
+                      "runs": runs})
+    return {"cases": cases}
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref/libdust2ref.so is not built (needs the reference tree)")
@@ -247,6 +287,8 @@ def main():
         json.dump(ode_general_fixture(), f)
     with open(os.path.join(HERE, "ode_mixed_dp5.json"), "w") as f:
         json.dump(ode_mixed_fixture(), f)
+    with open(os.path.join(HERE, "malaria_filter.json"), "w") as f:
+        json.dump(malaria_filter_fixture(), f)
 
 
 if __name__ == "__main__":

@@ -492,3 +492,23 @@ def ref_ode_mixed(model, row, targets, rng, y0=None, dt=1.0, deterministic=False
     _check(rc, buf)
     return {"state": state, "stats": stats, "step_times": times[:int(n[0])].copy()}
 
+
+def ref_malaria_filter(row, P, time_start, times, data, dt, part_rng, filt_rng):
+    """One group of the malaria particle filter through the reference solver
+    and the reference's own scale_log_weights / resample_weight
+    (oracle/_ref): the log-likelihood, the final states [P][20] and the
+    filter stream after the run."""
+    row = np.ascontiguousarray(row, dtype=np.float64)
+    times = np.ascontiguousarray(times, dtype=np.float64).reshape(-1)
+    data = np.ascontiguousarray(data, dtype=np.float64).reshape(times.size, 2)
+    pr = np.ascontiguousarray(part_rng, dtype=np.uint64).reshape(P, 4)
+    fr = np.ascontiguousarray(filt_rng, dtype=np.uint64).reshape(4).copy()
+    ll = np.zeros(1)
+    state = np.zeros((P, 20))
+    buf = C.create_string_buffer(512)
+    rc = ref().ref_malaria_filter(_dp(row), C.c_size_t(P), C.c_double(time_start), _dp(times),
+                                  C.c_size_t(times.size), _dp(data), C.c_double(dt), _u64(pr),
+                                  _u64(fr), _dp(ll), _dp(state), buf, 512)
+    _check(rc, buf)
+    return float(ll[0]), state, fr
+

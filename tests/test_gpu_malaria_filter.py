@@ -1,0 +1,104 @@
+"""The particle filter over a mixed-time model on the GPU (the malaria
+model; tests/testthat/test-filter.R:705-722): against the whole filter run
+through the reference solver and the reference's own resampling
+(tests/golden/malaria_filter.json: log-likelihoods of consecutive runs
+within 1e-10, final states within 1e-9), and against a filter assembled by
+hand from the system's entry points as the reference's test does
+(tests/testthat/helper-dust.R:5-48, filter_manual)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "malaria_filter.json")
+
+
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+def data_dict(case):
+    d = np.array([[[np.nan if v is None else v for v in f] for f in g] for g in case["data"]],
+                 dtype=float)
+    return {"time": np.array(case["times"]), "tested": d[:, :, 0], "positive": d[:, :, 1]}
+
+
+@pytest.mark.parametrize("case", golden(), ids=lambda c: c["name"])
+def test_matches_reference_filter(gpu_lib, case):
+    G, P = len(case["pars"]), case["P"]
+    streams = np.array(case["streams"], dtype=np.uint64)
+    obj = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data_dict(case), P, n_groups=G,
+                                     dt=case["dt"] or None, seed=streams)
+    pars = case["pars"] if G > 1 else case["pars"][0]
+    for r in case["runs"]:
+        ll = np.atleast_1d(gpu_lib.dust_likelihood_run(obj, pars))
+        np.testing.assert_allclose(ll, r["ll"], rtol=1e-10)
+        st = gpu_lib.dust_likelihood_last_state(obj)
+        np.testing.assert_allclose(st, np.array(r["state"]), rtol=1e-9, atol=1e-12)
+    # the particle streams never move (updates draw from copies); the
+    # filter streams advanced once per resampling
+    rng = gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4)
+    np.testing.assert_array_equal(rng[:, 2:], streams.reshape(G, P + 2, 4)[:, 2:])
+
+
+def test_filter_equals_filter_by_hand(gpu_lib):
+    # test-filter.R:705-722 with filter_manual (helper-dust.R:5-48)
+    times = np.array([4.0, 8.0, 12.0, 16.0])
+    data = {"time": times, "tested": [2, 4, 6, 8], "positive": [1, 2, 3, 3]}
+    P = 100
+    streams = ol.prng(P + 2, seed=42)
+    obj = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, P, dt=1, seed=streams)
+    res = [gpu_lib.dust_likelihood_run(obj, {}) for _ in range(5)]
+
+    sys = gpu_lib.dust_system_create(gpu_lib.malaria(), {}, P, dt=1, seed=streams[2:])
+    fr = streams[0].copy()
+    manual = []
+    for _ in range(5):
+        gpu_lib.dust_system_set_time(sys, 0.0)
+        gpu_lib.dust_system_set_state_initial(sys)
+        ll = 0.0
+        for i, t in enumerate(times):
+            gpu_lib.dust_system_run_to_time(sys, t)
+            tmp = gpu_lib.dust_system_compare_data(
+                sys, {"tested": data["tested"][i], "positive": data["positive"][i]})[0]
+            inc, w = ol.scale_log_weights(tmp)
+            ll += inc
+            u, fr = ol.random_real(fr, 1)
+            k = ol.resample_weight(w, u[0])
+            state = gpu_lib.dust_system_state(sys)
+            gpu_lib.dust_system_set_state(sys, state[:, k])
+        manual.append(ll)
+    np.testing.assert_allclose(res, manual, rtol=1e-12)
+
+
+def test_trajectories_follow_the_lineage(gpu_lib):
+    case = golden()[0]
+    P = case["P"]
+    obj = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data_dict(case), P, dt=1,
+                                     seed=np.array(case["streams"], dtype=np.uint64))
+    with pytest.raises(gpu_lib.DustError, match="Trajectories are not current"):
+        gpu_lib.dust_likelihood_last_trajectories(obj)
+    ll = gpu_lib.dust_likelihood_run(obj, {}, save_trajectories=True)
+    assert ll == pytest.approx(case["runs"][0]["ll"][0], rel=1e-10)
+    h = gpu_lib.dust_likelihood_last_trajectories(obj)
+    assert h.shape == (4, 1, P, 20)
+    np.testing.assert_array_equal(h[-1, 0], gpu_lib.dust_likelihood_last_state(obj)[0])
+    # each particle's path is continuous in the never-resampled compartment
+    # totals: humans Sh + Ih stay 1 along every lineage
+    np.testing.assert_allclose(h[..., 0] + h[..., 1], 1.0, rtol=1e-9)
+
+
+def test_filter_errors(gpu_lib):
+    data = {"time": [4.0, 8.0], "tested": [2, 4], "positive": [1, 2]}
+    with pytest.raises(gpu_lib.DustError, match="Expected 'time\\[1\\]'"):
+        gpu_lib.dust_likelihood_run(
+            gpu_lib.dust_filter_create(gpu_lib.malaria(), 5.0, data, 10, dt=1), {})
+    with pytest.raises(ValueError, match="continuous-time models"):
+        gpu_lib.dust_filter_create(gpu_lib.sirode(), 0.0, {"time": [1.0], "incidence": [1.0]},
+                                   10)

@@ -126,3 +126,38 @@ def test_golden_is_the_reference_run(name):
                              deterministic=c["deterministic"])
         np.testing.assert_array_equal(r["state"], p["state"])
         np.testing.assert_array_equal(r["stats"], p["stats"])
+
+
+@pytest.mark.skipif(not ol.ref_available(), reason="reference tree not built into oracle/_ref")
+def test_malaria_filter_golden_equals_filter_by_hand():
+    # tests/testthat/helper-dust.R:5-48 (filter_manual): each particle run
+    # between data times from its resampled state (the update re-initialises
+    # the solver at every interval, so a fresh start is the same run),
+    # R's dbinom, log-mean-exp weights and the resampling of
+    # dust2/filter_details.hpp
+    from scipy import stats
+    with open(os.path.join(os.path.dirname(__file__), "golden", "malaria_filter.json")) as f:
+        case = json.load(f)["cases"][0]
+    import dust2_amd
+    from dust2_amd.interface import pars_matrix
+    row = pars_matrix(dust2_amd.malaria(), {}, 1)[0]
+    P = case["P"]
+    streams = np.array(case["streams"], dtype=np.uint64).reshape(P + 2, 4)
+    fr = streams[0].copy()
+    times = [0.0] + case["times"]
+    for run in case["runs"]:
+        y = np.tile(ol.ref_ode_mixed("malaria", row, [0.0], streams[2], dt=1.0)["state"][0],
+                    (P, 1))
+        ll = 0.0
+        for i in range(1, len(times)):
+            y = np.array([ol.ref_ode_mixed("malaria", row, [times[i]], streams[2 + k], y0=y[k],
+                                           dt=1.0, time_start=times[i - 1])["state"][0]
+                          for k in range(P)])
+            tested, positive = case["data"][i - 1][0]
+            w = stats.binom.logpmf(positive, tested, y[:, 1])
+            inc, ww = ol.scale_log_weights(w)
+            ll += inc
+            u, fr = ol.random_real(fr, 1)
+            y = y[ol.resample_weight(ww, u[0])]
+        assert ll == pytest.approx(run["ll"][0], rel=1e-10)
+        np.testing.assert_allclose(y, np.array(run["state"][0]), rtol=1e-9, atol=1e-12)
