@@ -147,6 +147,14 @@ _PROTOS = {
     "dust2_filter_ode_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_free": (None, [VP]),
+    "dust2_unfilter_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double,
+                                           D, C.c_size_t, D, C.c_size_t, C.c_double, D,
+                                           C.POINTER(VP), ERR, ERRLEN]),
+    "dust2_unfilter_ode_run": (C.c_int, [VP, D, C.c_int, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_free": (None, [VP]),
     "dust2_amd_interpolate": (C.c_int, [C.c_int, D, C.c_size_t, D, C.c_size_t, C.c_size_t, D,
                                         C.c_size_t, D, ERR, ERRLEN]),
     "dust2_amd_rng_streams": (C.c_int, [U64, C.c_size_t, C.c_uint64, C.c_size_t, U64, ERR,

@@ -375,6 +375,7 @@ enum handle_tag : uint32_t {
   TAG_ODE_SYSTEM = 0x64327332u,
   TAG_ODE_GENERAL = 0x64327333u,
   TAG_FILTER_ODE = 0x64326632u,
+  TAG_UNFILTER_ODE = 0x64327534u,
   TAG_SERVER = 0x64327631u,
 };
 
@@ -3556,6 +3557,164 @@ int dust2_filter_ode_set_rng_state(dust2_filter_ode* h, const uint64_t* state, c
 
 void dust2_filter_ode_free(dust2_filter_ode* h) {
   if (handle_ok(h, TAG_FILTER_ODE)) delete h;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Deterministic likelihood ("unfilter") over a general model with data
+// (dust2/unfilter.hpp:42-72, 162-186 over dust_continuous)
+// ============================================================================
+struct dust2_unfilter_ode {
+  uint32_t tag = TAG_UNFILTER_ODE;
+  dust2_ode_general* sys = nullptr;   // owned, deterministic
+  size_t G = 0, P = 0, n_time = 0, n_groups_data = 0;
+  double time_start = 0;
+  std::vector<double> times;
+  dbuf d_data, d_ll, d_traj;
+  bool traj_current = false, has_run = false;
+  ~dust2_unfilter_ode() {
+    if (sys) dust2_system_ode_free(sys);
+  }
+};
+
+extern "C" {
+
+int dust2_unfilter_ode_alloc(const char* model, const double* pars, size_t n_groups,
+                             size_t n_particles, double time_start, const double* time,
+                             size_t n_time, const double* data, size_t n_groups_data, double dt,
+                             const double* control, dust2_unfilter_ode** out, char* err,
+                             size_t errlen) {
+  try {
+    check_sizes(n_groups, n_particles);
+    const int id = general_model_id(model);
+    const odem::model_info info = odem::info(id);
+    if (info.n_data == 0) throw std::runtime_error("This model has no compare_data");
+    check_time_sequence(time_start, time, n_time);
+    if (n_groups_data != 1 && n_groups_data != n_groups) {
+      throw std::runtime_error("Expected data for one group or every group");
+    }
+    auto h = std::make_unique<dust2_unfilter_ode>();
+    h->G = n_groups;
+    h->P = n_particles;
+    h->n_time = n_time;
+    h->n_groups_data = n_groups_data;
+    h->time_start = time_start;
+    h->times.assign(time, time + n_time);
+    char buf[1024];
+    // deterministic streams seeded with 42 (dust2/r/continuous/unfilter.hpp:38-42)
+    if (dust2_system_ode_alloc(model, pars, n_groups, n_particles, time_start, dt, control,
+                               nullptr, 0, 0, 0, 0, 0, nullptr, 0, 42, 1, &h->sys, buf,
+                               sizeof buf) != 0) {
+      throw std::runtime_error(buf);
+    }
+    hipStream_t s = h->sys->stream;
+    const size_t nd = n_time * n_groups_data * static_cast<size_t>(info.n_data);
+    h->d_data.alloc(nd * sizeof(double));
+    to_device(h->d_data, data, nd * sizeof(double), s);
+    h->d_ll.alloc(n_groups * n_particles * sizeof(double));
+    hip_check(hipStreamSynchronize(s), "sync");
+    *out = h.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// unfilter::run: from time_start (initial [G][P][S] or null), per data time
+// run_to_time and compare_data, each particle's log-likelihood summed;
+// ll_out[G][P].
+int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories,
+                           double* ll_out, char* err, size_t errlen) {
+  try {
+    check_handle(h, TAG_UNFILTER_ODE, "unfilter");
+    dust2_ode_general* sys = h->sys;
+    hipStream_t s = sys->stream;
+    const size_t n_total = h->G * h->P;
+    char buf[1024];
+    sys->errors_pending = false;
+    sys->time = h->time_start;
+    std::fill(sys->reinit.begin(), sys->reinit.end(), 1);
+    const int rc = initial ? dust2_system_ode_set_state(sys, initial, buf, sizeof buf)
+                           : dust2_system_ode_set_state_initial(sys, buf, sizeof buf);
+    if (rc != 0) throw std::runtime_error(buf);
+    hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), s), "memset");
+    h->traj_current = false;
+    const size_t block = sys->S * n_total;
+    if (save_trajectories) h->d_traj.alloc(h->n_time * block * sizeof(double));
+    const int nd = sys->info.n_data;
+    for (size_t t = 0; t < h->n_time; ++t) {
+      general_run(sys, &h->times[t], 1, nullptr);
+      gen_compare_launch c{};
+      c.model = sys->model;
+      c.pars = sys->d_pars.get<double>();
+      c.n_pars = sys->info.n_pars;
+      c.n_particles = static_cast<int>(h->P);
+      c.n_total = static_cast<int>(n_total);
+      c.time = sys->time;
+      c.state = sys->d_state.get<double>();
+      c.data = h->d_data.get<double>() + t * h->n_groups_data * nd;
+      c.shared_data = h->n_groups_data == 1 ? 1 : 0;
+      c.out = h->d_ll.get<double>();
+      c.accumulate = 1;
+      hip_check(launch_general_compare(c, s), "compare kernel");
+      if (save_trajectories) {
+        hip_check(hipMemcpyAsync(h->d_traj.get<double>() + t * block, sys->d_state.get(),
+                                 block * sizeof(double), hipMemcpyDeviceToDevice, s),
+                  "copy trajectory");
+      }
+    }
+    to_host(ll_out, h->d_ll, n_total * sizeof(double), s);
+    h->traj_current = save_trajectories != 0;
+    h->has_run = true;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_ode_update_pars(dust2_unfilter_ode* h, const double* pars, char* err,
+                                   size_t errlen) {
+  try {
+    check_handle(h, TAG_UNFILTER_ODE, "unfilter");
+    return dust2_system_ode_update_pars(h->sys, pars, err, errlen);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_ode_last_state(dust2_unfilter_ode* h, double* out, char* err,
+                                  size_t errlen) {
+  try {
+    check_handle(h, TAG_UNFILTER_ODE, "unfilter");
+    if (!h->has_run) throw std::runtime_error("System has not been run");
+    h->sys->errors_pending = false;
+    return dust2_system_ode_state(h->sys, out, err, errlen);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// the state at every data time: out[n_time][G][P][S]
+int dust2_unfilter_ode_last_trajectories(dust2_unfilter_ode* h, double* out, char* err,
+                                         size_t errlen) {
+  try {
+    check_handle(h, TAG_UNFILTER_ODE, "unfilter");
+    if (!h->traj_current) throw std::runtime_error("Trajectories are not current");
+    const size_t n_total = h->G * h->P, S = h->sys->S, block = S * n_total;
+    std::vector<double> tmp(h->n_time * block);
+    to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->sys->stream);
+    for (size_t t = 0; t < h->n_time; ++t) {
+      soa_to_aos(tmp.data() + t * block, out + t * block, 1, n_total, S, true);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+void dust2_unfilter_ode_free(dust2_unfilter_ode* h) {
+  if (handle_ok(h, TAG_UNFILTER_ODE)) delete h;
 }
 
 }  // extern "C"

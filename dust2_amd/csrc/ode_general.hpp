@@ -93,6 +93,7 @@ struct gen_compare_launch {
   const double* data;        // [G or 1][n_data]
   int shared_data;
   double* out;               // [n_total]
+  int accumulate;            // add to out instead of writing it
 };
 
 struct gen_resample_launch {

@@ -822,7 +822,8 @@ __global__ void __launch_bounds__(256) general_compare_kernel(gen_compare_launch
   double y[NS];
   for (int k = 0; k < NS; ++k) y[k] = a.state[k * NT + i];
   const double* d = a.data + static_cast<size_t>(a.shared_data ? 0 : g) * M::n_data;
-  a.out[i] = M::compare(a.time, y, d, a.pars + static_cast<size_t>(g) * a.n_pars);
+  const double v = M::compare(a.time, y, d, a.pars + static_cast<size_t>(g) * a.n_pars);
+  a.out[i] = a.accumulate ? a.out[i] + v : v;
 }
 
 // One group per workgroup: log-sum-exp of the particles' log-weights added

@@ -610,10 +610,68 @@ class ModelFilter:
         self.traj_current = bool(save_trajectories)
         return float(ll[0]) if self.n_groups == 1 else ll
 
+    def last_state(self):
+        if self.ptr is None:
+            raise DustError("System has not been run")
+        out = np.zeros((self.n_groups, self.n_particles, self.generator.n_state))
+        call(self.prefix + "_last_state", self.ptr, dptr(out))
+        return out
+
+    def last_trajectories(self):
+        if self.ptr is None or not self.traj_current:
+            raise DustError("Trajectories are not current")
+        out = np.zeros((self.time.size, self.n_groups, self.n_particles, self.generator.n_state))
+        call(self.prefix + "_last_trajectories", self.ptr, dptr(out))
+        return out
+
+    prefix = "dust2_filter_ode"
+
     def __del__(self):
         if getattr(self, "ptr", None) is not None and _lib._lib is not None:
-            _lib.lib().dust2_filter_ode_free(self.ptr)
+            getattr(_lib.lib(), self.prefix + "_free")(self.ptr)
             self.ptr = None
+
+
+class ModelUnfilter(ModelFilter):
+    """dust_unfilter_create for a model of the general engine with data
+    (R/interface-unfilter.R:20-71 over dust2/unfilter.hpp): deterministic
+    updates, one log-likelihood per particle."""
+    prefix = "dust2_unfilter_ode"
+
+    def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control):
+        super().__init__(generator, time_start, data, n_particles, n_groups, dt, 0, ode_control)
+
+    def _alloc(self, pars):
+        self.rows = pars_matrix(self.generator, pars, self.n_groups)
+        out = C.c_void_p()
+        call("dust2_unfilter_ode_alloc", self.generator.name.encode(), dptr(self.rows),
+             self.n_groups, self.n_particles, self.time_start, dptr(self.time), self.time.size,
+             dptr(self.data), self.n_groups_data, self.dt, dptr(self.ode_control.values),
+             C.byref(out))
+        self.ptr = out
+
+    def run(self, pars, index_group=None, initial=None, save_trajectories=False):
+        if index_group is not None:
+            raise ValueError("'index_group' is not supported by this unfilter")
+        if self.ptr is None:
+            if pars is None:
+                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+            self._alloc(pars)
+        elif pars is not None:
+            rows = pars_update(self.generator, pars, self.rows)
+            call("dust2_unfilter_ode_update_pars", self.ptr, dptr(rows))
+            self.rows = rows
+        init = None
+        if initial is not None:
+            init = np.broadcast_to(
+                f64(initial), (self.n_groups, self.n_particles, self.generator.n_state)).copy()
+        ll = np.zeros((self.n_groups, self.n_particles))
+        call("dust2_unfilter_ode_run", self.ptr, dptr(init), int(bool(save_trajectories)),
+             dptr(ll))
+        self.traj_current = bool(save_trajectories)
+        if self.n_particles == 1:
+            ll = ll[:, 0]
+        return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1) else ll
 
 
 def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=1,
@@ -701,7 +759,7 @@ def dust_likelihood_copy(obj, seed=None):
 
 
 def _filter_only(obj, what):
-    if not isinstance(obj, (Filter, ModelFilter)):
+    if not isinstance(obj, (Filter, ModelFilter)) or isinstance(obj, ModelUnfilter):
         raise ValueError(f"'{what}' is only available for particle filters")
 
 
@@ -716,13 +774,12 @@ def dust_likelihood_last_state(obj, select_random_particle=False):
         st = obj.last_state()
         return st[:, 0, :] if select_random_particle else st
     if isinstance(obj, ModelFilter):
-        if obj.ptr is None:
-            raise DustError("System has not been run")
+        st = obj.last_state()
         if select_random_particle:
+            if isinstance(obj, ModelUnfilter):
+                return st[:, 0, :]
             raise ValueError("'select_random_particle' is not supported by this filter")
-        out = np.zeros((obj.n_groups, obj.n_particles, obj.generator.n_state))
-        call("dust2_filter_ode_last_state", obj.ptr, dptr(out))
-        return out
+        return st
     if select_random_particle:
         n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
         out = np.zeros((n_groups, obj.generator.n_state))
@@ -756,12 +813,9 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
     if isinstance(obj, Unfilter):
         out = obj.last_trajectories()
     elif isinstance(obj, ModelFilter):
-        if obj.ptr is None or not obj.traj_current:
-            raise DustError("Trajectories are not current")
         if select_random_particle:
             raise ValueError("'select_random_particle' is not supported by this filter")
-        out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
-        call("dust2_filter_ode_last_trajectories", obj.ptr, dptr(out))
+        out = obj.last_trajectories()
     else:
         if obj.ptr is None:
             raise DustError("Trajectories are not current")
@@ -1283,6 +1337,8 @@ class Unfilter:
 
 def dust_unfilter_create(generator, time_start, data, n_particles=1, n_groups=1, dt=None,
                          ode_control=None, precision="fp64"):
+    if generator.name in ModelFilter.DATA_FIELDS:
+        return ModelUnfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control)
     return Unfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control,
                     precision)
 

@@ -55,6 +55,7 @@ typedef struct dust2_unfilter_sir dust2_unfilter_sir;
 typedef struct dust2_ode_system dust2_ode_system;
 typedef struct dust2_ode_general dust2_ode_general;
 typedef struct dust2_filter_ode dust2_filter_ode;
+typedef struct dust2_unfilter_ode dust2_unfilter_ode;
 
 /* ---- library ------------------------------------------------------------ */
 
@@ -436,6 +437,29 @@ int dust2_filter_ode_rng_state(dust2_filter_ode* h, uint64_t* out, char* err, si
 int dust2_filter_ode_set_rng_state(dust2_filter_ode* h, const uint64_t* state, char* err,
                                    size_t errlen);
 void dust2_filter_ode_free(dust2_filter_ode* h);
+
+/* ---- deterministic likelihood over a general model with data ------------
+ * dust_unfilter_create for a model of the general engine with compare_data
+ * (malaria), replacing the unfilter registrations of
+ * inst/template/compare.cpp:6-29 over unfilter<dust_continuous<T>>
+ * (dust2/unfilter.hpp:42-72, 162-186): deterministic updates (their draws'
+ * means, dust2/r/continuous/unfilter.hpp:38-42), per data time run_to_time
+ * and compare_data, one log-likelihood per particle.  Arguments as
+ * dust2_filter_ode_alloc; ll_out[G][P]; trajectories out[n_time][G][P][S]. */
+int dust2_unfilter_ode_alloc(const char* model, const double* pars, size_t n_groups,
+                             size_t n_particles, double time_start, const double* time,
+                             size_t n_time, const double* data, size_t n_groups_data, double dt,
+                             const double* control, dust2_unfilter_ode** out, char* err,
+                             size_t errlen);
+int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories,
+                           double* ll_out, char* err, size_t errlen);
+int dust2_unfilter_ode_update_pars(dust2_unfilter_ode* h, const double* pars, char* err,
+                                   size_t errlen);
+int dust2_unfilter_ode_last_state(dust2_unfilter_ode* h, double* out, char* err,
+                                  size_t errlen);
+int dust2_unfilter_ode_last_trajectories(dust2_unfilter_ode* h, double* out, char* err,
+                                         size_t errlen);
+void dust2_unfilter_ode_free(dust2_unfilter_ode* h);
 
 /* ---- deterministic discrete-time SIR unfilter with the adjoint ----------
  * Replaces dust2_discrete_unfilter_alloc<sir> (dust2/r/discrete/unfilter.hpp:12-58),

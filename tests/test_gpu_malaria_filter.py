@@ -102,3 +102,37 @@ def test_filter_errors(gpu_lib):
     with pytest.raises(ValueError, match="continuous-time models"):
         gpu_lib.dust_filter_create(gpu_lib.sirode(), 0.0, {"time": [1.0], "incidence": [1.0]},
                                    10)
+
+
+def test_unfilter_equals_the_reference_solver_run(gpu_lib):
+    # dust_unfilter_create on the malaria model (dust2/unfilter.hpp:42-72):
+    # deterministic, so the likelihood is the binomial log density of the
+    # data at the reference solver's trajectory (the malaria_continuous
+    # golden case, a = 0.4, no updates)
+    from scipy import stats
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ode_mixed_dp5.json")) as f:
+        case = next(c for c in json.load(f)["cases"] if c["name"] == "malaria_continuous")
+    times = np.array(case["targets"][1:])
+    ih = np.array(case["particles"][0]["state"])[1:, 1]
+    tested = np.arange(times.size) % 7 + 3.0
+    positive = np.floor(tested * 0.3)
+    positive[4] = np.nan
+    data = {"time": times, "tested": tested, "positive": positive}
+    obj = gpu_lib.dust_unfilter_create(gpu_lib.malaria(), 0.0, data, n_particles=3)
+    ll = gpu_lib.dust_likelihood_run(obj, {"a": 0.4})
+    ok = ~np.isnan(positive)
+    want = stats.binom.logpmf(positive[ok], tested[ok], ih[ok]).sum()
+    np.testing.assert_allclose(ll, want, rtol=1e-10)
+    # deterministic particles are identical; trajectories are the states at
+    # the data times
+    assert np.all(ll == ll[0, 0])
+    gpu_lib.dust_likelihood_run(obj, None, save_trajectories=True)
+    h = gpu_lib.dust_likelihood_last_trajectories(obj)
+    np.testing.assert_allclose(h[:, 0, 0, 1], ih, rtol=1e-10)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_last_state(obj), h[-1])
+    with pytest.raises(ValueError, match="only available for particle filters"):
+        gpu_lib.dust_likelihood_rng_state(obj)
+    # with dt the deterministic updates keep beta, so only the solver's
+    # stopping points change
+    obj1 = gpu_lib.dust_unfilter_create(gpu_lib.malaria(), 0.0, data, dt=1)
+    np.testing.assert_allclose(gpu_lib.dust_likelihood_run(obj1, {"a": 0.4}), want, rtol=1e-6)
