@@ -140,7 +140,7 @@ _PROTOS = {
     "dust2_filter_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double, D,
                                          C.c_size_t, D, C.c_size_t, C.c_double, D, U64,
                                          C.c_size_t, C.c_uint64, C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_filter_ode_run": (C.c_int, [VP, D, C.c_int, D, ERR, ERRLEN]),
+    "dust2_filter_ode_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_filter_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
@@ -150,7 +150,7 @@ _PROTOS = {
     "dust2_unfilter_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double,
                                            D, C.c_size_t, D, C.c_size_t, C.c_double, D,
                                            C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_unfilter_ode_run": (C.c_int, [VP, D, C.c_int, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),

@@ -2761,7 +2761,8 @@ struct dust2_ode_general {
   bool deterministic = false;
   hipStream_t stream = nullptr;
   dbuf d_pars, d_state, d_internals, d_reinit, d_targets, d_out, d_errors, d_counters,
-      d_critical, d_step_times, d_step_count, d_events, d_history, d_rng, d_data, d_ll;
+      d_critical, d_step_times, d_step_count, d_events, d_history, d_rng, d_data, d_ll,
+      d_active;
   bool errors_pending = false;
   ~dust2_ode_general() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -2807,12 +2808,19 @@ void check_general(dust2_ode_general* h, const char* action) {
   }
 }
 
-void general_run(dust2_ode_general* h, const double* targets, size_t n, double* out) {
+// active: [G] groups to run (the others keep their state, time and solver
+// flags), or null for every group.
+void general_run(dust2_ode_general* h, const double* targets, size_t n, double* out,
+                 const std::vector<uint8_t>* active = nullptr) {
   hipStream_t s = h->stream;
   const size_t n_total = h->G * h->P;
   h->d_targets.alloc(std::max<size_t>(n, 1) * sizeof(double));
   if (n) to_device(h->d_targets, targets, n * sizeof(double), s);
   to_device(h->d_reinit, h->reinit.data(), h->G, s);
+  if (active) {
+    h->d_active.alloc(h->G);
+    to_device(h->d_active, active->data(), h->G, s);
+  }
   if (out) h->d_out.alloc(std::max<size_t>(n, 1) * h->S * n_total * sizeof(double));
   hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
   gen_launch a{};
@@ -2822,6 +2830,7 @@ void general_run(dust2_ode_general* h, const double* targets, size_t n, double* 
   a.n_particles = static_cast<int>(h->P);
   a.n_total = static_cast<int>(n_total);
   a.reinit = h->d_reinit.get<uint8_t>();
+  a.active = active ? h->d_active.get<uint8_t>() : nullptr;
   a.time = h->time;
   a.targets = h->d_targets.get<double>();
   a.n_targets = static_cast<int>(n);
@@ -2845,7 +2854,10 @@ void general_run(dust2_ode_general* h, const double* targets, size_t n, double* 
   hip_check(launch_general_system(a, s), "general ODE system kernel");
   uint32_t count = 0;
   to_host(&count, h->d_counters, sizeof(count), s);
-  std::fill(h->reinit.begin(), h->reinit.end(), 0);
+  // initialise_solver_ clears the flags of the groups it ran (system.hpp:444-482)
+  for (size_t g = 0; g < h->G; ++g) {
+    if (!active || (*active)[g]) h->reinit[g] = 0;
+  }
   if (count != 0) {
     std::vector<error_record> rec(n_total);
     to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
@@ -3305,7 +3317,8 @@ struct dust2_filter_ode {
   double time_start = 0;
   std::vector<double> times;
   dbuf d_data, d_frng, d_ll, d_ll_step, d_idx, d_state2, d_words2, d_traj, d_ancestry,
-      d_fallback, d_groups, d_out;
+      d_fallback, d_groups, d_out, d_run_groups;
+  std::vector<uint32_t> last_groups;  // groups of the last run
   bool traj_current = false;
   ~dust2_filter_ode() {
     if (sys) dust2_system_ode_free(sys);
@@ -3315,6 +3328,24 @@ struct dust2_filter_ode {
 namespace {
 
 void check_filter_ode(dust2_filter_ode* h) { check_handle(h, TAG_FILTER_ODE, "filter"); }
+
+// index_group (0-based, or null for every group) as a run list and a mask
+void run_groups(const size_t* index_group, size_t n_index, size_t G,
+                std::vector<uint32_t>& groups, std::vector<uint8_t>& active) {
+  groups.clear();
+  active.assign(G, 0);
+  if (index_group == nullptr) {
+    for (size_t g = 0; g < G; ++g) groups.push_back(static_cast<uint32_t>(g));
+    std::fill(active.begin(), active.end(), 1);
+    return;
+  }
+  for (size_t i = 0; i < n_index; ++i) {
+    if (index_group[i] >= G) throw std::runtime_error("'index_group' out of range");
+    if (active[index_group[i]]) throw std::runtime_error("'index_group' has duplicates");
+    active[index_group[i]] = 1;
+    groups.push_back(static_cast<uint32_t>(index_group[i]));
+  }
+}
 
 }  // namespace
 
@@ -3394,17 +3425,25 @@ int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_group
 }
 
 // filter<T>::run (dust2/filter.hpp:42-104) with reset (:156-175): every
-// group from time_start (initial: [G][P][S] or null for the model's own),
-// per data time run_to_time, compare_data, log-sum-exp, one uniform per
-// group and systematic resampling, then reorder.  ll_out[G].
+// group reset to time_start (initial: [G][P][S] or null for the model's
+// own), then for the groups of index_group (null: all) per data time
+// run_to_time, compare_data, log-sum-exp, one uniform per group and
+// systematic resampling, then reorder.  ll_out[n_index] in index order.
 int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories,
-                         double* ll_out, char* err, size_t errlen) {
+                         const size_t* index_group, size_t n_index, double* ll_out, char* err,
+                         size_t errlen) {
   try {
     check_filter_ode(h);
     dust2_ode_general* sys = h->sys;
     hipStream_t s = sys->stream;
     const size_t G = h->G, P = h->P, n_total = G * P;
     char buf[1024];
+    std::vector<uint32_t> groups;
+    std::vector<uint8_t> active;
+    run_groups(index_group, n_index, G, groups, active);
+    const bool all = groups.size() == G;
+    h->d_run_groups.alloc(G * sizeof(uint32_t));
+    to_device(h->d_run_groups, groups.data(), groups.size() * sizeof(uint32_t), s);
     // reset: set_time(time_start), then the initial state
     sys->errors_pending = false;
     sys->time = h->time_start;
@@ -3421,7 +3460,7 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
     }
     const int nd = sys->info.n_data;
     for (size_t t = 0; t < h->n_time; ++t) {
-      general_run(sys, &h->times[t], 1, nullptr);
+      general_run(sys, &h->times[t], 1, nullptr, all ? nullptr : &active);
       gen_compare_launch c{};
       c.model = sys->model;
       c.pars = sys->d_pars.get<double>();
@@ -3433,13 +3472,15 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
       c.data = h->d_data.get<double>() + t * h->n_groups_data * nd;
       c.shared_data = h->n_groups_data == 1 ? 1 : 0;
       c.out = h->d_ll_step.get<double>();
+      c.active = all ? nullptr : sys->d_active.get<uint8_t>();
       hip_check(launch_general_compare(c, s), "compare kernel");
       gen_resample_launch r{};
       r.ll_step = h->d_ll_step.get<double>();
       r.filter_rng = h->d_frng.get<uint64_t>();
       r.ll = h->d_ll.get<double>();
       r.idx = h->d_idx.get<uint32_t>();
-      r.n_groups = static_cast<int>(G);
+      r.groups = h->d_run_groups.get<uint32_t>();
+      r.n_groups = static_cast<int>(groups.size());
       r.n_particles = static_cast<int>(P);
       r.fallback_count = h->d_fallback.get<uint32_t>();
       hip_check(launch_general_resample(r, s), "resample kernel");
@@ -3457,11 +3498,15 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
       o.ancestry = save_trajectories ? h->d_ancestry.get<uint32_t>() : nullptr;
       o.n_time = static_cast<int>(h->n_time);
       o.t = static_cast<int>(t);
+      o.active = all ? nullptr : sys->d_active.get<uint8_t>();
       hip_check(launch_general_reorder(o, s), "reorder kernel");
       sys->d_state.swap(h->d_state2);
       sys->d_internals.swap(h->d_words2);
     }
-    to_host(ll_out, h->d_ll, G * sizeof(double), s);
+    std::vector<double> ll(G);
+    to_host(ll.data(), h->d_ll, G * sizeof(double), s);
+    for (size_t i = 0; i < groups.size(); ++i) ll_out[i] = ll[groups[i]];
+    h->last_groups = groups;
     h->traj_current = save_trajectories != 0;
     return 0;
   } catch (const std::exception& e) {
@@ -3489,17 +3534,17 @@ int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, siz
   }
 }
 
-// trajectories following the ancestry (dust2/trajectories.hpp:128-189):
-// out [n_time][G][P][S]
+// trajectories following the ancestry (dust2/trajectories.hpp:128-189) of
+// the groups of the last run: out [n_time][n_groups_run][P][S]
 int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
                                        size_t errlen) {
   try {
     check_filter_ode(h);
     if (!h->traj_current) throw std::runtime_error("Trajectories are not current");
     hipStream_t s = h->sys->stream;
-    const size_t G = h->G, P = h->P, S = h->sys->S, T = h->n_time;
-    std::vector<uint32_t> groups(G);
-    for (size_t g = 0; g < G; ++g) groups[g] = static_cast<uint32_t>(g);
+    const size_t P = h->P, S = h->sys->S, T = h->n_time;
+    const std::vector<uint32_t>& groups = h->last_groups;
+    const size_t G = groups.size();
     h->d_groups.alloc(G * sizeof(uint32_t));
     to_device(h->d_groups, groups.data(), G * sizeof(uint32_t), s);
     h->d_out.alloc(T * G * P * S * sizeof(double));
@@ -3572,6 +3617,7 @@ struct dust2_unfilter_ode {
   double time_start = 0;
   std::vector<double> times;
   dbuf d_data, d_ll, d_traj;
+  std::vector<uint32_t> last_groups;
   bool traj_current = false, has_run = false;
   ~dust2_unfilter_ode() {
     if (sys) dust2_system_ode_free(sys);
@@ -3621,17 +3667,23 @@ int dust2_unfilter_ode_alloc(const char* model, const double* pars, size_t n_gro
   }
 }
 
-// unfilter::run: from time_start (initial [G][P][S] or null), per data time
+// unfilter::run: every group reset to time_start (initial [G][P][S] or
+// null), then for the groups of index_group (null: all) per data time
 // run_to_time and compare_data, each particle's log-likelihood summed;
-// ll_out[G][P].
+// ll_out[n_index][P].
 int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories,
-                           double* ll_out, char* err, size_t errlen) {
+                           const size_t* index_group, size_t n_index, double* ll_out,
+                           char* err, size_t errlen) {
   try {
     check_handle(h, TAG_UNFILTER_ODE, "unfilter");
     dust2_ode_general* sys = h->sys;
     hipStream_t s = sys->stream;
     const size_t n_total = h->G * h->P;
     char buf[1024];
+    std::vector<uint32_t> groups;
+    std::vector<uint8_t> active;
+    run_groups(index_group, n_index, h->G, groups, active);
+    const bool all = groups.size() == h->G;
     sys->errors_pending = false;
     sys->time = h->time_start;
     std::fill(sys->reinit.begin(), sys->reinit.end(), 1);
@@ -3644,7 +3696,7 @@ int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int sav
     if (save_trajectories) h->d_traj.alloc(h->n_time * block * sizeof(double));
     const int nd = sys->info.n_data;
     for (size_t t = 0; t < h->n_time; ++t) {
-      general_run(sys, &h->times[t], 1, nullptr);
+      general_run(sys, &h->times[t], 1, nullptr, all ? nullptr : &active);
       gen_compare_launch c{};
       c.model = sys->model;
       c.pars = sys->d_pars.get<double>();
@@ -3657,6 +3709,7 @@ int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int sav
       c.shared_data = h->n_groups_data == 1 ? 1 : 0;
       c.out = h->d_ll.get<double>();
       c.accumulate = 1;
+      c.active = all ? nullptr : sys->d_active.get<uint8_t>();
       hip_check(launch_general_compare(c, s), "compare kernel");
       if (save_trajectories) {
         hip_check(hipMemcpyAsync(h->d_traj.get<double>() + t * block, sys->d_state.get(),
@@ -3664,7 +3717,13 @@ int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int sav
                   "copy trajectory");
       }
     }
-    to_host(ll_out, h->d_ll, n_total * sizeof(double), s);
+    std::vector<double> ll(n_total);
+    to_host(ll.data(), h->d_ll, n_total * sizeof(double), s);
+    for (size_t i = 0; i < groups.size(); ++i) {
+      std::copy(ll.data() + groups[i] * h->P, ll.data() + (groups[i] + 1) * h->P,
+                ll_out + i * h->P);
+    }
+    h->last_groups = groups;
     h->traj_current = save_trajectories != 0;
     h->has_run = true;
     return 0;
@@ -3695,17 +3754,23 @@ int dust2_unfilter_ode_last_state(dust2_unfilter_ode* h, double* out, char* err,
   }
 }
 
-// the state at every data time: out[n_time][G][P][S]
+// the state at every data time of the groups of the last run:
+// out[n_time][n_groups_run][P][S]
 int dust2_unfilter_ode_last_trajectories(dust2_unfilter_ode* h, double* out, char* err,
                                          size_t errlen) {
   try {
     check_handle(h, TAG_UNFILTER_ODE, "unfilter");
     if (!h->traj_current) throw std::runtime_error("Trajectories are not current");
-    const size_t n_total = h->G * h->P, S = h->sys->S, block = S * n_total;
-    std::vector<double> tmp(h->n_time * block);
+    const size_t n_total = h->G * h->P, S = h->sys->S, block = S * n_total, P = h->P;
+    std::vector<double> tmp(h->n_time * block), aos(block);
     to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->sys->stream);
+    const size_t n_out = h->last_groups.size();
     for (size_t t = 0; t < h->n_time; ++t) {
-      soa_to_aos(tmp.data() + t * block, out + t * block, 1, n_total, S, true);
+      soa_to_aos(tmp.data() + t * block, aos.data(), 1, n_total, S, true);
+      for (size_t i = 0; i < n_out; ++i) {
+        const double* src = aos.data() + h->last_groups[i] * P * S;
+        std::copy(src, src + P * S, out + (t * n_out + i) * P * S);
+      }
     }
     return 0;
   } catch (const std::exception& e) {

@@ -61,6 +61,7 @@ struct gen_launch {
   int n_particles;
   int n_total;
   const uint8_t* reinit;     // [G]: initialise the solver first
+  const uint8_t* active;     // [G]: groups to run, or null for all
   double time;               // system time at the start
   const double* targets;     // [n_targets]
   int n_targets;
@@ -94,6 +95,7 @@ struct gen_compare_launch {
   int shared_data;
   double* out;               // [n_total]
   int accumulate;            // add to out instead of writing it
+  const uint8_t* active;     // [G] or null for all
 };
 
 struct gen_resample_launch {
@@ -101,6 +103,7 @@ struct gen_resample_launch {
   uint64_t* filter_rng;      // [G][2][4]: streams 2g (resampling) and 2g + 1
   double* ll;                // [G] accumulated log-likelihood
   uint32_t* idx;             // [G][P] ancestors
+  const uint32_t* groups;    // [n_groups] groups to resample (one workgroup each)
   int n_groups;
   int n_particles;           // at most 8192
   uint32_t* fallback_count;
@@ -120,6 +123,7 @@ struct gen_reorder_launch {
   uint32_t* ancestry;        // [G][n_time][P] or null
   int n_time;
   int t;
+  const uint8_t* active;     // [G] or null: inactive groups keep their order
 };
 
 int gen_words(int model);

@@ -733,6 +733,7 @@ __global__ void __launch_bounds__(256) general_system_kernel(gen_launch a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_total) return;
   const int g = i / a.n_particles;
+  if (a.active && !a.active[g]) return;
   constexpr int NY = gstate<M>::NY;
   constexpr int NS = NY + M::n_output;
   constexpr int W = hist_words<M>::value;
@@ -817,6 +818,7 @@ __global__ void __launch_bounds__(256) general_compare_kernel(gen_compare_launch
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_total) return;
   const int g = i / a.n_particles;
+  if (a.active && !a.active[g]) return;
   constexpr int NS = M::n_var + M::n_special + M::n_output;
   const size_t NT = static_cast<size_t>(a.n_total);
   double y[NS];
@@ -835,7 +837,7 @@ __global__ void __launch_bounds__(256) general_resample_kernel(gen_resample_laun
   extern __shared__ double lds[];
   double* scratch = lds;
   double* cum = lds + kScratchDoubles;
-  const int g = blockIdx.x;
+  const int g = a.groups ? static_cast<int>(a.groups[blockIdx.x]) : static_cast<int>(blockIdx.x);
   const int P = a.n_particles;
   const int p0 = threadIdx.x * PPT;
   double w[PPT];
@@ -875,7 +877,8 @@ __global__ void __launch_bounds__(256) general_reorder_kernel(gen_reorder_launch
   const int P = a.n_particles;
   const int g = i / P, p = i - g * P;
   const size_t NT = static_cast<size_t>(a.n_total);
-  const uint32_t from = static_cast<uint32_t>(g) * P + a.idx[i];
+  const bool moved = a.active == nullptr || a.active[g];
+  const uint32_t from = moved ? static_cast<uint32_t>(g) * P + a.idx[i] : static_cast<uint32_t>(i);
   for (int k = 0; k < a.S; ++k) {
     const double v = a.state_in[k * NT + from];
     a.state_out[k * NT + i] = v;
@@ -887,7 +890,7 @@ __global__ void __launch_bounds__(256) general_reorder_kernel(gen_reorder_launch
   double* dst = a.words_out + static_cast<size_t>(i) * a.words;
   for (int k = 0; k < a.words; ++k) dst[k] = src[k];
   if (a.ancestry) {
-    a.ancestry[(static_cast<size_t>(g) * a.n_time + a.t) * P + p] = a.idx[i];
+    a.ancestry[(static_cast<size_t>(g) * a.n_time + a.t) * P + p] = moved ? a.idx[i] : p;
   }
 }
 

@@ -589,26 +589,38 @@ class ModelFilter:
              u64ptr(raw), n_raw, seed, C.byref(out))
         self.ptr = out
 
-    def run(self, pars, index_group=None, initial=None, save_trajectories=False):
-        if index_group is not None:
-            raise ValueError("'index_group' is not supported by this filter")
+    def _update(self, pars, index_group):
+        """Parameters for the groups of index_group (R/interface-likelihood.R:
+        54-75); returns the index array for the C call."""
+        idx, n_idx, n_out = _index_group(index_group, self.n_groups)
         if self.ptr is None:
             if pars is None:
                 raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
+            if index_group is not None:
+                raise ValueError("'index_group' must be NULL, as 'obj' is not initialised")
             self._alloc(pars)
         elif pars is not None:
-            rows = pars_update(self.generator, pars, self.rows)
-            call("dust2_filter_ode_update_pars", self.ptr, dptr(rows))
+            sel = np.arange(self.n_groups) if idx is None else idx.astype(np.int64)
+            rows = self.rows.copy()
+            rows[sel] = pars_update(self.generator, pars, self.rows[sel])
+            call(self.prefix + "_update_pars", self.ptr, dptr(rows))
             self.rows = rows
-        init = None
-        if initial is not None:
-            init = np.broadcast_to(
-                f64(initial), (self.n_groups, self.n_particles, self.generator.n_state)).copy()
-        ll = np.zeros(self.n_groups)
-        call("dust2_filter_ode_run", self.ptr, dptr(init), int(bool(save_trajectories)),
-             dptr(ll))
+        return idx, n_idx, n_out
+
+    def _initial(self, initial):
+        if initial is None:
+            return None
+        return np.broadcast_to(
+            f64(initial), (self.n_groups, self.n_particles, self.generator.n_state)).copy()
+
+    def run(self, pars, index_group=None, initial=None, save_trajectories=False):
+        idx, n_idx, n_out = self._update(pars, index_group)
+        ll = np.zeros(n_out)
+        call("dust2_filter_ode_run", self.ptr, dptr(self._initial(initial)),
+             int(bool(save_trajectories)), szptr(idx), n_idx, dptr(ll))
         self.traj_current = bool(save_trajectories)
-        return float(ll[0]) if self.n_groups == 1 else ll
+        self.last_index = index_group
+        return float(ll[0]) if (self.n_groups == 1 and index_group is None) else ll
 
     def last_state(self):
         if self.ptr is None:
@@ -620,7 +632,8 @@ class ModelFilter:
     def last_trajectories(self):
         if self.ptr is None or not self.traj_current:
             raise DustError("Trajectories are not current")
-        out = np.zeros((self.time.size, self.n_groups, self.n_particles, self.generator.n_state))
+        n = self.n_groups if self.last_index is None else np.size(self.last_index)
+        out = np.zeros((self.time.size, n, self.n_particles, self.generator.n_state))
         call(self.prefix + "_last_trajectories", self.ptr, dptr(out))
         return out
 
@@ -651,27 +664,16 @@ class ModelUnfilter(ModelFilter):
         self.ptr = out
 
     def run(self, pars, index_group=None, initial=None, save_trajectories=False):
-        if index_group is not None:
-            raise ValueError("'index_group' is not supported by this unfilter")
-        if self.ptr is None:
-            if pars is None:
-                raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
-            self._alloc(pars)
-        elif pars is not None:
-            rows = pars_update(self.generator, pars, self.rows)
-            call("dust2_unfilter_ode_update_pars", self.ptr, dptr(rows))
-            self.rows = rows
-        init = None
-        if initial is not None:
-            init = np.broadcast_to(
-                f64(initial), (self.n_groups, self.n_particles, self.generator.n_state)).copy()
-        ll = np.zeros((self.n_groups, self.n_particles))
-        call("dust2_unfilter_ode_run", self.ptr, dptr(init), int(bool(save_trajectories)),
-             dptr(ll))
+        idx, n_idx, n_out = self._update(pars, index_group)
+        ll = np.zeros((n_out, self.n_particles))
+        call("dust2_unfilter_ode_run", self.ptr, dptr(self._initial(initial)),
+             int(bool(save_trajectories)), szptr(idx), n_idx, dptr(ll))
         self.traj_current = bool(save_trajectories)
+        self.last_index = index_group
         if self.n_particles == 1:
             ll = ll[:, 0]
-        return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1) else ll
+        return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1 and index_group is None) \
+            else ll
 
 
 def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=1,
@@ -916,6 +918,15 @@ class System:
         return (self.n_groups, self.n_particles, self.generator.n_state)
 
 
+def _log_cap(on, n_rows):
+    """Entries kept per particle by a per-particle log (step times, history):
+    up to 100,000, fewer for many particles so that the log stays within
+    about 1 GiB of FP64 words, never below 1,000."""
+    if not on:
+        return 0
+    return int(max(1000, min(100000, (1 << 27) // max(n_rows, 1))))
+
+
 class ContinuousSystem:
     """A continuous-time system (dust_system_create for an ODE model,
     R/interface.R:60-150; dust2/continuous/system.hpp): FP64, solver
@@ -931,7 +942,7 @@ class ContinuousSystem:
         self.rows = pars_matrix(generator, pars, self.n_groups)
         self.ode_control = _ode_control(ode_control)
         ctl = self.ode_control
-        self.record = 100000 if ctl.debug_record_step_times else 0
+        self.record = _log_cap(ctl.debug_record_step_times, self.n_groups * self.n_particles)
         out = C.c_void_p()
         call("dust2_system_sirode_alloc", dptr(self.rows), self.n_groups, self.n_particles,
              float(time), dptr(ctl.values), dptr(ctl.critical_times), ctl.critical_times.size,
@@ -972,10 +983,13 @@ class GeneralSystem:
             raise ValueError(f"Parameter layout of '{generator.name}' does not match the library")
         self.ode_control = _ode_control(ode_control)
         ctl = self.ode_control
-        self.record = 100000 if ctl.debug_record_step_times else 0
+        n_total = self.n_groups * self.n_particles
+        self.record = _log_cap(ctl.debug_record_step_times, n_total)
         self.save_history = bool(ctl.save_history) or self.has_delays
         if history_cap is None:
-            history_cap = 100000 if ctl.save_history else 4096
+            # every step for inspection, else enough for the delays
+            history_cap = _log_cap(True, n_total * (3 + 5 * self.n_hist)) \
+                if ctl.save_history else 4096
         self.history_cap = int(history_cap) if self.save_history else 0
         self.event_cap = int(event_cap) if self.max_events else 0
         raw, n_raw, s = _seed_args(seed)

@@ -423,13 +423,16 @@ int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_group
                            const double* control, const uint64_t* seed_state,
                            size_t n_seed_streams, uint64_t seed, dust2_filter_ode** out,
                            char* err, size_t errlen);
-/* initial: [G][P][S] or NULL (the model's initial state); ll_out[G] */
+/* initial: [G][P][S] or NULL (the model's initial state); index_group
+ * (0-based) and n_index, or NULL for every group; ll_out[n_index] */
 int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories,
-                         double* ll_out, char* err, size_t errlen);
+                         const size_t* index_group, size_t n_index, double* ll_out, char* err,
+                         size_t errlen);
 int dust2_filter_ode_update_pars(dust2_filter_ode* h, const double* pars, char* err,
                                  size_t errlen);
 int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, size_t errlen);
-/* out[n_time][G][P][S], each particle's lineage (dust2/trajectories.hpp:128-189) */
+/* out[n_time][groups of the last run][P][S], each particle's lineage
+ * (dust2/trajectories.hpp:128-189) */
 int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
                                        size_t errlen);
 /* [G][P + 2][4] words: per group the 2 filter streams, then the particles' */
@@ -445,14 +448,16 @@ void dust2_filter_ode_free(dust2_filter_ode* h);
  * (dust2/unfilter.hpp:42-72, 162-186): deterministic updates (their draws'
  * means, dust2/r/continuous/unfilter.hpp:38-42), per data time run_to_time
  * and compare_data, one log-likelihood per particle.  Arguments as
- * dust2_filter_ode_alloc; ll_out[G][P]; trajectories out[n_time][G][P][S]. */
+ * dust2_filter_ode_alloc; index_group as dust2_filter_ode_run; ll_out[n_index][P];
+ * trajectories out[n_time][groups of the last run][P][S]. */
 int dust2_unfilter_ode_alloc(const char* model, const double* pars, size_t n_groups,
                              size_t n_particles, double time_start, const double* time,
                              size_t n_time, const double* data, size_t n_groups_data, double dt,
                              const double* control, dust2_unfilter_ode** out, char* err,
                              size_t errlen);
 int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories,
-                           double* ll_out, char* err, size_t errlen);
+                           const size_t* index_group, size_t n_index, double* ll_out,
+                           char* err, size_t errlen);
 int dust2_unfilter_ode_update_pars(dust2_unfilter_ode* h, const double* pars, char* err,
                                    size_t errlen);
 int dust2_unfilter_ode_last_state(dust2_unfilter_ode* h, double* out, char* err,

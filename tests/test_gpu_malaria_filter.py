@@ -136,3 +136,39 @@ def test_unfilter_equals_the_reference_solver_run(gpu_lib):
     # stopping points change
     obj1 = gpu_lib.dust_unfilter_create(gpu_lib.malaria(), 0.0, data, dt=1)
     np.testing.assert_allclose(gpu_lib.dust_likelihood_run(obj1, {"a": 0.4}), want, rtol=1e-6)
+
+
+def test_index_group_runs_a_subset(gpu_lib):
+    # R/interface-likelihood.R:54-75: a subset of groups equals those groups
+    # of the whole run, their streams untouched by the others
+    times = np.array([3.0, 6.0, 9.0])
+    G, P = 3, 32
+    data = {"time": times, "tested": np.full((3, G), 6.0),
+            "positive": np.array([[1, 2, 3], [2, 3, 1], [3, 1, 2]], dtype=float)}
+    pars = [{"a": 0.3}, {"a": 0.35}, {"a": 0.4}]
+    streams = ol.prng(G * (P + 2), seed=7)
+    whole = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, P, n_groups=G, dt=1,
+                                       seed=streams)
+    ll = gpu_lib.dust_likelihood_run(whole, pars, save_trajectories=True)
+    h = gpu_lib.dust_likelihood_last_trajectories(whole)
+    part = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, P, n_groups=G, dt=1,
+                                      seed=streams)
+    gpu_lib.dust_likelihood_run(part, pars)
+    gpu_lib.dust_likelihood_set_rng_state(part, streams)
+    sub = gpu_lib.dust_likelihood_run(part, None, index_group=[2, 0], save_trajectories=True)
+    np.testing.assert_array_equal(sub, ll[[2, 0]])
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_last_trajectories(part), h[:, [2, 0]])
+    # the group left out keeps its initial state and its streams
+    st = gpu_lib.dust_likelihood_last_state(part)
+    np.testing.assert_array_equal(st[1], np.broadcast_to(st[1, :1], st[1].shape))
+    rng = gpu_lib.dust_likelihood_rng_state(part).reshape(G, P + 2, 4)
+    np.testing.assert_array_equal(rng[1], streams.reshape(G, P + 2, 4)[1])
+    # the unfilter: a subset's rows equal the whole run's
+    un = gpu_lib.dust_unfilter_create(gpu_lib.malaria(), 0.0, data, n_particles=2, n_groups=G)
+    ll_un = gpu_lib.dust_likelihood_run(un, pars)
+    np.testing.assert_array_equal(gpu_lib.dust_likelihood_run(un, None, index_group=[1]),
+                                  ll_un[[1]])
+    with pytest.raises(ValueError, match="'index_group' must be NULL"):
+        gpu_lib.dust_likelihood_run(
+            gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, P, n_groups=G, dt=1),
+            pars, index_group=[0])
