@@ -229,7 +229,12 @@ def bench_sir_replicates(lib, args):
     obj = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
                                  n_groups=G, dt=dt, seed=1, precision=prec)
     plist = [dict(zip(SIR_NAMES, row))] * G
-    tuned = None if args.no_tune else obj.autotune(plist)
+    if args.launch:
+        obj.set_launch(*[int(x) for x in args.launch.split(",")])
+    if args.engine != "auto" or args.launch:
+        obj.set_engine(args.engine, graph=not args.no_graph)
+    tuned = None if (args.no_tune or args.launch or args.engine != "auto") else \
+        obj.autotune(plist)
     ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
     n_steps = 400
     stats = lib.dust_likelihood_last_stats(obj)
