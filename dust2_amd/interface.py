@@ -760,9 +760,13 @@ def dust_likelihood_copy(obj, seed=None):
     return dst
 
 
-def _filter_only(obj, what):
+def _filter_only(obj, what, engine=False):
+    """Particle filters only; with engine, only the filters of the discrete
+    SIR/SEIR engines (their statistics, ancestry and raw trajectories)."""
     if not isinstance(obj, (Filter, ModelFilter)) or isinstance(obj, ModelUnfilter):
         raise ValueError(f"'{what}' is only available for particle filters")
+    if engine and isinstance(obj, ModelFilter):
+        raise ValueError(f"'{what}' is not available for filters over continuous-time models")
 
 
 def dust_likelihood_last_state(obj, select_random_particle=False):
@@ -832,14 +836,14 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
 
 def dust_likelihood_last_trajectories_raw(obj):
     """The saved post-resampling states, not following the ancestry."""
-    _filter_only(obj, "last_trajectories_raw")
+    _filter_only(obj, "last_trajectories_raw", engine=True)
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles, obj.generator.n_state))
     call("dust2_filter_last_trajectories_raw", obj.ptr, dptr(out))
     return out
 
 
 def dust_likelihood_last_ancestry(obj):
-    _filter_only(obj, "last_ancestry")
+    _filter_only(obj, "last_ancestry", engine=True)
     out = np.zeros((obj.time.size, obj.n_groups, obj.n_particles), dtype=np.uint32)
     call("dust2_filter_last_ancestry", obj.ptr, u32ptr(out))
     return out
@@ -879,7 +883,7 @@ def dust_likelihood_set_rng_state(obj, state):
 
 
 def dust_likelihood_last_stats(obj):
-    _filter_only(obj, "last_stats")
+    _filter_only(obj, "last_stats", engine=True)
     out = np.zeros(7)
     call("dust2_filter_last_stats", obj.ptr, dptr(out))
     engines = {1: "persistent", 2: "pipeline"}
