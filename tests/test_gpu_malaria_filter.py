@@ -172,8 +172,8 @@ def test_index_group_runs_a_subset(gpu_lib):
         gpu_lib.dust_likelihood_run(
             gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, P, n_groups=G, dt=1),
             pars, index_group=[0])
-    obj = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, 10, dt=1)
-    gpu_lib.dust_likelihood_run(obj, {})
+    obj = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, 10, n_groups=G, dt=1)
+    gpu_lib.dust_likelihood_run(obj, pars)
     for fn in [gpu_lib.dust_likelihood_last_stats, gpu_lib.dust_likelihood_last_ancestry]:
         with pytest.raises(ValueError, match="not available for filters over continuous-time"):
             fn(obj)
