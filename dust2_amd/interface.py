@@ -397,10 +397,40 @@ def _index_group(index_group, n_groups):
     return idx, idx.size, idx.size
 
 
-def prepare_data(data, n_groups, name="incidence", n_data=1):
+def _r_num(x):
+    """A number as R's cli prints it in messages (6, 0.25)."""
+    return f"{float(x):g}"
+
+
+def check_time_start(time_start, time):
+    """R/filter-support.R:94-104."""
+    if float(time_start) > float(time[0]):
+        raise ValueError(f"'time_start' ({_r_num(time_start)}) is later than the first time "
+                         f"in 'data' ({_r_num(time[0])})")
+
+
+def check_data_groups(n_groups_data, n_groups, shared_data):
+    """R/data.R:127-140: shared_data = True takes one group of data for
+    every parameter group, False one per group; None (this mirror's
+    default) accepts either."""
+    if shared_data is None:
+        return
+    expected = 1 if shared_data else n_groups
+    if n_groups_data != expected:
+        raise ValueError(f"Expected 'data' to have {expected} groups, but it had "
+                         f"{n_groups_data}")
+
+
+def prepare_data(data, n_groups, name="incidence", n_data=1, shared_data=None):
     """Data as in R/data.R:114-175: a dict with 'time' (n_time) and the
     observed values, shape (n_time,) / (n_time, n_data) shared by all groups,
     or (n_time, n_groups) / (n_time, n_groups, n_data)."""
+    out = _prepare_data(data, n_groups, name, n_data)
+    check_data_groups(out[2], n_groups, shared_data)
+    return out
+
+
+def _prepare_data(data, n_groups, name, n_data):
     time = f64(data["time"]).reshape(-1)
     nt = time.shape[0]
     obs = f64(data[name])
@@ -416,6 +446,11 @@ def prepare_data(data, n_groups, name="incidence", n_data=1):
             return time, np.ascontiguousarray(obs.reshape(nt, 1, n_data)), 1
         if obs.shape == (nt, n_groups, n_data):
             return time, np.ascontiguousarray(obs), n_groups
+    per_group = obs.ndim == (2 if n_data == 1 else 3) and obs.shape[0] == nt and \
+        (n_data == 1 or obs.shape[2] == n_data)
+    if per_group:
+        # R/data.R:134-139
+        raise ValueError(f"Expected 'data' to have {n_groups} groups, but it had {obs.shape[1]}")
     raise ValueError(f"Expected '{name}' to have {nt} rows of 1 or {n_groups} groups"
                      + (f" x {n_data} values" if n_data > 1 else ""))
 
@@ -428,13 +463,14 @@ class Filter:
     lazily on the first likelihood run (R/interface-likelihood.R:48-119)."""
 
     def __init__(self, generator, time_start, data, n_particles, n_groups, dt, seed,
-                 precision):
+                 precision, shared_data=None):
         if generator.time_type != "discrete":
             raise ValueError("Can't use 'dust_filter_create()' with continuous-time models")
         self.generator = generator
         self.time_start = float(time_start)
-        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups,
-                                                                n_data=generator.n_data)
+        self.time, self.data, self.n_groups_data = prepare_data(
+            data, n_groups, n_data=generator.n_data, shared_data=shared_data)
+        check_time_start(time_start, self.time)
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
         self.dt = float(dt)
@@ -548,7 +584,7 @@ class ModelFilter:
     DATA_FIELDS = {"malaria": ["tested", "positive"]}
 
     def __init__(self, generator, time_start, data, n_particles, n_groups, dt, seed,
-                 ode_control=None):
+                 ode_control=None, shared_data=None):
         self.generator = generator
         self.time_start = float(time_start)
         self.n_particles = int(n_particles)
@@ -561,8 +597,11 @@ class ModelFilter:
         cols = [np.asarray(f64(data[k])) for k in fields]
         cols = [c.reshape(self.time.size, -1) for c in cols]
         self.n_groups_data = cols[0].shape[1]
+        check_data_groups(self.n_groups_data, self.n_groups, shared_data)
         if self.n_groups_data not in (1, self.n_groups):
-            raise ValueError("Expected data for one group or every group")
+            raise ValueError(f"Expected 'data' to have {self.n_groups} groups, but it had "
+                             f"{self.n_groups_data}")
+        check_time_start(time_start, self.time)
         # [time][group][field]
         self.data = np.ascontiguousarray(np.stack(cols, axis=2))
         if seed is None:
@@ -651,8 +690,10 @@ class ModelUnfilter(ModelFilter):
     updates, one log-likelihood per particle."""
     prefix = "dust2_unfilter_ode"
 
-    def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control):
-        super().__init__(generator, time_start, data, n_particles, n_groups, dt, 0, ode_control)
+    def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control,
+                 shared_data=None):
+        super().__init__(generator, time_start, data, n_particles, n_groups, dt, 0, ode_control,
+                         shared_data)
 
     def _alloc(self, pars):
         self.rows = pars_matrix(self.generator, pars, self.n_groups)
@@ -677,11 +718,15 @@ class ModelUnfilter(ModelFilter):
 
 
 def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=1,
-                       seed=None, precision="fp64", ode_control=None):
+                       seed=None, precision="fp64", ode_control=None, shared_data=None):
+    """R/interface-filter.R:37-97.  shared_data: True for one group of data
+    used by every parameter group, False for one group of data per
+    parameter group; None (the default here) accepts either."""
     if generator.time_type == "mixed":
         return ModelFilter(generator, time_start, data, n_particles, n_groups, dt, seed,
-                           ode_control)
-    return Filter(generator, time_start, data, n_particles, n_groups, dt, seed, precision)
+                           ode_control, shared_data)
+    return Filter(generator, time_start, data, n_particles, n_groups, dt, seed, precision,
+                  shared_data)
 
 
 def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_group=None,
@@ -1229,7 +1274,7 @@ class Unfilter:
     give identical values)."""
 
     def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control,
-                 precision):
+                 precision, shared_data=None):
         if generator.name == "sirode":
             if dt is not None:
                 raise ValueError("Can't use 'dt' with continuous-time systems")
@@ -1243,7 +1288,9 @@ class Unfilter:
         self.generator = generator
         self.continuous = generator.name == "sirode"
         self.time_start = float(time_start)
-        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups)
+        self.time, self.data, self.n_groups_data = prepare_data(data, n_groups,
+                                                                shared_data=shared_data)
+        check_time_start(time_start, self.time)
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
         self.dt = 1.0 if dt is None else float(dt)
@@ -1354,11 +1401,13 @@ class Unfilter:
 
 
 def dust_unfilter_create(generator, time_start, data, n_particles=1, n_groups=1, dt=None,
-                         ode_control=None, precision="fp64"):
+                         ode_control=None, precision="fp64", shared_data=None):
+    """R/interface-unfilter.R:20-71; shared_data as dust_filter_create."""
     if generator.name in ModelFilter.DATA_FIELDS:
-        return ModelUnfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control)
+        return ModelUnfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control,
+                             shared_data)
     return Unfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control,
-                    precision)
+                    precision, shared_data)
 
 
 def dust_likelihood_last_gradient(obj):

@@ -96,9 +96,9 @@ def test_trajectories_follow_the_lineage(gpu_lib):
 
 def test_filter_errors(gpu_lib):
     data = {"time": [4.0, 8.0], "tested": [2, 4], "positive": [1, 2]}
-    with pytest.raises(gpu_lib.DustError, match="Expected 'time\\[1\\]'"):
-        gpu_lib.dust_likelihood_run(
-            gpu_lib.dust_filter_create(gpu_lib.malaria(), 5.0, data, 10, dt=1), {})
+    with pytest.raises(ValueError,
+                       match=r"'time_start' \(5\) is later than the first time in 'data' \(4\)"):
+        gpu_lib.dust_filter_create(gpu_lib.malaria(), 5.0, data, 10, dt=1)
     with pytest.raises(ValueError, match="continuous-time models"):
         gpu_lib.dust_filter_create(gpu_lib.sirode(), 0.0, {"time": [1.0], "incidence": [1.0]},
                                    10)
