@@ -1091,10 +1091,15 @@ def dust_system_run_to_time(sys, time):
     call(_prefix(sys) + "_run_to_time", sys.ptr, float(time))
 
 
-def dust_system_simulate(sys, times):
+def dust_system_simulate(sys, times, index_state=None):
+    """The state at each time, (n_times, n_groups, n_particles, n_state);
+    index_state (0-based) keeps some states (R/interface.R, simulate)."""
     t = f64(times).reshape(-1)
     out = np.zeros((t.size,) + sys.shape)
     call(_prefix(sys) + "_simulate", sys.ptr, dptr(t), t.size, dptr(out))
+    if index_state is not None:
+        idx = check_index(index_state, sys.shape[2], "index_state")
+        out = np.ascontiguousarray(out[..., idx])
     return out
 
 
@@ -1144,6 +1149,10 @@ def dust_system_compare_data(sys, data):
         call("dust2_system_ode_compare_data", sys.ptr, dptr(d), d.shape[0], dptr(out))
         return out
     d = f64(data).reshape(-1)
+    if d.size not in (1, sys.n_groups):
+        # R/interface.R (dust_system_compare_data)
+        raise ValueError(f"Expected 'data' to have length {sys.n_groups}, but it had length "
+                         f"{d.size}")
     out = np.zeros((sys.n_groups, sys.n_particles))
     call(_prefix(sys) + "_compare_data", sys.ptr, dptr(d), d.size, dptr(out))
     return out
