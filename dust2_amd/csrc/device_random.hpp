@@ -128,27 +128,25 @@ __device__ __forceinline__ real stirling_tail(real k) {
 
 // Inversion by sequential search, n*p < 10, p <= 1/2:
 // pmf(k) = pmf(k-1) * (p/q) * (n+1-k)/k.  Restarts with a fresh uniform if
-// rounding lets the search run past n.
+// rounding lets the search run past n.  One exit test per step: a search
+// that reaches k = n + 1 also updates f once more, a value the restart
+// discards, so the draws are those of testing k > n before the update.
 template <typename real>
 __device__ real binomial_inversion(rng256& rng, real n, real p) {
   const real q = 1 - p;
   const real r = p / q;
   const real g = r * (n + 1);
+  const real f0 = pow_one_minus(q, p, n);
   while (true) {
     real u = random_real<real>(rng);
-    real f = pow_one_minus(q, p, n);
+    real f = f0;
     real k = 0;
-    bool ok = true;
-    while (u >= f) {
+    while (u >= f && k <= n) {
       u -= f;
       k += 1;
-      if (k > n) {
-        ok = false;
-        break;
-      }
       f *= (g / k - r);
     }
-    if (ok) {
+    if (k <= n) {
       return k;
     }
   }
@@ -171,37 +169,37 @@ __device__ real binomial_btrs(rng256& rng, real n, real p) {
   bool have_bound = false;
   double alpha = 0, lpq = 0, m = 0, h = 0;
   const double nd = n;
-  while (true) {
+  // one exit test per proposal (a single loop exit keeps the divergent
+  // control flow cheap); every proposal draws u and v once, as before
+  real k;
+  bool accept;
+  do {
     const real u = random_real<real>(rng) - static_cast<real>(0.5);
     const real v = random_real<real>(rng);
     const real us = static_cast<real>(0.5) - d_fabs(u);
-    const real k = d_floor((2 * a / us + b) * u + c);
-    if (us >= static_cast<real>(0.07) && v <= v_r) {
-      return k;
+    k = d_floor((2 * a / us + b) * u + c);
+    accept = us >= static_cast<real>(0.07) && v <= v_r;
+    if (!accept && k >= 0 && k <= n) {
+      if (!have_bound) {
+        const double pd = p;
+        alpha = (2.83 + 5.1 / static_cast<double>(b)) * static_cast<double>(spq);
+        lpq = log(pd / static_cast<double>(q));
+        m = floor((nd + 1) * pd);
+        h = (m + 0.5) * log(m + 1) + stirling_tail(m) + (nd - m + 0.5) * log(nd - m + 1) +
+            stirling_tail(nd - m);
+        have_bound = true;
+      }
+      const double kd = k;
+      const double usd = us;
+      const double lv = log(static_cast<double>(v) * alpha /
+                            (static_cast<double>(a) / (usd * usd) + static_cast<double>(b)));
+      const double bound = h - (kd + 0.5) * log(kd + 1) - stirling_tail(kd) -
+                           (nd - kd + 0.5) * log(nd - kd + 1) - stirling_tail(nd - kd) +
+                           (kd - m) * lpq;
+      accept = lv <= bound;
     }
-    if (k < 0 || k > n) {
-      continue;
-    }
-    if (!have_bound) {
-      const double pd = p;
-      alpha = (2.83 + 5.1 / static_cast<double>(b)) * static_cast<double>(spq);
-      lpq = log(pd / static_cast<double>(q));
-      m = floor((nd + 1) * pd);
-      h = (m + 0.5) * log(m + 1) + stirling_tail(m) + (nd - m + 0.5) * log(nd - m + 1) +
-          stirling_tail(nd - m);
-      have_bound = true;
-    }
-    const double kd = k;
-    const double usd = us;
-    const double lv = log(static_cast<double>(v) * alpha /
-                          (static_cast<double>(a) / (usd * usd) + static_cast<double>(b)));
-    const double bound = h - (kd + 0.5) * log(kd + 1) - stirling_tail(kd) -
-                         (nd - kd + 0.5) * log(nd - kd + 1) - stirling_tail(nd - kd) +
-                         (kd - m) * lpq;
-    if (lv <= bound) {
-      return k;
-    }
-  }
+  } while (!accept);
+  return k;
 }
 
 // ---- FP32 fast path ----------------------------------------------------------
