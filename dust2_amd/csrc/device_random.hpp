@@ -306,34 +306,34 @@ __device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
   const float rb = f_rcp(b);
   const float v_r = 0.92f - 4.2f * rb;
   const float two_a = 2 * a;
-  while (true) {
+  // one exit test per proposal (cheap divergent control flow); every
+  // proposal draws one (u, v) pair
+  float k;
+  bool accept;
+  do {
     float u, v;
     random_pair_f32(rng, u, v);
     u -= 0.5f;
     const float us = 0.5f - fabsf(u);
-    const float k = floorf((two_a * f_rcp(us) + b) * u + c);
-    if (us >= 0.07f && v <= v_r) {
-      return k;
+    k = floorf((two_a * f_rcp(us) + b) * u + c);
+    accept = us >= 0.07f && v <= v_r;
+    if (!accept && k >= 0 && k <= n) {
+      const float alpha = (2.83f + 5.1f * rb) * spq;
+      const float m = floorf((n + 1) * p);
+      const float d = k - m;
+      const float rm = f_rcp(m + 1);
+      const float rnm = f_rcp(n - m + 1);
+      const float L = f_log(p * (n - m + 1) * f_rcp(q * (m + 1)));
+      const float ratio = d * L - (k + 0.5f) * f_log1p(d * rm) -
+                          (n - k + 0.5f) * f_log1p(-d * rnm) + stirling_tail_f32(m) +
+                          stirling_tail_f32(n - m) - stirling_tail_f32(k) -
+                          stirling_tail_f32(n - k);
+      const float us2 = us * us;
+      const float lv = f_log(v * alpha * us2 * f_rcp(a + b * us2));
+      accept = lv <= ratio;
     }
-    if (k < 0 || k > n) {
-      continue;
-    }
-    const float alpha = (2.83f + 5.1f * rb) * spq;
-    const float m = floorf((n + 1) * p);
-    const float d = k - m;
-    const float rm = f_rcp(m + 1);
-    const float rnm = f_rcp(n - m + 1);
-    const float L = f_log(p * (n - m + 1) * f_rcp(q * (m + 1)));
-    const float ratio = d * L - (k + 0.5f) * f_log1p(d * rm) -
-                        (n - k + 0.5f) * f_log1p(-d * rnm) + stirling_tail_f32(m) +
-                        stirling_tail_f32(n - m) - stirling_tail_f32(k) -
-                        stirling_tail_f32(n - k);
-    const float us2 = us * us;
-    const float lv = f_log(v * alpha * us2 * f_rcp(a + b * us2));
-    if (lv <= ratio) {
-      return k;
-    }
-  }
+  } while (!accept);
+  return k;
 }
 
 template <typename real>
