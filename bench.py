@@ -291,7 +291,8 @@ def bench_sirode(lib, args):
             el += _time.perf_counter() - t0
             ms.append(obj.last_stats()[1])  # kernel time, read outside the timed run
         stats, _ = obj.last_stats()
-        res[prec] = {"evals_per_s": n * args.steps / el, "kernel_ms": float(np.mean(ms)),
+        res[prec] = {"evals_per_s": n * args.steps / el, "wall_ms": el / args.steps * 1e3,
+                     "kernel_ms": float(np.mean(ms)),
                      "attempts_per_s": float(stats[:, 0].sum()) / (np.mean(ms) * 1e-3),
                      "mean_steps": float(stats[:, 0].mean()), "ll": ll}
     # likelihood and gradient (beta, gamma, I0) by forward sensitivities, FP64
@@ -305,7 +306,7 @@ def bench_sirode(lib, args):
     grad = gobj.last_gradient()
     out = {"metric": "likelihood evals/s", "value": res["fp64"]["evals_per_s"],
            "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": res["fp64"]["kernel_ms"], "higher_is_better": True, "scaling": "weak",
+           "ms_per_step": res["fp64"]["wall_ms"], "higher_is_better": True, "scaling": "weak",
            "gradient_evals_per_s": n * args.steps / grad_el,
            "gradient_kernel_ms": gobj.last_stats()[1],
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SIR simulation)",
