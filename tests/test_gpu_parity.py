@@ -111,6 +111,38 @@ def test_binomial_fp32_distribution(gpu_lib, n, p):
     assert abs(draws.mean() - mean) < 5 * sd / np.sqrt(draws.size)
 
 
+def test_binomial_fp32_mixed_wavefronts(gpu_lib):
+    """Neighbouring lanes of one wavefront draw with different parameters:
+    inversion (n q < 10) and BTRS lanes side by side, either side of the
+    threshold, with p > 1/2 flipped and trivial lanes (n = 0, p = 0, p = 1)
+    between them, so that lanes leave the search and rejection loops at
+    different iterations.  Each parameter set's draws are held to its exact
+    pmf."""
+    from scipy import stats
+    params = [(30, 0.2), (100, 0.3), (1e6, 9.9e-6), (1e6, 1.01e-5), (0, 0.3), (19, 0.52),
+              (400, 0.95), (12, 1.0), (5000, 0.0), (21, 0.5), (2000, 0.01), (3, 0.4)]
+    n_streams, n_draws = 64 * 60, 100
+    nn = np.array([params[i % len(params)][0] for i in range(n_streams)], dtype=float)
+    pp = np.array([params[i % len(params)][1] for i in range(n_streams)], dtype=float)
+    rng = ol.prng(n_streams, seed=5)
+    out, err, _ = gpu_lib.gpu_sample("binomial", rng, n_draws, nn, pp, precision="fp32")
+    assert not err.any()
+    for j, (n, p) in enumerate(params):
+        draws = out[j::len(params)].reshape(-1)
+        assert np.all(draws == np.round(draws)) and draws.min() >= 0 and draws.max() <= n
+        if n == 0 or p == 0:
+            assert np.all(draws == 0)
+            continue
+        if p == 1:
+            assert np.all(draws == n)
+            continue
+        sd = np.sqrt(n * p * (1 - p))
+        lo, hi = int(max(0, n * p - 8 * sd)), int(min(n, n * p + 8 * sd)) + 1
+        support = np.arange(lo, hi)
+        assert chisq_pvalue(draws, lambda k: stats.binom.pmf(k, n, p), support) > 1e-4, (n, p)
+        assert abs(draws.mean() - n * p) < 5 * sd / np.sqrt(draws.size), (n, p)
+
+
 def test_binomial_invalid_reports(gpu_lib):
     rng = ol.prng(2, seed=1)
     out, err, _ = gpu_lib.gpu_sample("binomial", rng, 1, [10.0, 10.0], [0.5, -0.2])
