@@ -143,6 +143,45 @@ def test_binomial_fp32_mixed_wavefronts(gpu_lib):
         assert abs(draws.mean() - n * p) < 5 * sd / np.sqrt(draws.size), (n, p)
 
 
+def test_exponential_normal_poisson_fp32_distribution(gpu_lib):
+    """The FP32 exponential (the compare noise of the FP32 filter), normal and
+    Poisson samplers against their exact distributions, parameters
+    interleaved lane by lane: Kolmogorov-Smirnov for the continuous ones,
+    chi-square on the pmf for the Poisson."""
+    from scipy import stats
+    n_streams, n_draws = 64 * 40, 100
+    rates = [0.5, 1.0, 1e6]
+    rng = ol.prng(n_streams, seed=8)
+    out, err, _ = gpu_lib.gpu_sample("exponential", rng, n_draws,
+                                     np.resize(rates, n_streams), precision="fp32")
+    assert not err.any()
+    for j, rate in enumerate(rates):
+        draws = out[j::len(rates)].reshape(-1)
+        assert draws.min() >= 0
+        assert stats.kstest(draws, stats.expon(scale=1 / rate).cdf).pvalue > 1e-4, rate
+    mu_sd = [(0.0, 1.0), (3.0, 0.5), (-20.0, 4.0)]
+    rng = ol.prng(n_streams, seed=9)
+    out, err, _ = gpu_lib.gpu_sample("normal", rng, n_draws,
+                                     np.resize([m for m, _ in mu_sd], n_streams),
+                                     np.resize([s for _, s in mu_sd], n_streams), precision="fp32")
+    assert not err.any()
+    for j, (mu, sd) in enumerate(mu_sd):
+        draws = out[j::len(mu_sd)].reshape(-1)
+        assert stats.kstest(draws, stats.norm(mu, sd).cdf).pvalue > 1e-4, (mu, sd)
+    means = [0.3, 4.0, 9.5, 20.0, 800.0]
+    rng = ol.prng(n_streams, seed=10)
+    out, err, _ = gpu_lib.gpu_sample("poisson", rng, n_draws, np.resize(means, n_streams),
+                                     precision="fp32")
+    assert not err.any()
+    for j, lam in enumerate(means):
+        draws = out[j::len(means)].reshape(-1)
+        assert np.all(draws == np.round(draws)) and draws.min() >= 0
+        sd = np.sqrt(lam)
+        support = np.arange(int(max(0, lam - 8 * sd)), int(lam + 8 * sd) + 2)
+        assert chisq_pvalue(draws, lambda k: stats.poisson.pmf(k, lam), support) > 1e-4, lam
+        assert abs(draws.mean() - lam) < 5 * sd / np.sqrt(draws.size), lam
+
+
 def test_binomial_invalid_reports(gpu_lib):
     rng = ol.prng(2, seed=1)
     out, err, _ = gpu_lib.gpu_sample("binomial", rng, 1, [10.0, 10.0], [0.5, -0.2])
