@@ -137,11 +137,19 @@ std::vector<size_t> check_index(const size_t* index, size_t n_index, size_t n_gr
     for (size_t i = 0; i < n_groups; ++i) ret.push_back(i);
     return ret;
   }
+  std::vector<char> seen(n_groups, 0);
   for (size_t i = 0; i < n_index; ++i) {
     if (index[i] >= n_groups) {
       throw std::runtime_error("Invalid index for 'index_group': values must lie in [0, " +
                                std::to_string(n_groups - 1) + "]");
     }
+    // a repeated group would be run twice, concurrently, on one state;
+    // dust_likelihood_run rejects duplicates (R/interface-likelihood.R:56,
+    // R/filter-support.R:85-86)
+    if (seen[index[i]]) {
+      throw std::runtime_error("All elements of 'index_group' must be distinct");
+    }
+    seen[index[i]] = 1;
     ret.push_back(index[i]);
   }
   return ret;
@@ -776,6 +784,18 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
   }
   // the final gather wrote buffer (T + 1) % 2 in the A/B naming
   if ((T + 1) % 2 == 1) {
+    if (initial_given) {
+      // groups outside index_group keep the caller's initial state, which
+      // is in buffer A; carry it over to B before B becomes the state
+      const size_t bytes = h->S * h->P * h->real_size();
+      for (size_t g = 0; g < G; ++g) {
+        if (selected[g]) continue;
+        hip_check(hipMemcpyAsync(h->d_state_b.get<char>() + g * bytes,
+                                 h->d_state.get<char>() + g * bytes, bytes,
+                                 hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync");
+      }
+    }
     hip_check(hipStreamSynchronize(s), "sync");
     h->d_state.swap(h->d_state_b);
   }
@@ -939,6 +959,10 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     const auto idx = check_index(index_group, n_index, h->G);
     const size_t G = h->G, P = h->P;
     hipStream_t s = h->stream;
+    // trajectories and snapshots become current only when the run completes
+    // (dust2/filter.hpp:95-99 sets the flag after the time loop)
+    h->traj_valid = false;
+    h->ancestry_valid = false;
     if (initial) {
       std::vector<char> tmp(G * h->S * P * h->real_size());
       aos_to_soa(initial, tmp.data(), G, P, h->S, h->fp64);
@@ -1016,8 +1040,6 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     h->last_ms = ms;
     h->last_fallbacks = counters[1];
-    h->traj_valid = save_trajectories_in != 0;
-    h->ancestry_valid = save_trajectories != 0;
     h->snap_slot.assign(h->n_time, -1);
     for (size_t i = 0, j = 0; i < h->n_time && j < snapshots.size(); ++i) {
       if (h->times[i] == snapshots[j]) h->snap_slot[i] = static_cast<int32_t>(j++);
@@ -1046,6 +1068,8 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
       }
       throw std::runtime_error(error_report(rec, count));
     }
+    h->traj_valid = save_trajectories_in != 0;
+    h->ancestry_valid = save_trajectories != 0;
     std::vector<double> ll(G);
     to_host(ll.data(), h->d_ll, G * sizeof(double), s);
     for (size_t i = 0; i < idx.size(); ++i) ll_out[i] = ll[idx[i]];
@@ -4216,7 +4240,7 @@ struct dust2_server {
   std::vector<srv_request> ode_queue;
   std::vector<srv_result> results;
   dbuf d_tables, d_status, d_ode_pars, d_ode_series, d_ode_lfact, d_ode_times, d_ode_off, d_ode_len,
-      d_ode_state, d_ode_ll, d_ode_errors, d_ode_counters;
+      d_ode_t0, d_ode_state, d_ode_ll, d_ode_errors, d_ode_counters;
   size_t max_ode_per_tick = 65536;
   ~dust2_server() {
     for (auto s : streams) {
@@ -4331,8 +4355,10 @@ void srv_run_ode(dust2_server* sv, hipStream_t s) {
   std::vector<double> pars(n * kSirodePars), times, obs, lf;
   std::vector<int64_t> off(n);
   std::vector<int32_t> len(n);
+  std::vector<double> t0(n);
   for (size_t i = 0; i < n; ++i) {
     const auto& rq = sv->ode_queue[i];
+    t0[i] = rq.time_start;
     std::copy(rq.pars.begin(), rq.pars.end(), pars.begin() + i * kSirodePars);
     off[i] = static_cast<int64_t>(times.size());
     len[i] = static_cast<int32_t>(rq.times.size());
@@ -4349,6 +4375,7 @@ void srv_run_ode(dust2_server* sv, hipStream_t s) {
   fit(sv->d_ode_lfact, lf.size() * 8);
   fit(sv->d_ode_off, n * 8);
   fit(sv->d_ode_len, n * 4);
+  fit(sv->d_ode_t0, n * 8);
   fit(sv->d_ode_state, n * 5 * 8);
   fit(sv->d_ode_ll, n * 8);
   fit(sv->d_ode_errors, n * sizeof(error_record));
@@ -4359,6 +4386,7 @@ void srv_run_ode(dust2_server* sv, hipStream_t s) {
   to_device(sv->d_ode_lfact, lf.data(), lf.size() * 8, s);
   to_device(sv->d_ode_off, off.data(), n * 8, s);
   to_device(sv->d_ode_len, len.data(), n * 4, s);
+  to_device(sv->d_ode_t0, t0.data(), n * 8, s);
   hip_check(hipMemsetAsync(sv->d_ode_errors.get(), 0, n * sizeof(error_record), s), "memset");
   hip_check(hipMemsetAsync(sv->d_ode_counters.get(), 0, 16, s), "memset");
   ode_launch a{};
@@ -4366,7 +4394,8 @@ void srv_run_ode(dust2_server* sv, hipStream_t s) {
   a.pars = sv->d_ode_pars.get<double>();
   a.n_particles = 1;
   a.n_total = static_cast<int>(n);
-  a.time_start = sv->ode_queue[0].time_start;
+  a.time_start = 0;
+  a.series_time_start = sv->d_ode_t0.get<double>();
   a.times = sv->d_ode_times.get<double>();
   a.obs = sv->d_ode_series.get<double>();
   a.lfact = sv->d_ode_lfact.get<double>();

@@ -211,6 +211,9 @@ struct ode_launch {
   // time; null for one series shared through obs_stride/obs_group_stride
   const int64_t* series_offset;
   const int32_t* series_len;
+  // with per-group series: the start time of group g's series, or null
+  // for time_start everywhere
+  const double* series_time_start;
   // forward sensitivity run (FP64): parameter 0 beta, 1 gamma, 2 I0, or -1
   // for none; grad[3 * particle + sens_param] receives d ll / d theta
   int sens_param;

@@ -418,6 +418,7 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
     doff = static_cast<size_t>(off);
     dstride = 1;
     n_time = args.series_len[g];
+    if (args.series_time_start) time = args.series_time_start[g];
   }
   const ode_record no_record{nullptr, nullptr, 0};
   for (int t = 0; t < n_time && code == 0; ++t) {

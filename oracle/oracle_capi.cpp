@@ -142,6 +142,42 @@ int simulate_impl(const double* pars, size_t G, size_t P, double time_start,
   }
 }
 
+// Deterministic likelihood of a discrete-time model: unfilter<T>::run
+// (dust2/unfilter.hpp:42-72) over a system whose streams are deterministic
+// (dust2/r/discrete/unfilter.hpp:38-46): per data time run_to_time,
+// compare_data and ll[k] += ll_step[k] for every particle, no resampling.
+// data: n_time x G_data x n_data; ll_out: G*P; state_out: G x P x S or NULL;
+// traj_out: n_time x G x P x S or NULL.
+template <typename T>
+int discrete_unfilter_impl(const double* pars, size_t G, size_t P, double time_start,
+                           const double* time, size_t n_time, double dt, const double* data,
+                           size_t n_groups_data, int n_threads, double* ll_out,
+                           double* state_out, double* traj_out, char* err, size_t errlen) {
+  try {
+    std::vector<uint64_t> seed(4);
+    seed_words(42, seed.data());
+    discrete_system<T> sys(read_shared<T>(pars, G), time_start, dt, P, seed, true,
+                           n_threads);
+    const auto d = read_data<T>(data, n_time * n_groups_data);
+    std::vector<size_t> all(G);
+    std::iota(all.begin(), all.end(), 0);
+    sys.set_state_initial();
+    std::vector<double> ll(G * P, 0.0), ll_step(G * P);
+    const size_t stride = G * P * T::n_state;
+    for (size_t t = 0; t < n_time; ++t) {
+      sys.run_to_time(time[t], all);
+      sys.compare_data(d.data() + t * n_groups_data, n_groups_data, all, ll_step.data());
+      for (size_t k = 0; k < ll.size(); ++k) ll[k] += ll_step[k];
+      if (traj_out) std::copy(sys.state().begin(), sys.state().end(), traj_out + t * stride);
+    }
+    std::copy(ll.begin(), ll.end(), ll_out);
+    if (state_out) std::copy(sys.state().begin(), sys.state().end(), state_out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -282,6 +318,27 @@ int oracle_seir16_simulate(const double* pars, size_t G, size_t P,
   return simulate_impl<seir16>(pars, G, P, time_start, dt, times, n_times,
                                rng_in, deterministic, n_threads, out, rng_out,
                                err, errlen);
+}
+
+// Deterministic unfilter of a discrete-time model (discrete_unfilter_impl).
+int oracle_sir_unfilter_det_run(const double* pars, size_t G, size_t P, double time_start,
+                                const double* time, size_t n_time, double dt,
+                                const double* data, size_t n_groups_data, int n_threads,
+                                double* ll_out, double* state_out, double* traj_out, char* err,
+                                size_t errlen) {
+  return discrete_unfilter_impl<sir>(pars, G, P, time_start, time, n_time, dt, data,
+                                     n_groups_data, n_threads, ll_out, state_out, traj_out, err,
+                                     errlen);
+}
+
+int oracle_seir16_unfilter_run(const double* pars, size_t G, size_t P, double time_start,
+                               const double* time, size_t n_time, double dt,
+                               const double* data, size_t n_groups_data, int n_threads,
+                               double* ll_out, double* state_out, double* traj_out, char* err,
+                               size_t errlen) {
+  return discrete_unfilter_impl<seir16>(pars, G, P, time_start, time, n_time, dt, data,
+                                        n_groups_data, n_threads, ll_out, state_out, traj_out,
+                                        err, errlen);
 }
 
 // ---- ODE unfilter ---------------------------------------------------------

@@ -77,6 +77,10 @@ def oracle():
         o.oracle_sir_simulate.argtypes = sim
         o.oracle_seir16_simulate.argtypes = sim
         o.oracle_sir_tv_simulate.argtypes = sim
+        unf = [D, C.c_size_t, C.c_size_t, C.c_double, D, C.c_size_t, C.c_double, D,
+               C.c_size_t, C.c_int, D, D, D, C.c_char_p, C.c_size_t]
+        o.oracle_sir_unfilter_det_run.argtypes = unf
+        o.oracle_seir16_unfilter_run.argtypes = unf
         o.oracle_sirode_unfilter_run.argtypes = [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                                  C.c_size_t, D, C.c_size_t, D, C.c_int, D, D,
                                                  D, C.c_char_p, C.c_size_t]
@@ -232,6 +236,30 @@ def sir_simulate(pars, n_particles, time_start, dt, times, rng, deterministic=Fa
             int(deterministic), n_threads, _dp(out), _u64(rng_out), buf, 4096)
     _check(rc, buf)
     return out, rng_out
+
+
+def discrete_unfilter(pars, n_particles, time_start, time, dt, data, model="sir",
+                      n_threads=1, trajectories=False):
+    """Deterministic unfilter of a discrete-time model: per-particle ll (G*P),
+    final state (G, P, S) and, when asked, the state at every data time
+    (n_time, G, P, S).  data: (n_time, G_data[, n_data])."""
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    P = n_particles
+    S = 80 if model == "seir16" else 5
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    n_groups_data = data.shape[1]
+    ll = np.zeros(G * P)
+    state = np.zeros((G, P, S))
+    traj = np.zeros((time.size, G, P, S)) if trajectories else None
+    buf = C.create_string_buffer(4096)
+    fn = {"sir": oracle().oracle_sir_unfilter_det_run,
+          "seir16": oracle().oracle_seir16_unfilter_run}[model]
+    rc = fn(_dp(pars), G, P, float(time_start), _dp(time), time.size, float(dt), _dp(data),
+            n_groups_data, n_threads, _dp(ll), _dp(state), _dp(traj), buf, 4096)
+    _check(rc, buf)
+    return {"ll": ll, "state": state, "traj": traj}
 
 
 def sirode_unfilter(pars, n_particles, time_start, time, data, control=None, n_threads=1):
