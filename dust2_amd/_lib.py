@@ -32,6 +32,10 @@ _PROTOS = {
                                             C.c_size_t, C.c_double, D, C.c_size_t, U64,
                                             C.c_size_t, C.c_uint64, C.c_int, C.POINTER(VP), ERR,
                                             ERRLEN]),
+    "dust2_filter_discrete_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t,
+                                              C.c_double, D, C.c_size_t, C.c_double, D,
+                                              C.c_size_t, U64, C.c_size_t, C.c_uint64, C.c_int,
+                                              C.POINTER(VP), ERR, ERRLEN]),
     "dust2_filter_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_filter_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_filter_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
@@ -54,19 +58,23 @@ _PROTOS = {
     "dust2_system_sir_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, C.c_double, U64,
                                          C.c_size_t, C.c_uint64, C.c_int, C.c_int,
                                          C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_system_sir_run_to_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
-    "dust2_system_sir_simulate": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
-    "dust2_system_sir_state": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_system_sir_set_state": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_system_sir_set_state_initial": (C.c_int, [VP, ERR, ERRLEN]),
-    "dust2_system_sir_time": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_system_sir_set_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
-    "dust2_system_sir_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_system_sir_compare_data": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
-    "dust2_system_sir_reorder": (C.c_int, [VP, U32, ERR, ERRLEN]),
-    "dust2_system_sir_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
-    "dust2_system_sir_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
-    "dust2_system_sir_free": (None, [VP]),
+    "dust2_system_discrete_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t,
+                                              C.c_double, C.c_double, U64, C.c_size_t,
+                                              C.c_uint64, C.c_int, C.c_int, C.POINTER(VP), ERR,
+                                              ERRLEN]),
+    "dust2_system_discrete_run_to_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
+    "dust2_system_discrete_simulate": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_system_discrete_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_discrete_set_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_discrete_set_state_initial": (C.c_int, [VP, ERR, ERRLEN]),
+    "dust2_system_discrete_time": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_discrete_set_time": (C.c_int, [VP, C.c_double, ERR, ERRLEN]),
+    "dust2_system_discrete_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_system_discrete_compare_data": (C.c_int, [VP, D, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_system_discrete_reorder": (C.c_int, [VP, U32, ERR, ERRLEN]),
+    "dust2_system_discrete_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
+    "dust2_system_discrete_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
+    "dust2_system_discrete_free": (None, [VP]),
     "dust2_unfilter_sirode_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                               C.c_size_t, D, C.c_size_t, D, C.c_int,
                                               C.POINTER(VP), ERR, ERRLEN]),

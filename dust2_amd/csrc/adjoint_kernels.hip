@@ -42,14 +42,14 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
   const double e_IR = pg[6];
   const size_t NT = static_cast<size_t>(a.n_total);
 
+  rng256 det_rng{0, 0, 0, 0};
   double y[5];
   if (a.set_initial) {
-    sir_model::initial(sh, y);
+    sir_model::initial(sh, det_rng, true, y);
   } else {
 #pragma unroll
     for (int c = 0; c < 5; ++c) y[c] = a.state[c * NT + k];
   }
-  rng256 det_rng{0, 0, 0, 0};
   double ll = 0;
   size_t step = 0;
   for (int i = 0; i < a.n_time; ++i) {
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
       }
       uint32_t err = 0;
       double ea = 0, eb = 0;
-      sir_model::update<double>(a.dt, y, sh, det_rng, true, err, ea, eb);
+      sir_model::update<double>(a.dt, y, sh, nullptr, 0, det_rng, true, err, ea, eb);
       if (err) {
         a.errors[k] = error_record{err, static_cast<uint32_t>(i), ea, eb};
         atomicAdd(a.error_count, 1u);

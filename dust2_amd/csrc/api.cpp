@@ -23,6 +23,7 @@
 
 #include "engine.hpp"
 #include "interpolate.hpp"
+#include "models.hpp"
 #include "ode_general.hpp"
 #include "ode_models.hpp"
 
@@ -367,6 +368,130 @@ void check_sir_pars(const double* row) {
   }
 }
 
+
+void check_finite_pars(const double* row, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(row[i])) {
+      throw std::runtime_error("Parameter " + std::to_string(i) + " must be finite");
+    }
+  }
+}
+
+// SEIR16 host parameter row: beta, sigma, gamma, exp_noise, I0, N[16],
+// C[16][16] (row-major).  Kernel block: seir16_model in models.hpp.
+constexpr size_t kSeirAge = 16;
+constexpr size_t kSeirPars = 5 + kSeirAge + kSeirAge * kSeirAge;
+constexpr size_t kSeirKpars = seir16_model::kpars;
+
+void seir_kernel_pars(const double* row, double dt, double* out) {
+  std::fill(out, out + kSeirKpars, 0.0);
+  const double beta = row[0];
+  out[0] = beta;
+  out[1] = static_cast<float>(-std::expm1(-row[1] * dt));
+  out[2] = static_cast<float>(-std::expm1(-row[2] * dt));
+  out[3] = row[3];
+  out[4] = row[4];
+  const double* N = row + 5;
+  const double* C = row + 5 + kSeirAge;
+  for (size_t a = 0; a < kSeirAge; ++a) {
+    out[5 + a] = N[a];
+    for (size_t b = 0; b < kSeirAge; ++b) {
+      out[seir16_model::kParA + a * kSeirAge + b] = beta * C[a * kSeirAge + b] / N[b];
+      out[seir16_model::kParC + a * kSeirAge + b] = C[a * kSeirAge + b];
+    }
+  }
+  // FP64: the reference-style expression 1 - exp(-rate * dt)
+  out[seir16_model::kParP64] = 1 - std::exp(-row[1] * dt);
+  out[seir16_model::kParP64 + 1] = 1 - std::exp(-row[2] * dt);
+}
+
+
+// Random walk (inst/examples/walk.cpp): host row sd, len, random_initial;
+// the kernel block is the row.
+constexpr size_t kWalkPars = 3;
+
+void check_walk_pars(const double* row) {
+  if (!std::isfinite(row[0])) throw std::runtime_error("Parameter 'sd' must be finite");
+  if (!(row[1] >= 1 && row[1] <= walk_model::max_len && row[1] == std::floor(row[1]))) {
+    throw std::runtime_error("Parameter 'len' must be an integer in [1, " +
+                             std::to_string(walk_model::max_len) + "]");
+  }
+  if (row[2] != 0 && row[2] != 1) {
+    throw std::runtime_error("Parameter 'random_initial' must be TRUE or FALSE");
+  }
+}
+
+void walk_kernel_pars(const double* row, double, double* out) {
+  std::copy(row, row + kWalkPars, out);
+}
+
+// Host side of each discrete-time model: its parameter row, validation,
+// kernel parameter block (models.hpp) and sizes.  The kernels themselves
+// are one set templated on the model.
+struct discrete_model_host {
+  int id;
+  const char* name;
+  size_t n_pars;     // host parameter row
+  size_t n_data;     // data values per time (0: the model has no compare_data)
+  size_t kpars;      // kernel parameter block (doubles)
+  double zero_period;
+  size_t (*n_state)(const double* row);
+  void (*check)(const double* row);
+  void (*kernel_pars)(const double* row, double dt, double* out);
+  // the parameters update_pars may change (others are constant,
+  // dust2/properties.hpp update_shared)
+  std::vector<size_t> updatable;
+};
+
+const discrete_model_host& discrete_model(int id) {
+  static const discrete_model_host sir{
+      MODEL_SIR, "sir", kSirPars, 1, 8, 1.0, [](const double*) -> size_t { return 5; },
+      check_sir_pars, sir_kernel_pars,
+      // I0, beta, gamma; N and exp_noise are constant (inst/examples/sir.cpp:89-93)
+      {1, 2, 3}};
+  static const discrete_model_host seir{
+      MODEL_SEIR16, "seir16", kSeirPars, kSeirAge, kSeirKpars, 7.0,
+      [](const double*) -> size_t { return 5 * kSeirAge; },
+      [](const double* row) { check_finite_pars(row, kSeirPars); }, seir_kernel_pars,
+      // beta, sigma, gamma, I0; exp_noise, N and C are constant
+      {0, 1, 2, 4}};
+  static const discrete_model_host walk{
+      MODEL_WALK, "walk", kWalkPars, 0, kWalkPars, 1.0,
+      [](const double* row) -> size_t { return static_cast<size_t>(row[1]); },
+      check_walk_pars, walk_kernel_pars,
+      // sd (walk.cpp update_shared); len and random_initial are constant
+      {0}};
+  switch (id) {
+    case MODEL_SIR: return sir;
+    case MODEL_SEIR16: return seir;
+    case MODEL_WALK: return walk;
+    default: throw std::runtime_error("unknown discrete-time model");
+  }
+}
+
+int discrete_model_by_name(const char* name) {
+  for (int id : {MODEL_SIR, MODEL_SEIR16, MODEL_WALK}) {
+    if (name && std::strcmp(name, discrete_model(id).name) == 0) return id;
+  }
+  throw std::runtime_error(std::string("unknown discrete-time model '") + (name ? name : "") +
+                           "'");
+}
+
+// Every group must have the state length of the first
+// (dust2/r/helpers.hpp:333-369).
+size_t discrete_state_size(const discrete_model_host& m, const double* pars, size_t n_groups) {
+  const size_t S = m.n_state(pars);
+  for (size_t g = 1; g < n_groups; ++g) {
+    const size_t Sg = m.n_state(pars + g * m.n_pars);
+    if (Sg != S) {
+      throw std::runtime_error("State length for group " + std::to_string(g + 1) +
+                               " was different to previous groups; total length was expected "
+                               "to be " + std::to_string(S) + " but it was " +
+                               std::to_string(Sg));
+    }
+  }
+  return S;
+}
 }  // namespace
 
 // ============================================================================
@@ -403,7 +528,7 @@ void check_handle(const H* h, uint32_t tag, const char* what) {
 
 struct dust2_filter {
   uint32_t tag = TAG_FILTER;
-  int model = 0;             // 0: sir, 1: seir16
+  int model = MODEL_SIR;     // discrete_model_id of models.hpp
   size_t S = 5;              // states per particle
   size_t n_pars = 5;         // host parameter row length
   size_t n_data = 1;         // data values per group per time
@@ -472,53 +597,11 @@ struct dust2_filter {
 
 namespace {
 
-// SEIR16 host parameter row: beta, sigma, gamma, exp_noise, I0, N[16],
-// C[16][16] (row-major).  Kernel block (doubles): [0] beta, [1] p_EI,
-// [2] p_IR (both as the FP32 value of 1 - exp(-rate*dt)), [3] exp_noise,
-// [4] I0, [5..20] N, [21 + 16a + b] beta * C[a][b] / N[b].
-constexpr size_t kSeirAge = 16;
-constexpr size_t kSeirPars = 5 + kSeirAge + kSeirAge * kSeirAge;
-constexpr size_t kSeirKpars = 5 + kSeirAge + kSeirAge * kSeirAge + 3;
-
-void seir_kernel_pars(const double* row, double dt, double* out) {
-  std::fill(out, out + kSeirKpars, 0.0);
-  const double beta = row[0];
-  out[0] = beta;
-  out[1] = static_cast<float>(-std::expm1(-row[1] * dt));
-  out[2] = static_cast<float>(-std::expm1(-row[2] * dt));
-  out[3] = row[3];
-  out[4] = row[4];
-  const double* N = row + 5;
-  const double* C = row + 5 + kSeirAge;
-  for (size_t a = 0; a < kSeirAge; ++a) {
-    out[5 + a] = N[a];
-    for (size_t b = 0; b < kSeirAge; ++b) {
-      out[5 + kSeirAge + a * kSeirAge + b] = beta * C[a * kSeirAge + b] / N[b];
-    }
-  }
-}
-
-void model_kernel_pars(const dust2_filter* h, const double* row, double* out) {
-  if (h->model == 1) {
-    seir_kernel_pars(row, h->dt, out);
-  } else {
-    sir_kernel_pars(row, h->dt, out);
-  }
-}
-
 hipError_t launch_advance(const dust2_filter* h, const pipe_slot* slots, int n_slots, int P,
                           int t, int parity, uint32_t mode, hipStream_t s) {
-  if (h->model == 1) return launch_seir_advance(slots, n_slots, P, t, parity, mode, s);
-  return launch_sir_advance(slots, n_slots, P, t, parity, mode, h->fp64, s);
+  return launch_discrete_advance(h->model, h->fp64, slots, n_slots, P, t, parity, mode, s);
 }
 
-void check_finite_pars(const double* row, size_t n) {
-  for (size_t i = 0; i < n; ++i) {
-    if (!std::isfinite(row[i])) {
-      throw std::runtime_error("Parameter " + std::to_string(i) + " must be finite");
-    }
-  }
-}
 
 void aos_to_soa(const double* src, void* dst, size_t G, size_t P, size_t S, bool fp64) {
   for (size_t g = 0; g < G; ++g) {
@@ -609,7 +692,7 @@ pipe_slot make_pipe_slot(dust2_filter* h, size_t g, bool save_traj) {
   sl.lfact = h->d_lfact.get<double>() + goff;
   sl.obs_stride = static_cast<uint32_t>(h->n_groups_data * h->n_data);
   sl.n_particles = static_cast<uint32_t>(P);
-  sl.flags = h->model == 1 ? SLOT_WEIGHT_IS_LL : 0u;
+  sl.flags = 0u;
   sl.group = static_cast<uint32_t>(g);
   sl.sched_time = h->d_sched_time.get<double>();
   sl.sched_steps = h->d_sched_steps.get<uint32_t>();
@@ -714,7 +797,8 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
   hipStream_t s = h->stream;
   const size_t G = h->G;
   std::vector<double> kp(G * h->kpars);
-  for (size_t g = 0; g < G; ++g) model_kernel_pars(h, &h->pars[g * h->n_pars], &kp[g * h->kpars]);
+  const discrete_model_host& m = discrete_model(h->model);
+  for (size_t g = 0; g < G; ++g) m.kernel_pars(&h->pars[g * h->n_pars], h->dt, &kp[g * h->kpars]);
   to_device(h->d_kpars, kp.data(), kp.size() * sizeof(double), s);
   std::vector<pipe_slot> slots;
   std::vector<char> selected(G, 0);
@@ -852,22 +936,20 @@ static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
     if (precision != DUST2_FP64 && precision != DUST2_FP32) {
       throw std::runtime_error("'precision' must be DUST2_FP64 or DUST2_FP32");
     }
+    const discrete_model_host& m = discrete_model(model);
+    if (m.n_data == 0) {
+      throw std::runtime_error(std::string("The '") + m.name +
+                               "' model does not support 'compare_data'");
+    }
+    for (size_t g = 0; g < n_groups; ++g) m.check(pars + g * m.n_pars);
     h = std::make_unique<dust2_filter>();
     h->model = model;
-    if (model == 1) {
-      if (precision != DUST2_FP32) {
-        throw std::runtime_error("the seir16 model runs in DUST2_FP32 only");
-      }
-      h->S = 5 * kSeirAge;
-      h->n_pars = kSeirPars;
-      h->n_data = kSeirAge;
-      h->kpars = kSeirKpars;
-      h->zero_period = 7;
-      h->engine = DUST2_ENGINE_PIPELINE;
-      for (size_t g = 0; g < n_groups; ++g) check_finite_pars(pars + g * h->n_pars, h->n_pars);
-    } else {
-      for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * h->n_pars);
-    }
+    h->S = discrete_state_size(m, pars, n_groups);
+    h->n_pars = m.n_pars;
+    h->n_data = m.n_data;
+    h->kpars = m.kpars;
+    h->zero_period = m.zero_period;
+    if (discrete_filter_max_particles(model) == 0) h->engine = DUST2_ENGINE_PIPELINE;
     h->fp64 = precision == DUST2_FP64;
     h->G = n_groups;
     h->P = n_particles;
@@ -883,7 +965,7 @@ static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
       throw std::runtime_error("'n_particles' too large: one group's weights must fit one "
                                "workgroup (at most 16384 particles)");
     }
-    h->cfg = sir_filter_default_config(static_cast<int>(n_particles), h->fp64);
+    h->cfg = discrete_filter_default_config(static_cast<int>(n_particles));
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
     for (int c = 0; c < dust2_filter::kMaxChains - 1; ++c) {
@@ -931,7 +1013,7 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
                            const double* data, size_t n_groups_data,
                            const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                            int precision, dust2_filter** out, char* err, size_t errlen) {
-  return filter_alloc_impl(0, pars, n_groups, n_particles, time_start, time, n_time, dt, data,
+  return filter_alloc_impl(MODEL_SIR, pars, n_groups, n_particles, time_start, time, n_time, dt, data,
                            n_groups_data, seed_state, n_seed_streams, seed, precision, out, err,
                            errlen);
 }
@@ -941,7 +1023,24 @@ int dust2_filter_seir16_alloc(const double* pars, size_t n_groups, size_t n_part
                               const double* data, size_t n_groups_data,
                               const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                               int precision, dust2_filter** out, char* err, size_t errlen) {
-  return filter_alloc_impl(1, pars, n_groups, n_particles, time_start, time, n_time, dt, data,
+  return filter_alloc_impl(MODEL_SEIR16, pars, n_groups, n_particles, time_start, time, n_time,
+                           dt, data, n_groups_data, seed_state, n_seed_streams, seed, precision,
+                           out, err, errlen);
+}
+
+int dust2_filter_discrete_alloc(const char* model, const double* pars, size_t n_groups,
+                                size_t n_particles, double time_start, const double* time,
+                                size_t n_time, double dt, const double* data,
+                                size_t n_groups_data, const uint64_t* seed_state,
+                                size_t n_seed_streams, uint64_t seed, int precision,
+                                dust2_filter** out, char* err, size_t errlen) {
+  int id = 0;
+  try {
+    id = discrete_model_by_name(model);
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+  return filter_alloc_impl(id, pars, n_groups, n_particles, time_start, time, n_time, dt, data,
                            n_groups_data, seed_state, n_seed_streams, seed, precision, out, err,
                            errlen);
 }
@@ -981,8 +1080,8 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
 
     int engine = h->engine;
     if (engine == DUST2_ENGINE_AUTO) {
-      engine = (h->model != 0 || h->beta_n > 0 ||
-                static_cast<int>(P) > sir_filter_max_particles(h->fp64))
+      engine = (h->beta_n > 0 ||
+                static_cast<int>(P) > discrete_filter_max_particles(h->model))
                    ? DUST2_ENGINE_PIPELINE
                    : DUST2_ENGINE_PERSISTENT;
     }
@@ -1001,7 +1100,7 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
       // groups outside index_group are reset (filter.hpp:248-250) but not run
       if (!selected[g] && initial) continue;
       filter_slot sl{};
-      sir_kernel_pars(&h->pars[g * h->n_pars], h->dt, sl.pars);
+      discrete_model(h->model).kernel_pars(&h->pars[g * h->n_pars], h->dt, sl.pars);
       sl.times = h->d_times.get<double>();
       const size_t goff = h->n_groups_data == 1 ? 0 : g;
       sl.obs = h->d_obs.get<double>() + goff;
@@ -1029,9 +1128,10 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     }
     to_device(h->d_slots, slots.data(), slots.size() * sizeof(filter_slot), s);
     hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-    hip_check(launch_sir_filter(h->d_slots.get<filter_slot>(), static_cast<int>(slots.size()),
-                                static_cast<int>(P), h->fp64, h->cfg, s),
-              "sir filter kernel");
+    hip_check(launch_discrete_filter(h->model, h->fp64, h->d_slots.get<filter_slot>(),
+                                     static_cast<int>(slots.size()), static_cast<int>(P),
+                                     h->cfg, s),
+              "filter kernel");
     hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
     }
     uint32_t counters[8];
@@ -1087,21 +1187,9 @@ int dust2_filter_update_pars(dust2_filter* h, const double* pars, const size_t* 
     for (size_t i = 0; i < idx.size(); ++i) {
       const double* row = pars + i * h->n_pars;
       double* dst = &h->pars[idx[i] * h->n_pars];
-      if (h->model == 1) {
-        // seir16: beta, sigma, gamma, I0 are updatable; N, C, exp_noise constant
-        check_finite_pars(row, h->n_pars);
-        dst[0] = row[0];
-        dst[1] = row[1];
-        dst[2] = row[2];
-        dst[4] = row[4];
-      } else {
-        // sir: I0, beta, gamma are updatable; N and exp_noise are constant
-        // (inst/examples/sir.cpp:89-93)
-        check_sir_pars(row);
-        dst[1] = row[1];
-        dst[2] = row[2];
-        dst[3] = row[3];
-      }
+      const discrete_model_host& m = discrete_model(h->model);
+      m.check(row);
+      for (size_t k : m.updatable) dst[k] = row[k];
     }
     return 0;
   } catch (const std::exception& e) {
@@ -1342,7 +1430,7 @@ int dust2_filter_set_launch(dust2_filter* h, int block, int ppt, char* err, size
   try {
     check_filter(h);
     const launch_config c{block, ppt};
-    if (!sir_filter_config_valid(c, h->fp64)) {
+    if (!discrete_filter_config_valid(c)) {
       throw std::runtime_error("unsupported launch shape");
     }
     if (static_cast<size_t>(block) * ppt < h->P) {
@@ -1382,7 +1470,9 @@ int dust2_filter_set_beta_schedule(dust2_filter* h, int kind, const double* t, s
       h->beta_groups = 0;
       return 0;
     }
-    if (h->model != 0) throw std::runtime_error("an interpolated beta is a SIR model input");
+    if (h->model != MODEL_SIR) {
+      throw std::runtime_error("an interpolated beta is a SIR model input");
+    }
     if (kind != INTERP_CONSTANT && kind != INTERP_LINEAR && kind != INTERP_SPLINE) {
       throw std::runtime_error("unknown interpolation kind");
     }
@@ -1441,13 +1531,13 @@ int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int prof
         engine != DUST2_ENGINE_PIPELINE) {
       throw std::runtime_error("unknown engine");
     }
-    if (engine == DUST2_ENGINE_PERSISTENT && h->model != 0) {
+    if (engine == DUST2_ENGINE_PERSISTENT && discrete_filter_max_particles(h->model) == 0) {
       throw std::runtime_error("the persistent engine runs the sir model only");
     }
     if (engine == DUST2_ENGINE_PERSISTENT &&
-        static_cast<int>(h->P) > sir_filter_max_particles(h->fp64)) {
+        static_cast<int>(h->P) > discrete_filter_max_particles(h->model)) {
       throw std::runtime_error("the persistent engine holds at most " +
-                               std::to_string(sir_filter_max_particles(h->fp64)) +
+                               std::to_string(discrete_filter_max_particles(h->model)) +
                                " particles per group");
     }
     h->engine = engine;
@@ -1474,7 +1564,7 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       int chains;
     };
     std::vector<candidate> cand;
-    if (h->model == 0 && h->beta_n == 0 && P <= sir_filter_max_particles(h->fp64)) {
+    if (h->beta_n == 0 && P <= discrete_filter_max_particles(h->model)) {
       static const int shapes[][2] = {{64, 1},   {64, 2},   {64, 4},   {128, 2}, {128, 4},
                                       {256, 2},  {256, 4},  {256, 8},  {512, 2}, {512, 4},
                                       {512, 8},  {1024, 2}, {1024, 4}, {1024, 8}};
@@ -1570,11 +1660,13 @@ void dust2_filter_free(dust2_filter* h) {
 // ============================================================================
 struct dust2_system {
   uint32_t tag = TAG_SYSTEM;
+  int model = MODEL_SIR;     // discrete_model_id of models.hpp
+  size_t S = 5;              // states per particle
   bool fp64 = true;
   bool deterministic = false;
   size_t G = 0, P = 0;
   double time = 0, dt = 1;
-  std::vector<double> pars;  // G x 5
+  std::vector<double> pars;  // G x n_pars host rows
   hipStream_t stream = nullptr;
   dbuf d_state, d_tmp, d_rng, d_pars, d_errors, d_counters, d_obs, d_lfact, d_out, d_idx;
   bool errors_pending = false;
@@ -1595,9 +1687,27 @@ void check_system(dust2_system* h, const char* action) {
 }
 
 void system_upload_pars(dust2_system* h) {
-  std::vector<double> kp(h->G * 8);
-  for (size_t g = 0; g < h->G; ++g) sir_kernel_pars(&h->pars[g * kSirPars], h->dt, &kp[g * 8]);
+  const discrete_model_host& m = discrete_model(h->model);
+  std::vector<double> kp(h->G * m.kpars);
+  for (size_t g = 0; g < h->G; ++g) m.kernel_pars(&h->pars[g * m.n_pars], h->dt, &kp[g * m.kpars]);
+  if (h->d_pars.bytes() < kp.size() * sizeof(double)) h->d_pars.alloc(kp.size() * sizeof(double));
   to_device(h->d_pars, kp.data(), kp.size() * sizeof(double), h->stream);
+}
+
+discrete_run_args system_args(dust2_system* h) {
+  discrete_run_args a{};
+  a.state = h->d_state.get();
+  a.rng = h->d_rng.get<uint64_t>();
+  a.kpars = h->d_pars.get<double>();
+  a.kstride = static_cast<int>(discrete_model(h->model).kpars);
+  a.G = static_cast<int>(h->G);
+  a.P = static_cast<int>(h->P);
+  a.time = h->time;
+  a.dt = h->dt;
+  a.flags = h->deterministic ? SLOT_DETERMINISTIC : 0u;
+  a.errors = h->d_errors.get<error_record>();
+  a.error_count = h->d_counters.get<uint32_t>();
+  return a;
 }
 
 void system_run(dust2_system* h, double time, bool set_initial) {
@@ -1605,15 +1715,10 @@ void system_run(dust2_system* h, double time, bool set_initial) {
   const size_t n_steps = static_cast<size_t>(std::round(diff / h->dt));
   hipStream_t s = h->stream;
   hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
-  const uint32_t flags = (h->deterministic ? SLOT_DETERMINISTIC : 0u) |
-                         (set_initial ? SLOT_SET_INITIAL : 0u);
-  hip_check(launch_sir_run_to_time(h->d_state.get(), h->d_rng.get<uint64_t>(),
-                                   h->d_pars.get<double>(), static_cast<int>(h->G),
-                                   static_cast<int>(h->P), h->time, h->dt,
-                                   static_cast<int>(n_steps), flags, h->fp64,
-                                   h->d_errors.get<error_record>(),
-                                   h->d_counters.get<uint32_t>(), s),
-            "run_to_time kernel");
+  discrete_run_args a = system_args(h);
+  a.n_steps = static_cast<int>(n_steps);
+  if (set_initial) a.flags |= SLOT_SET_INITIAL;
+  hip_check(launch_discrete_run(h->model, h->fp64, a, s), "run_to_time kernel");
   uint32_t count = 0;
   to_host(&count, h->d_counters, sizeof(count), s);
   if (count == 0) {
@@ -1634,31 +1739,34 @@ void system_run(dust2_system* h, double time, bool set_initial) {
 
 extern "C" {
 
-int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particles, double time,
-                           double dt, const uint64_t* seed_state, size_t n_seed_streams,
-                           uint64_t seed, int deterministic, int precision, dust2_system** out,
-                           char* err, size_t errlen) {
+int dust2_system_discrete_alloc(const char* model, const double* pars, size_t n_groups,
+                                size_t n_particles, double time, double dt,
+                                const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                                int deterministic, int precision, dust2_system** out, char* err,
+                                size_t errlen) {
   try {
+    const discrete_model_host& m = discrete_model(discrete_model_by_name(model));
     check_sizes(n_groups, n_particles);
     check_dt(dt);
     if (precision != DUST2_FP64 && precision != DUST2_FP32) {
       throw std::runtime_error("'precision' must be DUST2_FP64 or DUST2_FP32");
     }
-    for (size_t g = 0; g < n_groups; ++g) check_sir_pars(pars + g * kSirPars);
+    for (size_t g = 0; g < n_groups; ++g) m.check(pars + g * m.n_pars);
     auto h = std::make_unique<dust2_system>();
+    h->model = m.id;
+    h->S = discrete_state_size(m, pars, n_groups);
     h->fp64 = precision == DUST2_FP64;
     h->deterministic = deterministic != 0;
     h->G = n_groups;
     h->P = n_particles;
     h->time = time;
     h->dt = dt;
-    h->pars.assign(pars, pars + n_groups * kSirPars);
+    h->pars.assign(pars, pars + n_groups * m.n_pars);
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     const size_t G = n_groups, P = n_particles;
-    h->d_state.alloc(G * kSirState * P * h->real_size());
-    h->d_tmp.alloc(G * kSirState * P * h->real_size());
+    h->d_state.alloc(G * h->S * P * h->real_size());
+    h->d_tmp.alloc(G * h->S * P * h->real_size());
     h->d_rng.alloc(G * 4 * P * sizeof(uint64_t));
-    h->d_pars.alloc(G * 8 * sizeof(double));
     h->d_errors.alloc(G * P * sizeof(error_record));
     h->d_counters.alloc(4 * sizeof(uint32_t));
     hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
@@ -1681,7 +1789,16 @@ int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
   }
 }
 
-int dust2_system_sir_run_to_time(dust2_system* h, double time, char* err, size_t errlen) {
+int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particles, double time,
+                           double dt, const uint64_t* seed_state, size_t n_seed_streams,
+                           uint64_t seed, int deterministic, int precision, dust2_system** out,
+                           char* err, size_t errlen) {
+  return dust2_system_discrete_alloc("sir", pars, n_groups, n_particles, time, dt, seed_state,
+                                     n_seed_streams, seed, deterministic, precision, out, err,
+                                     errlen);
+}
+
+int dust2_system_discrete_run_to_time(dust2_system* h, double time, char* err, size_t errlen) {
   try {
     check_system(h, "run");
     if (time < h->time) {
@@ -1698,16 +1815,16 @@ int dust2_system_sir_run_to_time(dust2_system* h, double time, char* err, size_t
   }
 }
 
-int dust2_system_sir_simulate(dust2_system* h, const double* times, size_t n_times,
+int dust2_system_discrete_simulate(dust2_system* h, const double* times, size_t n_times,
                               double* out, char* err, size_t errlen) {
   try {
     check_system(h, "simulate");
-    const size_t block = h->G * h->P * kSirState;
+    const size_t block = h->G * h->P * h->S;
     for (size_t i = 0; i < n_times; ++i) {
       system_run(h, times[i], false);
       std::vector<char> tmp(h->d_state.bytes());
       to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
-      soa_to_aos(tmp.data(), out + i * block, h->G, h->P, kSirState, h->fp64);
+      soa_to_aos(tmp.data(), out + i * block, h->G, h->P, h->S, h->fp64);
     }
     return 0;
   } catch (const std::exception& e) {
@@ -1715,23 +1832,23 @@ int dust2_system_sir_simulate(dust2_system* h, const double* times, size_t n_tim
   }
 }
 
-int dust2_system_sir_state(dust2_system* h, double* out, char* err, size_t errlen) {
+int dust2_system_discrete_state(dust2_system* h, double* out, char* err, size_t errlen) {
   try {
     check_system(h, "get state from");
     std::vector<char> tmp(h->d_state.bytes());
     to_host(tmp.data(), h->d_state, tmp.size(), h->stream);
-    soa_to_aos(tmp.data(), out, h->G, h->P, kSirState, h->fp64);
+    soa_to_aos(tmp.data(), out, h->G, h->P, h->S, h->fp64);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
   }
 }
 
-int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err, size_t errlen) {
+int dust2_system_discrete_set_state(dust2_system* h, const double* state, char* err, size_t errlen) {
   try {
     check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     std::vector<char> tmp(h->d_state.bytes());
-    aos_to_soa(state, tmp.data(), h->G, h->P, kSirState, h->fp64);
+    aos_to_soa(state, tmp.data(), h->G, h->P, h->S, h->fp64);
     to_device(h->d_state, tmp.data(), tmp.size(), h->stream);
     hip_check(hipStreamSynchronize(h->stream), "sync");
     h->errors_pending = false;
@@ -1741,12 +1858,13 @@ int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err, 
   }
 }
 
-int dust2_system_sir_set_state_initial(dust2_system* h, char* err, size_t errlen) {
+int dust2_system_discrete_set_state_initial(dust2_system* h, char* err, size_t errlen) {
   try {
     check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     h->errors_pending = false;
-    // a zero-step run from the initial state writes initial(); the model's
-    // initial() draws no random numbers (sir.cpp:38-48)
+    // a zero-step run from the initial state writes initial(), drawing
+    // from the particles' streams where the model's initial() does
+    // (walk.cpp random_initial)
     system_run(h, h->time, true);
     return 0;
   } catch (const std::exception& e) {
@@ -1754,7 +1872,7 @@ int dust2_system_sir_set_state_initial(dust2_system* h, char* err, size_t errlen
   }
 }
 
-int dust2_system_sir_time(dust2_system* h, double* out, char* err, size_t errlen) {
+int dust2_system_discrete_time(dust2_system* h, double* out, char* err, size_t errlen) {
   try {
     check_system(h, "get time from");
     *out = h->time;
@@ -1764,7 +1882,7 @@ int dust2_system_sir_time(dust2_system* h, double* out, char* err, size_t errlen
   }
 }
 
-int dust2_system_sir_set_time(dust2_system* h, double time, char* err, size_t errlen) {
+int dust2_system_discrete_set_time(dust2_system* h, double time, char* err, size_t errlen) {
   try {
     check_system(h, "set time for");
     h->time = time;
@@ -1774,15 +1892,18 @@ int dust2_system_sir_set_time(dust2_system* h, double time, char* err, size_t er
   }
 }
 
-int dust2_system_sir_update_pars(dust2_system* h, const double* pars, char* err, size_t errlen) {
+int dust2_system_discrete_update_pars(dust2_system* h, const double* pars, char* err, size_t errlen) {
   try {
     check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
+    const discrete_model_host& m = discrete_model(h->model);
+    for (size_t g = 0; g < h->G; ++g) m.check(pars + g * m.n_pars);
+    (void)discrete_state_size(m, pars, h->G);
+    if (m.n_state(pars) != h->S) {
+      throw std::runtime_error("'pars' would change the state length of the system");
+    }
     for (size_t g = 0; g < h->G; ++g) {
-      const double* row = pars + g * kSirPars;
-      check_sir_pars(row);
-      h->pars[g * kSirPars + 1] = row[1];
-      h->pars[g * kSirPars + 2] = row[2];
-      h->pars[g * kSirPars + 3] = row[3];
+      const double* row = pars + g * m.n_pars;
+      for (size_t k : m.updatable) h->pars[g * m.n_pars + k] = row[k];
     }
     system_upload_pars(h);
     hip_check(hipStreamSynchronize(h->stream), "sync");
@@ -1792,30 +1913,34 @@ int dust2_system_sir_update_pars(dust2_system* h, const double* pars, char* err,
   }
 }
 
-int dust2_system_sir_compare_data(dust2_system* h, const double* data, size_t n_groups_data,
+int dust2_system_discrete_compare_data(dust2_system* h, const double* data, size_t n_groups_data,
                                   double* out, char* err, size_t errlen) {
   try {
     check_system(h, "compare data for");
+    const size_t nd = discrete_model(h->model).n_data;
+    if (nd == 0) {
+      throw std::runtime_error(std::string("The '") + discrete_model(h->model).name +
+                               "' model does not support 'compare_data'");
+    }
     if (n_groups_data != 1 && n_groups_data != h->G) {
       throw std::runtime_error("Expected 'data' to have length " + std::to_string(h->G) +
                                ", but it had length " + std::to_string(n_groups_data));
     }
-    std::vector<double> lf(n_groups_data);
-    for (size_t i = 0; i < n_groups_data; ++i) {
+    const size_t n = n_groups_data * nd;
+    std::vector<double> lf(n);
+    for (size_t i = 0; i < n; ++i) {
       lf[i] = std::isnan(data[i]) ? 0.0 : std::lgamma(data[i] + 1);
     }
     hipStream_t s = h->stream;
-    h->d_obs.alloc(n_groups_data * sizeof(double));
-    h->d_lfact.alloc(n_groups_data * sizeof(double));
+    h->d_obs.alloc(n * sizeof(double));
+    h->d_lfact.alloc(n * sizeof(double));
     h->d_out.alloc(h->G * h->P * sizeof(double));
-    to_device(h->d_obs, data, n_groups_data * sizeof(double), s);
-    to_device(h->d_lfact, lf.data(), n_groups_data * sizeof(double), s);
-    hip_check(launch_sir_compare(h->d_state.get(), h->d_rng.get<uint64_t>(),
-                                 h->d_pars.get<double>(), h->d_obs.get<double>(),
-                                 h->d_lfact.get<double>(), n_groups_data == 1 ? 0 : 1,
-                                 static_cast<int>(h->G), static_cast<int>(h->P),
-                                 h->deterministic ? SLOT_DETERMINISTIC : 0u, h->fp64,
-                                 h->d_out.get<double>(), s),
+    to_device(h->d_obs, data, n * sizeof(double), s);
+    to_device(h->d_lfact, lf.data(), n * sizeof(double), s);
+    hip_check(launch_discrete_compare(h->model, h->fp64, system_args(h), h->d_obs.get<double>(),
+                                      h->d_lfact.get<double>(),
+                                      n_groups_data == 1 ? 0 : static_cast<int>(nd),
+                                      h->d_out.get<double>(), s),
               "compare kernel");
     to_host(out, h->d_out, h->G * h->P * sizeof(double), s);
     return 0;
@@ -1824,7 +1949,7 @@ int dust2_system_sir_compare_data(dust2_system* h, const double* data, size_t n_
   }
 }
 
-int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err, size_t errlen) {
+int dust2_system_discrete_reorder(dust2_system* h, const uint32_t* index, char* err, size_t errlen) {
   try {
     check_system(h, "reorder");
     for (size_t i = 0; i < h->G * h->P; ++i) {
@@ -1834,7 +1959,7 @@ int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err, 
     h->d_idx.alloc(h->G * h->P * sizeof(uint32_t));
     to_device(h->d_idx, index, h->G * h->P * sizeof(uint32_t), s);
     hip_check(launch_reorder(h->d_state.get(), h->d_tmp.get(), h->d_idx.get<uint32_t>(),
-                             static_cast<int>(h->G), kSirState, static_cast<int>(h->P), h->fp64,
+                             static_cast<int>(h->G), h->S, static_cast<int>(h->P), h->fp64,
                              s),
               "reorder kernel");
     hip_check(hipStreamSynchronize(s), "sync");
@@ -1845,7 +1970,7 @@ int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err, 
   }
 }
 
-int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen) {
+int dust2_system_discrete_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen) {
   try {
     check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
     const size_t n = h->G * h->P;
@@ -1861,7 +1986,7 @@ int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t
   }
 }
 
-int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
+int dust2_system_discrete_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
                                    size_t errlen) {
   try {
     check_handle(h, TAG_SYSTEM, "system");  // allowed while errors are pending
@@ -1879,7 +2004,7 @@ int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char*
   }
 }
 
-void dust2_system_sir_free(dust2_system* h) {
+void dust2_system_discrete_free(dust2_system* h) {
   if (handle_ok(h, TAG_SYSTEM)) delete h;
 }
 
@@ -4251,13 +4376,16 @@ struct dust2_server {
 
 namespace {
 
-size_t model_state(int model) { return model == 1 ? 5 * kSeirAge : 5; }
+// Server model numbers: 0 sir, 1 seir16 (the discrete-time model ids of
+// models.hpp), 2 sirode.
+constexpr int kServerOde = 2;
+size_t model_state(int model) { return discrete_model(model).n_state(nullptr); }
 size_t model_n_pars(int model) {
-  return model == 1 ? kSeirPars : (model == 2 ? kSirodePars : kSirPars);
+  return model == kServerOde ? kSirodePars : discrete_model(model).n_pars;
 }
-size_t model_n_data(int model) { return model == 1 ? kSeirAge : 1; }
-size_t model_kpars(int model) { return model == 1 ? kSeirKpars : 8; }
-double model_zero_period(int model) { return model == 1 ? 7.0 : 1.0; }
+size_t model_n_data(int model) { return model == kServerOde ? 1 : discrete_model(model).n_data; }
+size_t model_kpars(int model) { return discrete_model(model).kpars; }
+double model_zero_period(int model) { return discrete_model(model).zero_period; }
 
 // Place a request in the arena and initialise its resident buffers.
 bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
@@ -4293,11 +4421,7 @@ bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
     lf[i] = std::isnan(rq.data[i]) ? 0.0 : std::lgamma(rq.data[i] + 1);
   }
   std::vector<double> kpars(kp, 0.0);
-  if (rq.model == 1) {
-    seir_kernel_pars(rq.pars.data(), rq.dt, kpars.data());
-  } else {
-    sir_kernel_pars(rq.pars.data(), rq.dt, kpars.data());
-  }
+  discrete_model(rq.model).kernel_pars(rq.pars.data(), rq.dt, kpars.data());
   auto up = [&](size_t at, const void* src, size_t bytes) {
     if (bytes) hip_check(hipMemcpyAsync(b + at, src, bytes, hipMemcpyHostToDevice, s), "upload");
   };
@@ -4323,7 +4447,7 @@ bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
   ps.lfact = reinterpret_cast<const double*>(b + L.lfact);
   ps.obs_stride = static_cast<uint32_t>(nd);
   ps.n_particles = static_cast<uint32_t>(P);
-  ps.flags = SLOT_AUTO_MODE | (rq.model == 1 ? SLOT_WEIGHT_IS_LL : 0u);
+  ps.flags = SLOT_AUTO_MODE;
   ps.group = 0;
   ps.sched_time = reinterpret_cast<const double*>(b + L.sched_time);
   ps.sched_steps = reinterpret_cast<const uint32_t*>(b + L.sched_steps);
@@ -4455,7 +4579,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
     srv_request rq;
     rq.id = sv->next_id++;
     rq.model = model;
-    rq.P = model == 2 ? 1 : n_particles;
+    rq.P = model == kServerOde ? 1 : n_particles;
     rq.time_start = time_start;
     rq.dt = dt;
     rq.seed = seed;
@@ -4463,7 +4587,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
     rq.pars.assign(pars, pars + np);
     rq.times.assign(time, time + n_time);
     rq.data.assign(data, data + n_time * model_n_data(model));
-    if (model == 2) {
+    if (model == kServerOde) {
       check_sirode_pars(pars);
       sv->ode_queue.push_back(std::move(rq));
     } else {
@@ -4472,11 +4596,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
       if (pipe_resample_ppt(static_cast<int>(n_particles)) == 0) {
         throw std::runtime_error("'n_particles' too large");
       }
-      if (model == 0) {
-        check_sir_pars(pars);
-      } else {
-        check_finite_pars(pars, np);
-      }
+      discrete_model(model).check(pars);
       sv->queue.push_back(std::move(rq));
     }
     *id_out = sv->next_id - 1;
@@ -4534,11 +4654,8 @@ int dust2_server_step(dust2_server* sv, size_t* n_active, size_t* n_queued, char
         const int n = static_cast<int>(std::get<3>(sp));
         hipStream_t s = sv->streams[(si++) % dust2_server::kStreams];
         for (int k = 0; k < sv->chunk; ++k) {
-          if (model == 1) {
-            hip_check(launch_seir_advance(tab, n, P, k, 0, 0, s), "seir advance");
-          } else {
-            hip_check(launch_sir_advance(tab, n, P, k, 0, 0, false, s), "sir advance");
-          }
+          hip_check(launch_discrete_advance(model, false, tab, n, P, k, 0, 0, s),
+                    "advance kernel");
           hip_check(launch_pipe_resample(tab, n, P, k, s), "resample");
         }
       }

@@ -31,7 +31,6 @@ enum slot_flags : uint32_t {
   SLOT_SET_INITIAL = 1u,    // start from the model's initial state
   SLOT_DETERMINISTIC = 2u,  // samplers return their means
   SLOT_SAVE_TRAJ = 4u,      // write the post-reorder state at every time
-  SLOT_WEIGHT_IS_LL = 8u,   // pipeline: lambda[] already holds the log-weight
   SLOT_AUTO_MODE = 16u,     // pipeline: the slot's own interval is t + t_offset;
                             // mode and buffer parity follow from it, and
                             // intervals past n_time do nothing (serving)
@@ -75,15 +74,18 @@ struct launch_config {
   int ppt;
 };
 
-// Persistent filter: every slot advances through intervals
-// [t_begin, t_end) with particle state held in registers.
-hipError_t launch_sir_filter(const filter_slot* d_slots, int n_slots, int max_particles,
-                             bool fp64, launch_config cfg, hipStream_t stream);
+// Persistent filter of a discrete-time model (models.hpp ids): every slot
+// advances through intervals [t_begin, t_end) with particle state held in
+// registers.  Models without a persistent kernel return
+// hipErrorInvalidValue.
+hipError_t launch_discrete_filter(int model, bool fp64, const filter_slot* d_slots, int n_slots,
+                                  int max_particles, launch_config cfg, hipStream_t stream);
 
-// Largest particle count one workgroup can hold for this precision.
-int sir_filter_max_particles(bool fp64);
-launch_config sir_filter_default_config(int n_particles, bool fp64);
-bool sir_filter_config_valid(launch_config cfg, bool fp64);
+// Largest particle count one persistent workgroup holds (0: the model has
+// no persistent kernel).
+int discrete_filter_max_particles(int model);
+launch_config discrete_filter_default_config(int n_particles);
+bool discrete_filter_config_valid(launch_config cfg);
 
 // ---- interval pipeline (state in HBM between intervals) -------------------
 enum pipe_mode : uint32_t {
@@ -110,7 +112,7 @@ struct pipe_slot {
   void* state_a;               // real [S][P], double buffered
   void* state_b;
   uint64_t* rng;               // [4][P + 2]
-  double* lambda;              // [P] Poisson means of the last interval
+  double* lambda;              // [P] FP64 log-weights of the last interval
   uint32_t* ancestors;         // [P] ancestors chosen by the last resample
   double* ll;
   uint32_t* ancestry;          // [n_time][P] or null
@@ -147,13 +149,13 @@ hipError_t launch_pipe_reset(const pipe_slot* d_slots, int n_slots, hipStream_t 
 // out[2i] = slot i's log-likelihood, out[2i+1] = its error count
 hipError_t launch_slot_status(const pipe_slot* d_slots, int n_slots, double* out,
                               hipStream_t stream);
-hipError_t launch_sir_advance(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
-                              int parity, uint32_t mode, bool fp64, hipStream_t stream);
+// One interval of every slot's particles (gather, steps, compare) for a
+// discrete-time model (models.hpp ids).
+hipError_t launch_discrete_advance(int model, bool fp64, const pipe_slot* d_slots, int n_slots,
+                                   int max_particles, int t, int parity, uint32_t mode,
+                                   hipStream_t stream);
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
                                 hipStream_t stream);
-// Age-structured SEIR (80 states, FP32), force of infection on MFMA.
-hipError_t launch_seir_advance(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
-                               int parity, uint32_t mode, hipStream_t stream);
 
 // ---- ODE unfilter ----------------------------------------------------------
 // dust2/continuous/control.hpp:11-24 (step_size_min already clamped to
@@ -292,15 +294,27 @@ hipError_t launch_interpolate(int kind, const double* t, const double* y, const 
                               hipStream_t stream);
 
 // ---- system (state resident in HBM between calls) -------------------------
-// state: real [G][S][P]; rng: [G][4][P]; pars: [G][8] doubles.
-hipError_t launch_sir_run_to_time(void* state, uint64_t* rng, const double* pars, int G,
-                                  int P, double time, double dt, int n_steps,
-                                  uint32_t flags, bool fp64, error_record* errors,
-                                  uint32_t* error_count, hipStream_t stream);
-hipError_t launch_sir_compare(const void* state, uint64_t* rng, const double* pars,
-                              const double* obs, const double* lfact, int obs_group_stride,
-                              int G, int P, uint32_t flags, bool fp64, double* out,
-                              hipStream_t stream);
+// state: real [G][S][P]; rng: [G][4][P]; kpars: [G][kstride] doubles (the
+// model's kernel parameter block).
+struct discrete_run_args {
+  void* state;
+  uint64_t* rng;
+  const double* kpars;
+  int kstride;
+  int G, P;
+  double time, dt;
+  int n_steps;
+  uint32_t flags;            // SLOT_SET_INITIAL, SLOT_DETERMINISTIC
+  error_record* errors;      // [G][P]
+  uint32_t* error_count;
+};
+hipError_t launch_discrete_run(int model, bool fp64, const discrete_run_args& a,
+                               hipStream_t stream);
+// out[g * P + p] = compare_data of each particle; obs/lfact offset by
+// g * obs_group_stride
+hipError_t launch_discrete_compare(int model, bool fp64, const discrete_run_args& a,
+                                   const double* obs, const double* lfact, int obs_group_stride,
+                                   double* out, hipStream_t stream);
 hipError_t launch_reorder(const void* state_in, void* state_out, const uint32_t* idx, int G,
                           int S, int P, bool fp64, hipStream_t stream);
 

@@ -1,6 +1,6 @@
 // System-level kernels with particle state resident in HBM between calls:
-// run_to_time, compare_data and reorder for dust_system_* style use
-// (dust2/discrete/system.hpp:61-92, 169-181, 252-287), plus the RNG
+// reorder for dust_system_* style use (dust2/discrete/system.hpp:169-181;
+// run_to_time and compare_data are in discrete_kernels.hip), plus the RNG
 // stream seeding and layout kernels and small kernels that expose the
 // samplers and the resampler to the tests.
 //
@@ -13,91 +13,10 @@
 #include "block_resample.hpp"
 #include "device_random.hpp"
 #include "engine.hpp"
-#include "models.hpp"
 
 namespace dust2_amd {
 
 namespace {
-
-template <typename real>
-__device__ __forceinline__ sir_model::shared<real> shared_from(const double* pars) {
-  sir_model::shared<real> sh;
-  sh.N = static_cast<real>(pars[0]);
-  sh.I0 = static_cast<real>(pars[1]);
-  sh.beta = static_cast<real>(pars[2]);
-  sh.gamma = static_cast<real>(pars[3]);
-  sh.exp_noise = pars[4];
-  sh.p_IR = static_cast<real>(sizeof(real) == 8 ? pars[5] : pars[6]);
-  return sh;
-}
-
-template <typename real>
-__global__ void __launch_bounds__(256)
-    sir_run_kernel(real* __restrict__ state, uint64_t* __restrict__ rng,
-                   const double* __restrict__ pars, int G, int P, double time, double dt,
-                   int n_steps, uint32_t flags, error_record* errors, uint32_t* error_count) {
-  const int g = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P || g >= G) return;
-  const sir_model::shared<real> sh = shared_from<real>(pars + 8 * g);
-  real* st = state + static_cast<size_t>(g) * 5 * P;
-  uint64_t* rg = rng + static_cast<size_t>(g) * 4 * P;
-  real y[5];
-  if (flags & SLOT_SET_INITIAL) {
-    sir_model::initial(sh, y);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 5; ++c) y[c] = st[c * P + p];
-  }
-  rng256 r{rg[p], rg[P + p], rg[2 * P + p], rg[3 * P + p]};
-  const bool det = (flags & SLOT_DETERMINISTIC) != 0;
-  const real dt_r = static_cast<real>(dt);
-  for (int s = 0; s < n_steps; ++s) {
-    const double time_s = time + s * dt;
-    if (fmod(time_s, sir_model::zero_period) == 0) {
-      y[sir_model::zero_first] = 0;
-    }
-    uint32_t err = 0;
-    real ea = 0, eb = 0;
-    sir_model::update<real>(dt_r, y, sh, r, det, err, ea, eb);
-    if (err) {
-      errors[static_cast<size_t>(g) * P + p] =
-          error_record{err, 0u, static_cast<double>(ea), static_cast<double>(eb)};
-      atomicAdd(error_count, 1u);
-      break;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 5; ++c) st[c * P + p] = y[c];
-  rg[p] = r.s0;
-  rg[P + p] = r.s1;
-  rg[2 * P + p] = r.s2;
-  rg[3 * P + p] = r.s3;
-}
-
-template <typename real>
-__global__ void __launch_bounds__(256)
-    sir_compare_kernel(const real* __restrict__ state, uint64_t* __restrict__ rng,
-                       const double* __restrict__ pars, const double* obs, const double* lfact,
-                       int obs_group_stride, int G, int P, uint32_t flags, double* out) {
-  const int g = blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P || g >= G) return;
-  const sir_model::shared<real> sh = shared_from<real>(pars + 8 * g);
-  const real* st = state + static_cast<size_t>(g) * 5 * P;
-  uint64_t* rg = rng + static_cast<size_t>(g) * 4 * P;
-  real y[5];
-#pragma unroll
-  for (int c = 0; c < 5; ++c) y[c] = st[c * P + p];
-  rng256 r{rg[p], rg[P + p], rg[2 * P + p], rg[3 * P + p]};
-  const int d = g * obs_group_stride;
-  out[static_cast<size_t>(g) * P + p] = sir_model::compare<real>(
-      y, obs + d, lfact + d, sh, r, (flags & SLOT_DETERMINISTIC) != 0);
-  rg[p] = r.s0;
-  rg[P + p] = r.s1;
-  rg[2 * P + p] = r.s2;
-  rg[3 * P + p] = r.s3;
-}
 
 template <typename real>
 __global__ void __launch_bounds__(256)
@@ -224,40 +143,6 @@ dim3 grid_gp(int G, int P, int block) {
 }
 
 }  // namespace
-
-hipError_t launch_sir_run_to_time(void* state, uint64_t* rng, const double* pars, int G,
-                                  int P, double time, double dt, int n_steps,
-                                  uint32_t flags, bool fp64, error_record* errors,
-                                  uint32_t* error_count, hipStream_t stream) {
-  if (G == 0 || P == 0) return hipSuccess;
-  if (fp64) {
-    hipLaunchKernelGGL(sir_run_kernel<double>, grid_gp(G, P, 256), dim3(256), 0, stream,
-                       static_cast<double*>(state), rng, pars, G, P, time, dt, n_steps, flags,
-                       errors, error_count);
-  } else {
-    hipLaunchKernelGGL(sir_run_kernel<float>, grid_gp(G, P, 256), dim3(256), 0, stream,
-                       static_cast<float*>(state), rng, pars, G, P, time, dt, n_steps, flags,
-                       errors, error_count);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_sir_compare(const void* state, uint64_t* rng, const double* pars,
-                              const double* obs, const double* lfact, int obs_group_stride,
-                              int G, int P, uint32_t flags, bool fp64, double* out,
-                              hipStream_t stream) {
-  if (G == 0 || P == 0) return hipSuccess;
-  if (fp64) {
-    hipLaunchKernelGGL(sir_compare_kernel<double>, grid_gp(G, P, 256), dim3(256), 0, stream,
-                       static_cast<const double*>(state), rng, pars, obs, lfact,
-                       obs_group_stride, G, P, flags, out);
-  } else {
-    hipLaunchKernelGGL(sir_compare_kernel<float>, grid_gp(G, P, 256), dim3(256), 0, stream,
-                       static_cast<const float*>(state), rng, pars, obs, lfact,
-                       obs_group_stride, G, P, flags, out);
-  }
-  return hipGetLastError();
-}
 
 hipError_t launch_reorder(const void* state_in, void* state_out, const uint32_t* idx, int G,
                           int S, int P, bool fp64, hipStream_t stream) {

@@ -25,7 +25,9 @@ class Generator:
 
     def __init__(self, name, time_type, parameters, defaults, constant, state, n_data=1,
                  sizes=None, precisions=("fp64", "fp32"), interpolate_beta=None,
-                 normalise=None, events=None):
+                 normalise=None, events=None, state_of=None):
+        # state_of: the state names of a parameter row, for models whose
+        # state length is a parameter (walk's len); None: `state` always
         # interpolate_beta: None, or the interpolation kind of a beta given
         # as knots 'beta_t' and values 'beta_y' (array parameters)
         self.interpolate_beta = interpolate_beta
@@ -43,6 +45,11 @@ class Generator:
         self.n_data = n_data
         self.sizes = sizes or {}
         self.precisions = precisions
+        self.state_of = state_of
+
+    def n_state_of(self, row):
+        """States of one parameter row."""
+        return self.n_state if self.state_of is None else len(self.state_of(row))
 
     def size(self, name):
         return self.sizes.get(name, 1)
@@ -130,6 +137,18 @@ def _set_beta(obj, arrays, groups):
          dptr(obj.beta_t), obj.beta_t.size, dptr(y), y.shape[0])
 
 
+def walk():
+    """Random walk (inst/examples/walk.cpp): 'len' walkers per particle
+    (1..8, constant), each x <- Normal(x, sd * dt) per step; the initial
+    state is Normal(0, sd) per walker when 'random_initial', else 0.  A
+    system only: the model has no compare_data."""
+    return Generator(
+        "walk", "discrete", ["sd", "len", "random_initial"],
+        {"len": 1.0, "random_initial": 0.0},
+        {"len", "random_initial"}, ["x"], n_data=0,
+        state_of=lambda row: ["x"] * int(row[1]))
+
+
 def sirode():
     """Deterministic SIR ODE (inst/examples/sirode.cpp:9-102)."""
     return Generator(
@@ -143,7 +162,8 @@ def seir16():
     reference; written to the same model contract).  N: population of each
     age group (16); C: 16 x 16 contact matrix; I0 infectious seeded in every
     age group; weekly incidence reset, Poisson observation of each age
-    group's incidence.  FP32 only."""
+    group's incidence.  FP32 (the force of infection on the matrix cores)
+    by default, FP64 in the CPU's arithmetic order when asked."""
     ages = range(16)
     state = ([f"S{a}" for a in ages] + [f"E{a}" for a in ages] + [f"I{a}" for a in ages]
              + [f"R{a}" for a in ages] + [f"inc{a}" for a in ages])
@@ -152,7 +172,7 @@ def seir16():
         {"beta": 0.2, "sigma": 0.25, "gamma": 0.1, "exp_noise": 1e6, "I0": 10.0,
          "N": np.full(16, 1e5), "C": np.full((16, 16), 1.0 / 16)},
         {"N", "C", "exp_noise"}, state, n_data=16, sizes={"N": 16, "C": 256},
-        precisions=("fp32",))
+        precisions=("fp32", "fp64"))
 
 
 # ---- continuous-time models with events, carried variables and delays ----
@@ -282,6 +302,7 @@ def malaria():
                      normalise=_malaria_normalise, events=lambda row: [])
 
 
+DISCRETE_MODELS = ("sir", "seir16", "walk")
 GENERAL_MODELS = ("bounce", "change", "eventzero", "specialevent", "sirdelay", "logdelay",
                   "special", "malaria")
 
@@ -311,7 +332,11 @@ def pars_matrix(generator, pars, n_groups):
             raise ValueError(f"Unknown parameters: {', '.join(sorted(unknown))}")
         row = []
         for k in generator.parameters:
-            v = np.asarray(p.get(k, generator.defaults[k]), dtype=np.float64).reshape(-1)
+            if k not in p and k not in generator.defaults:
+                # a required parameter (walk.cpp's sd, read_real without a default)
+                raise ValueError(f"A value is expected for '{k}'")
+            v = np.asarray(p[k] if k in p else generator.defaults[k],
+                           dtype=np.float64).reshape(-1)
             if v.size != generator.size(k):
                 raise ValueError(f"Expected '{k}' to have {generator.size(k)} values")
             row.extend(v.tolist())
@@ -479,7 +504,7 @@ class Filter:
         if seed is None:
             seed = int(np.random.default_rng().integers(0, 2**31 - 1))
         self.seed = seed
-        if precision == "fp64" and "fp64" not in generator.precisions:
+        if precision is None or (precision == "fp64" and "fp64" not in generator.precisions):
             precision = generator.precisions[0]
         self.precision = _precision(precision)
         self.ptr = None
@@ -505,8 +530,8 @@ class Filter:
         self.rows = rows
         raw, n_raw, seed = _seed_args(self.seed)
         out = C.c_void_p()
-        call(f"dust2_filter_{self.generator.name}_alloc", dptr(rows), self.n_groups,
-             self.n_particles,
+        call("dust2_filter_discrete_alloc", self.generator.name.encode(), dptr(rows),
+             self.n_groups, self.n_particles,
              self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),
              self.n_groups_data, u64ptr(raw), n_raw, seed, self.precision, C.byref(out))
         self.ptr = out
@@ -718,7 +743,7 @@ class ModelUnfilter(ModelFilter):
 
 
 def dust_filter_create(generator, time_start, data, n_particles, n_groups=1, dt=1,
-                       seed=None, precision="fp64", ode_control=None, shared_data=None):
+                       seed=None, precision=None, ode_control=None, shared_data=None):
     """R/interface-filter.R:37-97.  shared_data: True for one group of data
     used by every parameter group, False for one group of data per
     parameter group; None (the default here) accepts either."""
@@ -941,30 +966,38 @@ def dust_likelihood_last_stats(obj):
 # System
 # ---------------------------------------------------------------------------
 class System:
+    """A discrete-time system (dust_system_create, R/interface.R:60-150) of
+    any discrete-time model compiled into the library: one set of kernels
+    templated on the model, the model named at allocation."""
+    prefix = "dust2_system_discrete"
+
     def __init__(self, generator, pars, n_particles, n_groups, time, dt, seed, deterministic,
                  precision):
-        if generator.name != "sir":
+        if generator.name not in DISCRETE_MODELS or generator.interpolate_beta is not None:
             raise ValueError(f"No system kernels for '{generator.name}'")
         self.generator = generator
         self.n_particles = int(n_particles)
         self.n_groups = int(n_groups)
         rows = pars_matrix(generator, pars, self.n_groups)
         self.rows = rows
+        self.n_state = generator.n_state_of(rows[0])
+        if precision is None:
+            precision = generator.precisions[0]
         raw, n_raw, s = _seed_args(seed)
         out = C.c_void_p()
-        call("dust2_system_sir_alloc", dptr(rows), self.n_groups, self.n_particles,
-             float(time), float(dt), u64ptr(raw), n_raw, s, int(bool(deterministic)),
-             _precision(precision), C.byref(out))
+        call("dust2_system_discrete_alloc", generator.name.encode(), dptr(rows), self.n_groups,
+             self.n_particles, float(time), float(dt), u64ptr(raw), n_raw, s,
+             int(bool(deterministic)), _precision(precision), C.byref(out))
         self.ptr = out
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and _lib._lib is not None:
-            _lib.lib().dust2_system_sir_free(self.ptr)
+            _lib.lib().dust2_system_discrete_free(self.ptr)
             self.ptr = None
 
     @property
     def shape(self):
-        return (self.n_groups, self.n_particles, self.generator.n_state)
+        return (self.n_groups, self.n_particles, self.n_state)
 
 
 def _log_cap(on, n_rows):
@@ -1061,7 +1094,7 @@ class GeneralSystem:
 
 
 def dust_system_create(generator, pars=None, n_particles=1, n_groups=1, time=0, dt=None,
-                       seed=None, deterministic=False, precision="fp64", ode_control=None,
+                       seed=None, deterministic=False, precision=None, ode_control=None,
                        **kwargs):
     if pars is None:
         pars = {}
@@ -1084,7 +1117,7 @@ def dust_system_create(generator, pars=None, n_particles=1, n_groups=1, time=0, 
 
 
 def _prefix(sys):
-    return getattr(sys, "prefix", "dust2_system_sir")
+    return sys.prefix
 
 
 def dust_system_run_to_time(sys, time):
@@ -1148,13 +1181,14 @@ def dust_system_compare_data(sys, data):
         out = np.zeros((sys.n_groups, sys.n_particles))
         call("dust2_system_ode_compare_data", sys.ptr, dptr(d), d.shape[0], dptr(out))
         return out
+    nd = max(getattr(sys.generator, "n_data", 1), 1)
     d = f64(data).reshape(-1)
-    if d.size not in (1, sys.n_groups):
+    if d.size % nd != 0 or d.size // nd not in (1, sys.n_groups):
         # R/interface.R (dust_system_compare_data)
         raise ValueError(f"Expected 'data' to have length {sys.n_groups}, but it had length "
-                         f"{d.size}")
+                         f"{d.size // nd}")
     out = np.zeros((sys.n_groups, sys.n_particles))
-    call(_prefix(sys) + "_compare_data", sys.ptr, dptr(d), d.size, dptr(out))
+    call(_prefix(sys) + "_compare_data", sys.ptr, dptr(d), d.size // nd, dptr(out))
     return out
 
 
@@ -1225,7 +1259,7 @@ def dust_system_internals(sys, include_history=False):
 def dust_system_rng_state(sys):
     out = np.zeros(sys.n_groups * sys.n_particles * 4, dtype=np.uint64)
     name = "dust2_system_ode_rng_state" if isinstance(sys, GeneralSystem) else \
-        "dust2_system_sir_rng_state"
+        "dust2_system_discrete_rng_state"
     call(name, sys.ptr, u64ptr(out))
     return out
 
@@ -1233,7 +1267,7 @@ def dust_system_rng_state(sys):
 def dust_system_set_rng_state(sys, state):
     st = np.ascontiguousarray(state, dtype=np.uint64).reshape(-1)
     name = "dust2_system_ode_set_rng_state" if isinstance(sys, GeneralSystem) else \
-        "dust2_system_sir_set_rng_state"
+        "dust2_system_discrete_set_rng_state"
     call(name, sys.ptr, u64ptr(st))
 
 

@@ -81,13 +81,25 @@ int dust2_filter_sir_alloc(const double* pars, size_t n_groups, size_t n_particl
  * same generated-class contract).  Parameter row: beta, sigma, gamma,
  * exp_noise, I0, N[16], C[16*16] (row-major contact matrix); data: 16
  * weekly incidence values per group per time; state: S[16] E[16] I[16]
- * R[16] inc[16].  DUST2_FP32 only; runs on the pipeline engine with the force
- * of infection on the matrix cores.  All dust2_filter_* methods apply. */
+ * R[16] inc[16].  Runs on the pipeline engine: DUST2_FP32 with the force of
+ * infection on the matrix cores, DUST2_FP64 in the CPU's arithmetic order.
+ * All dust2_filter_* methods apply. */
 int dust2_filter_seir16_alloc(const double* pars, size_t n_groups, size_t n_particles,
                               double time_start, const double* time, size_t n_time, double dt,
                               const double* data, size_t n_groups_data,
                               const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
                               int precision, dust2_filter** out, char* err, size_t errlen);
+
+/* The filter of any discrete-time model by name ("sir", "seir16"; a model
+ * without compare_data, such as "walk", is refused), as the reference
+ * instantiates one filter<dust_discrete<T>> per generated class T
+ * (dust2/r/discrete/filter.hpp:12-88, inst/template/discrete/compare.cpp). */
+int dust2_filter_discrete_alloc(const char* model, const double* pars, size_t n_groups,
+                                size_t n_particles, double time_start, const double* time,
+                                size_t n_time, double dt, const double* data,
+                                size_t n_groups_data, const uint64_t* seed_state,
+                                size_t n_seed_streams, uint64_t seed, int precision,
+                                dust2_filter** out, char* err, size_t errlen);
 
 /* Replaces dust2_filter_run<T> (dust2/r/filter.hpp:21-52) / filter<T>::run
  * (dust2/filter.hpp:120-182).  initial: NULL (model initial state) or
@@ -207,32 +219,43 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
 
 void dust2_filter_free(dust2_filter* h);
 
-/* ---- discrete-time system (stochastic SIR) --------------------------------
- * Replaces dust2_discrete_alloc<sir> (dust2/r/discrete/system.hpp:12-54) and
- * the dust2_system_sir_* methods (inst/template/system.cpp:1-54). */
+/* ---- discrete-time system ----------------------------------------------------
+ * Replaces dust2_discrete_alloc<T> (dust2/r/discrete/system.hpp:12-54) and
+ * the per-model dust2_system_<name>_* methods (inst/template/system.cpp:1-54):
+ * one set of methods for every discrete-time model, the model named at
+ * allocation ("sir"; "seir16"; "walk", inst/examples/walk.cpp, parameter
+ * row sd, len (1..8), random_initial (0 or 1), no data).  All groups must
+ * have the same state length (dust2/r/helpers.hpp:333-369). */
+int dust2_system_discrete_alloc(const char* model, const double* pars, size_t n_groups,
+                                size_t n_particles, double time, double dt,
+                                const uint64_t* seed_state, size_t n_seed_streams, uint64_t seed,
+                                int deterministic, int precision, dust2_system** out, char* err,
+                                size_t errlen);
+/* dust2_system_discrete_alloc("sir", ...), registered by the reference as
+ * dust2_system_sir_alloc (inst/template/system.cpp:1-4). */
 int dust2_system_sir_alloc(const double* pars, size_t n_groups, size_t n_particles,
                            double time, double dt, const uint64_t* seed_state,
                            size_t n_seed_streams, uint64_t seed, int deterministic,
                            int precision, dust2_system** out, char* err, size_t errlen);
-int dust2_system_sir_run_to_time(dust2_system* h, double time, char* err, size_t errlen);
-int dust2_system_sir_simulate(dust2_system* h, const double* times, size_t n_times,
+int dust2_system_discrete_run_to_time(dust2_system* h, double time, char* err, size_t errlen);
+int dust2_system_discrete_simulate(dust2_system* h, const double* times, size_t n_times,
                               double* out, char* err, size_t errlen);
-int dust2_system_sir_state(dust2_system* h, double* out, char* err, size_t errlen);
-int dust2_system_sir_set_state(dust2_system* h, const double* state, char* err,
+int dust2_system_discrete_state(dust2_system* h, double* out, char* err, size_t errlen);
+int dust2_system_discrete_set_state(dust2_system* h, const double* state, char* err,
                                size_t errlen);
-int dust2_system_sir_set_state_initial(dust2_system* h, char* err, size_t errlen);
-int dust2_system_sir_time(dust2_system* h, double* out, char* err, size_t errlen);
-int dust2_system_sir_set_time(dust2_system* h, double time, char* err, size_t errlen);
-int dust2_system_sir_update_pars(dust2_system* h, const double* pars, char* err,
+int dust2_system_discrete_set_state_initial(dust2_system* h, char* err, size_t errlen);
+int dust2_system_discrete_time(dust2_system* h, double* out, char* err, size_t errlen);
+int dust2_system_discrete_set_time(dust2_system* h, double time, char* err, size_t errlen);
+int dust2_system_discrete_update_pars(dust2_system* h, const double* pars, char* err,
                                  size_t errlen);
-int dust2_system_sir_compare_data(dust2_system* h, const double* data, size_t n_groups_data,
+int dust2_system_discrete_compare_data(dust2_system* h, const double* data, size_t n_groups_data,
                                   double* out, char* err, size_t errlen);
-int dust2_system_sir_reorder(dust2_system* h, const uint32_t* index, char* err,
+int dust2_system_discrete_reorder(dust2_system* h, const uint32_t* index, char* err,
                              size_t errlen);
-int dust2_system_sir_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen);
-int dust2_system_sir_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
+int dust2_system_discrete_rng_state(dust2_system* h, uint64_t* out, char* err, size_t errlen);
+int dust2_system_discrete_set_rng_state(dust2_system* h, const uint64_t* state, char* err,
                                    size_t errlen);
-void dust2_system_sir_free(dust2_system* h);
+void dust2_system_discrete_free(dust2_system* h);
 
 /* ---- deterministic likelihood of the SIR ODE (Dormand-Prince 5(4)) --------
  * Replaces dust2_continuous_unfilter_alloc<sirode>

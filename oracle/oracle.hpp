@@ -585,6 +585,41 @@ struct sirode_sens {
   }
 };
 
+// Random walk, inst/examples/walk.cpp:1-100, with the walker count `len`
+// fixed at compile time (the reference reads it from the parameters as a
+// constant).  Parameter row: sd, len, random_initial.  No data.
+template <size_t LEN>
+struct walk {
+  static constexpr size_t n_state = LEN;
+  static constexpr size_t n_pars = 3;
+  struct shared_state {
+    double sd;
+    bool random_initial;
+  };
+  struct data_type {};
+  static shared_state build_shared(const double* p) { return shared_state{p[0], p[2] != 0}; }
+  static constexpr size_t n_data = 0;
+  static data_type build_data(const double*) { return data_type{}; }
+  static constexpr double zero_period = 1;
+  static constexpr size_t zero_first = 0;
+  static constexpr size_t zero_count = 0;
+  // walk.cpp:50-62
+  static void initial(double, const shared_state& sh, rng_state& rng, double* y) {
+    for (size_t i = 0; i < LEN; ++i) {
+      y[i] = sh.random_initial ? normal(rng, 0, sh.sd) : 0;
+    }
+  }
+  // walk.cpp:65-76
+  static void update(double, double dt, const double* y, const shared_state& sh,
+                     rng_state& rng, double* yn) {
+    for (size_t i = 0; i < LEN; ++i) yn[i] = normal(rng, y[i], sh.sd * dt);
+  }
+  static double compare_data(double, const double*, const data_type&, const shared_state&,
+                             rng_state&) {
+    return 0;
+  }
+};
+
 // Age-structured stochastic SEIR with 16 age groups.  No counterpart exists
 // in the reference; written to the same contract.  State layout:
 // S[0:16] E[16:32] I[32:48] R[48:64] inc[64:80]; incidence is reset every

@@ -341,6 +341,31 @@ int oracle_seir16_unfilter_run(const double* pars, size_t G, size_t P, double ti
                                         err, errlen);
 }
 
+// The random walk as a system (walk.cpp), len walkers in 1..8.
+int oracle_walk_simulate(size_t len, const double* pars, size_t G, size_t P, double time_start,
+                         double dt, const double* times, size_t n_times, const uint64_t* rng_in,
+                         int deterministic, int n_threads, double* out, uint64_t* rng_out,
+                         char* err, size_t errlen) {
+  switch (len) {
+#define D2_WALK(L)                                                                         \
+  case L:                                                                                  \
+    return simulate_impl<walk<L>>(pars, G, P, time_start, dt, times, n_times, rng_in,      \
+                                  deterministic, n_threads, out, rng_out, err, errlen);
+    D2_WALK(1)
+    D2_WALK(2)
+    D2_WALK(3)
+    D2_WALK(4)
+    D2_WALK(5)
+    D2_WALK(6)
+    D2_WALK(7)
+    D2_WALK(8)
+#undef D2_WALK
+    default:
+      if (err && errlen) std::snprintf(err, errlen, "walk: len must be in 1..8");
+      return 1;
+  }
+}
+
 // ---- ODE unfilter ---------------------------------------------------------
 // control: [max_steps, atol, rtol, step_size_min, step_size_max].
 // stats_out (G*P x 4, or NULL): n_steps, n_steps_accepted,

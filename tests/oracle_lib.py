@@ -77,6 +77,7 @@ def oracle():
         o.oracle_sir_simulate.argtypes = sim
         o.oracle_seir16_simulate.argtypes = sim
         o.oracle_sir_tv_simulate.argtypes = sim
+        o.oracle_walk_simulate.argtypes = [C.c_size_t] + sim
         unf = [D, C.c_size_t, C.c_size_t, C.c_double, D, C.c_size_t, C.c_double, D,
                C.c_size_t, C.c_int, D, D, D, C.c_char_p, C.c_size_t]
         o.oracle_sir_unfilter_det_run.argtypes = unf
@@ -234,6 +235,26 @@ def sir_simulate(pars, n_particles, time_start, dt, times, rng, deterministic=Fa
           "sir_tv": oracle().oracle_sir_tv_simulate}[model]
     rc = fn(_dp(pars), G, P, float(time_start), float(dt), _dp(times), times.size, _u64(rng),
             int(deterministic), n_threads, _dp(out), _u64(rng_out), buf, 4096)
+    _check(rc, buf)
+    return out, rng_out
+
+
+def walk_simulate(pars, n_particles, time_start, dt, times, rng, deterministic=False):
+    """The random walk as a system; pars (G, 3): sd, len, random_initial
+    (len the same for every group).  Returns (n_times, G, P, len) and the
+    streams after."""
+    pars = np.ascontiguousarray(pars, dtype=np.float64)
+    G = pars.shape[0]
+    L = int(pars[0, 1])
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    rng = np.ascontiguousarray(rng, dtype=np.uint64)
+    out = np.zeros((times.size, G, n_particles, L))
+    rng_out = np.zeros((G * n_particles, 4), dtype=np.uint64)
+    buf = C.create_string_buffer(4096)
+    rc = oracle().oracle_walk_simulate(L, _dp(pars), G, n_particles, float(time_start),
+                                       float(dt), _dp(times), times.size, _u64(rng),
+                                       int(deterministic), 1, _dp(out), _u64(rng_out), buf,
+                                       4096)
     _check(rc, buf)
     return out, rng_out
 
