@@ -549,6 +549,10 @@ struct dust2_filter {
   // interval pipeline
   dbuf d_state_b, d_lambda, d_anc, d_sched_time, d_sched_steps, d_sched_zero, d_kpars, d_pslots,
       d_pslots_init;
+  // groups above one workgroup's LDS resample across workgroups
+  // (resample_kernels.hip): one task per slot, partial results in scratch
+  bool multi_block = false;
+  dbuf d_mb_tasks, d_mb_scratch;
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
   // the pipeline splits the groups into `pipe_chains` (at most kMaxChains)
@@ -775,7 +779,12 @@ void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool sa
       hip_check(launch_advance(h, slots, count, P, t, parity, mode, s), "advance kernel");
       mark();
       parity ^= 1;
-      hip_check(launch_pipe_resample(slots, count, P, t, s), "resample kernel");
+      if (h->multi_block) {
+        hip_check(launch_mb_resample(h->d_mb_tasks.get<mb_resample>() + first, count, P, t, s),
+                  "resample kernels");
+      } else {
+        hip_check(launch_pipe_resample(slots, count, P, t, s), "resample kernel");
+      }
       mark();
     }
     hip_check(launch_advance(h, slots, count, P, T, parity,
@@ -820,6 +829,30 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
     }
   }
   to_device(h->d_pslots, slots.data(), slots.size() * sizeof(pipe_slot), s);
+  if (h->multi_block) {
+    const size_t per = mb_resample_scratch_doubles(static_cast<int>(h->P));
+    if (h->d_mb_scratch.bytes() < G * per * sizeof(double)) {
+      h->d_mb_scratch.alloc(G * per * sizeof(double));
+    }
+    if (h->d_mb_tasks.bytes() < G * sizeof(mb_resample)) {
+      h->d_mb_tasks.alloc(G * sizeof(mb_resample));
+    }
+    std::vector<mb_resample> tasks(slots.size());
+    for (size_t i = 0; i < slots.size(); ++i) {
+      const pipe_slot& sl = slots[i];
+      mb_resample& tk = tasks[i];
+      tk.w = sl.lambda;
+      tk.idx = sl.ancestors;
+      tk.ancestry = sl.ancestry;
+      tk.ll = sl.ll;
+      tk.frng = sl.rng;
+      tk.frng_stride = static_cast<uint32_t>(h->P + 2);
+      tk.n_fallback = sl.n_fallback;
+      tk.scratch = h->d_mb_scratch.get<double>() + sl.group * per;
+      tk.P = static_cast<int>(h->P);
+    }
+    to_device(h->d_mb_tasks, tasks.data(), tasks.size() * sizeof(mb_resample), s);
+  }
   hip_check(hipStreamSynchronize(s), "sync");
   const int n_slots = static_cast<int>(slots.size());
   const int T = static_cast<int>(h->n_time);
@@ -962,8 +995,14 @@ static int filter_alloc_impl(int model, const double* pars, size_t n_groups,
     const size_t lds = (kScratchDoubles + n_particles) * sizeof(double);
     if (lds > static_cast<size_t>(max_lds_per_block()) ||
         pipe_resample_ppt(static_cast<int>(n_particles)) == 0) {
-      throw std::runtime_error("'n_particles' too large: one group's weights must fit one "
-                               "workgroup (at most 16384 particles)");
+      // too large for one workgroup's weights: resample across workgroups
+      if (mb_resample_blocks(static_cast<int>(n_particles)) > kMbMaxBlocks) {
+        throw std::runtime_error("'n_particles' too large: at most " +
+                                 std::to_string(static_cast<size_t>(kMbMaxBlocks) * 8192) +
+                                 " particles per group");
+      }
+      h->multi_block = true;
+      h->engine = DUST2_ENGINE_PIPELINE;
     }
     h->cfg = discrete_filter_default_config(static_cast<int>(n_particles));
     hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -2014,13 +2053,31 @@ void dust2_system_discrete_free(dust2_system* h) {
 int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* idx,
                        double* out3, char* err, size_t errlen) {
   try {
-    if (n < 1 || n > 2048) throw std::runtime_error("n must lie in [1, 2048]");
+    const size_t n_max = static_cast<size_t>(kMbMaxBlocks) * 8192;
+    if (n < 1 || n > n_max) {
+      throw std::runtime_error("n must lie in [1, " + std::to_string(n_max) + "]");
+    }
     hipStream_t s = nullptr;
     dbuf dw(n * sizeof(double)), di(n * sizeof(uint32_t)), dl(3 * sizeof(double));
     to_device(dw, log_weights, n * sizeof(double), s);
-    hip_check(launch_resample_test(dw.get<double>(), static_cast<int>(n), u, di.get<uint32_t>(),
-                                   dl.get<double>(), s),
-              "resample kernel");
+    if (n <= 2048) {
+      // one workgroup, the filter kernels' own routine
+      hip_check(launch_resample_test(dw.get<double>(), static_cast<int>(n), u,
+                                     di.get<uint32_t>(), dl.get<double>(), s),
+                "resample kernel");
+    } else {
+      // many workgroups, partial results and cumulative sums in scratch
+      dbuf scratch(mb_resample_scratch_doubles(static_cast<int>(n)) * sizeof(double));
+      mb_resample task{};
+      task.w = dw.get<double>();
+      task.idx = di.get<uint32_t>();
+      task.u = u;
+      task.out3 = dl.get<double>();
+      task.scratch = scratch.get<double>();
+      task.P = static_cast<int>(n);
+      hip_check(launch_mb_resample_one(task, 0, s), "resample kernels");
+      hip_check(hipStreamSynchronize(s), "sync");
+    }
     to_host(idx, di, n * sizeof(uint32_t), s);
     to_host(out3, dl, 3 * sizeof(double), s);
     return 0;

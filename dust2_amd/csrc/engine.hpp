@@ -157,6 +157,36 @@ hipError_t launch_discrete_advance(int model, bool fp64, const pipe_slot* d_slot
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
                                 hipStream_t stream);
 
+// ---- systematic resampling of groups above one workgroup ----------------------
+// (resample_kernels.hip)  One group per task; every pointer is device
+// memory.  w: the P log-weights; idx: the P ancestors written; ancestry:
+// [n_time][P] with row t_row written, or null; ll: += the log-likelihood
+// increment, or null; frng: the group's first filter stream (words at
+// frng[k * frng_stride]) the uniform is drawn from, or null to use u;
+// n_fallback: += 1 per sequential fallback, or null; out3: [increment,
+// resampled, fallback] or null; scratch: mb_resample_scratch_doubles(P)
+// doubles, fully written before it is read.
+constexpr int kMbMaxBlocks = 4096;  // at most 4096 * 8192 particles per group
+struct mb_resample {
+  const double* w;
+  uint32_t* idx;
+  uint32_t* ancestry;
+  double* ll;
+  uint64_t* frng;
+  uint32_t frng_stride;
+  double u;
+  uint32_t* n_fallback;
+  double* out3;
+  double* scratch;
+  int P;
+};
+int mb_resample_blocks(int P);
+size_t mb_resample_scratch_doubles(int P);
+hipError_t launch_mb_resample(const mb_resample* d_tasks, int n_tasks, int max_particles,
+                              int t_row, hipStream_t stream);
+// One task passed by value (no task table in device memory).
+hipError_t launch_mb_resample_one(const mb_resample& task, int t_row, hipStream_t stream);
+
 // ---- ODE unfilter ----------------------------------------------------------
 // dust2/continuous/control.hpp:11-24 (step_size_min already clamped to
 // machine epsilon, dust2/r/continuous/control.hpp:15-17)

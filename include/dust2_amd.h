@@ -566,12 +566,14 @@ void dust2_server_free(dust2_server* s);
 
 /* ---- kernels exposed for parity tests --------------------------------------
  * Systematic resampling of one weight vector on the GPU with the filter's
- * own workgroup routine (replaces the reference test driver
- * test_resample_weight / test_scale_log_weights, src/test.cpp:10-23).
- * log_weights: n <= 2048 log-weights; idx: n ancestors; out3: [ll increment,
- * resampled flag, sequential-fallback flag]. */
+ * own routines (replaces the reference test driver test_resample_weight /
+ * test_scale_log_weights, src/test.cpp:10-23; dust2/filter_details.hpp:11-74):
+ * one workgroup for n <= 2048, the multi-workgroup resampler of large
+ * groups above (n <= 4096 * 8192).  log_weights: n log-weights; idx: n
+ * ancestors; out3: [ll increment, resampled flag, sequential-fallback flag]. */
 int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* idx,
                        double* out3, char* err, size_t errlen);
+
 
 /* Lineage export of given saved states (the kernel behind
  * dust2_filter_last_trajectories): raw [n_time][G][P][S], ancestry
