@@ -106,6 +106,8 @@ _PROTOS = {
     "dust2_server_memory": (C.c_int, [VP, SZ, SZ, ERR, ERRLEN]),
     "dust2_server_free": (None, [VP]),
     "dust2_amd_resample": (C.c_int, [D, C.c_size_t, C.c_double, U32, D, ERR, ERRLEN]),
+    "dust2_amd_resample_noalloc": (C.c_int, [VP, C.c_size_t, C.c_double, VP, VP, VP, ERR,
+                                             ERRLEN]),
     "dust2_amd_trajectory_lineage": (C.c_int, [D, U32, C.c_size_t, C.c_size_t, C.c_size_t,
                                                C.c_size_t, SZ, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_set_critical_times": (C.c_int, [VP, D, C.c_size_t, ERR, ERRLEN]),

@@ -936,6 +936,38 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
 
 }  // namespace
 
+namespace {
+
+// Scratch of the allocation-free resampling entry point
+// (dust2_amd_resample_noalloc): the multi-workgroup resampler's partial
+// results and cumulative sums for up to kNoAllocMaxParticles weights.  Every
+// GPU the library runs on holds its own copy until the process ends; host
+// code uses only the per-device address from hipGetSymbolAddress, never
+// the array's name.
+constexpr size_t kNoAllocMaxParticles = size_t(1) << 20;
+constexpr size_t kNoAllocScratchDoubles = 8 + 4 * 128 + kNoAllocMaxParticles;
+static __device__ __align__(256) double noalloc_scratch[kNoAllocScratchDoubles];
+
+// The scratch array's device address on the current device, looked up once
+// per device (at dust2_amd_init, or the first allocation-free call there).
+double* noalloc_scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<double*> per_device;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(mu);
+  if (per_device.size() <= static_cast<size_t>(dev)) per_device.resize(dev + 1, nullptr);
+  if (per_device[dev] == nullptr) {
+    void* ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch)), "hipGetSymbolAddress");
+    if (ptr == nullptr) throw std::runtime_error("hipGetSymbolAddress returned no address");
+    per_device[dev] = static_cast<double*>(ptr);
+  }
+  return per_device[dev];
+}
+
+}  // namespace
+
 extern "C" {
 
 int dust2_amd_init(char* err, size_t errlen) {
@@ -944,6 +976,7 @@ int dust2_amd_init(char* err, size_t errlen) {
     hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
     if (n < 1) throw std::runtime_error("no GPU device available");
     tables_for_current_device();
+    (void)noalloc_scratch_ptr();
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -2080,6 +2113,41 @@ int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* 
     }
     to_host(idx, di, n * sizeof(uint32_t), s);
     to_host(out3, dl, 3 * sizeof(double), s);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_amd_resample_noalloc(const double* d_log_weights, size_t n, double u, uint32_t* d_idx,
+                               double* d_out3, void* stream, char* err, size_t errlen) {
+  try {
+    if (n < 1 || n > kNoAllocMaxParticles) {
+      throw std::runtime_error("n must lie in [1, " + std::to_string(kNoAllocMaxParticles) +
+                               "] (the allocation-free scratch holds " +
+                               std::to_string(kNoAllocMaxParticles) + " weights)");
+    }
+    if (d_log_weights == nullptr || d_idx == nullptr || d_out3 == nullptr) {
+      throw std::runtime_error("the weights, indices and results must be device buffers");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n <= 2048) {
+      hip_check(launch_resample_test(d_log_weights, static_cast<int>(n), u, d_idx, d_out3, s),
+                "resample kernel");
+    } else {
+      double* scratch = noalloc_scratch_ptr();
+      if (mb_resample_scratch_doubles(static_cast<int>(n)) > kNoAllocScratchDoubles) {
+        throw std::runtime_error("allocation-free scratch too small");
+      }
+      mb_resample task{};
+      task.w = d_log_weights;
+      task.idx = d_idx;
+      task.u = u;
+      task.out3 = d_out3;
+      task.scratch = scratch;
+      task.P = static_cast<int>(n);
+      hip_check(launch_mb_resample_one(task, 0, s), "resample kernels");
+    }
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

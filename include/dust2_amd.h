@@ -574,6 +574,18 @@ void dust2_server_free(dust2_server* s);
 int dust2_amd_resample(const double* log_weights, size_t n, double u, uint32_t* idx,
                        double* out3, char* err, size_t errlen);
 
+/* The same resampling with nothing allocated: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * d_log_weights (n doubles), d_idx (n uint32) and d_out3 (3 doubles) are
+ * the caller's device buffers; the work is queued on `stream` (a
+ * hipStream_t, NULL for the default stream) and the call returns without
+ * waiting.  The multi-workgroup path keeps its partial results in a
+ * library-owned device array of 8.4 MB per GPU, so n <= 1,048,576 (larger n
+ * is refused).  Calls that overlap on one GPU share that array and race:
+ * order them on one stream or synchronise between them. */
+int dust2_amd_resample_noalloc(const double* d_log_weights, size_t n, double u, uint32_t* d_idx,
+                               double* d_out3, void* stream, char* err, size_t errlen);
 
 /* Lineage export of given saved states (the kernel behind
  * dust2_filter_last_trajectories): raw [n_time][G][P][S], ancestry
