@@ -96,6 +96,7 @@ _PROTOS = {
     "dust2_unfilter_sir_last_gradient": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sir_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_unfilter_sir_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_sir_set_adjoint_form": (C.c_int, [VP, C.c_int, ERR, ERRLEN]),
     "dust2_unfilter_sir_free": (None, [VP]),
     "dust2_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(VP), ERR, ERRLEN]),
     "dust2_server_submit": (C.c_int, [VP, C.c_int, D, C.c_size_t, C.c_double, D, C.c_size_t,

@@ -25,7 +25,7 @@ namespace {
 
 // Per particle-thread: k = g * P + p.  Parameter block per group (8
 // doubles): N, I0, beta, gamma, exp_noise, p_IR = 1 - exp(-gamma dt),
-// e_IR = exp(-gamma dt) (both from the host), unused.
+// e_IR = exp(-gamma dt), exp(-gamma) (all from the host).
 __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n_total) return;
@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
   sh.exp_noise = pg[4];
   sh.p_IR = pg[5];
   const double e_IR = pg[6];
+  const double e_gamma = pg[7];
   const size_t NT = static_cast<size_t>(a.n_total);
 
   rng256 det_rng{0, 0, 0, 0};
@@ -95,7 +96,9 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
   for (int i = a.n_time - 1; i >= 0; --i) {
     const size_t o = static_cast<size_t>(i) * a.obs_stride + g * a.obs_group_stride;
     const double x = a.obs[o];
-    if (!isnan(x)) {
+    // the reference's compare adjoint has no missing-data branch
+    // (sir.cpp:179-197), so NaN data give it a NaN gradient
+    if (!a.exact || !isnan(x)) {
       adj[4] += x / (a.inc[i * NT + k] + noise) - 1;
     }
     const double t_end = a.sched_time[i] + a.sched_steps[i] * a.dt;
@@ -103,10 +106,11 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
       --step;
       const double S = a.hist[(2 * step) * NT + k];
       const double I = a.hist[(2 * step + 1) * NT + k];
-      // sir.cpp:152-176 with the derivative of p_IR = 1 - exp(-gamma dt)
-      // taken as dt * exp(-gamma dt)
+      // sir.cpp:152-176.  Exact: the derivative of p_IR = 1 - exp(-gamma dt)
+      // is dt * exp(-gamma dt); the reference writes exp(-gamma) and
+      // evaluates p_SI in the forward model's association
       const double e_SI = exp(-sh.beta * I * a.dt / sh.N);
-      const double p_SI = 1 - e_SI;
+      const double p_SI = a.exact ? 1 - e_SI : 1 - exp(-sh.beta * I / sh.N * a.dt);
       const double adj_n_IR = -adj[1] + adj[2];
       const double adj_n_SI = adj[4] + adj[3] + adj[1] - adj[0];
       const double adj_p_IR = adj_n_IR * I;
@@ -114,7 +118,7 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
       adj[0] = adj_n_SI * p_SI + adj[0];
       adj[1] = adj_n_IR * sh.p_IR + adj_p_SI * (sh.beta * a.dt) * e_SI / sh.N + adj[1];
       adj[5] = adj[5] + adj_p_SI * I * a.dt * e_SI / sh.N;
-      adj[6] = adj[6] + adj_p_IR * a.dt * e_IR;
+      adj[6] = a.exact ? adj[6] + adj_p_IR * a.dt * e_IR : adj[6] + adj_p_IR * e_gamma;
       const double time_s = t_end - s * a.dt;
       if (fmod(time_s, sir_model::zero_period) == 0) {
         adj[sir_model::zero_first] = 0;
@@ -122,9 +126,14 @@ __global__ void __launch_bounds__(256) sir_unfilter_kernel(sir_unfilter_launch a
     }
   }
   // Initial conditions S0 = N - I0, I0 = I0 (sir.cpp:38-48): dS0/dI0 = -1,
-  // dI0/dI0 = 1.  A caller-supplied initial state does not depend on I0.
-  if (a.set_initial) {
-    adj[7] += adj[1] - adj[0];
+  // dI0/dI0 = 1, and a caller-supplied initial state does not depend on I0.
+  // The reference's adjoint_initial adds adj_I only (sir.cpp:126-141), and
+  // compute_gradient_ applies it whatever the initial state
+  // (dust2/unfilter.hpp:219-222).
+  if (a.exact) {
+    if (a.set_initial) adj[7] += adj[1] - adj[0];
+  } else {
+    adj[7] += adj[1];
   }
   a.grad[3 * k + 0] = adj[5];
   a.grad[3 * k + 1] = adj[6];

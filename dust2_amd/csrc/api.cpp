@@ -4075,6 +4075,7 @@ struct dust2_unfilter_sir {
   dbuf d_pars, d_sched_time, d_sched_steps, d_obs, d_lfact, d_state, d_ll, d_errors,
       d_counters, d_active, d_hist, d_inc, d_grad, d_traj;
   bool traj_valid = false;
+  bool exact_gradient = false;        // the reference's adjoint unless asked
   ~dust2_unfilter_sir() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -4088,12 +4089,12 @@ void check_unfilter_sir(dust2_unfilter_sir* h) {
 }
 
 // N, I0, beta, gamma, exp_noise -> N, I0, beta, gamma, exp_noise,
-// 1 - exp(-gamma dt), exp(-gamma dt), 0
+// 1 - exp(-gamma dt), exp(-gamma dt), exp(-gamma)
 void sir_adjoint_pars(const double* row, double dt, double* out) {
   for (int i = 0; i < 5; ++i) out[i] = row[i];
   out[5] = 1 - std::exp(-row[3] * dt);
   out[6] = std::exp(-row[3] * dt);
-  out[7] = 0;
+  out[7] = std::exp(-row[3]);
 }
 
 void upload_sir_adjoint_pars(dust2_unfilter_sir* h) {
@@ -4232,6 +4233,7 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int sav
     a.grad = adjoint ? h->d_grad.get<double>() : nullptr;
     a.errors = h->d_errors.get<error_record>();
     a.error_count = h->d_counters.get<uint32_t>();
+    a.exact = h->exact_gradient ? 1 : 0;
     hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
     hip_check(launch_sir_unfilter(a, s), "sir unfilter kernel");
     hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
@@ -4347,6 +4349,19 @@ int dust2_unfilter_sir_last_state(dust2_unfilter_sir* h, double* out, char* err,
     for (size_t k = 0; k < n; ++k) {
       for (size_t c = 0; c < 5; ++c) out[k * 5 + c] = tmp[c * n + k];
     }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_set_adjoint_form(dust2_unfilter_sir* h, int exact, char* err,
+                                        size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    if (exact != 0 && exact != 1) throw std::runtime_error("'exact' must be 0 or 1");
+    h->exact_gradient = exact != 0;
+    std::fill(h->grad_current.begin(), h->grad_current.end(), 0);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

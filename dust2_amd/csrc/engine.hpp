@@ -279,7 +279,8 @@ hipError_t launch_dp5_system(const ode_sys_launch& args, hipStream_t stream);
 
 // ---- deterministic discrete SIR unfilter and adjoint -------------------------
 struct sir_unfilter_launch {
-  const double* pars;           // [G][8]: N, I0, beta, gamma, exp_noise, p_IR, e_IR, -
+  const double* pars;           // [G][8]: N, I0, beta, gamma, exp_noise, p_IR, e_IR,
+                                //   exp(-gamma)
   const uint8_t* active;        // [G] groups to run, or null for all
   int n_particles;
   int n_total;                  // G * P
@@ -300,6 +301,9 @@ struct sir_unfilter_launch {
   double* traj;                 // [n_time][S][n_total] state at each data time, or null
   error_record* errors;         // [n_total]
   uint32_t* error_count;
+  // 0: the reference's hand-written adjoint (inst/examples/sir.cpp:126-198)
+  // term for term; 1: the derivative of the computed likelihood
+  int exact;
 };
 
 hipError_t launch_sir_unfilter(const sir_unfilter_launch& args, hipStream_t stream);

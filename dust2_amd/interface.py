@@ -1317,7 +1317,12 @@ class Unfilter:
     give identical values)."""
 
     def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control,
-                 precision, shared_data=None):
+                 precision, shared_data=None, adjoint_form="reference"):
+        if adjoint_form not in ("reference", "exact"):
+            raise ValueError("'adjoint_form' must be 'reference' or 'exact'")
+        # discrete SIR gradient: the reference's hand-written adjoint
+        # (inst/examples/sir.cpp:126-198) or the exact derivative
+        self.adjoint_form = adjoint_form
         if generator.name == "sirode":
             if dt is not None:
                 raise ValueError("Can't use 'dt' with continuous-time systems")
@@ -1367,6 +1372,8 @@ class Unfilter:
             call("dust2_unfilter_sir_alloc", dptr(self.rows), self.n_groups, self.n_particles,
                  self.time_start, dptr(self.time), self.time.size, self.dt, dptr(self.data),
                  self.n_groups_data, C.byref(out))
+            call("dust2_unfilter_sir_set_adjoint_form", out,
+                 int(self.adjoint_form == "exact"))
         self.ptr = out
 
     def run(self, pars, index_group=None, initial=None, adjoint=False,
@@ -1444,13 +1451,16 @@ class Unfilter:
 
 
 def dust_unfilter_create(generator, time_start, data, n_particles=1, n_groups=1, dt=None,
-                         ode_control=None, precision="fp64", shared_data=None):
-    """R/interface-unfilter.R:20-71; shared_data as dust_filter_create."""
+                         ode_control=None, precision="fp64", shared_data=None,
+                         adjoint_form="reference"):
+    """R/interface-unfilter.R:20-71; shared_data as dust_filter_create.
+    adjoint_form (discrete SIR): 'reference' for the reference's hand-written
+    adjoint gradient, 'exact' for the derivative of the likelihood."""
     if generator.name in ModelFilter.DATA_FIELDS:
         return ModelUnfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control,
                              shared_data)
     return Unfilter(generator, time_start, data, n_particles, n_groups, dt, ode_control,
-                    precision, shared_data)
+                    precision, shared_data, adjoint_form=adjoint_form)
 
 
 def dust_likelihood_last_gradient(obj):

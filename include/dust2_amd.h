@@ -520,6 +520,17 @@ int dust2_unfilter_sir_last_trajectories(dust2_unfilter_sir* h, double* out, cha
 int dust2_unfilter_sir_last_gradient(dust2_unfilter_sir* h, double* out, char* err,
                                      size_t errlen);
 
+/* Which gradient dust2_unfilter_sir_last_gradient returns.  exact = 0 (the
+ * default): the reference's hand-written adjoint term for term
+ * (inst/examples/sir.cpp:126-198 through dust2/unfilter.hpp:188-229): the
+ * gamma term multiplied by exp(-gamma) whatever dt (sir.cpp:175), no -adj_S
+ * from S0 = N - I0 in the I0 gradient (sir.cpp:126-141), NaN for missing
+ * data (sir.cpp:179-197).  exact = 1: the derivative of the computed
+ * log-likelihood (no counterpart in the reference).  Invalidates the last
+ * gradient. */
+int dust2_unfilter_sir_set_adjoint_form(dust2_unfilter_sir* h, int exact, char* err,
+                                        size_t errlen);
+
 /* Replaces dust2_unfilter_update_pars<T> (dust2/r/unfilter.hpp:10-19): I0,
  * beta and gamma update, N and exp_noise are constant. */
 int dust2_unfilter_sir_update_pars(dust2_unfilter_sir* h, const double* pars,

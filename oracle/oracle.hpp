@@ -442,25 +442,29 @@ struct sir {
   }
 
   // Discrete adjoint (sir.cpp:126-198).  n_adjoint = 5 states + 3
-  // parameters (beta, gamma, I0).  Two terms differ from the reference's
-  // hand-written adjoint, which the finite-difference checks in the tests
-  // settle: the gamma term uses the derivative of p_IR = 1 - exp(-gamma dt),
-  // dt*exp(-gamma dt), where the reference writes exp(-gamma) (sir.cpp:175;
-  // equal at dt = 1); and the initial condition S0 = N - I0 contributes
-  // -adj_S to the I0 gradient, which the reference's adjoint_initial
-  // (sir.cpp:126-141) leaves out.
+  // parameters (beta, gamma, I0).  `exact` false: the reference's
+  // hand-written adjoint term for term -- its gamma term multiplies by
+  // exp(-gamma) (sir.cpp:175; the derivative of p_IR = 1 - exp(-gamma dt)
+  // is dt exp(-gamma dt), equal only at dt = 1), its adjoint_initial adds
+  // adj_I but not the -adj_S of S0 = N - I0 to the I0 gradient
+  // (sir.cpp:126-141), and its compare adjoint has no missing-data branch
+  // (sir.cpp:179-197: NaN data give a NaN gradient).  `exact` true: the
+  // derivative of the likelihood the forward run computes, which the
+  // finite-difference checks in the tests settle.
   static constexpr size_t n_gradient = 3;
   static void adjoint_initial(const double*, const double* adj, const shared_state&,
-                              double* adj_next) {
+                              double* adj_next, bool exact) {
     for (size_t i = 0; i < 8; ++i) adj_next[i] = adj[i];
-    adj_next[7] = adj[7] + adj[1] - adj[0];
+    adj_next[7] = exact ? adj[7] + adj[1] - adj[0] : adj[7] + adj[1];
   }
   static void adjoint_update(double dt, const double* y, const double* adj,
-                             const shared_state& sh, double* an) {
+                             const shared_state& sh, double* an, bool exact) {
     const double S = y[0];
     const double I = y[1];
     const double e_SI = std::exp(-sh.beta * I * dt / sh.N);
-    const double p_SI = 1 - e_SI;
+    // the reference evaluates p_SI in the forward model's association
+    // (sir.cpp:161)
+    const double p_SI = exact ? 1 - e_SI : 1 - std::exp(-sh.beta * I / sh.N * dt);
     const double e_IR = std::exp(-sh.gamma * dt);
     const double p_IR = 1 - e_IR;
     const double adj_n_IR = -adj[1] + adj[2];
@@ -473,14 +477,14 @@ struct sir {
     an[3] = adj[3];
     an[4] = adj[4];
     an[5] = adj[5] + adj_p_SI * I * dt * e_SI / sh.N;
-    an[6] = adj[6] + adj_p_IR * dt * e_IR;
+    an[6] = exact ? adj[6] + adj_p_IR * dt * e_IR : adj[6] + adj_p_IR * std::exp(-sh.gamma);
     an[7] = adj[7];
   }
   static void adjoint_compare_data(const double* y, const double* adj,
                                    const data_type& d, const shared_state& sh,
-                                   double* an) {
+                                   double* an, bool exact) {
     for (size_t i = 0; i < 8; ++i) an[i] = adj[i];
-    if (std::isnan(d.incidence)) {
+    if (exact && std::isnan(d.incidence)) {
       return;
     }
     const double lambda = y[4] + 1 / sh.exp_noise;

@@ -412,7 +412,7 @@ int oracle_sirode_unfilter_run(const double* pars, size_t G, size_t P,
 int oracle_sir_unfilter_run(const double* pars, size_t G, double time_start,
                             const double* time, size_t n_time, double dt,
                             const double* data, double* ll_out,
-                            double* grad_out, char* err, size_t errlen) {
+                            double* grad_out, int exact, char* err, size_t errlen) {
   try {
     constexpr size_t S = sir::n_state;
     constexpr size_t A = S + sir::n_gradient;
@@ -450,7 +450,7 @@ int oracle_sir_unfilter_run(const double* pars, size_t G, double time_start,
         const double* state_i = history.data() + offset[i + 1];
         for (size_t g = 0; g < G; ++g) {
           sir::adjoint_compare_data(state_i + g * S, ac + g * A, d[i * G + g],
-                                    sys.shared()[g], an + g * A);
+                                    sys.shared()[g], an + g * A, exact != 0);
         }
         std::swap(ac, an);
         const size_t n_steps = std::round(std::max(0.0, tcur - time_i) / dt);
@@ -460,7 +460,7 @@ int oracle_sir_unfilter_run(const double* pars, size_t G, double time_start,
           for (size_t s = 1; s <= n_steps; ++s) {
             const double time_s = tcur - s * dt;
             const double* st = state_i + g * S - s * stride;
-            sir::adjoint_update(dt, st, c, sys.shared()[g], n);
+            sir::adjoint_update(dt, st, c, sys.shared()[g], n, exact != 0);
             std::swap(c, n);
             if (std::fmod(time_s, sir::zero_period) == 0) {
               c[sir::zero_first] = 0;
@@ -474,7 +474,7 @@ int oracle_sir_unfilter_run(const double* pars, size_t G, double time_start,
       }
       for (size_t g = 0; g < G; ++g) {
         sir::adjoint_initial(history.data() + g * S, ac + g * A, sys.shared()[g],
-                             an + g * A);
+                             an + g * A, exact != 0);
       }
       for (size_t g = 0; g < G; ++g) {
         for (size_t j = 0; j < sir::n_gradient; ++j) {

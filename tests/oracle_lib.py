@@ -86,7 +86,8 @@ def oracle():
                                                  C.c_size_t, D, C.c_size_t, D, C.c_int, D, D,
                                                  D, C.c_char_p, C.c_size_t]
         o.oracle_sir_unfilter_run.argtypes = [D, C.c_size_t, C.c_double, D, C.c_size_t,
-                                              C.c_double, D, D, D, C.c_char_p, C.c_size_t]
+                                              C.c_double, D, D, D, C.c_int, C.c_char_p,
+                                              C.c_size_t]
     return _oracle
 
 
@@ -338,7 +339,10 @@ def sirode_gradient(pars, time_start, time, data, control=None, n_threads=1):
     return ll, grad
 
 
-def sir_unfilter(pars, time_start, time, dt, data, gradient=True):
+def sir_unfilter(pars, time_start, time, dt, data, gradient=True, exact=False):
+    """Deterministic SIR unfilter and its adjoint gradient (beta, gamma, I0)
+    per group: the reference's hand-written adjoint (exact=False) or the
+    derivative of the likelihood (exact=True)."""
     pars = np.ascontiguousarray(pars, dtype=np.float64)
     G = pars.shape[0]
     time = np.ascontiguousarray(time, dtype=np.float64)
@@ -349,7 +353,7 @@ def sir_unfilter(pars, time_start, time, dt, data, gradient=True):
     buf = C.create_string_buffer(4096)
     rc = oracle().oracle_sir_unfilter_run(_dp(pars), G, float(time_start), _dp(time),
                                           time.size, float(dt), _dp(data), _dp(ll), _dp(grad),
-                                          buf, 4096)
+                                          int(bool(exact)), buf, 4096)
     _check(rc, buf)
     return ll, grad
 

@@ -97,7 +97,7 @@ def test_adjoint_gradient_matches_finite_differences(times, dt):
                      [1000, 10, 0.3, 0.05, 1e6]], dtype=float)
     time = np.array(times)
     data = np.tile(np.arange(1.0, 5.0)[:, None], (1, 3))
-    ll, grad = ol.sir_unfilter(pars, 0.0, time, dt, data)
+    ll, grad = ol.sir_unfilter(pars, 0.0, time, dt, data, exact=True)
     ll2, _ = ol.sir_unfilter(pars, 0.0, time, dt, data, gradient=False)
     assert np.array_equal(ll, ll2)  # the adjoint run gives the same likelihood
     fd = _fd_gradient(pars, time, dt, data)
@@ -110,9 +110,82 @@ def test_adjoint_gradient_with_missing_data_and_long_series():
     time = np.arange(1.0, 61.0)
     data = rng.poisson(20, (60, 1)).astype(float)
     data[[3, 17, 40], 0] = np.nan
-    _, grad = ol.sir_unfilter(pars, 0.0, time, 0.25, data)
+    _, grad = ol.sir_unfilter(pars, 0.0, time, 0.25, data, exact=True)
     fd = _fd_gradient(pars, time, 0.25, data)
     np.testing.assert_allclose(grad, fd, rtol=1e-5)
+
+
+def _numpy_sir_ll(row, state0, time, dt, data):
+    """Deterministic SIR log-likelihood from a given initial state, written
+    from inst/examples/sir.cpp (binomial -> n p, Exp(rate) -> 1 / rate,
+    incidence reset at integer times)."""
+    from scipy import stats
+    N, _, beta, gamma, noise = row
+    S, I, R, cum, inc = state0
+    t, ll = 0.0, 0.0
+    for i, target in enumerate(time):
+        n = int(round((target - t) / dt))
+        for s in range(n):
+            if np.fmod(t + s * dt, 1.0) == 0:
+                inc = 0.0
+            n_SI = S * (1 - np.exp(-beta * I / N * dt))
+            n_IR = I * (1 - np.exp(-gamma * dt))
+            S, I, R, cum, inc = S - n_SI, I + n_SI - n_IR, R + n_IR, cum + n_SI, inc + n_SI
+        t += n * dt
+        if not np.isnan(data[i]):
+            ll += stats.poisson.logpmf(data[i], inc + 1 / noise)
+    return ll
+
+
+@pytest.mark.parametrize("times,dt", [([4.0, 8.0, 12.0, 16.0], 1.0),    # test-unfilter-adjoint.R:14-27
+                                      ([3.0, 6.0, 9.0, 12.0], 1.0),     # :120-133
+                                      ([4.0, 8.0, 12.0, 16.0], 0.25),
+                                      ([2.5, 5.0, 7.5], 0.5)])
+def test_reference_adjoint_pinned_by_what_it_differentiates(times, dt):
+    # The reference's hand-written adjoint (inst/examples/sir.cpp:126-198)
+    # differentiates: beta exactly; gamma through exp(-gamma) where p_IR's
+    # derivative is dt exp(-gamma dt), a constant factor per group, so its
+    # gamma gradient is the true one times exp(-gamma) / (dt exp(-gamma dt));
+    # and I0 through I(0) alone, holding S(0) fixed.  Each is checked
+    # against central differences of an independent numpy recursion.
+    pars = np.array([[1000, 10, 0.1, 0.2, 1e6], [5000, 25, 0.35, 0.12, 1e6]], dtype=float)
+    time = np.array(times)
+    data = np.tile(np.arange(1.0, time.size + 1)[:, None], (1, 2))
+    _, grad = ol.sir_unfilter(pars, 0.0, time, dt, data)
+    fd = _fd_gradient(pars, time, dt, data)
+    for g in range(2):
+        N, I0, beta, gamma = pars[g, :4]
+        assert grad[g, 0] == pytest.approx(fd[g, 0], rel=1e-5)
+        factor = np.exp(-gamma) / (dt * np.exp(-gamma * dt))
+        assert grad[g, 1] == pytest.approx(fd[g, 1] * factor, rel=1e-5)
+        h = 1e-6 * I0
+        up = _numpy_sir_ll(pars[g], [N - I0, I0 + h, 0, 0, 0], time, dt, data[:, g])
+        dn = _numpy_sir_ll(pars[g], [N - I0, I0 - h, 0, 0, 0], time, dt, data[:, g])
+        assert grad[g, 2] == pytest.approx((up - dn) / (2 * h), rel=1e-5)
+        # the exact I0 gradient also moves S(0) = N - I0
+        assert abs(grad[g, 2] - fd[g, 2]) > 1e-3 * abs(fd[g, 2])
+
+
+def test_numpy_recursion_equals_oracle_likelihood():
+    pars = np.array([[5000, 25, 0.35, 0.12, 1e6]])
+    time = np.array([2.5, 5.0, 7.5])
+    data = np.array([[3.0], [np.nan], [9.0]])
+    ll, _ = ol.sir_unfilter(pars, 0.0, time, 0.5, data, gradient=False)
+    N, I0 = pars[0, :2]
+    assert ll[0] == pytest.approx(_numpy_sir_ll(pars[0], [N - I0, I0, 0, 0, 0], time, 0.5,
+                                                data[:, 0]), rel=1e-12)
+
+
+def test_reference_adjoint_is_nan_with_missing_data():
+    # the reference's compare adjoint has no missing-data branch
+    # (sir.cpp:179-197); the exact form skips missing data
+    pars = np.array([[1e4, 10, 0.25, 0.1, 1e6]])
+    time = np.arange(1.0, 11.0)
+    data = np.full((10, 1), 5.0)
+    data[4, 0] = np.nan
+    _, ref = ol.sir_unfilter(pars, 0.0, time, 1.0, data)
+    _, exact = ol.sir_unfilter(pars, 0.0, time, 1.0, data, exact=True)
+    assert np.all(np.isnan(ref)) and np.all(np.isfinite(exact))
 
 
 def _ode_fd(pars, time, data, control=None, rel=1e-6):
