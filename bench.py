@@ -43,7 +43,46 @@ SIR_NAMES = ["N", "I0", "beta", "gamma", "exp_noise"]
 
 
 def cpu_threads():
-    return min(16, os.cpu_count() or 1)
+    """nproc: the CPUs this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def sized_sample(run, g_max, target_s):
+    """The number of groups g whose oracle run takes about target_s: doubles
+    g until a run takes 0.3 s, then scales."""
+    g = 1
+    while True:
+        t0 = _time.perf_counter()
+        run(g)
+        el = _time.perf_counter() - t0
+        if el > 0.3 or g >= g_max:
+            break
+        g = min(g_max, g * 2)
+    return int(max(1, min(g_max, round(g * target_s / max(el, 1e-3)))))
+
+
+def median_runs(run, g, n_runs):
+    """Median wall time of n_runs calls run(g, k) (k = the replicate)."""
+    times, outs = [], []
+    for k in range(n_runs):
+        t0 = _time.perf_counter()
+        outs.append(run(g, k))
+        times.append(_time.perf_counter() - t0)
+    return float(np.median(times)), outs
 
 
 def oracle():
@@ -200,22 +239,73 @@ def bench_sir(lib, args):
         "mean_ll": float(np.mean(ll)),
         "roofline": rl,
     }
+    # the same workload in FP64 (the reference's precision)
+    if not args.no_fp64:
+        obj64 = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                       n_groups=G, dt=dt, seed=1, precision="fp64")
+        _, el64, ms64 = timed_runs(lib, obj64, plist, 2, 1)
+        out["fp64"] = {"particle_steps_per_s": G * P * n_steps * 2 / el64,
+                       "ms_per_run": 1e3 * el64 / 2, "gpu_ms_per_run": float(np.mean(ms64)),
+                       "engine": lib.dust_likelihood_last_stats(obj64)["engine"]}
+        del obj64
+    # the plain path: one filter object per group, one group per launch
+    # sequence (warmed by one run each, then one timed run each)
+    n_plain = 8
+    plain_s = 0.0
+    for g in range(n_plain):
+        one = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data[:, g]},
+                                     P, dt=dt, seed=100 + g, precision=args.precision)
+        lib.dust_likelihood_run(one, plist[g])
+        t0 = _time.perf_counter()
+        lib.dust_likelihood_run(one, plist[g])
+        plain_s += _time.perf_counter() - t0
+    out["plain_evals_per_s"] = n_plain / plain_s
+    out["batched_over_plain"] = out["likelihood_evals_per_s"] / out["plain_evals_per_s"]
     if not args.no_cpu:
         ol = oracle()
         threads = cpu_threads()
-        g = 1
-        while True:
-            t0 = _time.perf_counter()
-            ol.sir_filter(pars[:g], P, 0.0, time, dt, data[:, :g], seed=1, n_threads=threads)
-            el = _time.perf_counter() - t0
-            if el > 4 or g >= G:
-                break
-            g = min(G, max(g * 2, int(g * 4.5 / max(el, 1e-3))))
+
+        def cpu_run(g, k=0, n_threads=threads):
+            return ol.sir_filter(pars[:g], P, 0.0, time, dt, data[:, :g], seed=1 + k,
+                                 n_threads=n_threads)["ll"]
+
+        # nproc threads: about 3 s per run, median of 3 (seeds 1, 2, 3)
+        g = sized_sample(lambda n: cpu_run(n), G, 3.0)
+        el, ref = median_runs(cpu_run, g, 3)
+        # one thread: about 2 s per run, median of 3
+        g1 = sized_sample(lambda n: cpu_run(n, 0, 1), G, 2.0)
+        el1, _ = median_runs(lambda n, k: cpu_run(n, k, 1), g1, 3)
         out["cpu_baseline"] = {
             "value": g * P * n_steps / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{g} of {G} groups x {P} particles x {time.size} obs, FP64 oracle "
-                      f"(OpenMP over group x particle), {el:.1f} s"}
+                      f"(OpenMP over group x particle, {threads} threads = nproc), median of "
+                      f"3 runs of {el:.1f} s",
+            "one_thread": g1 * P * n_steps / el1,
+            "one_thread_sample": f"{g1} groups, median of 3 runs of {el1:.1f} s",
+            "cpu_model": cpu_model(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+        # FP32 batched filter against the FP64 oracle on the sampled groups:
+        # K GPU replicates of each (one batched launch of K * g groups)
+        # against the oracle's 3 replicates; z = sum of the per-group mean
+        # differences over its Monte Carlo standard error, with each group's
+        # variance from its GPU replicates
+        K = 16
+        rep = lib.dust_filter_create(lib.sir(), 0.0,
+                                     {"time": time, "incidence": np.repeat(data[:, :g], K, 1)},
+                                     P, n_groups=g * K, dt=dt, seed=2, precision=args.precision)
+        ll_rep = np.atleast_1d(lib.dust_likelihood_run(rep, [p for p in plist[:g]
+                                                             for _ in range(K)]))
+        del rep
+        ll_rep = ll_rep.reshape(g, K)
+        m_gpu = ll_rep.mean(axis=1)
+        v = ll_rep.var(axis=1, ddof=1)
+        m_cpu = np.mean(np.array(ref), axis=0)
+        se_g = np.sqrt(v / K + v / len(ref))
+        out["vs_fp64_oracle"] = {
+            "groups": g, "gpu_replicates": K, "oracle_replicates": len(ref),
+            "mean_delta_ll": float(np.mean(m_gpu - m_cpu)),
+            "delta_ll_in_mc_se": float(np.sum(m_gpu - m_cpu) / np.sqrt(np.sum(se_g ** 2))),
+            "max_abs_group_z": float(np.max(np.abs(m_gpu - m_cpu) / se_g))}
     return out
 
 
@@ -555,6 +645,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the engine autotuner")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run of the default "
+                                                            "workload")
     ap.add_argument("--chunk", type=int, default=16)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     args = ap.parse_args()
