@@ -233,23 +233,27 @@ __device__ __forceinline__ void random_pair_f32(rng256& r, float& u, float& v) {
 // Stirling-series remainder in FP32, log(k!) - (k + 1/2) log(k + 1) + (k + 1)
 // - log(2 pi) / 2: exact values for k < 4, the series 1/12 - 1/360 x^-2 +
 // 1/1260 x^-4 over x = k + 1 above (truncation below 8e-9 from k = 4),
-// without a branch so that lanes on either side run together.
+// without a per-lane branch so that lanes on either side run together; the
+// exact values are selected only when some lane of the wavefront needs one
+// (BTRS rarely evaluates the tail below k = 4).
 __device__ __forceinline__ float stirling_tail_f32(float k) {
   const float r = f_rcp(k + 1);
   const float r2 = r * r;
   float t = (0.083333333f - (0.0027777778f - r2 * 0.00079365079f) * r2) * r;
-  t = k == 3 ? 0.020790672103765093f : t;
-  t = k == 2 ? 0.02767792568499834f : t;
-  t = k == 1 ? 0.0413406959554093f : t;
-  t = k == 0 ? 0.08106146679532726f : t;
+  if (__any(k < 4)) {
+    t = k == 3 ? 0.020790672103765093f : t;
+    t = k == 2 ? 0.02767792568499834f : t;
+    t = k == 1 ? 0.0413406959554093f : t;
+    t = k == 0 ? 0.08106146679532726f : t;
+  }
   return t;
 }
 
-// 1/j for j = 0..112 (entry 0 unused), correctly rounded.
+// 1/j for j = 0..115 (entry 0 unused), correctly rounded.
 struct inv_int_table {
-  float v[113];
+  float v[116];
   constexpr inv_int_table() : v() {
-    for (int j = 1; j < 113; ++j) v[j] = 1.0f / static_cast<float>(j);
+    for (int j = 1; j < 116; ++j) v[j] = 1.0f / static_cast<float>(j);
   }
 };
 __constant__ constexpr inv_int_table kInvInt{};
@@ -263,6 +267,16 @@ __constant__ constexpr inv_int_table kInvInt{};
 // walk all the way to k = n before restarting, up to 1e6 iterations for one
 // lane that the rest of its wave waits on.  Restarting at 110 instead gives
 // the same draw sequence.
+//
+// The search is uniform across the wavefront: every lane steps j together
+// and a lane whose search has ended keeps its values through selects, so
+// the loop needs no per-lane exit bookkeeping (which made the divergent
+// form scalar-bound: about 14 scalar instructions per step against 8
+// vector ones).  Four steps per trip, the wavefront leaving when no lane is
+// searching; 1/j comes from a constant table, uniform across the wavefront.
+// Each lane's draw is that of the per-lane search.  A lane searching at the
+// start of a trip has k = j - 1 <= 110, so the trip reads entries up to
+// j + 3 <= 114.
 __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
   const float q = 1 - p;
   const float r = p * f_rcp(q);
@@ -273,17 +287,19 @@ __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
     float u = random_real<float>(rng);
     float f = f0;
     float k = 0;
-    // One exit test per step; leaving with k > k_max restarts the search.
-    // Every lane still searching is at step j together, so 1/j is one
-    // table entry for the whole wavefront (a scalar load, issued a step
-    // ahead; no per-lane reciprocal); j <= k_max + 1 <= 111.
-    float inv_next = kInvInt.v[1];
-    for (int j = 1; u >= f && k <= k_max; ++j) {
-      const float inv_j = inv_next;
-      inv_next = kInvInt.v[j + 1];
-      u -= f;
-      k = static_cast<float>(j);
-      f *= g * inv_j - r;
+    bool act = u >= f;
+    for (int j = 1; __any(act); j += 4) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float inv_j = kInvInt.v[j + i];
+        const float u_next = u - f;
+        const float f_next = f * (g * inv_j - r);
+        const float k_next = static_cast<float>(j + i);
+        u = act ? u_next : u;
+        f = act ? f_next : f;
+        k = act ? k_next : k;
+        act = act && u >= f && k <= k_max;
+      }
     }
     if (k <= k_max) {
       return k;
