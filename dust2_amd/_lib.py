@@ -100,7 +100,8 @@ _PROTOS = {
     "dust2_unfilter_sir_free": (None, [VP]),
     "dust2_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(VP), ERR, ERRLEN]),
     "dust2_server_submit": (C.c_int, [VP, C.c_int, D, C.c_size_t, C.c_double, D, C.c_size_t,
-                                      C.c_double, D, C.c_uint64, U64, ERR, ERRLEN]),
+                                      C.c_double, D, C.c_uint64, C.c_int64, U64, ERR, ERRLEN]),
+    "dust2_amd_rng_jump": (C.c_int, [U64, C.c_size_t, C.c_int, ERR, ERRLEN]),
     "dust2_server_step": (C.c_int, [VP, SZ, SZ, ERR, ERRLEN]),
     "dust2_server_poll": (C.c_int, [VP, U64, D, C.POINTER(C.c_int), C.c_size_t, SZ, ERR,
                                     ERRLEN]),

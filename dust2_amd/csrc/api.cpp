@@ -19,6 +19,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -177,11 +178,17 @@ uint64_t host_next(host_rng& r) {
 constexpr uint64_t kJumpPoly[4] = {0x180ec6d33cfd0abaULL, 0xd5a61266f0c9392cULL,
                                    0xa9582618e03fc9aaULL, 0x39abdc4529b1661cULL};
 
-void host_jump(host_rng& r) {
+// The long-jump polynomial of xoshiro256 (2^192 draws; Blackman & Vigna's
+// published constants): the per-chain start of monty's seeding
+// (R/monty.R:29-36, vignettes/design.Rmd:81).
+constexpr uint64_t kLongJumpPoly[4] = {0x76e15d3efefdcbbfULL, 0xc5004e441c522fb3ULL,
+                                       0x77710069854ee241ULL, 0x39109bb02acbe635ULL};
+
+void host_apply_poly(host_rng& r, const uint64_t* poly) {
   uint64_t acc[4] = {0, 0, 0, 0};
   for (int w = 0; w < 4; ++w) {
     for (int b = 0; b < 64; ++b) {
-      if (kJumpPoly[w] & (uint64_t(1) << b)) {
+      if (poly[w] & (uint64_t(1) << b)) {
         for (int i = 0; i < 4; ++i) acc[i] ^= r.s[i];
       }
       host_next(r);
@@ -189,6 +196,9 @@ void host_jump(host_rng& r) {
   }
   std::memcpy(r.s, acc, sizeof(acc));
 }
+
+void host_jump(host_rng& r) { host_apply_poly(r, kJumpPoly); }
+void host_long_jump(host_rng& r) { host_apply_poly(r, kLongJumpPoly); }
 
 // splitmix64-based expansion of an integer seed into the first stream.
 void host_seed_words(uint64_t seed, uint64_t* out) {
@@ -276,12 +286,9 @@ void expand_streams(const uint64_t* seed_state, size_t n_seed, uint64_t seed, si
     host_seed_words(seed, s0);
   }
   if (n_copy < n_streams) {
-    dbuf d_s0(32);
-    hip_check(hipMemcpyAsync(d_s0.get(), s0, 32, hipMemcpyHostToDevice, s), "copy s0");
-    hip_check(launch_seed_streams(d_s0.get<uint64_t>(), tables_for_current_device().jump.get<uint64_t>(),
-                                  kJumpMats, n_streams - n_copy, d_aos + 4 * n_copy, s),
+    hip_check(launch_seed_streams(s0, tables_for_current_device().jump.get<uint64_t>(),
+                                  kJumpMats, n_streams - n_copy, d_aos + 4 * n_copy, 0, s),
               "seed streams");
-    hip_check(hipStreamSynchronize(s), "seed sync");
   }
 }
 
@@ -4379,12 +4386,13 @@ void dust2_unfilter_sir_free(dust2_unfilter_sir* h) {
 // ============================================================================
 namespace {
 
-// First-fit allocator over one device allocation (256-byte granules), so
-// admitting and retiring requests never calls hipMalloc/hipFree.
+// First-fit allocator over one block of memory (256-byte granules): the
+// server's device HBM for resident requests and its pinned host staging for
+// their uploads, so admitting and retiring requests never allocates.
 class arena {
  public:
-  void init(size_t bytes) {
-    mem_.alloc(bytes);
+  void init(char* base, size_t bytes) {
+    base_ = base;
     cap_ = bytes;
     free_.clear();
     free_[0] = bytes;
@@ -4421,15 +4429,43 @@ class arena {
       }
     }
   }
-  char* base() const { return mem_.get<char>(); }
+  char* base() const { return base_; }
   size_t used() const { return used_; }
   size_t capacity() const { return cap_; }
 
  private:
-  dbuf mem_;
+  char* base_ = nullptr;
   size_t cap_ = 0;
   size_t used_ = 0;
   std::map<size_t, size_t> free_;
+};
+
+// Pinned host memory (hipHostMalloc), owned.
+class pinned {
+ public:
+  pinned() = default;
+  pinned(const pinned&) = delete;
+  pinned& operator=(const pinned&) = delete;
+  ~pinned() { release(); }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) hip_check(hipHostMalloc(&p_, bytes, hipHostMallocDefault), "hipHostMalloc");
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename T = void>
+  T* get() const {
+    return static_cast<T*>(p_);
+  }
+  size_t bytes() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
 };
 
 struct srv_request {
@@ -4439,6 +4475,7 @@ struct srv_request {
   double time_start = 0, dt = 1;
   std::vector<double> pars, times, data;
   uint64_t seed = 0;
+  int64_t chain = -1;
 };
 
 struct srv_result {
@@ -4448,15 +4485,16 @@ struct srv_result {
   std::string message;
 };
 
-// Sub-buffer layout of one resident request inside its arena block.
+// Sub-buffer layout of one resident request inside its arena block; the
+// region from `upload` to `total` is filled by one copy from pinned staging.
 struct srv_layout {
-  size_t state_a, state_b, rng, lambda, anc, errors, ll, counters, sched_time, sched_steps,
-      sched_zero, obs, lfact, kpars, total;
+  size_t state_a, state_b, rng, lambda, anc, errors, ll, counters, upload, sched_time,
+      sched_steps, sched_zero, obs, lfact, kpars, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-srv_layout srv_plan(size_t S, size_t P, size_t T, size_t n_data, size_t kpars) {
+srv_layout srv_plan(size_t S, size_t P, size_t T, size_t n_data, size_t kpars, bool own_rng) {
   srv_layout L{};
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -4466,12 +4504,13 @@ srv_layout srv_plan(size_t S, size_t P, size_t T, size_t n_data, size_t kpars) {
   };
   L.state_a = take(S * P * 4);
   L.state_b = take(S * P * 4);
-  L.rng = take(4 * (P + 2) * 8);
+  L.rng = own_rng ? take(4 * (P + 2) * 8) : 0;
   L.lambda = take(P * 8);
   L.anc = take(P * 4);
   L.errors = take(P * sizeof(error_record));
   L.ll = take(8);
   L.counters = take(8);
+  L.upload = o;
   L.sched_time = take(T * 8);
   L.sched_steps = take(T * 4);
   L.sched_zero = take(T * 8);
@@ -4484,34 +4523,82 @@ srv_layout srv_plan(size_t S, size_t P, size_t T, size_t n_data, size_t kpars) {
 
 struct srv_slot {
   srv_request req;
-  size_t off = 0;
+  size_t off = 0;        // device arena block
+  size_t stage_off = 0;  // pinned staging block
   srv_layout L{};
   size_t T = 0;
-  size_t done = 0;
+  size_t done = 0;       // intervals run
   pipe_slot ps{};
+};
+
+// An MCMC chain's filter streams, kept resident between its requests so
+// that each request continues the chain's streams (a persistent filter per
+// chain, as dust_likelihood_monty keeps one per chain, R/monty.R:153-204).
+// The streams start at long_jump^chain of the seed (R/monty.R:29-36).
+struct srv_chain {
+  size_t P = 0;
+  size_t off = 0;
+  bool seeded = false;
+  bool busy = false;
+};
+
+// The resident requests of one (model, particle count): one stream, one
+// device slot table of fixed capacity (unused entries do nothing), a HIP
+// graph of chunk x (advance, resample) replayed per launch, and the
+// completion event of the launch in flight.
+struct srv_group {
+  int model = 0;
+  size_t P = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  bool in_flight = false;
+  std::vector<std::unique_ptr<srv_slot>> members;  // table order at the next launch
+  std::vector<srv_slot*> launched;                 // members of the launch in flight
+  size_t cap = 0;
+  dbuf d_table, d_status;
+  pinned h_table, h_status;
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+  ~srv_group() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (ev) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Deterministic (ODE) requests: every queued one in one launch, each with
+// its own series; inputs packed into pinned staging, results copied back
+// into pinned memory, completion seen through an event.
+struct srv_ode {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  bool in_flight = false;
+  std::vector<srv_request> launched;
+  pinned h_in, h_out;
+  dbuf d_in, d_out;
+  ~srv_ode() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 }  // namespace
 
 struct dust2_server {
   uint32_t tag = TAG_SERVER;
-  static constexpr int kStreams = 4;
-  hipStream_t streams[kStreams] = {};
-  arena mem;
+  dbuf d_mem;
+  pinned h_mem;
+  arena mem;    // device
+  arena stage;  // pinned host
   int chunk = 8;
   uint64_t next_id = 1;
   std::deque<srv_request> queue;
-  std::vector<std::unique_ptr<srv_slot>> active;
-  std::vector<srv_request> ode_queue;
+  std::deque<srv_request> ode_queue;
+  std::map<std::pair<int, size_t>, std::unique_ptr<srv_group>> groups;
+  std::map<int64_t, srv_chain> chains;
+  srv_ode ode;
   std::vector<srv_result> results;
-  dbuf d_tables, d_status, d_ode_pars, d_ode_series, d_ode_lfact, d_ode_times, d_ode_off, d_ode_len,
-      d_ode_t0, d_ode_state, d_ode_ll, d_ode_errors, d_ode_counters;
-  size_t max_ode_per_tick = 65536;
-  ~dust2_server() {
-    for (auto s : streams) {
-      if (s) (void)hipStreamDestroy(s);
-    }
-  }
+  size_t max_ode_per_launch = 65536;
 };
 
 namespace {
@@ -4527,23 +4614,60 @@ size_t model_n_data(int model) { return model == kServerOde ? 1 : discrete_model
 size_t model_kpars(int model) { return discrete_model(model).kpars; }
 double model_zero_period(int model) { return discrete_model(model).zero_period; }
 
-// Place a request in the arena and initialise its resident buffers.
-bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
+srv_group& srv_group_of(dust2_server* sv, int model, size_t P) {
+  auto& g = sv->groups[{model, P}];
+  if (!g) {
+    g = std::make_unique<srv_group>();
+    g->model = model;
+    g->P = P;
+    hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&g->ev, hipEventDisableTiming), "hipEventCreate");
+  }
+  return *g;
+}
+
+// Place a request in the arena and queue the uploads and seeding of its
+// resident buffers on its group's stream: no allocation and no wait.
+bool srv_admit(dust2_server* sv, const srv_request& rq) {
   const size_t S = model_state(rq.model), P = rq.P, T = rq.times.size();
   const size_t nd = model_n_data(rq.model), kp = model_kpars(rq.model);
-  const srv_layout L = srv_plan(S, P, T, nd, kp);
-  size_t off = 0;
+  const bool own_rng = rq.chain < 0;
+  const srv_layout L = srv_plan(S, P, T, nd, kp, own_rng);
+  // a chain keeps one block of streams for its particle count
+  srv_chain* ch = nullptr;
+  if (!own_rng) {
+    ch = &sv->chains[rq.chain];
+    if (ch->P != P) {
+      if (ch->P != 0) sv->mem.release(ch->off, 4 * (ch->P + 2) * 8);
+      ch->P = 0;
+      ch->seeded = false;
+      if (!sv->mem.alloc(4 * (P + 2) * 8, ch->off)) return false;
+      ch->P = P;
+    }
+  }
+  size_t off = 0, soff = 0;
   if (!sv->mem.alloc(L.total, off)) return false;
+  const size_t up = L.total - L.upload;
+  if (!sv->stage.alloc(up, soff)) {
+    sv->mem.release(off, L.total);
+    return false;
+  }
   auto slot = std::make_unique<srv_slot>();
   slot->req = rq;
   slot->off = off;
+  slot->stage_off = soff;
   slot->L = L;
   slot->T = T;
   char* b = sv->mem.base() + off;
-  // schedule (dust2/discrete/system.hpp:61-92) and data
-  std::vector<double> st(T), lf(T * nd);
-  std::vector<uint32_t> ns(T);
-  std::vector<uint64_t> zm(T);
+  char* h = sv->stage.base() + soff;
+  // schedule (dust2/discrete/system.hpp:61-92), data and parameters into
+  // the staging block, laid out as the device block from `upload`
+  double* st = reinterpret_cast<double*>(h + (L.sched_time - L.upload));
+  uint32_t* ns = reinterpret_cast<uint32_t*>(h + (L.sched_steps - L.upload));
+  uint64_t* zm = reinterpret_cast<uint64_t*>(h + (L.sched_zero - L.upload));
+  double* ob = reinterpret_cast<double*>(h + (L.obs - L.upload));
+  double* lf = reinterpret_cast<double*>(h + (L.lfact - L.upload));
+  double* kpars = reinterpret_cast<double*>(h + (L.kpars - L.upload));
   double tcur = rq.time_start;
   const double period = model_zero_period(rq.model);
   for (size_t t = 0; t < T; ++t) {
@@ -4558,28 +4682,30 @@ bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
     tcur = tcur + n * rq.dt;
   }
   for (size_t i = 0; i < T * nd; ++i) {
+    ob[i] = rq.data[i];
     lf[i] = std::isnan(rq.data[i]) ? 0.0 : std::lgamma(rq.data[i] + 1);
   }
-  std::vector<double> kpars(kp, 0.0);
-  discrete_model(rq.model).kernel_pars(rq.pars.data(), rq.dt, kpars.data());
-  auto up = [&](size_t at, const void* src, size_t bytes) {
-    if (bytes) hip_check(hipMemcpyAsync(b + at, src, bytes, hipMemcpyHostToDevice, s), "upload");
-  };
-  up(L.sched_time, st.data(), T * 8);
-  up(L.sched_steps, ns.data(), T * 4);
-  up(L.sched_zero, zm.data(), T * 8);
-  up(L.obs, rq.data.data(), T * nd * 8);
-  up(L.lfact, lf.data(), T * nd * 8);
-  up(L.kpars, kpars.data(), kp * 8);
+  std::fill(kpars, kpars + kp, 0.0);
+  discrete_model(rq.model).kernel_pars(rq.pars.data(), rq.dt, kpars);
+  srv_group& g = srv_group_of(sv, rq.model, P);
+  hipStream_t s = g.stream;
+  hip_check(hipMemcpyAsync(b + L.upload, h, up, hipMemcpyHostToDevice, s), "upload");
   hip_check(hipMemsetAsync(b + L.ll, 0, 8, s), "memset");
   hip_check(hipMemsetAsync(b + L.counters, 0, 8, s), "memset");
-  {
-    dbuf tmp((P + 2) * 32);
-    expand_streams(nullptr, 0, rq.seed, P + 2, tmp.get<uint64_t>(), s);
-    hip_check(launch_rng_aos_to_soa(tmp.get<uint64_t>(), reinterpret_cast<uint64_t*>(b + L.rng),
-                                    1, static_cast<int>(P + 2), static_cast<int>(P + 2), s),
-              "rng layout");
-    hip_check(hipStreamSynchronize(s), "sync");
+  hip_check(hipMemsetAsync(b + L.errors, 0, P * sizeof(error_record), s), "memset");
+  uint64_t* rng = own_rng ? reinterpret_cast<uint64_t*>(b + L.rng)
+                          : reinterpret_cast<uint64_t*>(sv->mem.base() + ch->off);
+  if (own_rng || !ch->seeded) {
+    host_rng r;
+    host_seed_words(rq.seed, r.s);
+    for (int64_t c = 0; !own_rng && c < rq.chain; ++c) host_long_jump(r);
+    hip_check(launch_seed_streams(r.s, tables_for_current_device().jump.get<uint64_t>(),
+                                  kJumpMats, P + 2, rng, P + 2, s),
+              "seed streams");
+  }
+  if (ch) {
+    ch->seeded = true;
+    ch->busy = true;
   }
   pipe_slot& ps = slot->ps;
   ps.pars = reinterpret_cast<const double*>(b + L.kpars);
@@ -4595,99 +4721,239 @@ bool srv_admit(dust2_server* sv, const srv_request& rq, hipStream_t s) {
   ps.dt = rq.dt;
   ps.state_a = b + L.state_a;
   ps.state_b = b + L.state_b;
-  ps.rng = reinterpret_cast<uint64_t*>(b + L.rng);
+  ps.rng = rng;
   ps.lambda = reinterpret_cast<double*>(b + L.lambda);
   ps.ancestors = reinterpret_cast<uint32_t*>(b + L.anc);
   ps.ll = reinterpret_cast<double*>(b + L.ll);
-  ps.ancestry = nullptr;
-  ps.traj = nullptr;
   ps.errors = reinterpret_cast<error_record*>(b + L.errors);
   ps.error_count = reinterpret_cast<uint32_t*>(b + L.counters);
   ps.n_fallback = reinterpret_cast<uint32_t*>(b + L.counters) + 1;
   ps.t_offset = 0;
   ps.n_time = static_cast<uint32_t>(T);
-  hip_check(hipMemsetAsync(b + L.errors, 0, P * sizeof(error_record), s), "memset");
-  sv->active.push_back(std::move(slot));
+  g.members.push_back(std::move(slot));
   return true;
 }
 
-// All queued deterministic (ODE) requests in one launch, each with its own
-// series (one thread per request).
-void srv_run_ode(dust2_server* sv, hipStream_t s) {
-  if (sv->ode_queue.empty()) return;
-  const size_t n = std::min(sv->ode_queue.size(), sv->max_ode_per_tick);
-  std::vector<double> pars(n * kSirodePars), times, obs, lf;
-  std::vector<int64_t> off(n);
-  std::vector<int32_t> len(n);
-  std::vector<double> t0(n);
-  for (size_t i = 0; i < n; ++i) {
-    const auto& rq = sv->ode_queue[i];
-    t0[i] = rq.time_start;
-    std::copy(rq.pars.begin(), rq.pars.end(), pars.begin() + i * kSirodePars);
-    off[i] = static_cast<int64_t>(times.size());
-    len[i] = static_cast<int32_t>(rq.times.size());
-    times.insert(times.end(), rq.times.begin(), rq.times.end());
-    obs.insert(obs.end(), rq.data.begin(), rq.data.end());
-    for (double x : rq.data) lf.push_back(std::isnan(x) ? 0.0 : std::lgamma(x + 1));
+// Launch the group's next chunk of intervals for every member.
+void srv_launch(dust2_server* sv, srv_group& g) {
+  const size_t n = g.members.size();
+  if (n > g.cap) {
+    size_t cap = std::max<size_t>(16, g.cap);
+    while (cap < n) cap *= 2;
+    g.d_table.alloc(cap * sizeof(pipe_slot));
+    g.h_table.alloc(cap * sizeof(pipe_slot));
+    g.d_status.alloc(2 * cap * sizeof(double));
+    g.h_status.alloc(2 * cap * sizeof(double));
+    g.cap = cap;
+    if (g.graph) {
+      hip_check(hipGraphExecDestroy(g.graph), "hipGraphExecDestroy");
+      g.graph = nullptr;
+    }
   }
-  auto fit = [&](dbuf& d, size_t bytes) {
-    if (d.bytes() < bytes) d.alloc(std::max<size_t>(bytes, 256));
-  };
-  fit(sv->d_ode_pars, pars.size() * 8);
-  fit(sv->d_ode_times, times.size() * 8);
-  fit(sv->d_ode_series, obs.size() * 8);
-  fit(sv->d_ode_lfact, lf.size() * 8);
-  fit(sv->d_ode_off, n * 8);
-  fit(sv->d_ode_len, n * 4);
-  fit(sv->d_ode_t0, n * 8);
-  fit(sv->d_ode_state, n * 5 * 8);
-  fit(sv->d_ode_ll, n * 8);
-  fit(sv->d_ode_errors, n * sizeof(error_record));
-  fit(sv->d_ode_counters, 16);
-  to_device(sv->d_ode_pars, pars.data(), pars.size() * 8, s);
-  to_device(sv->d_ode_times, times.data(), times.size() * 8, s);
-  to_device(sv->d_ode_series, obs.data(), obs.size() * 8, s);
-  to_device(sv->d_ode_lfact, lf.data(), lf.size() * 8, s);
-  to_device(sv->d_ode_off, off.data(), n * 8, s);
-  to_device(sv->d_ode_len, len.data(), n * 4, s);
-  to_device(sv->d_ode_t0, t0.data(), n * 8, s);
-  hip_check(hipMemsetAsync(sv->d_ode_errors.get(), 0, n * sizeof(error_record), s), "memset");
-  hip_check(hipMemsetAsync(sv->d_ode_counters.get(), 0, 16, s), "memset");
+  const int P = static_cast<int>(g.P);
+  const int cap = static_cast<int>(g.cap);
+  if (!g.graph || g.graph_chunk != sv->chunk) {
+    if (g.graph) hip_check(hipGraphExecDestroy(g.graph), "hipGraphExecDestroy");
+    g.graph = nullptr;
+    hipGraph_t graph;
+    hip_check(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (int k = 0; k < sv->chunk; ++k) {
+      hip_check(launch_discrete_advance(g.model, false, g.d_table.get<pipe_slot>(), cap, P, k, 0,
+                                        0, g.stream),
+                "advance kernel");
+      hip_check(launch_pipe_resample(g.d_table.get<pipe_slot>(), cap, P, k, g.stream),
+                "resample kernel");
+    }
+    hip_check(hipStreamEndCapture(g.stream, &graph), "end capture");
+    hip_check(hipGraphInstantiate(&g.graph, graph, nullptr, nullptr, 0), "graph instantiate");
+    hip_check(hipGraphDestroy(graph), "hipGraphDestroy");
+    g.graph_chunk = sv->chunk;
+  }
+  pipe_slot* tab = g.h_table.get<pipe_slot>();
+  g.launched.clear();
+  for (size_t i = 0; i < n; ++i) {
+    srv_slot* sl = g.members[i].get();
+    tab[i] = sl->ps;
+    tab[i].t_offset = static_cast<int32_t>(sl->done);
+    g.launched.push_back(sl);
+  }
+  for (size_t i = n; i < g.cap; ++i) {
+    // unused entries: no particles and no intervals, so every kernel leaves
+    tab[i] = pipe_slot{};
+    tab[i].flags = SLOT_AUTO_MODE;
+  }
+  hip_check(hipMemcpyAsync(g.d_table.get(), tab, g.cap * sizeof(pipe_slot),
+                           hipMemcpyHostToDevice, g.stream),
+            "upload slot table");
+  hip_check(hipGraphLaunch(g.graph, g.stream), "hipGraphLaunch");
+  hip_check(launch_slot_status(g.d_table.get<pipe_slot>(), static_cast<int>(n),
+                               g.d_status.get<double>(), g.stream),
+            "status kernel");
+  hip_check(hipMemcpyAsync(g.h_status.get(), g.d_status.get(), 2 * n * sizeof(double),
+                           hipMemcpyDeviceToHost, g.stream),
+            "status copy");
+  hip_check(hipEventRecord(g.ev, g.stream), "hipEventRecord");
+  g.in_flight = true;
+}
+
+// Results of a completed launch: finished requests leave their slots.
+void srv_complete(dust2_server* sv, srv_group& g) {
+  const double* status = g.h_status.get<double>();
+  std::vector<srv_slot*> finished;
+  for (size_t i = 0; i < g.launched.size(); ++i) {
+    srv_slot* sl = g.launched[i];
+    sl->done += static_cast<size_t>(sv->chunk);
+    const uint32_t count = static_cast<uint32_t>(status[2 * i + 1]);
+    if (sl->done < sl->T && count == 0) continue;
+    srv_result r{sl->req.id, status[2 * i], 0, std::string()};
+    if (count > 0) {
+      std::vector<error_record> rec(sl->req.P);
+      hip_check(hipMemcpy(rec.data(), sv->mem.base() + sl->off + sl->L.errors,
+                          rec.size() * sizeof(error_record), hipMemcpyDeviceToHost),
+                "copy");
+      r.status = 1;
+      r.message = error_report(rec, count);
+    }
+    sv->results.push_back(std::move(r));
+    finished.push_back(sl);
+  }
+  g.launched.clear();
+  g.in_flight = false;
+  if (finished.empty()) return;
+  std::vector<std::unique_ptr<srv_slot>> keep;
+  for (auto& sl : g.members) {
+    if (std::find(finished.begin(), finished.end(), sl.get()) == finished.end()) {
+      keep.push_back(std::move(sl));
+      continue;
+    }
+    sv->mem.release(sl->off, sl->L.total);
+    sv->stage.release(sl->stage_off, sl->L.total - sl->L.upload);
+    if (sl->req.chain >= 0) sv->chains[sl->req.chain].busy = false;
+  }
+  g.members.swap(keep);
+}
+
+// All queued deterministic (ODE) requests in one launch, each with its own
+// series and start time (one thread per request).
+void srv_launch_ode(dust2_server* sv) {
+  srv_ode& o = sv->ode;
+  const size_t n = std::min(sv->ode_queue.size(), sv->max_ode_per_launch);
+  size_t n_t = 0;
+  for (size_t i = 0; i < n; ++i) n_t += sv->ode_queue[i].times.size();
+  // packed inputs: pars [n][4], t0 [n], off [n], len [n], times, obs, lfact
+  const size_t b_pars = align256(n * kSirodePars * 8), b_t0 = align256(n * 8),
+               b_off = align256(n * 8), b_len = align256(n * 4), b_t = align256(n_t * 8);
+  const size_t in_bytes = b_pars + b_t0 + b_off + b_len + 3 * b_t;
+  // results: ll [n], errors [n], state [5][n], counters
+  const size_t b_ll = align256(n * 8), b_err = align256(n * sizeof(error_record)),
+               b_state = align256(n * 5 * 8);
+  const size_t out_bytes = b_ll + b_err + b_state + 256;
+  if (o.h_in.bytes() < in_bytes) o.h_in.alloc(std::max(in_bytes, 2 * o.h_in.bytes()));
+  if (o.d_in.bytes() < in_bytes) o.d_in.alloc(o.h_in.bytes());
+  if (o.h_out.bytes() < out_bytes) o.h_out.alloc(std::max(out_bytes, 2 * o.h_out.bytes()));
+  if (o.d_out.bytes() < out_bytes) o.d_out.alloc(o.h_out.bytes());
+  char* h = o.h_in.get<char>();
+  double* pars = reinterpret_cast<double*>(h);
+  double* t0 = reinterpret_cast<double*>(h + b_pars);
+  int64_t* off = reinterpret_cast<int64_t*>(h + b_pars + b_t0);
+  int32_t* len = reinterpret_cast<int32_t*>(h + b_pars + b_t0 + b_off);
+  double* times = reinterpret_cast<double*>(h + b_pars + b_t0 + b_off + b_len);
+  double* obs = times + b_t / 8;
+  double* lf = obs + b_t / 8;
+  o.launched.clear();
+  size_t at = 0;
+  for (size_t i = 0; i < n; ++i) {
+    srv_request& rq = sv->ode_queue.front();
+    std::copy(rq.pars.begin(), rq.pars.end(), pars + i * kSirodePars);
+    t0[i] = rq.time_start;
+    off[i] = static_cast<int64_t>(at);
+    len[i] = static_cast<int32_t>(rq.times.size());
+    for (size_t k = 0; k < rq.times.size(); ++k, ++at) {
+      times[at] = rq.times[k];
+      obs[at] = rq.data[k];
+      lf[at] = std::isnan(rq.data[k]) ? 0.0 : std::lgamma(rq.data[k] + 1);
+    }
+    o.launched.push_back(std::move(rq));
+    sv->ode_queue.pop_front();
+  }
+  char* d = o.d_in.get<char>();
+  char* dout = o.d_out.get<char>();
+  hipStream_t s = o.stream;
+  hip_check(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s), "upload");
+  hip_check(hipMemsetAsync(dout + b_ll, 0, b_err, s), "memset");
+  hip_check(hipMemsetAsync(dout + b_ll + b_err + b_state, 0, 256, s), "memset");
   ode_launch a{};
   a.sens_param = -1;
-  a.pars = sv->d_ode_pars.get<double>();
+  a.pars = reinterpret_cast<const double*>(d);
   a.n_particles = 1;
   a.n_total = static_cast<int>(n);
   a.time_start = 0;
-  a.series_time_start = sv->d_ode_t0.get<double>();
-  a.times = sv->d_ode_times.get<double>();
-  a.obs = sv->d_ode_series.get<double>();
-  a.lfact = sv->d_ode_lfact.get<double>();
+  a.series_time_start = reinterpret_cast<const double*>(d + b_pars);
+  a.series_offset = reinterpret_cast<const int64_t*>(d + b_pars + b_t0);
+  a.series_len = reinterpret_cast<const int32_t*>(d + b_pars + b_t0 + b_off);
+  a.times = reinterpret_cast<const double*>(d + b_pars + b_t0 + b_off + b_len);
+  a.obs = a.times + b_t / 8;
+  a.lfact = a.obs + b_t / 8;
   a.control = ode_control_dev{10000, 1e-6, 1e-6, std::numeric_limits<double>::epsilon(),
                               std::numeric_limits<double>::infinity(), 0.9, 0.2, 10.0, 0.04,
                               0.2 - 0.04 * 0.75};
-  a.state = sv->d_ode_state.get();
-  a.ll = sv->d_ode_ll.get<double>();
-  a.errors = sv->d_ode_errors.get<error_record>();
-  a.error_count = sv->d_ode_counters.get<uint32_t>();
-  a.series_offset = sv->d_ode_off.get<int64_t>();
-  a.series_len = sv->d_ode_len.get<int32_t>();
+  a.ll = reinterpret_cast<double*>(dout);
+  a.errors = reinterpret_cast<error_record*>(dout + b_ll);
+  a.state = dout + b_ll + b_err;
+  a.error_count = reinterpret_cast<uint32_t*>(dout + b_ll + b_err + b_state);
   hip_check(launch_dp5_unfilter(a, true, s), "dp5 serving kernel");
-  std::vector<double> ll(n);
-  std::vector<error_record> rec(n);
-  to_host(ll.data(), sv->d_ode_ll, n * 8, s);
-  to_host(rec.data(), sv->d_ode_errors, n * sizeof(error_record), s);
+  hip_check(hipMemcpyAsync(o.h_out.get(), dout, b_ll + b_err, hipMemcpyDeviceToHost, s),
+            "result copy");
+  hip_check(hipEventRecord(o.ev, s), "hipEventRecord");
+  o.in_flight = true;
+}
+
+void srv_complete_ode(dust2_server* sv) {
+  srv_ode& o = sv->ode;
+  const size_t n = o.launched.size();
+  const double* ll = o.h_out.get<double>();
+  const error_record* rec =
+      reinterpret_cast<const error_record*>(o.h_out.get<char>() + align256(n * 8));
   for (size_t i = 0; i < n; ++i) {
-    srv_result r{sv->ode_queue[i].id, ll[i], rec[i].code ? 1 : 0,
-                 rec[i].code ? error_text(rec[i]) : std::string()};
-    sv->results.push_back(r);
+    sv->results.push_back(srv_result{o.launched[i].id, ll[i], rec[i].code ? 1 : 0,
+                                     rec[i].code ? error_text(rec[i]) : std::string()});
   }
-  sv->ode_queue.erase(sv->ode_queue.begin(), sv->ode_queue.begin() + n);
+  o.launched.clear();
+  o.in_flight = false;
+}
+
+// true when the launch behind `ev` has completed
+bool event_done(hipEvent_t ev) {
+  const hipError_t e = hipEventQuery(ev);
+  if (e == hipSuccess) return true;
+  if (e == hipErrorNotReady) return false;
+  hip_check(e, "hipEventQuery");
+  return false;
 }
 
 }  // namespace
 
 extern "C" {
+
+int dust2_amd_rng_jump(uint64_t* state, size_t n_streams, int long_jump, char* err,
+                       size_t errlen) {
+  try {
+    if (state == nullptr && n_streams > 0) throw std::runtime_error("'state' is NULL");
+    for (size_t i = 0; i < n_streams; ++i) {
+      host_rng r;
+      std::memcpy(r.s, state + 4 * i, 32);
+      if (long_jump) {
+        host_long_jump(r);
+      } else {
+        host_jump(r);
+      }
+      std::memcpy(state + 4 * i, r.s, 32);
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
 
 int dust2_server_create(size_t hbm_bytes, int chunk_intervals, dust2_server** out, char* err,
                         size_t errlen) {
@@ -4696,10 +4962,15 @@ int dust2_server_create(size_t hbm_bytes, int chunk_intervals, dust2_server** ou
     if (hbm_bytes < (size_t(1) << 20)) throw std::runtime_error("'hbm_bytes' is too small");
     auto sv = std::make_unique<dust2_server>();
     sv->chunk = chunk_intervals;
-    for (auto& s : sv->streams) {
-      hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
-    }
-    sv->mem.init(hbm_bytes);
+    sv->d_mem.alloc(hbm_bytes);
+    sv->mem.init(sv->d_mem.get<char>(), hbm_bytes);
+    // uploads are a small fraction of a request's device footprint
+    const size_t stage_bytes = std::max<size_t>(size_t(16) << 20, hbm_bytes / 8);
+    sv->h_mem.alloc(stage_bytes);
+    sv->stage.init(sv->h_mem.get<char>(), stage_bytes);
+    hip_check(hipStreamCreateWithFlags(&sv->ode.stream, hipStreamNonBlocking),
+              "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&sv->ode.ev, hipEventDisableTiming), "hipEventCreate");
     tables_for_current_device();
     *out = sv.release();
     return 0;
@@ -4710,8 +4981,8 @@ int dust2_server_create(size_t hbm_bytes, int chunk_intervals, dust2_server** ou
 
 int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t n_particles,
                         double time_start, const double* time, size_t n_time, double dt,
-                        const double* data, uint64_t seed, uint64_t* id_out, char* err,
-                        size_t errlen) {
+                        const double* data, uint64_t seed, int64_t chain, uint64_t* id_out,
+                        char* err, size_t errlen) {
   try {
     check_handle(sv, TAG_SERVER, "server");
     if (model < 0 || model > 2) throw std::runtime_error("unknown model");
@@ -4723,6 +4994,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
     rq.time_start = time_start;
     rq.dt = dt;
     rq.seed = seed;
+    rq.chain = model == kServerOde ? -1 : chain;
     const size_t np = model_n_pars(model);
     rq.pars.assign(pars, pars + np);
     rq.times.assign(time, time + n_time);
@@ -4736,6 +5008,7 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
       if (pipe_resample_ppt(static_cast<int>(n_particles)) == 0) {
         throw std::runtime_error("'n_particles' too large");
       }
+      if (chain > (int64_t(1) << 20)) throw std::runtime_error("'chain' is too large");
       discrete_model(model).check(pars);
       sv->queue.push_back(std::move(rq));
     }
@@ -4746,104 +5019,79 @@ int dust2_server_submit(dust2_server* sv, int model, const double* pars, size_t 
   }
 }
 
-// One scheduling tick: admit queued requests while their footprint fits,
-// advance every resident request by up to chunk_intervals data intervals
-// (requests of the same model and particle count share each launch;
-// distinct launch groups run on separate streams), run the queued ODE
-// requests, and retire finished requests.
+// One scheduling pass: collect the launches that have completed (their
+// finished requests become results and free their memory), admit queued
+// requests while their footprint fits, and launch the next chunk of every
+// idle group and the queued ODE requests.  Groups progress independently,
+// each on its own stream; the call waits only when nothing else could be
+// done while work is in flight.
 int dust2_server_step(dust2_server* sv, size_t* n_active, size_t* n_queued, char* err,
                       size_t errlen) {
   try {
     check_handle(sv, TAG_SERVER, "server");
-    hipStream_t s0 = sv->streams[0];
-    while (!sv->queue.empty()) {
-      if (!srv_admit(sv, sv->queue.front(), s0)) {
-        if (sv->active.empty()) {
-          throw std::runtime_error("request does not fit the server's memory budget");
-        }
-        break;
-      }
-      sv->queue.pop_front();
-    }
-    hip_check(hipStreamSynchronize(s0), "sync");
-    // launch groups: (model, particle count)
-    std::map<std::pair<int, size_t>, std::vector<srv_slot*>> groups;
-    for (auto& sl : sv->active) groups[{sl->req.model, sl->req.P}].push_back(sl.get());
-    std::vector<pipe_slot> table;
-    std::vector<std::tuple<int, size_t, size_t, size_t>> spans;  // model, P, first, count
-    for (auto& kv : groups) {
-      const size_t first = table.size();
-      for (auto* sl : kv.second) {
-        sl->ps.t_offset = static_cast<int32_t>(sl->done);
-        table.push_back(sl->ps);
-      }
-      spans.emplace_back(kv.first.first, kv.first.second, first, kv.second.size());
-    }
-    if (!table.empty()) {
-      if (sv->d_tables.bytes() < table.size() * sizeof(pipe_slot)) {
-        sv->d_tables.alloc(std::max<size_t>(table.size() * 2, 64) * sizeof(pipe_slot));
-      }
-      to_device(sv->d_tables, table.data(), table.size() * sizeof(pipe_slot), s0);
-      hip_check(hipStreamSynchronize(s0), "sync");
-      const pipe_slot* base = sv->d_tables.get<pipe_slot>();
-      int si = 0;
-      for (const auto& sp : spans) {
-        const int model = std::get<0>(sp);
-        const int P = static_cast<int>(std::get<1>(sp));
-        const pipe_slot* tab = base + std::get<2>(sp);
-        const int n = static_cast<int>(std::get<3>(sp));
-        hipStream_t s = sv->streams[(si++) % dust2_server::kStreams];
-        for (int k = 0; k < sv->chunk; ++k) {
-          hip_check(launch_discrete_advance(model, false, tab, n, P, k, 0, 0, s),
-                    "advance kernel");
-          hip_check(launch_pipe_resample(tab, n, P, k, s), "resample");
+    for (int pass = 0; pass < 2; ++pass) {
+      bool progress = false;
+      for (auto& kv : sv->groups) {
+        srv_group& g = *kv.second;
+        if (g.in_flight && event_done(g.ev)) {
+          srv_complete(sv, g);
+          progress = true;
         }
       }
-    }
-    srv_run_ode(sv, sv->streams[dust2_server::kStreams - 1]);
-    for (auto s : sv->streams) hip_check(hipStreamSynchronize(s), "sync");
-    // status of every resident request in one copy (the table is in the
-    // order of `groups`)
-    std::vector<double> status(2 * table.size());
-    if (!table.empty()) {
-      if (sv->d_status.bytes() < status.size() * 8) sv->d_status.alloc(status.size() * 16);
-      hip_check(launch_slot_status(sv->d_tables.get<pipe_slot>(), static_cast<int>(table.size()),
-                                   sv->d_status.get<double>(), s0),
-                "status kernel");
-      to_host(status.data(), sv->d_status, status.size() * 8, s0);
-    }
-    std::map<srv_slot*, size_t> pos;
-    {
-      size_t i = 0;
-      for (auto& kv : groups) {
-        for (auto* sl : kv.second) pos[sl] = i++;
+      if (sv->ode.in_flight && event_done(sv->ode.ev)) {
+        srv_complete_ode(sv);
+        progress = true;
       }
-    }
-    // retire finished requests
-    std::vector<std::unique_ptr<srv_slot>> keep;
-    for (auto& sl : sv->active) {
-      sl->done += sv->chunk;
-      const size_t i = pos[sl.get()];
-      const uint32_t count = static_cast<uint32_t>(status[2 * i + 1]);
-      const char* b = sv->mem.base() + sl->off;
-      if (sl->done >= sl->T || count > 0) {
-        srv_result r{sl->req.id, status[2 * i], 0, std::string()};
-        if (count > 0) {
-          std::vector<error_record> rec(sl->req.P);
-          hip_check(hipMemcpy(rec.data(), b + sl->L.errors, rec.size() * sizeof(error_record),
-                              hipMemcpyDeviceToHost),
-                    "copy");
-          r.status = 1;
-          r.message = error_report(rec, count);
+      // admission in order of arrival; a request of a chain with a request
+      // resident waits for it; a request that does not fit stops admission
+      bool resident = false;
+      for (auto& kv : sv->groups) resident = resident || !kv.second->members.empty();
+      for (auto it = sv->queue.begin(); it != sv->queue.end();) {
+        if (it->chain >= 0) {
+          auto c = sv->chains.find(it->chain);
+          if (c != sv->chains.end() && c->second.busy) {
+            ++it;
+            continue;
+          }
         }
-        sv->results.push_back(std::move(r));
-        sv->mem.release(sl->off, sl->L.total);
-      } else {
-        keep.push_back(std::move(sl));
+        if (!srv_admit(sv, *it)) {
+          if (!resident) {
+            throw std::runtime_error("request does not fit the server's memory budget");
+          }
+          break;
+        }
+        resident = true;
+        progress = true;
+        it = sv->queue.erase(it);
+      }
+      for (auto& kv : sv->groups) {
+        srv_group& g = *kv.second;
+        if (!g.in_flight && !g.members.empty()) {
+          srv_launch(sv, g);
+          progress = true;
+        }
+      }
+      if (!sv->ode.in_flight && !sv->ode_queue.empty()) {
+        srv_launch_ode(sv);
+        progress = true;
+      }
+      if (progress || pass == 1) break;
+      // nothing to do: wait until any launch in flight completes (groups
+      // finish in any order, so no single event is waited on)
+      bool any = sv->ode.in_flight;
+      for (auto& kv : sv->groups) any = any || kv.second->in_flight;
+      if (!any) break;
+      for (bool done = false; !done;) {
+        done = sv->ode.in_flight && event_done(sv->ode.ev);
+        for (auto& kv : sv->groups) {
+          done = done || (kv.second->in_flight && event_done(kv.second->ev));
+        }
+        if (!done) std::this_thread::yield();
       }
     }
-    sv->active.swap(keep);
-    if (n_active) *n_active = sv->active.size();
+    size_t active = sv->ode.launched.size();
+    for (auto& kv : sv->groups) active += kv.second->members.size();
+    if (n_active) *n_active = active;
     if (n_queued) *n_queued = sv->queue.size() + sv->ode_queue.size();
     return 0;
   } catch (const std::exception& e) {
@@ -4891,7 +5139,13 @@ int dust2_server_memory(dust2_server* sv, size_t* used, size_t* capacity, char* 
 }
 
 void dust2_server_free(dust2_server* sv) {
-  if (handle_ok(sv, TAG_SERVER)) delete sv;
+  if (!handle_ok(sv, TAG_SERVER)) return;
+  // launches in flight read the server's memory: let them finish first
+  for (auto& kv : sv->groups) {
+    if (kv.second->in_flight) (void)hipEventSynchronize(kv.second->ev);
+  }
+  if (sv->ode.in_flight) (void)hipEventSynchronize(sv->ode.ev);
+  delete sv;
 }
 
 }  // extern "C"

@@ -353,10 +353,16 @@ hipError_t launch_reorder(const void* state_in, void* state_out, const uint32_t*
                           int S, int P, bool fp64, hipStream_t stream);
 
 // ---- RNG -------------------------------------------------------------------
-// out[n] (AoS, 4 words per stream) = jump^n(s0) for n < n_streams, using
-// the column-major GF(2) matrices J_i = jump^(2^i), i < n_mats.
-hipError_t launch_seed_streams(const uint64_t* d_s0, const uint64_t* d_mats, int n_mats,
-                               uint64_t n_streams, uint64_t* d_out, hipStream_t stream);
+// Stream n = jump^n(s0) for n < n_streams, using the column-major GF(2)
+// matrices J_i = jump^(2^i), i < n_mats; s0 is passed by value.  Written
+// AoS (4 words per stream) when soa_stride is 0, else SoA: word w of
+// stream n at out[w * soa_stride + n].
+struct rng_words {
+  uint64_t w[4];
+};
+hipError_t launch_seed_streams(const uint64_t s0[4], const uint64_t* d_mats, int n_mats,
+                               uint64_t n_streams, uint64_t* d_out, uint64_t soa_stride,
+                               hipStream_t stream);
 // AoS raw streams <-> per-group SoA [G][4][stride] at column offset `first`.
 hipError_t launch_rng_aos_to_soa(const uint64_t* aos, uint64_t* soa, int G, int per_group,
                                  int stride, hipStream_t stream);

@@ -33,11 +33,11 @@ __global__ void __launch_bounds__(256)
 // stream n = jump^n(s0): apply J_i (column-major GF(2) 256x256 matrix,
 // column j = J_i e_j as 4 words) for every set bit i of n.
 __global__ void __launch_bounds__(256)
-    seed_streams_kernel(const uint64_t* __restrict__ s0, const uint64_t* __restrict__ mats,
-                        int n_mats, uint64_t n_streams, uint64_t* __restrict__ out) {
+    seed_streams_kernel(rng_words s0, const uint64_t* __restrict__ mats, int n_mats,
+                        uint64_t n_streams, uint64_t* __restrict__ out, uint64_t soa_stride) {
   const uint64_t n = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= n_streams) return;
-  uint64_t s[4] = {s0[0], s0[1], s0[2], s0[3]};
+  uint64_t s[4] = {s0.w[0], s0.w[1], s0.w[2], s0.w[3]};
   for (int i = 0; i < n_mats; ++i) {
     if (((n >> i) & 1u) == 0) continue;
     const uint64_t* m = mats + static_cast<size_t>(i) * 256 * 4;
@@ -58,10 +58,17 @@ __global__ void __launch_bounds__(256)
     s[2] = acc[2];
     s[3] = acc[3];
   }
-  out[4 * n + 0] = s[0];
-  out[4 * n + 1] = s[1];
-  out[4 * n + 2] = s[2];
-  out[4 * n + 3] = s[3];
+  if (soa_stride == 0) {
+    out[4 * n + 0] = s[0];
+    out[4 * n + 1] = s[1];
+    out[4 * n + 2] = s[2];
+    out[4 * n + 3] = s[3];
+  } else {
+    out[n] = s[0];
+    out[soa_stride + n] = s[1];
+    out[2 * soa_stride + n] = s[2];
+    out[3 * soa_stride + n] = s[3];
+  }
 }
 
 // aos: [G][per_group][4]; soa: [G][4][stride], stride >= per_group
@@ -159,12 +166,14 @@ hipError_t launch_reorder(const void* state_in, void* state_out, const uint32_t*
   return hipGetLastError();
 }
 
-hipError_t launch_seed_streams(const uint64_t* d_s0, const uint64_t* d_mats, int n_mats,
-                               uint64_t n_streams, uint64_t* d_out, hipStream_t stream) {
+hipError_t launch_seed_streams(const uint64_t s0[4], const uint64_t* d_mats, int n_mats,
+                               uint64_t n_streams, uint64_t* d_out, uint64_t soa_stride,
+                               hipStream_t stream) {
   if (n_streams == 0) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((n_streams + 255) / 256);
-  hipLaunchKernelGGL(seed_streams_kernel, dim3(blocks), dim3(256), 0, stream, d_s0, d_mats,
-                     n_mats, n_streams, d_out);
+  const rng_words w{{s0[0], s0[1], s0[2], s0[3]}};
+  hipLaunchKernelGGL(seed_streams_kernel, dim3(blocks), dim3(256), 0, stream, w, d_mats,
+                     n_mats, n_streams, d_out, soa_stride);
   return hipGetLastError();
 }
 

@@ -1513,6 +1513,14 @@ def gpu_interpolate(kind, t, y, z):
     return out if series else out[:, 0]
 
 
+def rng_jump(state, long_jump=False):
+    """monty's jump() (2^128 draws) or long_jump() (2^192 draws) applied to
+    each 4-word xoshiro256+ state (R/monty.R:29-36)."""
+    st = np.array(state, dtype=np.uint64).reshape(-1, 4).copy()
+    call("dust2_amd_rng_jump", u64ptr(st), st.shape[0], int(bool(long_jump)))
+    return st
+
+
 def gpu_rng_streams(n_streams, seed=None, seed_state=None):
     out = np.zeros(n_streams * 4, dtype=np.uint64)
     raw = None if seed_state is None else np.ascontiguousarray(seed_state, dtype=np.uint64)

@@ -29,9 +29,12 @@ class Server:
             _lib.lib().dust2_server_free(self.ptr)
             self.ptr = None
 
-    def submit(self, generator, pars, n_particles, time_start, data, dt=1.0, seed=0):
+    def submit(self, generator, pars, n_particles, time_start, data, dt=1.0, seed=0, chain=None):
         """Queue one likelihood evaluation; returns its id.  data: dict with
-        'time' and 'incidence' ((n_time,) or (n_time, n_data))."""
+        'time' and 'incidence' ((n_time,) or (n_time, n_data)).  chain: None
+        for streams seeded from `seed`; an MCMC chain index (>= 0) for
+        streams that continue from the chain's previous request, the first
+        seeded at long_jump^chain of `seed` (R/monty.R:29-36)."""
         row = pars_matrix(generator, pars, 1)[0].copy()
         time = f64(data["time"]).reshape(-1)
         obs = np.ascontiguousarray(f64(data["incidence"]).reshape(time.size, -1))
@@ -40,17 +43,19 @@ class Server:
         rid = np.zeros(1, dtype=np.uint64)
         call("dust2_server_submit", self.ptr, MODELS[generator.name], dptr(row), int(n_particles),
              float(time_start), dptr(time), time.size, float(dt), dptr(obs),
-             C.c_uint64(int(seed)), _lib.u64ptr(rid))
+             C.c_uint64(int(seed)), C.c_int64(-1 if chain is None else int(chain)),
+             _lib.u64ptr(rid))
         return int(rid[0])
 
     def step(self):
-        """One scheduling tick; returns (n_resident, n_queued)."""
+        """One scheduling pass (collect completed launches, admit, launch);
+        returns (n_resident, n_queued)."""
         a = np.zeros(1, dtype=np.uintp)
         q = np.zeros(1, dtype=np.uintp)
         call("dust2_server_step", self.ptr, _lib.szptr(a), _lib.szptr(q))
         return int(a[0]), int(q[0])
 
-    def poll(self, max_results=100000):
+    def poll(self, max_results=1024):
         """Finished results as a list of (id, log_likelihood, status, message)."""
         ids = np.zeros(max_results, dtype=np.uint64)
         ll = np.zeros(max_results)
@@ -72,12 +77,16 @@ class Server:
         call("dust2_server_memory", self.ptr, _lib.szptr(used), _lib.szptr(cap))
         return int(used[0]), int(cap[0])
 
-    def drain(self, max_ticks=100000):
+    def drain(self, max_ticks=10**7):
         """Step until every submitted request has finished; returns results."""
         out = []
         for _ in range(max_ticks):
             n_active, n_queued = self.step()
             out.extend(self.poll())
             if n_active == 0 and n_queued == 0:
-                return out
+                while True:
+                    more = self.poll()
+                    if not more:
+                        return out
+                    out.extend(more)
         raise RuntimeError("server did not drain")

@@ -546,25 +546,36 @@ void dust2_unfilter_sir_free(dust2_unfilter_sir* h);
  * likelihood at a time (R/monty.R:188-204).  Each request is one parameter
  * group's complete filter (sir, seir16: FP32) or ODE unfilter (sirode: FP64)
  * run on its own series.  Requests are admitted when their device footprint
- * fits the server's HBM budget (a first-fit arena, no allocation per
- * request), stay resident between ticks, and every tick advances all
- * resident requests by chunk_intervals data intervals; requests of the same
- * model and particle count share launches, other launch groups run on
- * separate streams.  A request's result depends only on its own inputs and
- * seed: it is identical alone, in any mix and with the plain filter entry
- * points on one group. */
+ * fits the server's HBM budget (a first-fit arena over one device block,
+ * uploads staged through one pinned host block: nothing is allocated per
+ * request), stay resident, and advance chunk_intervals data intervals per
+ * launch.  Requests of the same model and particle count share launches
+ * (one HIP graph per group, replayed); each group runs on its own stream and
+ * progresses at its own pace, its completion seen through an event.  A
+ * request's result depends only on its own inputs and streams: it is
+ * identical alone, in any mix and with the plain filter entry points on one
+ * group. */
 typedef struct dust2_server dust2_server;
 #define DUST2_MODEL_SIR 0
 #define DUST2_MODEL_SEIR16 1
 #define DUST2_MODEL_SIRODE 2
 int dust2_server_create(size_t hbm_bytes, int chunk_intervals, dust2_server** out, char* err,
                         size_t errlen);
-/* pars: one parameter row of the model; data: n_time x n_data values. */
+/* pars: one parameter row of the model; data: n_time x n_data values.
+ * chain < 0: the filter's streams are seeded from the integer `seed`
+ * (stream k = jump^k of the splitmix64 expansion, as dust2_filter_sir_alloc).
+ * chain >= 0: the request belongs to an MCMC chain whose streams stay
+ * resident and continue from one request of the chain to the next, as a
+ * chain's filter object does; the chain's first request seeds them at
+ * long_jump^chain of the splitmix64 expansion of `seed`, then jumps
+ * (R/monty.R:29-36).  A chain's next request waits while one is resident. */
 int dust2_server_submit(dust2_server* s, int model, const double* pars, size_t n_particles,
                         double time_start, const double* time, size_t n_time, double dt,
-                        const double* data, uint64_t seed, uint64_t* id_out, char* err,
-                        size_t errlen);
-/* One scheduling tick (blocking); reports resident and queued requests. */
+                        const double* data, uint64_t seed, int64_t chain, uint64_t* id_out,
+                        char* err, size_t errlen);
+/* One scheduling pass: collects completed launches, admits queued requests,
+ * launches idle groups; waits only when nothing could be done while work is
+ * in flight.  Reports resident and queued requests. */
 int dust2_server_step(dust2_server* s, size_t* n_active, size_t* n_queued, char* err,
                       size_t errlen);
 /* Up to max finished results: id, log-likelihood, status (0 ok, 1 the
@@ -574,6 +585,13 @@ int dust2_server_poll(dust2_server* s, uint64_t* ids, double* ll, int* status, s
 int dust2_server_memory(dust2_server* s, size_t* used, size_t* capacity, char* err,
                         size_t errlen);
 void dust2_server_free(dust2_server* s);
+
+/* Jump (2^128 draws) or, with long_jump != 0, long-jump (2^192 draws) each of
+ * n_streams xoshiro256+ states in place (4 words each): monty's jump() and
+ * long_jump() on exported streams (vignettes/design.Rmd:72-81,
+ * R/monty.R:29-36). */
+int dust2_amd_rng_jump(uint64_t* state, size_t n_streams, int long_jump, char* err,
+                       size_t errlen);
 
 /* ---- kernels exposed for parity tests --------------------------------------
  * Systematic resampling of one weight vector on the GPU with the filter's

@@ -1,8 +1,10 @@
 """Continuous-batching server: every result equals the same request run
 alone, run with other chunk sizes, and run through the plain filter entry
 points (same kernels, same seed) -- bit for bit, FP32 filters and FP64 ODE
-unfilters; requests wait for memory rather than fail; a failing request
-reports its error without disturbing the others."""
+unfilters, up to 8192 particles and 730 intervals; a chain's requests
+continue its streams as consecutive runs of one filter seeded at the
+chain's long jump; requests wait for memory rather than fail; a failing
+request reports its error without disturbing the others."""
 import numpy as np
 import pytest
 
@@ -128,3 +130,53 @@ def test_failing_request_is_isolated(gpu_lib):
     assert res[b][2] == 1 and "particles reported errors" in res[b][3]
     alone = serve(gpu_lib, [(gpu_lib.sir(), good_p, 128, d, 0.25, 1)], chunk=4)
     assert alone[0][1] == res[a][1]
+
+
+def test_rng_jump_and_long_jump_match_oracle(gpu_lib):
+    from dust2_amd.interface import rng_jump
+    st = ol.prng(5, seed=17)
+    for long in (False, True):
+        got = rng_jump(st, long_jump=long)
+        for i in range(5):
+            assert np.array_equal(got[i], ol.jump(st[i], long=long))
+
+
+def test_chain_streams_continue_across_requests(gpu_lib):
+    # a chain's requests are the runs of one filter object whose streams
+    # start at long_jump^chain of the seed (R/monty.R:29-36): two requests
+    # of chain 3 equal two consecutive runs of the plain filter seeded there
+    from dust2_amd.serving import Server
+    rng = np.random.default_rng(21)
+    p1, d = sir_request(rng, 30)
+    p2 = dict(p1, beta=p1["beta"] * 1.1)
+    start = ol.seed_words(77).reshape(1, 4)
+    for _ in range(3):
+        start = ol.jump(start[0], long=True).reshape(1, 4)
+    plain = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, d, 512, dt=0.25, seed=start,
+                                       precision="fp32")
+    plain.set_engine("pipeline", graph=False)
+    ll1 = gpu_lib.dust_likelihood_run(plain, p1)
+    ll2 = gpu_lib.dust_likelihood_run(plain, p2)
+    sv = Server(hbm_bytes=1 << 28, chunk_intervals=7)
+    a = sv.submit(gpu_lib.sir(), p1, 512, 0.0, d, dt=0.25, seed=77, chain=3)
+    b = sv.submit(gpu_lib.sir(), p2, 512, 0.0, d, dt=0.25, seed=77, chain=3)
+    other = sv.submit(gpu_lib.sir(), p1, 512, 0.0, d, dt=0.25, seed=77, chain=4)
+    res = {r[0]: r for r in sv.drain()}
+    assert res[a][1] == ll1 and res[b][1] == ll2
+    assert res[other][1] != ll1
+
+
+@pytest.mark.parametrize("model,P,T", [("sir", 8192, 730), ("seir16", 8192, 104),
+                                       ("sir", 128, 730)])
+def test_largest_requests_equal_plain_filter(gpu_lib, model, P, T):
+    rng = np.random.default_rng(P + T)
+    if model == "sir":
+        g, (p, d) = gpu_lib.sir(), sir_request(rng, T)
+    else:
+        g, (p, d) = gpu_lib.seir16(), seir_request(rng, T)
+    q, e = sir_request(rng, 60)
+    served = serve(gpu_lib, [(g, p, P, d, 0.25, 5), (gpu_lib.sir(), q, 512, e, 0.25, 6)],
+                   chunk=32)
+    obj = gpu_lib.dust_filter_create(g, 0.0, d, P, dt=0.25, seed=5, precision="fp32")
+    obj.set_engine("pipeline", graph=False)
+    assert served[0][1] == gpu_lib.dust_likelihood_run(obj, p)
