@@ -512,8 +512,8 @@ def bench_serving(lib, args):
             else:
                 time = np.arange(1.0, T + 1)
                 data = rng.poisson(30 if model == "sir" else 6, T).astype(float)
-            chains.append({"model": model, "T": T, "P": P, "data": {"time": time,
-                                                                   "incidence": data},
+            chains.append({"model": model, "T": T, "P": P, "index": c,
+                           "data": {"time": time, "incidence": data},
                            "theta": np.array([0.3, 0.1]), "ll": None, "busy": False,
                            "obj": None})
         return chains
@@ -561,8 +561,10 @@ def bench_serving(lib, args):
                     break
                 ch = idle[int(rng.integers(len(idle)))]
                 prop = ch["theta"] + rng.normal(0, 0.02, 2)
+                # each chain's streams stay resident and continue from its
+                # previous request, seeded at long_jump^chain (R/monty.R:29-36)
                 rid = sv.submit(gens[ch["model"]], pars(ch, prop), ch["P"], 0.0, ch["data"],
-                                dt=0.25, seed=int(rng.integers(0, 2**62)))
+                                dt=0.25, seed=11, chain=ch["index"])
                 ch["busy"] = True
                 pending[rid] = (ch, prop, arrivals[ai], _time.perf_counter() - t0)
                 ai += 1

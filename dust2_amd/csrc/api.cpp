@@ -80,6 +80,34 @@ class dbuf {
   size_t bytes_ = 0;
 };
 
+// Pinned host memory (hipHostMalloc), owned.
+class pinned {
+ public:
+  pinned() = default;
+  pinned(const pinned&) = delete;
+  pinned& operator=(const pinned&) = delete;
+  ~pinned() { release(); }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes) hip_check(hipHostMalloc(&p_, bytes, hipHostMallocDefault), "hipHostMalloc");
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename T = void>
+  T* get() const {
+    return static_cast<T*>(p_);
+  }
+  size_t bytes() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
 void to_device(dbuf& d, const void* src, size_t bytes, hipStream_t s) {
   if (bytes) hip_check(hipMemcpyAsync(d.get(), src, bytes, hipMemcpyHostToDevice, s), "copy to device");
 }
@@ -2325,6 +2353,10 @@ struct dust2_unfilter {
   dbuf d_critical;
   dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active,
       d_grad, d_traj;
+  // the error count and the log-likelihoods of a run come back in one
+  // stream synchronisation through pinned memory
+  pinned h_result;
+  std::vector<size_t> active_idx;     // groups whose flags d_active holds
   bool traj_valid = false;
   std::vector<uint8_t> grad_current;  // per group: the gradient of the last run is valid
   std::vector<size_t> last_index;     // groups of the last run
@@ -2460,9 +2492,14 @@ namespace {
 void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_param,
                    bool save_traj) {
   hipStream_t s = h->stream;
-  std::vector<uint8_t> active(h->G, 0);
-  for (size_t g : idx) active[g] = 1;
-  to_device(h->d_active, active.data(), h->G, s);
+  // every group: no flags; otherwise the flags, uploaded when they change
+  const bool all = idx.size() == h->G;
+  if (!all && idx != h->active_idx) {
+    std::vector<uint8_t> active(h->G, 0);
+    for (size_t g : idx) active[g] = 1;
+    to_device(h->d_active, active.data(), h->G, s);
+    h->active_idx = idx;
+  }
   hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
   ode_launch a{};
   a.sens_param = sens_param;
@@ -2472,7 +2509,7 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
   }
   a.traj = save_traj ? h->d_traj.get() : nullptr;
   a.pars = h->d_pars.get<double>();
-  a.active = h->d_active.get<uint8_t>();
+  a.active = all ? nullptr : h->d_active.get<uint8_t>();
   a.n_particles = static_cast<int>(h->P);
   a.n_total = static_cast<int>(h->G * h->P);
   a.time_start = h->time_start;
@@ -2491,8 +2528,18 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_dp5_unfilter(a, h->fp64, s), "dp5 unfilter kernel");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
-  uint32_t count = 0;
-  to_host(&count, h->d_counters, sizeof(count), s);
+  const size_t n_ll = h->G * h->P;
+  if (h->h_result.bytes() < 256 + n_ll * sizeof(double)) {
+    h->h_result.alloc(256 + n_ll * sizeof(double));
+  }
+  hip_check(hipMemcpyAsync(h->h_result.get(), h->d_counters.get(), sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s),
+            "copy to host");
+  hip_check(hipMemcpyAsync(h->h_result.get<char>() + 256, h->d_ll.get(), n_ll * sizeof(double),
+                           hipMemcpyDeviceToHost, s),
+            "copy to host");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  const uint32_t count = *h->h_result.get<uint32_t>();
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
   h->last_ms = ms;
@@ -2505,11 +2552,15 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
   }
 }
 
+// The log-likelihoods of the last launch, already in pinned memory.
 void sirode_copy_ll(dust2_unfilter* h, const std::vector<size_t>& idx, double* ll_out) {
-  std::vector<double> ll(h->G * h->P);
-  to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), h->stream);
+  const double* ll = reinterpret_cast<const double*>(h->h_result.get<char>() + 256);
+  if (idx.size() == h->G) {
+    std::memcpy(ll_out, ll, h->G * h->P * sizeof(double));
+    return;
+  }
   for (size_t i = 0; i < idx.size(); ++i) {
-    std::memcpy(ll_out + i * h->P, ll.data() + idx[i] * h->P, h->P * sizeof(double));
+    std::memcpy(ll_out + i * h->P, ll + idx[i] * h->P, h->P * sizeof(double));
   }
 }
 }  // namespace
@@ -4438,34 +4489,6 @@ class arena {
   size_t cap_ = 0;
   size_t used_ = 0;
   std::map<size_t, size_t> free_;
-};
-
-// Pinned host memory (hipHostMalloc), owned.
-class pinned {
- public:
-  pinned() = default;
-  pinned(const pinned&) = delete;
-  pinned& operator=(const pinned&) = delete;
-  ~pinned() { release(); }
-  void alloc(size_t bytes) {
-    release();
-    if (bytes) hip_check(hipHostMalloc(&p_, bytes, hipHostMallocDefault), "hipHostMalloc");
-    n_ = bytes;
-  }
-  void release() {
-    if (p_) (void)hipHostFree(p_);
-    p_ = nullptr;
-    n_ = 0;
-  }
-  template <typename T = void>
-  T* get() const {
-    return static_cast<T*>(p_);
-  }
-  size_t bytes() const { return n_; }
-
- private:
-  void* p_ = nullptr;
-  size_t n_ = 0;
 };
 
 struct srv_request {
