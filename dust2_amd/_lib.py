@@ -36,12 +36,12 @@ _PROTOS = {
                                               C.c_double, D, C.c_size_t, C.c_double, D,
                                               C.c_size_t, U64, C.c_size_t, C.c_uint64, C.c_int,
                                               C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_filter_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_filter_run": (C.c_int, [VP, D, C.c_int, D, C.c_size_t, SZ, C.c_size_t, SZ,
+                                   C.c_size_t, D, ERR, ERRLEN]),
     "dust2_filter_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_filter_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_last_trajectories": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
     "dust2_filter_last_trajectories_raw": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_set_save_snapshots": (C.c_int, [VP, D, C.c_size_t, ERR, ERRLEN]),
     "dust2_filter_last_snapshots": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
     "dust2_filter_last_state_selected": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_last_ancestry": (C.c_int, [VP, U32, ERR, ERRLEN]),
@@ -78,21 +78,25 @@ _PROTOS = {
     "dust2_unfilter_sirode_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                               C.c_size_t, D, C.c_size_t, D, C.c_int,
                                               C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_unfilter_sirode_run": (C.c_int, [VP, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_run": (C.c_int, [VP, C.c_int, D, C.c_size_t, SZ, C.c_size_t, SZ,
+                                            C.c_size_t, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_sirode_last_snapshots": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_last_stats": (C.c_int, [VP, D, D, ERR, ERRLEN]),
-    "dust2_unfilter_sirode_run_gradient": (C.c_int, [VP, C.c_int, SZ, C.c_size_t, D, ERR,
+    "dust2_unfilter_sirode_run_gradient": (C.c_int, [VP, C.c_int, D, C.c_size_t, SZ,
+                                                     C.c_size_t, SZ, C.c_size_t, D, ERR,
                                                      ERRLEN]),
     "dust2_unfilter_sirode_last_gradient": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sirode_free": (None, [VP]),
     "dust2_unfilter_sir_alloc": (C.c_int, [D, C.c_size_t, C.c_size_t, C.c_double, D,
                                            C.c_size_t, C.c_double, D, C.c_size_t,
                                            C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_unfilter_sir_run": (C.c_int, [VP, D, C.c_int, C.c_int, SZ, C.c_size_t, D, ERR,
-                                         ERRLEN]),
+    "dust2_unfilter_sir_run": (C.c_int, [VP, D, C.c_int, D, C.c_size_t, C.c_int, SZ,
+                                         C.c_size_t, SZ, C.c_size_t, D, ERR, ERRLEN]),
     "dust2_unfilter_sir_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_sir_last_snapshots": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sir_last_gradient": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_sir_update_pars": (C.c_int, [VP, D, SZ, C.c_size_t, ERR, ERRLEN]),
     "dust2_unfilter_sir_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
@@ -152,20 +156,24 @@ _PROTOS = {
     "dust2_filter_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double, D,
                                          C.c_size_t, D, C.c_size_t, C.c_double, D, U64,
                                          C.c_size_t, C.c_uint64, C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_filter_ode_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_filter_ode_run": (C.c_int, [VP, D, C.c_int, D, C.c_size_t, SZ, C.c_size_t, SZ,
+                                       C.c_size_t, D, ERR, ERRLEN]),
     "dust2_filter_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_ode_last_snapshots": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_free": (None, [VP]),
     "dust2_unfilter_ode_alloc": (C.c_int, [C.c_char_p, D, C.c_size_t, C.c_size_t, C.c_double,
                                            D, C.c_size_t, D, C.c_size_t, C.c_double, D,
                                            C.POINTER(VP), ERR, ERRLEN]),
-    "dust2_unfilter_ode_run": (C.c_int, [VP, D, C.c_int, SZ, C.c_size_t, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_run": (C.c_int, [VP, D, C.c_int, D, C.c_size_t, SZ, C.c_size_t, SZ,
+                                         C.c_size_t, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_unfilter_ode_last_snapshots": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_unfilter_ode_free": (None, [VP]),
     "dust2_amd_interpolate": (C.c_int, [C.c_int, D, C.c_size_t, D, C.c_size_t, C.c_size_t, D,
                                         C.c_size_t, D, ERR, ERRLEN]),

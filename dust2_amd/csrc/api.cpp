@@ -185,6 +185,105 @@ std::vector<size_t> check_index(const size_t* index, size_t n_index, size_t n_gr
   return ret;
 }
 
+// index_state of a run (0-based): the states its trajectories keep
+// (trajectories<T>::set_index_and_reset, dust2/trajectories.hpp:31-45),
+// checked as dust_likelihood_run does (R/interface-likelihood.R:97-98 with
+// check_index, R/filter-support.R:73-91: in range, distinct).  Null keeps
+// every state and gives an empty list.
+std::vector<uint32_t> check_state_index(const size_t* index, size_t n_index, size_t S) {
+  std::vector<uint32_t> ret;
+  if (index == nullptr) return ret;
+  if (n_index == 0) throw std::runtime_error("'index_state' must have nonzero length");
+  std::vector<char> seen(S, 0);
+  for (size_t i = 0; i < n_index; ++i) {
+    if (index[i] >= S) {
+      throw std::runtime_error("Invalid index for 'index_state': values must lie in [0, " +
+                               std::to_string(S - 1) + "]");
+    }
+    if (seen[index[i]]) {
+      throw std::runtime_error("All elements of 'index_state' must be distinct");
+    }
+    seen[index[i]] = 1;
+    ret.push_back(static_cast<uint32_t>(index[i]));
+  }
+  return ret;
+}
+
+// save_snapshots of a run, checked as R/interface-likelihood.R:83-93 does
+// (increasing, each one of the data times): per data time its snapshot
+// slot or -1, and the number of snapshots in n_snap.
+std::vector<int32_t> snapshot_slots(const double* snaps, size_t n, const std::vector<double>& times,
+                                    size_t& n_snap) {
+  if (snaps == nullptr) n = 0;
+  for (size_t i = 1; i < n; ++i) {
+    if (!(snaps[i] > snaps[i - 1])) {
+      throw std::runtime_error("Values in 'save_snapshots' must be increasing");
+    }
+  }
+  std::vector<int32_t> slot(times.size(), -1);
+  for (size_t i = 0; i < n; ++i) {
+    const auto it = std::find(times.begin(), times.end(), snaps[i]);
+    if (it == times.end()) {
+      throw std::runtime_error("All elements of 'save_snapshots' must be found in 'time'");
+    }
+    slot[static_cast<size_t>(it - times.begin())] = static_cast<int32_t>(i);
+  }
+  n_snap = n;
+  return slot;
+}
+
+// What a run keeps beside its log-likelihoods (trajectories<T>,
+// dust2/trajectories.hpp:31-57): trajectories of the states of
+// index_state and the whole state at the snapshot times, both read from
+// the states saved at every data time.  A run checks its request first,
+// clears the record when it starts and stores the request when it
+// completes, so a failed run leaves nothing current (dust2/filter.hpp:
+// 95-99 marks trajectories current after the time loop).
+struct saved_run {
+  bool traj = false;             // trajectories current
+  size_t n_snap = 0;             // snapshots current (0: none)
+  std::vector<uint32_t> states;  // index_state (empty: every state)
+  std::vector<int32_t> slot;     // per data time: snapshot slot or -1
+  bool any() const { return traj || n_snap > 0; }
+};
+
+saved_run check_save(int save_trajectories, const double* save_snapshots, size_t n_snapshots,
+                     const size_t* index_state, size_t n_index_state, size_t S,
+                     const std::vector<double>& times) {
+  saved_run r;
+  r.traj = save_trajectories != 0;
+  r.states = check_state_index(index_state, n_index_state, S);
+  r.slot = snapshot_slots(save_snapshots, n_snapshots, times, r.n_snap);
+  return r;
+}
+
+// The states an unfilter saved at every data time, device layout
+// [t][S][n_total] (particle k of group g at k = g * P + p), as
+// out[rows][groups][P][states]: rows are the data times t in order, or,
+// with `slot`, the times with slot[t] >= 0 at row slot[t]; `states`
+// empty means every state.
+template <typename real, typename Group>
+void export_saved_states(const real* saved, size_t T, size_t S, size_t n_total, size_t P,
+                         const std::vector<Group>& groups, const std::vector<int32_t>* slot,
+                         const std::vector<uint32_t>& states, double* out) {
+  const size_t n_out = groups.size();
+  const size_t S_out = states.empty() ? S : states.size();
+  for (size_t t = 0; t < T; ++t) {
+    const int64_t row = slot ? (*slot)[t] : static_cast<int64_t>(t);
+    if (row < 0) continue;
+    for (size_t i = 0; i < n_out; ++i) {
+      for (size_t p = 0; p < P; ++p) {
+        const size_t k = static_cast<size_t>(groups[i]) * P + p;
+        double* o = out + ((static_cast<size_t>(row) * n_out + i) * P + p) * S_out;
+        for (size_t c = 0; c < S_out; ++c) {
+          const size_t src = states.empty() ? c : states[c];
+          o[c] = static_cast<double>(saved[(t * S + src) * n_total + k]);
+        }
+      }
+    }
+  }
+}
+
 // ---- xoshiro256+ on the host (seeding only) --------------------------------
 struct host_rng {
   uint64_t s[4];
@@ -602,7 +701,7 @@ struct dust2_filter {
   bool traj_valid = false;
   bool ancestry_valid = false;
   std::vector<size_t> last_index;        // groups of the last run
-  std::vector<double> next_snapshots;    // snapshot times requested for the next run
+  std::vector<uint32_t> traj_states;     // index_state of the last run (empty: all)
   std::vector<int32_t> snap_slot;        // per data time: snapshot slot or -1
   size_t n_snap = 0;                     // snapshots of the last run
   std::vector<uint8_t> select_valid;     // per group: a random particle is chosen
@@ -611,7 +710,7 @@ struct dust2_filter {
   int beta_kind = -1;
   size_t beta_n = 0, beta_groups = 0;
   dbuf d_beta_t, d_beta_y, d_beta_k;
-  dbuf d_select, d_groups, d_traj_out;   // lineage export
+  dbuf d_select, d_groups, d_traj_out, d_traj_states;   // lineage export
   double last_ms = 0;
   double last_fallbacks = 0;
   double last_advance_ms = 0;   // mean advance-kernel time (profile mode)
@@ -1153,16 +1252,19 @@ int dust2_filter_discrete_alloc(const char* model, const double* pars, size_t n_
 }
 
 int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectories_in,
-                         const size_t* index_group, size_t n_index, double* ll_out, char* err,
-                         size_t errlen) {
+                     const double* save_snapshots, size_t n_snapshots,
+                     const size_t* index_state, size_t n_index_state,
+                     const size_t* index_group, size_t n_index, double* ll_out, char* err,
+                     size_t errlen) {
   try {
     check_filter(h);
+    const auto idx = check_index(index_group, n_index, h->G);
+    const auto states = check_state_index(index_state, n_index_state, h->S);
+    size_t n_snap = 0;
+    const auto slot = snapshot_slots(save_snapshots, n_snapshots, h->times, n_snap);
     // snapshots need the saved states and ancestry of every time, as the
     // reference's filter does (dust2/filter.hpp:47-49)
-    std::vector<double> snapshots;
-    snapshots.swap(h->next_snapshots);
-    const int save_trajectories = (save_trajectories_in || !snapshots.empty()) ? 1 : 0;
-    const auto idx = check_index(index_group, n_index, h->G);
+    const int save_trajectories = (save_trajectories_in || n_snap > 0) ? 1 : 0;
     const size_t G = h->G, P = h->P;
     hipStream_t s = h->stream;
     // trajectories and snapshots become current only when the run completes
@@ -1247,11 +1349,9 @@ int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectori
     hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     h->last_ms = ms;
     h->last_fallbacks = counters[1];
-    h->snap_slot.assign(h->n_time, -1);
-    for (size_t i = 0, j = 0; i < h->n_time && j < snapshots.size(); ++i) {
-      if (h->times[i] == snapshots[j]) h->snap_slot[i] = static_cast<int32_t>(j++);
-    }
-    h->n_snap = snapshots.size();
+    h->snap_slot = slot;
+    h->n_snap = n_snap;
+    h->traj_states = states;
     h->last_index = idx;
     // a new run forgets the chosen particles of its groups (filter.hpp:162-175)
     if (h->select_valid.size() != G) h->select_valid.assign(G, 0);
@@ -1358,13 +1458,22 @@ int dust2_filter_last_trajectories(dust2_filter* h, int select_random_particle, 
     const uint32_t* groups = lineage_groups(h, select);
     const size_t n_out = h->last_index.size();
     const size_t p_out = select ? 1 : h->P;
-    const size_t bytes = h->n_time * n_out * p_out * h->S * sizeof(double);
+    const size_t s_out = h->traj_states.empty() ? h->S : h->traj_states.size();
+    const size_t bytes = h->n_time * n_out * p_out * s_out * sizeof(double);
     if (h->d_traj_out.bytes() < bytes) h->d_traj_out.alloc(bytes);
+    if (!h->traj_states.empty()) {
+      if (h->d_traj_states.bytes() < h->S * sizeof(uint32_t)) {
+        h->d_traj_states.alloc(h->S * sizeof(uint32_t));
+      }
+      to_device(h->d_traj_states, h->traj_states.data(), s_out * sizeof(uint32_t), h->stream);
+    }
     hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(), groups,
                              static_cast<int>(n_out), static_cast<int>(h->n_time),
                              static_cast<int>(h->S), static_cast<int>(h->P),
                              select ? h->d_select.get<uint32_t>() : nullptr, nullptr,
-                             h->fp64, h->d_traj_out.get<double>(), h->stream),
+                             h->traj_states.empty() ? nullptr : h->d_traj_states.get<uint32_t>(),
+                             static_cast<int>(s_out), h->fp64, h->d_traj_out.get<double>(),
+                             h->stream),
               "lineage kernel");
     to_host(out, h->d_traj_out, bytes, h->stream);
     return 0;
@@ -1402,28 +1511,6 @@ int dust2_filter_last_trajectories_raw(dust2_filter* h, double* out, char* err,
   }
 }
 
-int dust2_filter_set_save_snapshots(dust2_filter* h, const double* times, size_t n,
-                                    char* err, size_t errlen) {
-  try {
-    check_filter(h);
-    std::vector<double> snaps(times, times + n);
-    for (size_t i = 1; i < n; ++i) {
-      if (!(snaps[i] > snaps[i - 1])) {
-        throw std::runtime_error("Values in 'save_snapshots' must be increasing");
-      }
-    }
-    for (double t : snaps) {
-      if (std::find(h->times.begin(), h->times.end(), t) == h->times.end()) {
-        throw std::runtime_error("All elements of 'save_snapshots' must be found in 'time'");
-      }
-    }
-    h->next_snapshots = snaps;
-    return 0;
-  } catch (const std::exception& e) {
-    return fail(e, err, errlen);
-  }
-}
-
 int dust2_filter_last_snapshots(dust2_filter* h, int select_random_particle, double* out,
                                 char* err, size_t errlen) {
   try {
@@ -1443,8 +1530,8 @@ int dust2_filter_last_snapshots(dust2_filter* h, int select_random_particle, dou
                              static_cast<int>(n_out), static_cast<int>(h->n_time),
                              static_cast<int>(h->S), static_cast<int>(h->P),
                              select ? h->d_select.get<uint32_t>() : nullptr,
-                             d_slot.get<int32_t>(), h->fp64, h->d_traj_out.get<double>(),
-                             h->stream),
+                             d_slot.get<int32_t>(), nullptr, 0, h->fp64,
+                             h->d_traj_out.get<double>(), h->stream),
               "lineage kernel");
     to_host(out, h->d_traj_out, bytes, h->stream);
     return 0;
@@ -1710,8 +1797,8 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       h->profile = false;
       for (int rep = 0; rep < 2; ++rep) {
         char buf[4096];
-        const int rc = dust2_filter_run(h, nullptr, 0, idx.data(), idx.size(), ll.data(), buf,
-                                        sizeof buf);
+        const int rc = dust2_filter_run(h, nullptr, 0, nullptr, 0, nullptr, 0, idx.data(),
+                                        idx.size(), ll.data(), buf, sizeof buf);
         hip_check(hipMemcpyAsync(h->d_rng.get(), rng0.get(), h->d_rng.bytes(),
                                  hipMemcpyDeviceToDevice, s), "copy");
         if (rc != 0) {
@@ -2233,8 +2320,8 @@ int dust2_amd_trajectory_lineage(const double* raw, const uint32_t* ancestry, si
     hip_check(launch_lineage(dt.get(), ancestry ? da.get<uint32_t>() : nullptr,
                              dg.get<uint32_t>(), static_cast<int>(G), static_cast<int>(T),
                              static_cast<int>(S), static_cast<int>(P),
-                             select ? ds.get<uint32_t>() : nullptr, nullptr, true,
-                             dout.get<double>(), s),
+                             select ? ds.get<uint32_t>() : nullptr, nullptr, nullptr, 0,
+                             true, dout.get<double>(), s),
               "lineage kernel");
     to_host(out, dout, T * G * p_out * S * sizeof(double), s);
     return 0;
@@ -2357,7 +2444,8 @@ struct dust2_unfilter {
   // stream synchronisation through pinned memory
   pinned h_result;
   std::vector<size_t> active_idx;     // groups whose flags d_active holds
-  bool traj_valid = false;
+  std::vector<double> times;          // data times
+  saved_run saved;                    // trajectories and snapshots of the last run
   std::vector<uint8_t> grad_current;  // per group: the gradient of the last run is valid
   std::vector<size_t> last_index;     // groups of the last run
   size_t real_size() const { return fp64 ? 8 : 4; }
@@ -2449,6 +2537,7 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
     h->n_time = n_time;
     h->n_groups_data = n_groups_data;
     h->time_start = time_start;
+    h->times.assign(time, time + n_time);
     h->pars.assign(pars, pars + n_groups * kSirodePars);
     h->grad_current.assign(n_groups, 0);
     h->control = ode_control_from(control);
@@ -2568,15 +2657,19 @@ void sirode_copy_ll(dust2_unfilter* h, const std::vector<size_t>& idx, double* l
 extern "C" {
 
 int dust2_unfilter_sirode_run(dust2_unfilter* h, int save_trajectories,
+                              const double* save_snapshots, size_t n_snapshots,
+                              const size_t* index_state, size_t n_index_state,
                               const size_t* index_group, size_t n_index, double* ll_out,
                               char* err, size_t errlen) {
   try {
     check_unfilter(h);
     const auto idx = check_index(index_group, n_index, h->G);
+    saved_run req = check_save(save_trajectories, save_snapshots, n_snapshots, index_state,
+                               n_index_state, 5, h->times);
     for (size_t g : idx) h->grad_current[g] = 0;
-    h->traj_valid = false;
-    sirode_launch(h, idx, -1, save_trajectories != 0);
-    h->traj_valid = save_trajectories != 0;
+    h->saved = saved_run{};
+    sirode_launch(h, idx, -1, req.any());
+    h->saved = std::move(req);
     sirode_copy_ll(h, idx, ll_out);
     h->last_index = idx;
     return 0;
@@ -2586,6 +2679,8 @@ int dust2_unfilter_sirode_run(dust2_unfilter* h, int save_trajectories,
 }
 
 int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, int save_trajectories,
+                                       const double* save_snapshots, size_t n_snapshots,
+                                       const size_t* index_state, size_t n_index_state,
                                        const size_t* index_group, size_t n_index,
                                        double* ll_out, char* err, size_t errlen) {
   try {
@@ -2594,15 +2689,17 @@ int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, int save_trajectories,
       throw std::runtime_error("The ODE gradient is computed in FP64 only");
     }
     const auto idx = check_index(index_group, n_index, h->G);
+    saved_run req = check_save(save_trajectories, save_snapshots, n_snapshots, index_state,
+                               n_index_state, 5, h->times);
     for (size_t g : idx) h->grad_current[g] = 0;
     if (h->d_grad.bytes() == 0) h->d_grad.alloc(h->G * h->P * 3 * sizeof(double));
     double total_ms = 0;
-    h->traj_valid = false;
+    h->saved = saved_run{};
     for (int param = 0; param < 3; ++param) {
-      sirode_launch(h, idx, param, param == 0 && save_trajectories != 0);
+      sirode_launch(h, idx, param, param == 0 && req.any());
       total_ms += h->last_ms;
     }
-    h->traj_valid = save_trajectories != 0;
+    h->saved = std::move(req);
     h->last_ms = total_ms;
     sirode_copy_ll(h, idx, ll_out);
     for (size_t g : idx) h->grad_current[g] = 1;
@@ -2634,28 +2731,48 @@ int dust2_unfilter_sirode_last_gradient(dust2_unfilter* h, double* out, char* er
   }
 }
 
+}  // extern "C"
+
+namespace {
+// trajectories (snapshots = false) or snapshots of the last run of the ODE
+// unfilter: device [t][S][G*P] -> out [t or snapshot][groups][P][states]
+void sirode_export(dust2_unfilter* h, bool snapshots, double* out) {
+  const size_t NT = h->G * h->P, T = h->n_time;
+  std::vector<char> tmp(T * 5 * NT * h->real_size());
+  to_host(tmp.data(), h->d_traj, tmp.size(), h->stream);
+  const std::vector<int32_t>* slot = snapshots ? &h->saved.slot : nullptr;
+  const std::vector<uint32_t> all;
+  const std::vector<uint32_t>& states = snapshots ? all : h->saved.states;
+  if (h->fp64) {
+    export_saved_states(reinterpret_cast<const double*>(tmp.data()), T, 5, NT, h->P,
+                        h->last_index, slot, states, out);
+  } else {
+    export_saved_states(reinterpret_cast<const float*>(tmp.data()), T, 5, NT, h->P,
+                        h->last_index, slot, states, out);
+  }
+}
+}  // namespace
+
+extern "C" {
+
 int dust2_unfilter_sirode_last_trajectories(dust2_unfilter* h, double* out, char* err,
                                             size_t errlen) {
   try {
     check_unfilter(h);
-    if (!h->traj_valid) throw std::runtime_error("Trajectories are not current");
-    const size_t NT = h->G * h->P, T = h->n_time;
-    std::vector<char> tmp(T * 5 * NT * h->real_size());
-    to_host(tmp.data(), h->d_traj, tmp.size(), h->stream);
-    // device [t][S][G*P] -> out [t][groups of the last run][P][S]
-    for (size_t t = 0; t < T; ++t) {
-      for (size_t i = 0; i < h->last_index.size(); ++i) {
-        for (size_t p = 0; p < h->P; ++p) {
-          const size_t k = h->last_index[i] * h->P + p;
-          for (size_t c = 0; c < 5; ++c) {
-            const size_t o = (t * 5 + c) * NT + k;
-            out[((t * h->last_index.size() + i) * h->P + p) * 5 + c] =
-                h->fp64 ? reinterpret_cast<const double*>(tmp.data())[o]
-                        : reinterpret_cast<const float*>(tmp.data())[o];
-          }
-        }
-      }
-    }
+    if (!h->saved.traj) throw std::runtime_error("Trajectories are not current");
+    sirode_export(h, false, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sirode_last_snapshots(dust2_unfilter* h, double* out, char* err,
+                                         size_t errlen) {
+  try {
+    check_unfilter(h);
+    if (h->saved.n_snap == 0) throw std::runtime_error("Snapshots are not current");
+    sirode_export(h, true, out);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -3649,9 +3766,9 @@ struct dust2_filter_ode {
   double time_start = 0;
   std::vector<double> times;
   dbuf d_data, d_frng, d_ll, d_ll_step, d_idx, d_state2, d_words2, d_traj, d_ancestry,
-      d_fallback, d_groups, d_out, d_run_groups;
+      d_fallback, d_groups, d_out, d_run_groups, d_states, d_slot;
   std::vector<uint32_t> last_groups;  // groups of the last run
-  bool traj_current = false;
+  saved_run saved;                    // trajectories and snapshots of the last run
   ~dust2_filter_ode() {
     if (sys) dust2_system_ode_free(sys);
   }
@@ -3761,7 +3878,9 @@ int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_group
 // own), then for the groups of index_group (null: all) per data time
 // run_to_time, compare_data, log-sum-exp, one uniform per group and
 // systematic resampling, then reorder.  ll_out[n_index] in index order.
-int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories,
+int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories_in,
+                         const double* save_snapshots, size_t n_snapshots,
+                         const size_t* index_state, size_t n_index_state,
                          const size_t* index_group, size_t n_index, double* ll_out, char* err,
                          size_t errlen) {
   try {
@@ -3773,6 +3892,10 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
     std::vector<uint32_t> groups;
     std::vector<uint8_t> active;
     run_groups(index_group, n_index, G, groups, active);
+    saved_run req = check_save(save_trajectories_in, save_snapshots, n_snapshots, index_state,
+                               n_index_state, sys->S, h->times);
+    // snapshots follow the ancestry of the saved states (dust2/filter.hpp:47-49)
+    const bool save_trajectories = req.any();
     const bool all = groups.size() == G;
     h->d_run_groups.alloc(G * sizeof(uint32_t));
     to_device(h->d_run_groups, groups.data(), groups.size() * sizeof(uint32_t), s);
@@ -3785,7 +3908,7 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
     if (rc != 0) throw std::runtime_error(buf);
     hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), s), "memset");
     hip_check(hipMemsetAsync(h->d_fallback.get(), 0, h->d_fallback.bytes(), s), "memset");
-    h->traj_current = false;
+    h->saved = saved_run{};
     if (save_trajectories) {
       h->d_traj.alloc(G * h->n_time * sys->S * P * sizeof(double));
       h->d_ancestry.alloc(G * h->n_time * P * sizeof(uint32_t));
@@ -3839,7 +3962,7 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
     to_host(ll.data(), h->d_ll, G * sizeof(double), s);
     for (size_t i = 0; i < groups.size(); ++i) ll_out[i] = ll[groups[i]];
     h->last_groups = groups;
-    h->traj_current = save_trajectories != 0;
+    h->saved = std::move(req);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -3866,26 +3989,65 @@ int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, siz
   }
 }
 
-// trajectories following the ancestry (dust2/trajectories.hpp:128-189) of
-// the groups of the last run: out [n_time][n_groups_run][P][S]
+}  // extern "C"
+
+namespace {
+// Trajectories (snapshots = false: the states of index_state at every data
+// time) or snapshots (the whole state at the snapshot times) following the
+// ancestry (dust2/trajectories.hpp:128-283) of the groups of the last run:
+// out [times][n_groups_run][P][states].
+void filter_ode_export(dust2_filter_ode* h, bool snapshots, double* out) {
+  hipStream_t s = h->sys->stream;
+  const size_t P = h->P, S = h->sys->S, T = h->n_time;
+  const std::vector<uint32_t>& groups = h->last_groups;
+  const size_t G = groups.size();
+  const std::vector<uint32_t>& states = h->saved.states;
+  const bool subset = !snapshots && !states.empty();
+  const size_t s_out = subset ? states.size() : S;
+  const size_t rows = snapshots ? h->saved.n_snap : T;
+  h->d_groups.alloc(G * sizeof(uint32_t));
+  to_device(h->d_groups, groups.data(), G * sizeof(uint32_t), s);
+  if (subset) {
+    h->d_states.alloc(s_out * sizeof(uint32_t));
+    to_device(h->d_states, states.data(), s_out * sizeof(uint32_t), s);
+  }
+  if (snapshots) {
+    h->d_slot.alloc(T * sizeof(int32_t));
+    to_device(h->d_slot, h->saved.slot.data(), T * sizeof(int32_t), s);
+  }
+  const size_t bytes = rows * G * P * s_out * sizeof(double);
+  h->d_out.alloc(bytes);
+  hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(),
+                           h->d_groups.get<uint32_t>(), static_cast<int>(G),
+                           static_cast<int>(T), static_cast<int>(S), static_cast<int>(P),
+                           nullptr, snapshots ? h->d_slot.get<int32_t>() : nullptr,
+                           subset ? h->d_states.get<uint32_t>() : nullptr,
+                           static_cast<int>(s_out), true, h->d_out.get<double>(), s),
+            "lineage kernel");
+  to_host(out, h->d_out, bytes, s);
+}
+}  // namespace
+
+extern "C" {
+
 int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
                                        size_t errlen) {
   try {
     check_filter_ode(h);
-    if (!h->traj_current) throw std::runtime_error("Trajectories are not current");
-    hipStream_t s = h->sys->stream;
-    const size_t P = h->P, S = h->sys->S, T = h->n_time;
-    const std::vector<uint32_t>& groups = h->last_groups;
-    const size_t G = groups.size();
-    h->d_groups.alloc(G * sizeof(uint32_t));
-    to_device(h->d_groups, groups.data(), G * sizeof(uint32_t), s);
-    h->d_out.alloc(T * G * P * S * sizeof(double));
-    hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(),
-                             h->d_groups.get<uint32_t>(), static_cast<int>(G),
-                             static_cast<int>(T), static_cast<int>(S), static_cast<int>(P),
-                             nullptr, nullptr, true, h->d_out.get<double>(), s),
-              "lineage kernel");
-    to_host(out, h->d_out, T * G * P * S * sizeof(double), s);
+    if (!h->saved.traj) throw std::runtime_error("Trajectories are not current");
+    filter_ode_export(h, false, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_ode_last_snapshots(dust2_filter_ode* h, double* out, char* err,
+                                    size_t errlen) {
+  try {
+    check_filter_ode(h);
+    if (h->saved.n_snap == 0) throw std::runtime_error("Snapshots are not current");
+    filter_ode_export(h, true, out);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -3950,7 +4112,8 @@ struct dust2_unfilter_ode {
   std::vector<double> times;
   dbuf d_data, d_ll, d_traj;
   std::vector<uint32_t> last_groups;
-  bool traj_current = false, has_run = false;
+  saved_run saved;                    // trajectories and snapshots of the last run
+  bool has_run = false;
   ~dust2_unfilter_ode() {
     if (sys) dust2_system_ode_free(sys);
   }
@@ -4003,7 +4166,9 @@ int dust2_unfilter_ode_alloc(const char* model, const double* pars, size_t n_gro
 // null), then for the groups of index_group (null: all) per data time
 // run_to_time and compare_data, each particle's log-likelihood summed;
 // ll_out[n_index][P].
-int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories,
+int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories_in,
+                           const double* save_snapshots, size_t n_snapshots,
+                           const size_t* index_state, size_t n_index_state,
                            const size_t* index_group, size_t n_index, double* ll_out,
                            char* err, size_t errlen) {
   try {
@@ -4015,6 +4180,9 @@ int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int sav
     std::vector<uint32_t> groups;
     std::vector<uint8_t> active;
     run_groups(index_group, n_index, h->G, groups, active);
+    saved_run req = check_save(save_trajectories_in, save_snapshots, n_snapshots, index_state,
+                               n_index_state, sys->S, h->times);
+    const bool save_trajectories = req.any();
     const bool all = groups.size() == h->G;
     sys->errors_pending = false;
     sys->time = h->time_start;
@@ -4023,7 +4191,7 @@ int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int sav
                            : dust2_system_ode_set_state_initial(sys, buf, sizeof buf);
     if (rc != 0) throw std::runtime_error(buf);
     hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), s), "memset");
-    h->traj_current = false;
+    h->saved = saved_run{};
     const size_t block = sys->S * n_total;
     if (save_trajectories) h->d_traj.alloc(h->n_time * block * sizeof(double));
     const int nd = sys->info.n_data;
@@ -4056,7 +4224,7 @@ int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int sav
                 ll_out + i * h->P);
     }
     h->last_groups = groups;
-    h->traj_current = save_trajectories != 0;
+    h->saved = std::move(req);
     h->has_run = true;
     return 0;
   } catch (const std::exception& e) {
@@ -4086,24 +4254,36 @@ int dust2_unfilter_ode_last_state(dust2_unfilter_ode* h, double* out, char* err,
   }
 }
 
-// the state at every data time of the groups of the last run:
-// out[n_time][n_groups_run][P][S]
+// the states of index_state at every data time of the groups of the last
+// run: out[n_time][n_groups_run][P][states]
 int dust2_unfilter_ode_last_trajectories(dust2_unfilter_ode* h, double* out, char* err,
                                          size_t errlen) {
   try {
     check_handle(h, TAG_UNFILTER_ODE, "unfilter");
-    if (!h->traj_current) throw std::runtime_error("Trajectories are not current");
-    const size_t n_total = h->G * h->P, S = h->sys->S, block = S * n_total, P = h->P;
-    std::vector<double> tmp(h->n_time * block), aos(block);
+    if (!h->saved.traj) throw std::runtime_error("Trajectories are not current");
+    const size_t n_total = h->G * h->P, S = h->sys->S;
+    std::vector<double> tmp(h->n_time * S * n_total);
     to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->sys->stream);
-    const size_t n_out = h->last_groups.size();
-    for (size_t t = 0; t < h->n_time; ++t) {
-      soa_to_aos(tmp.data() + t * block, aos.data(), 1, n_total, S, true);
-      for (size_t i = 0; i < n_out; ++i) {
-        const double* src = aos.data() + h->last_groups[i] * P * S;
-        std::copy(src, src + P * S, out + (t * n_out + i) * P * S);
-      }
-    }
+    export_saved_states(tmp.data(), h->n_time, S, n_total, h->P, h->last_groups, nullptr,
+                        h->saved.states, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+// the whole state at the snapshot times of the last run:
+// out[n_snapshots][n_groups_run][P][S]
+int dust2_unfilter_ode_last_snapshots(dust2_unfilter_ode* h, double* out, char* err,
+                                      size_t errlen) {
+  try {
+    check_handle(h, TAG_UNFILTER_ODE, "unfilter");
+    if (h->saved.n_snap == 0) throw std::runtime_error("Snapshots are not current");
+    const size_t n_total = h->G * h->P, S = h->sys->S;
+    std::vector<double> tmp(h->n_time * S * n_total);
+    to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->sys->stream);
+    export_saved_states(tmp.data(), h->n_time, S, n_total, h->P, h->last_groups,
+                        &h->saved.slot, {}, out);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
@@ -4132,7 +4312,7 @@ struct dust2_unfilter_sir {
   double last_ms = 0;
   dbuf d_pars, d_sched_time, d_sched_steps, d_obs, d_lfact, d_state, d_ll, d_errors,
       d_counters, d_active, d_hist, d_inc, d_grad, d_traj;
-  bool traj_valid = false;
+  saved_run saved;                    // trajectories and snapshots of the last run
   bool exact_gradient = false;        // the reference's adjoint unless asked
   ~dust2_unfilter_sir() {
     if (ev0) (void)hipEventDestroy(ev0);
@@ -4237,11 +4417,16 @@ int dust2_unfilter_sir_alloc(const double* pars, size_t n_groups, size_t n_parti
 }
 
 int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int save_trajectories,
-                           int adjoint, const size_t* index_group, size_t n_index,
-                           double* ll_out, char* err, size_t errlen) {
+                           const double* save_snapshots, size_t n_snapshots, int adjoint,
+                           const size_t* index_state, size_t n_index_state,
+                           const size_t* index_group, size_t n_index, double* ll_out,
+                           char* err, size_t errlen) {
   try {
     check_unfilter_sir(h);
     const auto idx = check_index(index_group, n_index, h->G);
+    saved_run req = check_save(save_trajectories, save_snapshots, n_snapshots, index_state,
+                               n_index_state, 5, h->times);
+    const bool save = req.any();
     hipStream_t s = h->stream;
     const size_t n_total = h->G * h->P;
     std::vector<uint8_t> active(h->G, 0);
@@ -4265,12 +4450,12 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int sav
         h->d_grad.alloc(n_total * 3 * sizeof(double));
       }
     }
-    if (save_trajectories && h->d_traj.bytes() == 0) {
+    if (save && h->d_traj.bytes() == 0) {
       h->d_traj.alloc(h->n_time * 5 * n_total * sizeof(double));
     }
     hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
     sir_unfilter_launch a{};
-    a.traj = save_trajectories ? h->d_traj.get<double>() : nullptr;
+    a.traj = save ? h->d_traj.get<double>() : nullptr;
     a.pars = h->d_pars.get<double>();
     a.active = h->d_active.get<uint8_t>();
     a.n_particles = static_cast<int>(h->P);
@@ -4301,7 +4486,7 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int sav
     hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
     h->last_ms = ms;
     for (size_t g : idx) h->grad_current[g] = 0;
-    h->traj_valid = false;
+    h->saved = saved_run{};
     if (count > 0) {
       std::vector<error_record> rec(n_total);
       to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
@@ -4312,7 +4497,7 @@ int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int sav
     if (adjoint) {
       for (size_t g : idx) h->grad_current[g] = 1;
     }
-    h->traj_valid = save_trajectories != 0;
+    h->saved = std::move(req);
     h->last_index = idx;
     std::vector<double> ll(n_total);
     to_host(ll.data(), h->d_ll, ll.size() * sizeof(double), s);
@@ -4352,20 +4537,27 @@ int dust2_unfilter_sir_last_trajectories(dust2_unfilter_sir* h, double* out, cha
                                          size_t errlen) {
   try {
     check_unfilter_sir(h);
-    if (!h->traj_valid) throw std::runtime_error("Trajectories are not current");
+    if (!h->saved.traj) throw std::runtime_error("Trajectories are not current");
     const size_t NT = h->G * h->P, T = h->n_time;
     std::vector<double> tmp(T * 5 * NT);
     to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->stream);
-    for (size_t t = 0; t < T; ++t) {
-      for (size_t i = 0; i < h->last_index.size(); ++i) {
-        for (size_t p = 0; p < h->P; ++p) {
-          const size_t k = h->last_index[i] * h->P + p;
-          for (size_t c = 0; c < 5; ++c) {
-            out[((t * h->last_index.size() + i) * h->P + p) * 5 + c] = tmp[(t * 5 + c) * NT + k];
-          }
-        }
-      }
-    }
+    export_saved_states(tmp.data(), T, 5, NT, h->P, h->last_index, nullptr, h->saved.states,
+                        out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_unfilter_sir_last_snapshots(dust2_unfilter_sir* h, double* out, char* err,
+                                      size_t errlen) {
+  try {
+    check_unfilter_sir(h);
+    if (h->saved.n_snap == 0) throw std::runtime_error("Snapshots are not current");
+    const size_t NT = h->G * h->P, T = h->n_time;
+    std::vector<double> tmp(T * 5 * NT);
+    to_host(tmp.data(), h->d_traj, tmp.size() * sizeof(double), h->stream);
+    export_saved_states(tmp.data(), T, 5, NT, h->P, h->last_index, &h->saved.slot, {}, out);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

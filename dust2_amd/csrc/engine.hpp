@@ -312,10 +312,12 @@ hipError_t launch_sir_unfilter(const sir_unfilter_launch& args, hipStream_t stre
 // Lineage export: traj real [G][T][S][P], ancestry [G][T][P] or null,
 // groups[n_groups_out], select [G] chosen particle or null (all particles),
 // t_slot [T] output slot per time or -1 (null: every time, slot t);
-// out double [slots][n_groups_out][P_out][S].
+// states [S_out] the states exported (null: all S);
+// out double [slots][n_groups_out][P_out][S_out].
 hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint32_t* groups,
                           int n_groups_out, int T, int S, int P, const uint32_t* select,
-                          const int32_t* t_slot, bool fp64, double* out, hipStream_t stream);
+                          const int32_t* t_slot, const uint32_t* states, int S_out, bool fp64,
+                          double* out, hipStream_t stream);
 // select[g] = floor(u * P), u from filter stream 2g + 1 of the [G][4][P + 2]
 // stream blocks, for the n listed groups.
 hipError_t launch_select_particle(uint64_t* rng, const uint32_t* groups, int n, int P,

@@ -23,14 +23,15 @@ namespace {
 // traj: real [G][T][S][P]; ancestry: [G][T][P] (null: no resampling ever);
 // groups: the n_groups_out groups to export; select: [G] chosen particle
 // per group, or null for every particle; t_slot: [T] output slot of each
-// time or -1 to skip it (null: slot t for time t, the trajectories).
-// out: double [slots][n_groups_out][P_out][S], P_out = select ? 1 : P.
+// time or -1 to skip it (null: slot t for time t, the trajectories);
+// states: the S_out states to export (null: S_out = S, every state).
+// out: double [slots][n_groups_out][P_out][S_out], P_out = select ? 1 : P.
 template <typename real>
 __global__ void __launch_bounds__(256)
     lineage_kernel(const real* __restrict__ traj, const uint32_t* __restrict__ ancestry,
                    const uint32_t* __restrict__ groups, int n_groups_out, int T, int S, int P,
                    const uint32_t* __restrict__ select, const int32_t* __restrict__ t_slot,
-                   double* __restrict__ out) {
+                   const uint32_t* __restrict__ states, int S_out, double* __restrict__ out) {
   const int p_out = select ? 1 : P;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_groups_out * p_out) return;
@@ -44,9 +45,10 @@ __global__ void __launch_bounds__(256)
     const int slot = t_slot ? t_slot[t] : t;
     if (slot >= 0) {
       const real* ts = tg + static_cast<size_t>(t) * S * P;
-      double* o = out + ((static_cast<size_t>(slot) * n_groups_out + j) * p_out + k) * S;
-      for (int c = 0; c < S; ++c) {
-        o[c] = static_cast<double>(ts[static_cast<size_t>(c) * P + idx]);
+      double* o = out + ((static_cast<size_t>(slot) * n_groups_out + j) * p_out + k) * S_out;
+      for (int c = 0; c < S_out; ++c) {
+        const int src = states ? static_cast<int>(states[c]) : c;
+        o[c] = static_cast<double>(ts[static_cast<size_t>(src) * P + idx]);
       }
     }
     if (ag) idx = ag[static_cast<size_t>(t) * P + idx];
@@ -75,18 +77,20 @@ __global__ void select_particle_kernel(uint64_t* __restrict__ rng, const uint32_
 
 hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint32_t* groups,
                           int n_groups_out, int T, int S, int P, const uint32_t* select,
-                          const int32_t* t_slot, bool fp64, double* out, hipStream_t stream) {
+                          const int32_t* t_slot, const uint32_t* states, int S_out, bool fp64,
+                          double* out, hipStream_t stream) {
   const int n = n_groups_out * (select ? 1 : P);
+  if (!states) S_out = S;
   const int block = 256;
   const dim3 grid((n + block - 1) / block);
   if (fp64) {
     hipLaunchKernelGGL(lineage_kernel<double>, grid, dim3(block), 0, stream,
                        static_cast<const double*>(traj), ancestry, groups, n_groups_out, T, S,
-                       P, select, t_slot, out);
+                       P, select, t_slot, states, S_out, out);
   } else {
     hipLaunchKernelGGL(lineage_kernel<float>, grid, dim3(block), 0, stream,
                        static_cast<const float*>(traj), ancestry, groups, n_groups_out, T, S,
-                       P, select, t_slot, out);
+                       P, select, t_slot, states, S_out, out);
   }
   return hipGetLastError();
 }

@@ -422,6 +422,29 @@ def _index_group(index_group, n_groups):
     return idx, idx.size, idx.size
 
 
+class _SaveArgs:
+    """save_snapshots and index_state as the run entry points take them
+    (R/interface-likelihood.R:83-98 checks them there): data times at which
+    the whole state is kept and the (0-based) states the trajectories keep;
+    `args` are the four pointer/length arguments of the C call."""
+
+    def __init__(self, save_snapshots, index_state):
+        self.snapshots = None
+        if save_snapshots is not None and np.size(save_snapshots) > 0:
+            self.snapshots = f64(save_snapshots).reshape(-1)
+        self.states = None
+        if index_state is not None:
+            self.states = np.ascontiguousarray(index_state, dtype=np.uintp).reshape(-1)
+        n_snap = 0 if self.snapshots is None else self.snapshots.size
+        n_states = 0 if self.states is None else self.states.size
+        self.args = (dptr(self.snapshots), n_snap, szptr(self.states), n_states)
+
+
+def _traj_width(obj):
+    states = getattr(obj, "last_index_state", None)
+    return obj.generator.n_state if states is None else np.size(states)
+
+
 def _r_num(x):
     """A number as R's cli prints it in messages (6, 0.25)."""
     return f"{float(x):g}"
@@ -637,7 +660,7 @@ class ModelFilter:
         self.rows = None
         self.last_index = None
         self.last_index_state = None
-        self.traj_current = False
+        self.last_snapshots = None
 
     @property
     def initialised(self):
@@ -677,14 +700,21 @@ class ModelFilter:
         return np.broadcast_to(
             f64(initial), (self.n_groups, self.n_particles, self.generator.n_state)).copy()
 
-    def run(self, pars, index_group=None, initial=None, save_trajectories=False):
+    def run(self, pars, index_group=None, initial=None, save_trajectories=False,
+            save_snapshots=None, index_state=None):
         idx, n_idx, n_out = self._update(pars, index_group)
+        save = _SaveArgs(save_snapshots, index_state)
         ll = np.zeros(n_out)
+        self.last_snapshots = None
         call("dust2_filter_ode_run", self.ptr, dptr(self._initial(initial)),
-             int(bool(save_trajectories)), szptr(idx), n_idx, dptr(ll))
-        self.traj_current = bool(save_trajectories)
-        self.last_index = index_group
+             int(bool(save_trajectories)), *save.args, szptr(idx), n_idx, dptr(ll))
+        self._saved(index_group, save)
         return float(ll[0]) if (self.n_groups == 1 and index_group is None) else ll
+
+    def _saved(self, index_group, save):
+        self.last_index = index_group
+        self.last_index_state = save.states
+        self.last_snapshots = save.snapshots
 
     def last_state(self):
         if self.ptr is None:
@@ -694,11 +724,19 @@ class ModelFilter:
         return out
 
     def last_trajectories(self):
-        if self.ptr is None or not self.traj_current:
+        if self.ptr is None:
             raise DustError("Trajectories are not current")
         n = self.n_groups if self.last_index is None else np.size(self.last_index)
-        out = np.zeros((self.time.size, n, self.n_particles, self.generator.n_state))
+        out = np.zeros((self.time.size, n, self.n_particles, _traj_width(self)))
         call(self.prefix + "_last_trajectories", self.ptr, dptr(out))
+        return out
+
+    def last_snapshots_state(self):
+        if self.ptr is None or self.last_snapshots is None:
+            raise DustError("Snapshots are not current")
+        n = self.n_groups if self.last_index is None else np.size(self.last_index)
+        out = np.zeros((self.last_snapshots.size, n, self.n_particles, self.generator.n_state))
+        call(self.prefix + "_last_snapshots", self.ptr, dptr(out))
         return out
 
     prefix = "dust2_filter_ode"
@@ -729,13 +767,15 @@ class ModelUnfilter(ModelFilter):
              C.byref(out))
         self.ptr = out
 
-    def run(self, pars, index_group=None, initial=None, save_trajectories=False):
+    def run(self, pars, index_group=None, initial=None, save_trajectories=False,
+            save_snapshots=None, index_state=None):
         idx, n_idx, n_out = self._update(pars, index_group)
+        save = _SaveArgs(save_snapshots, index_state)
         ll = np.zeros((n_out, self.n_particles))
+        self.last_snapshots = None
         call("dust2_unfilter_ode_run", self.ptr, dptr(self._initial(initial)),
-             int(bool(save_trajectories)), szptr(idx), n_idx, dptr(ll))
-        self.traj_current = bool(save_trajectories)
-        self.last_index = index_group
+             int(bool(save_trajectories)), *save.args, szptr(idx), n_idx, dptr(ll))
+        self._saved(index_group, save)
         if self.n_particles == 1:
             ll = ll[:, 0]
         return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1 and index_group is None) \
@@ -768,14 +808,12 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
             raise ValueError("'index_group' must be NULL, as 'obj' is not initialised")
     index_state = check_index(index_state, obj.generator.n_state, "index_state")
     if isinstance(obj, ModelFilter):
-        obj.last_index_state = index_state
-        return obj.run(pars, index_group, initial=initial, save_trajectories=save_trajectories)
+        return obj.run(pars, index_group, initial=initial, save_trajectories=save_trajectories,
+                       save_snapshots=save_snapshots, index_state=index_state)
     if isinstance(obj, Unfilter):
-        ll = obj.run(pars, index_group, initial=initial, adjoint=adjoint,
-                     save_trajectories=save_trajectories)
-        obj.last_index_state = index_state
-        return ll
-    obj.last_index_state = index_state
+        return obj.run(pars, index_group, initial=initial, adjoint=adjoint,
+                       save_trajectories=save_trajectories, save_snapshots=save_snapshots,
+                       index_state=index_state)
     if not obj.initialised:
         obj._alloc(pars)
     elif pars is not None:
@@ -793,14 +831,13 @@ def dust_likelihood_run(obj, pars, initial=None, save_trajectories=False, index_
         if init.size != obj.n_groups * obj.n_particles * obj.generator.n_state:
             raise ValueError("Expected 'initial' to have n_groups * n_particles * n_state values")
     ll = np.zeros(n_out)
-    obj.last_index = index_group
+    save = _SaveArgs(save_snapshots, index_state)
     obj.last_snapshots = None
-    if save_snapshots is not None and np.size(save_snapshots) > 0:
-        snaps = f64(save_snapshots).reshape(-1)
-        call("dust2_filter_set_save_snapshots", obj.ptr, dptr(snaps), snaps.size)
-        obj.last_snapshots = snaps
-    call("dust2_filter_run", obj.ptr, dptr(init), int(bool(save_trajectories)), szptr(idx),
-         n_idx, dptr(ll))
+    call("dust2_filter_run", obj.ptr, dptr(init), int(bool(save_trajectories)), *save.args,
+         szptr(idx), n_idx, dptr(ll))
+    obj.last_index = index_group
+    obj.last_index_state = save.states
+    obj.last_snapshots = save.snapshots
     return float(ll[0]) if (obj.n_groups == 1 and index_group is None) else ll
 
 
@@ -824,8 +861,8 @@ def dust_likelihood_copy(obj, seed=None):
     dst.ptr = None
     dst.last_index = None
     dst.last_index_state = None
+    dst.last_snapshots = None
     if isinstance(obj, Filter):
-        dst.last_snapshots = None
         dst.seed = int(np.random.default_rng().integers(0, 2**31 - 1)) if seed is None else seed
     return dst
 
@@ -868,10 +905,14 @@ def dust_likelihood_last_state(obj, select_random_particle=False):
 
 def dust_likelihood_last_snapshots(obj, select_random_particle=False):
     """The whole state at the snapshot times of the last run, following each
-    exported particle's ancestry: (n_snapshots, n_groups, n_particles or 1,
+    exported particle's ancestry (a filter) or as computed (an unfilter,
+    dust2/r/unfilter.hpp:97-128): (n_snapshots, n_groups, n_particles or 1,
     n_state)."""
-    if isinstance(obj, Unfilter) or obj.ptr is None or \
-            getattr(obj, "last_snapshots", None) is None:
+    if isinstance(obj, (Unfilter, ModelFilter)):
+        if select_random_particle and not isinstance(obj, Unfilter):
+            raise ValueError("'select_random_particle' is not supported by this filter")
+        return obj.last_snapshots_state()
+    if obj.ptr is None or getattr(obj, "last_snapshots", None) is None:
         raise DustError("Snapshots are not current")
     n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
     p_out = 1 if select_random_particle else obj.n_particles
@@ -897,11 +938,10 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
             raise DustError("Trajectories are not current")
         n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
         p_out = 1 if select_random_particle else obj.n_particles
-        out = np.zeros((obj.time.size, n_groups, p_out, obj.generator.n_state))
+        out = np.zeros((obj.time.size, n_groups, p_out, _traj_width(obj)))
         call("dust2_filter_last_trajectories", obj.ptr, int(bool(select_random_particle)),
              dptr(out))
-    index_state = getattr(obj, "last_index_state", None)
-    return out if index_state is None else np.ascontiguousarray(out[..., index_state])
+    return out
 
 
 def dust_likelihood_last_trajectories_raw(obj):
@@ -1349,6 +1389,7 @@ class Unfilter:
         self.rows = None
         self.last_index = None
         self.last_index_state = None
+        self.last_snapshots = None
 
     @property
     def initialised(self):
@@ -1377,8 +1418,9 @@ class Unfilter:
         self.ptr = out
 
     def run(self, pars, index_group=None, initial=None, adjoint=False,
-            save_trajectories=False):
+            save_trajectories=False, save_snapshots=None, index_state=None):
         idx, n_idx, n_out = _index_group(index_group, self.n_groups)
+        save = _SaveArgs(save_snapshots, index_state)
         if self.ptr is None:
             if pars is None:
                 raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
@@ -1389,20 +1431,26 @@ class Unfilter:
             call(self._prefix() + "_update_pars", self.ptr, dptr(rows), szptr(idx), n_idx)
             self.rows[sel] = rows
         ll = np.zeros((n_out, self.n_particles))
+        self.last_snapshots = None
+        snaps, n_snaps, states, n_states = save.args
         if self.continuous:
             if initial is not None:
                 raise ValueError("'initial' is not supported for the ODE unfilter")
             # adjoint = True: the gradient by forward sensitivities
             name = "dust2_unfilter_sirode_run_gradient" if adjoint else "dust2_unfilter_sirode_run"
-            call(name, self.ptr, int(bool(save_trajectories)), szptr(idx), n_idx, dptr(ll))
+            call(name, self.ptr, int(bool(save_trajectories)), snaps, n_snaps, states, n_states,
+                 szptr(idx), n_idx, dptr(ll))
         else:
             init = None
             if initial is not None:
                 init = np.ascontiguousarray(np.broadcast_to(
                     f64(initial), (self.n_groups, self.n_particles, 5)))
             call("dust2_unfilter_sir_run", self.ptr, dptr(init), int(bool(save_trajectories)),
-                 int(bool(adjoint)), szptr(idx), n_idx, dptr(ll))
+                 snaps, n_snaps, int(bool(adjoint)), states, n_states, szptr(idx), n_idx,
+                 dptr(ll))
         self.last_index = index_group
+        self.last_index_state = save.states
+        self.last_snapshots = save.snapshots
         if self.n_particles == 1:
             ll = ll[:, 0]
         return float(ll[0]) if (self.n_groups == 1 and ll.ndim == 1 and index_group is None) \
@@ -1434,8 +1482,18 @@ class Unfilter:
         if self.ptr is None:
             raise DustError("Trajectories are not current")
         n = self.n_groups if self.last_index is None else np.size(self.last_index)
-        out = np.zeros((self.time.size, n, self.n_particles, 5))
+        out = np.zeros((self.time.size, n, self.n_particles, _traj_width(self)))
         call(self._prefix() + "_last_trajectories", self.ptr, dptr(out))
+        return out
+
+    def last_snapshots_state(self):
+        """The whole state at the snapshot times of the last run:
+        (n_snapshots, n_groups of the last run, n_particles, n_state)."""
+        if self.ptr is None or self.last_snapshots is None:
+            raise DustError("Snapshots are not current")
+        n = self.n_groups if self.last_index is None else np.size(self.last_index)
+        out = np.zeros((self.last_snapshots.size, n, self.n_particles, 5))
+        call(self._prefix() + "_last_snapshots", self.ptr, dptr(out))
         return out
 
     def last_stats(self):

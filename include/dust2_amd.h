@@ -102,12 +102,23 @@ int dust2_filter_discrete_alloc(const char* model, const double* pars, size_t n_
                                 dust2_filter** out, char* err, size_t errlen);
 
 /* Replaces dust2_filter_run<T> (dust2/r/filter.hpp:21-52) / filter<T>::run
- * (dust2/filter.hpp:120-182).  initial: NULL (model initial state) or
- * [G][P][S].  index_group: NULL (all groups) or n_index group indices.
+ * (dust2/filter.hpp:42-104), with the argument checks of dust_likelihood_run
+ * (R/interface-likelihood.R:83-98); indices are 0-based.
+ *   initial: NULL (model initial state) or [G][P][S].
+ *   save_trajectories: keep the states of index_state at every data time.
+ *   save_snapshots: NULL or n_snapshots data times, strictly increasing, at
+ *     which the whole state is kept ("All elements of 'save_snapshots' must
+ *     be found in 'time'").
+ *   index_state: NULL (every state) or n_index_state distinct states kept
+ *     in the trajectories (trajectories<T>::set_index_and_reset,
+ *     dust2/trajectories.hpp:31-45); snapshots keep every state.
+ *   index_group: NULL (all groups) or n_index distinct group indices.
  * ll_out receives one log-likelihood per selected group, in index order. */
 int dust2_filter_run(dust2_filter* h, const double* initial, int save_trajectories,
-                         const size_t* index_group, size_t n_index, double* ll_out,
-                         char* err, size_t errlen);
+                     const double* save_snapshots, size_t n_snapshots,
+                     const size_t* index_state, size_t n_index_state,
+                     const size_t* index_group, size_t n_index, double* ll_out,
+                     char* err, size_t errlen);
 
 /* Replaces dust2_filter_update_pars<T> (dust2/r/filter.hpp:10-19); sir
  * updates I0, beta and gamma (inst/examples/sir.cpp:89-93); N and
@@ -124,17 +135,11 @@ int dust2_filter_last_state(dust2_filter* h, double* out, char* err, size_t errl
  * each exported particle's trajectory follows its ancestry back from the
  * last time.  select_random_particle != 0 exports one particle per group,
  * chosen once per run from filter stream 2g + 1 (filter<T>::select_random_particle,
- * dust2/filter.hpp:130-138).  out [n_time][n_groups of the last run][P or 1][S].
- * Requires save_trajectories on the last run ("Trajectories are not current"). */
+ * dust2/filter.hpp:130-138).  out [n_time][n_groups of the last run][P or 1]
+ * [n_index_state of the last run, or S].  Requires save_trajectories on the
+ * last run ("Trajectories are not current"). */
 int dust2_filter_last_trajectories(dust2_filter* h, int select_random_particle, double* out,
                                    char* err, size_t errlen);
-
-/* Snapshot times for the next dust2_filter_run (dust_likelihood_run's
- * save_snapshots, R/interface-likelihood.R:83-93): strictly increasing, each
- * one of the data times ("All elements of 'save_snapshots' must be found in
- * 'time'").  The run then keeps every time's state and ancestry. */
-int dust2_filter_set_save_snapshots(dust2_filter* h, const double* times, size_t n,
-                                    char* err, size_t errlen);
 
 /* Replaces dust2_filter_last_snapshots (dust2/r/filter.hpp:97-139) with
  * trajectories<T>::export_snapshots(reorder = true) (dust2/trajectories.hpp:
@@ -274,15 +279,27 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
 /* Replaces dust2_unfilter_run<T> (dust2/r/unfilter.hpp:21-62) /
  * unfilter<T>::run (dust2/unfilter.hpp:42-72): ll_out receives
  * n_particles log-likelihoods per selected group.  save_trajectories keeps
- * the state at every data time (trajectories_.add, dust2/unfilter.hpp:66). */
+ * the states of index_state at every data time (trajectories_.add,
+ * dust2/unfilter.hpp:66); save_snapshots, index_state and index_group as
+ * dust2_filter_run. */
 int dust2_unfilter_sirode_run(dust2_unfilter* h, int save_trajectories,
+                              const double* save_snapshots, size_t n_snapshots,
+                              const size_t* index_state, size_t n_index_state,
                               const size_t* index_group, size_t n_index, double* ll_out,
                               char* err, size_t errlen);
 
 /* Replaces dust2_unfilter_last_trajectories (dust2/r/unfilter.hpp:64-96),
- * no reordering: out [n_time][n_groups of the last run][P][S]. */
+ * no reordering: out [n_time][n_groups of the last run][P][n_index_state
+ * of the last run, or S]. */
 int dust2_unfilter_sirode_last_trajectories(dust2_unfilter* h, double* out, char* err,
                                             size_t errlen);
+/* Replaces dust2_unfilter_last_snapshots (dust2/r/unfilter.hpp:97-128,
+ * registered per model as dust2_unfilter_<name>_last_snapshots,
+ * inst/template/compare.cpp:22): the whole state at the snapshot times of
+ * the last run, out [n_snapshots][n_groups of the last run][P][S];
+ * "Snapshots are not current" if it saved none. */
+int dust2_unfilter_sirode_last_snapshots(dust2_unfilter* h, double* out, char* err,
+                                         size_t errlen);
 int dust2_unfilter_sirode_update_pars(dust2_unfilter* h, const double* pars,
                                       const size_t* index_group, size_t n_index, char* err,
                                       size_t errlen);
@@ -300,6 +317,8 @@ int dust2_unfilter_sirode_last_stats(dust2_unfilter* h, double* out, double* ker
  * discrete adjoint (dust2/unfilter.hpp:77-229); the interface mirrors that
  * one.  FP64 only. */
 int dust2_unfilter_sirode_run_gradient(dust2_unfilter* h, int save_trajectories,
+                                       const double* save_snapshots, size_t n_snapshots,
+                                       const size_t* index_state, size_t n_index_state,
                                        const size_t* index_group, size_t n_index,
                                        double* ll_out, char* err, size_t errlen);
 /* out [n_index of the last run][P][3] = d ll / d (beta, gamma, I0). */
@@ -446,18 +465,24 @@ int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_group
                            const double* control, const uint64_t* seed_state,
                            size_t n_seed_streams, uint64_t seed, dust2_filter_ode** out,
                            char* err, size_t errlen);
-/* initial: [G][P][S] or NULL (the model's initial state); index_group
- * (0-based) and n_index, or NULL for every group; ll_out[n_index] */
+/* initial: [G][P][S] or NULL (the model's initial state); save_snapshots,
+ * index_state and index_group as dust2_filter_run; ll_out[n_index] */
 int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_trajectories,
+                         const double* save_snapshots, size_t n_snapshots,
+                         const size_t* index_state, size_t n_index_state,
                          const size_t* index_group, size_t n_index, double* ll_out, char* err,
                          size_t errlen);
 int dust2_filter_ode_update_pars(dust2_filter_ode* h, const double* pars, char* err,
                                  size_t errlen);
 int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, size_t errlen);
-/* out[n_time][groups of the last run][P][S], each particle's lineage
- * (dust2/trajectories.hpp:128-189) */
+/* out[n_time][groups of the last run][P][n_index_state or S], each
+ * particle's lineage (dust2/trajectories.hpp:128-189) */
 int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
                                        size_t errlen);
+/* out[n_snapshots][groups of the last run][P][S] along the same lineages
+ * (dust2/r/filter.hpp:97-139, dust2/trajectories.hpp:191-283) */
+int dust2_filter_ode_last_snapshots(dust2_filter_ode* h, double* out, char* err,
+                                    size_t errlen);
 /* [G][P + 2][4] words: per group the 2 filter streams, then the particles' */
 int dust2_filter_ode_rng_state(dust2_filter_ode* h, uint64_t* out, char* err, size_t errlen);
 int dust2_filter_ode_set_rng_state(dust2_filter_ode* h, const uint64_t* state, char* err,
@@ -471,14 +496,17 @@ void dust2_filter_ode_free(dust2_filter_ode* h);
  * (dust2/unfilter.hpp:42-72, 162-186): deterministic updates (their draws'
  * means, dust2/r/continuous/unfilter.hpp:38-42), per data time run_to_time
  * and compare_data, one log-likelihood per particle.  Arguments as
- * dust2_filter_ode_alloc; index_group as dust2_filter_ode_run; ll_out[n_index][P];
- * trajectories out[n_time][groups of the last run][P][S]. */
+ * dust2_filter_ode_alloc and dust2_filter_ode_run; ll_out[n_index][P];
+ * trajectories out[n_time][groups of the last run][P][n_index_state or S];
+ * snapshots out[n_snapshots][groups of the last run][P][S]. */
 int dust2_unfilter_ode_alloc(const char* model, const double* pars, size_t n_groups,
                              size_t n_particles, double time_start, const double* time,
                              size_t n_time, const double* data, size_t n_groups_data, double dt,
                              const double* control, dust2_unfilter_ode** out, char* err,
                              size_t errlen);
 int dust2_unfilter_ode_run(dust2_unfilter_ode* h, const double* initial, int save_trajectories,
+                           const double* save_snapshots, size_t n_snapshots,
+                           const size_t* index_state, size_t n_index_state,
                            const size_t* index_group, size_t n_index, double* ll_out,
                            char* err, size_t errlen);
 int dust2_unfilter_ode_update_pars(dust2_unfilter_ode* h, const double* pars, char* err,
@@ -487,6 +515,8 @@ int dust2_unfilter_ode_last_state(dust2_unfilter_ode* h, double* out, char* err,
                                   size_t errlen);
 int dust2_unfilter_ode_last_trajectories(dust2_unfilter_ode* h, double* out, char* err,
                                          size_t errlen);
+int dust2_unfilter_ode_last_snapshots(dust2_unfilter_ode* h, double* out, char* err,
+                                      size_t errlen);
 void dust2_unfilter_ode_free(dust2_unfilter_ode* h);
 
 /* ---- deterministic discrete-time SIR unfilter with the adjoint ----------
@@ -500,19 +530,29 @@ int dust2_unfilter_sir_alloc(const double* pars, size_t n_groups, size_t n_parti
                              dust2_unfilter_sir** out, char* err, size_t errlen);
 
 /* Replaces dust2_unfilter_run<T> (dust2/r/unfilter.hpp:21-62) ->
- * unfilter<T>::run / run_adjoint (dust2/unfilter.hpp:42-114).  initial: NULL
- * (model initial state) or [G][P][S].  adjoint != 0 also computes the
- * gradient of the log-likelihood with respect to (beta, gamma, I0).
- * ll_out: [n_index][P], one log-likelihood per particle. */
+ * unfilter<T>::run / run_adjoint (dust2/unfilter.hpp:42-114), arguments in
+ * the same order.  initial: NULL (model initial state) or [G][P][S].
+ * adjoint != 0 also computes the gradient of the log-likelihood with
+ * respect to (beta, gamma, I0).  save_snapshots, index_state and
+ * index_group as dust2_filter_run.  ll_out: [n_index][P], one
+ * log-likelihood per particle. */
 int dust2_unfilter_sir_run(dust2_unfilter_sir* h, const double* initial, int save_trajectories,
-                           int adjoint, const size_t* index_group, size_t n_index,
-                           double* ll_out, char* err, size_t errlen);
+                           const double* save_snapshots, size_t n_snapshots, int adjoint,
+                           const size_t* index_state, size_t n_index_state,
+                           const size_t* index_group, size_t n_index, double* ll_out,
+                           char* err, size_t errlen);
 
 /* Replaces dust2_unfilter_last_trajectories (dust2/r/unfilter.hpp:64-96):
- * the state at every data time, out [n_time][n_groups of the last run][P][S]
- * (identical with and without the adjoint). */
+ * the states of index_state at every data time, out [n_time][n_groups of
+ * the last run][P][n_index_state or S] (identical with and without the
+ * adjoint). */
 int dust2_unfilter_sir_last_trajectories(dust2_unfilter_sir* h, double* out, char* err,
                                          size_t errlen);
+/* Replaces dust2_unfilter_last_snapshots (dust2/r/unfilter.hpp:97-128,
+ * dust2_unfilter_<name>_last_snapshots of inst/template/compare.cpp:22):
+ * out [n_snapshots][n_groups of the last run][P][S]. */
+int dust2_unfilter_sir_last_snapshots(dust2_unfilter_sir* h, double* out, char* err,
+                                      size_t errlen);
 
 /* Replaces dust2_discrete_unfilter_last_gradient (dust2/r/unfilter.hpp:153-179):
  * out [n_index of the last run][3] = d ll / d (beta, gamma, I0).  Errors as the

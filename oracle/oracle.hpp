@@ -960,8 +960,11 @@ class filter {
         index_(sys.n_groups() * sys.n_particles()) {}
 
   // `ancestry`, when given, receives the n_time x G x P ancestor indices.
+  // ancestry [T][G][P] and traj [T][G][P][S] (the states after each
+  // time's reorder, trajectories_.add of dust2/filter.hpp:89-91), when
+  // given, for the groups of index_group
   void run(bool set_initial, const std::vector<size_t>& index_group,
-           uint32_t* ancestry = nullptr) {
+           uint32_t* ancestry = nullptr, double* traj = nullptr) {
     const size_t P = sys.n_particles();
     std::fill(ll_.begin(), ll_.end(), 0);
     sys.set_time(time_start_);
@@ -994,6 +997,13 @@ class filter {
             ancestry[(t * sys.n_groups() + i) * P + j] =
                 static_cast<uint32_t>(index_[i * P + j]);
           }
+        }
+      }
+      if (traj) {
+        const size_t block = P * T::n_state;
+        for (size_t i : index_group) {
+          std::copy(sys.state().begin() + i * block, sys.state().begin() + (i + 1) * block,
+                    traj + (t * sys.n_groups() + i) * block);
         }
       }
     }

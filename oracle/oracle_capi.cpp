@@ -72,7 +72,7 @@ int filter_run_impl(const double* pars, size_t G, size_t P, double time_start,
                     const uint64_t* rng_in, uint64_t seed, int n_threads,
                     const double* initial, const size_t* index_group,
                     size_t n_index, double* ll_out, double* state_out,
-                    uint64_t* rng_out, uint32_t* ancestry, char* err,
+                    uint64_t* rng_out, uint32_t* ancestry, double* traj, char* err,
                     size_t errlen) {
   try {
     const size_t n_streams = G * (P + 2);
@@ -94,7 +94,7 @@ int filter_run_impl(const double* pars, size_t G, size_t P, double time_start,
       obj.sys.set_state(initial);
     }
     const auto idx = make_index(index_group, n_index, G);
-    obj.run(initial == nullptr, idx, ancestry);
+    obj.run(initial == nullptr, idx, ancestry, traj);
     for (size_t i = 0; i < G; ++i) ll_out[i] = obj.log_likelihood()[i];
     if (state_out) {
       std::copy(obj.sys.state().begin(), obj.sys.state().end(), state_out);
@@ -258,12 +258,12 @@ int oracle_sir_filter_run(const double* pars, size_t G, size_t P,
                           const uint64_t* rng_in, uint64_t seed, int n_threads,
                           const double* initial, const size_t* index_group,
                           size_t n_index, double* ll_out, double* state_out,
-                          uint64_t* rng_out, uint32_t* ancestry, char* err,
-                          size_t errlen) {
+                          uint64_t* rng_out, uint32_t* ancestry, double* traj,
+                          char* err, size_t errlen) {
   return filter_run_impl<sir>(pars, G, P, time_start, time, n_time, dt, data,
                               n_groups_data, rng_in, seed, n_threads, initial,
                               index_group, n_index, ll_out, state_out, rng_out,
-                              ancestry, err, errlen);
+                              ancestry, traj, err, errlen);
 }
 
 int oracle_sir_tv_filter_run(const double* pars, size_t G, size_t P, double time_start,
@@ -271,10 +271,11 @@ int oracle_sir_tv_filter_run(const double* pars, size_t G, size_t P, double time
                              size_t n_groups_data, const uint64_t* rng_in, uint64_t seed,
                              int n_threads, const double* initial, const size_t* index_group,
                              size_t n_index, double* ll_out, double* state_out,
-                             uint64_t* rng_out, uint32_t* ancestry, char* err, size_t errlen) {
+                             uint64_t* rng_out, uint32_t* ancestry, double* traj, char* err,
+                             size_t errlen) {
   return filter_run_impl<sir_tv>(pars, G, P, time_start, time, n_time, dt, data, n_groups_data,
                                  rng_in, seed, n_threads, initial, index_group, n_index, ll_out,
-                                 state_out, rng_out, ancestry, err, errlen);
+                                 state_out, rng_out, ancestry, traj, err, errlen);
 }
 
 int oracle_seir16_filter_run(const double* pars, size_t G, size_t P,
@@ -284,11 +285,11 @@ int oracle_seir16_filter_run(const double* pars, size_t G, size_t P,
                              uint64_t seed, int n_threads, const double* initial,
                              const size_t* index_group, size_t n_index,
                              double* ll_out, double* state_out, uint64_t* rng_out,
-                             uint32_t* ancestry, char* err, size_t errlen) {
+                             uint32_t* ancestry, double* traj, char* err, size_t errlen) {
   return filter_run_impl<seir16>(pars, G, P, time_start, time, n_time, dt, data,
                                  n_groups_data, rng_in, seed, n_threads, initial,
                                  index_group, n_index, ll_out, state_out,
-                                 rng_out, ancestry, err, errlen);
+                                 rng_out, ancestry, traj, err, errlen);
 }
 
 // System simulate (dust2/discrete/system.hpp:121-128): state at each time,

@@ -67,7 +67,7 @@ def oracle():
         o.oracle_sample.argtypes = [C.c_int, U64, C.c_int, C.c_size_t, D, D, D, C.c_char_p,
                                     C.c_size_t]
         filt = [D, C.c_size_t, C.c_size_t, C.c_double, D, C.c_size_t, C.c_double, D,
-                C.c_size_t, U64, C.c_uint64, C.c_int, D, SZ, C.c_size_t, D, D, U64, U32,
+                C.c_size_t, U64, C.c_uint64, C.c_int, D, SZ, C.c_size_t, D, D, U64, U32, D,
                 C.c_char_p, C.c_size_t]
         o.oracle_sir_filter_run.argtypes = filt
         o.oracle_seir16_filter_run.argtypes = filt
@@ -193,9 +193,10 @@ def resample_weight(w, u):
 
 
 def sir_filter(pars, n_particles, time_start, time, dt, data, rng=None, seed=42, n_threads=1,
-               initial=None, index_group=None, ancestry=False, model="sir"):
+               initial=None, index_group=None, ancestry=False, model="sir", traj=False):
     """pars: (G, n_pars); data: (n_time, G_data) (or (n_time, G_data, n_data)).
-    Returns dict(ll, state (G,P,S), rng (G*(P+2),4), ancestry)."""
+    Returns dict(ll, state (G,P,S), rng (G*(P+2),4), ancestry (T,G,P) and
+    traj (T,G,P,S), the states after each time's resampling, when asked)."""
     pars = np.ascontiguousarray(pars, dtype=np.float64)
     G = pars.shape[0]
     P = n_particles
@@ -207,6 +208,7 @@ def sir_filter(pars, n_particles, time_start, time, dt, data, rng=None, seed=42,
     state = np.zeros((G, P, S))
     rng_out = np.zeros((G * (P + 2), 4), dtype=np.uint64)
     anc = np.zeros((time.size, G, P), dtype=np.uint32) if ancestry else None
+    raw = np.zeros((time.size, G, P, S)) if traj else None
     rng_in = None if rng is None else np.ascontiguousarray(rng, dtype=np.uint64)
     init = None if initial is None else np.ascontiguousarray(initial, dtype=np.float64)
     idx = None if index_group is None else np.ascontiguousarray(index_group, dtype=np.uintp)
@@ -216,9 +218,9 @@ def sir_filter(pars, n_particles, time_start, time, dt, data, rng=None, seed=42,
     rc = fn(_dp(pars), G, P, float(time_start), _dp(time), time.size, float(dt), _dp(data),
             n_groups_data, _u64(rng_in), C.c_uint64(seed), n_threads, _dp(init),
             None if idx is None else idx.ctypes.data_as(SZ), 0 if idx is None else idx.size,
-            _dp(ll), _dp(state), _u64(rng_out), _u32(anc), buf, 4096)
+            _dp(ll), _dp(state), _u64(rng_out), _u32(anc), _dp(raw), buf, 4096)
     _check(rc, buf)
-    return {"ll": ll, "state": state, "rng": rng_out, "ancestry": anc}
+    return {"ll": ll, "state": state, "rng": rng_out, "ancestry": anc, "traj": raw}
 
 
 def sir_simulate(pars, n_particles, time_start, dt, times, rng, deterministic=False,

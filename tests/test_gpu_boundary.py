@@ -44,8 +44,8 @@ def test_repeated_index_group_refused_at_the_c_abi(gpu_lib):
     idx = np.array([1, 1], dtype=np.uintp)
     ll = np.zeros(2)
     buf = C.create_string_buffer(1024)
-    rc = _lib.lib().dust2_filter_run(obj.ptr, None, 0, _lib.szptr(idx), 2, _lib.dptr(ll), buf,
-                                     len(buf))
+    rc = _lib.lib().dust2_filter_run(obj.ptr, None, 0, None, 0, None, 0, _lib.szptr(idx), 2,
+                                     _lib.dptr(ll), buf, len(buf))
     assert rc != 0
     assert "must be distinct" in buf.value.decode()
 
