@@ -80,10 +80,17 @@ def test_fp32_state_within_tolerance(gpu_lib):
     data = rng.poisson(6, 365).astype(float)
     obj, ll = make_unfilter(gpu_lib, pars, time, data, precision="fp32")
     ref = ol.sirode_unfilter(pars, 1, 0.0, time, data, n_threads=8)
-    # cumulative cases: relative error within 1e-4 (rtol 1e-6 per step,
-    # FP32 stage arithmetic); log-likelihood within 1e-4 relative
-    np.testing.assert_allclose(obj.last_state()[:, 0, 3], ref["state"][:, 0, 3], rtol=1e-4)
-    np.testing.assert_allclose(ll, ref["ll"], rtol=1e-4)
+    # FP32 state and stages, FP64 error norm and controller, 365 days at
+    # rtol 1e-6 per step: S, R and cumulative cases within 1e-5 relative
+    # (measured 2.7e-6), I and the daily incidence, which fall to ~1e-3 as
+    # epidemics end, within 1e-5 of N absolute (measured 3.6e-6), the
+    # log-likelihood within 1e-5 relative (measured 2.5e-6)
+    st = obj.last_state()
+    for c in [0, 2, 3]:
+        np.testing.assert_allclose(st[:, 0, c], ref["state"][:, 0, c], rtol=1e-5)
+    for c in [1, 4]:
+        np.testing.assert_allclose(st[:, 0, c], ref["state"][:, 0, c], rtol=0, atol=1e-5 * 1000)
+    np.testing.assert_allclose(ll, ref["ll"], rtol=1e-5)
 
 
 def test_replicated_particles_identical(gpu_lib):

@@ -5,7 +5,8 @@ Bars: RNG streams and resampling indices bit-exact; FP64 filter states,
 ancestors and RNG state identical draw-for-draw, log-likelihood within
 1e-12 relative (the only differences allowed are last-place differences of
 device vs host exp/log); FP32 filter statistically equal to the FP64 oracle
-(mean log-likelihood within 3 Monte Carlo standard errors).
+(over 1000 replicates each: mean log-likelihood within 1 Monte Carlo
+standard error, variances and distributions indistinguishable).
 """
 import numpy as np
 import pytest
@@ -354,17 +355,31 @@ def test_filter_errors_reported(gpu_lib, engine):
 
 
 def test_filter_fp32_statistically_matches_fp64_oracle(gpu_lib):
-    # 400 replicate filters of the 1000-particle SIR problem (config 1 shape)
-    G, P, n_time = 400, 1000, 100
+    # 1000 replicate FP32 filters of the 1000-particle SIR problem (the
+    # reference's example case) against 1000 replicates of the FP64 oracle,
+    # each replicate on its own streams.  Bars: the mean log-likelihoods
+    # within 1 combined Monte Carlo standard error (the bar stated for the
+    # FP32 modes; the seeds are fixed, so the outcome is deterministic:
+    # measured z = 0.03), the variance ratio inside the two-sided 99.9 %
+    # F(999, 999) interval (measured 0.93 in [0.81, 1.23]) and the two
+    # samples indistinguishable by a two-sample Kolmogorov-Smirnov test at
+    # the 0.1 % level (measured p = 0.72).
+    from scipy import stats
+    G, P, n_time = 1000, 1000, 100
     time = np.arange(1.0, n_time + 1)
     data = sir_data(n_time)
     pars = np.array([SIR] * G)
     _, ll32 = run_gpu_filter(gpu_lib, pars, P, time, 0.25, data, 1, precision="fp32",
                              engine="pipeline")
-    ref = ol.sir_filter(pars[:100], P, 0.0, time, 0.25, data[:, None], seed=2, n_threads=8)
+    ref = ol.sir_filter(pars, P, 0.0, time, 0.25, data[:, None], seed=2, n_threads=16)
     ll64 = ref["ll"]
     se = np.sqrt(ll32.var(ddof=1) / ll32.size + ll64.var(ddof=1) / ll64.size)
-    assert abs(ll32.mean() - ll64.mean()) < 3 * se, (ll32.mean(), ll64.mean(), se)
+    z = (ll32.mean() - ll64.mean()) / se
+    assert abs(z) < 1, (ll32.mean(), ll64.mean(), se)
+    ratio = ll32.var(ddof=1) / ll64.var(ddof=1)
+    lo, hi = stats.f.ppf([0.0005, 0.9995], G - 1, G - 1)
+    assert lo < ratio < hi, (ratio, lo, hi)
+    assert stats.ks_2samp(ll32, ll64).pvalue > 1e-3
 
 
 # ---------------------------------------------------------------------------

@@ -1,6 +1,7 @@
 """GPU checks of the age-structured SEIR16 filter (FP32, force of infection
 on MFMA).  No reference model exists: pinned by the CPU oracle's seir16
-(mean log-likelihood within 3 Monte Carlo standard errors), exact
+(over 1000 replicates, mean log-likelihood within 1 Monte Carlo standard
+error of the FP64 filter, which is the oracle draw for draw), exact
 invariants (population conservation per age, weekly incidence sums) and a
 contact structure whose transpose would infect a different age group."""
 import numpy as np
@@ -68,19 +69,39 @@ def test_force_of_infection_uses_contact_rows(gpu_lib):
 
 
 def test_mean_log_likelihood_matches_fp64_oracle(gpu_lib):
-    G, P = 64, 256
+    # 1000 replicate FP32 filters (256 particles, 6 weeks) against 1000
+    # replicates of the FP64 filter, which is the oracle draw for draw
+    # (test_gpu_discrete_models.py); 24 further replicates run on the
+    # oracle itself.  Bars: the FP32 and FP64 means within 1 combined Monte
+    # Carlo standard error (the bar stated for the FP32 modes; fixed seeds,
+    # so the outcome is deterministic), the variance ratio inside the
+    # two-sided 99.9 % F(999, 999) interval, the distributions
+    # indistinguishable by a two-sample Kolmogorov-Smirnov test at 0.1 %,
+    # and the oracle's own mean within 3 standard errors of the FP32 mean.
+    from scipy import stats
+    G, P = 1000, 256
     time = np.arange(7.0, 7.0 * 7, 7.0)
     p = seir_pars()
     # observations: one oracle simulation of the model
     sim, _ = ol.sir_simulate(row(p)[None, :], 1, 0.0, 0.25, time, ol.prng(1, seed=9),
                              model="seir16")
     data = sim[:, 0, 0, 64:80].copy()
-    _, ll_gpu = run(gpu_lib, [p] * G, P, time, data, 1)
+    _, ll32 = run(gpu_lib, [p] * G, P, time, data, 1)
+    obj = gpu_lib.dust_filter_create(gpu_lib.seir16(), 0.0, {"time": time, "incidence": data},
+                                     P, n_groups=G, dt=0.25, seed=2, precision="fp64")
+    ll64 = np.atleast_1d(gpu_lib.dust_likelihood_run(obj, [p] * G))
+    se = np.sqrt(ll32.var(ddof=1) / G + ll64.var(ddof=1) / G)
+    z = (ll32.mean() - ll64.mean()) / se
+    assert abs(z) < 1, (ll32.mean(), ll64.mean(), se)
+    lo, hi = stats.f.ppf([0.0005, 0.9995], G - 1, G - 1)
+    assert lo < ll32.var(ddof=1) / ll64.var(ddof=1) < hi
+    assert stats.ks_2samp(ll32, ll64).pvalue > 1e-3
     ref = ol.sir_filter(np.tile(row(p), (24, 1)), P, 0.0, time, 0.25, data[:, None, :],
-                        seed=2, n_threads=16, model="seir16")
-    ll_cpu = ref["ll"]
-    se = np.sqrt(ll_gpu.var(ddof=1) / G + ll_cpu.var(ddof=1) / ll_cpu.size)
-    assert abs(ll_gpu.mean() - ll_cpu.mean()) < 3 * se + 1e-9, (ll_gpu.mean(), ll_cpu.mean(), se)
+                        seed=3, n_threads=16, model="seir16")["ll"]
+    se_ref = np.sqrt(ll32.var(ddof=1) / G + ref.var(ddof=1) / ref.size)
+    assert abs(ll32.mean() - ref.mean()) < 3 * se_ref, (ll32.mean(), ref.mean(), se_ref)
+    print(f"SEIR16 FP32 - FP64 over {G} replicates: z {z:.3f}, variance ratio "
+          f"{ll32.var(ddof=1) / ll64.var(ddof=1):.3f}")
 
 
 def test_invalid_rate_reports_errors(gpu_lib):
