@@ -2438,8 +2438,8 @@ struct dust2_unfilter {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
   dbuf d_critical;
-  dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_ll, d_stats, d_errors, d_counters, d_active,
-      d_grad, d_traj;
+  dbuf d_pars, d_times, d_obs, d_lfact, d_state, d_stats, d_errors, d_counters, d_active, d_grad,
+      d_traj;
   // the error count and the log-likelihoods of a run come back in one
   // stream synchronisation through pinned memory
   pinned h_result;
@@ -2553,7 +2553,6 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
     h->d_obs.alloc(nd * sizeof(double));
     h->d_lfact.alloc(nd * sizeof(double));
     h->d_state.alloc(n_total * 5 * h->real_size());
-    h->d_ll.alloc(n_total * sizeof(double));
     h->d_stats.alloc(n_total * 4 * sizeof(double));
     h->d_errors.alloc(n_total * sizeof(error_record));
     h->d_counters.alloc(4 * sizeof(uint32_t));
@@ -2563,7 +2562,6 @@ int dust2_unfilter_sirode_alloc(const double* pars, size_t n_groups, size_t n_pa
     to_device(h->d_obs, data, nd * sizeof(double), h->stream);
     to_device(h->d_lfact, lfact.data(), nd * sizeof(double), h->stream);
     hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), h->stream), "memset");
-    hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), h->stream), "memset");
     hip_check(hipMemsetAsync(h->d_state.get(), 0, h->d_state.bytes(), h->stream), "memset");
     hip_check(hipStreamSynchronize(h->stream), "sync");
     *out = h.release();
@@ -2610,21 +2608,20 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
   a.obs_group_stride = h->n_groups_data == 1 ? 0 : 1;
   a.control = h->control;
   a.state = h->d_state.get();
-  a.ll = h->d_ll.get<double>();
+  // each particle's log-likelihood is written once, at the end of its
+  // run, straight into pinned host memory (no copy after the kernel)
+  const size_t n_ll = h->G * h->P;
+  if (h->h_result.bytes() < 256 + n_ll * sizeof(double)) {
+    h->h_result.alloc(256 + n_ll * sizeof(double));
+  }
+  a.ll = reinterpret_cast<double*>(h->h_result.get<char>() + 256);
   a.stats = h->d_stats.get<double>();
   a.errors = h->d_errors.get<error_record>();
   a.error_count = h->d_counters.get<uint32_t>();
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_dp5_unfilter(a, h->fp64, s), "dp5 unfilter kernel");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
-  const size_t n_ll = h->G * h->P;
-  if (h->h_result.bytes() < 256 + n_ll * sizeof(double)) {
-    h->h_result.alloc(256 + n_ll * sizeof(double));
-  }
   hip_check(hipMemcpyAsync(h->h_result.get(), h->d_counters.get(), sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s),
-            "copy to host");
-  hip_check(hipMemcpyAsync(h->h_result.get<char>() + 256, h->d_ll.get(), n_ll * sizeof(double),
                            hipMemcpyDeviceToHost, s),
             "copy to host");
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
