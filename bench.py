@@ -174,10 +174,11 @@ def timed_runs(lib, obj, plist, steps, warmup):
 
 def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, key=None):
     b_obs = 2 * S * s_x + 2 * 32 + 2 * 8 + 2 * 4
+    real = "float" if s_x == 4 else "double"
     if engine_used == "persistent":
         b_launch = G * P * n_obs * b_obs
-        kernel = "sir_filter_kernel<%s, %d, %d>" % (
-            ("float" if s_x == 4 else "double",) + lib.dust_likelihood_last_stats(obj)["launch"])
+        kernel = "filter_kernel<sir_model, %s, %d, %d>" % (
+            (real,) + lib.dust_likelihood_last_stats(obj)["launch"])
         return roofline(b_launch / (run_ms * 1e-3) / 1e9, kernel, run_ms, b_launch, b_obs,
                         key)
     # pipeline: the advance kernel moves state, RNG, weight write and
@@ -187,7 +188,12 @@ def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, k
     k_ms = lib.dust_likelihood_last_stats(obj)["advance_ms"]
     obj.set_engine("pipeline")
     b_launch = G * P * (2 * S * s_x + 2 * 32 + 8 + 4)
-    name = "seir_advance_kernel" if S > 5 else "sir_advance_kernel"
+    # the kernel's template arguments: model, real, workgroup size and the
+    # minimum waves per SIMD of its launch bounds (discrete_kernels.hip)
+    if S > 5:
+        name = "advance_kernel<seir16_model, %s, %s>" % (real, "64, 3" if s_x == 4 else "256, 1")
+    else:
+        name = "advance_kernel<sir_model, %s, 256, 1>" % real
     out = roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs, key)
     # the whole run (advance + resample launches) against the same bytes
     out["run_ms"] = run_ms
