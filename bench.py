@@ -85,6 +85,24 @@ def median_runs(run, g, n_runs):
     return float(np.median(times)), outs
 
 
+def cpu_protocol(run, g_max, per_g, unit, n_runs, what, target_s=3.0, target_1_s=2.0):
+    """The oracle's rate at nproc threads and at one thread: run(g, k, n_threads)
+    does g work units (per_g units of `unit` each) with replicate k.  The
+    sizing runs are the warm-up; then the median of n_runs runs."""
+    threads = cpu_threads()
+    g = sized_sample(lambda n: run(n, 0, threads), g_max, target_s)
+    el, outs = median_runs(lambda n, k: run(n, k, threads), g, n_runs)
+    g1 = sized_sample(lambda n: run(n, 0, 1), g_max, target_1_s)
+    el1, _ = median_runs(lambda n, k: run(n, k, 1), g1, n_runs)
+    return {"value": g * per_g / el, "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{what(g)}, FP64 oracle (OpenMP, {threads} threads = nproc), "
+                      f"median of {n_runs} runs of {el:.2f} s",
+            "one_thread": g1 * per_g / el1,
+            "one_thread_sample": f"{what(g1)}, median of {n_runs} runs of {el1:.2f} s",
+            "cpu_model": cpu_model(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}, g, outs
+
+
 def oracle():
     """The CPU oracle, used only for the cpu_baseline leg."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -269,27 +287,15 @@ def bench_sir(lib, args):
     out["batched_over_plain"] = out["likelihood_evals_per_s"] / out["plain_evals_per_s"]
     if not args.no_cpu:
         ol = oracle()
-        threads = cpu_threads()
 
-        def cpu_run(g, k=0, n_threads=threads):
+        def cpu_run(g, k, n_threads):
             return ol.sir_filter(pars[:g], P, 0.0, time, dt, data[:, :g], seed=1 + k,
                                  n_threads=n_threads)["ll"]
 
-        # nproc threads: about 3 s per run, median of 3 (seeds 1, 2, 3)
-        g = sized_sample(lambda n: cpu_run(n), G, 3.0)
-        el, ref = median_runs(cpu_run, g, 3)
-        # one thread: about 2 s per run, median of 3
-        g1 = sized_sample(lambda n: cpu_run(n, 0, 1), G, 2.0)
-        el1, _ = median_runs(lambda n, k: cpu_run(n, k, 1), g1, 3)
-        out["cpu_baseline"] = {
-            "value": g * P * n_steps / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{g} of {G} groups x {P} particles x {time.size} obs, FP64 oracle "
-                      f"(OpenMP over group x particle, {threads} threads = nproc), median of "
-                      f"3 runs of {el:.1f} s",
-            "one_thread": g1 * P * n_steps / el1,
-            "one_thread_sample": f"{g1} groups, median of 3 runs of {el1:.1f} s",
-            "cpu_model": cpu_model(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+        # about 3 s per run at nproc threads, 2 s at one; median of 3 (seeds 1, 2, 3)
+        out["cpu_baseline"], g, ref = cpu_protocol(
+            cpu_run, G, P * n_steps, "particle-steps/s", 3,
+            lambda n: f"{n} of {G} groups x {P} particles x {time.size} obs")
         # FP32 batched filter against the FP64 oracle on the sampled groups:
         # K GPU replicates of each (one batched launch of K * g groups)
         # against the oracle's 3 replicates; z = sum of the per-group mean
@@ -350,18 +356,29 @@ def bench_sir_replicates(lib, args):
            "ll_mc_se": float(ll.std(ddof=1) / np.sqrt(G)), "roofline": rl}
     if not args.no_cpu:
         ol = oracle()
-        threads = cpu_threads()
-        n_ref = 200
-        t0 = _time.perf_counter()
-        ref = ol.sir_filter(np.tile(row, (n_ref, 1)), P, 0.0, time, dt, data[:, None], seed=7,
-                            n_threads=threads)["ll"]
-        el = _time.perf_counter() - t0
-        se = float(np.sqrt(ll.var(ddof=1) / G + ref.var(ddof=1) / n_ref))
+
+        def cpu_run(g, k, n_threads):
+            return ol.sir_filter(np.tile(row, (g, 1)), P, 0.0, time, dt, data[:, None],
+                                 seed=100 + k, n_threads=n_threads)["ll"]
+
+        # about 3 s per run at nproc threads, 2 s at one; median of 5 (seeds 100 to 104)
+        out["cpu_baseline"], g, runs = cpu_protocol(
+            cpu_run, G, 1, "evals/s", 5, lambda n: f"{n} replicate filters")
+        # the timed runs' streams (seed 1) are independent of the oracle's
+        ref = np.concatenate(runs)
+        se = float(np.sqrt(ll.var(ddof=1) / G + ref.var(ddof=1) / ref.size))
+        out["oracle_replicates"] = int(ref.size)
         out["oracle_ll_mean"] = float(ref.mean())
         out["delta_ll_in_mc_se"] = float((ll.mean() - ref.mean()) / se)
-        out["cpu_baseline"] = {"value": n_ref / el, "unit": "evals/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"{n_ref} replicate filters, FP64 oracle, {el:.1f} s"}
+        if prec == "fp64":
+            # same streams: a fresh filter seeded as the oracle's first run
+            # reproduces its g replicates draw for draw
+            same = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
+                                          n_groups=G, dt=dt, seed=100, precision=prec)
+            ll_same = np.asarray(lib.dust_likelihood_run(same, plist))[:g]
+            del same
+            out["same_streams_max_rel_ll"] = float(np.max(np.abs(ll_same - runs[0])
+                                                          / np.abs(runs[0])))
     return out
 
 
@@ -417,19 +434,20 @@ def bench_sirode(lib, args):
     if not args.no_cpu:
         ol = oracle()
         threads = cpu_threads()
-        m = n
-        t0 = _time.perf_counter()
-        ref = ol.sirode_unfilter(pars[:m], 1, 0.0, time, data, n_threads=threads)
-        el = _time.perf_counter() - t0
+
+        def cpu_run(g, k, n_threads):
+            return ol.sirode_unfilter(pars[:g], 1, 0.0, time, data, n_threads=n_threads)
+
+        # up to all n sets per run at nproc threads (about 2 s at one); median of 5
+        out["cpu_baseline"], m, runs = cpu_protocol(
+            cpu_run, n, 1, "evals/s", 5, lambda g: f"{g} of {n} parameter sets")
+        ref = runs[0]
         out["fp64_vs_oracle_max_rel_ll"] = float(np.max(np.abs(res["fp64"]["ll"][:m] - ref["ll"])
                                                         / np.abs(ref["ll"])))
         k = 4096
         _, ref_grad = ol.sirode_gradient(pars[:k], 0.0, time, data, n_threads=threads)
         out["gradient_vs_oracle_max_rel"] = float(np.max(np.abs(grad[:k] - ref_grad)
                                                          / np.abs(ref_grad)))
-        out["cpu_baseline"] = {"value": m / el, "unit": "evals/s", "cores": threads,
-                               "kind": "port", "sample": f"{m} parameter sets, FP64 oracle, "
-                                                         f"{el:.1f} s"}
     return out
 
 
@@ -475,25 +493,20 @@ def bench_seir16(lib, args):
            "mean_ll": float(np.mean(ll)), "roofline": rl}
     if not args.no_cpu:
         ol = oracle()
-        threads = cpu_threads()
         rows = np.array([np.concatenate([[q["beta"], q["sigma"], q["gamma"], q["exp_noise"],
                                           q["I0"]], q["N"], np.ravel(q["C"])])
                          for q in plist[:1]])
-        n = 10
-        while True:
-            t0 = _time.perf_counter()
-            ol.sir_filter(rows, P, 0.0, time[:n], dt, data[:n, :1, :], seed=1,
-                          n_threads=threads, model="seir16")
-            el = _time.perf_counter() - t0
-            if el > 3 or n >= time.size:
-                break
-            n = min(time.size, n * 2)
-        steps = int(round(time[n - 1] / dt))
-        out["cpu_baseline"] = {
-            "value": P * steps / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"1 of {G} groups x {P} particles x {n} of {time.size} weekly obs, "
-                      f"FP64 oracle, {el:.1f} s"}
+
+        def cpu_run(p, k, n_threads):
+            return ol.sir_filter(rows, p, 0.0, time, dt, data[:, :1, :], seed=1 + k,
+                                 n_threads=n_threads, model="seir16")
+
+        # one group over all the weekly observations with p of its particles
+        # (so that the sample follows the whole epidemic); up to P particles
+        # per run at nproc threads, about 2 s at one; median of 3
+        out["cpu_baseline"], _, _ = cpu_protocol(
+            cpu_run, P, int(round(time[-1] / dt)), "particle-steps/s", 3,
+            lambda p: f"1 of {G} groups x {p} of {P} particles x {time.size} weekly obs")
     return out
 
 
