@@ -2587,7 +2587,6 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
     to_device(h->d_active, active.data(), h->G, s);
     h->active_idx = idx;
   }
-  hip_check(hipMemsetAsync(h->d_counters.get(), 0, h->d_counters.bytes(), s), "memset");
   ode_launch a{};
   a.sens_param = sens_param;
   a.grad = sens_param >= 0 ? h->d_grad.get<double>() : nullptr;
@@ -2617,21 +2616,24 @@ void sirode_launch(dust2_unfilter* h, const std::vector<size_t>& idx, int sens_p
   a.ll = reinterpret_cast<double*>(h->h_result.get<char>() + 256);
   a.stats = h->d_stats.get<double>();
   a.errors = h->d_errors.get<error_record>();
-  a.error_count = h->d_counters.get<uint32_t>();
+  // the error flag also lives in pinned memory: cleared here by the host
+  // before the launch is queued, set by a failing particle, read after the
+  // synchronisation (no memset or copy around the kernel)
+  volatile uint32_t* flag = h->h_result.get<uint32_t>();
+  *flag = 0;
+  a.error_flag = h->h_result.get<uint32_t>();
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_dp5_unfilter(a, h->fp64, s), "dp5 unfilter kernel");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
-  hip_check(hipMemcpyAsync(h->h_result.get(), h->d_counters.get(), sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, s),
-            "copy to host");
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-  const uint32_t count = *h->h_result.get<uint32_t>();
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
   h->last_ms = ms;
-  if (count > 0) {
+  if (*flag != 0) {
     std::vector<error_record> rec(h->G * h->P);
     to_host(rec.data(), h->d_errors, rec.size() * sizeof(error_record), s);
+    size_t count = 0;
+    for (const error_record& r : rec) count += r.code != ERR_NONE ? 1 : 0;
     hip_check(hipMemsetAsync(h->d_errors.get(), 0, h->d_errors.bytes(), s), "memset");
     hip_check(hipStreamSynchronize(s), "sync");
     throw std::runtime_error(error_report(rec, count));
@@ -5112,7 +5114,7 @@ void srv_launch_ode(dust2_server* sv) {
   a.ll = reinterpret_cast<double*>(dout);
   a.errors = reinterpret_cast<error_record*>(dout + b_ll);
   a.state = dout + b_ll + b_err;
-  a.error_count = reinterpret_cast<uint32_t*>(dout + b_ll + b_err + b_state);
+  a.error_flag = reinterpret_cast<uint32_t*>(dout + b_ll + b_err + b_state);
   hip_check(launch_dp5_unfilter(a, true, s), "dp5 serving kernel");
   hip_check(hipMemcpyAsync(o.h_out.get(), dout, b_ll + b_err, hipMemcpyDeviceToHost, s),
             "result copy");

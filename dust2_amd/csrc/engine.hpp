@@ -237,7 +237,9 @@ struct ode_launch {
   double* ll;                // [n_total]
   double* stats;             // [n_total][4] or null
   error_record* errors;      // [n_total]
-  uint32_t* error_count;
+  // set to 1 (a plain store) by every particle that fails; zeroed by the
+  // caller; may be pinned host memory
+  uint32_t* error_flag;
   // optional per-group data series (serving): group g reads times, obs and
   // lfact from offset series_offset[g], series_len[g] values, one value per
   // time; null for one series shared through obs_stride/obs_group_stride

@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
   }
   if (code) {
     args.errors[i] = error_record{code, 0u, 0.0, 0.0};
-    atomicAdd(args.error_count, 1u);
+    *args.error_flag = 1u;
   }
 }
 

@@ -109,6 +109,23 @@ def test_too_many_steps_error(gpu_lib):
     with pytest.raises(gpu_lib.DustError, match="too many steps"):
         make_unfilter(gpu_lib, pars, np.arange(1.0, 11.0), np.ones(10),
                       control=gpu_lib.dust_ode_control(max_steps=5))
+    # every failing particle is counted, on each of two failing runs (the
+    # error flag and records are cleared between runs), and only the
+    # selected groups' particles when a subset runs
+    pars = np.array([[1000, 10, 0.3, 0.1]] * 3)
+    time = np.arange(1.0, 11.0)
+    obj = gpu_lib.dust_unfilter_create(gpu_lib.sirode(), 0.0,
+                                       {"time": time, "incidence": np.ones(10)},
+                                       n_particles=2, n_groups=3,
+                                       ode_control=gpu_lib.dust_ode_control(max_steps=5))
+    plist = [dict(zip(["N", "I0", "beta", "gamma"], p)) for p in pars]
+    for _ in range(2):
+        with pytest.raises(gpu_lib.DustError) as e:
+            obj.run(plist)
+        assert str(e.value).startswith("6 particles reported errors.")
+        assert "too many steps" in str(e.value)
+    with pytest.raises(gpu_lib.DustError, match="^2 particles reported errors."):
+        obj.run(None, index_group=[1])
 
 
 def test_gradient_matches_oracle_and_keeps_likelihood(gpu_lib):
