@@ -32,15 +32,33 @@ __host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) {
   return (x << k) | (x >> (64 - k));
 }
 
-__host__ __device__ __forceinline__ uint64_t rng_next(rng256& r) {
+// One xoshiro256+ step.  The update written on 32-bit halves with every
+// new word a three-input XOR of old words (one v_bitop3_b32 per half, truth
+// table 0x96) and the rotation by 45 as two funnel shifts: 12 vector
+// instructions per step instead of 15.
+//   s2' = s2 ^ s0 ^ (s1 << 17)    s1' = s1 ^ s2 ^ s0
+//   s0' = s0 ^ s3 ^ s1            s3' = rotl(s3 ^ s1, 45)
+__device__ __forceinline__ uint32_t xor3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+__device__ __forceinline__ uint64_t join_u32(uint32_t hi, uint32_t lo) {
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t rng_next(rng256& r) {
   const uint64_t out = r.s0 + r.s3;
   const uint64_t t = r.s1 << 17;
-  r.s2 ^= r.s0;
-  r.s3 ^= r.s1;
-  r.s1 ^= r.s2;
-  r.s0 ^= r.s3;
-  r.s2 ^= t;
-  r.s3 = rotl64(r.s3, 45);
+  const uint32_t l0 = static_cast<uint32_t>(r.s0), h0 = static_cast<uint32_t>(r.s0 >> 32);
+  const uint32_t l1 = static_cast<uint32_t>(r.s1), h1 = static_cast<uint32_t>(r.s1 >> 32);
+  const uint32_t l2 = static_cast<uint32_t>(r.s2), h2 = static_cast<uint32_t>(r.s2 >> 32);
+  const uint32_t l3 = static_cast<uint32_t>(r.s3), h3 = static_cast<uint32_t>(r.s3 >> 32);
+  const uint32_t lt = static_cast<uint32_t>(t), ht = static_cast<uint32_t>(t >> 32);
+  r.s2 = join_u32(xor3_u32(h2, h0, ht), xor3_u32(l2, l0, lt));
+  r.s1 = join_u32(xor3_u32(h1, h2, h0), xor3_u32(l1, l2, l0));
+  r.s0 = join_u32(xor3_u32(h0, h3, h1), xor3_u32(l0, l3, l1));
+  const uint32_t l = l3 ^ l1, h = h3 ^ h1;
+  // rotl by 45 = rotr by 19: low word (l >> 19) | (h << 13), high word
+  // (h >> 19) | (l << 13)
+  r.s3 = join_u32(__builtin_amdgcn_alignbit(l, h, 19), __builtin_amdgcn_alignbit(h, l, 19));
   return out;
 }
 
@@ -268,15 +286,26 @@ __constant__ constexpr inv_int_table kInvInt{};
 // lane that the rest of its wave waits on.  Restarting at 110 instead gives
 // the same draw sequence.
 //
-// The search is uniform across the wavefront: every lane steps j together
-// and a lane whose search has ended keeps its values through selects, so
-// the loop needs no per-lane exit bookkeeping (which made the divergent
+// j as a float for j = 0..115, read by scalar load beside 1/j.
+struct int_float_table {
+  float v[116];
+  constexpr int_float_table() : v() {
+    for (int j = 0; j < 116; ++j) v[j] = static_cast<float>(j);
+  }
+};
+__constant__ constexpr int_float_table kIntF{};
+
+// The search is uniform across the wavefront: every lane steps j together,
+// so the loop needs no per-lane exit bookkeeping (which made the divergent
 // form scalar-bound: about 14 scalar instructions per step against 8
-// vector ones).  Four steps per trip, the wavefront leaving when no lane is
-// searching; 1/j comes from a constant table, uniform across the wavefront.
-// Each lane's draw is that of the per-lane search.  A lane searching at the
-// start of a trip has k = j - 1 <= 110, so the trip reads entries up to
-// j + 3 <= 114.
+// vector ones).  A lane whose search has ended keeps stepping its u and f
+// (bounded: f stays a pmf value, or a value of magnitude at most r <= 1
+// times one past k = n) but no longer moves its k, which is all it
+// returns: five vector instructions per step.  Four steps per trip, the
+// wavefront leaving when no lane is searching; 1/j and j come from
+// constant tables, uniform across the wavefront.  Each lane's draw is that
+// of the per-lane search.  A lane searching at the start of a trip has
+// k = j - 1 <= 110, so the trip reads entries up to j + 3 <= 114.
 __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
   const float q = 1 - p;
   const float r = p * f_rcp(q);
@@ -292,13 +321,11 @@ __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float inv_j = kInvInt.v[j + i];
-        const float u_next = u - f;
-        const float f_next = f * (g * inv_j - r);
-        const float k_next = static_cast<float>(j + i);
-        u = act ? u_next : u;
-        f = act ? f_next : f;
-        k = act ? k_next : k;
-        act = act && u >= f && k <= k_max;
+        const float jf = kIntF.v[j + i];
+        u = u - f;
+        f = f * __builtin_fmaf(g, inv_j, -r);
+        k = act ? jf : k;
+        act = act && u >= f && jf <= k_max;
       }
     }
     if (k <= k_max) {
@@ -313,42 +340,61 @@ __device__ inline float binomial_inversion_f32(rng256& rng, float n, float p) {
 //                             + tail(m) + tail(n-m) - tail(k) - tail(n-k),  d = k - m,
 // the same quantity as the reference form (Stirling's formula for the
 // factorials) with the n*log(n)-sized terms cancelled analytically.
-__device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
+struct btrs_f32 {
+  float n, p;                        // trials and success probability (p <= 1/2)
+  float b, a, c, two_a, v_r, alpha;  // envelope and squeeze
+  float m, rm, rnm, L, tail_m;       // mode terms of the acceptance test
+};
+
+__device__ __forceinline__ btrs_f32 btrs_setup_f32(float n, float p) {
+  btrs_f32 s;
   const float q = 1 - p;
   const float spq = __builtin_amdgcn_sqrtf(n * p * q);
-  const float b = 1.15f + 2.53f * spq;
-  const float a = -0.0873f + 0.0248f * b + 0.01f * p;
-  const float c = n * p + 0.5f;
-  const float rb = f_rcp(b);
-  const float v_r = 0.92f - 4.2f * rb;
-  const float two_a = 2 * a;
+  s.n = n;
+  s.p = p;
+  s.b = 1.15f + 2.53f * spq;
+  s.a = -0.0873f + 0.0248f * s.b + 0.01f * p;
+  s.c = n * p + 0.5f;
+  const float rb = f_rcp(s.b);
+  s.v_r = 0.92f - 4.2f * rb;
+  s.two_a = 2 * s.a;
+  s.alpha = (2.83f + 5.1f * rb) * spq;
+  s.m = floorf((n + 1) * p);
+  s.rm = f_rcp(s.m + 1);
+  s.rnm = f_rcp(n - s.m + 1);
+  s.L = f_log(p * (n - s.m + 1) * f_rcp(q * (s.m + 1)));
+  s.tail_m = stirling_tail_f32(s.m) + stirling_tail_f32(n - s.m);
+  return s;
+}
+
+// One BTRS proposal: draws one (u, v) pair; true with the draw in k when
+// accepted (by the squeeze or by the full test).
+__device__ __forceinline__ bool btrs_try_f32(const btrs_f32& s, rng256& rng, float& k) {
+  float u, v;
+  random_pair_f32(rng, u, v);
+  u -= 0.5f;
+  const float us = 0.5f - fabsf(u);
+  k = floorf((s.two_a * f_rcp(us) + s.b) * u + s.c);
+  bool accept = us >= 0.07f && v <= s.v_r;
+  if (!accept && k >= 0 && k <= s.n) {
+    const float d = k - s.m;
+    const float ratio = d * s.L - (k + 0.5f) * f_log1p(d * s.rm) -
+                        (s.n - k + 0.5f) * f_log1p(-d * s.rnm) + s.tail_m -
+                        stirling_tail_f32(k) - stirling_tail_f32(s.n - k);
+    const float us2 = us * us;
+    const float lv = f_log(v * s.alpha * us2 * f_rcp(s.a + s.b * us2));
+    accept = lv <= ratio;
+  }
+  return accept;
+}
+
+__device__ inline float binomial_btrs_f32(rng256& rng, float n, float p) {
+  const btrs_f32 s = btrs_setup_f32(n, p);
   // one exit test per proposal (cheap divergent control flow); every
   // proposal draws one (u, v) pair
   float k;
-  bool accept;
-  do {
-    float u, v;
-    random_pair_f32(rng, u, v);
-    u -= 0.5f;
-    const float us = 0.5f - fabsf(u);
-    k = floorf((two_a * f_rcp(us) + b) * u + c);
-    accept = us >= 0.07f && v <= v_r;
-    if (!accept && k >= 0 && k <= n) {
-      const float alpha = (2.83f + 5.1f * rb) * spq;
-      const float m = floorf((n + 1) * p);
-      const float d = k - m;
-      const float rm = f_rcp(m + 1);
-      const float rnm = f_rcp(n - m + 1);
-      const float L = f_log(p * (n - m + 1) * f_rcp(q * (m + 1)));
-      const float ratio = d * L - (k + 0.5f) * f_log1p(d * rm) -
-                          (n - k + 0.5f) * f_log1p(-d * rnm) + stirling_tail_f32(m) +
-                          stirling_tail_f32(n - m) - stirling_tail_f32(k) -
-                          stirling_tail_f32(n - k);
-      const float us2 = us * us;
-      const float lv = f_log(v * alpha * us2 * f_rcp(a + b * us2));
-      accept = lv <= ratio;
-    }
-  } while (!accept);
+  while (!btrs_try_f32(s, rng, k)) {
+  }
   return k;
 }
 
@@ -381,6 +427,7 @@ __device__ __forceinline__ float binomial_core<float>(rng256& rng, float n, floa
   rng = d.rng;
   return d.k;
 }
+
 
 // Binomial(n, p).  Returns ERR_BINOMIAL in *err (and 0) on invalid input,
 // as the reference throws; deterministic streams return n*p.
