@@ -160,8 +160,9 @@ _PROTOS = {
                                        C.c_size_t, D, ERR, ERRLEN]),
     "dust2_filter_ode_update_pars": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_state": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_ode_last_trajectories": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_filter_ode_last_snapshots": (C.c_int, [VP, D, ERR, ERRLEN]),
+    "dust2_filter_ode_last_trajectories": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
+    "dust2_filter_ode_last_snapshots": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
+    "dust2_filter_ode_last_state_selected": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_ode_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_free": (None, [VP]),

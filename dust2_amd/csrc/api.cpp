@@ -1437,6 +1437,7 @@ const uint32_t* lineage_groups(dust2_filter* h, bool select) {
       to_device(d_todo, todo.data(), todo.size() * sizeof(uint32_t), h->stream);
       hip_check(launch_select_particle(h->d_rng.get<uint64_t>(), d_todo.get<uint32_t>(),
                                        static_cast<int>(todo.size()), static_cast<int>(h->P),
+                                       4 * (h->P + 2), 1, h->P + 2,
                                        h->d_select.get<uint32_t>(), h->stream),
                 "select particle kernel");
       hip_check(hipStreamSynchronize(h->stream), "sync");
@@ -3765,8 +3766,9 @@ struct dust2_filter_ode {
   double time_start = 0;
   std::vector<double> times;
   dbuf d_data, d_frng, d_ll, d_ll_step, d_idx, d_state2, d_words2, d_traj, d_ancestry,
-      d_fallback, d_groups, d_out, d_run_groups, d_states, d_slot;
+      d_fallback, d_groups, d_out, d_run_groups, d_states, d_slot, d_select, d_todo;
   std::vector<uint32_t> last_groups;  // groups of the last run
+  std::vector<uint8_t> select_valid;  // chosen particle drawn this run (per group)
   saved_run saved;                    // trajectories and snapshots of the last run
   ~dust2_filter_ode() {
     if (sys) dust2_system_ode_free(sys);
@@ -3864,6 +3866,8 @@ int dust2_filter_ode_alloc(const char* model, const double* pars, size_t n_group
     h->d_state2.alloc(h->sys->d_state.bytes());
     h->d_words2.alloc(h->sys->d_internals.bytes());
     h->d_fallback.alloc(sizeof(uint32_t));
+    h->d_select.alloc(G * sizeof(uint32_t));
+    h->select_valid.assign(G, 0);
     hip_check(hipStreamSynchronize(s), "sync");
     *out = h.release();
     return 0;
@@ -3908,6 +3912,9 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
     hip_check(hipMemsetAsync(h->d_ll.get(), 0, h->d_ll.bytes(), s), "memset");
     hip_check(hipMemsetAsync(h->d_fallback.get(), 0, h->d_fallback.bytes(), s), "memset");
     h->saved = saved_run{};
+    // a new run forgets the chosen particles (dust2/filter.hpp:173)
+    std::fill(h->select_valid.begin(), h->select_valid.end(), 0);
+    h->last_groups.clear();
     if (save_trajectories) {
       h->d_traj.alloc(G * h->n_time * sys->S * P * sizeof(double));
       h->d_ancestry.alloc(G * h->n_time * P * sizeof(uint32_t));
@@ -3991,11 +3998,31 @@ int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, siz
 }  // extern "C"
 
 namespace {
+// filter<T>::select_random_particle (dust2/filter.hpp:130-138) for the groups
+// of the last run: a group's particle is drawn once per run, from its second
+// filter stream, the first time any export asks for it.
+void filter_ode_select(dust2_filter_ode* h) {
+  hipStream_t s = h->sys->stream;
+  std::vector<uint32_t> todo;
+  for (uint32_t g : h->last_groups) {
+    if (!h->select_valid[g]) todo.push_back(g);
+  }
+  if (todo.empty()) return;
+  h->d_todo.alloc(todo.size() * sizeof(uint32_t));
+  to_device(h->d_todo, todo.data(), todo.size() * sizeof(uint32_t), s);
+  hip_check(launch_select_particle(h->d_frng.get<uint64_t>(), h->d_todo.get<uint32_t>(),
+                                   static_cast<int>(todo.size()), static_cast<int>(h->P), 8, 4,
+                                   1, h->d_select.get<uint32_t>(), s),
+            "select particle kernel");
+  hip_check(hipStreamSynchronize(s), "sync");
+  for (uint32_t g : todo) h->select_valid[g] = 1;
+}
+
 // Trajectories (snapshots = false: the states of index_state at every data
 // time) or snapshots (the whole state at the snapshot times) following the
 // ancestry (dust2/trajectories.hpp:128-283) of the groups of the last run:
-// out [times][n_groups_run][P][states].
-void filter_ode_export(dust2_filter_ode* h, bool snapshots, double* out) {
+// out [times][n_groups_run][P, or 1 when select][states].
+void filter_ode_export(dust2_filter_ode* h, bool snapshots, bool select, double* out) {
   hipStream_t s = h->sys->stream;
   const size_t P = h->P, S = h->sys->S, T = h->n_time;
   const std::vector<uint32_t>& groups = h->last_groups;
@@ -4014,12 +4041,14 @@ void filter_ode_export(dust2_filter_ode* h, bool snapshots, double* out) {
     h->d_slot.alloc(T * sizeof(int32_t));
     to_device(h->d_slot, h->saved.slot.data(), T * sizeof(int32_t), s);
   }
-  const size_t bytes = rows * G * P * s_out * sizeof(double);
+  if (select) filter_ode_select(h);
+  const size_t bytes = rows * G * (select ? 1 : P) * s_out * sizeof(double);
   h->d_out.alloc(bytes);
   hip_check(launch_lineage(h->d_traj.get(), h->d_ancestry.get<uint32_t>(),
                            h->d_groups.get<uint32_t>(), static_cast<int>(G),
                            static_cast<int>(T), static_cast<int>(S), static_cast<int>(P),
-                           nullptr, snapshots ? h->d_slot.get<int32_t>() : nullptr,
+                           select ? h->d_select.get<uint32_t>() : nullptr,
+                           snapshots ? h->d_slot.get<int32_t>() : nullptr,
                            subset ? h->d_states.get<uint32_t>() : nullptr,
                            static_cast<int>(s_out), true, h->d_out.get<double>(), s),
             "lineage kernel");
@@ -4029,24 +4058,49 @@ void filter_ode_export(dust2_filter_ode* h, bool snapshots, double* out) {
 
 extern "C" {
 
-int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
-                                       size_t errlen) {
+int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, int select_random_particle,
+                                       double* out, char* err, size_t errlen) {
   try {
     check_filter_ode(h);
     if (!h->saved.traj) throw std::runtime_error("Trajectories are not current");
-    filter_ode_export(h, false, out);
+    filter_ode_export(h, false, select_random_particle != 0, out);
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);
   }
 }
 
-int dust2_filter_ode_last_snapshots(dust2_filter_ode* h, double* out, char* err,
-                                    size_t errlen) {
+int dust2_filter_ode_last_snapshots(dust2_filter_ode* h, int select_random_particle,
+                                    double* out, char* err, size_t errlen) {
   try {
     check_filter_ode(h);
     if (h->saved.n_snap == 0) throw std::runtime_error("Snapshots are not current");
-    filter_ode_export(h, true, out);
+    filter_ode_export(h, true, select_random_particle != 0, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
+int dust2_filter_ode_last_state_selected(dust2_filter_ode* h, double* out, char* err,
+                                         size_t errlen) {
+  try {
+    check_filter_ode(h);
+    if (h->last_groups.empty()) throw std::runtime_error("State is not current");
+    filter_ode_select(h);
+    const size_t P = h->P, S = h->sys->S;
+    std::vector<uint32_t> sel(h->G);
+    to_host(sel.data(), h->d_select, h->G * sizeof(uint32_t), h->sys->stream);
+    std::vector<double> st(h->G * P * S);
+    h->sys->errors_pending = false;
+    char buf[1024];
+    if (dust2_system_ode_state(h->sys, st.data(), buf, sizeof buf) != 0) {
+      throw std::runtime_error(buf);
+    }
+    for (size_t i = 0; i < h->last_groups.size(); ++i) {
+      const size_t g = h->last_groups[i];
+      std::copy_n(st.data() + (g * P + sel[g]) * S, S, out + i * S);
+    }
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

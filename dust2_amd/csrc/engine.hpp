@@ -320,9 +320,13 @@ hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint
                           int n_groups_out, int T, int S, int P, const uint32_t* select,
                           const int32_t* t_slot, const uint32_t* states, int S_out, bool fp64,
                           double* out, hipStream_t stream);
-// select[g] = floor(u * P), u from filter stream 2g + 1 of the [G][4][P + 2]
-// stream blocks, for the n listed groups.
+// select[g] = floor(u * P), u from group g's second filter stream (stream
+// 2g + 1 of the reference's numbering), for the n listed groups; word w of
+// that stream at rng[g * group_words + first + w * word_stride] (the
+// [G][4][P + 2] blocks of the discrete filter: 4 (P + 2), 1, P + 2; the
+// [G][2][4] filter streams of the mixed-time filter: 8, 4, 1).
 hipError_t launch_select_particle(uint64_t* rng, const uint32_t* groups, int n, int P,
+                                  size_t group_words, size_t first, size_t word_stride,
                                   uint32_t* select, hipStream_t stream);
 
 // out[q * n_series + j]: series j of the interpolator (kind, t[n], y, k) at

@@ -57,13 +57,16 @@ __global__ void __launch_bounds__(256)
 
 // One draw from filter stream 2g + 1 (column 1 of the group's [4][P + 2]
 // stream block) per listed group: particle floor(u * P).
+// Group g's second filter stream: word w at rng[g * group_words + first +
+// w * word_stride].
 __global__ void select_particle_kernel(uint64_t* __restrict__ rng, const uint32_t* groups,
-                                       int n, int P, uint32_t* __restrict__ select) {
+                                       int n, int P, size_t group_words, size_t first,
+                                       size_t word_stride, uint32_t* __restrict__ select) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const size_t g = groups[j];
-  const size_t stride = static_cast<size_t>(P) + 2;
-  uint64_t* r = rng + g * 4 * stride + 1;
+  const size_t stride = word_stride;
+  uint64_t* r = rng + g * group_words + first;
   rng256 s{r[0], r[stride], r[2 * stride], r[3 * stride]};
   const double u = random_real<double>(s);
   r[0] = s.s0;
@@ -96,10 +99,11 @@ hipError_t launch_lineage(const void* traj, const uint32_t* ancestry, const uint
 }
 
 hipError_t launch_select_particle(uint64_t* rng, const uint32_t* groups, int n, int P,
+                                  size_t group_words, size_t first, size_t word_stride,
                                   uint32_t* select, hipStream_t stream) {
   const int block = 64;
   hipLaunchKernelGGL(select_particle_kernel, dim3((n + block - 1) / block), dim3(block), 0,
-                     stream, rng, groups, n, P, select);
+                     stream, rng, groups, n, P, group_words, first, word_stride, select);
   return hipGetLastError();
 }
 

@@ -723,20 +723,37 @@ class ModelFilter:
         call(self.prefix + "_last_state", self.ptr, dptr(out))
         return out
 
-    def last_trajectories(self):
+    def last_state_selected(self):
+        """The chosen particle's state for each group of the last run
+        (filter<T>::select_random_particle, dust2/filter.hpp:130-138)."""
+        if self.ptr is None:
+            raise DustError("State is not current")
+        n = self.n_groups if self.last_index is None else np.size(self.last_index)
+        out = np.zeros((n, self.generator.n_state))
+        call("dust2_filter_ode_last_state_selected", self.ptr, dptr(out))
+        return out
+
+    def _select_args(self, select_random_particle):
+        return (int(bool(select_random_particle)),)
+
+    def last_trajectories(self, select_random_particle=False):
         if self.ptr is None:
             raise DustError("Trajectories are not current")
         n = self.n_groups if self.last_index is None else np.size(self.last_index)
-        out = np.zeros((self.time.size, n, self.n_particles, _traj_width(self)))
-        call(self.prefix + "_last_trajectories", self.ptr, dptr(out))
+        p = 1 if select_random_particle else self.n_particles
+        out = np.zeros((self.time.size, n, p, _traj_width(self)))
+        call(self.prefix + "_last_trajectories", self.ptr,
+             *self._select_args(select_random_particle), dptr(out))
         return out
 
-    def last_snapshots_state(self):
+    def last_snapshots_state(self, select_random_particle=False):
         if self.ptr is None or self.last_snapshots is None:
             raise DustError("Snapshots are not current")
         n = self.n_groups if self.last_index is None else np.size(self.last_index)
-        out = np.zeros((self.last_snapshots.size, n, self.n_particles, self.generator.n_state))
-        call(self.prefix + "_last_snapshots", self.ptr, dptr(out))
+        p = 1 if select_random_particle else self.n_particles
+        out = np.zeros((self.last_snapshots.size, n, p, self.generator.n_state))
+        call(self.prefix + "_last_snapshots", self.ptr,
+             *self._select_args(select_random_particle), dptr(out))
         return out
 
     prefix = "dust2_filter_ode"
@@ -752,6 +769,13 @@ class ModelUnfilter(ModelFilter):
     (R/interface-unfilter.R:20-71 over dust2/unfilter.hpp): deterministic
     updates, one log-likelihood per particle."""
     prefix = "dust2_unfilter_ode"
+
+    def _select_args(self, select_random_particle):
+        # every particle of an unfilter is the same deterministic run: the
+        # callers take the first
+        if select_random_particle:
+            raise ValueError("'select_random_particle' is not supported by this filter")
+        return ()
 
     def __init__(self, generator, time_start, data, n_particles, n_groups, dt, ode_control,
                  shared_data=None):
@@ -887,12 +911,11 @@ def dust_likelihood_last_state(obj, select_random_particle=False):
         st = obj.last_state()
         return st[:, 0, :] if select_random_particle else st
     if isinstance(obj, ModelFilter):
-        st = obj.last_state()
         if select_random_particle:
             if isinstance(obj, ModelUnfilter):
-                return st[:, 0, :]
-            raise ValueError("'select_random_particle' is not supported by this filter")
-        return st
+                return obj.last_state()[:, 0, :]
+            return obj.last_state_selected()
+        return obj.last_state()
     if select_random_particle:
         n_groups = obj.n_groups if obj.last_index is None else np.size(obj.last_index)
         out = np.zeros((n_groups, obj.generator.n_state))
@@ -908,9 +931,13 @@ def dust_likelihood_last_snapshots(obj, select_random_particle=False):
     exported particle's ancestry (a filter) or as computed (an unfilter,
     dust2/r/unfilter.hpp:97-128): (n_snapshots, n_groups, n_particles or 1,
     n_state)."""
-    if isinstance(obj, (Unfilter, ModelFilter)):
-        if select_random_particle and not isinstance(obj, Unfilter):
+    if isinstance(obj, ModelUnfilter):
+        if select_random_particle:
             raise ValueError("'select_random_particle' is not supported by this filter")
+        return obj.last_snapshots_state()
+    if isinstance(obj, ModelFilter):
+        return obj.last_snapshots_state(select_random_particle)
+    if isinstance(obj, Unfilter):
         return obj.last_snapshots_state()
     if obj.ptr is None or getattr(obj, "last_snapshots", None) is None:
         raise DustError("Snapshots are not current")
@@ -930,9 +957,7 @@ def dust_likelihood_last_trajectories(obj, select_random_particle=False):
     if isinstance(obj, Unfilter):
         out = obj.last_trajectories()
     elif isinstance(obj, ModelFilter):
-        if select_random_particle:
-            raise ValueError("'select_random_particle' is not supported by this filter")
-        out = obj.last_trajectories()
+        out = obj.last_trajectories(select_random_particle)
     else:
         if obj.ptr is None:
             raise DustError("Trajectories are not current")

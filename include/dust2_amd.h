@@ -475,14 +475,23 @@ int dust2_filter_ode_run(dust2_filter_ode* h, const double* initial, int save_tr
 int dust2_filter_ode_update_pars(dust2_filter_ode* h, const double* pars, char* err,
                                  size_t errlen);
 int dust2_filter_ode_last_state(dust2_filter_ode* h, double* out, char* err, size_t errlen);
-/* out[n_time][groups of the last run][P][n_index_state or S], each
- * particle's lineage (dust2/trajectories.hpp:128-189) */
-int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, double* out, char* err,
-                                       size_t errlen);
-/* out[n_snapshots][groups of the last run][P][S] along the same lineages
+/* out[n_time][groups of the last run][P or 1][n_index_state or S], each
+ * particle's lineage (dust2/trajectories.hpp:128-189); select_random_particle
+ * != 0 exports one particle per group, drawn once per run from the group's
+ * second filter stream (filter<T>::select_random_particle,
+ * dust2/filter.hpp:130-138; dust2_filter_last_trajectories,
+ * dust2/r/filter.hpp:54-95) */
+int dust2_filter_ode_last_trajectories(dust2_filter_ode* h, int select_random_particle,
+                                       double* out, char* err, size_t errlen);
+/* out[n_snapshots][groups of the last run][P or 1][S] along the same lineages
  * (dust2/r/filter.hpp:97-139, dust2/trajectories.hpp:191-283) */
-int dust2_filter_ode_last_snapshots(dust2_filter_ode* h, double* out, char* err,
-                                    size_t errlen);
+int dust2_filter_ode_last_snapshots(dust2_filter_ode* h, int select_random_particle,
+                                    double* out, char* err, size_t errlen);
+/* dust2_filter_last_state with select_random_particle (dust2/r/filter.hpp:
+ * 141-180): the chosen particle's state for each group of the last run,
+ * out[n_groups of the last run][S]. */
+int dust2_filter_ode_last_state_selected(dust2_filter_ode* h, double* out, char* err,
+                                         size_t errlen);
 /* [G][P + 2][4] words: per group the 2 filter streams, then the particles' */
 int dust2_filter_ode_rng_state(dust2_filter_ode* h, uint64_t* out, char* err, size_t errlen);
 int dust2_filter_ode_set_rng_state(dust2_filter_ode* h, const uint64_t* state, char* err,

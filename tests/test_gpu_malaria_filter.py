@@ -94,6 +94,58 @@ def test_trajectories_follow_the_lineage(gpu_lib):
     np.testing.assert_allclose(h[..., 0] + h[..., 1], 1.0, rtol=1e-9)
 
 
+def test_select_random_particle(gpu_lib):
+    # filter<T>::select_random_particle (dust2/filter.hpp:130-138): one draw
+    # per group of the last run from its second filter stream, chosen
+    # particle floor(u * P), the same particle for the trajectories,
+    # snapshots and state, kept until the next run (:173)
+    times = np.array([3.0, 6.0, 9.0])
+    G, P = 3, 40
+    data = {"time": times, "tested": np.full((3, G), 6.0),
+            "positive": np.array([[1, 2, 3], [2, 3, 1], [3, 1, 2]], dtype=float)}
+    pars = [{"a": 0.3}, {"a": 0.35}, {"a": 0.4}]
+    obj = gpu_lib.dust_filter_create(gpu_lib.malaria(), 0.0, data, P, n_groups=G, dt=1,
+                                     seed=ol.prng(G * (P + 2), seed=11))
+    gpu_lib.dust_likelihood_run(obj, pars, save_trajectories=True, save_snapshots=[6.0, 9.0])
+    s1 = gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4)
+    h = gpu_lib.dust_likelihood_last_trajectories(obj)
+    snap = gpu_lib.dust_likelihood_last_snapshots(obj)
+    st = gpu_lib.dust_likelihood_last_state(obj)
+    h1 = gpu_lib.dust_likelihood_last_trajectories(obj, select_random_particle=True)
+    assert h1.shape == (3, G, 1, 20)
+    s2 = gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4)
+    for g in range(G):
+        u, after = ol.random_real(s1[g, 1], 1)
+        assert np.array_equal(s2[g, 1], after)
+        k = int(np.floor(u[0] * P))
+        np.testing.assert_array_equal(h1[:, g, 0], h[:, g, k])
+        np.testing.assert_array_equal(
+            gpu_lib.dust_likelihood_last_snapshots(obj, select_random_particle=True)[:, g, 0],
+            snap[:, g, k])
+        np.testing.assert_array_equal(
+            gpu_lib.dust_likelihood_last_state(obj, select_random_particle=True)[g], st[g, k])
+    assert np.array_equal(np.delete(s2, 1, axis=1), np.delete(s1, 1, axis=1))
+    # cached: no further draws
+    assert np.array_equal(gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4), s2)
+    # a run over a subset: the choice is redrawn for the groups of that run
+    gpu_lib.dust_likelihood_run(obj, None, index_group=[2, 0], save_trajectories=True)
+    s3 = gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4)
+    h = gpu_lib.dust_likelihood_last_trajectories(obj)
+    h1 = gpu_lib.dust_likelihood_last_trajectories(obj, select_random_particle=True)
+    assert h1.shape == (3, 2, 1, 20)
+    for i, g in enumerate([2, 0]):
+        u, _ = ol.random_real(s3[g, 1], 1)
+        np.testing.assert_array_equal(h1[:, i, 0], h[:, i, int(np.floor(u[0] * P))])
+    s4 = gpu_lib.dust_likelihood_rng_state(obj).reshape(G, P + 2, 4)
+    assert np.array_equal(s4[1], s3[1])
+    # deterministic unfilters take their first particle
+    un = gpu_lib.dust_unfilter_create(gpu_lib.malaria(), 0.0, data, n_particles=2, n_groups=G)
+    gpu_lib.dust_likelihood_run(un, pars)
+    np.testing.assert_array_equal(
+        gpu_lib.dust_likelihood_last_state(un, select_random_particle=True),
+        gpu_lib.dust_likelihood_last_state(un)[:, 0])
+
+
 def test_filter_errors(gpu_lib):
     data = {"time": [4.0, 8.0], "tested": [2, 4], "positive": [1, 2]}
     with pytest.raises(ValueError,

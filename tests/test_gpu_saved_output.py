@@ -216,9 +216,9 @@ def test_general_engine_filter_and_unfilter_snapshots(gpu_lib):
     call("dust2_filter_ode_run", obj.ptr, None, 1, *save, None, 0, dptr(ll))
     np.testing.assert_allclose(ll, case["runs"][0]["ll"], rtol=1e-10)
     traj = np.zeros((T, 1, P, 2))
-    call("dust2_filter_ode_last_trajectories", obj.ptr, dptr(traj))
+    call("dust2_filter_ode_last_trajectories", obj.ptr, 0, dptr(traj))
     snap = np.zeros((2, 1, P, 20))
-    call("dust2_filter_ode_last_snapshots", obj.ptr, dptr(snap))
+    call("dust2_filter_ode_last_snapshots", obj.ptr, 0, dptr(snap))
     final = np.array(case["runs"][0]["state"])
     np.testing.assert_allclose(snap[1], final, rtol=1e-9, atol=1e-12)
     np.testing.assert_array_equal(traj[-1, ..., 0], snap[1, ..., 1])
