@@ -4810,23 +4810,27 @@ struct srv_chain {
   bool busy = false;
 };
 
-// The resident requests of one (model, particle count): one stream, one
-// device slot table of fixed capacity (unused entries do nothing), a HIP
-// graph of chunk x (advance, resample) replayed per launch, and the
-// completion event of the launch in flight.
+// The resident requests of one model, whatever their particle counts and
+// data lengths: one stream, one device slot table of fixed capacity
+// (unused entries do nothing) with a block map that lays every member's
+// particles onto the advance launch's workgroups, a HIP graph of
+// chunk x (advance over the block map, resample with one workgroup per
+// member) replayed per launch, and the completion event of the launch in
+// flight.
 struct srv_group {
   int model = 0;
-  size_t P = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
   bool in_flight = false;
   std::vector<std::unique_ptr<srv_slot>> members;  // table order at the next launch
   std::vector<srv_slot*> launched;                 // members of the launch in flight
-  size_t cap = 0;
-  dbuf d_table, d_status;
-  pinned h_table, h_status;
+  size_t cap = 0;   // slot table entries
+  size_t bcap = 0;  // block map entries
+  dbuf d_table, d_status, d_bmap;
+  pinned h_table, h_status, h_bmap;
   hipGraphExec_t graph = nullptr;
   int graph_chunk = 0;
+  int graph_ppt = 0;  // particles per thread of the captured resample launches
   ~srv_group() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (ev) (void)hipEventDestroy(ev);
@@ -4862,7 +4866,7 @@ struct dust2_server {
   uint64_t next_id = 1;
   std::deque<srv_request> queue;
   std::deque<srv_request> ode_queue;
-  std::map<std::pair<int, size_t>, std::unique_ptr<srv_group>> groups;
+  std::map<int, std::unique_ptr<srv_group>> groups;  // by model
   std::map<int64_t, srv_chain> chains;
   srv_ode ode;
   std::vector<srv_result> results;
@@ -4882,12 +4886,11 @@ size_t model_n_data(int model) { return model == kServerOde ? 1 : discrete_model
 size_t model_kpars(int model) { return discrete_model(model).kpars; }
 double model_zero_period(int model) { return discrete_model(model).zero_period; }
 
-srv_group& srv_group_of(dust2_server* sv, int model, size_t P) {
-  auto& g = sv->groups[{model, P}];
+srv_group& srv_group_of(dust2_server* sv, int model) {
+  auto& g = sv->groups[model];
   if (!g) {
     g = std::make_unique<srv_group>();
     g->model = model;
-    g->P = P;
     hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&g->ev, hipEventDisableTiming), "hipEventCreate");
   }
@@ -4955,7 +4958,7 @@ bool srv_admit(dust2_server* sv, const srv_request& rq) {
   }
   std::fill(kpars, kpars + kp, 0.0);
   discrete_model(rq.model).kernel_pars(rq.pars.data(), rq.dt, kpars);
-  srv_group& g = srv_group_of(sv, rq.model, P);
+  srv_group& g = srv_group_of(sv, rq.model);
   hipStream_t s = g.stream;
   hip_check(hipMemcpyAsync(b + L.upload, h, up, hipMemcpyHostToDevice, s), "upload");
   hip_check(hipMemsetAsync(b + L.ll, 0, 8, s), "memset");
@@ -5005,6 +5008,7 @@ bool srv_admit(dust2_server* sv, const srv_request& rq) {
 // Launch the group's next chunk of intervals for every member.
 void srv_launch(dust2_server* sv, srv_group& g) {
   const size_t n = g.members.size();
+  bool recapture = !g.graph || g.graph_chunk != sv->chunk;
   if (n > g.cap) {
     size_t cap = std::max<size_t>(16, g.cap);
     while (cap < n) cap *= 2;
@@ -5013,23 +5017,41 @@ void srv_launch(dust2_server* sv, srv_group& g) {
     g.d_status.alloc(2 * cap * sizeof(double));
     g.h_status.alloc(2 * cap * sizeof(double));
     g.cap = cap;
-    if (g.graph) {
-      hip_check(hipGraphExecDestroy(g.graph), "hipGraphExecDestroy");
-      g.graph = nullptr;
-    }
+    recapture = true;
   }
-  const int P = static_cast<int>(g.P);
+  // the members' particles in runs of the advance workgroup's size
+  const size_t B = static_cast<size_t>(discrete_advance_block(g.model, false));
+  size_t n_blocks = 0, max_p = 1;
+  for (const auto& sl : g.members) {
+    n_blocks += (sl->req.P + B - 1) / B;
+    max_p = std::max(max_p, sl->req.P);
+  }
+  if (n_blocks > g.bcap) {
+    size_t bcap = std::max<size_t>(256, g.bcap);
+    while (bcap < n_blocks) bcap *= 2;
+    g.d_bmap.alloc(bcap * sizeof(uint32_t));
+    g.h_bmap.alloc(bcap * sizeof(uint32_t));
+    g.bcap = bcap;
+    recapture = true;
+  }
+  // the resample workgroup holds the largest member; recaptured when a
+  // larger one arrives
+  const int ppt = pipe_resample_ppt(static_cast<int>(max_p));
+  if (ppt > g.graph_ppt) recapture = true;
   const int cap = static_cast<int>(g.cap);
-  if (!g.graph || g.graph_chunk != sv->chunk) {
+  if (recapture) {
     if (g.graph) hip_check(hipGraphExecDestroy(g.graph), "hipGraphExecDestroy");
     g.graph = nullptr;
+    g.graph_ppt = std::max(g.graph_ppt, ppt);
+    const int p_res = 1024 * g.graph_ppt;
     hipGraph_t graph;
     hip_check(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int k = 0; k < sv->chunk; ++k) {
-      hip_check(launch_discrete_advance(g.model, false, g.d_table.get<pipe_slot>(), cap, P, k, 0,
-                                        0, g.stream),
+      hip_check(launch_discrete_advance_mapped(g.model, false, g.d_table.get<pipe_slot>(),
+                                               g.d_bmap.get<uint32_t>(),
+                                               static_cast<int>(g.bcap), k, g.stream),
                 "advance kernel");
-      hip_check(launch_pipe_resample(g.d_table.get<pipe_slot>(), cap, P, k, g.stream),
+      hip_check(launch_pipe_resample(g.d_table.get<pipe_slot>(), cap, p_res, k, g.stream),
                 "resample kernel");
     }
     hip_check(hipStreamEndCapture(g.stream, &graph), "end capture");
@@ -5038,21 +5060,31 @@ void srv_launch(dust2_server* sv, srv_group& g) {
     g.graph_chunk = sv->chunk;
   }
   pipe_slot* tab = g.h_table.get<pipe_slot>();
+  uint32_t* bmap = g.h_bmap.get<uint32_t>();
+  size_t nb = 0;
   g.launched.clear();
   for (size_t i = 0; i < n; ++i) {
     srv_slot* sl = g.members[i].get();
     tab[i] = sl->ps;
     tab[i].t_offset = static_cast<int32_t>(sl->done);
     g.launched.push_back(sl);
+    const size_t blocks = (sl->req.P + B - 1) / B;
+    for (size_t b = 0; b < blocks; ++b) {
+      bmap[nb++] = static_cast<uint32_t>(i << kBlockBits | b);
+    }
   }
   for (size_t i = n; i < g.cap; ++i) {
     // unused entries: no particles and no intervals, so every kernel leaves
     tab[i] = pipe_slot{};
     tab[i].flags = SLOT_AUTO_MODE;
   }
+  std::fill(bmap + nb, bmap + g.bcap, kNoBlock);
   hip_check(hipMemcpyAsync(g.d_table.get(), tab, g.cap * sizeof(pipe_slot),
                            hipMemcpyHostToDevice, g.stream),
             "upload slot table");
+  hip_check(hipMemcpyAsync(g.d_bmap.get(), bmap, g.bcap * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, g.stream),
+            "upload block map");
   hip_check(hipGraphLaunch(g.graph, g.stream), "hipGraphLaunch");
   hip_check(launch_slot_status(g.d_table.get<pipe_slot>(), static_cast<int>(n),
                                g.d_status.get<double>(), g.stream),

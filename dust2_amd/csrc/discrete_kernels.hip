@@ -58,17 +58,31 @@ __device__ __forceinline__ void zero_every(float* y) {
 #define DUST2_COUPLED_WAVES_PER_SIMD 3
 #endif
 
+// Workgroups map onto (slot, block of the slot's particles) either as the
+// 2-D grid (block, slot), or, with a block map, as entry blockIdx.x of the
+// map (slot << kBlockBits | block; kNoBlock: nothing to do), so that slots
+// of very different particle counts share one launch without idle
+// workgroups.
 template <class Model, typename real, int BLOCK, int MINW>
 __global__ void __launch_bounds__(BLOCK, MINW)
-    advance_kernel(const pipe_slot* __restrict__ slots, int t, int parity, uint32_t mode) {
+    advance_kernel(const pipe_slot* __restrict__ slots, const uint32_t* __restrict__ block_map,
+                   int t, int parity, uint32_t mode) {
   constexpr bool coupled = Model::template coupled<real>;
   __shared__ float lds[coupled ? Model::lds_floats : 1];
-  const pipe_slot& sd = slots[blockIdx.y];
+  int slot = blockIdx.y;
+  int blk = blockIdx.x;
+  if (block_map) {
+    const uint32_t e = block_map[blockIdx.x];
+    if (e == kNoBlock) return;
+    slot = static_cast<int>(e >> kBlockBits);
+    blk = static_cast<int>(e & ((1u << kBlockBits) - 1));
+  }
+  const pipe_slot& sd = slots[slot];
   const int P = static_cast<int>(sd.n_particles);
-  const int p = blockIdx.x * BLOCK + threadIdx.x;
+  const int p = blk * BLOCK + threadIdx.x;
   const int lane = threadIdx.x % 64;
   if (coupled) {
-    if (static_cast<int>(blockIdx.x) * BLOCK >= P) return;  // whole wavefront idle
+    if (blk * BLOCK >= P) return;  // whole wavefront idle
   } else if (p >= P) {
     return;
   }
@@ -185,8 +199,18 @@ hipError_t launch_advance_t(const pipe_slot* d_slots, int n_slots, int max_parti
   constexpr int B = launch_shape<Model, real>::block;
   constexpr int W = launch_shape<Model, real>::coupled ? DUST2_COUPLED_WAVES_PER_SIMD : 1;
   const dim3 grid((max_particles + B - 1) / B, n_slots);
-  hipLaunchKernelGGL((advance_kernel<Model, real, B, W>), grid, dim3(B), 0, stream, d_slots, t,
-                     parity, mode);
+  hipLaunchKernelGGL((advance_kernel<Model, real, B, W>), grid, dim3(B), 0, stream, d_slots,
+                     nullptr, t, parity, mode);
+  return hipGetLastError();
+}
+
+template <class Model, typename real>
+hipError_t launch_advance_mapped_t(const pipe_slot* d_slots, const uint32_t* d_block_map,
+                                   int n_blocks, int t, hipStream_t stream) {
+  constexpr int B = launch_shape<Model, real>::block;
+  constexpr int W = launch_shape<Model, real>::coupled ? DUST2_COUPLED_WAVES_PER_SIMD : 1;
+  hipLaunchKernelGGL((advance_kernel<Model, real, B, W>), dim3(n_blocks), dim3(B), 0, stream,
+                     d_slots, d_block_map, t, 0, 0u);
   return hipGetLastError();
 }
 
@@ -551,6 +575,46 @@ hipError_t launch_discrete_advance(int model, bool fp64, const pipe_slot* d_slot
                                                           parity, mode, stream)
                   : launch_advance_t<walk_model, float>(d_slots, n_slots, max_particles, t,
                                                          parity, mode, stream);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+int discrete_advance_block(int model, bool fp64) {
+  switch (model) {
+    case MODEL_SIR:
+      return fp64 ? launch_shape<sir_model, double>::block : launch_shape<sir_model, float>::block;
+    case MODEL_SEIR16:
+      return fp64 ? launch_shape<seir16_model, double>::block
+                  : launch_shape<seir16_model, float>::block;
+    case MODEL_WALK:
+      return fp64 ? launch_shape<walk_model, double>::block
+                  : launch_shape<walk_model, float>::block;
+    default:
+      return 0;
+  }
+}
+
+hipError_t launch_discrete_advance_mapped(int model, bool fp64, const pipe_slot* d_slots,
+                                          const uint32_t* d_block_map, int n_blocks, int t,
+                                          hipStream_t stream) {
+  if (n_blocks == 0) return hipSuccess;
+  switch (model) {
+    case MODEL_SIR:
+      return fp64 ? launch_advance_mapped_t<sir_model, double>(d_slots, d_block_map, n_blocks,
+                                                                t, stream)
+                  : launch_advance_mapped_t<sir_model, float>(d_slots, d_block_map, n_blocks, t,
+                                                               stream);
+    case MODEL_SEIR16:
+      return fp64 ? launch_advance_mapped_t<seir16_model, double>(d_slots, d_block_map,
+                                                                   n_blocks, t, stream)
+                  : launch_advance_mapped_t<seir16_model, float>(d_slots, d_block_map, n_blocks,
+                                                                  t, stream);
+    case MODEL_WALK:
+      return fp64 ? launch_advance_mapped_t<walk_model, double>(d_slots, d_block_map, n_blocks,
+                                                                 t, stream)
+                  : launch_advance_mapped_t<walk_model, float>(d_slots, d_block_map, n_blocks,
+                                                                t, stream);
     default:
       return hipErrorInvalidValue;
   }

@@ -156,6 +156,16 @@ hipError_t launch_discrete_advance(int model, bool fp64, const pipe_slot* d_slot
                                    hipStream_t stream);
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
                                 hipStream_t stream);
+// The advance launch over a block map: entry b = slot << kBlockBits | block
+// (the block-th run of discrete_advance_block(model) particles of that
+// slot), kNoBlock for an unused entry; n_blocks entries, one workgroup
+// each, every slot in SLOT_AUTO_MODE (its own interval, parity and mode).
+constexpr uint32_t kBlockBits = 12;
+constexpr uint32_t kNoBlock = 0xffffffffu;
+int discrete_advance_block(int model, bool fp64);
+hipError_t launch_discrete_advance_mapped(int model, bool fp64, const pipe_slot* d_slots,
+                                          const uint32_t* d_block_map, int n_blocks, int t,
+                                          hipStream_t stream);
 
 // ---- systematic resampling of groups above one workgroup ----------------------
 // (resample_kernels.hip)  One group per task; every pointer is device

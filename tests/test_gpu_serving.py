@@ -80,6 +80,34 @@ def test_results_identical_alone_in_mix_and_chunked(gpu_lib):
         assert alone[0][1] == mixed[k][1]
 
 
+def test_uneven_particle_counts_share_one_launch(gpu_lib):
+    """Requests of one model run in one advance launch per interval whatever
+    their particle counts (a block map lays each request's particles onto
+    the launch's workgroups) and data lengths.  A larger request admitted
+    while smaller ones are part-way through their series enlarges the
+    captured launches (more blocks, a wider resample workgroup) without
+    changing anyone's result."""
+    from dust2_amd.serving import Server
+    rng = np.random.default_rng(5)
+    small = []
+    for i, P in enumerate([64, 300, 1000, 130]):
+        p, d = sir_request(rng, int(rng.integers(30, 50)))
+        small.append((gpu_lib.sir(), p, P, d, 0.25, 200 + i))
+    p, d = sir_request(rng, 25)
+    big = (gpu_lib.sir(), p, 5000, d, 0.25, 300)
+    sv = Server(hbm_bytes=1 << 30, chunk_intervals=4)
+    ids = [sv.submit(g, pp, P, 0.0, dd, dt=dt, seed=s) for (g, pp, P, dd, dt, s) in small]
+    sv.step()
+    sv.step()
+    ids.append(sv.submit(*big[:3], 0.0, big[3], dt=big[4], seed=big[5]))
+    res = {r[0]: r for r in sv.drain()}
+    got = [res[i] for i in ids]
+    assert all(r[2] == 0 for r in got)
+    for req, r in zip(small + [big], got):
+        alone = serve(gpu_lib, [req], chunk=7)
+        assert alone[0][1] == r[1]
+
+
 def test_server_matches_plain_entry_points(gpu_lib):
     reqs = make_mix(gpu_lib, seed=2, n=6)
     served = serve(gpu_lib, reqs)
