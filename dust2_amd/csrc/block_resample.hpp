@@ -217,12 +217,23 @@ __device__ resample_result block_resample(double (&w)[PPT], int (&idx)[PPT], int
     const double uu0 = tot * u / Pd;
     const double du = tot / Pd;
     int amb = 0;
+    int j_prev = 0;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
       const int i = p0 + k;
       if (i < P) {
         const double uu = uu0 + static_cast<double>(i) * du;
-        const int j = lower_bound_lds(cum, P, uu);
+        // the thread's points are increasing, so each later point's
+        // ancestor is at or after the previous one's: a few steps forward
+        // from there, then a binary search over what is left
+        int j = j_prev;
+        if (k > 0) {
+          for (int s = 0; s < 4 && j < P && cum[j] < uu; ++s) ++j;
+          if (j < P && cum[j] < uu) j += lower_bound_lds(cum + j, P - j, uu);
+        } else {
+          j = lower_bound_lds(cum, P, uu);
+        }
+        j_prev = j;
         if ((j < P && cum[j] - uu <= delta) || (j > 0 && uu - cum[j - 1] <= delta)) {
           amb = 1;
         }
