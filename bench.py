@@ -21,6 +21,9 @@ Other workloads (--workload):
                   {128, 512, 2048, 8192}, data length U{50..730}) at up to
                   2000 requests/s: continuous batching vs one request at a
                   time, evals/s and p50/p99 latency
+  valu-peak       the vector-ALU issue rate of independent fused
+                  multiply-adds (FP32, FP64) at 1-8 wavefronts per SIMD,
+                  the compute ceiling of the sampler kernels
 
 Each prints one JSON line.  `roofline` refers to the dominant kernel:
 algorithmic bytes per launch (state gather-read and write, xoshiro256+
@@ -649,13 +652,41 @@ def bench_serving(lib, args):
             "batched_over_plain": batched["evals_per_s"] / max(plain["evals_per_s"], 1e-9)}
 
 
+def bench_valu_peak(lib, args):
+    """The vector-ALU issue rate of independent fused multiply-adds at 1, 2,
+    4 and 8 wavefronts per SIMD (dust2_amd_valu_probe), FP32 and FP64: the
+    peak against which the sampler kernels' VALU instruction counts are
+    set."""
+    import ctypes as C
+    from dust2_amd._lib import call
+    out = {}
+    for prec, code in (("fp32", 1), ("fp64", 0)):
+        rows = []
+        for w in (1, 2, 4, 8):
+            ms = np.zeros(1)
+            rate = np.zeros(1)
+            call("dust2_amd_valu_probe", code, w, 20000,
+                 ms.ctypes.data_as(C.POINTER(C.c_double)),
+                 rate.ctypes.data_as(C.POINTER(C.c_double)))
+            rows.append({"waves_per_simd": w, "ms": float(ms[0]),
+                         "wave_instr_per_simd_per_ns": float(rate[0])})
+        out[prec] = rows
+    peak = max(r["wave_instr_per_simd_per_ns"] for r in out["fp32"])
+    return {"metric": "VALU wave-instructions per SIMD per ns", "value": peak,
+            "unit": "wave-instr/SIMD/ns", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": out["fp32"][-1]["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (independent FMA chains)",
+            "config": {"workload": "vector-ALU issue-rate calibration"}, "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="sir",
-                    choices=["sir", "sir-replicates", "sirode", "seir16", "serving"])
+                    choices=["sir", "sir-replicates", "sirode", "seir16", "serving",
+                             "valu-peak"])
     ap.add_argument("--groups", type=int, default=512)
     ap.add_argument("--particles", type=int, default=2048)
     ap.add_argument("--n-obs", type=int, default=365)
@@ -684,7 +715,8 @@ def main():
     from dust2_amd._lib import call
     call("dust2_amd_init")
     fn = {"sir": bench_sir, "sir-replicates": bench_sir_replicates, "sirode": bench_sirode,
-          "seir16": bench_seir16, "serving": bench_serving}[args.workload]
+          "seir16": bench_seir16, "serving": bench_serving,
+          "valu-peak": bench_valu_peak}[args.workload]
     print(json.dumps(fn(lib, args)))
 
 

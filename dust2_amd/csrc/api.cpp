@@ -2398,6 +2398,46 @@ int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uin
   }
 }
 
+int dust2_amd_valu_probe(int precision, int waves_per_simd, int iters, double* ms_out,
+                         double* rate_out, char* err, size_t errlen) {
+  try {
+    if (waves_per_simd < 1 || waves_per_simd > 8) {
+      throw std::runtime_error("'waves_per_simd' must be between 1 and 8");
+    }
+    if (iters < 1) throw std::runtime_error("'iters' must be at least 1");
+    const bool fp64 = precision == DUST2_FP64;
+    int dev = 0, n_cu = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+    // four wavefronts per workgroup, one per SIMD of a compute unit
+    const int n_blocks = n_cu * waves_per_simd;
+    hipStream_t s = nullptr;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    dbuf out(static_cast<size_t>(n_blocks) * 256 * sizeof(double));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(launch_valu_probe(fp64, n_blocks, 10, out.get<double>(), s), "probe kernel");
+    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+    hip_check(launch_valu_probe(fp64, n_blocks, iters, out.get<double>(), s), "probe kernel");
+    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    const double wave_instr = 4.0 * n_blocks * static_cast<double>(iters) *
+                              valu_probe_fmas_per_iteration(fp64);
+    *ms_out = ms;
+    *rate_out = wave_instr / (4.0 * n_cu) / (ms * 1e6);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams, size_t n_draws,
                      const double* a, const double* b, int deterministic, double* out,
                      uint32_t* err_out, char* err, size_t errlen) {

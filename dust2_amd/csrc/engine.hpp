@@ -387,6 +387,14 @@ hipError_t launch_rng_aos_to_soa(const uint64_t* aos, uint64_t* soa, int G, int 
 hipError_t launch_rng_soa_to_aos(const uint64_t* soa, uint64_t* aos, int G, int per_group,
                                  int stride, hipStream_t stream);
 
+// ---- vector-ALU calibration (calibration_kernels.hip) --------------------------
+// n_blocks workgroups of four wavefronts, each wavefront issuing
+// iters * valu_probe_fmas_per_iteration(fp64) independent fused
+// multiply-adds (plus loop overhead); out: n_blocks * 256 doubles.
+int valu_probe_fmas_per_iteration(bool fp64);
+hipError_t launch_valu_probe(bool fp64, int n_blocks, int iters, double* out,
+                             hipStream_t stream);
+
 // ---- test entry points -----------------------------------------------------
 hipError_t launch_resample_test(const double* w, int n, double u, uint32_t* idx,
                                 double* ll_increment, hipStream_t stream);

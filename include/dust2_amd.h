@@ -695,6 +695,17 @@ int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams,
                      size_t n_draws, const double* a, const double* b, int deterministic,
                      double* out, uint32_t* err_out, char* err, size_t errlen);
 
+/* Vector-ALU issue-rate calibration (no counterpart in the reference; it
+ * supports the measurement of the sampler kernels, which are bound by
+ * vector instructions, not HBM): waves_per_simd wavefronts per SIMD on
+ * every compute unit, each issuing `iters` iterations of independent FP32
+ * (precision DUST2_FP32: 8 per iteration) or FP64 (DUST2_FP64: 4) fused
+ * multiply-adds.  ms_out: the kernel time (HIP events, after one warm-up
+ * launch); rate_out: fused multiply-add wave-instructions per SIMD per
+ * nanosecond. */
+int dust2_amd_valu_probe(int precision, int waves_per_simd, int iters, double* ms_out,
+                         double* rate_out, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif
