@@ -174,12 +174,52 @@ def measured_traffic(key, kernel):
     return 1024 * (tab["fetch_scale"] * e["fetch_kib"] + tab["write_scale"] * e["write_kib"])
 
 
-def roofline(achieved, kernel, k_ms, b_launch, b_obs, traffic_key=None):
+_VALU_PEAK = {}
+
+
+def valu_peak(fp64):
+    """Wave-instructions per SIMD per ns of independent fused multiply-adds
+    at 8 wavefronts per SIMD, measured now (dust2_amd_valu_probe), and the
+    device's SIMD count."""
+    if fp64 not in _VALU_PEAK:
+        import ctypes as C
+        from dust2_amd._lib import call
+        ms, rate = np.zeros(1), np.zeros(1)
+        n_simd = np.zeros(1, dtype=np.int32)
+        call("dust2_amd_valu_probe", 0 if fp64 else 1, 8, 20000,
+             ms.ctypes.data_as(C.POINTER(C.c_double)),
+             rate.ctypes.data_as(C.POINTER(C.c_double)),
+             n_simd.ctypes.data_as(C.POINTER(C.c_int)))
+        _VALU_PEAK[fp64] = (float(rate[0]), int(n_simd[0]))
+    return _VALU_PEAK[fp64]
+
+
+def valu_issue(key, kernel, k_ms, fp64):
+    """The dominant kernel's vector-instruction issue rate: its VALU
+    instructions per launch (the committed SQ_INSTS_VALU profile of this
+    workload and launch shape) over its event-timed duration, against the
+    measured issue rate of independent FMAs; None without a profile."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            e = json.load(f).get(key)
+    except OSError:
+        return None
+    if e is None or "valu_instructions" not in e or not e["kernel"].startswith(kernel):
+        return None
+    peak, n_simd = valu_peak(fp64)
+    achieved = e["valu_instructions"] / n_simd / (k_ms * 1e6)
+    return {"instructions_per_launch": e["valu_instructions"], "achieved": achieved,
+            "peak": peak, "unit": "wave-instructions/SIMD/ns", "frac": achieved / peak,
+            "peak_of": "FP64 FMA" if fp64 else "FP32 FMA", "simds": n_simd}
+
+
+def roofline(achieved, kernel, k_ms, b_launch, b_obs, traffic_key=None, fp64=False):
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(traffic_key, kernel) if traffic_key else None,
             "kernel": kernel, "kernel_ms": k_ms, "bytes_per_launch": b_launch,
-            "bytes_per_particle_obs": b_obs}
+            "bytes_per_particle_obs": b_obs,
+            "valu_issue": valu_issue(traffic_key, kernel, k_ms, fp64) if traffic_key else None}
 
 
 def timed_runs(lib, obj, plist, steps, warmup):
@@ -201,7 +241,7 @@ def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, k
         kernel = "filter_kernel<sir_model, %s, %d, %d>" % (
             (real,) + lib.dust_likelihood_last_stats(obj)["launch"])
         return roofline(b_launch / (run_ms * 1e-3) / 1e9, kernel, run_ms, b_launch, b_obs,
-                        key)
+                        key, fp64=s_x == 8)
     # pipeline: the advance kernel moves state, RNG, weight write and
     # ancestor read of one interval; timed per launch in one profiled run
     obj.set_engine("pipeline", graph=False, profile=True)
@@ -215,7 +255,8 @@ def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, k
         name = "advance_kernel<seir16_model, %s, %s>" % (real, "64, 3" if s_x == 4 else "256, 1")
     else:
         name = "advance_kernel<sir_model, %s, 256, 1>" % real
-    out = roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs, key)
+    out = roofline(b_launch / (k_ms * 1e-3) / 1e9, name, k_ms, b_launch, b_obs, key,
+                   fp64=s_x == 8)
     # the whole run (advance + resample launches) against the same bytes
     out["run_ms"] = run_ms
     out["run_achieved"] = G * P * n_obs * b_obs / (run_ms * 1e-3) / 1e9
@@ -665,9 +706,11 @@ def bench_valu_peak(lib, args):
         for w in (1, 2, 4, 8):
             ms = np.zeros(1)
             rate = np.zeros(1)
+            n_simd = np.zeros(1, dtype=np.int32)
             call("dust2_amd_valu_probe", code, w, 20000,
                  ms.ctypes.data_as(C.POINTER(C.c_double)),
-                 rate.ctypes.data_as(C.POINTER(C.c_double)))
+                 rate.ctypes.data_as(C.POINTER(C.c_double)),
+                 n_simd.ctypes.data_as(C.POINTER(C.c_int)))
             rows.append({"waves_per_simd": w, "ms": float(ms[0]),
                          "wave_instr_per_simd_per_ns": float(rate[0])})
         out[prec] = rows

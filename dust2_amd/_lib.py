@@ -163,7 +163,8 @@ _PROTOS = {
     "dust2_filter_ode_last_trajectories": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_snapshots": (C.c_int, [VP, C.c_int, D, ERR, ERRLEN]),
     "dust2_filter_ode_last_state_selected": (C.c_int, [VP, D, ERR, ERRLEN]),
-    "dust2_amd_valu_probe": (C.c_int, [C.c_int, C.c_int, C.c_int, D, D, ERR, ERRLEN]),
+    "dust2_amd_valu_probe": (C.c_int, [C.c_int, C.c_int, C.c_int, D, D, C.POINTER(C.c_int),
+                                       ERR, ERRLEN]),
     "dust2_filter_ode_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_set_rng_state": (C.c_int, [VP, U64, ERR, ERRLEN]),
     "dust2_filter_ode_free": (None, [VP]),

@@ -2399,7 +2399,7 @@ int dust2_amd_rng_streams(const uint64_t* seed_state, size_t n_seed_streams, uin
 }
 
 int dust2_amd_valu_probe(int precision, int waves_per_simd, int iters, double* ms_out,
-                         double* rate_out, char* err, size_t errlen) {
+                         double* rate_out, int* n_simd_out, char* err, size_t errlen) {
   try {
     if (waves_per_simd < 1 || waves_per_simd > 8) {
       throw std::runtime_error("'waves_per_simd' must be between 1 and 8");
@@ -2432,6 +2432,7 @@ int dust2_amd_valu_probe(int precision, int waves_per_simd, int iters, double* m
                               valu_probe_fmas_per_iteration(fp64);
     *ms_out = ms;
     *rate_out = wave_instr / (4.0 * n_cu) / (ms * 1e6);
+    *n_simd_out = 4 * n_cu;
     return 0;
   } catch (const std::exception& e) {
     return fail(e, err, errlen);

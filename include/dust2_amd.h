@@ -702,9 +702,9 @@ int dust2_amd_sample(int kind, int precision, uint64_t* rng, size_t n_streams,
  * (precision DUST2_FP32: 8 per iteration) or FP64 (DUST2_FP64: 4) fused
  * multiply-adds.  ms_out: the kernel time (HIP events, after one warm-up
  * launch); rate_out: fused multiply-add wave-instructions per SIMD per
- * nanosecond. */
+ * nanosecond; n_simd_out: the SIMDs of the device (4 per compute unit). */
 int dust2_amd_valu_probe(int precision, int waves_per_simd, int iters, double* ms_out,
-                         double* rate_out, char* err, size_t errlen);
+                         double* rate_out, int* n_simd_out, char* err, size_t errlen);
 
 #ifdef __cplusplus
 }
