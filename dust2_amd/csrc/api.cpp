@@ -5049,6 +5049,9 @@ bool srv_admit(dust2_server* sv, const srv_request& rq) {
 // Launch the group's next chunk of intervals for every member.
 void srv_launch(dust2_server* sv, srv_group& g) {
   const size_t n = g.members.size();
+  if (n >= (size_t(1) << (32 - kBlockBits))) {
+    throw std::runtime_error("too many resident requests for one launch");
+  }
   bool recapture = !g.graph || g.graph_chunk != sv->chunk;
   if (n > g.cap) {
     size_t cap = std::max<size_t>(16, g.cap);
