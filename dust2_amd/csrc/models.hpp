@@ -394,6 +394,21 @@ struct walk_model {
   }
 };
 
+// a / b correctly rounded, from inv_b = RN(1 / b): q = RN(a * inv_b) is
+// within an ulp of a / b, the residual a - q * b is exact in one fused
+// multiply-add, and RN(q + residual * inv_b) is the correctly rounded
+// quotient (Markstein's theorem: P. Markstein, "Computation of elementary
+// functions on the IBM RISC System/6000 processor", IBM J. Res. Dev. 34
+// (1990) 111-119; an inverse within half an ulp of 1 / b suffices).  Three
+// dependent operations instead of a full division, for finite a and b in
+// the normal range (b = 0 gives NaN where a / b gives an infinity).
+template <typename real>
+__device__ __forceinline__ real div_rn(real a, real b, real inv_b) {
+  const real q = a * inv_b;
+  const real r = fma(-q, b, a);
+  return fma(r, inv_b, q);
+}
+
 // Deterministic SIR ODE, inst/examples/sirode.cpp:9-102.
 // Host-side parameter vector per set: N, I0, beta, gamma.
 struct sirode_model {
@@ -406,7 +421,19 @@ struct sirode_model {
   template <typename real>
   struct shared {
     real N, I0, beta, gamma;
+    real inv_N;  // RN(1 / N): the divisions by N below are div_rn
   };
+
+  template <typename real>
+  __device__ __forceinline__ static shared<real> load(const double* pr) {
+    shared<real> sh;
+    sh.N = static_cast<real>(pr[0]);
+    sh.I0 = static_cast<real>(pr[1]);
+    sh.beta = static_cast<real>(pr[2]);
+    sh.gamma = static_cast<real>(pr[3]);
+    sh.inv_N = 1 / sh.N;
+    return sh;
+  }
 
   template <typename real>
   __device__ __forceinline__ static void initial(const shared<real>& sh, real* y) {
@@ -421,7 +448,7 @@ struct sirode_model {
   __device__ __forceinline__ static void rhs(const real* y, const shared<real>& sh, real* dy) {
     const real S = y[0];
     const real I = y[1];
-    const real rate_SI = sh.beta * S * I / sh.N;
+    const real rate_SI = div_rn(sh.beta * S * I, sh.N, sh.inv_N);
     const real rate_IR = sh.gamma * I;
     dy[0] = -rate_SI;
     dy[1] = rate_SI - rate_IR;
@@ -438,9 +465,9 @@ struct sirode_model {
                                                   real* ds) {
     const real S = y[0];
     const real I = y[1];
-    real d_SI = sh.beta * (s[0] * I + S * s[1]) / sh.N;
+    real d_SI = div_rn(sh.beta * (s[0] * I + S * s[1]), sh.N, sh.inv_N);
     real d_IR = sh.gamma * s[1];
-    if (param == 0) d_SI += S * I / sh.N;
+    if (param == 0) d_SI += div_rn(S * I, sh.N, sh.inv_N);
     if (param == 1) d_IR += I;
     ds[0] = -d_SI;
     ds[1] = d_SI - d_IR;

@@ -42,6 +42,10 @@ constexpr double D1 = -12715105075.0 / 11282082432.0, D3 = 87487479700.0 / 32700
                  D6 = -1453857185.0 / 822651844.0, D7 = 69997945.0 / 29380423.0;
 }  // namespace dp
 
+// log(1e-4): the floor of the old error in the step-size controller
+// (solver.hpp:210-211, max(error_old, 1e-4)), correctly rounded
+constexpr double kLogFacMin = -9.210340371976182;
+
 // Per-particle integrator state.  Only the reset variable's dense-output
 // coefficients are kept: they are the only ones apply_zero_every reads.
 template <typename real>
@@ -50,6 +54,7 @@ struct ode_state {
   real dydt[NV];
   double step_size;
   double error;
+  double log_fac_old;  // log(max(error, 1e-4)), the controller's memory
   double t0, h;  // last accepted step
   real c1, c2, c3, c4, c5;  // dense output of variable zero_first
   uint32_t n_steps, n_accepted, n_rejected;
@@ -190,7 +195,8 @@ __device__ __forceinline__ double try_step(double t, double h, const ode_state<r
                                     fabs(static_cast<double>(y_next[i]))));
     err += sk * sk;
   }
-  return sqrt(err / NV);
+  static_assert(NV == 5, "the mean below divides by 5");
+  return sqrt(div_rn(err, 5.0, 0.2));
 }
 
 // solver::step: returns the new time, or NaN with *code set on failure.
@@ -200,6 +206,7 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
                            uint32_t& code) {
   const double facc1 = 1 / ctl.factor_min;
   const double facc2 = 1 / ctl.factor_max;
+  const double inv_safe = 1 / ctl.factor_safe;
   bool reject = false;
   bool truncated = false;
   double h = s.step_size;
@@ -226,8 +233,15 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
     const double err =
         try_step<real, SENS>(t, h, s, sh, ctl, y_next, k2, c5, sy_next, ks2, sc5);
     s.n_steps++;
-    const double fac11 = pow(err, ctl.constant);
-    if (err <= 1) {
+    // The controller's err^constant, over max(error_old, 1e-4)^beta when
+    // the step is accepted (solver.hpp:182, 210-212), as one exp of logs: the
+    // log of the old error is kept from its own step.  log(0) = -inf and
+    // NaN give the same factors as pow.
+    const double log_err = log(err);
+    const bool accept = err <= 1;
+    const double fac11 =
+        exp(ctl.constant * log_err - (accept ? ctl.beta * s.log_fac_old : 0.0));
+    if (accept) {
       // dense output + accept (solver.hpp:335-357)
       const int z = sirode_model::zero_first;
       const real ydiff = y_next[z] - s.y[z];
@@ -258,13 +272,12 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
       }
       s.n_accepted++;
       if (!truncated) {
-        const double fac_old = fmax(s.error, 1e-4);
-        double fac = fac11 / pow(fac_old, ctl.beta);
-        fac = fmax(fmin(fac / ctl.factor_safe, facc1), facc2);
+        const double fac = fmax(fmin(div_rn(fac11, ctl.factor_safe, inv_safe), facc1), facc2);
         const double h_new = h / fac;
         const double h_max = reject ? h : ctl.step_size_max;
         s.step_size = fmin(h_new, h_max);
         s.error = err;
+        s.log_fac_old = fmax(log_err, kLogFacMin);
       }
       return truncated ? t_end : t + h;
     }
@@ -272,7 +285,7 @@ __device__ double ode_step(double t, double t_end, ode_state<real>& s,
     if (s.n_accepted >= 1) {
       s.n_rejected++;
     }
-    h /= fmin(facc1, fac11 / ctl.factor_safe);
+    h /= fmin(facc1, div_rn(fac11, ctl.factor_safe, inv_safe));
   }
 }
 
@@ -340,6 +353,7 @@ __device__ void ode_initialise(ode_state<real>& s, const sirode_model::shared<re
                                const ode_control_dev& ctl, uint32_t& code) {
   s.step_size = 0;
   s.error = 0;
+  s.log_fac_old = kLogFacMin;
   s.n_steps = 0;
   s.n_accepted = 0;
   s.n_rejected = 0;
@@ -383,11 +397,7 @@ __global__ void __launch_bounds__(256) dp5_unfilter_kernel(ode_launch args) {
   const int g = i / args.n_particles;
   if (args.active && !args.active[g]) return;
   const double* pr = args.pars + 4 * g;
-  sirode_model::shared<real> sh;
-  sh.N = static_cast<real>(pr[0]);
-  sh.I0 = static_cast<real>(pr[1]);
-  sh.beta = static_cast<real>(pr[2]);
-  sh.gamma = static_cast<real>(pr[3]);
+  const sirode_model::shared<real> sh = sirode_model::load<real>(pr);
   const ode_control_dev ctl = args.control;
   real* state = static_cast<real*>(args.state);
 
@@ -476,11 +486,7 @@ __global__ void __launch_bounds__(256) dp5_system_kernel(ode_sys_launch a) {
   if (i >= a.n_total) return;
   const int g = i / a.n_particles;
   const double* pr = a.pars + 4 * g;
-  sirode_model::shared<double> sh;
-  sh.N = pr[0];
-  sh.I0 = pr[1];
-  sh.beta = pr[2];
-  sh.gamma = pr[3];
+  const sirode_model::shared<double> sh = sirode_model::load<double>(pr);
   const size_t NT = static_cast<size_t>(a.n_total);
   ode_state<double> s;
   for (int k = 0; k < NV; ++k) s.y[k] = a.state[k * NT + i];
@@ -504,6 +510,7 @@ __global__ void __launch_bounds__(256) dp5_system_kernel(ode_sys_launch a) {
     for (int k = 0; k < NV; ++k) s.dydt[k] = in.dydt[k];
     s.step_size = in.step_size;
     s.error = in.error;
+    s.log_fac_old = fmax(log(in.error), kLogFacMin);
     s.n_steps = in.n_steps;
     s.n_accepted = in.n_accepted;
     s.n_rejected = in.n_rejected;
