@@ -50,54 +50,72 @@ __device__ __forceinline__ double wave_scan(double v, int lane) {
   return v;
 }
 
+// Cross-wave steps: each wave writes one value to scratch[wave]; after the
+// barrier every lane reads scratch[lane % n_wave], so each aligned segment
+// of n_wave lanes holds all the waves' values and combines them with
+// log2(n_wave) lane exchanges (n_wave = BLOCK / 64, a power of two up to
+// 16) instead of a loop of n_wave LDS reads per lane.
 template <int BLOCK>
 __device__ __forceinline__ double block_max(double v, double* scratch) {
+  constexpr int n_wave = BLOCK / kWave;
+  static_assert(n_wave >= 1 && n_wave <= 16 && (n_wave & (n_wave - 1)) == 0, "block size");
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   v = wave_max(v);
+  if constexpr (n_wave == 1) return v;
   if (lane == 0) scratch[wave] = v;
   __syncthreads();
-  double r = scratch[0];
+  double r = scratch[lane % n_wave];
 #pragma unroll
-  for (int w = 1; w < BLOCK / kWave; ++w) r = fmax(r, scratch[w]);
+  for (int o = n_wave / 2; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, kWave));
   __syncthreads();
   return r;
 }
 
 template <int BLOCK>
 __device__ __forceinline__ int block_or(int v, double* scratch) {
+  constexpr int n_wave = BLOCK / kWave;
   int* s = reinterpret_cast<int*>(scratch);
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   const int any = __any(v) ? 1 : 0;
+  if constexpr (n_wave == 1) return any;
   if (lane == 0) s[wave] = any;
   __syncthreads();
-  int r = 0;
-#pragma unroll
-  for (int w = 0; w < BLOCK / kWave; ++w) r |= s[w];
+  const int r = __any(s[lane % n_wave]) ? 1 : 0;
   __syncthreads();
   return r;
 }
 
 // Exclusive scan of one value per thread across the block; returns the
-// exclusive prefix and writes the block total to *total.
+// exclusive prefix and writes the block total to *total.  The waves'
+// totals are scanned by every n_wave-lane segment (Hillis-Steele), so the
+// summation depth is log2(64) + log2(n_wave) + 1.
 template <int BLOCK>
 __device__ __forceinline__ double block_exclusive_scan(double v, double* scratch,
                                                        double* total) {
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   constexpr int n_wave = BLOCK / kWave;
+  static_assert(n_wave >= 1 && n_wave <= 16 && (n_wave & (n_wave - 1)) == 0, "block size");
   const double inc = wave_scan(v, lane);
+  if constexpr (n_wave == 1) {
+    *total = __shfl(inc, kWave - 1, kWave);
+    return inc - v;
+  }
   if (lane == kWave - 1) scratch[wave] = inc;
   __syncthreads();
-  double wave_prefix = 0;
-  double tot = 0;
+  const int sl = lane % n_wave;
+  double s = scratch[sl];
 #pragma unroll
-  for (int w = 0; w < n_wave; ++w) {
-    const double s = scratch[w];
-    if (w < wave) wave_prefix += s;
-    tot += s;
+  for (int o = 1; o < n_wave; o <<= 1) {
+    const double n = __shfl_up(s, o, n_wave);
+    if (sl >= o) s += n;
   }
+  // lane k of the segment holds the sum of waves 0..k
+  const double tot = __shfl(s, n_wave - 1, kWave);
+  const double before = __shfl(s, wave > 0 ? wave - 1 : 0, kWave);
+  const double wave_prefix = wave > 0 ? before : 0.0;
   __syncthreads();
   *total = tot;
   return wave_prefix + inc - v;
@@ -170,15 +188,14 @@ __device__ resample_result block_resample(double (&w)[PPT], int (&idx)[PPT], int
       mx = fmax(mx, w[k]);
     }
   }
-  mx = block_max<BLOCK>(mx, scratch);
-
-  double u = 0;
+  // the uniform is drawn here whatever the maximum (filter.hpp:153 draws it
+  // for every resampled group); its slot is read after the maximum's
+  // barrier, and nothing writes it again in this call
   if (tid == 0) {
     scratch[kScratchDoubles - 1] = u_draw();
   }
-  __syncthreads();
-  u = scratch[kScratchDoubles - 1];
-  __syncthreads();
+  mx = block_max<BLOCK>(mx, scratch);
+  const double u = scratch[kScratchDoubles - 1];
 
   if (mx == neg_inf) {
     // Every particle impossible: the reference adds -Inf and its systematic
