@@ -231,13 +231,19 @@ __device__ __forceinline__ float f_log(float x) {
   return __builtin_amdgcn_logf(x) * 0.693147181f;
 }
 
-// log(1 + x) accurate for small |x| from the hardware log (Goldberg 1991,
-// "What every computer scientist should know about floating-point
-// arithmetic", theorem 4): log(u) * x / (u - 1) with u = 1 + x rounded.
+// log(1 + x) accurate for small |x| from the hardware log and no
+// reciprocal.  With u = 1 + x rounded and d = u - 1 (exact for u in
+// [1/2, 2^24)), log(1 + x) = log(u) + log(1 + (x - d) / u), where
+// |x - d| / u <= 2^-24, so the second term is (x - d) / u to float
+// precision.  For |x| < 1/2, 1 / u is taken as 1 - x (error x^2 / u; where
+// u rounds to 1 the result is x - x^2, log1p(x) to within x^3).  For
+// x <= -1/2, 1 + x is exact and the term is zero; for x >= 1 it is
+// dropped, an absolute error below 2^-24 against log(u) >= log 2.
 __device__ __forceinline__ float f_log1p(float x) {
   const float u = 1.0f + x;
   const float d = u - 1.0f;
-  return d == 0.0f ? x : f_log(u) * (x * f_rcp(d));
+  return __builtin_fmaf(__builtin_amdgcn_logf(u), 0.693147181f,
+                        (x - d) * fmaxf(1.0f - x, 0.0f));
 }
 
 // Two uniforms on the 2^-24 grid from one 64-bit output: bits 40..63 and
@@ -249,22 +255,31 @@ __device__ __forceinline__ void random_pair_f32(rng256& r, float& u, float& v) {
 }
 
 // Stirling-series remainder in FP32, log(k!) - (k + 1/2) log(k + 1) + (k + 1)
-// - log(2 pi) / 2: exact values for k < 4, the series 1/12 - 1/360 x^-2 +
-// 1/1260 x^-4 over x = k + 1 above (truncation below 8e-9 from k = 4),
-// without a per-lane branch so that lanes on either side run together; the
-// exact values are selected only when some lane of the wavefront needs one
-// (BTRS rarely evaluates the tail below k = 4).
-__device__ __forceinline__ float stirling_tail_f32(float k) {
-  const float r = f_rcp(k + 1);
+// - log(2 pi) / 2, for a pair of arguments a and b given r_a = 1/(a + 1)
+// and r_b = 1/(b + 1) (the callers form both from one reciprocal of the
+// product): the series 1/12 - 1/360 x^-2 + 1/1260 x^-4 over x = k + 1
+// (truncation below 8e-9 from k = 4), exact values for k < 4, selected
+// without a per-lane branch and only when some lane of the wavefront needs
+// one (BTRS rarely evaluates the tail below k = 4).
+__device__ __forceinline__ float stirling_series_f32(float r) {
   const float r2 = r * r;
-  float t = (0.083333333f - (0.0027777778f - r2 * 0.00079365079f) * r2) * r;
-  if (__any(k < 4)) {
-    t = k == 3 ? 0.020790672103765093f : t;
-    t = k == 2 ? 0.02767792568499834f : t;
-    t = k == 1 ? 0.0413406959554093f : t;
-    t = k == 0 ? 0.08106146679532726f : t;
-  }
+  return (0.083333333f - (0.0027777778f - r2 * 0.00079365079f) * r2) * r;
+}
+__device__ __forceinline__ float stirling_small_f32(float k, float t) {
+  t = k == 3 ? 0.020790672103765093f : t;
+  t = k == 2 ? 0.02767792568499834f : t;
+  t = k == 1 ? 0.0413406959554093f : t;
+  t = k == 0 ? 0.08106146679532726f : t;
   return t;
+}
+__device__ __forceinline__ float stirling_tail_pair_f32(float a, float b, float ra, float rb) {
+  float ta = stirling_series_f32(ra);
+  float tb = stirling_series_f32(rb);
+  if (__any(a < 4 || b < 4)) {
+    ta = stirling_small_f32(a, ta);
+    tb = stirling_small_f32(b, tb);
+  }
+  return ta + tb;
 }
 
 // 1/j for j = 0..115 (entry 0 unused), correctly rounded.
@@ -346,6 +361,9 @@ struct btrs_f32 {
   float m, rm, rnm, L, tail_m;       // mode terms of the acceptance test
 };
 
+// Setup: four quarter-rate instructions (the square root, 1/b, one
+// reciprocal of q (m + 1) (n - m + 1) that gives 1/(m + 1), 1/(n - m + 1)
+// and 1/(q (m + 1)), and the log of L).
 __device__ __forceinline__ btrs_f32 btrs_setup_f32(float n, float p) {
   btrs_f32 s;
   const float q = 1 - p;
@@ -360,15 +378,22 @@ __device__ __forceinline__ btrs_f32 btrs_setup_f32(float n, float p) {
   s.two_a = 2 * s.a;
   s.alpha = (2.83f + 5.1f * rb) * spq;
   s.m = floorf((n + 1) * p);
-  s.rm = f_rcp(s.m + 1);
-  s.rnm = f_rcp(n - s.m + 1);
-  s.L = f_log(p * (n - s.m + 1) * f_rcp(q * (s.m + 1)));
-  s.tail_m = stirling_tail_f32(s.m) + stirling_tail_f32(n - s.m);
+  const float m1 = s.m + 1;
+  const float nm1 = n - s.m + 1;
+  const float r = f_rcp(q * m1 * nm1);
+  const float qr = q * r;
+  s.rm = nm1 * qr;
+  s.rnm = m1 * qr;
+  s.L = f_log(p * nm1 * (nm1 * r));
+  s.tail_m = stirling_tail_pair_f32(s.m, n - s.m, s.rm, s.rnm);
   return s;
 }
 
 // One BTRS proposal: draws one (u, v) pair; true with the draw in k when
-// accepted (by the squeeze or by the full test).
+// accepted (by the squeeze or by the full test).  The full test compares
+// v alpha us^2 <= (a + b us^2) exp(ratio) (the same as log(v alpha /
+// (a / us^2 + b)) <= ratio, with a + b us^2 > 0) and forms both Stirling
+// tails from one reciprocal: four quarter-rate instructions.
 __device__ __forceinline__ bool btrs_try_f32(const btrs_f32& s, rng256& rng, float& k) {
   float u, v;
   random_pair_f32(rng, u, v);
@@ -378,12 +403,14 @@ __device__ __forceinline__ bool btrs_try_f32(const btrs_f32& s, rng256& rng, flo
   bool accept = us >= 0.07f && v <= s.v_r;
   if (!accept && k >= 0 && k <= s.n) {
     const float d = k - s.m;
+    const float k1 = k + 1;
+    const float nk1 = s.n - k + 1;
+    const float r = f_rcp(k1 * nk1);
     const float ratio = d * s.L - (k + 0.5f) * f_log1p(d * s.rm) -
                         (s.n - k + 0.5f) * f_log1p(-d * s.rnm) + s.tail_m -
-                        stirling_tail_f32(k) - stirling_tail_f32(s.n - k);
+                        stirling_tail_pair_f32(k, s.n - k, nk1 * r, k1 * r);
     const float us2 = us * us;
-    const float lv = f_log(v * s.alpha * us2 * f_rcp(s.a + s.b * us2));
-    accept = lv <= ratio;
+    accept = v * s.alpha * us2 <= (s.a + s.b * us2) * __builtin_amdgcn_exp2f(ratio * 1.44269504f);
   }
   return accept;
 }
