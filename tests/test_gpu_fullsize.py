@@ -6,9 +6,9 @@ not depend on size.
   sampled groups (each group is an independent filter over its own block of
   RNG streams, exported from the device before the run); the FP32 run is
   compared with the FP64 run group by group (paired differences centred on
-  zero) and, for the largest- and smallest-population groups, over 1000
-  replicates in each precision; every final state conserves the population
-  exactly.
+  zero), over 32 replicates of every group in each precision, and, for the
+  largest- and smallest-population groups, over 1000 replicates in each
+  precision; every final state conserves the population exactly.
 * SIR ODE unfilter, 65,536 parameter sets x 365 observations: every
   log-likelihood and final state against the oracle.
 * Age-structured SEIR16 filter, 128 groups x 8192 particles x 200 weekly
@@ -87,6 +87,40 @@ def test_sir_full_size_fp32_paired_with_fp64(sir_c2):
     d = sir_c2["fp32"]["ll"] - sir_c2["fp64"]["ll"]
     se = d.std(ddof=1) / np.sqrt(d.size)
     assert abs(d.mean()) < 2 * se, (d.mean(), se)
+
+
+def test_sir_full_size_fp32_unbiased_all_groups(gpu_lib):
+    # every one of the 512 groups, 32 FP32 replicate filters against 32 of
+    # the FP64 filter (the oracle draw for draw), each replicate on its own
+    # streams: the summed mean differences over their standard error below
+    # 3.3 (p = 0.001), the per-group z spread near 1 (a bias or a variance
+    # change confined to some parameter regimes widens it), and the mean
+    # log ratio of the two variances within 4 standard errors of zero.
+    # Measured: summed z 1.33, per-group z standard deviation 1.03 (5.7 %
+    # beyond 2), log variance ratio -0.017 +- 0.018.
+    G, K, P = 512, 32, 2048
+    time = np.arange(1.0, 366.0)
+    pars = sir_groups(G)
+    data = np.repeat(sir_data(pars, time, 7), K, axis=1)
+    plist = [dict(zip(SIR_NAMES, p)) for p in pars for _ in range(K)]
+    ll = {}
+    for prec, seed in (("fp32", 303), ("fp64", 404)):
+        obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data},
+                                         P, n_groups=G * K, dt=0.25, seed=seed, precision=prec)
+        ll[prec] = np.atleast_1d(gpu_lib.dust_likelihood_run(obj, plist)).reshape(G, K)
+        del obj
+    a, b = ll["fp32"], ll["fp64"]
+    va, vb = a.var(axis=1, ddof=1), b.var(axis=1, ddof=1)
+    se = np.sqrt(va / K + vb / K)
+    d = a.mean(axis=1) - b.mean(axis=1)
+    z_sum = d.sum() / np.sqrt(np.sum(se ** 2))
+    z = d / se
+    lr = np.log(va / vb)
+    print(f"z_sum {z_sum:.3f} z_sd {z.std():.3f} frac|z|>2 {np.mean(np.abs(z) > 2):.4f} "
+          f"log var ratio {lr.mean():.4f} +- {lr.std(ddof=1) / np.sqrt(G):.4f}")
+    assert abs(z_sum) < 3.3, z_sum
+    assert 0.88 < z.std() < 1.15, z.std()
+    assert abs(lr.mean()) < 4 * lr.std(ddof=1) / np.sqrt(G), lr.mean()
 
 
 def test_sir_full_size_fp32_unbiased_per_group(gpu_lib):
