@@ -233,6 +233,16 @@ def timed_runs(lib, obj, plist, steps, warmup):
     return np.atleast_1d(ll), _time.perf_counter() - t0, gpu_ms
 
 
+def pipeline_shape(obj, engine):
+    """Launch chains and resample workgroup width of a pipeline-engine filter
+    (the library's defaults, 2 and 1024, unless set or autotuned)."""
+    if engine != "pipeline":
+        return None
+    chains = getattr(obj, "chains", None)
+    block = getattr(obj, "resample_block", None)
+    return {"chains": chains or 2, "resample_block": block or 1024}
+
+
 def filter_roofline(lib, obj, plist, G, P, n_obs, S, s_x, engine_used, run_ms, key=None):
     b_obs = 2 * S * s_x + 2 * 32 + 2 * 8 + 2 * 4
     real = "float" if s_x == 4 else "double"
@@ -300,7 +310,8 @@ def bench_sir(lib, args):
         "config": {"workload": "batched stochastic SIR bootstrap filter",
                    "groups": G, "particles": P, "n_obs": int(time.size), "dt": dt,
                    "precision": args.precision, "engine": stats["engine"],
-                   "launch": stats["launch"] if stats["engine"] == "persistent" else None},
+                   "launch": stats["launch"] if stats["engine"] == "persistent" else None,
+                   "pipeline": pipeline_shape(obj, stats["engine"])},
         "autotune": tuned,
         "likelihood_evals_per_s": G * args.steps / elapsed,
         "gpu_ms_per_step": float(np.mean(gpu_ms)),
@@ -531,7 +542,8 @@ def bench_seir16(lib, args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded simulation of each group's model)",
            "config": {"workload": "age-structured SEIR16 bootstrap filter (MFMA FOI)",
-                      "groups": G, "particles": P, "n_obs": n_obs, "dt": dt},
+                      "groups": G, "particles": P, "n_obs": n_obs, "dt": dt,
+                      "pipeline": pipeline_shape(obj, "pipeline")},
            "likelihood_evals_per_s": batched_evals, "plain_evals_per_s": plain_evals,
            "batched_over_plain": batched_evals / plain_evals, "autotune": tuned,
            "mean_ll": float(np.mean(ll)), "roofline": rl}

@@ -51,6 +51,7 @@ _PROTOS = {
     "dust2_filter_last_stats": (C.c_int, [VP, D, ERR, ERRLEN]),
     "dust2_filter_set_engine": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, ERR, ERRLEN]),
     "dust2_filter_set_pipeline_chains": (C.c_int, [VP, C.c_int, ERR, ERRLEN]),
+    "dust2_filter_set_pipeline_resample": (C.c_int, [VP, C.c_int, ERR, ERRLEN]),
     "dust2_filter_set_beta_schedule": (C.c_int, [VP, C.c_int, D, C.c_size_t, D, C.c_size_t,
                                                  ERR, ERRLEN]),
     "dust2_filter_autotune": (C.c_int, [VP, SZ, C.c_size_t, D, C.c_size_t, SZ, ERR, ERRLEN]),

@@ -695,6 +695,9 @@ struct dust2_filter {
   // chain_stream[c - 1] and counts its errors in counter 1 + c
   static constexpr int kMaxChains = 4;
   int pipe_chains = 2;
+  // threads per resample workgroup (1024, 512 or 256; the particles per
+  // thread follow from the group size)
+  int pipe_resample_block = 1024;
   hipStream_t chain_stream[kMaxChains - 1] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kMaxChains - 1] = {};
@@ -917,7 +920,8 @@ void pipeline_launches(dust2_filter* h, int n_slots, bool initial_given, bool sa
         hip_check(launch_mb_resample(h->d_mb_tasks.get<mb_resample>() + first, count, P, t, s),
                   "resample kernels");
       } else {
-        hip_check(launch_pipe_resample(slots, count, P, t, s), "resample kernel");
+        hip_check(launch_pipe_resample(slots, count, P, t, s, h->pipe_resample_block),
+                  "resample kernel");
       }
       mark();
     }
@@ -1011,7 +1015,8 @@ void pipeline_run(dust2_filter* h, const std::vector<size_t>& idx, bool initial_
   } else if (h->use_graph) {
     const std::vector<uint64_t> key = {static_cast<uint64_t>(n_slots), h->P, h->n_time,
                                        initial_given, save_traj, h->fp64,
-                                       static_cast<uint64_t>(pipeline_chains(h, n_slots))};
+                                       static_cast<uint64_t>(pipeline_chains(h, n_slots)),
+                                       static_cast<uint64_t>(h->pipe_resample_block)};
     if (!h->graph || key != h->graph_key) {
       if (h->graph) {
         hip_check(hipGraphExecDestroy(h->graph), "hipGraphExecDestroy");
@@ -1718,6 +1723,24 @@ int dust2_filter_set_pipeline_chains(dust2_filter* h, int chains, char* err, siz
   }
 }
 
+int dust2_filter_set_pipeline_resample(dust2_filter* h, int block, char* err, size_t errlen) {
+  try {
+    check_filter(h);
+    if (block != 1024 && block != 512 && block != 256) {
+      throw std::runtime_error("'block' must be 1024, 512 or 256");
+    }
+    if (!h->multi_block && pipe_resample_ppt(static_cast<int>(h->P), block) == 0) {
+      throw std::runtime_error("a resample workgroup of " + std::to_string(block) +
+                               " threads does not hold " + std::to_string(h->P) +
+                               " particles");
+    }
+    h->pipe_resample_block = block;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, errlen);
+  }
+}
+
 int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
                                 char* err, size_t errlen) {
   try {
@@ -1757,6 +1780,7 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       int engine;
       launch_config cfg;
       int chains;
+      int rblock;
     };
     std::vector<candidate> cand;
     if (h->beta_n == 0 && P <= discrete_filter_max_particles(h->model)) {
@@ -1766,13 +1790,15 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       for (const auto& sh : shapes) {
         const int cap = sh[0] * sh[1];
         if (cap >= P && cap < 2 * P + 64) {
-          cand.push_back({DUST2_ENGINE_PERSISTENT, launch_config{sh[0], sh[1]}, 1});
+          cand.push_back({DUST2_ENGINE_PERSISTENT, launch_config{sh[0], sh[1]}, 1, 1024});
         }
       }
     }
     for (int chains = 1; chains <= dust2_filter::kMaxChains; ++chains) {
-      if (chains == 1 || idx.size() >= static_cast<size_t>(chains)) {
-        cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, chains});
+      if (chains > 1 && idx.size() < static_cast<size_t>(chains)) continue;
+      for (const int rblock : {1024, 512, 256}) {
+        if (rblock != 1024 && (h->multi_block || pipe_resample_ppt(P, rblock) == 0)) continue;
+        cand.push_back({DUST2_ENGINE_PIPELINE, h->cfg, chains, rblock});
       }
     }
     // tuning runs must leave the object as they found it: the particle and
@@ -1787,6 +1813,7 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
     const launch_config cfg0 = h->cfg;
     const bool graph0 = h->use_graph, profile0 = h->profile;
     const int chains0 = h->pipe_chains;
+    const int rblock0 = h->pipe_resample_block;
     std::vector<double> ll(idx.size());
     std::vector<double> ms(cand.size(), 0.0);
     std::string failure;
@@ -1794,6 +1821,7 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       h->engine = cand[i].engine;
       h->cfg = cand[i].cfg;
       h->pipe_chains = cand[i].chains;
+      h->pipe_resample_block = cand[i].rblock;
       h->use_graph = true;
       h->profile = false;
       for (int rep = 0; rep < 2; ++rep) {
@@ -1820,6 +1848,7 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
       h->engine = engine0;
       h->cfg = cfg0;
       h->pipe_chains = chains0;
+      h->pipe_resample_block = rblock0;
       throw std::runtime_error(failure);
     }
     size_t best = 0;
@@ -1829,11 +1858,13 @@ int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_i
     h->engine = cand[best].engine;
     h->cfg = h->engine == DUST2_ENGINE_PERSISTENT ? cand[best].cfg : cfg0;
     h->pipe_chains = h->engine == DUST2_ENGINE_PIPELINE ? cand[best].chains : chains0;
+    h->pipe_resample_block =
+        h->engine == DUST2_ENGINE_PIPELINE ? cand[best].rblock : rblock0;
     const size_t n = std::min(report_rows, cand.size());
     for (size_t i = 0; report && i < n; ++i) {
       const bool pers = cand[i].engine == DUST2_ENGINE_PERSISTENT;
       report[4 * i + 0] = cand[i].engine;
-      report[4 * i + 1] = pers ? cand[i].cfg.block : 0;
+      report[4 * i + 1] = pers ? cand[i].cfg.block : cand[i].rblock;
       report[4 * i + 2] = pers ? cand[i].cfg.ppt : cand[i].chains;
       report[4 * i + 3] = ms[i];
     }

@@ -17,6 +17,14 @@
 // sum is ambiguous; if any point in the group is, the group is redone with
 // the sequential sum computed by one lane.  Resampling indices are thereby
 // identical to the reference's for identical weights, at scan speed.
+//
+// The group total that gives the log-likelihood increment, log(tot / P) +
+// max, and the systematic points is the aligned pairwise sum of the
+// weights (within the same bound of the sequential total), so a group's
+// log-likelihood has the same bits whatever workgroup size and particles
+// per thread resample it: the pipeline at any resample width, the
+// persistent kernel at any launch shape and a server launch sized for its
+// largest request agree exactly.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -91,19 +99,36 @@ __device__ __forceinline__ int block_or(int v, double* scratch) {
 // exclusive prefix and writes the block total to *total.  The waves'
 // totals are scanned by every n_wave-lane segment (Hillis-Steele), so the
 // summation depth is log2(64) + log2(n_wave) + 1.
+//
+// With pair_total, also the aligned pairwise sum of the threads' values
+// (lane butterflies within the wave, then across the waves), in every
+// lane.  When each thread's value is itself the pairwise sum of its PPT
+// consecutive elements (PPT a power of two), this is the pairwise sum of
+// the elements over the aligned binary tree of their indices, zero-padded:
+// the same bits whatever the block size, particles per thread or padding,
+// since a subtree of zeros adds exactly 0.
 template <int BLOCK>
-__device__ __forceinline__ double block_exclusive_scan(double v, double* scratch,
-                                                       double* total) {
+__device__ __forceinline__ double block_exclusive_scan(double v, double* scratch, double* total,
+                                                       double* pair_total = nullptr) {
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   constexpr int n_wave = BLOCK / kWave;
   static_assert(n_wave >= 1 && n_wave <= 16 && (n_wave & (n_wave - 1)) == 0, "block size");
   const double inc = wave_scan(v, lane);
+  double pw = v;
+  if (pair_total) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) pw += __shfl_xor(pw, o, kWave);
+  }
   if constexpr (n_wave == 1) {
     *total = __shfl(inc, kWave - 1, kWave);
+    if (pair_total) *pair_total = pw;
     return inc - v;
   }
-  if (lane == kWave - 1) scratch[wave] = inc;
+  if (lane == kWave - 1) {
+    scratch[wave] = inc;
+    if (pair_total) scratch[16 + wave] = pw;
+  }
   __syncthreads();
   const int sl = lane % n_wave;
   double s = scratch[sl];
@@ -112,6 +137,12 @@ __device__ __forceinline__ double block_exclusive_scan(double v, double* scratch
     const double n = __shfl_up(s, o, n_wave);
     if (sl >= o) s += n;
   }
+  if (pair_total) {
+    double ps = scratch[16 + sl];
+#pragma unroll
+    for (int o = 1; o < n_wave; o <<= 1) ps += __shfl_xor(ps, o, kWave);
+    *pair_total = ps;
+  }
   // lane k of the segment holds the sum of waves 0..k
   const double tot = __shfl(s, n_wave - 1, kWave);
   const double before = __shfl(s, wave > 0 ? wave - 1 : 0, kWave);
@@ -119,6 +150,17 @@ __device__ __forceinline__ double block_exclusive_scan(double v, double* scratch
   __syncthreads();
   *total = tot;
   return wave_prefix + inc - v;
+}
+
+// Pairwise sum of N consecutive values (N a power of two) over the aligned
+// binary tree of their indices.
+template <int N>
+__device__ __forceinline__ double pairwise_sum(const double* e) {
+  if constexpr (N == 1) {
+    return e[0];
+  } else {
+    return pairwise_sum<N / 2>(e) + pairwise_sum<N / 2>(e + N / 2);
+  }
 }
 
 // First index j in [0, P) with cum[j] >= x, or P if none.
@@ -206,16 +248,19 @@ __device__ resample_result block_resample(double (&w)[PPT], int (&idx)[PPT], int
     return res;
   }
 
-  // exp-scaled weights, local inclusive prefix, block scan
+  // exp-scaled weights, block scan for the running sums, and the group's
+  // total as the aligned pairwise sum of the weights: the log-likelihood
+  // and the systematic points take that total, so they do not depend on
+  // the workgroup size or particles per thread (nor on the engine)
+  static_assert((PPT & (PPT - 1)) == 0, "particles per thread");
   double e[PPT];
-  double local = 0;
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     e[k] = (p0 + k < P) ? exp(w[k] - mx) : 0.0;
-    local += e[k];
   }
-  double tot;
-  const double base = block_exclusive_scan<BLOCK>(local, scratch, &tot);
+  const double local = pairwise_sum<PPT>(e);
+  double scan_tot, tot;
+  const double base = block_exclusive_scan<BLOCK>(local, scratch, &scan_tot, &tot);
   {
     double run = base;
 #pragma unroll
@@ -264,7 +309,8 @@ __device__ resample_result block_resample(double (&w)[PPT], int (&idx)[PPT], int
     }
   }
 
-  // Sequential fallback: exact left-to-right running sums.
+  // Sequential fallback: exact left-to-right running sums for the points;
+  // the log-likelihood keeps the pairwise total.
   res.n_fallback = 1;
   __syncthreads();
 #pragma unroll
@@ -293,7 +339,7 @@ __device__ resample_result block_resample(double (&w)[PPT], int (&idx)[PPT], int
       idx[k] = j < P ? j : P - 1;
     }
   }
-  res.ll_increment = log(tot_seq / Pd) + mx;
+  res.ll_increment = log(tot / Pd) + mx;
   __syncthreads();
   return res;
 }

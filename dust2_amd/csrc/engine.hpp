@@ -144,7 +144,11 @@ __host__ __device__ inline bool pipe_slot_interval(const pipe_slot& sd, int& t, 
   return true;
 }
 
-int pipe_resample_ppt(int max_particles);
+// Particles per thread of the resample workgroup of `block` threads (1024,
+// 512 or 256) for groups of up to max_particles, or 0 when that width does
+// not hold them (1024 threads hold up to 16,384 particles, 512 and 256
+// threads up to 8 per thread).
+int pipe_resample_ppt(int max_particles, int block = 1024);
 hipError_t launch_pipe_reset(const pipe_slot* d_slots, int n_slots, hipStream_t stream);
 // out[2i] = slot i's log-likelihood, out[2i+1] = its error count
 hipError_t launch_slot_status(const pipe_slot* d_slots, int n_slots, double* out,
@@ -155,7 +159,7 @@ hipError_t launch_discrete_advance(int model, bool fp64, const pipe_slot* d_slot
                                    int max_particles, int t, int parity, uint32_t mode,
                                    hipStream_t stream);
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
-                                hipStream_t stream);
+                                hipStream_t stream, int block = 1024);
 // The advance launch over a block map: entry b = slot << kBlockBits | block
 // (the block-th run of discrete_advance_block(model) particles of that
 // slot), kNoBlock for an unused entry; n_blocks entries, one workgroup

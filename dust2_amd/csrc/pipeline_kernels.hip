@@ -87,12 +87,12 @@ __global__ void reset_ll_kernel(const pipe_slot* __restrict__ slots, int n_slots
 
 // Particles per thread of the 1024-thread resample workgroup; 0 when a
 // group is too large for one workgroup.
-int pipe_resample_ppt(int max_particles) {
-  if (max_particles <= 1024) return 1;
-  if (max_particles <= 2048) return 2;
-  if (max_particles <= 4096) return 4;
-  if (max_particles <= 8192) return 8;
-  if (max_particles <= 16384) return 16;
+int pipe_resample_ppt(int max_particles, int block) {
+  // widths offered: 1024 threads up to 16 particles each, 512 and 256 up to 8
+  const int max_ppt = block == 1024 ? 16 : (block == 512 || block == 256) ? 8 : 0;
+  for (int ppt = 1; ppt <= max_ppt; ppt *= 2) {
+    if (block * ppt >= max_particles) return ppt;
+  }
   return 0;
 }
 
@@ -110,34 +110,48 @@ hipError_t launch_pipe_reset(const pipe_slot* d_slots, int n_slots, hipStream_t 
   return hipGetLastError();
 }
 
+namespace {
+
+template <int BLOCK, int PPT>
+void launch_resample_shape(const pipe_slot* d_slots, int n_slots, size_t lds, int t,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL((resample_kernel<BLOCK, PPT>), dim3(n_slots), dim3(BLOCK), lds, stream,
+                     d_slots, t);
+}
+
+template <int BLOCK>
+bool launch_resample_ppt(int ppt, const pipe_slot* d_slots, int n_slots, size_t lds, int t,
+                         hipStream_t stream) {
+  switch (ppt) {
+    case 1: launch_resample_shape<BLOCK, 1>(d_slots, n_slots, lds, t, stream); return true;
+    case 2: launch_resample_shape<BLOCK, 2>(d_slots, n_slots, lds, t, stream); return true;
+    case 4: launch_resample_shape<BLOCK, 4>(d_slots, n_slots, lds, t, stream); return true;
+    case 8: launch_resample_shape<BLOCK, 8>(d_slots, n_slots, lds, t, stream); return true;
+    case 16:
+      if constexpr (BLOCK == 1024) {
+        launch_resample_shape<BLOCK, 16>(d_slots, n_slots, lds, t, stream);
+        return true;
+      }
+      return false;
+    default: return false;
+  }
+}
+
+}  // namespace
+
 hipError_t launch_pipe_resample(const pipe_slot* d_slots, int n_slots, int max_particles, int t,
-                                hipStream_t stream) {
+                                hipStream_t stream, int block) {
   if (n_slots == 0) return hipSuccess;
   const size_t lds = (kScratchDoubles + static_cast<size_t>(max_particles)) * sizeof(double);
-  switch (pipe_resample_ppt(max_particles)) {
-    case 1:
-      hipLaunchKernelGGL((resample_kernel<1024, 1>), dim3(n_slots), dim3(1024), lds, stream,
-                         d_slots, t);
-      break;
-    case 2:
-      hipLaunchKernelGGL((resample_kernel<1024, 2>), dim3(n_slots), dim3(1024), lds, stream,
-                         d_slots, t);
-      break;
-    case 4:
-      hipLaunchKernelGGL((resample_kernel<1024, 4>), dim3(n_slots), dim3(1024), lds, stream,
-                         d_slots, t);
-      break;
-    case 8:
-      hipLaunchKernelGGL((resample_kernel<1024, 8>), dim3(n_slots), dim3(1024), lds, stream,
-                         d_slots, t);
-      break;
-    case 16:
-      hipLaunchKernelGGL((resample_kernel<1024, 16>), dim3(n_slots), dim3(1024), lds, stream,
-                         d_slots, t);
-      break;
-    default:
-      return hipErrorInvalidConfiguration;
+  const int ppt = pipe_resample_ppt(max_particles, block);
+  bool ok = false;
+  switch (block) {
+    case 1024: ok = launch_resample_ppt<1024>(ppt, d_slots, n_slots, lds, t, stream); break;
+    case 512: ok = launch_resample_ppt<512>(ppt, d_slots, n_slots, lds, t, stream); break;
+    case 256: ok = launch_resample_ppt<256>(ppt, d_slots, n_slots, lds, t, stream); break;
+    default: break;
   }
+  if (!ok) return hipErrorInvalidConfiguration;
   return hipGetLastError();
 }
 

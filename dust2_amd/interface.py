@@ -534,6 +534,7 @@ class Filter:
         self.launch = None
         self.engine = None
         self.chains = None
+        self.resample_block = None
         self.beta_t = None
         self.beta_y = None
         self.last_index = None
@@ -565,6 +566,8 @@ class Filter:
             call("dust2_filter_set_engine", self.ptr, *self.engine)
         if self.chains is not None:
             call("dust2_filter_set_pipeline_chains", self.ptr, self.chains)
+        if self.resample_block is not None:
+            call("dust2_filter_set_pipeline_resample", self.ptr, self.resample_block)
 
     def set_launch(self, block, particles_per_thread):
         self.launch = (int(block), int(particles_per_thread))
@@ -573,21 +576,28 @@ class Filter:
 
     ENGINES = {"auto": 0, "persistent": 1, "pipeline": 2}
 
-    def set_engine(self, engine="auto", graph=True, profile=False, chains=None):
+    def set_engine(self, engine="auto", graph=True, profile=False, chains=None,
+                   resample_block=None):
         """Choose the filter engine ('auto', 'persistent', 'pipeline'),
-        HIP-graph replay of the pipeline, per-kernel profiling and the
-        pipeline's number of launch chains (1-4)."""
+        HIP-graph replay of the pipeline, per-kernel profiling, the
+        pipeline's number of launch chains (1-4) and its resample workgroup
+        width (1024, 512 or 256 threads)."""
         self.engine = (self.ENGINES[engine], int(bool(graph)), int(bool(profile)))
         if chains is not None:
             self.chains = int(chains)
+        if resample_block is not None:
+            self.resample_block = int(resample_block)
         if self.ptr is not None:
             call("dust2_filter_set_engine", self.ptr, *self.engine)
             if self.chains is not None:
                 call("dust2_filter_set_pipeline_chains", self.ptr, self.chains)
+            if self.resample_block is not None:
+                call("dust2_filter_set_pipeline_resample", self.ptr, self.resample_block)
 
     def autotune(self, pars=None, index_group=None):
         """Time every engine / launch shape on this workload and keep the
-        fastest; returns the candidates as dicts (engine, block, ppt, ms)."""
+        fastest; returns the candidates as dicts: persistent (block, ppt,
+        ms) and pipeline (chains, resample_block, ms)."""
         if self.ptr is None:
             if pars is None:
                 raise ValueError("'pars' cannot be NULL, as 'obj' is not initialised")
@@ -609,11 +619,15 @@ class Filter:
             if e == "persistent":
                 out.append({"engine": e, "block": int(r[1]), "ppt": int(r[2]), "ms": float(r[3])})
             else:
-                out.append({"engine": e, "chains": int(r[2]), "ms": float(r[3])})
+                out.append({"engine": e, "chains": int(r[2]), "resample_block": int(r[1]),
+                            "ms": float(r[3])})
         best = min(out, key=lambda r: r["ms"])
         self.engine = (self.ENGINES[best["engine"]], 1, 0)
         if best["engine"] == "persistent":
             self.launch = (best["block"], best["ppt"])
+        else:
+            self.chains = best["chains"]
+            self.resample_block = best["resample_block"]
         return out
 
     def __del__(self):

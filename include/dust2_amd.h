@@ -201,6 +201,16 @@ int dust2_filter_set_beta_schedule(dust2_filter* h, int kind, const double* t, s
  * counterpart in the reference. */
 int dust2_filter_set_pipeline_chains(dust2_filter* h, int chains, char* err, size_t errlen);
 
+/* Threads per resample workgroup of the pipeline engine: 1024 (default),
+ * 512 or 256, the particles per thread following from the group size
+ * (1024 threads hold up to 16,384 particles, 512 and 256 threads up to 8
+ * per thread; a width that does not hold the groups is rejected).
+ * dust2_filter_autotune chooses it with the chain count.  Results do not
+ * depend on it: the resampling indices are exact and the log-likelihood
+ * takes the weights' aligned pairwise sum, whose bits do not depend on the
+ * workgroup shape.  No counterpart in the reference. */
+int dust2_filter_set_pipeline_resample(dust2_filter* h, int block, char* err, size_t errlen);
+
 /* Engine selection (DUST2_ENGINE_*), HIP-graph replay of the pipeline
  * (default on) and per-kernel event profiling (default off, disables the
  * graph).  No counterpart in the reference (its one engine is the CPU). */
@@ -213,11 +223,13 @@ int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int prof
  * the group, and the interval pipeline -- and keeps the fastest.  Results do
  * not depend on the choice; the RNG streams and state are restored, so the
  * object is left as it was apart from the engine choice (trajectories of
- * earlier runs are no longer current).  The pipeline is tried with one
- * launch chain and with the groups split into two chains on two streams.
- * report (may be NULL): up to report_rows rows of [engine, block,
- * particles per thread (persistent) or launch chains (pipeline),
- * milliseconds]; *n_report receives the number of candidates. */
+ * earlier runs are no longer current).  The pipeline is tried with one to
+ * four launch chains, each with resample workgroups of 1024, 512 and 256
+ * threads where they hold the groups.  report (may be NULL): up to
+ * report_rows rows of [engine, threads per workgroup (persistent) or per
+ * resample workgroup (pipeline), particles per thread (persistent) or
+ * launch chains (pipeline), milliseconds]; *n_report receives the number
+ * of candidates (at most 26). */
 int dust2_filter_autotune(dust2_filter* h, const size_t* index_group, size_t n_index,
                           double* report, size_t report_rows, size_t* n_report, char* err,
                           size_t errlen);

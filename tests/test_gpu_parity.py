@@ -455,6 +455,38 @@ def test_autotune_keeps_results_and_streams(gpu_lib):
                                   gpu_lib.dust_likelihood_rng_state(ref))
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_results_do_not_depend_on_launch_shape(gpu_lib, precision):
+    # the resampling indices are exact and the log-likelihood takes the
+    # weights' aligned pairwise sum, so every engine, persistent launch
+    # shape, pipeline chain count and resample workgroup width gives the
+    # same bits: log-likelihoods, trajectories, final states and streams
+    G, P, T = 4, 1000, 30
+    pars = np.array([[2000 + 500 * g, 10, 0.25 + 0.05 * g, 0.1, 1e6] for g in range(G)])
+    time = np.arange(1.0, T + 1)
+    data = np.stack([sir_data(T, seed=21 + g) for g in range(G)], axis=1)
+    plist = [dict(zip(["N", "I0", "beta", "gamma", "exp_noise"], p)) for p in pars]
+    configs = [("pipeline", c, b) for c in (1, 2, 4) for b in (1024, 512, 256)]
+    configs += [("persistent", 256, 4), ("persistent", 512, 2), ("persistent", 1024, 2)]
+    out = []
+    for engine, a, b in configs:
+        obj = gpu_lib.dust_filter_create(gpu_lib.sir(), 0.0, {"time": time, "incidence": data},
+                                         P, n_groups=G, dt=0.25, seed=5, precision=precision)
+        if engine == "persistent":
+            obj.set_launch(a, b)
+            obj.set_engine("persistent")
+        else:
+            obj.set_engine("pipeline", chains=a, resample_block=b)
+        ll = np.atleast_1d(gpu_lib.dust_likelihood_run(obj, plist, save_trajectories=True))
+        assert gpu_lib.dust_likelihood_last_stats(obj)["engine"] == engine
+        out.append((ll, gpu_lib.dust_likelihood_last_trajectories(obj),
+                    gpu_lib.dust_likelihood_last_state(obj),
+                    gpu_lib.dust_likelihood_rng_state(obj)))
+    for cfg, r in zip(configs[1:], out[1:]):
+        for a, b in zip(out[0], r):
+            assert np.array_equal(a, b), cfg
+
+
 NAMES = ["N", "I0", "beta", "gamma", "exp_noise"]
 
 
