@@ -160,16 +160,31 @@ def simulate_seir_data(lib, plist, time, dt, seed):
     return np.ascontiguousarray(tr[:, :, 0, 64:80])            # (T, G, 16)
 
 
-def measured_traffic(key, kernel):
-    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
-    profile of this workload and launch shape, or None."""
+def traffic_entry(key, kernel):
+    """The committed counters of this workload's launch shape for `kernel`
+    (profiles/hbm_traffic.json: one entry, or a list of entries when the
+    autotuner may choose between kernels), or None."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
             tab = json.load(f)
     except OSError:
-        return None
-    e = tab.get(key)
-    if e is None or not e["kernel"].startswith(kernel):
+        return None, None
+    entries = tab.get(key)
+    if entries is None:
+        return tab, None
+    if isinstance(entries, dict):
+        entries = [entries]
+    for e in entries:
+        if e["kernel"].startswith(kernel):
+            return tab, e
+    return tab, None
+
+
+def measured_traffic(key, kernel):
+    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+    profile of this workload and launch shape, or None."""
+    tab, e = traffic_entry(key, kernel)
+    if e is None:
         return None
     return 1024 * (tab["fetch_scale"] * e["fetch_kib"] + tab["write_scale"] * e["write_kib"])
 
@@ -199,12 +214,8 @@ def valu_issue(key, kernel, k_ms, fp64):
     instructions per launch (the committed SQ_INSTS_VALU profile of this
     workload and launch shape) over its event-timed duration, against the
     measured issue rate of independent FMAs; None without a profile."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
-            e = json.load(f).get(key)
-    except OSError:
-        return None
-    if e is None or "valu_instructions" not in e or not e["kernel"].startswith(kernel):
+    _, e = traffic_entry(key, kernel)
+    if e is None or "valu_instructions" not in e:
         return None
     peak, n_simd = valu_peak(fp64)
     achieved = e["valu_instructions"] / n_simd / (k_ms * 1e6)
@@ -290,7 +301,7 @@ def bench_sir(lib, args):
         obj.set_launch(*[int(x) for x in args.launch.split(",")])
     obj.set_engine(args.engine, graph=not args.no_graph)
     if args.engine == "auto" and not args.launch and not args.no_tune:
-        # untimed: the autotuner runs each engine / launch shape twice and
+        # untimed: the autotuner runs each engine / launch shape three times and
         # restores the RNG streams, then keeps the fastest
         tuned = obj.autotune(plist)
     # every call passes the parameters, as an MCMC driver does
@@ -515,7 +526,7 @@ def bench_seir16(lib, args):
     data = simulate_seir_data(lib, plist, time, dt, seed=42)
     obj = lib.dust_filter_create(lib.seir16(), 0.0, {"time": time, "incidence": data}, P,
                                  n_groups=G, dt=dt, seed=1, precision="fp32")
-    # untimed: one and two launch chains, twice each, streams restored
+    # untimed: one to four launch chains, three runs each, streams restored
     tuned = None if args.no_tune else obj.autotune(plist)
     ll, elapsed, gpu_ms = timed_runs(lib, obj, plist, args.steps, args.warmup)
     n_steps = int(round(time[-1] / dt))
