@@ -204,6 +204,12 @@ hipError_t launch_mb_resample_one(const mb_resample& task, int t_row, hipStream_
 // ---- ODE unfilter ----------------------------------------------------------
 // dust2/continuous/control.hpp:11-24 (step_size_min already clamped to
 // machine epsilon, dust2/r/continuous/control.hpp:15-17)
+// log(1e-4): the floor of the old error in the step-size controller
+// (dust2/continuous/solver.hpp:210-211, max(error_old, 1e-4)), correctly
+// rounded; both DP5 engines keep the log of the old error and take the
+// controller's err^constant / max(error_old, 1e-4)^beta as one exp.
+constexpr double kLogFacMin = -9.210340371976182;
+
 struct ode_control_dev {
   uint32_t max_steps;
   double atol, rtol, step_size_min, step_size_max;

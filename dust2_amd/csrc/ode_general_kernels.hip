@@ -63,6 +63,7 @@ struct gstate {
   double t0, t1, h;  // last accepted step (history_step t0, t1, h)
   double c1[NY], c2[NV], c3[NV], c4[NV], c5[NV];
   double step_size, error;
+  double log_fac_old;  // log(max(error, 1e-4)), not stored: rebuilt on load
   double n_steps, n_accepted, n_rejected;
   double last_event[ME];
   double hist_count, hist_first_t0;
@@ -447,8 +448,13 @@ __device__ double step(gctx& c, double t, double t_end, gstate<M>& s, const ode_
     const double err = try_step<M>(c, t, h, s, ctl, y_next, k2, c5);
     if (code) return t;
     s.n_steps += 1;
-    const double fac11 = pow(err, ctl.constant);
-    if (err <= 1) {
+    // err^constant, over max(error_old, 1e-4)^beta when accepted
+    // (solver.hpp:182, 210-212), as one exp of logs
+    const double log_err = log(err);
+    const bool accept = err <= 1;
+    const double fac11 =
+        exp(ctl.constant * log_err - (accept ? ctl.beta * s.log_fac_old : 0.0));
+    if (accept) {
       // update_interpolation (solver.hpp:335-352)
       s.t0 = t;
       s.t1 = t + h;
@@ -483,13 +489,12 @@ __device__ double step(gctx& c, double t, double t_end, gstate<M>& s, const ode_
       record_time(lg.rec, lg.i, truncated ? t_end : t + h);
       save_history<M>(s, lg);
       if (!truncated && !event) {
-        const double fac_old = fmax(s.error, 1e-4);
-        double fac = fac11 / pow(fac_old, ctl.beta);
-        fac = fmax(fmin(fac / ctl.factor_safe, facc1), facc2);
+        const double fac = fmax(fmin(fac11 / ctl.factor_safe, facc1), facc2);
         const double h_new = h / fac;
         const double h_max = reject ? h : ctl.step_size_max;
         s.step_size = fmin(h_new, h_max);
         s.error = err;
+        s.log_fac_old = fmax(log_err, kLogFacMin);
       }
       return truncated ? t_end : t + h;
     }
@@ -578,6 +583,7 @@ __device__ void initialise(gctx& c, double t, gstate<M>& s, const ode_control_de
   constexpr int NY = gstate<M>::NY;
   s.step_size = 0;
   s.error = 0;
+  s.log_fac_old = kLogFacMin;
   s.n_steps = 0;
   s.n_accepted = 0;
   s.n_rejected = 0;
@@ -681,6 +687,7 @@ __device__ void load(gstate<M>& s, const double* w) {
   }
   s.step_size = w[L.step_size];
   s.error = w[L.error];
+  s.log_fac_old = fmax(log(s.error), kLogFacMin);
   s.n_steps = w[L.n_steps];
   s.n_accepted = w[L.n_accepted];
   s.n_rejected = w[L.n_rejected];

@@ -42,10 +42,6 @@ constexpr double D1 = -12715105075.0 / 11282082432.0, D3 = 87487479700.0 / 32700
                  D6 = -1453857185.0 / 822651844.0, D7 = 69997945.0 / 29380423.0;
 }  // namespace dp
 
-// log(1e-4): the floor of the old error in the step-size controller
-// (solver.hpp:210-211, max(error_old, 1e-4)), correctly rounded
-constexpr double kLogFacMin = -9.210340371976182;
-
 // Per-particle integrator state.  Only the reset variable's dense-output
 // coefficients are kept: they are the only ones apply_zero_every reads.
 template <typename real>
