@@ -237,6 +237,7 @@ __device__ resample_result block_resample(double (&w)[PPT], int (&idx)[PPT], int
     scratch[kScratchDoubles - 1] = u_draw();
   }
   mx = block_max<BLOCK>(mx, scratch);
+  if constexpr (BLOCK == kWave) __syncthreads();  // block_max has no barrier for one wave
   const double u = scratch[kScratchDoubles - 1];
 
   if (mx == neg_inf) {
