@@ -361,9 +361,9 @@ struct btrs_f32 {
   float m, rm, rnm, L, tail_m;       // mode terms of the acceptance test
 };
 
-// Setup: four quarter-rate instructions (the square root, 1/b, one
-// reciprocal of q (m + 1) (n - m + 1) that gives 1/(m + 1), 1/(n - m + 1)
-// and 1/(q (m + 1)), and the log of L).
+// Setup: three quarter-rate instructions (the square root, one reciprocal
+// of b q (m + 1) (n - m + 1) that gives 1/b, 1/(m + 1), 1/(n - m + 1) and
+// 1/(q (m + 1)), and the log of L).
 __device__ __forceinline__ btrs_f32 btrs_setup_f32(float n, float p) {
   btrs_f32 s;
   const float q = 1 - p;
@@ -373,18 +373,19 @@ __device__ __forceinline__ btrs_f32 btrs_setup_f32(float n, float p) {
   s.b = 1.15f + 2.53f * spq;
   s.a = -0.0873f + 0.0248f * s.b + 0.01f * p;
   s.c = n * p + 0.5f;
-  const float rb = f_rcp(s.b);
-  s.v_r = 0.92f - 4.2f * rb;
-  s.two_a = 2 * s.a;
-  s.alpha = (2.83f + 5.1f * rb) * spq;
   s.m = floorf((n + 1) * p);
   const float m1 = s.m + 1;
   const float nm1 = n - s.m + 1;
-  const float r = f_rcp(q * m1 * nm1);
-  const float qr = q * r;
+  const float qmn = q * m1 * nm1;
+  const float r = f_rcp(s.b * qmn);  // 1 / (b q (m + 1) (n - m + 1))
+  const float rb = qmn * r;
+  const float qr = q * (s.b * r);    // 1 / ((m + 1) (n - m + 1))
+  s.v_r = 0.92f - 4.2f * rb;
+  s.two_a = 2 * s.a;
+  s.alpha = (2.83f + 5.1f * rb) * spq;
   s.rm = nm1 * qr;
   s.rnm = m1 * qr;
-  s.L = f_log(p * nm1 * (nm1 * r));
+  s.L = f_log(p * nm1 * (nm1 * (s.b * r)));
   s.tail_m = stirling_tail_pair_f32(s.m, n - s.m, s.rm, s.rnm);
   return s;
 }
