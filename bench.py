@@ -333,10 +333,14 @@ def bench_sir(lib, args):
     if not args.no_fp64:
         obj64 = lib.dust_filter_create(lib.sir(), 0.0, {"time": time, "incidence": data}, P,
                                        n_groups=G, dt=dt, seed=1, precision="fp64")
+        if not args.no_tune:
+            obj64.autotune(plist)  # untimed, streams restored
         _, el64, ms64 = timed_runs(lib, obj64, plist, 2, 1)
         out["fp64"] = {"particle_steps_per_s": G * P * n_steps * 2 / el64,
                        "ms_per_run": 1e3 * el64 / 2, "gpu_ms_per_run": float(np.mean(ms64)),
-                       "engine": lib.dust_likelihood_last_stats(obj64)["engine"]}
+                       "engine": lib.dust_likelihood_last_stats(obj64)["engine"],
+                       "pipeline": pipeline_shape(obj64,
+                                                  lib.dust_likelihood_last_stats(obj64)["engine"])}
         del obj64
     # the plain path: one filter object per group, one group per launch
     # sequence (warmed by one run each, then one timed run each)
