@@ -213,7 +213,9 @@ int dust2_filter_set_pipeline_resample(dust2_filter* h, int block, char* err, si
 
 /* Engine selection (DUST2_ENGINE_*), HIP-graph replay of the pipeline
  * (default on) and per-kernel event profiling (default off, disables the
- * graph).  No counterpart in the reference (its one engine is the CPU). */
+ * graph).  AUTO runs the pipeline, except the persistent kernel for 256 or
+ * more SIR groups of at most 512 particles.  No counterpart in the
+ * reference (its one engine is the CPU). */
 int dust2_filter_set_engine(dust2_filter* h, int engine, int use_graph, int profile,
                                 char* err, size_t errlen);
 
